@@ -1,0 +1,302 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings to ``oracle/liboracle.so``, the CPU FP64 restatement of
+ASPECT's matrix-free Stokes solve (see the header of ``oracle/oracle.cpp`` for
+the reference files it follows).  Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` import this package; the product
+in ``aspect_amd/`` never does.
+
+Parity status of each function (how it is pinned) is listed in DESIGN.md.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with g++ (OpenMP).  Returns the library path."""
+    src = os.path.join(_HERE, "oracle.cpp")
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(src) > os.path.getmtime(_LIB_PATH):
+        cmd = ["g++", "-O3", "-march=x86-64-v2", "-fopenmp", "-shared", "-fPIC", "-std=c++17",
+               "-o", _LIB_PATH + ".tmp", src]
+        subprocess.run(cmd, check=True)
+        os.replace(_LIB_PATH + ".tmp", _LIB_PATH)
+    return _LIB_PATH
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("degree", C.c_int),
+        ("coarse", C.c_int * 3),
+        ("n_levels", C.c_int),
+        ("extent", C.c_double * 3),
+        ("bc", C.c_int * 6),
+        ("compressible", C.c_int),
+        ("pressure_scaling", C.c_double),
+    ]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double),
+        ("n_cheap_steps", C.c_int),
+        ("n_expensive_steps", C.c_int),
+        ("restart", C.c_int),
+        ("a_block_tolerance", C.c_double),
+        ("s_block_tolerance", C.c_double),
+        ("level_eta_mode", C.c_int),
+    ]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [
+        ("cheap_iterations", C.c_int),
+        ("expensive_iterations", C.c_int),
+        ("a_preconditioner_applications", C.c_int),
+        ("s_preconditioner_applications", C.c_int),
+        ("converged", C.c_int),
+        ("initial_nonlinear_residual", C.c_double),
+        ("final_linear_residual", C.c_double),
+        ("solver_tolerance", C.c_double),
+        ("r0_norm", C.c_double),
+        ("pressure_normalization_adjustment", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+def make_problem(dim=3, degree=2, coarse=(1, 1, 1), n_levels=3, extent=(1.0, 1.0, 1.0),
+                 bc=(1, 1, 1, 1, 1, 1), compressible=False, pressure_scaling=1.0) -> Problem:
+    p = Problem()
+    p.dim = dim
+    p.degree = degree
+    for i in range(3):
+        p.coarse[i] = int(coarse[i]) if i < len(coarse) else 1
+        p.extent[i] = float(extent[i]) if i < len(extent) else 1.0
+    p.n_levels = n_levels
+    for i in range(6):
+        p.bc[i] = int(bc[i])
+    p.compressible = int(compressible)
+    p.pressure_scaling = pressure_scaling
+    return p
+
+
+def sizes(p: Problem, level: int):
+    out = np.zeros(4, dtype=np.int64)
+    _check(lib().orc_sizes(C.byref(p), level, _p(out)))
+    return dict(cells=int(out[0]), nu=int(out[1]), np=int(out[2]), vnodes=int(out[3]))
+
+
+def n_qp(p: Problem) -> int:
+    return (p.degree + 1) ** p.dim
+
+
+def constrained(p, level):
+    s = sizes(p, level)
+    out = np.zeros(s["nu"], dtype=np.uint8)
+    _check(lib().orc_constrained(C.byref(p), level, _p(out)))
+    return out.astype(bool)
+
+
+def qp_points(p, level):
+    s = sizes(p, level)
+    out = np.zeros((s["cells"], n_qp(p), p.dim))
+    _check(lib().orc_qp_points(C.byref(p), level, _p(out)))
+    return out
+
+
+def quadrature_weights(p, level):
+    out = np.zeros(n_qp(p))
+    _check(lib().orc_quadrature_weights(C.byref(p), level, _p(out)))
+    return out
+
+
+def velocity_support_points(p, level):
+    s = sizes(p, level)
+    out = np.zeros((s["vnodes"], p.dim))
+    _check(lib().orc_velocity_support_points(C.byref(p), level, _p(out)))
+    return out
+
+
+def pressure_support_points(p, level):
+    s = sizes(p, level)
+    out = np.zeros((s["np"], p.dim))
+    _check(lib().orc_pressure_support_points(C.byref(p), level, _p(out)))
+    return out
+
+
+def _eta(eta):
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    per_qp = 1 if eta.ndim == 2 else 0
+    return eta, per_qp
+
+
+def stokes_vmult(p, level, eta, src):
+    eta, per_qp = _eta(eta)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_stokes_vmult(C.byref(p), level, _p(eta), per_qp, _p(src), _p(dst)))
+    return dst
+
+
+def ablock_vmult(p, level, eta, src):
+    eta, per_qp = _eta(eta)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_ablock_vmult(C.byref(p), level, _p(eta), per_qp, _p(src), _p(dst)))
+    return dst
+
+
+def bt_vmult(p, level, src_p):
+    s = sizes(p, level)
+    src_p = np.ascontiguousarray(src_p, dtype=np.float64)
+    dst = np.zeros(s["nu"])
+    _check(lib().orc_bt_vmult(C.byref(p), level, _p(src_p), _p(dst)))
+    return dst
+
+
+def mass_vmult(p, level, eta, src):
+    eta, per_qp = _eta(eta)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_mass_vmult(C.byref(p), level, _p(eta), per_qp, _p(src), _p(dst)))
+    return dst
+
+
+def ablock_diagonal(p, level, eta):
+    eta, per_qp = _eta(eta)
+    out = np.zeros(sizes(p, level)["nu"])
+    _check(lib().orc_ablock_diagonal(C.byref(p), level, _p(eta), per_qp, _p(out)))
+    return out
+
+
+def mass_diagonal(p, level, eta):
+    eta, per_qp = _eta(eta)
+    out = np.zeros(sizes(p, level)["np"])
+    _check(lib().orc_mass_diagonal(C.byref(p), level, _p(eta), per_qp, _p(out)))
+    return out
+
+
+def cell_matrices(p, level, eta):
+    eta, per_qp = _eta(eta)
+    s = sizes(p, level)
+    nv = (p.degree + 1) ** p.dim
+    npl = p.degree ** p.dim
+    nloc = p.dim * nv + npl
+    K = np.zeros((s["cells"], nloc, nloc))
+    M = np.zeros((s["cells"], npl, npl))
+    D = np.zeros((s["cells"], nloc), dtype=np.int64)
+    _check(lib().orc_cell_matrices(C.byref(p), level, _p(eta), per_qp, _p(K), _p(M), _p(D)))
+    return K, M, D
+
+
+def prolongate(p, fine_level, kind, src):
+    s = sizes(p, fine_level)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros(s["nu"] if kind == 0 else s["np"])
+    _check(lib().orc_prolongate(C.byref(p), fine_level, kind, _p(src), _p(dst)))
+    return dst
+
+
+def restrict(p, fine_level, kind, src):
+    s = sizes(p, fine_level - 1)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros(s["nu"] if kind == 0 else s["np"])
+    _check(lib().orc_restrict(C.byref(p), fine_level, kind, _p(src), _p(dst)))
+    return dst
+
+
+def level_viscosity(p, eta):
+    eta, per_qp = _eta(eta)
+    counts = [sizes(p, l)["cells"] for l in range(p.n_levels)]
+    out = np.zeros(sum(counts))
+    _check(lib().orc_level_viscosity(C.byref(p), _p(eta), per_qp, _p(out)))
+    res, off = [], 0
+    for c in counts:
+        res.append(out[off:off + c].copy())
+        off += c
+    return res
+
+
+def smoother_setup(p, eta):
+    eta, per_qp = _eta(eta)
+    out = np.zeros((p.n_levels, 2, 5))
+    _check(lib().orc_smoother_setup(C.byref(p), _p(eta), per_qp, _p(out)))
+    return out
+
+
+def vcycle(p, eta, kind, src):
+    eta, per_qp = _eta(eta)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_vcycle(C.byref(p), _p(eta), per_qp, kind, _p(src), _p(dst)))
+    return dst
+
+
+def chebyshev(p, eta, kind, level, src):
+    eta, per_qp = _eta(eta)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_chebyshev(C.byref(p), _p(eta), per_qp, kind, level, _p(src), _p(dst)))
+    return dst
+
+
+def rhs_buoyancy(p, rho_qp, g):
+    s = sizes(p, p.n_levels - 1)
+    rho_qp = np.ascontiguousarray(rho_qp, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    out = np.zeros(s["nu"] + s["np"])
+    _check(lib().orc_rhs_buoyancy(C.byref(p), _p(rho_qp), _p(g), _p(out)))
+    return out
+
+
+def default_params(tolerance=1e-7, n_cheap=1000, n_expensive=1000, restart=100,
+                   a_tol=1e-2, s_tol=1e-6) -> SolveParams:
+    prm = SolveParams()
+    prm.tolerance = tolerance
+    prm.n_cheap_steps = n_cheap
+    prm.n_expensive_steps = n_expensive
+    prm.restart = restart
+    prm.a_block_tolerance = a_tol
+    prm.s_block_tolerance = s_tol
+    prm.level_eta_mode = 0
+    return prm
+
+
+def solve(p, eta, rhs, x0=None, params: SolveParams | None = None):
+    eta, per_qp = _eta(eta)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x0 = np.zeros_like(rhs) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    params = params or default_params()
+    sol = np.zeros_like(rhs)
+    res = SolveResult()
+    _check(lib().orc_solve(C.byref(p), _p(eta), per_qp, C.byref(params), _p(rhs), _p(x0), _p(sol),
+                           C.byref(res)))
+    return sol, {f: getattr(res, f) for f, _ in SolveResult._fields_}
