@@ -1,0 +1,1321 @@
+// Host side of the MI355X matrix-free Stokes solver: the StokesMatrixFreeHandler
+// shape (setup_dofs / assemble / build_preconditioner / solve) over device data,
+// and the C-ABI declared in include/aspect_mfs.h.
+//
+// Reference structure followed (ASPECT, source/simulator/solver/):
+//   setup_dofs              stokes_matrix_free_local_smoothing.cc:1450-1851
+//   evaluate_material_model :173-294, fill_active_cell_data matrix_free_operators.cc:137-330
+//   build_preconditioner    :1855-1867
+//   solve                   :758-1446 (smoothers :771-849, tolerance :977-1041,
+//                           cheap GMRES :1262-1313, expensive FGMRES :1314-1397,
+//                           post-steps :1406-1445)
+//   block preconditioner    include/aspect/simulator/solver/block_stokes_preconditioner.h:43-352
+//
+// Numerics restated (not copied) from the published algorithms: Jacobi
+// preconditioned Chebyshev semi-iteration with a CG-Lanczos eigenvalue
+// estimate, V-cycle multigrid, right-preconditioned restarted GMRES / flexible
+// GMRES with classical Gram-Schmidt and one reorthogonalization pass, and CG.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aspect_mfs.h"
+#include "mfs_internal.h"
+
+using namespace mfs;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  long long n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void alloc(long long nn) {
+    release();
+    if (nn > 0) MFS_HIP(hipMalloc(reinterpret_cast<void **>(&p), (size_t)nn * sizeof(T)));
+    n = nn;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  long long bytes() const { return n * (long long)sizeof(T); }
+};
+
+struct ChebInfo {
+  double lmin = 1, lmax = 1, theta = 1, delta = 0;
+  int degree = 4, its = 0;
+};
+
+LevelDesc make_level_desc(const mfs_box &b, int l) {
+  LevelDesc L{};
+  L.dim = b.dim;
+  L.k = b.degree;
+  int t[3];
+  tile_cells(b.dim, b.degree, t);
+  L.cells = 1;
+  L.vnodes = 1;
+  L.pnodes = 1;
+  L.ntiles_total = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < b.dim) {
+      L.n[d] = b.coarse[d] << l;
+      L.h[d] = b.extent[d] / L.n[d];
+      L.nv[d] = b.degree * L.n[d] + 1;
+      L.np[d] = (b.degree - 1) * L.n[d] + 1;
+      L.ntiles[d] = (L.n[d] + t[d] - 1) / t[d];
+    } else {
+      L.n[d] = 1;
+      L.h[d] = 1.0;
+      L.nv[d] = 1;
+      L.np[d] = 1;
+      L.ntiles[d] = 1;
+    }
+    L.cells *= L.n[d];
+    L.vnodes *= L.nv[d];
+    L.pnodes *= L.np[d];
+    L.ntiles_total *= L.ntiles[d];
+  }
+  L.s = b.pressure_scaling;
+  for (int f = 0; f < 6; ++f) L.bc[f] = (f < 2 * b.dim) ? b.bc[f] : 0;
+  L.compressible = b.compressible;
+  return L;
+}
+
+int n_qp(const mfs_box &b) {
+  const int q = b.degree + 1;
+  return b.dim == 3 ? q * q * q : q * q;
+}
+int cells_per_tile(const mfs_box &b) {
+  int t[3];
+  return tile_cells(b.dim, b.degree, t);
+}
+
+// ------------------------------------------------------------------ multigrid in precision T
+template <typename T>
+struct GMGLevel {
+  LevelDesc L;
+  long long nu = 0, np = 0;
+  DBuf<T> eta, dinvA, dinvS;
+  DBuf<T> ux, ub, ur, ud, px, pb, pr, pd;
+  ChebInfo chA, chS;
+};
+
+template <typename T>
+struct GMG {
+  hipStream_t st;
+  std::vector<std::unique_ptr<GMGLevel<T>>> lv;
+  DBuf<double> *dot_work = nullptr, *dot_out = nullptr;
+  int top = 0;
+
+  void setup(const mfs_box &b, hipStream_t s, DBuf<double> *work, DBuf<double> *out) {
+    st = s;
+    dot_work = work;
+    dot_out = out;
+    lv.clear();
+    const int ct = cells_per_tile(b);
+    for (int l = 0; l < b.n_levels; ++l) {
+      auto g = std::make_unique<GMGLevel<T>>();
+      g->L = make_level_desc(b, l);
+      g->nu = (long long)b.dim * g->L.vnodes;
+      g->np = g->L.pnodes;
+      g->eta.alloc(g->L.ntiles_total * ct);
+      v_fill<T>(g->eta.p, (T)1, g->eta.n, st);
+      for (DBuf<T> *v : {&g->dinvA, &g->ux, &g->ub, &g->ur, &g->ud}) {
+        v->alloc(g->nu);
+        v_fill<T>(v->p, (T)0, v->n, st);
+      }
+      for (DBuf<T> *v : {&g->dinvS, &g->px, &g->pb, &g->pr, &g->pd}) {
+        v->alloc(g->np);
+        v_fill<T>(v->p, (T)0, v->n, st);
+      }
+      lv.push_back(std::move(g));
+    }
+    top = b.n_levels - 1;
+  }
+
+  // level viscosity: finest from the active table, coarser by averaging children
+  void set_eta(const double *active, int active_per_qp) {
+    GMGLevel<T> &f = *lv[top];
+    if (active_per_qp)
+      launch_eta_qp_to_cell<T, double>(f.L, active, f.eta.p, st);
+    else
+      v_cast<double, T>(active, f.eta.p, f.eta.n, st);
+    for (int l = top - 1; l >= 0; --l) launch_eta_restrict<T>(lv[l]->L, lv[l + 1]->L, lv[l + 1]->eta.p, lv[l]->eta.p, st);
+  }
+
+  void applyA(int l, Mode mode, const T *x, const T *rhs, T *dst) {
+    GMGLevel<T> &g = *lv[l];
+    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, 0, rhs, nullptr, dst, nullptr, st);
+  }
+  void applyS(int l, Mode mode, const T *x, const T *rhs, T *dst) {
+    GMGLevel<T> &g = *lv[l];
+    launch_cell_op<T>(OP_MASS, mode, g.L, nullptr, x, g.eta.p, 0, nullptr, rhs, nullptr, dst, st);
+  }
+  void apply(int kind, int l, Mode mode, const T *x, const T *rhs, T *dst) {
+    if (kind == 0)
+      applyA(l, mode, x, rhs, dst);
+    else
+      applyS(l, mode, x, rhs, dst);
+  }
+  double dot(const T *a, const T *b, long long n) {
+    v_dot<T>(a, b, n, dot_work->p, dot_out->p, st);
+    double h = 0;
+    MFS_HIP(hipMemcpyAsync(&h, dot_out->p, sizeof(double), hipMemcpyDeviceToHost, st));
+    MFS_HIP(hipStreamSynchronize(st));
+    return h;
+  }
+
+  // Chebyshev eigenvalue estimate: preconditioned CG on D^{-1}A started from
+  // the mean-free pattern (i mod 11); Lanczos tridiagonal from the CG
+  // coefficients; lambda_max scaled by 1.2 (set-up at local_smoothing.cc:771-849).
+  static double tridiag_eig(const std::vector<double> &d, const std::vector<double> &e, bool want_max) {
+    double lo = 1e300, hi = -1e300;
+    for (size_t i = 0; i < d.size(); ++i) {
+      double r = (i ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < d.size() ? std::fabs(e[i]) : 0.0);
+      lo = std::min(lo, d[i] - r);
+      hi = std::max(hi, d[i] + r);
+    }
+    const int target = want_max ? (int)d.size() : 1;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      int cnt = 0;
+      double q = 1.0;
+      for (size_t i = 0; i < d.size(); ++i) {
+        q = d[i] - mid - (i ? e[i - 1] * e[i - 1] / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0) ++cnt;
+      }
+      if (cnt >= target)
+        hi = mid;
+      else
+        lo = mid;
+      if (hi - lo <= 1e-15 * std::max(1.0, std::fabs(hi))) break;
+    }
+    return 0.5 * (lo + hi);
+  }
+
+  void estimate(int kind, int l, ChebInfo &ch, int max_its, double range) {
+    GMGLevel<T> &g = *lv[l];
+    const long long n = kind == 0 ? g.nu : g.np;
+    T *dinv = kind == 0 ? g.dinvA.p : g.dinvS.p;
+    T *r = kind == 0 ? g.ur.p : g.pr.p;
+    T *z = kind == 0 ? g.ud.p : g.pd.p;
+    T *p = kind == 0 ? g.ux.p : g.px.p;
+    T *q = kind == 0 ? g.ub.p : g.pb.p;
+    std::vector<T> pat(n);
+    double mean = 0;
+    for (long long i = 0; i < n; ++i) mean += (double)(i % 11);
+    mean /= (double)n;
+    for (long long i = 0; i < n; ++i) pat[i] = (T)((double)(i % 11) - mean);
+    MFS_HIP(hipMemcpyAsync(r, pat.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
+    std::vector<double> alphas, betas;
+    double res = std::sqrt(dot(r, r, n));
+    if (res >= 1e-2) {
+      v_cheb_first<T>(dinv, r, z, p, (T)1, true, n, st);  // z = p = D^{-1} r
+      double rz = dot(r, z, n);
+      for (int it = 1; it <= max_its; ++it) {
+        apply(kind, l, MODE_SET, p, nullptr, q);
+        const double alpha = rz / dot(p, q, n);
+        v_axpby<T>((T)(-alpha), q, (T)1, r, n, st);
+        alphas.push_back(alpha);
+        res = std::sqrt(dot(r, r, n));
+        if (res < 1e-2 || it == max_its) break;
+        v_mul<T>(dinv, r, z, n, st);
+        const double rz2 = dot(r, z, n);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        v_axpby<T>((T)1, z, (T)beta, p, n, st);
+        betas.push_back(beta);
+      }
+    }
+    ch.its = (int)alphas.size();
+    if (alphas.empty()) {
+      ch.lmin = ch.lmax = 1.0;
+    } else {
+      const size_t m = alphas.size();
+      std::vector<double> d(m), e(m > 1 ? m - 1 : 0);
+      d[0] = 1.0 / alphas[0];
+      for (size_t k = 1; k < m; ++k) d[k] = 1.0 / alphas[k] + betas[k - 1] / alphas[k - 1];
+      for (size_t k = 0; k + 1 < m; ++k) e[k] = std::sqrt(betas[k]) / alphas[k];
+      ch.lmin = tridiag_eig(d, e, false);
+      ch.lmax = 1.2 * tridiag_eig(d, e, true);
+    }
+    const double a = range > 1.0 ? ch.lmax / range : std::min(0.9 * ch.lmax, ch.lmin);
+    ch.delta = 0.5 * (ch.lmax - a);
+    ch.theta = 0.5 * (ch.lmax + a);
+  }
+
+  // diagonals (build_preconditioner) and Chebyshev set-up (solve() :771-849)
+  void build() {
+    for (int l = 0; l <= top; ++l) {
+      GMGLevel<T> &g = *lv[l];
+      launch_cell_op<T>(OP_ADIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, 0, nullptr, nullptr, g.dinvA.p, nullptr,
+                        st);
+      launch_cell_op<T>(OP_MDIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, 0, nullptr, nullptr, nullptr, g.dinvS.p,
+                        st);
+      v_invert<T>(g.dinvA.p, g.nu, st);
+      v_invert<T>(g.dinvS.p, g.np, st);
+    }
+    for (int l = 0; l <= top; ++l) {
+      GMGLevel<T> &g = *lv[l];
+      const bool coarse = l == 0;
+      g.chA.degree = g.chS.degree = coarse ? 8 : 4;
+      estimate(0, l, g.chA, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
+      estimate(1, l, g.chS, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
+    }
+  }
+
+  // Chebyshev semi-iteration (Saad, Alg. 12.1) with Jacobi preconditioning;
+  // steps applications, the first from zero when x_zero.
+  void smooth(int kind, int l, T *x, const T *b, bool x_zero, int steps) {
+    GMGLevel<T> &g = *lv[l];
+    const ChebInfo &ch = kind == 0 ? g.chA : g.chS;
+    const long long n = kind == 0 ? g.nu : g.np;
+    T *dinv = kind == 0 ? g.dinvA.p : g.dinvS.p;
+    T *r = kind == 0 ? g.ur.p : g.pr.p;
+    T *d = kind == 0 ? g.ud.p : g.pd.p;
+    for (int s = 0; s < steps; ++s) {
+      const bool zero = x_zero && s == 0;
+      const T *res = b;
+      if (!zero) {
+        apply(kind, l, MODE_RESID, x, b, r);
+        res = r;
+      }
+      v_cheb_first<T>(dinv, res, d, x, (T)(1.0 / ch.theta), zero, n, st);
+      if (ch.degree < 2 || std::fabs(ch.delta) < 1e-40) continue;
+      const double sigma = ch.theta / ch.delta;
+      double rho_old = ch.delta / ch.theta;
+      for (int k = 1; k < ch.degree; ++k) {
+        apply(kind, l, MODE_RESID, x, b, r);
+        const double rho = 1.0 / (2.0 * sigma - rho_old);
+        const double f1 = rho * rho_old, f2 = 2.0 * rho / ch.delta;
+        v_cheb_step<T>(dinv, r, d, x, (T)f1, (T)f2, n, st);
+        rho_old = rho;
+      }
+    }
+  }
+
+  void vcycle_level(int kind, int l) {
+    GMGLevel<T> &g = *lv[l];
+    T *x = kind == 0 ? g.ux.p : g.px.p;
+    T *b = kind == 0 ? g.ub.p : g.pb.p;
+    T *r = kind == 0 ? g.ur.p : g.pr.p;
+    const int steps = kind == 0 ? 1 : 4;
+    if (l == 0) {
+      smooth(kind, 0, x, b, true, steps);
+      return;
+    }
+    smooth(kind, l, x, b, true, steps);
+    apply(kind, l, MODE_RESID, x, b, r);
+    GMGLevel<T> &c = *lv[l - 1];
+    launch_restrict<T>(kind, c.L, g.L, r, kind == 0 ? c.ub.p : c.pb.p, st);
+    vcycle_level(kind, l - 1);
+    launch_prolongate<T>(kind, c.L, g.L, kind == 0 ? c.ux.p : c.px.p, x, true, st);
+    smooth(kind, l, x, b, false, steps);
+  }
+
+  // PreconditionMG::vmult in FP64 at the boundary, precision T inside
+  void vmult(int kind, const double *src, double *dst) {
+    GMGLevel<T> &g = *lv[top];
+    const long long n = kind == 0 ? g.nu : g.np;
+    v_cast<double, T>(src, kind == 0 ? g.ub.p : g.pb.p, n, st);
+    vcycle_level(kind, top);
+    v_cast<T, double>(kind == 0 ? g.ux.p : g.px.p, dst, n, st);
+  }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ the handle
+struct mfs_handle {
+  mfs_box box{};
+  int prec = MFS_GMG_FP64;
+  hipStream_t st = nullptr;
+  std::vector<LevelDesc> L;
+  int top = 0;
+  long long nu = 0, np = 0, n = 0;
+  int nq = 0, ct = 0;
+  DBuf<double> eta_nat, eta_active, ones_cell;
+  int active_per_qp = 0;
+  bool have_eta = false, have_prec = false;
+  double eta_min = 0, eta_max = 0;
+  GMG<float> g32;
+  GMG<double> g64;
+  DBuf<double> dot_work, dot_out, hbuf;
+  DBuf<double> w1, w2, w3, w4, w5, pnorm_w;
+  std::vector<double *> vchunks, zchunks;
+  int a_apps = 0, s_apps = 0;
+
+  ~mfs_handle() {
+    for (double *p : vchunks) (void)hipFree(p);
+    for (double *p : zchunks) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  const LevelDesc &Ltop() const { return L[top]; }
+
+  void gmg_vmult(int kind, const double *src, double *dst) {
+    if (prec == MFS_GMG_FP32)
+      g32.vmult(kind, src, dst);
+    else
+      g64.vmult(kind, src, dst);
+    if (kind == 0)
+      ++a_apps;
+    else
+      ++s_apps;
+  }
+
+  double dot(const double *a, const double *b, long long nn) {
+    v_dot<double>(a, b, nn, dot_work.p, dot_out.p, st);
+    double h = 0;
+    MFS_HIP(hipMemcpyAsync(&h, dot_out.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    MFS_HIP(hipStreamSynchronize(st));
+    return h;
+  }
+
+  // ---- active-level FP64 operators
+  void stokes(Mode mode, const double *x, const double *rhs, double *dst) {
+    launch_cell_op<double>(OP_STOKES, mode, Ltop(), x, x + nu, eta_active.p, active_per_qp, rhs,
+                           rhs ? rhs + nu : nullptr, dst, dst + nu, st);
+  }
+  void ablock_active(Mode mode, const double *x, const double *rhs, double *dst) {
+    launch_cell_op<double>(OP_ABLOCK, mode, Ltop(), x, nullptr, eta_active.p, active_per_qp, rhs, nullptr, dst,
+                           nullptr, st);
+  }
+  void mass_active(Mode mode, const double *x, const double *rhs, double *dst) {
+    launch_cell_op<double>(OP_MASS, mode, Ltop(), nullptr, x, eta_active.p, active_per_qp, nullptr, rhs, nullptr,
+                           dst, st);
+  }
+
+  // BlockSchurPreconditioner::vmult, cheap: p = -S^{-1} r_p; t = r_u - B^T p; u = A^{-1} t
+  void block_prec_cheap(const double *src, double *dst) {
+    gmg_vmult(1, src + nu, w5.p);
+    v_scale_neg_copy<double>(w5.p, dst + nu, np, st);
+    launch_cell_op<double>(OP_BT, MODE_RESID, Ltop(), nullptr, dst + nu, nullptr, 0, src, nullptr, w5.p, nullptr,
+                           st);
+    gmg_vmult(0, w5.p, dst);
+  }
+
+  // preconditioned CG (inner solves of the expensive preconditioner)
+  int cg(int kind, const double *b, double *x, long long nn, double tol, int maxit, double *r, double *z, double *p,
+         double *q) {
+    auto A = [&](const double *in, double *out) {
+      if (kind == 0)
+        ablock_active(MODE_SET, in, nullptr, out);
+      else
+        mass_active(MODE_SET, in, nullptr, out);
+    };
+    auto P = [&](const double *in, double *out) {
+      if (prec == MFS_GMG_FP32)
+        g32.vmult(kind, in, out);
+      else
+        g64.vmult(kind, in, out);
+    };
+    v_fill<double>(x, 0.0, nn, st);
+    MFS_HIP(hipMemcpyAsync(r, b, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double res = std::sqrt(dot(r, r, nn));
+    if (res <= tol) return 0;
+    P(r, z);
+    double rz = dot(r, z, nn);
+    MFS_HIP(hipMemcpyAsync(p, z, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int it = 1; it <= maxit; ++it) {
+      A(p, q);
+      const double alpha = rz / dot(p, q, nn);
+      v_axpby<double>(alpha, p, 1.0, x, nn, st);
+      v_axpby<double>(-alpha, q, 1.0, r, nn, st);
+      res = std::sqrt(dot(r, r, nn));
+      if (res <= tol) return it;
+      P(r, z);
+      const double rz2 = dot(r, z, nn);
+      const double beta = rz2 / rz;
+      rz = rz2;
+      v_axpby<double>(1.0, z, beta, p, nn, st);
+    }
+    throw std::runtime_error("inner CG solve of the expensive Stokes preconditioner did not converge");
+  }
+
+  // BlockSchurPreconditioner::vmult, expensive: inner CG solves to relative tolerances
+  double a_tol = 1e-2, s_tol = 1e-6;
+  DBuf<double> cu1, cu2, cu3, cu4, cp1, cp2, cp3, cp4;
+  void block_prec_expensive(const double *src, double *dst) {
+    const double nsp = std::sqrt(dot(src + nu, src + nu, np));
+    if (nsp > 1e-50) {
+      s_apps += cg(1, src + nu, w5.p, np, nsp * s_tol, 100, cp1.p, cp2.p, cp3.p, cp4.p);
+      v_scale_neg_copy<double>(w5.p, dst + nu, np, st);
+    } else {
+      v_fill<double>(dst + nu, 0.0, np, st);
+    }
+    launch_cell_op<double>(OP_BT, MODE_RESID, Ltop(), nullptr, dst + nu, nullptr, 0, src, nullptr, w5.p, nullptr,
+                           st);
+    const double nt = std::sqrt(dot(w5.p, w5.p, nu));
+    a_apps += cg(0, w5.p, dst, nu, nt * a_tol, 10000, cu1.p, cu2.p, cu3.p, cu4.p);
+  }
+
+  double *vec(std::vector<double *> &chunks, int i) {
+    const int c = i / 8;
+    while ((int)chunks.size() <= c) {
+      double *p = nullptr;
+      MFS_HIP(hipMalloc(reinterpret_cast<void **>(&p), (size_t)8 * n * sizeof(double)));
+      chunks.push_back(p);
+    }
+    return chunks[c] + (long long)(i % 8) * n;
+  }
+
+  // h = V[0..m)^T w ; w -= V h   (one classical Gram-Schmidt pass), h to host
+  void cgs_pass(int m, double *w, std::vector<double> &h) {
+    for (int c = 0; c * 8 < m; ++c) {
+      const int mg = std::min(8, m - 8 * c);
+      v_multidot_partial(vec(vchunks, 8 * c), n, 8 * c, mg, w, n, dot_work.p, st);
+    }
+    v_sum_partials(dot_work.p, m, hbuf.p, st);
+    for (int c = 0; c * 8 < m; ++c) {
+      const int mg = std::min(8, m - 8 * c);
+      v_multiaxpy(vec(vchunks, 8 * c), n, mg, hbuf.p + 8 * c, w, n, -1.0, st);
+    }
+    h.resize(m);
+    MFS_HIP(hipMemcpyAsync(h.data(), hbuf.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    MFS_HIP(hipStreamSynchronize(st));
+  }
+
+  struct KrylovStatus {
+    bool converged = false;
+    int its = 0;
+    double res = 0;
+  };
+
+  // restarted right-preconditioned GMRES (flexible=false) or FGMRES (flexible=true)
+  template <typename Prec>
+  KrylovStatus gmres(const double *b, double *x, double tol, int maxit, int restart, bool flexible, Prec P) {
+    KrylovStatus ks;
+    int acc = 0;
+    std::vector<double> h, hh;
+    while (true) {
+      double *v0 = vec(vchunks, 0);
+      stokes(MODE_RESID, x, b, v0);
+      double rho = std::sqrt(dot(v0, v0, n));
+      ks.res = rho;
+      if (rho <= tol) {
+        ks.converged = true;
+        ks.its = acc;
+        return ks;
+      }
+      if (acc >= maxit) {
+        ks.its = acc;
+        return ks;
+      }
+      v_scale(v0, 1.0 / rho, n, st);
+      std::vector<std::vector<double>> H;
+      std::vector<double> cs, sn, g(1, rho);
+      bool done = false;
+      int j = 0;
+      for (; j < restart; ++j) {
+        ++acc;
+        double *z = flexible ? vec(zchunks, j) : w1.p;
+        P(vec(vchunks, j), z);
+        double *w = vec(vchunks, j + 1);
+        stokes(MODE_SET, z, nullptr, w);
+        std::vector<double> col(j + 2, 0.0);
+        cgs_pass(j + 1, w, h);
+        cgs_pass(j + 1, w, hh);
+        for (int i = 0; i <= j; ++i) col[i] = h[i] + hh[i];
+        const double hn = std::sqrt(dot(w, w, n));
+        col[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+          const double a = col[i], bb = col[i + 1];
+          col[i] = cs[i] * a + sn[i] * bb;
+          col[i + 1] = -sn[i] * a + cs[i] * bb;
+        }
+        const double den = std::hypot(col[j], col[j + 1]);
+        const double c = den > 0 ? col[j] / den : 1.0, s = den > 0 ? col[j + 1] / den : 0.0;
+        cs.push_back(c);
+        sn.push_back(s);
+        col[j] = den;
+        col[j + 1] = 0.0;
+        g.push_back(-s * g[j]);
+        g[j] = c * g[j];
+        H.push_back(col);
+        rho = std::fabs(g[j + 1]);
+        ks.res = rho;
+        if (hn > 0) v_scale(w, 1.0 / hn, n, st);
+        if (rho <= tol || hn == 0.0) {
+          ks.converged = true;
+          done = true;
+          ++j;
+          break;
+        }
+        if (acc >= maxit) {
+          done = true;
+          ++j;
+          break;
+        }
+      }
+      const int m = j;
+      std::vector<double> y(m, 0.0);
+      for (int i = m - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int k = i + 1; k < m; ++k) s -= H[k][i] * y[k];
+        y[i] = s / H[i][i];
+      }
+      MFS_HIP(hipMemcpyAsync(hbuf.p, y.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+      if (flexible) {
+        for (int c = 0; c * 8 < m; ++c)
+          v_multiaxpy(vec(zchunks, 8 * c), n, std::min(8, m - 8 * c), hbuf.p + 8 * c, x, n, 1.0, st);
+      } else {
+        v_fill<double>(w2.p, 0.0, n, st);
+        for (int c = 0; c * 8 < m; ++c)
+          v_multiaxpy(vec(vchunks, 8 * c), n, std::min(8, m - 8 * c), hbuf.p + 8 * c, w2.p, n, 1.0, st);
+        P(w2.p, w3.p);
+        v_axpby<double>(1.0, w3.p, 1.0, x, n, st);
+      }
+      MFS_HIP(hipStreamSynchronize(st));
+      if (done) {
+        ks.its = acc;
+        return ks;
+      }
+    }
+  }
+
+  // the handler's solve (device vectors)
+  int solve_dev(const double *rhs, const double *x0, double *sol, const mfs_solve_params &prm, mfs_solve_result &res) {
+    if (!have_eta) throw std::runtime_error("mfs_solve: viscosity not set (call mfs_set_viscosity)");
+    if (!have_prec) throw std::runtime_error("mfs_solve: preconditioner not built (call mfs_build_preconditioner)");
+    if (prm.restart < 1 || prm.restart > 1000) throw std::runtime_error("mfs_solve: restart must be in [1, 1000]");
+    if (prm.n_cheap_steps < 0 || prm.n_expensive_steps < 0) throw std::runtime_error("mfs_solve: negative step limit");
+    std::memset(&res, 0, sizeof(res));
+    const double s = box.pressure_scaling;
+    a_apps = s_apps = 0;
+    a_tol = prm.a_block_tolerance;
+    s_tol = prm.s_block_tolerance;
+    // initial guess: constrained velocity entries zeroed, pressure / s (:944-975)
+    if (x0)
+      MFS_HIP(hipMemcpyAsync(sol, x0, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    else
+      v_fill<double>(sol, 0.0, n, st);
+    v_zero_constrained<double>(Ltop(), sol, st);
+    v_scale(sol + nu, 1.0 / s, np, st);
+    // residuals for the nonlinear residual and the solver tolerance (:977-1041)
+    stokes(MODE_RESID, sol, rhs, w1.p);
+    res.initial_nonlinear_residual = std::sqrt(dot(w1.p, w1.p, n));
+    MFS_HIP(hipMemcpyAsync(w2.p, sol, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    v_fill<double>(w2.p, 0.0, nu, st);
+    stokes(MODE_RESID, w2.p, rhs, w1.p);
+    const double ru2 = dot(w1.p, w1.p, nu);
+    const double rp2 = dot(rhs + nu, rhs + nu, np);
+    res.r0_norm = std::sqrt(ru2 + rp2);
+    const double tol = prm.tolerance * res.r0_norm;
+    res.solver_tolerance = tol;
+    hipEvent_t e0, e1;
+    MFS_HIP(hipEventCreate(&e0));
+    MFS_HIP(hipEventCreate(&e1));
+    MFS_HIP(hipEventRecord(e0, st));
+    bool ok = false;
+    if (prm.n_cheap_steps > 0) {
+      // With an FP32 V-cycle the preconditioner is linear only up to FP32
+      // rounding, so the cheap phase stores the preconditioned vectors
+      // (flexible update x += Z y) to keep the residual estimate exact.
+      const bool flexible = prec == MFS_GMG_FP32;
+      KrylovStatus ks = gmres(rhs, sol, tol, prm.n_cheap_steps, prm.restart, flexible,
+                              [&](const double *a, double *b) { block_prec_cheap(a, b); });
+      res.cheap_iterations = ks.its;
+      res.final_linear_residual = ks.res;
+      ok = ks.converged;
+    }
+    if (!ok && prm.n_expensive_steps > 0) {
+      for (DBuf<double> *v : {&cu1, &cu2, &cu3, &cu4})
+        if (v->n != nu) v->alloc(nu);
+      for (DBuf<double> *v : {&cp1, &cp2, &cp3, &cp4})
+        if (v->n != np) v->alloc(np);
+      KrylovStatus ks = gmres(rhs, sol, tol, prm.n_expensive_steps, prm.restart, true,
+                              [&](const double *a, double *b) { block_prec_expensive(a, b); });
+      res.expensive_iterations = ks.its;
+      res.final_linear_residual = ks.res;
+      ok = ks.converged;
+    }
+    MFS_HIP(hipEventRecord(e1, st));
+    MFS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res.solve_seconds = ms * 1e-3;
+    res.a_preconditioner_applications = a_apps;
+    res.s_preconditioner_applications = s_apps;
+    res.converged = ok ? 1 : 0;
+    // post-steps (:1406-1445): pressure *= s, volume normalization
+    v_scale(sol + nu, s, np, st);
+    const double ip = dot(sol + nu, pnorm_w.p, np);
+    const double vol = dot(pnorm_w.p, ones_p(), np);
+    const double adj = ip / vol;
+    v_fill<double>(w3.p, 1.0, np, st);
+    v_axpby<double>(-adj, w3.p, 1.0, sol + nu, np, st);
+    res.pressure_normalization_adjustment = -adj;
+    MFS_HIP(hipStreamSynchronize(st));
+    return ok ? MFS_OK : MFS_NO_CONVERGENCE;
+  }
+  DBuf<double> ones_p_buf;
+  const double *ones_p() {
+    if (ones_p_buf.n != np) {
+      ones_p_buf.alloc(np);
+      v_fill<double>(ones_p_buf.p, 1.0, np, st);
+    }
+    return ones_p_buf.p;
+  }
+};
+
+namespace {
+
+void create(const mfs_box *b, int prec, mfs_handle **out) {
+  if (!b || !out) throw std::runtime_error("mfs_create: null argument");
+  if (b->dim != 2 && b->dim != 3) throw std::runtime_error("mfs_create: dim must be 2 or 3");
+  if (b->degree != 2 && b->degree != 3) throw std::runtime_error("mfs_create: degree must be 2 or 3");
+  if (b->n_levels < 1 || b->n_levels > 16) throw std::runtime_error("mfs_create: n_levels out of range");
+  for (int d = 0; d < b->dim; ++d) {
+    if (b->coarse[d] < 1) throw std::runtime_error("mfs_create: coarse cells must be >= 1");
+    if (!(b->extent[d] > 0)) throw std::runtime_error("mfs_create: extents must be positive");
+    if (((long long)b->coarse[d] << (b->n_levels - 1)) > (1LL << 20))
+      throw std::runtime_error("mfs_create: level too large");
+  }
+  for (int f = 0; f < 2 * b->dim; ++f)
+    if (b->bc[f] < 0 || b->bc[f] > 2) throw std::runtime_error("mfs_create: invalid boundary type");
+  if (!(b->pressure_scaling > 0)) throw std::runtime_error("mfs_create: pressure scaling must be positive");
+  if (prec != MFS_GMG_FP64 && prec != MFS_GMG_FP32) throw std::runtime_error("mfs_create: invalid GMG precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    throw std::runtime_error("mfs_create: no HIP device available");
+  upload_basis_tables();
+  auto h = std::make_unique<mfs_handle>();
+  h->box = *b;
+  for (int d = b->dim; d < 3; ++d) {
+    h->box.coarse[d] = 1;
+    h->box.extent[d] = 1.0;
+  }
+  h->prec = prec;
+  MFS_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  for (int l = 0; l < b->n_levels; ++l) h->L.push_back(make_level_desc(h->box, l));
+  h->top = b->n_levels - 1;
+  h->nu = (long long)b->dim * h->Ltop().vnodes;
+  h->np = h->Ltop().pnodes;
+  h->n = h->nu + h->np;
+  h->nq = n_qp(h->box);
+  h->ct = cells_per_tile(h->box);
+  h->dot_work.alloc((long long)dot_work_size() * 2048);
+  h->dot_out.alloc(16);
+  h->hbuf.alloc(2048);
+  for (DBuf<double> *v : {&h->w1, &h->w2, &h->w3, &h->w4, &h->w5}) {
+    v->alloc(h->n);
+    v_fill<double>(v->p, 0.0, v->n, h->st);
+  }
+  if (prec == MFS_GMG_FP32)
+    h->g32.setup(h->box, h->st, &h->dot_work, &h->dot_out);
+  else
+    h->g64.setup(h->box, h->st, &h->dot_work, &h->dot_out);
+  // pressure normalization weights: int psi_i (unit mass applied to ones)
+  {
+    LevelDesc Lu = h->Ltop();
+    Lu.s = 1.0;
+    h->ones_cell.alloc(Lu.ntiles_total * h->ct);
+    v_fill<double>(h->ones_cell.p, 1.0, h->ones_cell.n, h->st);
+    h->pnorm_w.alloc(h->np);
+    v_fill<double>(h->w3.p, 1.0, h->np, h->st);
+    launch_cell_op<double>(OP_MASS, MODE_SET, Lu, nullptr, h->w3.p, h->ones_cell.p, 0, nullptr, nullptr, nullptr,
+                           h->pnorm_w.p, h->st);
+  }
+  MFS_HIP(hipStreamSynchronize(h->st));
+  *out = h.release();
+}
+
+void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
+  if (!eta_qp) throw std::runtime_error("mfs_set_viscosity: null viscosity");
+  if (averaging < 0 || averaging > 3) throw std::runtime_error("mfs_set_viscosity: invalid averaging");
+  const LevelDesc &L = h->Ltop();
+  const long long nqv = L.cells * h->nq;
+  double mn = 1e300, mx = -1e300;
+  for (long long i = 0; i < nqv; ++i) {
+    if (!(eta_qp[i] > 0) || !std::isfinite(eta_qp[i]))
+      throw std::runtime_error("mfs_set_viscosity: viscosities must be positive and finite");
+  }
+  if (h->eta_nat.n != nqv) h->eta_nat.alloc(nqv);
+  MFS_HIP(hipMemcpyAsync(h->eta_nat.p, eta_qp, nqv * sizeof(double), hipMemcpyHostToDevice, h->st));
+  h->active_per_qp = averaging == MFS_AVERAGE_NONE ? 1 : 0;
+  const long long na = L.ntiles_total * h->ct * (h->active_per_qp ? h->nq : 1);
+  if (h->eta_active.n != na) h->eta_active.alloc(na);
+  v_fill<double>(h->eta_active.p, 1.0, na, h->st);
+  if (h->active_per_qp) {
+    launch_eta_to_tiles<double>(L, h->eta_nat.p, 1, h->eta_active.p, h->st);
+    for (long long i = 0; i < nqv; ++i) {
+      mn = std::min(mn, eta_qp[i]);
+      mx = std::max(mx, eta_qp[i]);
+    }
+  } else {
+    launch_eta_average<double>(L, h->eta_nat.p, averaging, h->eta_active.p, h->st);
+    // min/max over the averaged values (Viscosity range report)
+    std::vector<double> tmp(na);
+    MFS_HIP(hipMemcpyAsync(tmp.data(), h->eta_active.p, na * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    MFS_HIP(hipStreamSynchronize(h->st));
+    // padding entries are 1; scan only real cells via the natural order recomputation
+    for (long long c = 0; c < L.cells; ++c) {
+      double v;
+      const double *e = eta_qp + c * h->nq;
+      if (averaging == MFS_AVERAGE_ARITHMETIC) {
+        double s = 0;
+        for (int q = 0; q < h->nq; ++q) s += e[q];
+        v = s / h->nq;
+      } else if (averaging == MFS_AVERAGE_HARMONIC) {
+        double s = 0;
+        for (int q = 0; q < h->nq; ++q) s += 1.0 / e[q];
+        v = 1.0 / (s / h->nq);
+      } else {
+        double s = 0;
+        for (int q = 0; q < h->nq; ++q) s += std::log(e[q]);
+        v = std::exp(s / h->nq);
+      }
+      mn = std::min(mn, v);
+      mx = std::max(mx, v);
+    }
+  }
+  h->eta_min = mn;
+  h->eta_max = mx;
+  if (h->prec == MFS_GMG_FP32)
+    h->g32.set_eta(h->eta_active.p, h->active_per_qp);
+  else
+    h->g64.set_eta(h->eta_active.p, h->active_per_qp);
+  MFS_HIP(hipStreamSynchronize(h->st));
+  h->have_eta = true;
+  h->have_prec = false;
+}
+
+// host <-> device helpers
+struct HostIO {
+  mfs_handle *h;
+  void in(double *d, const double *hsrc, long long n) {
+    MFS_HIP(hipMemcpyAsync(d, hsrc, n * sizeof(double), hipMemcpyHostToDevice, h->st));
+  }
+  void out(double *hdst, const double *d, long long n) {
+    MFS_HIP(hipMemcpyAsync(hdst, d, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    MFS_HIP(hipStreamSynchronize(h->st));
+  }
+};
+
+void check_level(const mfs_handle *h, int level, bool allow_active) {
+  if (level == -1 && allow_active) return;
+  if (level < 0 || level > h->top) throw std::runtime_error("level out of range");
+}
+void need_eta(const mfs_handle *h) {
+  if (!h->have_eta) throw std::runtime_error("viscosity not set (call mfs_set_viscosity)");
+}
+
+template <typename F>
+int guard(F &&f) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return MFS_ERROR;
+  }
+}
+
+// run a level operation in precision T on FP64 host vectors
+template <typename T, typename F>
+void level_op_host(mfs_handle *h, GMG<T> &g, long long nin, long long nout, const double *src, double *dst, F f) {
+  HostIO io{h};
+  io.in(h->w1.p, src, nin);
+  DBuf<T> a, b;
+  a.alloc(std::max(nin, 1LL));
+  b.alloc(std::max(nout, 1LL));
+  v_cast<double, T>(h->w1.p, a.p, nin, h->st);
+  f(a.p, b.p);
+  v_cast<T, double>(b.p, h->w2.p, nout, h->st);
+  io.out(dst, h->w2.p, nout);
+  (void)g;
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+extern "C" {
+
+const char *mfs_last_error(void) { return g_err.c_str(); }
+
+int mfs_create(const mfs_box *box, int gmg_precision, mfs_handle **out) {
+  return guard([&] {
+    create(box, gmg_precision, out);
+    return MFS_OK;
+  });
+}
+
+int mfs_destroy(mfs_handle *h) {
+  return guard([&] {
+    delete h;
+    return MFS_OK;
+  });
+}
+
+int mfs_name(const mfs_handle *h, char *buf, int len) {
+  return guard([&] {
+    if (!h || !buf || len <= 0) throw std::runtime_error("mfs_name: bad argument");
+    std::snprintf(buf, (size_t)len, "%s", "GMG");
+    return MFS_OK;
+  });
+}
+
+int mfs_sizes(const mfs_handle *h, int level, long long out[4]) {
+  return guard([&] {
+    check_level(h, level, false);
+    const LevelDesc &L = h->L[level];
+    out[0] = L.cells;
+    out[1] = (long long)L.dim * L.vnodes;
+    out[2] = L.pnodes;
+    out[3] = L.vnodes;
+    return MFS_OK;
+  });
+}
+
+int mfs_set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
+  return guard([&] {
+    set_viscosity(h, eta_qp, averaging);
+    return MFS_OK;
+  });
+}
+
+int mfs_set_pressure_scaling(mfs_handle *h, double s) {
+  return guard([&] {
+    if (!(s > 0)) throw std::runtime_error("pressure scaling must be positive");
+    h->box.pressure_scaling = s;
+    for (auto &L : h->L) L.s = s;
+    auto upd = [&](auto &g) {
+      for (auto &lv : g.lv) lv->L.s = s;
+    };
+    upd(h->g32);
+    upd(h->g64);
+    h->have_prec = false;
+    return MFS_OK;
+  });
+}
+
+int mfs_build_preconditioner(mfs_handle *h) {
+  return guard([&] {
+    need_eta(h);
+    if (h->prec == MFS_GMG_FP32)
+      h->g32.build();
+    else
+      h->g64.build();
+    MFS_HIP(hipStreamSynchronize(h->st));
+    h->have_prec = true;
+    return MFS_OK;
+  });
+}
+
+int mfs_stokes_vmult(mfs_handle *h, const double *src, double *dst) {
+  return guard([&] {
+    need_eta(h);
+    HostIO io{h};
+    io.in(h->w1.p, src, h->n);
+    h->stokes(MODE_SET, h->w1.p, nullptr, h->w2.p);
+    io.out(dst, h->w2.p, h->n);
+    return MFS_OK;
+  });
+}
+
+int mfs_stokes_vmult_device(mfs_handle *h, const double *d_src, double *d_dst) {
+  return guard([&] {
+    need_eta(h);
+    h->stokes(MODE_SET, d_src, nullptr, d_dst);
+    MFS_HIP(hipGetLastError());
+    return MFS_OK;
+  });
+}
+
+int mfs_ablock_vmult(mfs_handle *h, int level, const double *src, double *dst) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, true);
+    if (level == -1) {
+      HostIO io{h};
+      io.in(h->w1.p, src, h->nu);
+      h->ablock_active(MODE_SET, h->w1.p, nullptr, h->w2.p);
+      io.out(dst, h->w2.p, h->nu);
+      return MFS_OK;
+    }
+    const long long nu = (long long)h->box.dim * h->L[level].vnodes;
+    auto run = [&](auto &g) {
+      level_op_host(h, g, nu, nu, src, dst, [&](auto *a, auto *b) { g.applyA(level, MODE_SET, a, nullptr, b); });
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_ablock_vmult_device(mfs_handle *h, int level, const void *d_src, void *d_dst) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, false);
+    if (h->prec == MFS_GMG_FP32)
+      h->g32.applyA(level, MODE_SET, (const float *)d_src, nullptr, (float *)d_dst);
+    else
+      h->g64.applyA(level, MODE_SET, (const double *)d_src, nullptr, (double *)d_dst);
+    MFS_HIP(hipGetLastError());
+    return MFS_OK;
+  });
+}
+
+int mfs_mass_vmult(mfs_handle *h, int level, const double *src, double *dst) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, true);
+    if (level == -1) {
+      HostIO io{h};
+      io.in(h->w1.p, src, h->np);
+      h->mass_active(MODE_SET, h->w1.p, nullptr, h->w2.p);
+      io.out(dst, h->w2.p, h->np);
+      return MFS_OK;
+    }
+    const long long np = h->L[level].pnodes;
+    auto run = [&](auto &g) {
+      level_op_host(h, g, np, np, src, dst, [&](auto *a, auto *b) { g.applyS(level, MODE_SET, a, nullptr, b); });
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_bt_vmult(mfs_handle *h, const double *src_p, double *dst_u) {
+  return guard([&] {
+    HostIO io{h};
+    io.in(h->w1.p, src_p, h->np);
+    launch_cell_op<double>(OP_BT, MODE_SET, h->Ltop(), nullptr, h->w1.p, nullptr, 0, nullptr, nullptr, h->w2.p,
+                           nullptr, h->st);
+    io.out(dst_u, h->w2.p, h->nu);
+    return MFS_OK;
+  });
+}
+
+int mfs_ablock_diagonal(mfs_handle *h, int level, double *diag) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, false);
+    const long long nu = (long long)h->box.dim * h->L[level].vnodes;
+    auto run = [&](auto &g) {
+      using T = std::remove_pointer_t<decltype(g.lv[0]->eta.p)>;
+      DBuf<T> d;
+      d.alloc(nu);
+      launch_cell_op<T>(OP_ADIAG, MODE_SET, g.lv[level]->L, nullptr, nullptr, g.lv[level]->eta.p, 0, nullptr,
+                        nullptr, d.p, nullptr, h->st);
+      v_cast<T, double>(d.p, h->w2.p, nu, h->st);
+      HostIO{h}.out(diag, h->w2.p, nu);
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_mass_diagonal(mfs_handle *h, int level, double *diag) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, false);
+    const long long np = h->L[level].pnodes;
+    auto run = [&](auto &g) {
+      using T = std::remove_pointer_t<decltype(g.lv[0]->eta.p)>;
+      DBuf<T> d;
+      d.alloc(np);
+      launch_cell_op<T>(OP_MDIAG, MODE_SET, g.lv[level]->L, nullptr, nullptr, g.lv[level]->eta.p, 0, nullptr,
+                        nullptr, nullptr, d.p, h->st);
+      v_cast<T, double>(d.p, h->w2.p, np, h->st);
+      HostIO{h}.out(diag, h->w2.p, np);
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_prolongate(mfs_handle *h, int fine_level, int kind, const double *src, double *dst) {
+  return guard([&] {
+    if (fine_level < 1 || fine_level > h->top) throw std::runtime_error("fine_level out of range");
+    if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 or 1");
+    const LevelDesc &Lc = h->L[fine_level - 1], &Lf = h->L[fine_level];
+    const long long nc = kind == 0 ? (long long)h->box.dim * Lc.vnodes : Lc.pnodes;
+    const long long nf = kind == 0 ? (long long)h->box.dim * Lf.vnodes : Lf.pnodes;
+    HostIO io{h};
+    io.in(h->w1.p, src, nc);
+    launch_prolongate<double>(kind, Lc, Lf, h->w1.p, h->w2.p, false, h->st);
+    io.out(dst, h->w2.p, nf);
+    return MFS_OK;
+  });
+}
+
+int mfs_restrict(mfs_handle *h, int fine_level, int kind, const double *src, double *dst) {
+  return guard([&] {
+    if (fine_level < 1 || fine_level > h->top) throw std::runtime_error("fine_level out of range");
+    if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 or 1");
+    const LevelDesc &Lc = h->L[fine_level - 1], &Lf = h->L[fine_level];
+    const long long nc = kind == 0 ? (long long)h->box.dim * Lc.vnodes : Lc.pnodes;
+    const long long nf = kind == 0 ? (long long)h->box.dim * Lf.vnodes : Lf.pnodes;
+    HostIO io{h};
+    io.in(h->w1.p, src, nf);
+    launch_restrict<double>(kind, Lc, Lf, h->w1.p, h->w2.p, h->st);
+    io.out(dst, h->w2.p, nc);
+    return MFS_OK;
+  });
+}
+
+int mfs_level_viscosity(mfs_handle *h, int level, double *out) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, false);
+    const LevelDesc &L = h->L[level];
+    int t[3];
+    tile_cells(L.dim, L.k, t);
+    auto run = [&](auto &g) {
+      using T = std::remove_pointer_t<decltype(g.lv[0]->eta.p)>;
+      std::vector<T> tiles(g.lv[level]->eta.n);
+      MFS_HIP(hipMemcpyAsync(tiles.data(), g.lv[level]->eta.p, tiles.size() * sizeof(T), hipMemcpyDeviceToHost,
+                             h->st));
+      MFS_HIP(hipStreamSynchronize(h->st));
+      const long long CT = (long long)t[0] * t[1] * t[2];
+      for (long long c = 0; c < L.cells; ++c) {
+        const int gx = (int)(c % L.n[0]), gy = (int)((c / L.n[0]) % L.n[1]), gz = (int)(c / ((long long)L.n[0] * L.n[1]));
+        const long long tile = gx / t[0] + (long long)L.ntiles[0] * (gy / t[1] + (long long)L.ntiles[1] * (gz / t[2]));
+        const long long ci = gx % t[0] + t[0] * (gy % t[1] + t[1] * (gz % t[2]));
+        out[c] = (double)tiles[tile * CT + ci];
+      }
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_smoother_info(mfs_handle *h, double *out) {
+  return guard([&] {
+    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    auto run = [&](auto &g) {
+      for (int l = 0; l <= h->top; ++l) {
+        const ChebInfo *cs[2] = {&g.lv[l]->chA, &g.lv[l]->chS};
+        for (int k = 0; k < 2; ++k) {
+          double *o = out + (l * 2 + k) * 5;
+          o[0] = cs[k]->lmin;
+          o[1] = cs[k]->lmax;
+          o[2] = cs[k]->theta;
+          o[3] = cs[k]->delta;
+          o[4] = cs[k]->its;
+        }
+      }
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_chebyshev(mfs_handle *h, int kind, int level, const double *src, double *dst) {
+  return guard([&] {
+    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    check_level(h, level, false);
+    const long long nn = kind == 0 ? (long long)h->box.dim * h->L[level].vnodes : h->L[level].pnodes;
+    auto run = [&](auto &g) {
+      level_op_host(h, g, nn, nn, src, dst, [&](auto *a, auto *b) {
+        auto &lv = *g.lv[level];
+        (void)lv;
+        g.smooth(kind, level, b, a, true, 1);
+      });
+    };
+    if (h->prec == MFS_GMG_FP32)
+      run(h->g32);
+    else
+      run(h->g64);
+    return MFS_OK;
+  });
+}
+
+int mfs_vcycle(mfs_handle *h, int kind, const double *src, double *dst) {
+  return guard([&] {
+    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    const long long nn = kind == 0 ? h->nu : h->np;
+    HostIO io{h};
+    io.in(h->w1.p, src, nn);
+    if (h->prec == MFS_GMG_FP32)
+      h->g32.vmult(kind, h->w1.p, h->w2.p);
+    else
+      h->g64.vmult(kind, h->w1.p, h->w2.p);
+    io.out(dst, h->w2.p, nn);
+    return MFS_OK;
+  });
+}
+
+int mfs_block_preconditioner(mfs_handle *h, const double *src, double *dst) {
+  return guard([&] {
+    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    HostIO io{h};
+    io.in(h->w3.p, src, h->n);
+    h->block_prec_cheap(h->w3.p, h->w4.p);
+    io.out(dst, h->w4.p, h->n);
+    return MFS_OK;
+  });
+}
+
+int mfs_solve_device(mfs_handle *h, const double *d_rhs, const double *d_x0, double *d_sol,
+                     const mfs_solve_params *prm, mfs_solve_result *res) {
+  return guard([&] {
+    if (!prm || !res) throw std::runtime_error("mfs_solve: null params/result");
+    return h->solve_dev(d_rhs, d_x0, d_sol, *prm, *res);
+  });
+}
+
+int mfs_solve(mfs_handle *h, const double *rhs, const double *x0, double *sol, const mfs_solve_params *prm,
+              mfs_solve_result *res) {
+  return guard([&] {
+    if (!prm || !res || !rhs || !sol) throw std::runtime_error("mfs_solve: null argument");
+    DBuf<double> drhs, dx0, dsol;
+    drhs.alloc(h->n);
+    dsol.alloc(h->n);
+    MFS_HIP(hipMemcpyAsync(drhs.p, rhs, h->n * sizeof(double), hipMemcpyHostToDevice, h->st));
+    if (x0) {
+      dx0.alloc(h->n);
+      MFS_HIP(hipMemcpyAsync(dx0.p, x0, h->n * sizeof(double), hipMemcpyHostToDevice, h->st));
+    }
+    const int rc = h->solve_dev(drhs.p, x0 ? dx0.p : nullptr, dsol.p, *prm, *res);
+    HostIO{h}.out(sol, dsol.p, h->n);
+    return rc;
+  });
+}
+
+static void assemble_rhs_dev(mfs_handle *h, const double *d_rho_nat, const double *gravity, double *d_rhs) {
+  if (!gravity) throw std::runtime_error("mfs_assemble_buoyancy_rhs: null gravity");
+  LevelDesc L = h->Ltop();
+  for (int d = 0; d < 3; ++d) L.gravity[d] = d < h->box.dim ? gravity[d] : 0.0;
+  DBuf<double> rho;
+  rho.alloc(L.ntiles_total * h->ct * h->nq);
+  v_fill<double>(rho.p, 0.0, rho.n, h->st);
+  launch_eta_to_tiles<double>(L, d_rho_nat, 1, rho.p, h->st);
+  launch_cell_op<double>(OP_RHS, MODE_SET, L, nullptr, nullptr, rho.p, 1, nullptr, nullptr, d_rhs, nullptr, h->st);
+  v_fill<double>(d_rhs + h->nu, 0.0, h->np, h->st);
+  MFS_HIP(hipStreamSynchronize(h->st));
+}
+
+int mfs_assemble_buoyancy_rhs(mfs_handle *h, const double *rho_qp, const double *gravity, double *rhs) {
+  return guard([&] {
+    if (!rho_qp || !rhs) throw std::runtime_error("mfs_assemble_buoyancy_rhs: null argument");
+    const long long nqv = h->Ltop().cells * h->nq;
+    DBuf<double> rho, out;
+    rho.alloc(nqv);
+    out.alloc(h->n);
+    MFS_HIP(hipMemcpyAsync(rho.p, rho_qp, nqv * sizeof(double), hipMemcpyHostToDevice, h->st));
+    assemble_rhs_dev(h, rho.p, gravity, out.p);
+    HostIO{h}.out(rhs, out.p, h->n);
+    return MFS_OK;
+  });
+}
+
+int mfs_assemble_buoyancy_rhs_device(mfs_handle *h, const double *d_rho_qp, const double *gravity, double *d_rhs) {
+  return guard([&] {
+    assemble_rhs_dev(h, d_rho_qp, gravity, d_rhs);
+    return MFS_OK;
+  });
+}
+
+int mfs_time_operator(mfs_handle *h, int which, int level, const void *d_src, void *d_dst, int reps,
+                      double *avg_seconds) {
+  return guard([&] {
+    need_eta(h);
+    if (reps < 1 || !avg_seconds) throw std::runtime_error("mfs_time_operator: bad arguments");
+    if ((which >= 3) && !h->have_prec) throw std::runtime_error("preconditioner not built");
+    if (which == 1 || which == 2) check_level(h, level, false);
+    auto body = [&]() {
+      switch (which) {
+        case 0:
+          h->stokes(MODE_SET, (const double *)d_src, nullptr, (double *)d_dst);
+          break;
+        case 1:
+        case 2:
+          if (h->prec == MFS_GMG_FP32)
+            h->g32.apply(which - 1, level, MODE_SET, (const float *)d_src, nullptr, (float *)d_dst);
+          else
+            h->g64.apply(which - 1, level, MODE_SET, (const double *)d_src, nullptr, (double *)d_dst);
+          break;
+        case 3:
+        case 4:
+          if (h->prec == MFS_GMG_FP32)
+            h->g32.vmult(which - 3, (const double *)d_src, (double *)d_dst);
+          else
+            h->g64.vmult(which - 3, (const double *)d_src, (double *)d_dst);
+          break;
+        case 5:
+          h->block_prec_cheap((const double *)d_src, (double *)d_dst);
+          break;
+        default:
+          throw std::runtime_error("mfs_time_operator: unknown operator");
+      }
+    };
+    hipEvent_t e0, e1;
+    MFS_HIP(hipEventCreate(&e0));
+    MFS_HIP(hipEventCreate(&e1));
+    MFS_HIP(hipEventRecord(e0, h->st));
+    for (int r = 0; r < reps; ++r) body();
+    MFS_HIP(hipEventRecord(e1, h->st));
+    MFS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_seconds = ms * 1e-3 / reps;
+    return MFS_OK;
+  });
+}
+
+int mfs_synchronize(mfs_handle *h) {
+  return guard([&] {
+    MFS_HIP(hipStreamSynchronize(h->st));
+    return MFS_OK;
+  });
+}
+
+int mfs_memory_consumption(const mfs_handle *h, long long out[3]) {
+  return guard([&] {
+    long long vec = 0, cell = h->eta_active.bytes() + h->eta_nat.bytes();
+    for (const DBuf<double> *v : {&h->w1, &h->w2, &h->w3, &h->w4, &h->w5, &h->pnorm_w}) vec += v->bytes();
+    auto acc = [&](const auto &g) {
+      for (const auto &lv : g.lv) {
+        cell += lv->eta.bytes();
+        vec += lv->dinvA.bytes() + lv->dinvS.bytes() + lv->ux.bytes() + lv->ub.bytes() + lv->ur.bytes() +
+               lv->ud.bytes() + lv->px.bytes() + lv->pb.bytes() + lv->pr.bytes() + lv->pd.bytes();
+      }
+    };
+    acc(h->g32);
+    acc(h->g64);
+    out[0] = vec;
+    out[1] = cell;
+    out[2] = (long long)(h->vchunks.size() + h->zchunks.size()) * 8 * h->n * (long long)sizeof(double);
+    return MFS_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,1328 @@
+// gfx950 kernels for the matrix-free Q_k x Q_{k-1} Stokes operators on uniform
+// box levels, the level transfers, viscosity tables and the vector kernels of
+// the Chebyshev / Krylov iterations.
+//
+// Operator kernels restate, per cell:
+//   StokesOperator::local_apply   source/simulator/solver/matrix_free_operators.cc:376-547
+//   BTBlockOperator::local_apply  :668-701
+//   MassMatrixOperator::inner_cell_operation  :808-845
+//   ABlockOperator::inner_cell_operation      :974-1009
+//   compute_diagonal of A and of the mass     :1068-1098, :861-908
+//
+// Work decomposition: one workgroup per tile of cells (Tile<dim,k>), one thread
+// per (cell, y-slice).  A thread keeps the x and z directions of its slice in
+// registers; the y contraction of the sum factorization goes through LDS.
+// Contributions of the cells of a tile are assembled in LDS in 2^dim cell-parity
+// phases (cells of one parity share no node), so the tile sum is
+// deterministic.  Tiles are launched in 2^dim colours (tile parity); tiles of
+// one colour share no node, so a tile writes every node of its box without
+// atomics: the lowest colour touching a node stores it, later colours add.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "mfs_internal.h"
+
+namespace mfs {
+
+// ------------------------------------------------------------------ 1D tables
+struct BasisConst {
+  double S[4][4];   // velocity basis value  [q][node]
+  double G[4][4];   // velocity basis derivative on [0,1]
+  double SP[4][4];  // pressure basis value
+  double S2[4][4], G2[4][4], SP2[4][4];  // squares (diagonal kernels)
+  double w[4];      // Gauss weights on [0,1]
+  double W[7][4];   // prolongation weights, velocity degree k:  [r][m], r < 2k+1
+  double WP[5][3];  // prolongation weights, pressure degree k-1
+};
+__constant__ BasisConst c_basis[2];  // index k-2
+
+namespace {
+double lag(const std::vector<double> &pts, int i, double x) {
+  double v = 1.0;
+  for (size_t j = 0; j < pts.size(); ++j)
+    if ((int)j != i) v *= (x - pts[j]) / (pts[i] - pts[j]);
+  return v;
+}
+double lagd(const std::vector<double> &pts, int i, double x) {
+  double s = 0.0;
+  for (size_t m = 0; m < pts.size(); ++m) {
+    if ((int)m == i) continue;
+    double t = 1.0 / (pts[i] - pts[m]);
+    for (size_t j = 0; j < pts.size(); ++j)
+      if ((int)j != i && j != m) t *= (x - pts[j]) / (pts[i] - pts[j]);
+    s += t;
+  }
+  return s;
+}
+// Gauss-Legendre on [0,1] by Newton iteration on P_n
+void gauss(int n, std::vector<double> &x, std::vector<double> &w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[n - 1 - i] = 0.5 * (z + 1.0);
+    w[n - 1 - i] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+// FE_Q support points: Gauss-Lobatto (endpoints + roots of P'_k)
+std::vector<double> lobatto(int k) {
+  std::vector<double> x(k + 1);
+  x[0] = 0;
+  x[k] = 1;
+  for (int i = 1; i < k; ++i) {
+    double z = -std::cos(M_PI * i / k);
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int m = 2; m <= k; ++m) {
+        double p2 = ((2.0 * m - 1.0) * z * p1 - (m - 1.0) * p0) / m;
+        p0 = p1;
+        p1 = p2;
+      }
+      double d1 = k * (z * p1 - p0) / (z * z - 1.0);
+      double d2 = (2.0 * z * d1 - k * (k + 1.0) * p1) / (1.0 - z * z);
+      double dz = d1 / d2;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = 0.5 * (z + 1.0);
+  }
+  return x;
+}
+// prolongation weights of a degree kk Lagrange basis: fine node r (0..2kk) of a
+// coarse cell sits at xi = (r/kk + pts[r%kk]) / 2; weights are the coarse basis there
+void prolong_table(int kk, double *tab, int ld) {
+  std::vector<double> pts = lobatto(kk);
+  for (int r = 0; r <= 2 * kk; ++r) {
+    int fc = r < 2 * kk ? r / kk : 1;
+    int i = r < 2 * kk ? r - kk * fc : kk;
+    double xi = 0.5 * (fc + pts[i]);
+    for (int m = 0; m <= kk; ++m) {
+      double v = lag(pts, m, xi);
+      tab[r * ld + m] = std::fabs(v) < 1e-15 ? 0.0 : v;
+    }
+  }
+}
+}  // namespace
+
+void upload_basis_tables() {
+  static bool done = false;
+  if (done) return;
+  BasisConst b[2] = {};
+  for (int k = 2; k <= 3; ++k) {
+    BasisConst &B = b[k - 2];
+    std::vector<double> qx, qw;
+    gauss(k + 1, qx, qw);
+    std::vector<double> vp = lobatto(k), pp = lobatto(k - 1);
+    for (int q = 0; q <= k; ++q) {
+      B.w[q] = qw[q];
+      for (int i = 0; i <= k; ++i) {
+        B.S[q][i] = lag(vp, i, qx[q]);
+        B.G[q][i] = lagd(vp, i, qx[q]);
+        B.S2[q][i] = B.S[q][i] * B.S[q][i];
+        B.G2[q][i] = B.G[q][i] * B.G[q][i];
+      }
+      for (int i = 0; i < k; ++i) {
+        B.SP[q][i] = lag(pp, i, qx[q]);
+        B.SP2[q][i] = B.SP[q][i] * B.SP[q][i];
+      }
+    }
+    prolong_table(k, &B.W[0][0], 4);
+    prolong_table(k - 1, &B.WP[0][0], 3);
+  }
+  MFS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_basis), b, sizeof(b)));
+  done = true;
+}
+
+// ------------------------------------------------------------------ configuration
+template <int DIM, int K>
+struct Tile;
+template <>
+struct Tile<3, 2> {
+  static constexpr int TX = 4, TY = 4, TZ = 4;
+};
+template <>
+struct Tile<3, 3> {
+  static constexpr int TX = 4, TY = 4, TZ = 2;
+};
+template <>
+struct Tile<2, 2> {
+  static constexpr int TX = 8, TY = 8, TZ = 1;
+};
+template <>
+struct Tile<2, 3> {
+  static constexpr int TX = 8, TY = 8, TZ = 1;
+};
+
+template <int DIM, int K>
+struct Cfg {
+  static constexpr int N = K + 1, Q = K + 1, NP = K;
+  static constexpr int ZN = DIM == 3 ? N : 1, ZQ = DIM == 3 ? Q : 1, ZNP = DIM == 3 ? NP : 1;
+  static constexpr int TX = Tile<DIM, K>::TX, TY = Tile<DIM, K>::TY, TZ = Tile<DIM, K>::TZ;
+  static constexpr int CT = TX * TY * TZ;
+  static constexpr int THREADS = CT * Q;
+  static constexpr int VX = K * TX + 1, VY = K * TY + 1, VZ = DIM == 3 ? K * TZ + 1 : 1;
+  static constexpr int PX = (K - 1) * TX + 1, PY = (K - 1) * TY + 1, PZ = DIM == 3 ? (K - 1) * TZ + 1 : 1;
+  static constexpr int VN = VX * VY * VZ, PN = PX * PY * PZ;
+  static constexpr int NQ = ZQ * Q * Q, NPL = ZNP * NP * NP;
+  static constexpr int QS = ZQ * Q;  // quadrature points of one y-slice
+  static constexpr int XBUF = 3 * QS * Q * CT;
+  static constexpr int PBUF = NPL * Q * CT;
+  static constexpr int NBUF = DIM * VN + PN;
+  static constexpr int SMEM = (XBUF > NBUF ? (XBUF > PBUF ? XBUF : PBUF) : (NBUF > PBUF ? NBUF : PBUF));
+};
+
+int tile_cells(int dim, int k, int out[3]) {
+  if (dim == 3 && k == 2) {
+    out[0] = Tile<3, 2>::TX, out[1] = Tile<3, 2>::TY, out[2] = Tile<3, 2>::TZ;
+  } else if (dim == 3 && k == 3) {
+    out[0] = Tile<3, 3>::TX, out[1] = Tile<3, 3>::TY, out[2] = Tile<3, 3>::TZ;
+  } else if (dim == 2 && k == 2) {
+    out[0] = Tile<2, 2>::TX, out[1] = Tile<2, 2>::TY, out[2] = 1;
+  } else if (dim == 2 && k == 3) {
+    out[0] = Tile<2, 3>::TX, out[1] = Tile<2, 3>::TY, out[2] = 1;
+  } else {
+    throw std::runtime_error("unsupported (dim, degree)");
+  }
+  return out[0] * out[1] * out[2];
+}
+int tile_threads(int dim, int k) {
+  int t[3];
+  return tile_cells(dim, k, t) * (k + 1);
+}
+
+#define CB c_basis[K - 2]
+
+__device__ __forceinline__ bool is_con(const LevelDesc &L, int comp, int gx, int gy, int gz) {
+  const int g[3] = {gx, gy, gz};
+  for (int d = 0; d < L.dim; ++d) {
+    int b;
+    if (g[d] == 0)
+      b = L.bc[2 * d];
+    else if (g[d] == L.nv[d] - 1)
+      b = L.bc[2 * d + 1];
+    else
+      continue;
+    if (b == 2 || (b == 1 && comp == d)) return true;
+  }
+  return false;
+}
+
+// x/z interpolation of one scalar node slice in[ZN][N] into
+// a (values), ax (x-derivative), az (z-derivative), all [ZQ][Q].
+template <int DIM, int K, typename T>
+__device__ __forceinline__ void eval_xz(const T (&in)[Cfg<DIM, K>::ZN][K + 1], T (&a)[Cfg<DIM, K>::ZQ][K + 1],
+                                        T (&ax)[Cfg<DIM, K>::ZQ][K + 1], T (&az)[Cfg<DIM, K>::ZQ][K + 1]) {
+  constexpr int N = K + 1, Q = K + 1, ZN = Cfg<DIM, K>::ZN, ZQ = Cfg<DIM, K>::ZQ;
+  T x0[ZN][Q], x1[ZN][Q];
+#pragma unroll
+  for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      T s0 = 0, s1 = 0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        s0 += (T)CB.S[qx][i] * in[kz][i];
+        s1 += (T)CB.G[qx][i] * in[kz][i];
+      }
+      x0[kz][qx] = s0;
+      x1[kz][qx] = s1;
+    }
+  if constexpr (DIM == 3) {
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int kz = 0; kz < ZN; ++kz) {
+          s0 += (T)CB.S[qz][kz] * x0[kz][qx];
+          s1 += (T)CB.S[qz][kz] * x1[kz][qx];
+          s2 += (T)CB.G[qz][kz] * x0[kz][qx];
+        }
+        a[qz][qx] = s0;
+        ax[qz][qx] = s1;
+        az[qz][qx] = s2;
+      }
+  } else {
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      a[0][qx] = x0[0][qx];
+      ax[0][qx] = x1[0][qx];
+      az[0][qx] = 0;
+    }
+  }
+}
+
+// transposed x/z step: out[kz][i] = sum_q [ MX1[qx][i] MZ1[qz][kz] t1 + MX2[qx][i] (MZ2[qz][kz] t2 + MZ3[qz][kz] t3) ]
+// with (MX1, MZ1) = (G, S), (MX2, MZ2, MZ3) = (S, S, G) for the gradient test functions.
+template <int DIM, int K, typename T, bool SQUARED>
+__device__ __forceinline__ void integ_xz(const T (&bx)[Cfg<DIM, K>::ZQ][K + 1], const T (&by)[Cfg<DIM, K>::ZQ][K + 1],
+                                         const T (&bz)[Cfg<DIM, K>::ZQ][K + 1], T (&out)[Cfg<DIM, K>::ZN][K + 1]) {
+  constexpr int N = K + 1, Q = K + 1, ZN = Cfg<DIM, K>::ZN, ZQ = Cfg<DIM, K>::ZQ;
+  T t1[ZN][Q], t2[ZN][Q];
+  if constexpr (DIM == 3) {
+#pragma unroll
+    for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        T s1 = 0, s2 = 0;
+#pragma unroll
+        for (int qz = 0; qz < ZQ; ++qz) {
+          const T sv = SQUARED ? (T)CB.S2[qz][kz] : (T)CB.S[qz][kz];
+          const T gv = SQUARED ? (T)CB.G2[qz][kz] : (T)CB.G[qz][kz];
+          s1 += sv * bx[qz][qx];
+          s2 += sv * by[qz][qx] + gv * bz[qz][qx];
+        }
+        t1[kz][qx] = s1;
+        t2[kz][qx] = s2;
+      }
+  } else {
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      t1[0][qx] = bx[0][qx];
+      t2[0][qx] = by[0][qx];
+    }
+  }
+#pragma unroll
+  for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      T s = 0;
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        const T sv = SQUARED ? (T)CB.S2[qx][i] : (T)CB.S[qx][i];
+        const T gv = SQUARED ? (T)CB.G2[qx][i] : (T)CB.G[qx][i];
+        s += gv * t1[kz][qx] + sv * t2[kz][qx];
+      }
+      out[kz][i] = s;
+    }
+}
+
+// ------------------------------------------------------------------ the cell kernel
+template <int DIM, int K, int OP, int MODE, typename T>
+__global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
+    k_cell(const LevelDesc L, const int color, const T *__restrict__ src_u, const T *__restrict__ src_p,
+           const T *__restrict__ eta, const int eta_per_qp, const T *__restrict__ rhs_u,
+           const T *__restrict__ rhs_p, T *__restrict__ dst_u, T *__restrict__ dst_p) {
+  using C = Cfg<DIM, K>;
+  constexpr int N = C::N, Q = C::Q, NP = C::NP, ZN = C::ZN, ZQ = C::ZQ, ZNP = C::ZNP, CT = C::CT;
+  constexpr int QS = C::QS;
+  constexpr bool NEED_U = (OP == OP_STOKES || OP == OP_ABLOCK);
+  constexpr bool NEED_P = (OP == OP_STOKES || OP == OP_BT || OP == OP_MASS);
+  constexpr bool OUT_U = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT || OP == OP_ADIAG || OP == OP_RHS);
+  constexpr bool OUT_P = (OP == OP_STOKES || OP == OP_MASS || OP == OP_MDIAG);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);
+  T *nb = sm;                 // [DIM][VZ][VY][VX] tile velocity nodes
+  T *pb = sm + DIM * C::VN;   // [PZ][PY][PX] tile pressure nodes
+  T *xb = sm;                 // exchange buffer (aliases nb/pb)
+
+  const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+  const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
+  int b = blockIdx.x;
+  const int tx = 2 * (b % hx) + px;
+  b /= hx;
+  const int ty = 2 * (b % hy) + py;
+  b /= hy;
+  const int tz = 2 * b + pz;
+  const long long tile_id = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
+  const int tid = threadIdx.x;
+  const int c = tid % CT, j = tid / CT;
+  const int cx = c % C::TX, cy = (c / C::TX) % C::TY, cz = c / (C::TX * C::TY);
+  const bool valid = (tx * C::TX + cx < L.n[0]) && (ty * C::TY + cy < L.n[1]) &&
+                     (DIM == 2 || tz * C::TZ + cz < L.n[2]);
+  const int vx0 = K * C::TX * tx, vy0 = K * C::TY * ty, vz0 = DIM == 3 ? K * C::TZ * tz : 0;
+  const int qx0 = (K - 1) * C::TX * tx, qy0 = (K - 1) * C::TY * ty, qz0 = DIM == 3 ? (K - 1) * C::TZ * tz : 0;
+
+  // ---- 1. cooperative tile load (constrained velocity DoFs read as zero)
+  if constexpr (NEED_U) {
+    for (int t = tid; t < DIM * C::VN; t += C::THREADS) {
+      const int comp = t / C::VN, r = t - comp * C::VN;
+      const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
+      const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
+      T v = 0;
+      if (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2] && !is_con(L, comp, gx, gy, gz))
+        v = src_u[comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz)];
+      nb[t] = v;
+    }
+  }
+  if constexpr (NEED_P) {
+    for (int t = tid; t < C::PN; t += C::THREADS) {
+      const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
+      const int gx = qx0 + lx, gy = qy0 + ly, gz = qz0 + lz;
+      T v = 0;
+      if (gx < L.np[0] && gy < L.np[1] && gz < L.np[2])
+        v = src_p[gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz)];
+      pb[t] = v;
+    }
+  }
+  if constexpr (NEED_U || NEED_P) __syncthreads();
+
+  T u[DIM][ZN][N];
+  T p[ZNP][NP][NP];
+  if constexpr (NEED_U) {
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+      for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+          u[comp][kz][i] = nb[comp * C::VN + ((K * cz + kz) * C::VY + (K * cy + j)) * C::VX + K * cx + i];
+  }
+  if constexpr (NEED_P) {
+#pragma unroll
+    for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+      for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+          p[kz][jy][i] = pb[(((K - 1) * cz + kz) * C::PY + ((K - 1) * cy + jy)) * C::PX + (K - 1) * cx + i];
+  }
+  __syncthreads();  // nb/pb now reused as exchange buffer
+
+  const T ih[3] = {(T)(1.0 / L.h[0]), (T)(1.0 / L.h[1]), (T)(DIM == 3 ? 1.0 / L.h[2] : 1.0)};
+  const double jac = L.h[0] * L.h[1] * (DIM == 3 ? L.h[2] : 1.0);
+
+  // ---- 2. velocity gradients at the quadrature points of slice qy=j
+  T grad[DIM][DIM][ZQ][Q];
+  if constexpr (NEED_U) {
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp) {
+      T a[ZQ][Q], ax[ZQ][Q], az[ZQ][Q];
+      eval_xz<DIM, K, T>(u[comp], a, ax, az);
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          const int q = qz * Q + qx;
+          xb[((0 * QS + q) * Q + j) * CT + c] = ax[qz][qx];
+          xb[((1 * QS + q) * Q + j) * CT + c] = a[qz][qx];
+          if constexpr (DIM == 3) xb[((2 * QS + q) * Q + j) * CT + c] = az[qz][qx];
+        }
+      __syncthreads();
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          const int q = qz * Q + qx;
+          T gx = 0, gy = 0, gz = 0;
+#pragma unroll
+          for (int jj = 0; jj < N; ++jj) {
+            gx += (T)CB.S[j][jj] * xb[((0 * QS + q) * Q + jj) * CT + c];
+            gy += (T)CB.G[j][jj] * xb[((1 * QS + q) * Q + jj) * CT + c];
+            if constexpr (DIM == 3) gz += (T)CB.S[j][jj] * xb[((2 * QS + q) * Q + jj) * CT + c];
+          }
+          grad[comp][0][qz][qx] = gx * ih[0];
+          grad[comp][1][qz][qx] = gy * ih[1];
+          if constexpr (DIM == 3) grad[comp][2][qz][qx] = gz * ih[2];
+        }
+      __syncthreads();
+    }
+  }
+
+  // pressure values at the slice's quadrature points (all in registers)
+  T pq[ZQ][Q];
+  if constexpr (NEED_P) {
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        T s = 0;
+#pragma unroll
+        for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+          for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+              const T wz = DIM == 3 ? (T)CB.SP[qz][kz] : (T)1;
+              s += wz * (T)CB.SP[j][jy] * (T)CB.SP[qx][i] * p[kz][jy][i];
+            }
+        pq[qz][qx] = s;
+      }
+  }
+
+  // ---- 3. quadrature point operation
+  T flux[DIM][DIM][ZQ][Q];  // [comp][dir]: stress * JxW / h_dir
+  T fp[ZQ][Q];              // pressure test-function flux * JxW
+  const T s = (T)L.s;
+#pragma unroll
+  for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) {
+      const int qn = qx + Q * (j + Q * qz);
+      const double wq = CB.w[qx] * CB.w[j] * (DIM == 3 ? CB.w[qz] : 1.0);
+      const T JxW = (T)(jac * wq);
+      T et = 1;
+      if constexpr (OP != OP_BT) {
+        et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
+      }
+      if constexpr (OP == OP_STOKES || OP == OP_ABLOCK) {
+        T div = 0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) div += grad[a][a][qz][qx];
+        const T e2 = 2 * et;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+          for (int bb = 0; bb < DIM; ++bb) {
+            T sig = et * (grad[a][bb][qz][qx] + grad[bb][a][qz][qx]);
+            if (a == bb) {
+              if (L.compressible) sig -= e2 / 3 * div;
+              if constexpr (OP == OP_STOKES) sig -= s * pq[qz][qx];
+            }
+            flux[a][bb][qz][qx] = sig * JxW * ih[bb];
+          }
+        if constexpr (OP == OP_STOKES) fp[qz][qx] = -s * div * JxW;
+      } else if constexpr (OP == OP_BT) {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+          for (int bb = 0; bb < DIM; ++bb) flux[a][bb][qz][qx] = a == bb ? -s * pq[qz][qx] * JxW * ih[bb] : (T)0;
+      } else if constexpr (OP == OP_MASS) {
+        fp[qz][qx] = s * s / et * pq[qz][qx] * JxW;
+      } else if constexpr (OP == OP_MDIAG) {
+        fp[qz][qx] = s * s / et * JxW;
+      } else if constexpr (OP == OP_RHS) {
+        flux[0][0][qz][qx] = et * JxW;
+      } else if constexpr (OP == OP_ADIAG) {
+        // D_d = sum_q eta JxW (dN/dx_d)^2 : per direction squared-matrix integrals
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) flux[0][d][qz][qx] = et * JxW * ih[d] * ih[d];
+      }
+    }
+
+  // ---- 4. integrate velocity test functions
+  T out[DIM][ZN][N];
+  if constexpr (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT) {
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp) {
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          const int q = qz * Q + qx;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) xb[((d * QS + q) * Q + j) * CT + c] = flux[comp][d][qz][qx];
+        }
+      __syncthreads();
+      T bx[ZQ][Q], by[ZQ][Q], bz[ZQ][Q];
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          const int q = qz * Q + qx;
+          T sx = 0, sy = 0, sz = 0;
+#pragma unroll
+          for (int qy = 0; qy < Q; ++qy) {
+            sx += (T)CB.S[qy][j] * xb[((0 * QS + q) * Q + qy) * CT + c];
+            sy += (T)CB.G[qy][j] * xb[((1 * QS + q) * Q + qy) * CT + c];
+            if constexpr (DIM == 3) sz += (T)CB.S[qy][j] * xb[((2 * QS + q) * Q + qy) * CT + c];
+          }
+          bx[qz][qx] = sx;
+          by[qz][qx] = sy;
+          bz[qz][qx] = sz;
+        }
+      __syncthreads();
+      integ_xz<DIM, K, T, false>(bx, by, bz, out[comp]);
+    }
+  } else if constexpr (OP == OP_RHS) {
+    // value test functions: S along every direction, times the gravity component
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) xb[((qz * Q + qx) * Q + j) * CT + c] = flux[0][0][qz][qx];
+    __syncthreads();
+    T by[ZQ][Q], zero[ZQ][Q];
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        T sy = 0;
+#pragma unroll
+        for (int qy = 0; qy < Q; ++qy) sy += (T)CB.S[qy][j] * xb[((qz * Q + qx) * Q + qy) * CT + c];
+        by[qz][qx] = sy;
+        zero[qz][qx] = 0;
+      }
+    __syncthreads();
+    T val[ZN][N];
+    integ_xz<DIM, K, T, false>(zero, by, zero, val);
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+      for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+        for (int i = 0; i < N; ++i) out[comp][kz][i] = (T)L.gravity[comp] * val[kz][i];
+  } else if constexpr (OP == OP_ADIAG) {
+    // exchange eta JxW / h_d^2 per direction and integrate with squared matrices
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        const int q = qz * Q + qx;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xb[((d * QS + q) * Q + j) * CT + c] = flux[0][d][qz][qx];
+      }
+    __syncthreads();
+    // D_x: G2 x S2 y S2 z ; D_y: S2 x G2 y S2 z ; D_z: S2 x S2 y G2 z
+    T bxs[ZQ][Q], bys[ZQ][Q], bzs[ZQ][Q], zero[ZQ][Q];
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < Q; ++qx) {
+        const int q = qz * Q + qx;
+        T sx = 0, sy = 0, sz = 0;
+#pragma unroll
+        for (int qy = 0; qy < Q; ++qy) {
+          sx += (T)CB.S2[qy][j] * xb[((0 * QS + q) * Q + qy) * CT + c];
+          sy += (T)CB.G2[qy][j] * xb[((1 * QS + q) * Q + qy) * CT + c];
+          if constexpr (DIM == 3) sz += (T)CB.S2[qy][j] * xb[((2 * QS + q) * Q + qy) * CT + c];
+        }
+        bxs[qz][qx] = sx;
+        bys[qz][qx] = sy;
+        bzs[qz][qx] = sz;
+        zero[qz][qx] = 0;
+      }
+    __syncthreads();
+    T Dx[ZN][N], Dy[ZN][N], Dz[ZN][N];
+    integ_xz<DIM, K, T, true>(bxs, zero, zero, Dx);   // G2 along x, S2 along z
+    integ_xz<DIM, K, T, true>(zero, bys, zero, Dy);   // S2 along x, S2 along z
+    if constexpr (DIM == 3) integ_xz<DIM, K, T, true>(zero, zero, bzs, Dz);  // S2 along x, G2 along z
+    const T fc = L.compressible ? (T)(1.0 / 3.0) : (T)1;
+#pragma unroll
+    for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const T sum = Dx[kz][i] + Dy[kz][i] + (DIM == 3 ? Dz[kz][i] : (T)0);
+        out[0][kz][i] = sum + fc * Dx[kz][i];
+        out[1][kz][i] = sum + fc * Dy[kz][i];
+        if constexpr (DIM == 3) out[2][kz][i] = sum + fc * Dz[kz][i];
+      }
+  }
+
+  // ---- 5. integrate pressure test functions (partial over this slice, reduced over slices)
+  T pout[ZNP][NP][NP];
+  if constexpr (OUT_P) {
+    constexpr bool SQ = (OP == OP_MDIAG);
+#pragma unroll
+    for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+      for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+          T acc = 0;
+#pragma unroll
+          for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+            for (int qx = 0; qx < Q; ++qx) {
+              const T wz = DIM == 3 ? (SQ ? (T)CB.SP2[qz][kz] : (T)CB.SP[qz][kz]) : (T)1;
+              const T wy = SQ ? (T)CB.SP2[j][jy] : (T)CB.SP[j][jy];
+              const T wx = SQ ? (T)CB.SP2[qx][i] : (T)CB.SP[qx][i];
+              acc += wz * wy * wx * fp[qz][qx];
+            }
+          xb[(((kz * NP + jy) * NP + i) * Q + j) * CT + c] = acc;
+        }
+    __syncthreads();
+    if (j == 0) {
+#pragma unroll
+      for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+        for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+          for (int i = 0; i < NP; ++i) {
+            T acc = 0;
+#pragma unroll
+            for (int jj = 0; jj < Q; ++jj) acc += xb[(((kz * NP + jy) * NP + i) * Q + jj) * CT + c];
+            pout[kz][jy][i] = acc;
+          }
+    }
+    __syncthreads();
+  }
+
+  // ---- 6. assemble the tile in LDS by cell parity
+  for (int t = tid; t < DIM * C::VN + C::PN; t += C::THREADS) sm[t] = 0;
+  __syncthreads();
+  const int parity = (cx & 1) | ((cy & 1) << 1) | ((DIM == 3 ? (cz & 1) : 0) << 2);
+  for (int ph = 0; ph < (1 << DIM); ++ph) {
+    if (valid && parity == ph) {
+      if constexpr (OUT_U) {
+#pragma unroll
+        for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+          for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+              nb[comp * C::VN + ((K * cz + kz) * C::VY + (K * cy + j)) * C::VX + K * cx + i] += out[comp][kz][i];
+      }
+      if constexpr (OUT_P) {
+        if (j == 0) {
+#pragma unroll
+          for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+            for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+              for (int i = 0; i < NP; ++i)
+                pb[(((K - 1) * cz + kz) * C::PY + ((K - 1) * cy + jy)) * C::PX + (K - 1) * cx + i] +=
+                    pout[kz][jy][i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- 7. write the tile box: lowest colour stores, later colours add
+  if constexpr (OUT_U) {
+    for (int t = tid; t < DIM * C::VN; t += C::THREADS) {
+      const int comp = t / C::VN, r = t - comp * C::VN;
+      const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
+      const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
+      if (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2]) continue;
+      const bool shx = (lx == 0 && tx > 0) || (lx == C::VX - 1 && tx + 1 < L.ntiles[0]);
+      const bool shy = (ly == 0 && ty > 0) || (ly == C::VY - 1 && ty + 1 < L.ntiles[1]);
+      const bool shz = DIM == 3 && ((lz == 0 && tz > 0) || (lz == C::VZ - 1 && tz + 1 < L.ntiles[2]));
+      const bool first = !((shx && px) || (shy && py) || (shz && pz));
+      const long long gi = comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz);
+      T v;
+      if (is_con(L, comp, gx, gy, gz)) {
+        if (!first) continue;
+        if constexpr (OP == OP_ADIAG)
+          v = 1;
+        else if constexpr (OP == OP_BT || OP == OP_RHS)
+          v = 0;
+        else
+          v = src_u[gi];
+      } else {
+        v = nb[t];
+      }
+      if constexpr (MODE == MODE_SET)
+        dst_u[gi] = first ? v : dst_u[gi] + v;
+      else if constexpr (MODE == MODE_RESID)
+        dst_u[gi] = first ? rhs_u[gi] - v : dst_u[gi] - v;
+      else
+        dst_u[gi] += v;
+    }
+  }
+  if constexpr (OUT_P) {
+    for (int t = tid; t < C::PN; t += C::THREADS) {
+      const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
+      const int gx = qx0 + lx, gy = qy0 + ly, gz = qz0 + lz;
+      if (gx >= L.np[0] || gy >= L.np[1] || gz >= L.np[2]) continue;
+      const bool shx = (lx == 0 && tx > 0) || (lx == C::PX - 1 && tx + 1 < L.ntiles[0]);
+      const bool shy = (ly == 0 && ty > 0) || (ly == C::PY - 1 && ty + 1 < L.ntiles[1]);
+      const bool shz = DIM == 3 && ((lz == 0 && tz > 0) || (lz == C::PZ - 1 && tz + 1 < L.ntiles[2]));
+      const bool first = !((shx && px) || (shy && py) || (shz && pz));
+      const long long gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
+      const T v = pb[t];
+      if constexpr (MODE == MODE_SET)
+        dst_p[gi] = first ? v : dst_p[gi] + v;
+      else if constexpr (MODE == MODE_RESID)
+        dst_p[gi] = first ? rhs_p[gi] - v : dst_p[gi] - v;
+      else
+        dst_p[gi] += v;
+    }
+  }
+}
+
+template <int DIM, int K, int OP, int MODE, typename T>
+void launch_dkm(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru, const T *rp,
+                T *du, T *dp, hipStream_t st) {
+  using C = Cfg<DIM, K>;
+  const size_t smem = (size_t)C::SMEM * sizeof(T);
+  for (int color = 0; color < (1 << DIM); ++color) {
+    const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+    const long long hx = (L.ntiles[0] - px + 1) / 2, hy = (L.ntiles[1] - py + 1) / 2,
+                    hz = DIM == 3 ? (L.ntiles[2] - pz + 1) / 2 : 1;
+    const long long nb = hx * hy * hz;
+    if (nb <= 0) continue;
+    hipLaunchKernelGGL((k_cell<DIM, K, OP, MODE, T>), dim3((unsigned)nb), dim3(C::THREADS), smem, st, L, color,
+                       su, sp, eta, pq, ru, rp, du, dp);
+  }
+  MFS_HIP(hipGetLastError());
+}
+
+template <int DIM, int K, typename T>
+void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru,
+               const T *rp, T *du, T *dp, hipStream_t st) {
+#define MFS_CASE(O)                                                                          \
+  case O:                                                                                    \
+    if (mode == MODE_SET)                                                                    \
+      launch_dkm<DIM, K, O, MODE_SET, T>(L, su, sp, eta, pq, ru, rp, du, dp, st);           \
+    else if (mode == MODE_RESID)                                                             \
+      launch_dkm<DIM, K, O, MODE_RESID, T>(L, su, sp, eta, pq, ru, rp, du, dp, st);         \
+    else                                                                                     \
+      launch_dkm<DIM, K, O, MODE_ADD, T>(L, su, sp, eta, pq, ru, rp, du, dp, st);           \
+    break;
+  switch (op) {
+    MFS_CASE(OP_STOKES)
+    MFS_CASE(OP_ABLOCK)
+    MFS_CASE(OP_BT)
+    MFS_CASE(OP_MASS)
+    MFS_CASE(OP_ADIAG)
+    MFS_CASE(OP_MDIAG)
+    MFS_CASE(OP_RHS)
+  }
+#undef MFS_CASE
+}
+
+template <typename T>
+void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
+                    int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st) {
+  if (L.dim == 3 && L.k == 2)
+    launch_dk<3, 2, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
+  else if (L.dim == 3 && L.k == 3)
+    launch_dk<3, 3, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
+  else if (L.dim == 2 && L.k == 2)
+    launch_dk<2, 2, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
+  else if (L.dim == 2 && L.k == 3)
+    launch_dk<2, 3, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
+  else
+    throw std::runtime_error("unsupported (dim, degree)");
+}
+template void launch_cell_op<double>(Op, Mode, const LevelDesc &, const double *, const double *, const double *,
+                                     int, const double *, const double *, double *, double *, hipStream_t);
+template void launch_cell_op<float>(Op, Mode, const LevelDesc &, const float *, const float *, const float *, int,
+                                    const float *, const float *, float *, float *, hipStream_t);
+
+// ------------------------------------------------------------------ transfers
+// prolongation: one thread per fine node (all components)
+template <int DIM, typename T>
+__global__ void k_prolongate(const LevelDesc Lc, const LevelDesc Lf, const int kind, const T *__restrict__ src,
+                             T *__restrict__ dst, const int add) {
+  const int kidx = Lf.k - 2;
+  const bool pres = kind == 1;
+  const int kk = pres ? Lf.k - 1 : Lf.k;
+  const int *nf = pres ? Lf.np : Lf.nv;
+  const int *ncn = pres ? Lc.np : Lc.nv;
+  const long long nfine = (long long)nf[0] * nf[1] * nf[2];
+  const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f >= nfine) return;
+  const int g[3] = {(int)(f % nf[0]), (int)((f / nf[0]) % nf[1]), (int)(f / ((long long)nf[0] * nf[1]))};
+  int e[3], r[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d >= DIM) {
+      e[d] = 0;
+      r[d] = 0;
+      continue;
+    }
+    e[d] = g[d] / (2 * kk);
+    r[d] = g[d] - 2 * kk * e[d];
+    if (e[d] == Lc.n[d]) {
+      e[d] = Lc.n[d] - 1;
+      r[d] = 2 * kk;
+    }
+  }
+  const int ncomp = pres ? 1 : DIM;
+  const long long ncoarse = (long long)ncn[0] * ncn[1] * ncn[2];
+  for (int comp = 0; comp < ncomp; ++comp) {
+    double acc = 0;
+    for (int mz = 0; mz <= (DIM == 3 ? kk : 0); ++mz) {
+      const double wz = DIM == 3 ? (pres ? c_basis[kidx].WP[r[2]][mz] : c_basis[kidx].W[r[2]][mz]) : 1.0;
+      if (wz == 0.0) continue;
+      const int cz = DIM == 3 ? kk * e[2] + mz : 0;
+      for (int my = 0; my <= kk; ++my) {
+        const double wy = pres ? c_basis[kidx].WP[r[1]][my] : c_basis[kidx].W[r[1]][my];
+        if (wy == 0.0) continue;
+        const int cy = kk * e[1] + my;
+        for (int mx = 0; mx <= kk; ++mx) {
+          const double wx = pres ? c_basis[kidx].WP[r[0]][mx] : c_basis[kidx].W[r[0]][mx];
+          if (wx == 0.0) continue;
+          const int cx = kk * e[0] + mx;
+          if (!pres && is_con(Lc, comp, cx, cy, cz)) continue;
+          acc += wz * wy * wx * (double)src[comp * ncoarse + cx + (long long)ncn[0] * (cy + (long long)ncn[1] * cz)];
+        }
+      }
+    }
+    if (!pres && is_con(Lf, comp, g[0], g[1], g[2])) acc = 0.0;
+    const long long gi = comp * nfine + f;
+    dst[gi] = add ? (T)(dst[gi] + (T)acc) : (T)acc;
+  }
+}
+
+// restriction (transpose of prolongation): one thread per coarse node (all components)
+template <int DIM, typename T>
+__global__ void k_restrict(const LevelDesc Lc, const LevelDesc Lf, const int kind, const T *__restrict__ src,
+                           T *__restrict__ dst) {
+  const int kidx = Lf.k - 2;
+  const bool pres = kind == 1;
+  const int kk = pres ? Lf.k - 1 : Lf.k;
+  const int *nf = pres ? Lf.np : Lf.nv;
+  const int *ncn = pres ? Lc.np : Lc.nv;
+  const long long ncoarse = (long long)ncn[0] * ncn[1] * ncn[2];
+  const long long cidx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cidx >= ncoarse) return;
+  const int m[3] = {(int)(cidx % ncn[0]), (int)((cidx / ncn[0]) % ncn[1]), (int)(cidx / ((long long)ncn[0] * ncn[1]))};
+  // per direction: fine nodes g in [2m - 2kk, 2m + 2kk] with nonzero weight
+  int gl[3][13];
+  double wl[3][13];
+  int cnt[3];
+  for (int d = 0; d < 3; ++d) {
+    cnt[d] = 0;
+    if (d >= DIM) {
+      gl[d][0] = 0;
+      wl[d][0] = 1.0;
+      cnt[d] = 1;
+      continue;
+    }
+    for (int g = 2 * m[d] - 2 * kk; g <= 2 * m[d] + 2 * kk; ++g) {
+      if (g < 0 || g >= nf[d]) continue;
+      int e = g / (2 * kk), r = g - 2 * kk * e;
+      if (e == Lc.n[d]) {
+        e = Lc.n[d] - 1;
+        r = 2 * kk;
+      }
+      const int off = m[d] - kk * e;
+      if (off < 0 || off > kk) continue;
+      const double w = pres ? c_basis[kidx].WP[r][off] : c_basis[kidx].W[r][off];
+      if (w == 0.0) continue;
+      gl[d][cnt[d]] = g;
+      wl[d][cnt[d]] = w;
+      ++cnt[d];
+    }
+  }
+  const long long nfine = (long long)nf[0] * nf[1] * nf[2];
+  const int ncomp = pres ? 1 : DIM;
+  for (int comp = 0; comp < ncomp; ++comp) {
+    double acc = 0;
+    if (pres || !is_con(Lc, comp, m[0], m[1], m[2])) {
+      for (int a = 0; a < cnt[2]; ++a)
+        for (int bb = 0; bb < cnt[1]; ++bb)
+          for (int cc = 0; cc < cnt[0]; ++cc) {
+            if (!pres && is_con(Lf, comp, gl[0][cc], gl[1][bb], gl[2][a])) continue;
+            acc += wl[2][a] * wl[1][bb] * wl[0][cc] *
+                   (double)src[comp * nfine + gl[0][cc] + (long long)nf[0] * (gl[1][bb] + (long long)nf[1] * gl[2][a])];
+          }
+    }
+    dst[comp * ncoarse + cidx] = (T)acc;
+  }
+}
+
+template <typename T>
+void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
+                       hipStream_t st) {
+  const long long n = kind == 0 ? Lf.vnodes : Lf.pnodes;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((n + bs - 1) / bs);
+  if (Lf.dim == 3)
+    hipLaunchKernelGGL((k_prolongate<3, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_c, dst_f, add ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_prolongate<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_c, dst_f, add ? 1 : 0);
+  MFS_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_f, T *dst_c,
+                     hipStream_t st) {
+  const long long n = kind == 0 ? Lc.vnodes : Lc.pnodes;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((n + bs - 1) / bs);
+  if (Lf.dim == 3)
+    hipLaunchKernelGGL((k_restrict<3, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_f, dst_c);
+  else
+    hipLaunchKernelGGL((k_restrict<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_f, dst_c);
+  MFS_HIP(hipGetLastError());
+}
+template void launch_prolongate<double>(int, const LevelDesc &, const LevelDesc &, const double *, double *, bool,
+                                        hipStream_t);
+template void launch_prolongate<float>(int, const LevelDesc &, const LevelDesc &, const float *, float *, bool,
+                                       hipStream_t);
+template void launch_restrict<double>(int, const LevelDesc &, const LevelDesc &, const double *, double *,
+                                      hipStream_t);
+template void launch_restrict<float>(int, const LevelDesc &, const LevelDesc &, const float *, float *,
+                                     hipStream_t);
+
+// ------------------------------------------------------------------ viscosity tables
+__device__ __forceinline__ long long tile_index(const LevelDesc &L, int gcx, int gcy, int gcz, int TX, int TY,
+                                                int TZ) {
+  const int tx = gcx / TX, ty = gcy / TY, tz = gcz / TZ;
+  const int cx = gcx - tx * TX, cy = gcy - ty * TY, cz = gcz - tz * TZ;
+  const long long tile = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
+  return tile * (TX * TY * TZ) + cx + TX * (cy + TY * cz);
+}
+
+// natural [cell][q] (or [cell]) -> tile-blocked; one thread per (cell, q)
+template <typename T>
+__global__ void k_eta_to_tiles(const LevelDesc L, const double *__restrict__ eta_nat, const int per_qp, const int nq,
+                               const int TX, const int TY, const int TZ, T *__restrict__ out) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int nqq = per_qp ? nq : 1;
+  if (idx >= L.cells * nqq) return;
+  const long long cell = idx / nqq;
+  const int q = (int)(idx - cell * nqq);
+  const int gcx = (int)(cell % L.n[0]), gcy = (int)((cell / L.n[0]) % L.n[1]),
+            gcz = (int)(cell / ((long long)L.n[0] * L.n[1]));
+  const long long ti = tile_index(L, gcx, gcy, gcz, TX, TY, TZ);
+  const long long CT = TX * TY * TZ;
+  const long long tile = ti / CT, c = ti - tile * CT;
+  out[(tile * nqq + q) * CT + c] = (T)eta_nat[idx];
+}
+
+// cellwise averaging of natural per-qp values into a tile-blocked per-cell table
+template <typename T>
+__global__ void k_eta_average(const LevelDesc L, const double *__restrict__ eta_nat, const int nq, const int Q,
+                              const int mode, const int TX, const int TY, const int TZ, T *__restrict__ out) {
+  const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cell >= L.cells) return;
+  const int kidx = L.k - 2;
+  double acc = 0, wsum = 0;
+  bool zero = false;
+  for (int q = 0; q < nq; ++q) {
+    const double v = eta_nat[cell * nq + q];
+    if (mode == 1)
+      acc += v;
+    else if (mode == 2) {
+      if (v == 0.0) zero = true;
+      else acc += 1.0 / v;
+    } else if (mode == 3)
+      acc += log(v);
+    else {
+      const int qx = q % Q, qy = (q / Q) % Q, qz = q / (Q * Q);
+      const double w = c_basis[kidx].w[qx] * c_basis[kidx].w[qy] * (L.dim == 3 ? c_basis[kidx].w[qz] : 1.0);
+      acc += w * v;
+      wsum += w;
+    }
+  }
+  double r;
+  if (mode == 1)
+    r = acc / nq;
+  else if (mode == 2)
+    r = zero ? 0.0 : 1.0 / (acc / nq);
+  else if (mode == 3)
+    r = exp(acc / nq);
+  else
+    r = acc / wsum;
+  const int gcx = (int)(cell % L.n[0]), gcy = (int)((cell / L.n[0]) % L.n[1]),
+            gcz = (int)(cell / ((long long)L.n[0] * L.n[1]));
+  out[tile_index(L, gcx, gcy, gcz, TX, TY, TZ)] = (T)r;
+}
+
+// quadrature-weighted cell mean of a tile-blocked per-qp table
+template <typename T, typename S>
+__global__ void k_eta_qp_to_cell(const LevelDesc L, const S *__restrict__ in, const int Q, const int nq,
+                                 const int TX, const int TY, const int TZ, T *__restrict__ out) {
+  const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cell >= L.cells) return;
+  const int kidx = L.k - 2;
+  const int gcx = (int)(cell % L.n[0]), gcy = (int)((cell / L.n[0]) % L.n[1]),
+            gcz = (int)(cell / ((long long)L.n[0] * L.n[1]));
+  const long long CT = TX * TY * TZ;
+  const long long ti = tile_index(L, gcx, gcy, gcz, TX, TY, TZ);
+  const long long tile = ti / CT, c = ti - tile * CT;
+  double acc = 0, wsum = 0;
+  for (int q = 0; q < nq; ++q) {
+    const int qx = q % Q, qy = (q / Q) % Q, qz = q / (Q * Q);
+    const double w = c_basis[kidx].w[qx] * c_basis[kidx].w[qy] * (L.dim == 3 ? c_basis[kidx].w[qz] : 1.0);
+    acc += w * (double)in[(tile * nq + q) * CT + c];
+    wsum += w;
+  }
+  out[ti] = (T)(acc / wsum);
+}
+
+// coarse cell = arithmetic mean of its 2^dim children (FE_DGQ(0) restriction)
+template <typename T>
+__global__ void k_eta_restrict(const LevelDesc Lc, const LevelDesc Lf, const int TX, const int TY, const int TZ,
+                               const T *__restrict__ fine, T *__restrict__ coarse) {
+  const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cell >= Lc.cells) return;
+  const int gcx = (int)(cell % Lc.n[0]), gcy = (int)((cell / Lc.n[0]) % Lc.n[1]),
+            gcz = (int)(cell / ((long long)Lc.n[0] * Lc.n[1]));
+  double acc = 0;
+  int cnt = 0;
+  for (int dz = 0; dz < (Lc.dim == 3 ? 2 : 1); ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        acc += (double)fine[tile_index(Lf, 2 * gcx + dx, 2 * gcy + dy, Lc.dim == 3 ? 2 * gcz + dz : 0, TX, TY, TZ)];
+        ++cnt;
+      }
+  coarse[tile_index(Lc, gcx, gcy, gcz, TX, TY, TZ)] = (T)(acc / cnt);
+}
+
+static void tiles_of(const LevelDesc &L, int t[3]) { tile_cells(L.dim, L.k, t); }
+
+template <typename T>
+void launch_eta_to_tiles(const LevelDesc &L, const double *eta_nat, int per_qp, T *eta_tiles, hipStream_t st) {
+  int t[3];
+  tiles_of(L, t);
+  const int nq = L.dim == 3 ? (L.k + 1) * (L.k + 1) * (L.k + 1) : (L.k + 1) * (L.k + 1);
+  const long long n = L.cells * (per_qp ? nq : 1);
+  hipLaunchKernelGGL((k_eta_to_tiles<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, eta_nat, per_qp,
+                     nq, t[0], t[1], t[2], eta_tiles);
+  MFS_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_eta_average(const LevelDesc &L, const double *eta_nat_qp, int mode, T *out, hipStream_t st) {
+  int t[3];
+  tiles_of(L, t);
+  const int Q = L.k + 1;
+  const int nq = L.dim == 3 ? Q * Q * Q : Q * Q;
+  hipLaunchKernelGGL((k_eta_average<T>), dim3((unsigned)((L.cells + 255) / 256)), dim3(256), 0, st, L, eta_nat_qp,
+                     nq, Q, mode, t[0], t[1], t[2], out);
+  MFS_HIP(hipGetLastError());
+}
+template <typename T, typename S>
+void launch_eta_qp_to_cell(const LevelDesc &L, const S *in, T *out, hipStream_t st) {
+  int t[3];
+  tiles_of(L, t);
+  const int Q = L.k + 1;
+  const int nq = L.dim == 3 ? Q * Q * Q : Q * Q;
+  hipLaunchKernelGGL((k_eta_qp_to_cell<T, S>), dim3((unsigned)((L.cells + 255) / 256)), dim3(256), 0, st, L, in, Q,
+                     nq, t[0], t[1], t[2], out);
+  MFS_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_eta_restrict(const LevelDesc &Lc, const LevelDesc &Lf, const T *fine, T *coarse, hipStream_t st) {
+  int t[3];
+  tiles_of(Lc, t);
+  hipLaunchKernelGGL((k_eta_restrict<T>), dim3((unsigned)((Lc.cells + 255) / 256)), dim3(256), 0, st, Lc, Lf, t[0],
+                     t[1], t[2], fine, coarse);
+  MFS_HIP(hipGetLastError());
+}
+template void launch_eta_to_tiles<double>(const LevelDesc &, const double *, int, double *, hipStream_t);
+template void launch_eta_to_tiles<float>(const LevelDesc &, const double *, int, float *, hipStream_t);
+template void launch_eta_average<double>(const LevelDesc &, const double *, int, double *, hipStream_t);
+template void launch_eta_average<float>(const LevelDesc &, const double *, int, float *, hipStream_t);
+template void launch_eta_qp_to_cell<double, double>(const LevelDesc &, const double *, double *, hipStream_t);
+template void launch_eta_qp_to_cell<float, double>(const LevelDesc &, const double *, float *, hipStream_t);
+template void launch_eta_restrict<double>(const LevelDesc &, const LevelDesc &, const double *, double *,
+                                          hipStream_t);
+template void launch_eta_restrict<float>(const LevelDesc &, const LevelDesc &, const float *, float *, hipStream_t);
+
+// ------------------------------------------------------------------ vector kernels
+static inline unsigned nblocks(long long n, int bs = 256) {
+  long long b = (n + bs - 1) / bs;
+  if (b > 65535LL * 16) b = 65535LL * 16;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+#define GRID_STRIDE(i, n) \
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+template <typename T>
+__global__ void k_fill(T *x, T v, long long n) {
+  GRID_STRIDE(i, n) x[i] = v;
+}
+template <typename A, typename B>
+__global__ void k_cast(const A *__restrict__ x, B *__restrict__ y, long long n) {
+  GRID_STRIDE(i, n) y[i] = (B)x[i];
+}
+template <typename T>
+__global__ void k_axpby(T a, const T *__restrict__ x, T b, T *__restrict__ y, long long n) {
+  GRID_STRIDE(i, n) y[i] = a * x[i] + b * y[i];
+}
+template <typename T>
+__global__ void k_invert(T *x, long long n) {
+  GRID_STRIDE(i, n) x[i] = (T)1 / x[i];
+}
+template <typename T>
+__global__ void k_cheb_first(const T *__restrict__ dinv, const T *__restrict__ r, T *__restrict__ d,
+                             T *__restrict__ x, T inv_theta, int x_zero, long long n) {
+  GRID_STRIDE(i, n) {
+    const T dd = dinv[i] * r[i] * inv_theta;
+    d[i] = dd;
+    x[i] = x_zero ? dd : x[i] + dd;
+  }
+}
+template <typename T>
+__global__ void k_cheb_step(const T *__restrict__ dinv, const T *__restrict__ r, T *__restrict__ d,
+                            T *__restrict__ x, T f1, T f2, long long n) {
+  GRID_STRIDE(i, n) {
+    const T dd = f1 * d[i] + f2 * dinv[i] * r[i];
+    d[i] = dd;
+    x[i] += dd;
+  }
+}
+template <typename T>
+__global__ void k_neg_copy(const T *__restrict__ x, T *__restrict__ y, long long n) {
+  GRID_STRIDE(i, n) y[i] = -x[i];
+}
+
+template <typename T>
+void v_fill(T *x, T v, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(n)), dim3(256), 0, st, x, v, n);
+}
+template <typename A, typename B>
+void v_cast(const A *x, B *y, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_cast<A, B>), dim3(nblocks(n)), dim3(256), 0, st, x, y, n);
+}
+template <typename T>
+void v_axpby(T a, const T *x, T b, T *y, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_axpby<T>), dim3(nblocks(n)), dim3(256), 0, st, a, x, b, y, n);
+}
+template <typename T>
+void v_invert(T *x, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_invert<T>), dim3(nblocks(n)), dim3(256), 0, st, x, n);
+}
+template <typename T>
+void v_cheb_first(const T *dinv, const T *r, T *d, T *x, T inv_theta, bool x_zero, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_cheb_first<T>), dim3(nblocks(n)), dim3(256), 0, st, dinv, r, d, x, inv_theta,
+                     x_zero ? 1 : 0, n);
+}
+template <typename T>
+void v_cheb_step(const T *dinv, const T *r, T *d, T *x, T f1, T f2, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_cheb_step<T>), dim3(nblocks(n)), dim3(256), 0, st, dinv, r, d, x, f1, f2, n);
+}
+template <typename T>
+void v_scale_neg_copy(const T *x, T *y, long long n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_neg_copy<T>), dim3(nblocks(n)), dim3(256), 0, st, x, y, n);
+}
+
+#define INST_T(T)                                                                  \
+  template void v_fill<T>(T *, T, long long, hipStream_t);                         \
+  template void v_axpby<T>(T, const T *, T, T *, long long, hipStream_t);          \
+  template void v_invert<T>(T *, long long, hipStream_t);                          \
+  template void v_cheb_first<T>(const T *, const T *, T *, T *, T, bool, long long, hipStream_t); \
+  template void v_cheb_step<T>(const T *, const T *, T *, T *, T, T, long long, hipStream_t);     \
+  template void v_scale_neg_copy<T>(const T *, T *, long long, hipStream_t);
+INST_T(double)
+INST_T(float)
+template void v_cast<double, float>(const double *, float *, long long, hipStream_t);
+template void v_cast<float, double>(const float *, double *, long long, hipStream_t);
+template void v_cast<double, double>(const double *, double *, long long, hipStream_t);
+template void v_cast<float, float>(const float *, float *, long long, hipStream_t);
+
+// ---- deterministic reductions: fixed grid of DOT_BLOCKS blocks, fixed-order tree
+constexpr int DOT_BLOCKS = 512;
+constexpr int DOT_THREADS = 256;
+int dot_work_size() { return DOT_BLOCKS; }
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = DOT_THREADS / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(DOT_THREADS) k_dot_partial(const T *__restrict__ x, const T *__restrict__ y,
+                                                             long long n, double *__restrict__ part) {
+  __shared__ double sh[DOT_THREADS];
+  double acc = 0;
+  GRID_STRIDE(i, n) acc += (double)x[i] * (double)y[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(DOT_THREADS) k_sum_partials(const double *__restrict__ part, int m, int nblk,
+                                                              double *__restrict__ out) {
+  __shared__ double sh[DOT_THREADS];
+  // one block per output value
+  const int v = blockIdx.x;
+  double acc = 0;
+  for (int i = threadIdx.x; i < nblk; i += DOT_THREADS) acc += part[(long long)v * nblk + i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[v] = s;
+  (void)m;
+}
+
+template <typename T>
+void v_dot(const T *x, const T *y, long long n, double *work, double *out, hipStream_t st) {
+  hipLaunchKernelGGL((k_dot_partial<T>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, st, x, y, n, work);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(DOT_THREADS), 0, st, work, 1, DOT_BLOCKS, out);
+}
+template void v_dot<double>(const double *, const double *, long long, double *, double *, hipStream_t);
+template void v_dot<float>(const float *, const float *, long long, double *, double *, hipStream_t);
+
+// h[i] = V_i . w for i < m, processed in groups of 8 vectors per pass over w
+constexpr int MD_GROUP = 8;
+__global__ void __launch_bounds__(DOT_THREADS) k_multidot_partial(const double *__restrict__ V, long long ld, int m0,
+                                                                  int mg, const double *__restrict__ w, long long n,
+                                                                  double *__restrict__ part) {
+  __shared__ double sh[DOT_THREADS];
+  double acc[MD_GROUP];
+#pragma unroll
+  for (int g = 0; g < MD_GROUP; ++g) acc[g] = 0;
+  GRID_STRIDE(i, n) {
+    const double wi = w[i];
+#pragma unroll
+    for (int g = 0; g < MD_GROUP; ++g)
+      if (g < mg) acc[g] += V[g * ld + i] * wi;
+  }
+  for (int g = 0; g < mg; ++g) {
+    const double s = block_sum(acc[g], sh);
+    if (threadIdx.x == 0) part[(long long)(m0 + g) * DOT_BLOCKS + blockIdx.x] = s;
+    __syncthreads();
+  }
+}
+void v_multidot_partial(const double *Vchunk, long long ld, int m0, int mg, const double *w, long long n,
+                        double *work, hipStream_t st) {
+  hipLaunchKernelGGL(k_multidot_partial, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, st, Vchunk, ld, m0, mg, w,
+                     n, work);
+}
+void v_sum_partials(const double *work, int m, double *h, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(m), dim3(DOT_THREADS), 0, st, work, m, DOT_BLOCKS, h);
+}
+__global__ void k_multiaxpy(const double *__restrict__ V, long long ld, int m, const double *__restrict__ h,
+                            double *__restrict__ w, long long n, double sign) {
+  GRID_STRIDE(i, n) {
+    double acc = 0;
+    for (int g = 0; g < m; ++g) acc += h[g] * V[g * ld + i];
+    w[i] += sign * acc;
+  }
+}
+void v_multiaxpy(const double *Vchunk, long long ld, int mg, const double *h, double *w, long long n, double sign,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_multiaxpy, dim3(nblocks(n)), dim3(256), 0, st, Vchunk, ld, mg, h, w, n, sign);
+}
+template <typename T>
+__global__ void k_mul(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ c, long long n) {
+  GRID_STRIDE(i, n) c[i] = a[i] * b[i];
+}
+template <typename T>
+void v_mul(const T *a, const T *b, T *c, long long n, hipStream_t st) {
+  hipLaunchKernelGGL((k_mul<T>), dim3(nblocks(n)), dim3(256), 0, st, a, b, c, n);
+}
+template void v_mul<double>(const double *, const double *, double *, long long, hipStream_t);
+template void v_mul<float>(const float *, const float *, float *, long long, hipStream_t);
+__global__ void k_scale(double *x, double a, long long n) {
+  GRID_STRIDE(i, n) x[i] *= a;
+}
+void v_scale(double *x, double a, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(n)), dim3(256), 0, st, x, a, n);
+}
+__global__ void k_scaled_copy_dev(const double *__restrict__ x, const double *__restrict__ a, double *__restrict__ y,
+                                  long long n) {
+  const double inv = 1.0 / a[0];
+  GRID_STRIDE(i, n) y[i] = x[i] * inv;
+}
+void v_scaled_copy_dev(const double *x, const double *a, double *y, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_copy_dev, dim3(nblocks(n)), dim3(256), 0, st, x, a, y, n);
+}
+
+template <typename T>
+__global__ void k_zero_con(const LevelDesc L, T *u) {
+  const long long n = L.vnodes;
+  GRID_STRIDE(i, n) {
+    const int g[3] = {(int)(i % L.nv[0]), (int)((i / L.nv[0]) % L.nv[1]), (int)(i / ((long long)L.nv[0] * L.nv[1]))};
+    for (int comp = 0; comp < L.dim; ++comp)
+      if (is_con(L, comp, g[0], g[1], g[2])) u[comp * n + i] = 0;
+  }
+}
+template <typename T>
+void v_zero_constrained(const LevelDesc &L, T *u, hipStream_t st) {
+  hipLaunchKernelGGL((k_zero_con<T>), dim3(nblocks(L.vnodes)), dim3(256), 0, st, L, u);
+}
+template void v_zero_constrained<double>(const LevelDesc &, double *, hipStream_t);
+template void v_zero_constrained<float>(const LevelDesc &, float *, hipStream_t);
+
+}  // namespace mfs

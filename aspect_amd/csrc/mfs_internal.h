@@ -1,0 +1,121 @@
+// Internal interface between the host-side handler and the gfx950 kernels.
+// Not part of the C-ABI (see include/aspect_mfs.h for that).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace mfs {
+
+#define MFS_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw std::runtime_error(std::string("HIP error '") + hipGetErrorString(e_) + "' at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));                \
+  } while (0)
+
+// OP_RHS: buoyancy right-hand side  sum_q rho g . phi JxW  (rho per quadrature point in the eta slot)
+enum Op { OP_STOKES = 0, OP_ABLOCK = 1, OP_BT = 2, OP_MASS = 3, OP_ADIAG = 4, OP_MDIAG = 5, OP_RHS = 6 };
+// SET: dst = A src;  RESID: dst = rhs - A src;  ADD: dst += A src
+enum Mode { MODE_SET = 0, MODE_RESID = 1, MODE_ADD = 2 };
+
+// Geometry of one uniform box level.  Velocity nodes: (k*n+1) per direction,
+// pressure nodes ((k-1)*n+1); both lexicographic with x fastest; the velocity
+// block is component-major.  Cell data (viscosity) is tile-blocked:
+// index = (tile_id * NQ_or_1 + q) * CT + cell_in_tile.
+struct LevelDesc {
+  int dim, k;
+  int n[3];
+  int nv[3];
+  int np[3];
+  int ntiles[3];
+  long long vnodes, pnodes, cells, ntiles_total;
+  double h[3];
+  double s;       // pressure scaling
+  int bc[6];      // 0 open, 1 no normal flux, 2 zero velocity
+  int compressible;
+  double gravity[3];  // used by OP_RHS only
+};
+
+// Tile shapes (cells) per (dim, degree); one workgroup per tile, one thread
+// per (cell, y-slice).
+int tile_cells(int dim, int k, int out[3]);
+int tile_threads(int dim, int k);
+
+void upload_basis_tables();
+
+template <typename T>
+void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
+                    int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st);
+
+// Transfers between level Lc (coarse) and Lf = refined Lc.  kind 0 velocity, 1 pressure.
+template <typename T>
+void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
+                       hipStream_t st);
+template <typename T>
+void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_f, T *dst_c,
+                     hipStream_t st);
+
+// viscosity tables
+// natural layout eta[cell*nq + q] (or eta[cell]) -> tile-blocked layout of L
+template <typename T>
+void launch_eta_to_tiles(const LevelDesc &L, const double *eta_nat, int per_qp, T *eta_tiles, hipStream_t st);
+// cellwise averaging of natural per-qp values into tile-blocked per-cell values.
+// mode: 1 arithmetic, 2 harmonic, 3 geometric (unweighted over quadrature
+// points), 4 quadrature-weighted mean (DGQ0 projection)
+template <typename T>
+void launch_eta_average(const LevelDesc &L, const double *eta_nat_qp, int mode, T *eta_tiles_cell,
+                        hipStream_t st);
+// per-cell table of fine level -> mean of children on coarse level
+template <typename T>
+void launch_eta_restrict(const LevelDesc &Lc, const LevelDesc &Lf, const T *fine_cell, T *coarse_cell,
+                         hipStream_t st);
+// DGQ0 projection (quadrature-weighted mean) of a tile-blocked per-qp table
+template <typename T, typename S>
+void launch_eta_qp_to_cell(const LevelDesc &L, const S *eta_qp_tiles, T *eta_cell_tiles, hipStream_t st);
+
+// vector kernels
+template <typename T>
+void v_fill(T *x, T v, long long n, hipStream_t st);
+template <typename A, typename B>
+void v_cast(const A *x, B *y, long long n, hipStream_t st);
+template <typename T>
+void v_axpby(T a, const T *x, T b, T *y, long long n, hipStream_t st);  // y = a x + b y
+template <typename T>
+void v_invert(T *x, long long n, hipStream_t st);
+template <typename T>
+void v_cheb_first(const T *dinv, const T *r, T *d, T *x, T inv_theta, bool x_zero, long long n,
+                  hipStream_t st);  // d = dinv r / theta; x = d (x_zero) or x += d
+template <typename T>
+void v_cheb_step(const T *dinv, const T *r, T *d, T *x, T f1, T f2, long long n,
+                 hipStream_t st);  // d = f1 d + f2 dinv r; x += d
+template <typename T>
+void v_scale_neg_copy(const T *x, T *y, long long n, hipStream_t st);  // y = -x
+
+// Deterministic dot products: partial sums into work[nblk], then a fixed-order
+// final reduction into out[0] (device).  work must hold dot_work_size() doubles.
+int dot_work_size();
+template <typename T>
+void v_dot(const T *x, const T *y, long long n, double *work, double *out, hipStream_t st);
+// Multi-dot over a chunk of mg vectors (stride ld): partial sums into
+// work[(m0+g)*dot_work_size() + block]; v_sum_partials reduces m of them into h.
+void v_multidot_partial(const double *Vchunk, long long ld, int m0, int mg, const double *w, long long n,
+                        double *work, hipStream_t st);
+void v_sum_partials(const double *work, int m, double *h, hipStream_t st);
+// w += sign * sum_{g<mg} h[g] Vchunk[g]
+void v_multiaxpy(const double *Vchunk, long long ld, int mg, const double *h, double *w, long long n, double sign,
+                 hipStream_t st);
+template <typename T>
+void v_mul(const T *a, const T *b, T *c, long long n, hipStream_t st);  // c = a * b
+// x *= a
+void v_scale(double *x, double a, long long n, hipStream_t st);
+// y = x / a (a on device)
+void v_scaled_copy_dev(const double *x, const double *a, double *y, long long n, hipStream_t st);
+// zero constrained velocity entries of a level vector
+template <typename T>
+void v_zero_constrained(const LevelDesc &L, T *u, hipStream_t st);
+
+}  // namespace mfs

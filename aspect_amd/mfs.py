@@ -1,0 +1,336 @@
+"""Python mirror of ASPECT's ``StokesMatrixFreeHandler`` over the C-ABI of
+``libaspect_mfs.so`` (declared in ``include/aspect_mfs.h``).
+
+The class keeps the reference interface's method names and meaning
+(include/aspect/simulator/solver/stokes_matrix_free.h:47-98 and
+include/aspect/simulator/solver/interface.h:36-101): ``setup_dofs``,
+``assemble``, ``build_preconditioner``, ``solve`` returning ``SolverOutputs``,
+``name``, and the memory getters.  All arithmetic runs in the HIP library;
+this module only marshals arrays.  Loading fails loudly when the library is
+missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaspect_mfs.so")
+
+MFS_OK, MFS_ERROR, MFS_NO_CONVERGENCE = 0, 1, 2
+GMG_FP64, GMG_FP32 = 0, 1
+AVERAGE = {"none": 0, "arithmetic average only viscosity": 1, "harmonic average only viscosity": 2,
+           "geometric average only viscosity": 3, "arithmetic": 1, "harmonic": 2, "geometric": 3}
+BC_OPEN, BC_TANGENTIAL, BC_ZERO = 0, 1, 2
+
+# every symbol include/aspect_mfs.h declares
+EXPORTED = [
+    "mfs_last_error", "mfs_create", "mfs_destroy", "mfs_name", "mfs_sizes", "mfs_set_viscosity",
+    "mfs_set_pressure_scaling", "mfs_build_preconditioner", "mfs_stokes_vmult", "mfs_ablock_vmult",
+    "mfs_mass_vmult", "mfs_bt_vmult", "mfs_ablock_diagonal", "mfs_mass_diagonal", "mfs_prolongate",
+    "mfs_restrict", "mfs_level_viscosity", "mfs_smoother_info", "mfs_chebyshev", "mfs_vcycle",
+    "mfs_block_preconditioner", "mfs_solve", "mfs_assemble_buoyancy_rhs", "mfs_stokes_vmult_device",
+    "mfs_ablock_vmult_device", "mfs_solve_device", "mfs_assemble_buoyancy_rhs_device", "mfs_synchronize",
+    "mfs_memory_consumption", "mfs_time_operator",
+]
+
+
+class Box(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("degree", C.c_int),
+        ("coarse", C.c_int * 3),
+        ("n_levels", C.c_int),
+        ("extent", C.c_double * 3),
+        ("bc", C.c_int * 6),
+        ("compressible", C.c_int),
+        ("pressure_scaling", C.c_double),
+    ]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double),
+        ("n_cheap_steps", C.c_int),
+        ("n_expensive_steps", C.c_int),
+        ("restart", C.c_int),
+        ("a_block_tolerance", C.c_double),
+        ("s_block_tolerance", C.c_double),
+    ]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [
+        ("cheap_iterations", C.c_int),
+        ("expensive_iterations", C.c_int),
+        ("a_preconditioner_applications", C.c_int),
+        ("s_preconditioner_applications", C.c_int),
+        ("converged", C.c_int),
+        ("initial_nonlinear_residual", C.c_double),
+        ("final_linear_residual", C.c_double),
+        ("solver_tolerance", C.c_double),
+        ("r0_norm", C.c_double),
+        ("pressure_normalization_adjustment", C.c_double),
+        ("solve_seconds", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                              " (no CPU fallback exists)")
+        lib = C.CDLL(path)
+        lib.mfs_last_error.restype = C.c_char_p
+        for name in EXPORTED:
+            getattr(lib, name)  # raises AttributeError if a declared symbol is missing
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+class NoConvergence(SolverError):
+    """Mirror of SolverControl::NoConvergence raised by the handler's solve."""
+
+    def __init__(self, msg, outputs):
+        super().__init__(msg)
+        self.outputs = outputs
+
+
+def _check(rc):
+    if rc == MFS_ERROR:
+        raise SolverError(load_library().mfs_last_error().decode())
+    return rc
+
+
+@dataclass
+class SolverOutputs:
+    """StokesSolver::SolverOutputs (interface.h:36-58) plus solver counters."""
+    initial_nonlinear_residual: float
+    final_linear_residual: float
+    pressure_normalization_adjustment: float
+    cheap_iterations: int
+    expensive_iterations: int
+    a_preconditioner_applications: int
+    s_preconditioner_applications: int
+    converged: bool
+    solver_tolerance: float
+    r0_norm: float
+    solve_seconds: float
+
+
+def make_box(dim=3, degree=2, coarse=(1, 1, 1), n_levels=3, extent=(1.0, 1.0, 1.0), bc=(1,) * 6,
+             compressible=False, pressure_scaling=1.0) -> Box:
+    b = Box()
+    b.dim, b.degree, b.n_levels = dim, degree, n_levels
+    for i in range(3):
+        b.coarse[i] = int(coarse[i]) if i < len(coarse) else 1
+        b.extent[i] = float(extent[i]) if i < len(extent) else 1.0
+    for i in range(6):
+        b.bc[i] = int(bc[i]) if i < len(bc) else 0
+    b.compressible = int(compressible)
+    b.pressure_scaling = float(pressure_scaling)
+    return b
+
+
+class StokesMatrixFreeHandler:
+    """GPU implementation of the matrix-free Stokes handler on box meshes.
+
+    ``gmg_precision`` selects GMGNumberType of the V-cycle: "fp64" (the
+    reference's ``using GMGNumberType = double``) or "fp32".
+    """
+
+    def __init__(self, box: Box, gmg_precision: str = "fp64"):
+        self.lib = load_library()
+        self.box = box
+        self.gmg_precision = gmg_precision
+        self._h = C.c_void_p()
+        self.setup_dofs()
+
+    # -- StokesMatrixFreeHandler interface ---------------------------------
+    def setup_dofs(self):
+        if self._h:
+            self.lib.mfs_destroy(self._h)
+            self._h = C.c_void_p()
+        prec = GMG_FP32 if self.gmg_precision == "fp32" else GMG_FP64
+        _check(self.lib.mfs_create(C.byref(self.box), prec, C.byref(self._h)))
+
+    def assemble(self, eta_qp, averaging="none", pressure_scaling=None):
+        """evaluate_material_model(): viscosity at every quadrature point."""
+        if pressure_scaling is not None:
+            _check(self.lib.mfs_set_pressure_scaling(self._h, C.c_double(pressure_scaling)))
+        eta = np.ascontiguousarray(eta_qp, dtype=np.float64)
+        s = self.sizes(self.top)
+        if eta.size != s["cells"] * self.n_qp:
+            raise ValueError(f"viscosity table must hold {s['cells']} x {self.n_qp} values")
+        mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
+        _check(self.lib.mfs_set_viscosity(self._h, _ptr(eta), mode))
+
+    def build_preconditioner(self):
+        _check(self.lib.mfs_build_preconditioner(self._h))
+
+    def solve(self, rhs, x0=None, tolerance=1e-7, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
+              a_block_tolerance=1e-2, s_block_tolerance=1e-6, raise_on_failure=True):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        sol = np.zeros_like(rhs)
+        prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
+                          s_block_tolerance)
+        res = SolveResult()
+        xp = None
+        if x0 is not None:
+            xp = np.ascontiguousarray(x0, dtype=np.float64)
+        rc = _check(self.lib.mfs_solve(self._h, _ptr(rhs), _ptr(xp) if xp is not None else None, _ptr(sol),
+                                       C.byref(prm), C.byref(res)))
+        out = SolverOutputs(res.initial_nonlinear_residual, res.final_linear_residual,
+                            res.pressure_normalization_adjustment, res.cheap_iterations, res.expensive_iterations,
+                            res.a_preconditioner_applications, res.s_preconditioner_applications,
+                            bool(res.converged), res.solver_tolerance, res.r0_norm, res.solve_seconds)
+        if rc == MFS_NO_CONVERGENCE and raise_on_failure:
+            raise NoConvergence("Stokes solver did not converge", out)
+        return sol, out
+
+    def name(self) -> str:
+        buf = C.create_string_buffer(64)
+        _check(self.lib.mfs_name(self._h, buf, 64))
+        return buf.value.decode()
+
+    def memory_consumption(self):
+        out = (C.c_longlong * 3)()
+        _check(self.lib.mfs_memory_consumption(self._h, out))
+        return dict(vectors=out[0], cell_data=out[1], krylov=out[2])
+
+    # -- operators and pieces (parity entry points) ------------------------
+    @property
+    def top(self):
+        return self.box.n_levels - 1
+
+    @property
+    def n_qp(self):
+        return (self.box.degree + 1) ** self.box.dim
+
+    def sizes(self, level):
+        out = (C.c_longlong * 4)()
+        _check(self.lib.mfs_sizes(self._h, level, out))
+        return dict(cells=out[0], nu=out[1], np=out[2], vnodes=out[3])
+
+    def _unary(self, fn, level_args, src, nout):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros(nout)
+        _check(fn(self._h, *level_args, _ptr(src), _ptr(dst)))
+        return dst
+
+    def stokes_vmult(self, src):
+        s = self.sizes(self.top)
+        return self._unary(self.lib.mfs_stokes_vmult, (), src, s["nu"] + s["np"])
+
+    def ablock_vmult(self, level, src):
+        s = self.sizes(self.top if level == -1 else level)
+        return self._unary(self.lib.mfs_ablock_vmult, (level,), src, s["nu"])
+
+    def mass_vmult(self, level, src):
+        s = self.sizes(self.top if level == -1 else level)
+        return self._unary(self.lib.mfs_mass_vmult, (level,), src, s["np"])
+
+    def bt_vmult(self, src_p):
+        return self._unary(self.lib.mfs_bt_vmult, (), src_p, self.sizes(self.top)["nu"])
+
+    def ablock_diagonal(self, level):
+        out = np.zeros(self.sizes(level)["nu"])
+        _check(self.lib.mfs_ablock_diagonal(self._h, level, _ptr(out)))
+        return out
+
+    def mass_diagonal(self, level):
+        out = np.zeros(self.sizes(level)["np"])
+        _check(self.lib.mfs_mass_diagonal(self._h, level, _ptr(out)))
+        return out
+
+    def prolongate(self, fine_level, kind, src):
+        s = self.sizes(fine_level)
+        return self._unary(self.lib.mfs_prolongate, (fine_level, kind), src, s["nu"] if kind == 0 else s["np"])
+
+    def restrict(self, fine_level, kind, src):
+        s = self.sizes(fine_level - 1)
+        return self._unary(self.lib.mfs_restrict, (fine_level, kind), src, s["nu"] if kind == 0 else s["np"])
+
+    def level_viscosity(self, level):
+        out = np.zeros(self.sizes(level)["cells"])
+        _check(self.lib.mfs_level_viscosity(self._h, level, _ptr(out)))
+        return out
+
+    def smoother_info(self):
+        out = np.zeros((self.box.n_levels, 2, 5))
+        _check(self.lib.mfs_smoother_info(self._h, _ptr(out)))
+        return out
+
+    def chebyshev(self, kind, level, src):
+        s = self.sizes(level)
+        return self._unary(self.lib.mfs_chebyshev, (kind, level), src, s["nu"] if kind == 0 else s["np"])
+
+    def vcycle(self, kind, src):
+        s = self.sizes(self.top)
+        return self._unary(self.lib.mfs_vcycle, (kind,), src, s["nu"] if kind == 0 else s["np"])
+
+    def block_preconditioner(self, src):
+        s = self.sizes(self.top)
+        return self._unary(self.lib.mfs_block_preconditioner, (), src, s["nu"] + s["np"])
+
+    def assemble_buoyancy_rhs(self, rho_qp, gravity):
+        rho = np.ascontiguousarray(rho_qp, dtype=np.float64)
+        g = np.ascontiguousarray(np.resize(np.asarray(gravity, dtype=np.float64), 3))
+        s = self.sizes(self.top)
+        out = np.zeros(s["nu"] + s["np"])
+        _check(self.lib.mfs_assemble_buoyancy_rhs(self._h, _ptr(rho), _ptr(g), _ptr(out)))
+        return out
+
+    # device-pointer entry points (torch tensors or raw ints)
+    def stokes_vmult_device(self, d_src: int, d_dst: int):
+        _check(self.lib.mfs_stokes_vmult_device(self._h, C.c_void_p(d_src), C.c_void_p(d_dst)))
+
+    def ablock_vmult_device(self, level: int, d_src: int, d_dst: int):
+        _check(self.lib.mfs_ablock_vmult_device(self._h, level, C.c_void_p(d_src), C.c_void_p(d_dst)))
+
+    def assemble_buoyancy_rhs_device(self, d_rho: int, gravity, d_rhs: int):
+        g = np.ascontiguousarray(np.resize(np.asarray(gravity, dtype=np.float64), 3))
+        _check(self.lib.mfs_assemble_buoyancy_rhs_device(self._h, C.c_void_p(d_rho), _ptr(g), C.c_void_p(d_rhs)))
+
+    def solve_device(self, d_rhs: int, d_x0, d_sol: int, tolerance=1e-7, n_cheap_steps=1000,
+                     n_expensive_steps=1000, restart=100, a_block_tolerance=1e-2, s_block_tolerance=1e-6):
+        prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
+                          s_block_tolerance)
+        res = SolveResult()
+        rc = _check(self.lib.mfs_solve_device(self._h, C.c_void_p(d_rhs), C.c_void_p(d_x0) if d_x0 else None,
+                                              C.c_void_p(d_sol), C.byref(prm), C.byref(res)))
+        return rc, {f: getattr(res, f) for f, _ in SolveResult._fields_}
+
+    def time_operator(self, which: int, level: int, d_src: int, d_dst: int, reps: int) -> float:
+        """Average device seconds of reps applications (hipEvents on the handle's stream)."""
+        out = C.c_double()
+        _check(self.lib.mfs_time_operator(self._h, which, level, C.c_void_p(d_src), C.c_void_p(d_dst), reps,
+                                          C.byref(out)))
+        return out.value
+
+    def synchronize(self):
+        _check(self.lib.mfs_synchronize(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.lib.mfs_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass

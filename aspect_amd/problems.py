@@ -1,0 +1,179 @@
+"""Host-side problem definitions for the box configurations the solver is
+measured and tested on: quadrature-point coordinates, material fields
+(viscosity, density) and ASPECT's pressure scaling rule.
+
+These play the role of ASPECT's material-model plugins and Simulator helpers
+that sit above the Stokes handler; they produce the per-quadrature-point
+inputs that ``StokesMatrixFreeHandler.assemble`` receives.
+
+References (ASPECT tree):
+  * nsinker material law        benchmarks/nsinker/nsinker.cc:61-76, 282-314
+  * pressure scaling            source/simulator/helper_functions.cc:255-345
+  * box length scale            source/geometry_model/box.cc:252-255
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .mfs import BC_TANGENTIAL, BC_ZERO, make_box
+
+
+def gauss_points(n: int):
+    """Gauss-Legendre points and weights on [0, 1]."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def qp_coordinates(dim, degree, n_cells, extent):
+    """Coordinates of the QGauss(degree+1) points of every cell:
+    array [cells, nq, dim], cells lexicographic (x fastest), q lexicographic (x fastest)."""
+    xq, _ = gauss_points(degree + 1)
+    h = [extent[d] / n_cells[d] for d in range(dim)]
+    axes = []
+    for d in range(dim):
+        # [n_cells_d, Q] coordinates along d
+        axes.append(((np.arange(n_cells[d])[:, None] + xq[None, :]) * h[d]))
+    Q = degree + 1
+    if dim == 2:
+        X = axes[0][None, :, None, :]  # [cy, cx, qy, qx]
+        Y = axes[1][:, None, :, None]
+        X, Y = np.broadcast_arrays(X, Y)
+        return np.stack([X, Y], axis=-1).reshape(n_cells[1] * n_cells[0], Q * Q, 2)
+    X = axes[0][None, None, :, None, None, :]  # cz, cy, cx, qz, qy, qx
+    Y = axes[1][None, :, None, None, :, None]
+    Z = axes[2][:, None, None, :, None, None]
+    X, Y, Z = np.broadcast_arrays(X, Y, Z)
+    out = np.stack([X, Y, Z], axis=-1)
+    return out.reshape(n_cells[2] * n_cells[1] * n_cells[0], Q ** 3, 3)
+
+
+def cell_midpoints(dim, n_cells, extent):
+    h = [extent[d] / n_cells[d] for d in range(dim)]
+    grids = np.meshgrid(*[(np.arange(n_cells[d]) + 0.5) * h[d] for d in range(dim)][::-1], indexing="ij")
+    return np.stack(grids[::-1], axis=-1).reshape(-1, dim)
+
+
+def pressure_scaling(eta_at_midpoints: np.ndarray, length_scale: float) -> float:
+    """compute_pressure_scaling_factor: exp(volume-weighted mean of log(eta)
+    at cell midpoints) / length_scale (uniform cells: plain mean)."""
+    return float(np.exp(np.mean(np.log(eta_at_midpoints))) / length_scale)
+
+
+# ------------------------------------------------------------------ nsinker
+NSINKER_CENTERS = np.array([  # first published sinker centres of the nsinker benchmark
+    [2.4257829890e-01, 1.3469574514e-02, 3.8313885004e-01],
+    [4.1465269048e-01, 6.7768972864e-02, 9.9312692973e-01],
+    [4.8430804651e-01, 7.6533776604e-01, 3.1833815403e-02],
+    [3.0935481671e-02, 9.3264044027e-01, 8.8787953411e-01],
+])
+
+
+def nsinker_chi(x, n_sinkers=4, delta=200.0, omega=0.1):
+    """inside_outside_factor: product over sinkers of 1 - exp(-delta max(0, d - omega/2)^2)."""
+    chi = np.ones(x.shape[:-1])
+    for s in NSINKER_CENTERS[:n_sinkers]:
+        d = np.linalg.norm(x - s[: x.shape[-1]], axis=-1)
+        chi *= 1.0 - np.exp(-delta * np.maximum(0.0, d - omega / 2.0) ** 2)
+    return chi
+
+
+def nsinker_material(x, viscosity_ratio=1e4, sinker_density=10.0, n_sinkers=4):
+    chi = nsinker_chi(x, n_sinkers)
+    r = np.sqrt(viscosity_ratio)
+    eta = (r - 1.0 / r) * (1.0 - chi) + 1.0 / r
+    rho = sinker_density * (1.0 - chi)
+    return eta, rho
+
+
+# ------------------------------------------------------------------ seeded sinker configurations
+def smooth_sinkers(x, centres, radius=0.05, width=0.01):
+    """max over sinkers of 1/2 (1 - tanh((|x - c| - r) / w))."""
+    chi = np.zeros(x.shape[:-1])
+    for c in centres:
+        d = np.linalg.norm(x - c, axis=-1)
+        chi = np.maximum(chi, 0.5 * (1.0 - np.tanh((d - radius) / width)))
+    return chi
+
+
+@dataclass
+class BoxConfig:
+    name: str
+    dim: int
+    coarse: tuple
+    n_levels: int
+    bc: tuple
+    degree: int = 2
+    extent: tuple = (1.0, 1.0, 1.0)
+    averaging: str = "none"
+    gmg_precision: str = "fp64"
+    tolerance: float = 1e-8
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_cells(self):
+        return tuple(self.coarse[d] << (self.n_levels - 1) for d in range(self.dim))
+
+    def box(self, pressure_scaling=1.0):
+        return make_box(self.dim, self.degree, self.coarse, self.n_levels, self.extent, self.bc,
+                        pressure_scaling=pressure_scaling)
+
+    def gravity(self):
+        return np.array([0.0, -1.0]) if self.dim == 2 else np.array([0.0, 0.0, -1.0])
+
+
+def config_2d_32(**kw):
+    """2D unit box, 32x32, eta = 1, rho = 1 + 0.01 sin(pi x) sin(pi y), free slip."""
+    return BoxConfig("2d_box_32", 2, (1, 1), 6, (BC_TANGENTIAL,) * 4, **kw)
+
+
+def config_sinker_64(**kw):
+    """3D unit box, 64^3, 8 seeded sinkers, eta 1e3 inside / 1 outside, free slip."""
+    return BoxConfig("sinker_64", 3, (1, 1, 1), 7, (BC_TANGENTIAL,) * 6,
+                     meta=dict(n_sinkers=8, contrast=1e3, seed=2), **kw)
+
+
+def config_sinker_128(**kw):
+    """3D unit box, 128^3 (2x2x2 coarse box, 7 levels), 16 seeded sinkers, 1e6 contrast."""
+    kw.setdefault("gmg_precision", "fp32")
+    return BoxConfig("sinker_128", 3, (2, 2, 2), 7, (BC_TANGENTIAL,) * 6,
+                     meta=dict(n_sinkers=16, contrast=1e6, seed=3), **kw)
+
+
+def material(cfg: BoxConfig, x):
+    """(eta, rho) at points x [..., dim] for a configuration."""
+    if cfg.name.startswith("2d_box"):
+        eta = np.ones(x.shape[:-1])
+        rho = 1.0 + 0.01 * np.sin(np.pi * x[..., 0]) * np.sin(np.pi * x[..., 1])
+        return eta, rho
+    if cfg.name.startswith("sinker"):
+        rng = np.random.default_rng(cfg.meta["seed"])
+        centres = rng.uniform(0.1, 0.9, size=(cfg.meta["n_sinkers"], cfg.dim))
+        chi = smooth_sinkers(x, centres)
+        eta = 1.0 + (cfg.meta["contrast"] - 1.0) * chi
+        rho = 1.0 + chi
+        return eta, rho
+    if cfg.name.startswith("nsinker"):
+        return nsinker_material(x, cfg.meta.get("viscosity_ratio", 1e4), cfg.meta.get("sinker_density", 10.0))
+    raise ValueError(cfg.name)
+
+
+def config_nsinker(nref=2, averaging="none", **kw):
+    """ASPECT's nsinker benchmark (benchmarks/nsinker/nsinker_gmg.prm): unit cube,
+    zero velocity on all faces, 4 sinkers, ratio 1e4, pressure scaling 1."""
+    return BoxConfig(f"nsinker_{nref}", 3, (1, 1, 1), nref + 1, (BC_ZERO,) * 6, averaging=averaging,
+                     tolerance=1e-6, meta=dict(viscosity_ratio=1e4, sinker_density=10.0), **kw)
+
+
+def fields(cfg: BoxConfig):
+    """eta and rho at every quadrature point ([cells, nq]) and the pressure scaling."""
+    x = qp_coordinates(cfg.dim, cfg.degree, cfg.n_cells, cfg.extent)
+    eta, rho = material(cfg, x)
+    if cfg.name.startswith("nsinker"):
+        s = 1.0  # nsinker's pressure-scaling signal returns 1
+    else:
+        mid = cell_midpoints(cfg.dim, cfg.n_cells, cfg.extent)
+        eta_mid, _ = material(cfg, mid)
+        s = pressure_scaling(eta_mid, 0.01 * cfg.extent[0])
+    return eta, rho, s

@@ -1,0 +1,188 @@
+"""Benchmark of the MI355X matrix-free Stokes solver.
+
+Workload: 3D unit box, 128^3 Q2xQ1 cells (2x2x2 coarse box, 7 levels), 16
+seeded sinkers with a 1e6 viscosity contrast, free-slip walls, viscosity per
+quadrature point on the active level.
+
+A step is one FP64 application of the fine-level Stokes operator (the metric's
+"Stokes matvec"), inputs resident in HBM.  ``value`` is GDoF/s = (N_u + N_p) /
+average device time per step, timed with hipEvents on the library's stream.
+After the timed steps the script reports, outside ``value``: the FP32 level
+A-block apply rate, and time-to-solve to 1e-8 relative residual with the FP32
+V-cycle and with the FP64 V-cycle (both inside FP64 GMRES/FGMRES).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-solve] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_config(size):
+    from aspect_amd import problems as P
+    cfg = P.config_sinker_128()
+    if size != 128:
+        cfg.n_levels = int(np.log2(size // 2)) + 1
+        cfg.name = "sinker_128"  # same material law, smaller mesh
+    return cfg
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+    from aspect_amd import StokesMatrixFreeHandler, problems as P
+
+    cfg = make_config(args.size)
+    t0 = time.perf_counter()
+    eta, rho, s = P.fields(cfg)
+    log(f"fields {cfg.n_cells} cells in {time.perf_counter() - t0:.1f}s, pressure scaling {s:.4g}")
+
+    h32 = StokesMatrixFreeHandler(cfg.box(s), "fp32")
+    h32.assemble(eta, "none")
+    t0 = time.perf_counter()
+    h32.build_preconditioner()
+    log(f"FP32 GMG set-up {time.perf_counter() - t0:.2f}s")
+    sz = h32.sizes(h32.top)
+    nu, npd, cells = sz["nu"], sz["np"], sz["cells"]
+    n = nu + npd
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    x = torch.randn(n, dtype=torch.float64, device=dev, generator=gen)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+
+    # warm-up, then exactly K timed steps bracketed by synchronization
+    if args.warmup:
+        h32.time_operator(0, 0, x.data_ptr(), y.data_ptr(), args.warmup)
+    h32.synchronize()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    t_step = h32.time_operator(0, 0, x.data_ptr(), y.data_ptr(), args.steps)
+    h32.synchronize()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - w0) / args.steps
+    gdofs = n / t_step / 1e9
+    nq = 27
+    bytes_stokes = 8 * 2 * n + 8 * cells * nq
+    achieved = bytes_stokes / t_step / 1e9
+    log(f"Stokes FP64 apply: {t_step * 1e3:.3f} ms (wall {wall * 1e3:.3f} ms) {gdofs:.3f} GDoF/s "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    # FP32 fine-level A-block apply (cellwise level viscosity)
+    u32 = torch.randn(nu, dtype=torch.float32, device=dev, generator=gen)
+    v32 = torch.empty_like(u32)
+    torch.cuda.synchronize()
+    h32.time_operator(1, h32.top, u32.data_ptr(), v32.data_ptr(), 2)
+    t_a32 = h32.time_operator(1, h32.top, u32.data_ptr(), v32.data_ptr(), max(5, args.steps // 2))
+    bytes_a32 = 4 * 2 * nu + 4 * cells
+    log(f"A-block FP32 fine level: {t_a32 * 1e3:.3f} ms {nu / t_a32 / 1e9:.3f} GDoF/s "
+        f"{bytes_a32 / t_a32 / 1e9:.0f} GB/s algorithmic")
+
+    result = {
+        "metric": "Stokes matvec GDoF/s; solve time to 1e-8 residual, FP32-GMG vs FP64",
+        "value": gdofs,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"sinker box {args.size}^3 Q2xQ1, 16 seeded sinkers, contrast 1e6, "
+                               "per-qp viscosity, free slip; step = one FP64 Stokes operator apply",
+                   "cells": cells, "velocity_dofs": nu, "pressure_dofs": npd, "levels": cfg.n_levels},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "ablock_fp32_gdofs": nu / t_a32 / 1e9,
+        "ablock_fp32_ms": t_a32 * 1e3,
+        "wall_ms_per_step": wall * 1e3,
+    }
+
+    if not args.no_solve:
+        g = cfg.gravity()
+        rhs = torch.empty(n, dtype=torch.float64, device=dev)
+        rho_d = torch.from_numpy(np.ascontiguousarray(rho.reshape(-1))).to(dev)
+        torch.cuda.synchronize()
+        h32.assemble_buoyancy_rhs_device(rho_d.data_ptr(), g, rhs.data_ptr())
+        del rho_d
+        sol32 = torch.zeros(n, dtype=torch.float64, device=dev)
+        rc, r32 = h32.solve_device(rhs.data_ptr(), 0, sol32.data_ptr(), tolerance=cfg.tolerance)
+        log(f"FP32-GMG solve: {r32['cheap_iterations']}+{r32['expensive_iterations']} its "
+            f"{r32['solve_seconds']:.3f}s converged={r32['converged']}")
+        del h32
+        torch.cuda.empty_cache()
+        h64 = StokesMatrixFreeHandler(cfg.box(s), "fp64")
+        h64.assemble(eta, "none")
+        h64.build_preconditioner()
+        sol64 = torch.zeros(n, dtype=torch.float64, device=dev)
+        rc, r64 = h64.solve_device(rhs.data_ptr(), 0, sol64.data_ptr(), tolerance=cfg.tolerance)
+        log(f"FP64-GMG solve: {r64['cheap_iterations']}+{r64['expensive_iterations']} its "
+            f"{r64['solve_seconds']:.3f}s converged={r64['converged']}")
+        h64.synchronize()
+        du = torch.linalg.norm(sol32[:nu] - sol64[:nu]) / torch.linalg.norm(sol64[:nu])
+        dp = torch.linalg.norm(sol32[nu:] - sol64[nu:]) / torch.linalg.norm(sol64[nu:])
+        result["solve"] = {
+            "fp32_gmg_seconds": r32["solve_seconds"], "fp32_gmg_iterations": r32["cheap_iterations"],
+            "fp32_gmg_expensive_iterations": r32["expensive_iterations"], "fp32_converged": bool(r32["converged"]),
+            "fp64_gmg_seconds": r64["solve_seconds"], "fp64_gmg_iterations": r64["cheap_iterations"],
+            "fp64_gmg_expensive_iterations": r64["expensive_iterations"], "fp64_converged": bool(r64["converged"]),
+            "rel_l2_velocity_fp32_vs_fp64": float(du), "rel_l2_pressure_fp32_vs_fp64": float(dp),
+            "tolerance": cfg.tolerance,
+        }
+        del h64
+
+    if not args.no_cpu:
+        # CPU baseline: the oracle's OpenMP sum-factorized FP64 Stokes apply
+        # (CPU restatement, not the reference binary), one apply of the same workload.
+        import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(cores)
+        p = O.make_problem(3, 2, cfg.coarse, cfg.n_levels, cfg.extent, cfg.bc, pressure_scaling=s)
+        xc = x.cpu().numpy()
+        O.stokes_vmult(O.make_problem(3, 2, (1, 1, 1), 2, (1, 1, 1), cfg.bc, pressure_scaling=s), 1,
+                       np.ones(64), np.zeros(3 * 9 ** 3 + 5 ** 3))
+        t0 = time.perf_counter()
+        O.stokes_vmult(p, cfg.n_levels - 1, eta, xc)
+        tc = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": n / tc / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "port",
+                                  "sample": f"one FP64 Stokes apply of the {args.size}^3 workload "
+                                            f"({tc:.2f}s), oracle OpenMP sum factorization"}
+        log(f"CPU baseline {n / tc / 1e9:.4f} GDoF/s on {cores} threads ({tc:.2f}s)")
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,188 @@
+/*
+ * aspect_mfs.h — C-ABI of the MI355X (gfx950) matrix-free Stokes solver.
+ *
+ * This is the drop-in boundary for ASPECT's StokesMatrixFreeHandler
+ * (include/aspect/simulator/solver/stokes_matrix_free.h:47-98): host C++ on the
+ * ASPECT side (a StokesMatrixFreeHandler subclass returned by
+ * create_matrix_free_solver, source/simulator/solver/stokes_matrix_free.cc:29-62)
+ * calls these functions with plain pointers and sizes.  No torch or HIP types
+ * appear in the signatures.
+ *
+ * Conventions
+ *   - Every function returns an int status: MFS_OK, MFS_ERROR (message in
+ *     mfs_last_error()), or MFS_NO_CONVERGENCE (solve only; result filled).
+ *   - Host buffers are caller-owned and copied in/out; device buffers are
+ *     handle-owned.  *_device entry points take device pointers that the caller
+ *     owns (allocated with hipMalloc on the handle's device).
+ *   - One handle per host thread; calls on one handle are not reentrant.
+ *   - Vector layout: velocity block first, component-major, nodes
+ *     lexicographic with x fastest ((k*n_x+1) x (k*n_y+1) [x (k*n_z+1)] nodes per
+ *     component); the pressure block follows ((k-1)*n+1 nodes per direction).
+ *     Per-cell data: cells lexicographic with x fastest; per-quadrature-point
+ *     data: [cell][q] with q lexicographic over the QGauss(k+1) points, x fastest.
+ */
+#ifndef ASPECT_MFS_H
+#define ASPECT_MFS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFS_OK 0
+#define MFS_ERROR 1
+#define MFS_NO_CONVERGENCE 2
+
+/* GMGNumberType of the V-cycle (stokes_matrix_free.h:40 fixes it to double). */
+#define MFS_GMG_FP64 0
+#define MFS_GMG_FP32 1
+
+/* Material averaging applied to per-quadrature-point viscosities
+ * (source/material_model/interface.cc:556-680). */
+#define MFS_AVERAGE_NONE 0
+#define MFS_AVERAGE_ARITHMETIC 1
+#define MFS_AVERAGE_HARMONIC 2
+#define MFS_AVERAGE_GEOMETRIC 3
+
+/* Boundary type per face (x-, x+, y-, y+, z-, z+). */
+#define MFS_BC_OPEN 0
+#define MFS_BC_TANGENTIAL 1 /* "Tangential velocity boundary indicators" (no normal flux) */
+#define MFS_BC_ZERO 2       /* "Zero velocity boundary indicators" */
+
+typedef struct mfs_handle mfs_handle;
+
+/* A box geometry refined globally (GeometryModel::Box with repetitions, plus
+ * "Initial global refinement"); the level hierarchy is level 0 = coarse mesh. */
+typedef struct {
+  int dim;                 /* 2 or 3 */
+  int degree;              /* "Stokes velocity polynomial degree": 2 or 3 */
+  int coarse[3];           /* cells per direction on level 0 */
+  int n_levels;            /* levels; finest has coarse << (n_levels-1) cells */
+  double extent[3];        /* box extents, origin at 0 */
+  int bc[6];               /* MFS_BC_* per face */
+  int compressible;        /* subtract (2/3) eta div(u) I (matrix_free_operators.cc:466-469) */
+  double pressure_scaling; /* Simulator::get_pressure_scaling() */
+} mfs_box;
+
+typedef struct {
+  double tolerance;         /* "Linear solver tolerance" */
+  int n_cheap_steps;        /* "Number of cheap Stokes solver steps" */
+  int n_expensive_steps;    /* "Maximum number of expensive Stokes solver steps" */
+  int restart;              /* "GMRES solver restart length" */
+  double a_block_tolerance; /* "Linear solver A block tolerance" */
+  double s_block_tolerance; /* "Linear solver S block tolerance" */
+} mfs_solve_params;
+
+/* StokesSolver::SolverOutputs (include/aspect/simulator/solver/interface.h:36-58)
+ * plus the counters ASPECT's post_stokes_solver signal reports. */
+typedef struct {
+  int cheap_iterations;
+  int expensive_iterations;
+  int a_preconditioner_applications;
+  int s_preconditioner_applications;
+  int converged;
+  double initial_nonlinear_residual;
+  double final_linear_residual;
+  double solver_tolerance;
+  double r0_norm;
+  double pressure_normalization_adjustment;
+  double solve_seconds; /* device time of the Krylov solve (hipEvent) */
+} mfs_solve_result;
+
+/* Error message of the last failing call on this thread. */
+const char *mfs_last_error(void);
+
+/* setup_dofs(): stokes_matrix_free_local_smoothing.cc:1450-1851 — level meshes,
+ * DoF layout, level constraints and device storage. */
+int mfs_create(const mfs_box *box, int gmg_precision, mfs_handle **out);
+int mfs_destroy(mfs_handle *h);
+
+/* name(): local_smoothing.cc:146-151 ("GMG"). Writes at most len bytes. */
+int mfs_name(const mfs_handle *h, char *buf, int len);
+
+/* Sizes of a level: out[0] cells, out[1] velocity DoFs, out[2] pressure DoFs,
+ * out[3] velocity nodes. */
+int mfs_sizes(const mfs_handle *h, int level, long long out[4]);
+
+/* assemble() / evaluate_material_model(): local_smoothing.cc:173-294 and
+ * fill_active_cell_data (matrix_free_operators.cc:137-330).  eta holds the
+ * material model's viscosity at every quadrature point of the active level
+ * ([cell][q]); averaging selects the active table (per quadrature point for
+ * MFS_AVERAGE_NONE, otherwise one value per cell).  Level tables are the DGQ0
+ * projection restricted by averaging children. */
+int mfs_set_viscosity(mfs_handle *h, const double *eta_qp, int averaging);
+int mfs_set_pressure_scaling(mfs_handle *h, double pressure_scaling);
+
+/* build_preconditioner(): local_smoothing.cc:1855-1867 (level diagonals) and the
+ * Chebyshev eigenvalue estimates of solve() (:827-849). */
+int mfs_build_preconditioner(mfs_handle *h);
+
+/* StokesOperator::vmult (matrix_free_operators.cc:376-547, 606-627), FP64,
+ * active level.  src/dst hold velocity then pressure. */
+int mfs_stokes_vmult(mfs_handle *h, const double *src, double *dst);
+/* ABlockOperator::vmult (:974-1064).  level >= 0: multigrid level operator in
+ * the GMG precision with the level viscosity; level == -1: the active-level
+ * operator (FP64, active viscosity table) used by the expensive solver. */
+int mfs_ablock_vmult(mfs_handle *h, int level, const double *src, double *dst);
+/* MassMatrixOperator::vmult (:787-857), same level convention. */
+int mfs_mass_vmult(mfs_handle *h, int level, const double *src, double *dst);
+/* BTBlockOperator::vmult (:668-725), active level, FP64: dst_u = B^T src_p. */
+int mfs_bt_vmult(mfs_handle *h, const double *src_p, double *dst_u);
+/* compute_diagonal of the A block / mass (:1068-1098, :861-908), level >= 0,
+ * the diagonal itself (ones on constrained DoFs), not its inverse. */
+int mfs_ablock_diagonal(mfs_handle *h, int level, double *diag);
+int mfs_mass_diagonal(mfs_handle *h, int level, double *diag);
+/* MGTransferMatrixFree prolongate / restrict_and_add (set up at
+ * local_smoothing.cc:1843-1850); kind 0 velocity, 1 pressure. */
+int mfs_prolongate(mfs_handle *h, int fine_level, int kind, const double *src_coarse, double *dst_fine);
+int mfs_restrict(mfs_handle *h, int fine_level, int kind, const double *src_fine, double *dst_coarse);
+/* Level viscosity table (cells lexicographic), local_smoothing.cc:211-294. */
+int mfs_level_viscosity(mfs_handle *h, int level, double *out);
+/* Chebyshev set-up per level: out[(level*2+kind)*5 + {0..4}] =
+ * lambda_min, lambda_max, theta, delta, eigenvalue-CG iterations. */
+int mfs_smoother_info(mfs_handle *h, double *out);
+/* One PreconditionChebyshev application with zero initial guess. */
+int mfs_chebyshev(mfs_handle *h, int kind, int level, const double *src, double *dst);
+/* One PreconditionMG::vmult V-cycle (kind 0: A block, 1: Schur mass). */
+int mfs_vcycle(mfs_handle *h, int kind, const double *src, double *dst);
+/* BlockSchurPreconditioner::vmult, cheap variant
+ * (block_stokes_preconditioner.h:220-255). */
+int mfs_block_preconditioner(mfs_handle *h, const double *src, double *dst);
+
+/* solve(): local_smoothing.cc:758-1446.  rhs is the Stokes right-hand side,
+ * x0 the current linearization point (NULL: zero); sol receives velocity and
+ * pressure (pressure rescaled and volume-normalized). */
+int mfs_solve(mfs_handle *h, const double *rhs, const double *x0, double *sol, const mfs_solve_params *prm,
+              mfs_solve_result *res);
+
+/* Buoyancy right-hand side of the incompressible Stokes system,
+ * rhs_u = int rho g . phi_u, rhs_p = 0, constrained rows zero
+ * (source/simulator/assemblers/stokes.cc:393-396).  rho_qp: density at every
+ * quadrature point of the active level ([cell][q]); gravity: dim components. */
+int mfs_assemble_buoyancy_rhs(mfs_handle *h, const double *rho_qp, const double *gravity, double *rhs);
+
+/* Device-resident variants: all pointers are device pointers. */
+int mfs_stokes_vmult_device(mfs_handle *h, const double *d_src, double *d_dst);
+/* Level A-block apply in the GMG precision (float or double device arrays). */
+int mfs_ablock_vmult_device(mfs_handle *h, int level, const void *d_src, void *d_dst);
+int mfs_solve_device(mfs_handle *h, const double *d_rhs, const double *d_x0, double *d_sol,
+                     const mfs_solve_params *prm, mfs_solve_result *res);
+int mfs_assemble_buoyancy_rhs_device(mfs_handle *h, const double *d_rho_qp, const double *gravity, double *d_rhs);
+int mfs_synchronize(mfs_handle *h);
+
+/* "Execute solver timings" harness (local_smoothing.cc:1119-1258): average
+ * device time of reps applications of an operator, bracketed by hipEvents on
+ * the handle's stream.  which: 0 Stokes (FP64, active level; d_src/d_dst FP64
+ * block vectors), 1 level A block (GMG precision), 2 level mass (GMG
+ * precision), 3 A-block V-cycle, 4 Schur V-cycle (FP64 vectors),
+ * 5 cheap block preconditioner (FP64 block vectors). */
+int mfs_time_operator(mfs_handle *h, int which, int level, const void *d_src, void *d_dst, int reps,
+                      double *avg_seconds);
+
+/* Memory consumption getters of StokesMatrixFreeHandler (stokes_matrix_free.h:81-97),
+ * bytes of device memory: out[0] vectors/levels, out[1] cell data, out[2] Krylov space. */
+int mfs_memory_consumption(const mfs_handle *h, long long out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASPECT_MFS_H */

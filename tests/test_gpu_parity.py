@@ -1,0 +1,217 @@
+"""GPU parity tests: the HIP path (through the C-ABI) against the oracle on the
+same seeded inputs.  Bars: 1e-12 relative for FP64 operator applies, 1e-5 for
+FP32 level operators, iteration counts within 10% for full solves."""
+import numpy as np
+import pytest
+
+import oracle as O
+from aspect_amd import StokesMatrixFreeHandler, make_box, problems as P
+
+pytestmark = pytest.mark.gpu
+
+FP64 = 1e-12
+FP32 = 1e-5
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def setup_pair(dim, deg, coarse, nl, extent, bc, s=1.0, per_qp=True, seed=0, prec="fp64"):
+    p = O.make_problem(dim, deg, coarse, nl, extent, bc, pressure_scaling=s)
+    box = make_box(dim, deg, coarse, nl, extent, bc, pressure_scaling=s)
+    h = StokesMatrixFreeHandler(box, prec)
+    top = nl - 1
+    sz = O.sizes(p, top)
+    rng = np.random.default_rng(seed)
+    eta = np.exp(rng.normal(size=(sz["cells"], O.n_qp(p))))
+    h.assemble(eta, "none" if per_qp else "arithmetic")
+    eta_o = eta if per_qp else eta.mean(axis=1)
+    return p, h, eta_o, sz, rng
+
+
+CASES = [
+    (2, 2, (1, 1, 1), 6, (1.0, 1.0, 1.0), (1, 1, 1, 1, 0, 0)),   # 2D 32x32 free slip
+    (3, 2, (1, 1, 1), 3, (1.0, 1.0, 1.0), (2,) * 6),             # 3D 4^3 no slip
+    (3, 2, (3, 2, 1), 3, (1.5, 1.0, 0.5), (1, 2, 0, 1, 1, 2)),   # 3D 12x8x4 mixed, partial tiles
+    (2, 3, (1, 2, 1), 4, (1.0, 2.0, 1.0), (0, 1, 2, 1, 0, 0)),   # 2D Q3xQ2
+    (3, 3, (1, 1, 1), 3, (1.0, 1.0, 1.0), (1,) * 6),             # 3D Q3xQ2
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_stokes_vmult_fp64(case):
+    p, h, eta, sz, rng = setup_pair(*case, s=2.5)
+    x = rng.normal(size=sz["nu"] + sz["np"])
+    top = case[3] - 1
+    ref = O.stokes_vmult(p, top, eta, x)
+    assert rel(h.stokes_vmult(x), ref) < FP64
+    # cellwise viscosity table too
+    p2, h2, eta2, _, _ = setup_pair(*case, s=2.5, per_qp=False)
+    assert rel(h2.stokes_vmult(x), O.stokes_vmult(p2, top, eta2, x)) < FP64
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_block_operators_fp64(case):
+    p, h, eta, sz, rng = setup_pair(*case, s=0.7)
+    top = case[3] - 1
+    u = rng.normal(size=sz["nu"])
+    q = rng.normal(size=sz["np"])
+    assert rel(h.ablock_vmult(-1, u), O.ablock_vmult(p, top, eta, u)) < FP64
+    assert rel(h.mass_vmult(-1, q), O.mass_vmult(p, top, eta, q)) < FP64
+    assert rel(h.bt_vmult(q), O.bt_vmult(p, top, q)) < FP64
+    lv = O.level_viscosity(p, eta)
+    for lev in range(case[3]):
+        np.testing.assert_allclose(h.level_viscosity(lev), lv[lev], rtol=1e-14)
+        s = O.sizes(p, lev)
+        ul = rng.normal(size=s["nu"])
+        ql = rng.normal(size=s["np"])
+        assert rel(h.ablock_vmult(lev, ul), O.ablock_vmult(p, lev, lv[lev], ul)) < FP64
+        assert rel(h.mass_vmult(lev, ql), O.mass_vmult(p, lev, lv[lev], ql)) < FP64
+        assert rel(h.ablock_diagonal(lev), O.ablock_diagonal(p, lev, lv[lev])) < FP64
+        assert rel(h.mass_diagonal(lev), O.mass_diagonal(p, lev, lv[lev])) < FP64
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_level_operators_fp32(case):
+    p, h, eta, sz, rng = setup_pair(*case, s=1.3, prec="fp32")
+    lv = O.level_viscosity(p, eta)
+    for lev in range(case[3]):
+        s = O.sizes(p, lev)
+        ul = rng.normal(size=s["nu"])
+        ql = rng.normal(size=s["np"])
+        assert rel(h.ablock_vmult(lev, ul), O.ablock_vmult(p, lev, lv[lev], ul)) < FP32
+        assert rel(h.mass_vmult(lev, ql), O.mass_vmult(p, lev, lv[lev], ql)) < FP32
+        assert rel(h.ablock_diagonal(lev), O.ablock_diagonal(p, lev, lv[lev])) < FP32
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_transfers(case):
+    p, h, eta, sz, rng = setup_pair(*case)
+    for fl in range(1, case[3]):
+        for kind in (0, 1):
+            key = "nu" if kind == 0 else "np"
+            xc = rng.normal(size=O.sizes(p, fl - 1)[key])
+            xf = rng.normal(size=O.sizes(p, fl)[key])
+            assert rel(h.prolongate(fl, kind, xc), O.prolongate(p, fl, kind, xc)) < FP64
+            assert rel(h.restrict(fl, kind, xf), O.restrict(p, fl, kind, xf)) < FP64
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_smoother_setup_and_vcycle_fp64(case):
+    p, h, eta, sz, rng = setup_pair(*case, s=1.9)
+    h.build_preconditioner()
+    info = h.smoother_info()
+    ref = O.smoother_setup(p, eta)
+    np.testing.assert_allclose(info[:, :, 4], ref[:, :, 4])  # eigen-CG iteration counts
+    np.testing.assert_allclose(info[:, :, :4], ref[:, :, :4], rtol=1e-9)
+    top = case[3] - 1
+    for kind, key in ((0, "nu"), (1, "np")):
+        x = rng.normal(size=sz[key])
+        assert rel(h.chebyshev(kind, top, x), O.chebyshev(p, eta, kind, top, x)) < 1e-10
+        assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < 1e-10
+
+
+def test_vcycle_fp32_close_to_fp64():
+    case = CASES[2]
+    p, h, eta, sz, rng = setup_pair(*case, s=1.9, prec="fp32")
+    h.build_preconditioner()
+    for kind, key in ((0, "nu"), (1, "np")):
+        x = rng.normal(size=sz[key])
+        assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < 1e-4
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_buoyancy_rhs(case):
+    p, h, eta, sz, rng = setup_pair(*case)
+    rho = rng.uniform(0.5, 2.0, size=(sz["cells"], O.n_qp(p)))
+    g = np.array([0.3, -1.0, 0.2])[: case[0]]
+    assert rel(h.assemble_buoyancy_rhs(rho, g), O.rhs_buoyancy(p, rho, g)) < FP64
+
+
+def test_many_tiles_ragged():
+    # 3D 20x12x36 cells: 5x3x9 tiles of 4^3, tile colours shared in all directions
+    case = (3, 2, (5, 3, 9), 3, (1.0, 0.6, 1.8), (1, 1, 2, 2, 0, 1))
+    p, h, eta, sz, rng = setup_pair(*case, s=1.1)
+    x = rng.normal(size=sz["nu"] + sz["np"])
+    assert rel(h.stokes_vmult(x), O.stokes_vmult(p, 2, eta, x)) < FP64
+    lv = O.level_viscosity(p, eta)
+    u = rng.normal(size=sz["nu"])
+    assert rel(h.ablock_vmult(2, u), O.ablock_vmult(p, 2, lv[2], u)) < FP64
+
+
+def test_determinism():
+    p, h, eta, sz, rng = setup_pair(*CASES[2])
+    x = rng.normal(size=sz["nu"] + sz["np"])
+    a, b = h.stokes_vmult(x), h.stokes_vmult(x)
+    assert np.array_equal(a, b)
+
+
+def _nsinker_gpu(avg, prec):
+    cfg = P.config_nsinker(2, averaging=avg)
+    eta, rho, s = P.fields(cfg)
+    h = StokesMatrixFreeHandler(cfg.box(1.0), prec)
+    h.assemble(eta, avg)
+    h.build_preconditioner()
+    rhs = h.assemble_buoyancy_rhs(rho, cfg.gravity())
+    sol, out = h.solve(rhs, tolerance=1e-6, n_cheap_steps=200, n_expensive_steps=0, raise_on_failure=False)
+    return sol, out, eta, rho, cfg
+
+
+@pytest.mark.parametrize("avg,golden", [("none", 172), ("harmonic", 21)])
+def test_golden_nsinker_gpu(avg, golden):
+    # tests/nsinker_gmg_no_avg and tests/nsinker_gmg_gc screen-output (ASPECT tree)
+    sol, out, eta, rho, cfg = _nsinker_gpu(avg, "fp64")
+    assert out.converged
+    assert abs(out.cheap_iterations - golden) <= 0.1 * golden
+    p = O.make_problem(3, 2, (1, 1, 1), 3, (1, 1, 1), (2,) * 6)
+    eta_o = eta if avg == "none" else 1.0 / np.mean(1.0 / eta, axis=1)
+    ref, res = O.solve(p, eta_o, O.rhs_buoyancy(p, rho, cfg.gravity()),
+                       params=O.default_params(tolerance=1e-6, n_cheap=200, n_expensive=0))
+    assert out.cheap_iterations == res["cheap_iterations"]
+    nu = O.sizes(p, 2)["nu"]
+    assert rel(sol[:nu], ref[:nu]) < 1e-6
+    assert rel(sol[nu:], ref[nu:]) < 1e-6
+
+
+def test_2d_config_solve_fp64_and_fp32():
+    cfg = P.config_2d_32()
+    eta, rho, s = P.fields(cfg)
+    p = O.make_problem(2, 2, (1, 1), 6, (1, 1, 1), (1, 1, 1, 1, 0, 0), pressure_scaling=s)
+    rhs_o = O.rhs_buoyancy(p, rho, cfg.gravity())
+    ref, res = O.solve(p, eta, rhs_o, params=O.default_params(tolerance=1e-8))
+    for prec in ("fp64", "fp32"):
+        h = StokesMatrixFreeHandler(cfg.box(s), prec)
+        h.assemble(eta, "none")
+        h.build_preconditioner()
+        rhs = h.assemble_buoyancy_rhs(rho, cfg.gravity())
+        assert rel(rhs, rhs_o) < FP64
+        sol, out = h.solve(rhs, tolerance=1e-8)
+        assert out.converged
+        assert abs(out.cheap_iterations - res["cheap_iterations"]) <= max(1, 0.1 * res["cheap_iterations"])
+        nu = O.sizes(p, 5)["nu"]
+        assert rel(sol[:nu], ref[:nu]) < 1e-6
+        assert rel(sol[nu:], ref[nu:]) < 1e-6
+
+
+def test_expensive_fgmres_path():
+    # no cheap steps: FGMRES with inner CG solves (local_smoothing.cc:1314-1397)
+    case = CASES[1]
+    p, h, eta, sz, rng = setup_pair(*case)
+    h.build_preconditioner()
+    rhs = O.rhs_buoyancy(p, rng.uniform(size=(sz["cells"], O.n_qp(p))), np.array([0, 0, -1.0]))
+    sol, out = h.solve(rhs, tolerance=1e-8, n_cheap_steps=0)
+    ref, res = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-8, n_cheap=0))
+    assert out.converged and out.cheap_iterations == 0
+    assert abs(out.expensive_iterations - res["expensive_iterations"]) <= 1
+    assert rel(sol, ref) < 1e-6
+
+
+def test_no_convergence_is_reported():
+    case = CASES[1]
+    p, h, eta, sz, rng = setup_pair(*case)
+    h.build_preconditioner()
+    rhs = O.rhs_buoyancy(p, np.ones((sz["cells"], O.n_qp(p))) + rng.uniform(size=(sz["cells"], O.n_qp(p))),
+                         np.array([0, 0, -1.0]))
+    sol, out = h.solve(rhs, tolerance=1e-14, n_cheap_steps=2, n_expensive_steps=0, raise_on_failure=False)
+    assert not out.converged and out.cheap_iterations == 2
