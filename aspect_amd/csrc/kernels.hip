@@ -180,7 +180,7 @@ struct Cfg {
   static constexpr int VN = VX * VY * VZ, PN = PX * PY * PZ;
   static constexpr int NQ = ZQ * Q * Q, NPL = ZNP * NP * NP;
   static constexpr int QS = ZQ * Q;  // quadrature points of one y-slice
-  static constexpr int XBUF = 3 * QS * Q * CT;
+  static constexpr int XBUF = DIM * 3 * Q * Q * CT;
   static constexpr int PBUF = NPL * Q * CT;
   static constexpr int NBUF = DIM * VN + PN;
   static constexpr int SMEM = (XBUF > NBUF ? (XBUF > PBUF ? XBUF : PBUF) : (NBUF > PBUF ? NBUF : PBUF));
@@ -222,97 +222,6 @@ __device__ __forceinline__ bool is_con(const LevelDesc &L, int comp, int gx, int
   return false;
 }
 
-// x/z interpolation of one scalar node slice in[ZN][N] into
-// a (values), ax (x-derivative), az (z-derivative), all [ZQ][Q].
-template <int DIM, int K, typename T>
-__device__ __forceinline__ void eval_xz(const T (&in)[Cfg<DIM, K>::ZN][K + 1], T (&a)[Cfg<DIM, K>::ZQ][K + 1],
-                                        T (&ax)[Cfg<DIM, K>::ZQ][K + 1], T (&az)[Cfg<DIM, K>::ZQ][K + 1]) {
-  constexpr int N = K + 1, Q = K + 1, ZN = Cfg<DIM, K>::ZN, ZQ = Cfg<DIM, K>::ZQ;
-  T x0[ZN][Q], x1[ZN][Q];
-#pragma unroll
-  for (int kz = 0; kz < ZN; ++kz)
-#pragma unroll
-    for (int qx = 0; qx < Q; ++qx) {
-      T s0 = 0, s1 = 0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        s0 += (T)CB.S[qx][i] * in[kz][i];
-        s1 += (T)CB.G[qx][i] * in[kz][i];
-      }
-      x0[kz][qx] = s0;
-      x1[kz][qx] = s1;
-    }
-  if constexpr (DIM == 3) {
-#pragma unroll
-    for (int qz = 0; qz < ZQ; ++qz)
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) {
-        T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-        for (int kz = 0; kz < ZN; ++kz) {
-          s0 += (T)CB.S[qz][kz] * x0[kz][qx];
-          s1 += (T)CB.S[qz][kz] * x1[kz][qx];
-          s2 += (T)CB.G[qz][kz] * x0[kz][qx];
-        }
-        a[qz][qx] = s0;
-        ax[qz][qx] = s1;
-        az[qz][qx] = s2;
-      }
-  } else {
-#pragma unroll
-    for (int qx = 0; qx < Q; ++qx) {
-      a[0][qx] = x0[0][qx];
-      ax[0][qx] = x1[0][qx];
-      az[0][qx] = 0;
-    }
-  }
-}
-
-// transposed x/z step: out[kz][i] = sum_q [ MX1[qx][i] MZ1[qz][kz] t1 + MX2[qx][i] (MZ2[qz][kz] t2 + MZ3[qz][kz] t3) ]
-// with (MX1, MZ1) = (G, S), (MX2, MZ2, MZ3) = (S, S, G) for the gradient test functions.
-template <int DIM, int K, typename T, bool SQUARED>
-__device__ __forceinline__ void integ_xz(const T (&bx)[Cfg<DIM, K>::ZQ][K + 1], const T (&by)[Cfg<DIM, K>::ZQ][K + 1],
-                                         const T (&bz)[Cfg<DIM, K>::ZQ][K + 1], T (&out)[Cfg<DIM, K>::ZN][K + 1]) {
-  constexpr int N = K + 1, Q = K + 1, ZN = Cfg<DIM, K>::ZN, ZQ = Cfg<DIM, K>::ZQ;
-  T t1[ZN][Q], t2[ZN][Q];
-  if constexpr (DIM == 3) {
-#pragma unroll
-    for (int kz = 0; kz < ZN; ++kz)
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) {
-        T s1 = 0, s2 = 0;
-#pragma unroll
-        for (int qz = 0; qz < ZQ; ++qz) {
-          const T sv = SQUARED ? (T)CB.S2[qz][kz] : (T)CB.S[qz][kz];
-          const T gv = SQUARED ? (T)CB.G2[qz][kz] : (T)CB.G[qz][kz];
-          s1 += sv * bx[qz][qx];
-          s2 += sv * by[qz][qx] + gv * bz[qz][qx];
-        }
-        t1[kz][qx] = s1;
-        t2[kz][qx] = s2;
-      }
-  } else {
-#pragma unroll
-    for (int qx = 0; qx < Q; ++qx) {
-      t1[0][qx] = bx[0][qx];
-      t2[0][qx] = by[0][qx];
-    }
-  }
-#pragma unroll
-  for (int kz = 0; kz < ZN; ++kz)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      T s = 0;
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) {
-        const T sv = SQUARED ? (T)CB.S2[qx][i] : (T)CB.S[qx][i];
-        const T gv = SQUARED ? (T)CB.G2[qx][i] : (T)CB.G[qx][i];
-        s += gv * t1[kz][qx] + sv * t2[kz][qx];
-      }
-      out[kz][i] = s;
-    }
-}
-
 // ------------------------------------------------------------------ the cell kernel
 template <int DIM, int K, int OP, int MODE, typename T>
 __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
@@ -342,7 +251,9 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
   const int tz = 2 * b + pz;
   const long long tile_id = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
   const int tid = threadIdx.x;
-  const int c = tid % CT, j = tid / CT;
+  const int c = tid % CT;
+  // the y-slice index is uniform over a wave when a slice spans whole waves
+  const int j = (CT % 64 == 0) ? __builtin_amdgcn_readfirstlane(tid / CT) : tid / CT;
   const int cx = c % C::TX, cy = (c / C::TX) % C::TY, cz = c / (C::TX * C::TY);
   const bool valid = (tx * C::TX + cx < L.n[0]) && (ty * C::TY + cy < L.n[1]) &&
                      (DIM == 2 || tz * C::TZ + cz < L.n[2]);
@@ -397,52 +308,91 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
 
   const T ih[3] = {(T)(1.0 / L.h[0]), (T)(1.0 / L.h[1]), (T)(DIM == 3 ? 1.0 / L.h[2] : 1.0)};
   const double jac = L.h[0] * L.h[1] * (DIM == 3 ? L.h[2] : 1.0);
+  const T s = (T)L.s;
 
-  // ---- 2. velocity gradients at the quadrature points of slice qy=j
-  T grad[DIM][DIM][ZQ][Q];
-  if constexpr (NEED_U) {
+  // exchange slot of (component, array, qx) for y-slice jj of this cell
+  auto X = [&](int comp, int arr, int qx, int jj) -> T & {
+    return xb[(((comp * 3 + arr) * Q + qx) * Q + jj) * CT + c];
+  };
+
+  T out[DIM][ZN][N];      // integrals against the velocity test functions of node-y j
+  T pacc[ZNP][NP][NP];    // integrals against the pressure test functions (this slice's share)
 #pragma unroll
-    for (int comp = 0; comp < DIM; ++comp) {
-      T a[ZQ][Q], ax[ZQ][Q], az[ZQ][Q];
-      eval_xz<DIM, K, T>(u[comp], a, ax, az);
+  for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
-      for (int qz = 0; qz < ZQ; ++qz)
+    for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[comp][kz][i] = 0;
+#pragma unroll
+  for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+    for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+      for (int i = 0; i < NP; ++i) pacc[kz][jy][i] = 0;
+
+  // ---- 2-4. one z-plane of quadrature points at a time: evaluate, quadrature
+  // point operation, integrate and accumulate into the nodal results
+#pragma unroll 1
+  for (int qz = 0; qz < ZQ; ++qz) {
+    T grad[DIM][DIM][Q];
+    if constexpr (NEED_U) {
+#pragma unroll
+      for (int comp = 0; comp < DIM; ++comp) {
+        T zr[N], zd[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          if constexpr (DIM == 3) {
+            T s0 = 0, s1 = 0;
+#pragma unroll
+            for (int kz = 0; kz < N; ++kz) {
+              s0 += (T)CB.S[qz][kz] * u[comp][kz][i];
+              s1 += (T)CB.G[qz][kz] * u[comp][kz][i];
+            }
+            zr[i] = s0;
+            zd[i] = s1;
+          } else {
+            zr[i] = u[comp][0][i];
+            zd[i] = 0;
+          }
+        }
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) {
-          const int q = qz * Q + qx;
-          xb[((0 * QS + q) * Q + j) * CT + c] = ax[qz][qx];
-          xb[((1 * QS + q) * Q + j) * CT + c] = a[qz][qx];
-          if constexpr (DIM == 3) xb[((2 * QS + q) * Q + j) * CT + c] = az[qz][qx];
+          T a = 0, ax = 0, az = 0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            a += (T)CB.S[qx][i] * zr[i];
+            ax += (T)CB.G[qx][i] * zr[i];
+            if constexpr (DIM == 3) az += (T)CB.S[qx][i] * zd[i];
+          }
+          X(comp, 0, qx, j) = ax;
+          X(comp, 1, qx, j) = a;
+          if constexpr (DIM == 3) X(comp, 2, qx, j) = az;
         }
+      }
       __syncthreads();
 #pragma unroll
-      for (int qz = 0; qz < ZQ; ++qz)
+      for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) {
-          const int q = qz * Q + qx;
           T gx = 0, gy = 0, gz = 0;
 #pragma unroll
           for (int jj = 0; jj < N; ++jj) {
-            gx += (T)CB.S[j][jj] * xb[((0 * QS + q) * Q + jj) * CT + c];
-            gy += (T)CB.G[j][jj] * xb[((1 * QS + q) * Q + jj) * CT + c];
-            if constexpr (DIM == 3) gz += (T)CB.S[j][jj] * xb[((2 * QS + q) * Q + jj) * CT + c];
+            gx += (T)CB.S[j][jj] * X(comp, 0, qx, jj);
+            gy += (T)CB.G[j][jj] * X(comp, 1, qx, jj);
+            if constexpr (DIM == 3) gz += (T)CB.S[j][jj] * X(comp, 2, qx, jj);
           }
-          grad[comp][0][qz][qx] = gx * ih[0];
-          grad[comp][1][qz][qx] = gy * ih[1];
-          if constexpr (DIM == 3) grad[comp][2][qz][qx] = gz * ih[2];
+          grad[comp][0][qx] = gx * ih[0];
+          grad[comp][1][qx] = gy * ih[1];
+          if constexpr (DIM == 3) grad[comp][2][qx] = gz * ih[2];
         }
       __syncthreads();
     }
-  }
 
-  // pressure values at the slice's quadrature points (all in registers)
-  T pq[ZQ][Q];
-  if constexpr (NEED_P) {
-#pragma unroll
-    for (int qz = 0; qz < ZQ; ++qz)
+    T pq[Q];
+    if constexpr (NEED_P) {
 #pragma unroll
       for (int qx = 0; qx < Q; ++qx) {
-        T s = 0;
+        T acc = 0;
 #pragma unroll
         for (int kz = 0; kz < ZNP; ++kz)
 #pragma unroll
@@ -450,192 +400,212 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
               const T wz = DIM == 3 ? (T)CB.SP[qz][kz] : (T)1;
-              s += wz * (T)CB.SP[j][jy] * (T)CB.SP[qx][i] * p[kz][jy][i];
+              acc += wz * (T)CB.SP[j][jy] * (T)CB.SP[qx][i] * p[kz][jy][i];
             }
-        pq[qz][qx] = s;
+        pq[qx] = acc;
       }
-  }
+    }
 
-  // ---- 3. quadrature point operation
-  T flux[DIM][DIM][ZQ][Q];  // [comp][dir]: stress * JxW / h_dir
-  T fp[ZQ][Q];              // pressure test-function flux * JxW
-  const T s = (T)L.s;
-#pragma unroll
-  for (int qz = 0; qz < ZQ; ++qz)
+    // quadrature point operation
+    T flux[DIM][DIM][Q];  // [comp][dir]: stress * JxW / h_dir
+    T fp[Q];              // pressure test-function flux * JxW
 #pragma unroll
     for (int qx = 0; qx < Q; ++qx) {
       const int qn = qx + Q * (j + Q * qz);
       const double wq = CB.w[qx] * CB.w[j] * (DIM == 3 ? CB.w[qz] : 1.0);
       const T JxW = (T)(jac * wq);
       T et = 1;
-      if constexpr (OP != OP_BT) {
-        et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
-      }
+      if constexpr (OP != OP_BT) et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
       if constexpr (OP == OP_STOKES || OP == OP_ABLOCK) {
         T div = 0;
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) div += grad[a][a][qz][qx];
-        const T e2 = 2 * et;
+        for (int a = 0; a < DIM; ++a) div += grad[a][a][qx];
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
           for (int bb = 0; bb < DIM; ++bb) {
-            T sig = et * (grad[a][bb][qz][qx] + grad[bb][a][qz][qx]);
+            T sig = et * (grad[a][bb][qx] + grad[bb][a][qx]);
             if (a == bb) {
-              if (L.compressible) sig -= e2 / 3 * div;
-              if constexpr (OP == OP_STOKES) sig -= s * pq[qz][qx];
+              if (L.compressible) sig -= 2 * et / 3 * div;
+              if constexpr (OP == OP_STOKES) sig -= s * pq[qx];
             }
-            flux[a][bb][qz][qx] = sig * JxW * ih[bb];
+            flux[a][bb][qx] = sig * JxW * ih[bb];
           }
-        if constexpr (OP == OP_STOKES) fp[qz][qx] = -s * div * JxW;
+        if constexpr (OP == OP_STOKES) fp[qx] = -s * div * JxW;
       } else if constexpr (OP == OP_BT) {
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
-          for (int bb = 0; bb < DIM; ++bb) flux[a][bb][qz][qx] = a == bb ? -s * pq[qz][qx] * JxW * ih[bb] : (T)0;
+          for (int bb = 0; bb < DIM; ++bb) flux[a][bb][qx] = a == bb ? -s * pq[qx] * JxW * ih[bb] : (T)0;
       } else if constexpr (OP == OP_MASS) {
-        fp[qz][qx] = s * s / et * pq[qz][qx] * JxW;
+        fp[qx] = s * s / et * pq[qx] * JxW;
       } else if constexpr (OP == OP_MDIAG) {
-        fp[qz][qx] = s * s / et * JxW;
-      } else if constexpr (OP == OP_RHS) {
-        flux[0][0][qz][qx] = et * JxW;
-      } else if constexpr (OP == OP_ADIAG) {
-        // D_d = sum_q eta JxW (dN/dx_d)^2 : per direction squared-matrix integrals
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) flux[0][d][qz][qx] = et * JxW * ih[d] * ih[d];
+        fp[qx] = s * s / et * JxW;
+      } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
+        flux[0][0][qx] = et * JxW;  // rho JxW for the right-hand side, eta JxW for the diagonal
       }
     }
 
-  // ---- 4. integrate velocity test functions
-  T out[DIM][ZN][N];
-  if constexpr (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT) {
+    // integrate against the velocity test functions
+    if constexpr (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT) {
 #pragma unroll
-    for (int comp = 0; comp < DIM; ++comp) {
+      for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
-      for (int qz = 0; qz < ZQ; ++qz)
+        for (int d = 0; d < DIM; ++d)
 #pragma unroll
-        for (int qx = 0; qx < Q; ++qx) {
-          const int q = qz * Q + qx;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) xb[((d * QS + q) * Q + j) * CT + c] = flux[comp][d][qz][qx];
-        }
+          for (int qx = 0; qx < Q; ++qx) X(comp, d, qx, j) = flux[comp][d][qx];
       __syncthreads();
-      T bx[ZQ][Q], by[ZQ][Q], bz[ZQ][Q];
 #pragma unroll
-      for (int qz = 0; qz < ZQ; ++qz)
+      for (int comp = 0; comp < DIM; ++comp) {
+        T bx[Q], by[Q], bz[Q];
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) {
-          const int q = qz * Q + qx;
           T sx = 0, sy = 0, sz = 0;
 #pragma unroll
           for (int qy = 0; qy < Q; ++qy) {
-            sx += (T)CB.S[qy][j] * xb[((0 * QS + q) * Q + qy) * CT + c];
-            sy += (T)CB.G[qy][j] * xb[((1 * QS + q) * Q + qy) * CT + c];
-            if constexpr (DIM == 3) sz += (T)CB.S[qy][j] * xb[((2 * QS + q) * Q + qy) * CT + c];
+            sx += (T)CB.S[qy][j] * X(comp, 0, qx, qy);
+            sy += (T)CB.G[qy][j] * X(comp, 1, qx, qy);
+            if constexpr (DIM == 3) sz += (T)CB.S[qy][j] * X(comp, 2, qx, qy);
           }
-          bx[qz][qx] = sx;
-          by[qz][qx] = sy;
-          bz[qz][qx] = sz;
+          bx[qx] = sx;
+          by[qx] = sy;
+          bz[qx] = sz;
         }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          T t1 = 0, t3 = 0;
+#pragma unroll
+          for (int qx = 0; qx < Q; ++qx) {
+            t1 += (T)CB.G[qx][i] * bx[qx] + (T)CB.S[qx][i] * by[qx];
+            if constexpr (DIM == 3) t3 += (T)CB.S[qx][i] * bz[qx];
+          }
+          if constexpr (DIM == 3) {
+#pragma unroll
+            for (int kz = 0; kz < N; ++kz) out[comp][kz][i] += (T)CB.S[qz][kz] * t1 + (T)CB.G[qz][kz] * t3;
+          } else {
+            out[comp][0][i] += t1;
+          }
+        }
+      }
       __syncthreads();
-      integ_xz<DIM, K, T, false>(bx, by, bz, out[comp]);
-    }
-  } else if constexpr (OP == OP_RHS) {
-    // value test functions: S along every direction, times the gravity component
+    } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
 #pragma unroll
-    for (int qz = 0; qz < ZQ; ++qz)
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) xb[((qz * Q + qx) * Q + j) * CT + c] = flux[0][0][qz][qx];
-    __syncthreads();
-    T by[ZQ][Q], zero[ZQ][Q];
-#pragma unroll
-    for (int qz = 0; qz < ZQ; ++qz)
+      for (int qx = 0; qx < Q; ++qx) X(0, 0, qx, j) = flux[0][0][qx];
+      __syncthreads();
+      constexpr bool SQ = (OP == OP_ADIAG);
+      T bs[Q], bg[Q];
 #pragma unroll
       for (int qx = 0; qx < Q; ++qx) {
-        T sy = 0;
+        T s0 = 0, s1 = 0;
 #pragma unroll
-        for (int qy = 0; qy < Q; ++qy) sy += (T)CB.S[qy][j] * xb[((qz * Q + qx) * Q + qy) * CT + c];
-        by[qz][qx] = sy;
-        zero[qz][qx] = 0;
+        for (int qy = 0; qy < Q; ++qy) {
+          const T f = X(0, 0, qx, qy);
+          s0 += (SQ ? (T)CB.S2[qy][j] : (T)CB.S[qy][j]) * f;
+          if constexpr (SQ) s1 += (T)CB.G2[qy][j] * f;
+        }
+        bs[qx] = s0;
+        bg[qx] = s1;
       }
-    __syncthreads();
-    T val[ZN][N];
-    integ_xz<DIM, K, T, false>(zero, by, zero, val);
+      __syncthreads();
+      if constexpr (OP == OP_RHS) {
+        // out[0] accumulates the value integral; expanded to components after the loop
 #pragma unroll
-    for (int comp = 0; comp < DIM; ++comp)
+        for (int i = 0; i < N; ++i) {
+          T row = 0;
+#pragma unroll
+          for (int qx = 0; qx < Q; ++qx) row += (T)CB.S[qx][i] * bs[qx];
+          if constexpr (DIM == 3) {
+#pragma unroll
+            for (int kz = 0; kz < N; ++kz) out[0][kz][i] += (T)CB.S[qz][kz] * row;
+          } else {
+            out[0][0][i] += row;
+          }
+        }
+      } else {
+        // out[d] accumulates D_d = sum_q eta JxW (dN/dx_d)^2
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          T rx = 0, ry = 0, rz = 0;
+#pragma unroll
+          for (int qx = 0; qx < Q; ++qx) {
+            rx += (T)CB.G2[qx][i] * bs[qx];
+            ry += (T)CB.S2[qx][i] * bg[qx];
+            rz += (T)CB.S2[qx][i] * bs[qx];
+          }
+          rx *= ih[0] * ih[0];
+          ry *= ih[1] * ih[1];
+          rz *= ih[2] * ih[2];
+          if constexpr (DIM == 3) {
+#pragma unroll
+            for (int kz = 0; kz < N; ++kz) {
+              out[0][kz][i] += (T)CB.S2[qz][kz] * rx;
+              out[1][kz][i] += (T)CB.S2[qz][kz] * ry;
+              out[2][kz][i] += (T)CB.G2[qz][kz] * rz;
+            }
+          } else {
+            out[0][0][i] += rx;
+            out[1][0][i] += ry;
+          }
+        }
+      }
+    }
+
+    // this slice's share of the pressure test-function integrals
+    if constexpr (OUT_P) {
+      constexpr bool SQ = (OP == OP_MDIAG);
+#pragma unroll
+      for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+        for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+          for (int i = 0; i < NP; ++i) {
+            T acc = 0;
+#pragma unroll
+            for (int qx = 0; qx < Q; ++qx) {
+              const T wx = SQ ? (T)CB.SP2[qx][i] : (T)CB.SP[qx][i];
+              acc += wx * fp[qx];
+            }
+            const T wz = DIM == 3 ? (SQ ? (T)CB.SP2[qz][kz] : (T)CB.SP[qz][kz]) : (T)1;
+            const T wy = SQ ? (T)CB.SP2[j][jy] : (T)CB.SP[j][jy];
+            pacc[kz][jy][i] += wz * wy * acc;
+          }
+    }
+  }
+
+  if constexpr (OP == OP_RHS) {
+#pragma unroll
+    for (int comp = DIM - 1; comp >= 0; --comp)
 #pragma unroll
       for (int kz = 0; kz < ZN; ++kz)
 #pragma unroll
-        for (int i = 0; i < N; ++i) out[comp][kz][i] = (T)L.gravity[comp] * val[kz][i];
+        for (int i = 0; i < N; ++i) out[comp][kz][i] = (T)L.gravity[comp] * out[0][kz][i];
   } else if constexpr (OP == OP_ADIAG) {
-    // exchange eta JxW / h_d^2 per direction and integrate with squared matrices
-#pragma unroll
-    for (int qz = 0; qz < ZQ; ++qz)
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) {
-        const int q = qz * Q + qx;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) xb[((d * QS + q) * Q + j) * CT + c] = flux[0][d][qz][qx];
-      }
-    __syncthreads();
-    // D_x: G2 x S2 y S2 z ; D_y: S2 x G2 y S2 z ; D_z: S2 x S2 y G2 z
-    T bxs[ZQ][Q], bys[ZQ][Q], bzs[ZQ][Q], zero[ZQ][Q];
-#pragma unroll
-    for (int qz = 0; qz < ZQ; ++qz)
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) {
-        const int q = qz * Q + qx;
-        T sx = 0, sy = 0, sz = 0;
-#pragma unroll
-        for (int qy = 0; qy < Q; ++qy) {
-          sx += (T)CB.S2[qy][j] * xb[((0 * QS + q) * Q + qy) * CT + c];
-          sy += (T)CB.G2[qy][j] * xb[((1 * QS + q) * Q + qy) * CT + c];
-          if constexpr (DIM == 3) sz += (T)CB.S2[qy][j] * xb[((2 * QS + q) * Q + qy) * CT + c];
-        }
-        bxs[qz][qx] = sx;
-        bys[qz][qx] = sy;
-        bzs[qz][qx] = sz;
-        zero[qz][qx] = 0;
-      }
-    __syncthreads();
-    T Dx[ZN][N], Dy[ZN][N], Dz[ZN][N];
-    integ_xz<DIM, K, T, true>(bxs, zero, zero, Dx);   // G2 along x, S2 along z
-    integ_xz<DIM, K, T, true>(zero, bys, zero, Dy);   // S2 along x, S2 along z
-    if constexpr (DIM == 3) integ_xz<DIM, K, T, true>(zero, zero, bzs, Dz);  // S2 along x, G2 along z
+    // 2 eta eps(phi):eps(phi) = eta (|grad N|^2 + (dN/dx_c)^2)  [- 2/3 eta (dN/dx_c)^2 compressible]
     const T fc = L.compressible ? (T)(1.0 / 3.0) : (T)1;
 #pragma unroll
     for (int kz = 0; kz < ZN; ++kz)
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const T sum = Dx[kz][i] + Dy[kz][i] + (DIM == 3 ? Dz[kz][i] : (T)0);
-        out[0][kz][i] = sum + fc * Dx[kz][i];
-        out[1][kz][i] = sum + fc * Dy[kz][i];
-        if constexpr (DIM == 3) out[2][kz][i] = sum + fc * Dz[kz][i];
+        T D[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) D[d] = out[d][kz][i];
+        T sum = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sum += D[d];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) out[d][kz][i] = sum + fc * D[d];
       }
   }
 
-  // ---- 5. integrate pressure test functions (partial over this slice, reduced over slices)
+  // pressure: reduce the slices' shares (thread j == 0 holds the cell's result)
   T pout[ZNP][NP][NP];
   if constexpr (OUT_P) {
-    constexpr bool SQ = (OP == OP_MDIAG);
 #pragma unroll
     for (int kz = 0; kz < ZNP; ++kz)
 #pragma unroll
       for (int jy = 0; jy < NP; ++jy)
 #pragma unroll
-        for (int i = 0; i < NP; ++i) {
-          T acc = 0;
-#pragma unroll
-          for (int qz = 0; qz < ZQ; ++qz)
-#pragma unroll
-            for (int qx = 0; qx < Q; ++qx) {
-              const T wz = DIM == 3 ? (SQ ? (T)CB.SP2[qz][kz] : (T)CB.SP[qz][kz]) : (T)1;
-              const T wy = SQ ? (T)CB.SP2[j][jy] : (T)CB.SP[j][jy];
-              const T wx = SQ ? (T)CB.SP2[qx][i] : (T)CB.SP[qx][i];
-              acc += wz * wy * wx * fp[qz][qx];
-            }
-          xb[(((kz * NP + jy) * NP + i) * Q + j) * CT + c] = acc;
-        }
+        for (int i = 0; i < NP; ++i) xb[(((kz * NP + jy) * NP + i) * Q + j) * CT + c] = pacc[kz][jy][i];
     __syncthreads();
     if (j == 0) {
 #pragma unroll

@@ -314,26 +314,21 @@ void tensor_apply(int dim, int m, int n, const double *const *M, const double *i
   }
 }
 
-// transpose version: out (n per dim) += sum_b prod_d M_d[b_d][a_d] in[b] (m per dim)
-void tensor_apply_T_add(int dim, int m, int n, const double *const *M, const double *in, double *out,
-                        double *work) {
-  std::vector<double> Mt[3];
-  const double *MT[3];
-  for (int d = 0; d < dim; ++d) {
-    Mt[d].resize(n * m);
-    for (int b = 0; b < m; ++b)
-      for (int a = 0; a < n; ++a) Mt[d][a * m + b] = M[d][b * n + a];
-    MT[d] = Mt[d].data();
-  }
-  int sz = dim == 2 ? n * n : n * n * n;
-  std::vector<double> tmp(sz);
-  tensor_apply(dim, n, m, MT, in, tmp.data(), work);
-  for (int i = 0; i < sz; ++i) out[i] += tmp[i];
-}
-
 struct CellKernels {
   RefElement re;
-  explicit CellKernels(int dim, int k) : re(dim, k) {}
+  std::vector<double> St, Gt, SPt;  // transposes (N x Q, NP x Q)
+  explicit CellKernels(int dim, int k) : re(dim, k) {
+    St.resize(re.N * re.Q);
+    Gt.resize(re.N * re.Q);
+    SPt.resize(re.NP * re.Q);
+    for (int q = 0; q < re.Q; ++q) {
+      for (int i = 0; i < re.N; ++i) {
+        St[i * re.Q + q] = re.S[q * re.N + i];
+        Gt[i * re.Q + q] = re.G[q * re.N + i];
+      }
+      for (int i = 0; i < re.NP; ++i) SPt[i * re.Q + q] = re.SP[q * re.NP + i];
+    }
+  }
 
   // gradient of a scalar nodal field at all qps: grad[d*nq + q]
   void grad(const double *u, const double h[3], double *grad, double *work) const {
@@ -347,13 +342,16 @@ struct CellKernels {
   }
   // out[a] += sum_q sum_d f[d][q] dphi_a/dx_d(q)  (f already multiplied by JxW)
   void grad_T_add(const double *f, const double h[3], double *out, double *work) const {
-    const int dim = re.dim, nq = re.nq();
-    std::vector<double> tmp(nq);
+    const int dim = re.dim, nq = re.nq(), nv = re.nvloc();
+    double *tmp = work;            // nq
+    double *res = work + nq;       // nv
+    double *w2 = work + nq + nv;   // tensor_apply scratch
     for (int d = 0; d < dim; ++d) {
       const double *M[3];
-      for (int e = 0; e < dim; ++e) M[e] = (e == d) ? re.G.data() : re.S.data();
+      for (int e = 0; e < dim; ++e) M[e] = (e == d) ? Gt.data() : St.data();
       for (int q = 0; q < nq; ++q) tmp[q] = f[d * nq + q] / h[d];
-      tensor_apply_T_add(dim, re.Q, re.N, M, tmp.data(), out, work);
+      tensor_apply(dim, re.N, re.Q, M, tmp, res, w2);
+      for (int a = 0; a < nv; ++a) out[a] += res[a];
     }
   }
   void pvalues(const double *p, double *out, double *work) const {
@@ -361,8 +359,11 @@ struct CellKernels {
     tensor_apply(re.dim, re.Q, re.NP, M, p, out, work);
   }
   void pvalues_T_add(const double *f, double *out, double *work) const {
-    const double *M[3] = {re.SP.data(), re.SP.data(), re.SP.data()};
-    tensor_apply_T_add(re.dim, re.Q, re.NP, M, f, out, work);
+    const double *M[3] = {SPt.data(), SPt.data(), SPt.data()};
+    const int npl = re.nploc();
+    double *res = work;
+    tensor_apply(re.dim, re.NP, re.Q, M, f, res, work + npl);
+    for (int a = 0; a < npl; ++a) out[a] += res[a];
   }
 };
 
