@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "mfs_internal.h"
@@ -33,11 +34,27 @@ struct BasisConst {
   double G[4][4];   // velocity basis derivative on [0,1]
   double SP[4][4];  // pressure basis value
   double S2[4][4], G2[4][4], SP2[4][4];  // squares (diagonal kernels)
+  double D[4][4];   // collocation derivative on the Gauss points: D[q][r] = l_r'(x_q)
   double w[4];      // Gauss weights on [0,1]
   double W[7][4];   // prolongation weights, velocity degree k:  [r][m], r < 2k+1
   double WP[5][3];  // prolongation weights, pressure degree k-1
 };
 __constant__ BasisConst c_basis[2];  // index k-2
+
+// the cell kernel's tables in its own arithmetic type (no conversions at use)
+template <typename T>
+struct BasisT {
+  T S[4][4], G[4][4], D[4][4], SP[4][4], S2[4][4], G2[4][4], SP2[4][4], w[4];
+};
+__constant__ BasisT<double> c_tabd[2];
+__constant__ BasisT<float> c_tabf[2];
+template <typename T, int K>
+__device__ __forceinline__ const BasisT<T> &tb() {
+  if constexpr (std::is_same<T, double>::value)
+    return c_tabd[K - 2];
+  else
+    return c_tabf[K - 2];
+}
 
 namespace {
 double lag(const std::vector<double> &pts, int i, double x) {
@@ -141,10 +158,30 @@ void upload_basis_tables() {
         B.SP2[q][i] = B.SP[q][i] * B.SP[q][i];
       }
     }
+    for (int q = 0; q <= k; ++q)
+      for (int r = 0; r <= k; ++r) B.D[q][r] = lagd(qx, r, qx[q]);
     prolong_table(k, &B.W[0][0], 4);
     prolong_table(k - 1, &B.WP[0][0], 3);
   }
   MFS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_basis), b, sizeof(b)));
+  BasisT<double> td[2];
+  BasisT<float> tf[2];
+  for (int k = 0; k < 2; ++k) {
+    for (int q = 0; q < 4; ++q) {
+      td[k].w[q] = b[k].w[q];
+      tf[k].w[q] = (float)b[k].w[q];
+      for (int i = 0; i < 4; ++i) {
+        td[k].S[q][i] = b[k].S[q][i], td[k].G[q][i] = b[k].G[q][i], td[k].D[q][i] = b[k].D[q][i];
+        td[k].SP[q][i] = b[k].SP[q][i], td[k].S2[q][i] = b[k].S2[q][i], td[k].G2[q][i] = b[k].G2[q][i];
+        td[k].SP2[q][i] = b[k].SP2[q][i];
+        tf[k].S[q][i] = (float)b[k].S[q][i], tf[k].G[q][i] = (float)b[k].G[q][i], tf[k].D[q][i] = (float)b[k].D[q][i];
+        tf[k].SP[q][i] = (float)b[k].SP[q][i], tf[k].S2[q][i] = (float)b[k].S2[q][i];
+        tf[k].G2[q][i] = (float)b[k].G2[q][i], tf[k].SP2[q][i] = (float)b[k].SP2[q][i];
+      }
+    }
+  }
+  MFS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tabd), td, sizeof(td)));
+  MFS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tabf), tf, sizeof(tf)));
   done = true;
 }
 
@@ -180,8 +217,8 @@ struct Cfg {
   static constexpr int VN = VX * VY * VZ, PN = PX * PY * PZ;
   static constexpr int NQ = ZQ * Q * Q, NPL = ZNP * NP * NP;
   static constexpr int QS = ZQ * Q;  // quadrature points of one y-slice
-  static constexpr int XBUF = DIM * 3 * Q * Q * CT;
-  static constexpr int PBUF = NPL * Q * CT;
+  static constexpr int XBUF = DIM * ZQ * Q * Q * CT;
+  static constexpr int PBUF = DIM * ZN * N * N * CT + NPL * Q * CT;
   static constexpr int NBUF = DIM * VN + PN;
   static constexpr int SMEM = (XBUF > NBUF ? (XBUF > PBUF ? XBUF : PBUF) : (NBUF > PBUF ? NBUF : PBUF));
 };
@@ -205,7 +242,7 @@ int tile_threads(int dim, int k) {
   return tile_cells(dim, k, t) * (k + 1);
 }
 
-#define CB c_basis[K - 2]
+#define CB tb<T, K>()
 
 __device__ __forceinline__ bool is_con(const LevelDesc &L, int comp, int gx, int gy, int gz) {
   const int g[3] = {gx, gy, gz};
@@ -223,23 +260,37 @@ __device__ __forceinline__ bool is_con(const LevelDesc &L, int comp, int gx, int
 }
 
 // ------------------------------------------------------------------ the cell kernel
+// Collocation form of the sum factorization: nodal values are interpolated to
+// quadrature-point values with S (Q x N) along every direction, and
+// derivatives are taken on the Gauss points with D (Q x Q), D S = G.  Only the y
+// direction crosses threads (LDS); x and z stay in registers.
+#ifndef MFS_MINW_F32
+#define MFS_MINW_F32 1
+#endif
+#ifndef MFS_MINW_F64
+#define MFS_MINW_F64 1
+#endif
+template <typename T>
+struct MinWaves {
+  static constexpr int value = std::is_same<T, double>::value ? MFS_MINW_F64 : MFS_MINW_F32;
+};
 template <int DIM, int K, int OP, int MODE, typename T>
-__global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
+__global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
     k_cell(const LevelDesc L, const int color, const T *__restrict__ src_u, const T *__restrict__ src_p,
            const T *__restrict__ eta, const int eta_per_qp, const T *__restrict__ rhs_u,
            const T *__restrict__ rhs_p, T *__restrict__ dst_u, T *__restrict__ dst_p) {
   using C = Cfg<DIM, K>;
   constexpr int N = C::N, Q = C::Q, NP = C::NP, ZN = C::ZN, ZQ = C::ZQ, ZNP = C::ZNP, CT = C::CT;
-  constexpr int QS = C::QS;
   constexpr bool NEED_U = (OP == OP_STOKES || OP == OP_ABLOCK);
   constexpr bool NEED_P = (OP == OP_STOKES || OP == OP_BT || OP == OP_MASS);
   constexpr bool OUT_U = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT || OP == OP_ADIAG || OP == OP_RHS);
   constexpr bool OUT_P = (OP == OP_STOKES || OP == OP_MASS || OP == OP_MDIAG);
+  constexpr bool GRAD_TEST = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
-  T *nb = sm;                 // [DIM][VZ][VY][VX] tile velocity nodes
-  T *pb = sm + DIM * C::VN;   // [PZ][PY][PX] tile pressure nodes
-  T *xb = sm;                 // exchange buffer (aliases nb/pb)
+  T *nb = sm;                // [DIM][VZ][VY][VX] tile velocity nodes (load phase)
+  T *pb = sm + DIM * C::VN;  // [PZ][PY][PX] tile pressure nodes (load phase)
+  T *xb = sm;                // exchange buffer, then per-cell contributions
 
   const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
   const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
@@ -255,10 +306,8 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
   // the y-slice index is uniform over a wave when a slice spans whole waves
   const int j = (CT % 64 == 0) ? __builtin_amdgcn_readfirstlane(tid / CT) : tid / CT;
   const int cx = c % C::TX, cy = (c / C::TX) % C::TY, cz = c / (C::TX * C::TY);
-  const bool valid = (tx * C::TX + cx < L.n[0]) && (ty * C::TY + cy < L.n[1]) &&
-                     (DIM == 2 || tz * C::TZ + cz < L.n[2]);
   const int vx0 = K * C::TX * tx, vy0 = K * C::TY * ty, vz0 = DIM == 3 ? K * C::TZ * tz : 0;
-  const int qx0 = (K - 1) * C::TX * tx, qy0 = (K - 1) * C::TY * ty, qz0 = DIM == 3 ? (K - 1) * C::TZ * tz : 0;
+  const int ox0 = (K - 1) * C::TX * tx, oy0 = (K - 1) * C::TY * ty, oz0 = DIM == 3 ? (K - 1) * C::TZ * tz : 0;
 
   // ---- 1. cooperative tile load (constrained velocity DoFs read as zero)
   if constexpr (NEED_U) {
@@ -275,7 +324,7 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
   if constexpr (NEED_P) {
     for (int t = tid; t < C::PN; t += C::THREADS) {
       const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
-      const int gx = qx0 + lx, gy = qy0 + ly, gz = qz0 + lz;
+      const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
       T v = 0;
       if (gx < L.np[0] && gy < L.np[1] && gz < L.np[2])
         v = src_p[gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz)];
@@ -309,20 +358,71 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
   const T ih[3] = {(T)(1.0 / L.h[0]), (T)(1.0 / L.h[1]), (T)(DIM == 3 ? 1.0 / L.h[2] : 1.0)};
   const double jac = L.h[0] * L.h[1] * (DIM == 3 ? L.h[2] : 1.0);
   const T s = (T)L.s;
-
-  // exchange slot of (component, array, qx) for y-slice jj of this cell
-  auto X = [&](int comp, int arr, int qx, int jj) -> T & {
-    return xb[(((comp * 3 + arr) * Q + qx) * Q + jj) * CT + c];
+  // exchange slot of (component, qz, qx) for y-slice jj of this cell
+  auto X = [&](int comp, int qz, int qx, int jj) -> T & {
+    return xb[(((comp * ZQ + qz) * Q + qx) * Q + jj) * CT + c];
   };
 
-  T out[DIM][ZN][N];      // integrals against the velocity test functions of node-y j
-  T pacc[ZNP][NP][NP];    // integrals against the pressure test functions (this slice's share)
+  // ---- 2. quadrature-point values of u on slice qy = j
+  T v[DIM][ZQ][Q];
+  if constexpr (NEED_U) {
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp) {
+      T vx[ZN][Q];
+#pragma unroll
+      for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          T a = 0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) a += CB.S[qx][i] * u[comp][kz][i];
+          vx[kz][qx] = a;
+        }
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          T a = 0;
+          if constexpr (DIM == 3) {
+#pragma unroll
+            for (int kz = 0; kz < N; ++kz) a += CB.S[qz][kz] * vx[kz][qx];
+          } else {
+            a = vx[0][qx];
+          }
+          X(comp, qz, qx, j) = a;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          T a = 0;
+#pragma unroll
+          for (int jj = 0; jj < N; ++jj) a += CB.S[j][jj] * X(comp, qz, qx, jj);
+          v[comp][qz][qx] = a;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) X(comp, qz, qx, j) = v[comp][qz][qx];
+    __syncthreads();
+  }
+
+  // ---- 3. plane by plane: gradients, quadrature-point operation, test-function sums
+  T g[DIM][ZQ][Q];     // velocity test-function sums in quadrature-point space (gradient tests)
+  T pacc[ZNP][NP][NP]; // pressure test-function sums of this slice
 #pragma unroll
   for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
-    for (int kz = 0; kz < ZN; ++kz)
+    for (int qz = 0; qz < ZQ; ++qz)
 #pragma unroll
-      for (int i = 0; i < N; ++i) out[comp][kz][i] = 0;
+      for (int qx = 0; qx < Q; ++qx) g[comp][qz][qx] = 0;
 #pragma unroll
   for (int kz = 0; kz < ZNP; ++kz)
 #pragma unroll
@@ -330,379 +430,368 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS))
 #pragma unroll
       for (int i = 0; i < NP; ++i) pacc[kz][jy][i] = 0;
 
-  // ---- 2-4. one z-plane of quadrature points at a time: evaluate, quadrature
-  // point operation, integrate and accumulate into the nodal results
-#pragma unroll 1
+#pragma unroll
   for (int qz = 0; qz < ZQ; ++qz) {
-    T grad[DIM][DIM][Q];
-    if constexpr (NEED_U) {
+    T fy[DIM][Q];  // y-flux of this plane, stashed into LDS after the plane's u reads
+    T fz_or_val[Q];
 #pragma unroll
-      for (int comp = 0; comp < DIM; ++comp) {
-        T zr[N], zd[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          if constexpr (DIM == 3) {
-            T s0 = 0, s1 = 0;
-#pragma unroll
-            for (int kz = 0; kz < N; ++kz) {
-              s0 += (T)CB.S[qz][kz] * u[comp][kz][i];
-              s1 += (T)CB.G[qz][kz] * u[comp][kz][i];
-            }
-            zr[i] = s0;
-            zd[i] = s1;
-          } else {
-            zr[i] = u[comp][0][i];
-            zd[i] = 0;
-          }
-        }
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) {
-          T a = 0, ax = 0, az = 0;
-#pragma unroll
-          for (int i = 0; i < N; ++i) {
-            a += (T)CB.S[qx][i] * zr[i];
-            ax += (T)CB.G[qx][i] * zr[i];
-            if constexpr (DIM == 3) az += (T)CB.S[qx][i] * zd[i];
-          }
-          X(comp, 0, qx, j) = ax;
-          X(comp, 1, qx, j) = a;
-          if constexpr (DIM == 3) X(comp, 2, qx, j) = az;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int comp = 0; comp < DIM; ++comp)
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) {
-          T gx = 0, gy = 0, gz = 0;
-#pragma unroll
-          for (int jj = 0; jj < N; ++jj) {
-            gx += (T)CB.S[j][jj] * X(comp, 0, qx, jj);
-            gy += (T)CB.G[j][jj] * X(comp, 1, qx, jj);
-            if constexpr (DIM == 3) gz += (T)CB.S[j][jj] * X(comp, 2, qx, jj);
-          }
-          grad[comp][0][qx] = gx * ih[0];
-          grad[comp][1][qx] = gy * ih[1];
-          if constexpr (DIM == 3) grad[comp][2][qx] = gz * ih[2];
-        }
-      __syncthreads();
-    }
-
-    T pq[Q];
-    if constexpr (NEED_P) {
-#pragma unroll
-      for (int qx = 0; qx < Q; ++qx) {
-        T acc = 0;
+    for (int qx = 0; qx < Q; ++qx) {
+      const int qn = qx + Q * (j + Q * qz);
+      const double wq = c_basis[K - 2].w[qx] * c_basis[K - 2].w[j] * (DIM == 3 ? c_basis[K - 2].w[qz] : 1.0);
+      const T JxW = (T)(jac * wq);
+      T et = 1;
+      if constexpr (OP != OP_BT) et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
+      T pqv = 0;
+      if constexpr (NEED_P) {
 #pragma unroll
         for (int kz = 0; kz < ZNP; ++kz)
 #pragma unroll
           for (int jy = 0; jy < NP; ++jy)
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
-              const T wz = DIM == 3 ? (T)CB.SP[qz][kz] : (T)1;
-              acc += wz * (T)CB.SP[j][jy] * (T)CB.SP[qx][i] * p[kz][jy][i];
+              const T wz = DIM == 3 ? CB.SP[qz][kz] : (T)1;
+              pqv += wz * CB.SP[j][jy] * CB.SP[qx][i] * p[kz][jy][i];
             }
-        pq[qx] = acc;
       }
-    }
-
-    // quadrature point operation
-    T flux[DIM][DIM][Q];  // [comp][dir]: stress * JxW / h_dir
-    T fp[Q];              // pressure test-function flux * JxW
+      T gr[DIM][DIM];
+      if constexpr (NEED_U) {
 #pragma unroll
-    for (int qx = 0; qx < Q; ++qx) {
-      const int qn = qx + Q * (j + Q * qz);
-      const double wq = CB.w[qx] * CB.w[j] * (DIM == 3 ? CB.w[qz] : 1.0);
-      const T JxW = (T)(jac * wq);
-      T et = 1;
-      if constexpr (OP != OP_BT) et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
-      if constexpr (OP == OP_STOKES || OP == OP_ABLOCK) {
-        T div = 0;
+        for (int comp = 0; comp < DIM; ++comp) {
+          T dx = 0, dy = 0, dz = 0;
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) div += grad[a][a][qx];
-#pragma unroll
-        for (int a = 0; a < DIM; ++a)
-#pragma unroll
-          for (int bb = 0; bb < DIM; ++bb) {
-            T sig = et * (grad[a][bb][qx] + grad[bb][a][qx]);
-            if (a == bb) {
-              if (L.compressible) sig -= 2 * et / 3 * div;
-              if constexpr (OP == OP_STOKES) sig -= s * pq[qx];
-            }
-            flux[a][bb][qx] = sig * JxW * ih[bb];
+          for (int r = 0; r < Q; ++r) {
+            dx += CB.D[qx][r] * v[comp][qz][r];
+            dy += CB.D[j][r] * X(comp, qz, qx, r);
+            if constexpr (DIM == 3) dz += CB.D[qz][r] * v[comp][r][qx];
           }
-        if constexpr (OP == OP_STOKES) fp[qx] = -s * div * JxW;
-      } else if constexpr (OP == OP_BT) {
+          gr[comp][0] = dx * ih[0];
+          gr[comp][1] = dy * ih[1];
+          if constexpr (DIM == 3) gr[comp][2] = dz * ih[2];
+        }
+      }
+      T fp = 0;
+      if constexpr (GRAD_TEST) {
+        T flux[DIM][DIM];
+        if constexpr (OP == OP_STOKES || OP == OP_ABLOCK) {
+          T div = 0;
 #pragma unroll
-        for (int a = 0; a < DIM; ++a)
+          for (int a = 0; a < DIM; ++a) div += gr[a][a];
 #pragma unroll
-          for (int bb = 0; bb < DIM; ++bb) flux[a][bb][qx] = a == bb ? -s * pq[qx] * JxW * ih[bb] : (T)0;
+          for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int bb = 0; bb < DIM; ++bb) {
+              T sig = et * (gr[a][bb] + gr[bb][a]);
+              if (a == bb) {
+                if (L.compressible) sig -= 2 * et / 3 * div;
+                if constexpr (OP == OP_STOKES) sig -= s * pqv;
+              }
+              flux[a][bb] = sig * JxW;
+            }
+          if constexpr (OP == OP_STOKES) fp = -s * div * JxW;
+        } else {  // B^T: -s p I
+#pragma unroll
+          for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int bb = 0; bb < DIM; ++bb) flux[a][bb] = a == bb ? -s * pqv * JxW : (T)0;
+        }
+        // x and z tests accumulate in registers, y test is stashed
+#pragma unroll
+        for (int comp = 0; comp < DIM; ++comp) {
+#pragma unroll
+          for (int r = 0; r < Q; ++r) {
+            g[comp][qz][r] += CB.D[qx][r] * flux[comp][0] * ih[0];
+            if constexpr (DIM == 3) g[comp][r][qx] += CB.D[qz][r] * flux[comp][2] * ih[2];
+          }
+          fy[comp][qx] = flux[comp][1] * ih[1];
+        }
       } else if constexpr (OP == OP_MASS) {
-        fp[qx] = s * s / et * pq[qx] * JxW;
+        fp = s * s / et * pqv * JxW;
       } else if constexpr (OP == OP_MDIAG) {
-        fp[qx] = s * s / et * JxW;
+        fp = s * s / et * JxW;
       } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
-        flux[0][0][qx] = et * JxW;  // rho JxW for the right-hand side, eta JxW for the diagonal
+        fz_or_val[qx] = et * JxW;  // rho JxW for the right-hand side, eta JxW for the diagonal
+      }
+      if constexpr (OUT_P) {
+        constexpr bool SQ = (OP == OP_MDIAG);
+#pragma unroll
+        for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+          for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+              const T wz = DIM == 3 ? (SQ ? CB.SP2[qz][kz] : CB.SP[qz][kz]) : (T)1;
+              const T wy = SQ ? CB.SP2[j][jy] : CB.SP[j][jy];
+              const T wx = SQ ? CB.SP2[qx][i] : CB.SP[qx][i];
+              pacc[kz][jy][i] += wz * wy * wx * fp;
+            }
       }
     }
-
-    // integrate against the velocity test functions
-    if constexpr (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT) {
+    if constexpr (GRAD_TEST) {
+      if constexpr (NEED_U) __syncthreads();  // every slice has read this plane's u values
 #pragma unroll
       for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d)
-#pragma unroll
-          for (int qx = 0; qx < Q; ++qx) X(comp, d, qx, j) = flux[comp][d][qx];
-      __syncthreads();
-#pragma unroll
-      for (int comp = 0; comp < DIM; ++comp) {
-        T bx[Q], by[Q], bz[Q];
-#pragma unroll
-        for (int qx = 0; qx < Q; ++qx) {
-          T sx = 0, sy = 0, sz = 0;
-#pragma unroll
-          for (int qy = 0; qy < Q; ++qy) {
-            sx += (T)CB.S[qy][j] * X(comp, 0, qx, qy);
-            sy += (T)CB.G[qy][j] * X(comp, 1, qx, qy);
-            if constexpr (DIM == 3) sz += (T)CB.S[qy][j] * X(comp, 2, qx, qy);
-          }
-          bx[qx] = sx;
-          by[qx] = sy;
-          bz[qx] = sz;
-        }
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          T t1 = 0, t3 = 0;
-#pragma unroll
-          for (int qx = 0; qx < Q; ++qx) {
-            t1 += (T)CB.G[qx][i] * bx[qx] + (T)CB.S[qx][i] * by[qx];
-            if constexpr (DIM == 3) t3 += (T)CB.S[qx][i] * bz[qx];
-          }
-          if constexpr (DIM == 3) {
-#pragma unroll
-            for (int kz = 0; kz < N; ++kz) out[comp][kz][i] += (T)CB.S[qz][kz] * t1 + (T)CB.G[qz][kz] * t3;
-          } else {
-            out[comp][0][i] += t1;
-          }
-        }
-      }
-      __syncthreads();
+        for (int qx = 0; qx < Q; ++qx) X(comp, qz, qx, j) = fy[comp][qx];
     } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
 #pragma unroll
-      for (int qx = 0; qx < Q; ++qx) X(0, 0, qx, j) = flux[0][0][qx];
-      __syncthreads();
-      constexpr bool SQ = (OP == OP_ADIAG);
-      T bs[Q], bg[Q];
+      for (int qx = 0; qx < Q; ++qx) X(0, qz, qx, j) = fz_or_val[qx];
+    }
+  }
+
+  // ---- 4. y tests across slices, back to nodal values
+  T out[DIM][ZN][N];
+  if constexpr (GRAD_TEST) {
+    __syncthreads();
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          T a = 0;
+#pragma unroll
+          for (int qy = 0; qy < Q; ++qy) a += CB.D[qy][j] * X(comp, qz, qx, qy);
+          g[comp][qz][qx] += a;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) X(comp, qz, qx, j) = g[comp][qz][qx];
+    __syncthreads();
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp) {
+      T h[ZQ][Q];
+#pragma unroll
+      for (int qz = 0; qz < ZQ; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < Q; ++qx) {
+          T a = 0;
+#pragma unroll
+          for (int qy = 0; qy < Q; ++qy) a += CB.S[qy][j] * X(comp, qz, qx, qy);
+          h[qz][qx] = a;
+        }
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T hx[ZQ];
+#pragma unroll
+        for (int qz = 0; qz < ZQ; ++qz) {
+          T a = 0;
+#pragma unroll
+          for (int qx = 0; qx < Q; ++qx) a += CB.S[qx][i] * h[qz][qx];
+          hx[qz] = a;
+        }
+#pragma unroll
+        for (int kz = 0; kz < ZN; ++kz) {
+          T a = 0;
+          if constexpr (DIM == 3) {
+#pragma unroll
+            for (int qz = 0; qz < ZQ; ++qz) a += CB.S[qz][kz] * hx[qz];
+          } else {
+            a = hx[0];
+          }
+          out[comp][kz][i] = a;
+        }
+      }
+    }
+  } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
+    __syncthreads();
+    constexpr bool SQ = (OP == OP_ADIAG);
+    T bs[ZQ][Q], bg[ZQ][Q];
+#pragma unroll
+    for (int qz = 0; qz < ZQ; ++qz)
 #pragma unroll
       for (int qx = 0; qx < Q; ++qx) {
         T s0 = 0, s1 = 0;
 #pragma unroll
         for (int qy = 0; qy < Q; ++qy) {
-          const T f = X(0, 0, qx, qy);
-          s0 += (SQ ? (T)CB.S2[qy][j] : (T)CB.S[qy][j]) * f;
-          if constexpr (SQ) s1 += (T)CB.G2[qy][j] * f;
+          const T f = X(0, qz, qx, qy);
+          s0 += (SQ ? CB.S2[qy][j] : CB.S[qy][j]) * f;
+          if constexpr (SQ) s1 += CB.G2[qy][j] * f;
         }
-        bs[qx] = s0;
-        bg[qx] = s1;
+        bs[qz][qx] = s0;
+        bg[qz][qx] = s1;
       }
-      __syncthreads();
-      if constexpr (OP == OP_RHS) {
-        // out[0] accumulates the value integral; expanded to components after the loop
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          T row = 0;
+    for (int i = 0; i < N; ++i) {
+      T rs[ZQ], rg[ZQ], rsg[ZQ];
 #pragma unroll
-          for (int qx = 0; qx < Q; ++qx) row += (T)CB.S[qx][i] * bs[qx];
-          if constexpr (DIM == 3) {
+      for (int qz = 0; qz < ZQ; ++qz) {
+        T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
-            for (int kz = 0; kz < N; ++kz) out[0][kz][i] += (T)CB.S[qz][kz] * row;
+        for (int qx = 0; qx < Q; ++qx) {
+          if constexpr (SQ) {
+            a0 += CB.G2[qx][i] * bs[qz][qx];  // D_x: G2 x, S2 y
+            a1 += CB.S2[qx][i] * bg[qz][qx];  // D_y: S2 x, G2 y
+            a2 += CB.S2[qx][i] * bs[qz][qx];  // D_z: S2 x, S2 y
           } else {
-            out[0][0][i] += row;
+            a0 += CB.S[qx][i] * bs[qz][qx];
           }
         }
-      } else {
-        // out[d] accumulates D_d = sum_q eta JxW (dN/dx_d)^2
+        rs[qz] = a0;
+        rg[qz] = a1;
+        rsg[qz] = a2;
+      }
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          T rx = 0, ry = 0, rz = 0;
+      for (int kz = 0; kz < ZN; ++kz) {
+        T a0 = 0, a1 = 0, a2 = 0;
+        if constexpr (DIM == 3) {
 #pragma unroll
-          for (int qx = 0; qx < Q; ++qx) {
-            rx += (T)CB.G2[qx][i] * bs[qx];
-            ry += (T)CB.S2[qx][i] * bg[qx];
-            rz += (T)CB.S2[qx][i] * bs[qx];
-          }
-          rx *= ih[0] * ih[0];
-          ry *= ih[1] * ih[1];
-          rz *= ih[2] * ih[2];
-          if constexpr (DIM == 3) {
-#pragma unroll
-            for (int kz = 0; kz < N; ++kz) {
-              out[0][kz][i] += (T)CB.S2[qz][kz] * rx;
-              out[1][kz][i] += (T)CB.S2[qz][kz] * ry;
-              out[2][kz][i] += (T)CB.G2[qz][kz] * rz;
+          for (int qz = 0; qz < ZQ; ++qz) {
+            if constexpr (SQ) {
+              a0 += CB.S2[qz][kz] * rs[qz];
+              a1 += CB.S2[qz][kz] * rg[qz];
+              a2 += CB.G2[qz][kz] * rsg[qz];
+            } else {
+              a0 += CB.S[qz][kz] * rs[qz];
             }
-          } else {
-            out[0][0][i] += rx;
-            out[1][0][i] += ry;
           }
+        } else {
+          a0 = rs[0];
+          a1 = rg[0];
+        }
+        if constexpr (SQ) {
+          const T Dx = a0 * ih[0] * ih[0], Dy = a1 * ih[1] * ih[1], Dz = a2 * ih[2] * ih[2];
+          const T sum = Dx + Dy + (DIM == 3 ? Dz : (T)0);
+          // 2 eta eps(phi):eps(phi) = eta (|grad N|^2 + (dN/dx_c)^2)  [- 2/3 eta (dN/dx_c)^2 compressible]
+          const T fc = L.compressible ? (T)(1.0 / 3.0) : (T)1;
+          out[0][kz][i] = sum + fc * Dx;
+          out[1][kz][i] = sum + fc * Dy;
+          if constexpr (DIM == 3) out[2][kz][i] = sum + fc * Dz;
+        } else {
+#pragma unroll
+          for (int comp = 0; comp < DIM; ++comp) out[comp][kz][i] = (T)L.gravity[comp] * a0;
         }
       }
-    }
-
-    // this slice's share of the pressure test-function integrals
-    if constexpr (OUT_P) {
-      constexpr bool SQ = (OP == OP_MDIAG);
-#pragma unroll
-      for (int kz = 0; kz < ZNP; ++kz)
-#pragma unroll
-        for (int jy = 0; jy < NP; ++jy)
-#pragma unroll
-          for (int i = 0; i < NP; ++i) {
-            T acc = 0;
-#pragma unroll
-            for (int qx = 0; qx < Q; ++qx) {
-              const T wx = SQ ? (T)CB.SP2[qx][i] : (T)CB.SP[qx][i];
-              acc += wx * fp[qx];
-            }
-            const T wz = DIM == 3 ? (SQ ? (T)CB.SP2[qz][kz] : (T)CB.SP[qz][kz]) : (T)1;
-            const T wy = SQ ? (T)CB.SP2[j][jy] : (T)CB.SP[j][jy];
-            pacc[kz][jy][i] += wz * wy * acc;
-          }
     }
   }
 
-  if constexpr (OP == OP_RHS) {
+  // ---- 5. per-cell contributions to LDS, then every tile node sums its
+  // contributors in a fixed order (deterministic) and is written out
+  constexpr int CU_OFF = 0;                         // [comp][kz][j][i][CT]
+  constexpr int CP_OFF = DIM * ZN * N * N * CT;     // [kz][jy][i][Q][CT] pressure shares per slice
+  __syncthreads();
+  if constexpr (OUT_U) {
 #pragma unroll
-    for (int comp = DIM - 1; comp >= 0; --comp)
+    for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
       for (int kz = 0; kz < ZN; ++kz)
 #pragma unroll
-        for (int i = 0; i < N; ++i) out[comp][kz][i] = (T)L.gravity[comp] * out[0][kz][i];
-  } else if constexpr (OP == OP_ADIAG) {
-    // 2 eta eps(phi):eps(phi) = eta (|grad N|^2 + (dN/dx_c)^2)  [- 2/3 eta (dN/dx_c)^2 compressible]
-    const T fc = L.compressible ? (T)(1.0 / 3.0) : (T)1;
-#pragma unroll
-    for (int kz = 0; kz < ZN; ++kz)
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T D[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) D[d] = out[d][kz][i];
-        T sum = 0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sum += D[d];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) out[d][kz][i] = sum + fc * D[d];
-      }
+        for (int i = 0; i < N; ++i) xb[CU_OFF + (((comp * ZN + kz) * N + j) * N + i) * CT + c] = out[comp][kz][i];
   }
-
-  // pressure: reduce the slices' shares (thread j == 0 holds the cell's result)
-  T pout[ZNP][NP][NP];
   if constexpr (OUT_P) {
 #pragma unroll
     for (int kz = 0; kz < ZNP; ++kz)
 #pragma unroll
       for (int jy = 0; jy < NP; ++jy)
 #pragma unroll
-        for (int i = 0; i < NP; ++i) xb[(((kz * NP + jy) * NP + i) * Q + j) * CT + c] = pacc[kz][jy][i];
-    __syncthreads();
-    if (j == 0) {
-#pragma unroll
-      for (int kz = 0; kz < ZNP; ++kz)
-#pragma unroll
-        for (int jy = 0; jy < NP; ++jy)
-#pragma unroll
-          for (int i = 0; i < NP; ++i) {
-            T acc = 0;
-#pragma unroll
-            for (int jj = 0; jj < Q; ++jj) acc += xb[(((kz * NP + jy) * NP + i) * Q + jj) * CT + c];
-            pout[kz][jy][i] = acc;
-          }
-    }
-    __syncthreads();
+        for (int i = 0; i < NP; ++i) xb[CP_OFF + ((((kz * NP + jy) * NP + i) * Q + j) * CT) + c] = pacc[kz][jy][i];
   }
-
-  // ---- 6. assemble the tile in LDS by cell parity
-  for (int t = tid; t < DIM * C::VN + C::PN; t += C::THREADS) sm[t] = 0;
   __syncthreads();
-  const int parity = (cx & 1) | ((cy & 1) << 1) | ((DIM == 3 ? (cz & 1) : 0) << 2);
-  for (int ph = 0; ph < (1 << DIM); ++ph) {
-    if (valid && parity == ph) {
-      if constexpr (OUT_U) {
-#pragma unroll
-        for (int comp = 0; comp < DIM; ++comp)
-#pragma unroll
-          for (int kz = 0; kz < ZN; ++kz)
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-              nb[comp * C::VN + ((K * cz + kz) * C::VY + (K * cy + j)) * C::VX + K * cx + i] += out[comp][kz][i];
-      }
-      if constexpr (OUT_P) {
-        if (j == 0) {
-#pragma unroll
-          for (int kz = 0; kz < ZNP; ++kz)
-#pragma unroll
-            for (int jy = 0; jy < NP; ++jy)
-#pragma unroll
-              for (int i = 0; i < NP; ++i)
-                pb[(((K - 1) * cz + kz) * C::PY + ((K - 1) * cy + jy)) * C::PX + (K - 1) * cx + i] +=
-                    pout[kz][jy][i];
-        }
-      }
-    }
-    __syncthreads();
-  }
 
-  // ---- 7. write the tile box: lowest colour stores, later colours add
   if constexpr (OUT_U) {
     for (int t = tid; t < DIM * C::VN; t += C::THREADS) {
       const int comp = t / C::VN, r = t - comp * C::VN;
       const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
       const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
       if (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2]) continue;
+      // contributing (cell, local index) per direction: at most two
+      int ccx[2], iix[2], ncx = 0, ccy[2], iiy[2], ncy = 0, ccz[2], iiz[2], ncz = 0;
+      if (lx % K == 0 && lx > 0) { ccx[ncx] = lx / K - 1; iix[ncx++] = K; }
+      if (lx / K < C::TX) { ccx[ncx] = lx / K; iix[ncx++] = lx % K; }
+      if (ly % K == 0 && ly > 0) { ccy[ncy] = ly / K - 1; iiy[ncy++] = K; }
+      if (ly / K < C::TY) { ccy[ncy] = ly / K; iiy[ncy++] = ly % K; }
+      if (DIM == 3) {
+        if (lz % K == 0 && lz > 0) { ccz[ncz] = lz / K - 1; iiz[ncz++] = K; }
+        if (lz / K < C::TZ) { ccz[ncz] = lz / K; iiz[ncz++] = lz % K; }
+      } else {
+        ccz[0] = 0;
+        iiz[0] = 0;
+        ncz = 1;
+      }
+      T val = 0;
+      for (int a = 0; a < ncz; ++a) {
+        if (tz * C::TZ + ccz[a] >= (DIM == 3 ? L.n[2] : 1)) continue;
+        for (int bb = 0; bb < ncy; ++bb) {
+          if (ty * C::TY + ccy[bb] >= L.n[1]) continue;
+          for (int e = 0; e < ncx; ++e) {
+            if (tx * C::TX + ccx[e] >= L.n[0]) continue;
+            const int cell = ccx[e] + C::TX * (ccy[bb] + C::TY * ccz[a]);
+            val += xb[CU_OFF + (((comp * ZN + iiz[a]) * N + iiy[bb]) * N + iix[e]) * CT + cell];
+          }
+        }
+      }
       const bool shx = (lx == 0 && tx > 0) || (lx == C::VX - 1 && tx + 1 < L.ntiles[0]);
       const bool shy = (ly == 0 && ty > 0) || (ly == C::VY - 1 && ty + 1 < L.ntiles[1]);
       const bool shz = DIM == 3 && ((lz == 0 && tz > 0) || (lz == C::VZ - 1 && tz + 1 < L.ntiles[2]));
       const bool first = !((shx && px) || (shy && py) || (shz && pz));
       const long long gi = comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz);
-      T v;
+      T vv;
       if (is_con(L, comp, gx, gy, gz)) {
         if (!first) continue;
         if constexpr (OP == OP_ADIAG)
-          v = 1;
+          vv = 1;
         else if constexpr (OP == OP_BT || OP == OP_RHS)
-          v = 0;
+          vv = 0;
         else
-          v = src_u[gi];
+          vv = src_u[gi];
       } else {
-        v = nb[t];
+        vv = val;
       }
       if constexpr (MODE == MODE_SET)
-        dst_u[gi] = first ? v : dst_u[gi] + v;
+        dst_u[gi] = first ? vv : dst_u[gi] + vv;
       else if constexpr (MODE == MODE_RESID)
-        dst_u[gi] = first ? rhs_u[gi] - v : dst_u[gi] - v;
+        dst_u[gi] = first ? rhs_u[gi] - vv : dst_u[gi] - vv;
       else
-        dst_u[gi] += v;
+        dst_u[gi] += vv;
     }
   }
   if constexpr (OUT_P) {
+    constexpr int KP = K - 1;
     for (int t = tid; t < C::PN; t += C::THREADS) {
       const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
-      const int gx = qx0 + lx, gy = qy0 + ly, gz = qz0 + lz;
+      const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
       if (gx >= L.np[0] || gy >= L.np[1] || gz >= L.np[2]) continue;
+      int ccx[2], iix[2], ncx = 0, ccy[2], iiy[2], ncy = 0, ccz[2], iiz[2], ncz = 0;
+      if (lx % KP == 0 && lx > 0) { ccx[ncx] = lx / KP - 1; iix[ncx++] = KP; }
+      if (lx / KP < C::TX) { ccx[ncx] = lx / KP; iix[ncx++] = lx % KP; }
+      if (ly % KP == 0 && ly > 0) { ccy[ncy] = ly / KP - 1; iiy[ncy++] = KP; }
+      if (ly / KP < C::TY) { ccy[ncy] = ly / KP; iiy[ncy++] = ly % KP; }
+      if (DIM == 3) {
+        if (lz % KP == 0 && lz > 0) { ccz[ncz] = lz / KP - 1; iiz[ncz++] = KP; }
+        if (lz / KP < C::TZ) { ccz[ncz] = lz / KP; iiz[ncz++] = lz % KP; }
+      } else {
+        ccz[0] = 0;
+        iiz[0] = 0;
+        ncz = 1;
+      }
+      T val = 0;
+      for (int a = 0; a < ncz; ++a) {
+        if (tz * C::TZ + ccz[a] >= (DIM == 3 ? L.n[2] : 1)) continue;
+        for (int bb = 0; bb < ncy; ++bb) {
+          if (ty * C::TY + ccy[bb] >= L.n[1]) continue;
+          for (int e = 0; e < ncx; ++e) {
+            if (tx * C::TX + ccx[e] >= L.n[0]) continue;
+            const int cell = ccx[e] + C::TX * (ccy[bb] + C::TY * ccz[a]);
+#pragma unroll
+            for (int jj = 0; jj < Q; ++jj)
+              val += xb[CP_OFF + ((((iiz[a] * NP + iiy[bb]) * NP + iix[e]) * Q + jj) * CT) + cell];
+          }
+        }
+      }
       const bool shx = (lx == 0 && tx > 0) || (lx == C::PX - 1 && tx + 1 < L.ntiles[0]);
       const bool shy = (ly == 0 && ty > 0) || (ly == C::PY - 1 && ty + 1 < L.ntiles[1]);
       const bool shz = DIM == 3 && ((lz == 0 && tz > 0) || (lz == C::PZ - 1 && tz + 1 < L.ntiles[2]));
       const bool first = !((shx && px) || (shy && py) || (shz && pz));
       const long long gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
-      const T v = pb[t];
       if constexpr (MODE == MODE_SET)
-        dst_p[gi] = first ? v : dst_p[gi] + v;
+        dst_p[gi] = first ? val : dst_p[gi] + val;
       else if constexpr (MODE == MODE_RESID)
-        dst_p[gi] = first ? rhs_p[gi] - v : dst_p[gi] - v;
+        dst_p[gi] = first ? rhs_p[gi] - val : dst_p[gi] - val;
       else
-        dst_p[gi] += v;
+        dst_p[gi] += val;
     }
   }
 }
