@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaspect_mfs.so")
-SOURCES = ["kernels.hip", "handler.hip"]
+SOURCES = ["kernels.hip", "kernels_lines.hip", "handler.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
@@ -20,7 +20,7 @@ def objects():
     return [os.path.join(CSRC, s) for s in SOURCES]
 
 
-def build(verbose: bool = False, jobs: int = 2) -> str:
+def build(verbose: bool = False, jobs: int = 3) -> str:
     flags = [
         "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -41,6 +41,10 @@ struct DBuf {
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
   ~DBuf() { release(); }
   void alloc(long long nn) {
     release();
@@ -99,6 +103,11 @@ int n_qp(const mfs_box &b) {
   const int q = b.degree + 1;
   return b.dim == 3 ? q * q * q : q * q;
 }
+int cells_per_tile_of(const LevelDesc &L) {
+  int t[3];
+  return tile_cells(L.dim, L.k, t);
+}
+int qp_per_cell_of(const LevelDesc &L) { return L.dim == 3 ? (L.k + 1) * (L.k + 1) * (L.k + 1) : (L.k + 1) * (L.k + 1); }
 int cells_per_tile(const mfs_box &b) {
   int t[3];
   return tile_cells(b.dim, b.degree, t);
@@ -147,8 +156,13 @@ struct GMG {
     top = b.n_levels - 1;
   }
 
-  // level viscosity: finest from the active table, coarser by averaging children
+  int eta_pq = 0;  // level tables per quadrature point (DGQ1) or per cell (DGQ0)
+
+  // DGQ0 level viscosity: finest from the active table, coarser by averaging children
   void set_eta(const double *active, int active_per_qp) {
+    eta_pq = 0;
+    for (auto &g : lv)
+      if (g->eta.n != g->L.ntiles_total * cells_per_tile_of(g->L)) g->eta.alloc(g->L.ntiles_total * cells_per_tile_of(g->L));
     GMGLevel<T> &f = *lv[top];
     if (active_per_qp)
       launch_eta_qp_to_cell<T, double>(f.L, active, f.eta.p, st);
@@ -157,13 +171,27 @@ struct GMG {
     for (int l = top - 1; l >= 0; --l) launch_eta_restrict<T>(lv[l]->L, lv[l + 1]->L, lv[l + 1]->eta.p, lv[l]->eta.p, st);
   }
 
+  // DGQ1 level viscosity: per-qp tables from the Q1 coefficients of every level
+  void set_eta_q1(const std::vector<DBuf<double>> &coef, double lo, double hi) {
+    eta_pq = 1;
+    for (int l = 0; l <= top; ++l) {
+      GMGLevel<T> &g = *lv[l];
+      const long long na = g.L.ntiles_total * cells_per_tile_of(g.L) * qp_per_cell_of(g.L);
+      if (g.eta.n != na) {
+        g.eta.alloc(na);
+        v_fill<T>(g.eta.p, (T)1, na, st);
+      }
+      launch_eta_q1_eval<T>(g.L, coef[l].p, lo, hi, g.eta.p, st);
+    }
+  }
+
   void applyA(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];
-    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, 0, rhs, nullptr, dst, nullptr, st);
+    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst, nullptr, st);
   }
   void applyS(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];
-    launch_cell_op<T>(OP_MASS, mode, g.L, nullptr, x, g.eta.p, 0, nullptr, rhs, nullptr, dst, st);
+    launch_cell_op<T>(OP_MASS, mode, g.L, nullptr, x, g.eta.p, eta_pq, nullptr, rhs, nullptr, dst, st);
   }
   void apply(int kind, int l, Mode mode, const T *x, const T *rhs, T *dst) {
     if (kind == 0)
@@ -263,10 +291,10 @@ struct GMG {
   void build() {
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
-      launch_cell_op<T>(OP_ADIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, 0, nullptr, nullptr, g.dinvA.p, nullptr,
-                        st);
-      launch_cell_op<T>(OP_MDIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, 0, nullptr, nullptr, nullptr, g.dinvS.p,
-                        st);
+      launch_cell_op<T>(OP_ADIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, eta_pq, nullptr, nullptr, g.dinvA.p,
+                        nullptr, st);
+      launch_cell_op<T>(OP_MDIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, eta_pq, nullptr, nullptr, nullptr,
+                        g.dinvS.p, st);
       v_invert<T>(g.dinvA.p, g.nu, st);
       v_invert<T>(g.dinvS.p, g.np, st);
     }
@@ -350,6 +378,7 @@ struct mfs_handle {
   long long nu = 0, np = 0, n = 0;
   int nq = 0, ct = 0;
   DBuf<double> eta_nat, eta_active, ones_cell;
+  std::vector<DBuf<double>> q1_coef;  // DGQ1 coefficients per level ("project to Q1")
   int active_per_qp = 0;
   bool have_eta = false, have_prec = false;
   double eta_min = 0, eta_max = 0;
@@ -589,12 +618,118 @@ struct mfs_handle {
     }
   }
 
+  // IDR(s) with biorthogonalization (van Gijzen and Sonneveld, ACM TOMS 38(1),
+  // 2011, Algorithm 913), deal.II's SolverIDR scheme for the cheap steps
+  // (stokes_matrix_free_local_smoothing.cc:1291-1301).  One step = s inner
+  // updates + one dimension-reduction update, one preconditioner application
+  // each; convergence on the recursive residual after every update.  x and r
+  // are always updated with the same U_k and G_k = A U_k, so the residual stays
+  // consistent with x whatever the precision of the preconditioner.
+  template <typename Prec>
+  KrylovStatus idrs(const double *b, double *x, double tol, int maxit, int s, Prec P) {
+    KrylovStatus ks;
+    double *r = vec(vchunks, 3 * s), *v = vec(vchunks, 3 * s + 1), *uhat = vec(vchunks, 3 * s + 2),
+           *vhat = vec(vchunks, 3 * s + 3);
+    auto Q = [&](int i) { return vec(vchunks, i); };
+    auto G = [&](int i) { return vec(vchunks, s + i); };
+    auto U = [&](int i) { return vec(vchunks, 2 * s + i); };
+    const size_t bytes = (size_t)n * sizeof(double);
+    stokes(MODE_RESID, x, b, r);
+    double res = std::sqrt(dot(r, r, n));
+    ks.res = res;
+    if (res <= tol) {
+      ks.converged = true;
+      return ks;
+    }
+    for (int k = 0; k < s; ++k) {  // shadow space, modified Gram-Schmidt
+      v_idr_shadow(Q(k), k, n, st);
+      for (int j = 0; j < k; ++j) v_axpby<double>(-dot(Q(k), Q(j), n), Q(j), 1.0, Q(k), n, st);
+      v_scale(Q(k), 1.0 / std::sqrt(dot(Q(k), Q(k), n)), n, st);
+      v_fill<double>(G(k), 0.0, n, st);
+      v_fill<double>(U(k), 0.0, n, st);
+    }
+    std::vector<double> M(s * s, 0.0), f(s), c(s);
+    for (int i = 0; i < s; ++i) M[i * s + i] = 1.0;
+    double omega = 1.0;
+    int step = 0;
+    while (true) {
+      ++step;
+      for (int i = 0; i < s; ++i) f[i] = dot(Q(i), r, n);
+      for (int k = 0; k < s; ++k) {
+        for (int i = k; i < s; ++i) {  // M(k:s, k:s) is lower triangular
+          double a = f[i];
+          for (int j = k; j < i; ++j) a -= M[i * s + j] * c[j];
+          c[i] = a / M[i * s + i];
+        }
+        MFS_HIP(hipMemcpyAsync(v, r, bytes, hipMemcpyDeviceToDevice, st));
+        for (int i = k; i < s; ++i) v_axpby<double>(-c[i], G(i), 1.0, v, n, st);
+        P(v, vhat);
+        MFS_HIP(hipMemcpyAsync(uhat, vhat, bytes, hipMemcpyDeviceToDevice, st));
+        v_scale(uhat, omega, n, st);
+        for (int i = k; i < s; ++i) v_axpby<double>(c[i], U(i), 1.0, uhat, n, st);
+        stokes(MODE_SET, uhat, nullptr, G(k));
+        for (int i = 0; i < k; ++i) {  // G_k orthogonal to Q_0..Q_{k-1}
+          const double alpha = dot(Q(i), G(k), n) / M[i * s + i];
+          v_axpby<double>(-alpha, G(i), 1.0, G(k), n, st);
+          v_axpby<double>(-alpha, U(i), 1.0, uhat, n, st);
+        }
+        MFS_HIP(hipMemcpyAsync(U(k), uhat, bytes, hipMemcpyDeviceToDevice, st));
+        for (int i = k; i < s; ++i) M[i * s + k] = dot(Q(i), G(k), n);
+        if (M[k * s + k] == 0.0) {  // breakdown: leave it to the expensive phase
+          ks.its = step;
+          return ks;
+        }
+        const double beta = f[k] / M[k * s + k];
+        v_axpby<double>(-beta, G(k), 1.0, r, n, st);
+        v_axpby<double>(beta, U(k), 1.0, x, n, st);
+        res = std::sqrt(dot(r, r, n));
+        ks.res = res;
+        if (res <= tol) {
+          ks.converged = true;
+          ks.its = step;
+          return ks;
+        }
+        if (step >= maxit) {
+          ks.its = step;
+          return ks;
+        }
+        for (int i = k + 1; i < s; ++i) f[i] -= beta * M[i * s + k];
+      }
+      // dimension reduction
+      P(r, uhat);
+      stokes(MODE_SET, uhat, nullptr, v);
+      const double vv = dot(v, v, n);
+      if (vv == 0.0) {
+        ks.its = step;
+        return ks;
+      }
+      omega = dot(v, r, n) / vv;
+      v_axpby<double>(-omega, v, 1.0, r, n, st);
+      v_axpby<double>(omega, uhat, 1.0, x, n, st);
+      res = std::sqrt(dot(r, r, n));
+      ks.res = res;
+      if (res <= tol) {
+        ks.converged = true;
+        ks.its = step;
+        return ks;
+      }
+      if (step >= maxit) {
+        ks.its = step;
+        return ks;
+      }
+    }
+  }
+
   // the handler's solve (device vectors)
   int solve_dev(const double *rhs, const double *x0, double *sol, const mfs_solve_params &prm, mfs_solve_result &res) {
     if (!have_eta) throw std::runtime_error("mfs_solve: viscosity not set (call mfs_set_viscosity)");
     if (!have_prec) throw std::runtime_error("mfs_solve: preconditioner not built (call mfs_build_preconditioner)");
     if (prm.restart < 1 || prm.restart > 1000) throw std::runtime_error("mfs_solve: restart must be in [1, 1000]");
     if (prm.n_cheap_steps < 0 || prm.n_expensive_steps < 0) throw std::runtime_error("mfs_solve: negative step limit");
+    if (prm.cheap_krylov != MFS_KRYLOV_GMRES && prm.cheap_krylov != MFS_KRYLOV_IDRS)
+      throw std::runtime_error("mfs_solve: unknown Krylov method for the cheap steps");
+    if (prm.cheap_krylov == MFS_KRYLOV_IDRS && (prm.idr_s < 1 || prm.idr_s > 8))
+      throw std::runtime_error("mfs_solve: IDR(s) parameter must be in [1, 8]");
     std::memset(&res, 0, sizeof(res));
     const double s = box.pressure_scaling;
     a_apps = s_apps = 0;
@@ -628,8 +763,10 @@ struct mfs_handle {
       // rounding, so the cheap phase stores the preconditioned vectors
       // (flexible update x += Z y) to keep the residual estimate exact.
       const bool flexible = prec == MFS_GMG_FP32;
-      KrylovStatus ks = gmres(rhs, sol, tol, prm.n_cheap_steps, prm.restart, flexible,
-                              [&](const double *a, double *b) { block_prec_cheap(a, b); });
+      auto pc = [&](const double *a, double *b) { block_prec_cheap(a, b); };
+      KrylovStatus ks = prm.cheap_krylov == MFS_KRYLOV_IDRS
+                            ? idrs(rhs, sol, tol, prm.n_cheap_steps, prm.idr_s, pc)
+                            : gmres(rhs, sol, tol, prm.n_cheap_steps, prm.restart, flexible, pc);
       res.cheap_iterations = ks.its;
       res.final_linear_residual = ks.res;
       ok = ks.converged;
@@ -740,7 +877,8 @@ void create(const mfs_box *b, int prec, mfs_handle **out) {
 
 void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
   if (!eta_qp) throw std::runtime_error("mfs_set_viscosity: null viscosity");
-  if (averaging < 0 || averaging > 3) throw std::runtime_error("mfs_set_viscosity: invalid averaging");
+  if (averaging < 0 || averaging > MFS_AVERAGE_PROJECT_Q1)
+    throw std::runtime_error("mfs_set_viscosity: invalid averaging");
   const LevelDesc &L = h->Ltop();
   const long long nqv = L.cells * h->nq;
   double mn = 1e300, mx = -1e300;
@@ -750,11 +888,27 @@ void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
   }
   if (h->eta_nat.n != nqv) h->eta_nat.alloc(nqv);
   MFS_HIP(hipMemcpyAsync(h->eta_nat.p, eta_qp, nqv * sizeof(double), hipMemcpyHostToDevice, h->st));
-  h->active_per_qp = averaging == MFS_AVERAGE_NONE ? 1 : 0;
+  const bool q1 = averaging == MFS_AVERAGE_PROJECT_Q1;
+  h->active_per_qp = (averaging == MFS_AVERAGE_NONE || q1) ? 1 : 0;
   const long long na = L.ntiles_total * h->ct * (h->active_per_qp ? h->nq : 1);
   if (h->eta_active.n != na) h->eta_active.alloc(na);
   v_fill<double>(h->eta_active.p, 1.0, na, h->st);
-  if (h->active_per_qp) {
+  if (q1) {
+    // Q1 coefficients on every level, active table = clamped values at the
+    // quadrature points (matrix_free_operators.cc:308-326)
+    for (long long i = 0; i < nqv; ++i) {
+      mn = std::min(mn, eta_qp[i]);
+      mx = std::max(mx, eta_qp[i]);
+    }
+    const int nlev = (int)h->L.size(), nv = 1 << L.dim;
+    h->q1_coef.resize(nlev);
+    for (int l = 0; l < nlev; ++l)
+      if (h->q1_coef[l].n != h->L[l].cells * nv) h->q1_coef[l].alloc(h->L[l].cells * nv);
+    launch_eta_q1_project(L, h->eta_nat.p, h->q1_coef[nlev - 1].p, h->st);
+    for (int l = nlev - 2; l >= 0; --l)
+      launch_eta_q1_restrict(h->L[l], h->L[l + 1], h->q1_coef[l + 1].p, h->q1_coef[l].p, h->st);
+    launch_eta_q1_eval<double>(L, h->q1_coef[nlev - 1].p, mn, mx, h->eta_active.p, h->st);
+  } else if (h->active_per_qp) {
     launch_eta_to_tiles<double>(L, h->eta_nat.p, 1, h->eta_active.p, h->st);
     for (long long i = 0; i < nqv; ++i) {
       mn = std::min(mn, eta_qp[i]);
@@ -789,10 +943,16 @@ void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
   }
   h->eta_min = mn;
   h->eta_max = mx;
-  if (h->prec == MFS_GMG_FP32)
+  if (q1) {
+    if (h->prec == MFS_GMG_FP32)
+      h->g32.set_eta_q1(h->q1_coef, mn, mx);
+    else
+      h->g64.set_eta_q1(h->q1_coef, mn, mx);
+  } else if (h->prec == MFS_GMG_FP32) {
     h->g32.set_eta(h->eta_active.p, h->active_per_qp);
-  else
+  } else {
     h->g64.set_eta(h->eta_active.p, h->active_per_qp);
+  }
   MFS_HIP(hipStreamSynchronize(h->st));
   h->have_eta = true;
   h->have_prec = false;
@@ -1018,7 +1178,7 @@ int mfs_ablock_diagonal(mfs_handle *h, int level, double *diag) {
       using T = std::remove_pointer_t<decltype(g.lv[0]->eta.p)>;
       DBuf<T> d;
       d.alloc(nu);
-      launch_cell_op<T>(OP_ADIAG, MODE_SET, g.lv[level]->L, nullptr, nullptr, g.lv[level]->eta.p, 0, nullptr,
+      launch_cell_op<T>(OP_ADIAG, MODE_SET, g.lv[level]->L, nullptr, nullptr, g.lv[level]->eta.p, g.eta_pq, nullptr,
                         nullptr, d.p, nullptr, h->st);
       v_cast<T, double>(d.p, h->w2.p, nu, h->st);
       HostIO{h}.out(diag, h->w2.p, nu);
@@ -1040,7 +1200,7 @@ int mfs_mass_diagonal(mfs_handle *h, int level, double *diag) {
       using T = std::remove_pointer_t<decltype(g.lv[0]->eta.p)>;
       DBuf<T> d;
       d.alloc(np);
-      launch_cell_op<T>(OP_MDIAG, MODE_SET, g.lv[level]->L, nullptr, nullptr, g.lv[level]->eta.p, 0, nullptr,
+      launch_cell_op<T>(OP_MDIAG, MODE_SET, g.lv[level]->L, nullptr, nullptr, g.lv[level]->eta.p, g.eta_pq, nullptr,
                         nullptr, nullptr, d.p, h->st);
       v_cast<T, double>(d.p, h->w2.p, np, h->st);
       HostIO{h}.out(diag, h->w2.p, np);
@@ -1097,11 +1257,12 @@ int mfs_level_viscosity(mfs_handle *h, int level, double *out) {
                              h->st));
       MFS_HIP(hipStreamSynchronize(h->st));
       const long long CT = (long long)t[0] * t[1] * t[2];
+      const int nq = g.eta_pq ? qp_per_cell_of(L) : 1;
       for (long long c = 0; c < L.cells; ++c) {
         const int gx = (int)(c % L.n[0]), gy = (int)((c / L.n[0]) % L.n[1]), gz = (int)(c / ((long long)L.n[0] * L.n[1]));
         const long long tile = gx / t[0] + (long long)L.ntiles[0] * (gy / t[1] + (long long)L.ntiles[1] * (gz / t[2]));
         const long long ci = gx % t[0] + t[0] * (gy % t[1] + t[1] * (gz % t[2]));
-        out[c] = (double)tiles[tile * CT + ci];
+        for (int q = 0; q < nq; ++q) out[c * nq + q] = (double)tiles[(tile * nq + q) * CT + ci];
       }
     };
     if (h->prec == MFS_GMG_FP32)

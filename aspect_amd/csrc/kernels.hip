@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "mfs_internal.h"
+#include "tile_common.h"
 
 namespace mfs {
 
@@ -36,6 +37,7 @@ struct BasisConst {
   double S2[4][4], G2[4][4], SP2[4][4];  // squares (diagonal kernels)
   double D[4][4];   // collocation derivative on the Gauss points: D[q][r] = l_r'(x_q)
   double w[4];      // Gauss weights on [0,1]
+  double x[4];      // Gauss points on [0,1]
   double W[7][4];   // prolongation weights, velocity degree k:  [r][m], r < 2k+1
   double WP[5][3];  // prolongation weights, pressure degree k-1
 };
@@ -147,6 +149,7 @@ void upload_basis_tables() {
     std::vector<double> vp = lobatto(k), pp = lobatto(k - 1);
     for (int q = 0; q <= k; ++q) {
       B.w[q] = qw[q];
+      B.x[q] = qx[q];
       for (int i = 0; i <= k; ++i) {
         B.S[q][i] = lag(vp, i, qx[q]);
         B.G[q][i] = lagd(vp, i, qx[q]);
@@ -185,44 +188,6 @@ void upload_basis_tables() {
   done = true;
 }
 
-// ------------------------------------------------------------------ configuration
-template <int DIM, int K>
-struct Tile;
-template <>
-struct Tile<3, 2> {
-  static constexpr int TX = 4, TY = 4, TZ = 4;
-};
-template <>
-struct Tile<3, 3> {
-  static constexpr int TX = 4, TY = 4, TZ = 2;
-};
-template <>
-struct Tile<2, 2> {
-  static constexpr int TX = 8, TY = 8, TZ = 1;
-};
-template <>
-struct Tile<2, 3> {
-  static constexpr int TX = 8, TY = 8, TZ = 1;
-};
-
-template <int DIM, int K>
-struct Cfg {
-  static constexpr int N = K + 1, Q = K + 1, NP = K;
-  static constexpr int ZN = DIM == 3 ? N : 1, ZQ = DIM == 3 ? Q : 1, ZNP = DIM == 3 ? NP : 1;
-  static constexpr int TX = Tile<DIM, K>::TX, TY = Tile<DIM, K>::TY, TZ = Tile<DIM, K>::TZ;
-  static constexpr int CT = TX * TY * TZ;
-  static constexpr int THREADS = CT * Q;
-  static constexpr int VX = K * TX + 1, VY = K * TY + 1, VZ = DIM == 3 ? K * TZ + 1 : 1;
-  static constexpr int PX = (K - 1) * TX + 1, PY = (K - 1) * TY + 1, PZ = DIM == 3 ? (K - 1) * TZ + 1 : 1;
-  static constexpr int VN = VX * VY * VZ, PN = PX * PY * PZ;
-  static constexpr int NQ = ZQ * Q * Q, NPL = ZNP * NP * NP;
-  static constexpr int QS = ZQ * Q;  // quadrature points of one y-slice
-  static constexpr int XBUF = DIM * ZQ * Q * Q * CT;
-  static constexpr int PBUF = DIM * ZN * N * N * CT + NPL * Q * CT;
-  static constexpr int NBUF = DIM * VN + PN;
-  static constexpr int SMEM = (XBUF > NBUF ? (XBUF > PBUF ? XBUF : PBUF) : (NBUF > PBUF ? NBUF : PBUF));
-};
-
 int tile_cells(int dim, int k, int out[3]) {
   if (dim == 3 && k == 2) {
     out[0] = Tile<3, 2>::TX, out[1] = Tile<3, 2>::TY, out[2] = Tile<3, 2>::TZ;
@@ -244,26 +209,14 @@ int tile_threads(int dim, int k) {
 
 #define CB tb<T, K>()
 
-__device__ __forceinline__ bool is_con(const LevelDesc &L, int comp, int gx, int gy, int gz) {
-  const int g[3] = {gx, gy, gz};
-  for (int d = 0; d < L.dim; ++d) {
-    int b;
-    if (g[d] == 0)
-      b = L.bc[2 * d];
-    else if (g[d] == L.nv[d] - 1)
-      b = L.bc[2 * d + 1];
-    else
-      continue;
-    if (b == 2 || (b == 1 && comp == d)) return true;
-  }
-  return false;
-}
-
 // ------------------------------------------------------------------ the cell kernel
 // Collocation form of the sum factorization: nodal values are interpolated to
 // quadrature-point values with S (Q x N) along every direction, and
 // derivatives are taken on the Gauss points with D (Q x Q), D S = G.  Only the y
 // direction crosses threads (LDS); x and z stay in registers.
+#ifndef MFS_CELL_LINES
+#define MFS_CELL_LINES 1  // 3D Q2 Stokes / A block through the line kernel
+#endif
 #ifndef MFS_MINW_F32
 #define MFS_MINW_F32 1
 #endif
@@ -691,63 +644,9 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
   }
   __syncthreads();
 
-  if constexpr (OUT_U) {
-    for (int t = tid; t < DIM * C::VN; t += C::THREADS) {
-      const int comp = t / C::VN, r = t - comp * C::VN;
-      const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
-      const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
-      if (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2]) continue;
-      // contributing (cell, local index) per direction: at most two
-      int ccx[2], iix[2], ncx = 0, ccy[2], iiy[2], ncy = 0, ccz[2], iiz[2], ncz = 0;
-      if (lx % K == 0 && lx > 0) { ccx[ncx] = lx / K - 1; iix[ncx++] = K; }
-      if (lx / K < C::TX) { ccx[ncx] = lx / K; iix[ncx++] = lx % K; }
-      if (ly % K == 0 && ly > 0) { ccy[ncy] = ly / K - 1; iiy[ncy++] = K; }
-      if (ly / K < C::TY) { ccy[ncy] = ly / K; iiy[ncy++] = ly % K; }
-      if (DIM == 3) {
-        if (lz % K == 0 && lz > 0) { ccz[ncz] = lz / K - 1; iiz[ncz++] = K; }
-        if (lz / K < C::TZ) { ccz[ncz] = lz / K; iiz[ncz++] = lz % K; }
-      } else {
-        ccz[0] = 0;
-        iiz[0] = 0;
-        ncz = 1;
-      }
-      T val = 0;
-      for (int a = 0; a < ncz; ++a) {
-        if (tz * C::TZ + ccz[a] >= (DIM == 3 ? L.n[2] : 1)) continue;
-        for (int bb = 0; bb < ncy; ++bb) {
-          if (ty * C::TY + ccy[bb] >= L.n[1]) continue;
-          for (int e = 0; e < ncx; ++e) {
-            if (tx * C::TX + ccx[e] >= L.n[0]) continue;
-            const int cell = ccx[e] + C::TX * (ccy[bb] + C::TY * ccz[a]);
-            val += xb[CU_OFF + (((comp * ZN + iiz[a]) * N + iiy[bb]) * N + iix[e]) * CT + cell];
-          }
-        }
-      }
-      const bool shx = (lx == 0 && tx > 0) || (lx == C::VX - 1 && tx + 1 < L.ntiles[0]);
-      const bool shy = (ly == 0 && ty > 0) || (ly == C::VY - 1 && ty + 1 < L.ntiles[1]);
-      const bool shz = DIM == 3 && ((lz == 0 && tz > 0) || (lz == C::VZ - 1 && tz + 1 < L.ntiles[2]));
-      const bool first = !((shx && px) || (shy && py) || (shz && pz));
-      const long long gi = comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz);
-      T vv;
-      if (is_con(L, comp, gx, gy, gz)) {
-        if (!first) continue;
-        if constexpr (OP == OP_ADIAG)
-          vv = 1;
-        else if constexpr (OP == OP_BT || OP == OP_RHS)
-          vv = 0;
-        else
-          vv = src_u[gi];
-      } else {
-        vv = val;
-      }
-      if constexpr (MODE == MODE_SET)
-        dst_u[gi] = first ? vv : dst_u[gi] + vv;
-      else if constexpr (MODE == MODE_RESID)
-        dst_u[gi] = first ? rhs_u[gi] - vv : dst_u[gi] - vv;
-      else
-        dst_u[gi] += vv;
-    }
-  }
+  if constexpr (OUT_U)
+    tile_write_u<DIM, K, OP, MODE, T, C::THREADS>(L, xb + CU_OFF, tid, tx, ty, tz, px, py, pz, src_u, rhs_u,
+                                                  dst_u);
   if constexpr (OUT_P) {
     constexpr int KP = K - 1;
     for (int t = tid; t < C::PN; t += C::THREADS) {
@@ -840,6 +739,9 @@ void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, c
 template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st) {
+#if MFS_CELL_LINES
+  if (launch_cell_lines<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st)) return;
+#endif
   if (L.dim == 3 && L.k == 2)
     launch_dk<3, 2, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
   else if (L.dim == 3 && L.k == 3)
@@ -1110,6 +1012,88 @@ __global__ void k_eta_restrict(const LevelDesc Lc, const LevelDesc Lf, const int
 
 static void tiles_of(const LevelDesc &L, int t[3]) { tile_cells(L.dim, L.k, t); }
 
+// DGQ1 ("project to Q1 only viscosity") tables.  Cellwise L2 projection of the
+// per-qp values onto Q1: with the dual basis psi_0 = 4 - 6x, psi_1 = 6x - 2 of
+// Q1 on [0,1], vertex coefficient v = sum_q w_q eta_q prod_d psi_{v_d}(x_q,d)
+// (Utilities::project_cellwise, utilities.cc:2738-2800).  Coefficients are in
+// natural cell order, [cell][2^dim] with x fastest.
+__global__ void k_eta_q1_project(const LevelDesc L, const double *__restrict__ eta_nat, double *__restrict__ coef) {
+  const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cell >= L.cells) return;
+  const BasisConst &B = c_basis[L.k - 2];
+  const int Q = L.k + 1, nq = L.dim == 3 ? Q * Q * Q : Q * Q, nv = 1 << L.dim;
+  double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < nq; ++q) {
+    const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
+    double wq = eta_nat[cell * nq + q];
+    for (int d = 0; d < L.dim; ++d) wq *= B.w[qi[d]];
+    for (int v = 0; v < nv; ++v) {
+      double t = wq;
+      for (int d = 0; d < L.dim; ++d) {
+        const double x = B.x[qi[d]];
+        t *= ((v >> d) & 1) ? 6.0 * x - 2.0 : 4.0 - 6.0 * x;
+      }
+      c[v] += t;
+    }
+  }
+  for (int v = 0; v < nv; ++v) coef[cell * nv + v] = c[v];
+}
+// parent coefficients = L2 projection of the children's piecewise-Q1 function:
+// per direction child k contributes R_k[a][j], R_0 = [[3/4, 1/2], [-1/4, 0]],
+// R_1 = [[0, -1/4], [1/2, 3/4]] (parent vertex a, child vertex j)
+__global__ void k_eta_q1_restrict(const LevelDesc Lc, const LevelDesc Lf, const double *__restrict__ fine,
+                                  double *__restrict__ coarse) {
+  const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cell >= Lc.cells) return;
+  const double R[2][2][2] = {{{0.75, 0.5}, {-0.25, 0.0}}, {{0.0, -0.25}, {0.5, 0.75}}};
+  const int dim = Lc.dim, nv = 1 << dim;
+  const int gcx = (int)(cell % Lc.n[0]), gcy = (int)((cell / Lc.n[0]) % Lc.n[1]),
+            gcz = (int)(cell / ((long long)Lc.n[0] * Lc.n[1]));
+  double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int ch = 0; ch < nv; ++ch) {
+    const int k[3] = {ch & 1, (ch >> 1) & 1, (ch >> 2) & 1};
+    const long long fc = (2LL * gcx + k[0]) +
+                         Lf.n[0] * ((2LL * gcy + k[1]) + (long long)Lf.n[1] * (dim == 3 ? 2LL * gcz + k[2] : 0));
+    for (int a = 0; a < nv; ++a)
+      for (int j = 0; j < nv; ++j) {
+        double w = 1.0;
+        for (int d = 0; d < dim; ++d) w *= R[k[d]][(a >> d) & 1][(j >> d) & 1];
+        c[a] += w * fine[fc * nv + j];
+      }
+  }
+  for (int a = 0; a < nv; ++a) coarse[cell * nv + a] = c[a];
+}
+// Q1 values at the quadrature points, clamped to [lo, hi], tile-blocked per-qp table
+template <typename T>
+__global__ void k_eta_q1_eval(const LevelDesc L, const double *__restrict__ coef, const double lo, const double hi,
+                              const int TX, const int TY, const int TZ, T *__restrict__ out) {
+  const int Q = L.k + 1, nq = L.dim == 3 ? Q * Q * Q : Q * Q, nv = 1 << L.dim;
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= L.cells * nq) return;
+  const long long cell = idx / nq;
+  const int q = (int)(idx - cell * nq);
+  const BasisConst &B = c_basis[L.k - 2];
+  const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
+  double val = 0;
+  for (int v = 0; v < nv; ++v) {
+    double t = coef[cell * nv + v];
+    for (int d = 0; d < L.dim; ++d) {
+      const double x = B.x[qi[d]];
+      t *= ((v >> d) & 1) ? x : 1.0 - x;
+    }
+    val += t;
+  }
+  val = fmin(fmax(val, lo), hi);
+  const int gcx = (int)(cell % L.n[0]), gcy = (int)((cell / L.n[0]) % L.n[1]),
+            gcz = (int)(cell / ((long long)L.n[0] * L.n[1]));
+  const long long ti = tile_index(L, gcx, gcy, gcz, TX, TY, TZ);
+  const long long CT = TX * TY * TZ;
+  const long long tile = ti / CT, c = ti - tile * CT;
+  out[(tile * nq + q) * CT + c] = (T)val;
+}
+
+
+
 template <typename T>
 void launch_eta_to_tiles(const LevelDesc &L, const double *eta_nat, int per_qp, T *eta_tiles, hipStream_t st) {
   int t[3];
@@ -1148,6 +1132,28 @@ void launch_eta_restrict(const LevelDesc &Lc, const LevelDesc &Lf, const T *fine
                      t[1], t[2], fine, coarse);
   MFS_HIP(hipGetLastError());
 }
+void launch_eta_q1_project(const LevelDesc &L, const double *eta_nat_qp, double *coef, hipStream_t st) {
+  hipLaunchKernelGGL(k_eta_q1_project, dim3((unsigned)((L.cells + 255) / 256)), dim3(256), 0, st, L, eta_nat_qp, coef);
+  MFS_HIP(hipGetLastError());
+}
+void launch_eta_q1_restrict(const LevelDesc &Lc, const LevelDesc &Lf, const double *fine, double *coarse,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_eta_q1_restrict, dim3((unsigned)((Lc.cells + 255) / 256)), dim3(256), 0, st, Lc, Lf, fine,
+                     coarse);
+  MFS_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_eta_q1_eval(const LevelDesc &L, const double *coef, double lo, double hi, T *out, hipStream_t st) {
+  int t[3];
+  tiles_of(L, t);
+  const int Q = L.k + 1, nq = L.dim == 3 ? Q * Q * Q : Q * Q;
+  const long long n = L.cells * nq;
+  hipLaunchKernelGGL((k_eta_q1_eval<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, coef, lo, hi, t[0],
+                     t[1], t[2], out);
+  MFS_HIP(hipGetLastError());
+}
+template void launch_eta_q1_eval<double>(const LevelDesc &, const double *, double, double, double *, hipStream_t);
+template void launch_eta_q1_eval<float>(const LevelDesc &, const double *, double, double, float *, hipStream_t);
 template void launch_eta_to_tiles<double>(const LevelDesc &, const double *, int, double *, hipStream_t);
 template void launch_eta_to_tiles<float>(const LevelDesc &, const double *, int, float *, hipStream_t);
 template void launch_eta_average<double>(const LevelDesc &, const double *, int, double *, hipStream_t);
@@ -1256,6 +1262,22 @@ template void v_cast<double, float>(const double *, float *, long long, hipStrea
 template void v_cast<float, double>(const float *, double *, long long, hipStream_t);
 template void v_cast<double, double>(const double *, double *, long long, hipStream_t);
 template void v_cast<float, float>(const float *, float *, long long, hipStream_t);
+
+// IDR(s) shadow-space vector k: uniform in [-1, 1) from a counter hash
+// (splitmix64 finalizer) of (k, i); the oracle draws the same bits.
+__global__ void k_idr_shadow(double *__restrict__ q, const int k, const long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned long long z = (unsigned long long)(k + 1) * 0x9E3779B97F4A7C15ull + (unsigned long long)i * 0xD1B54A32D192ED03ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  q[i] = (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+void v_idr_shadow(double *q, int k, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_idr_shadow, dim3(nblocks(n)), dim3(256), 0, st, q, k, n);
+  MFS_HIP(hipGetLastError());
+}
 
 // ---- deterministic reductions: fixed grid of DOT_BLOCKS blocks, fixed-order tree
 constexpr int DOT_BLOCKS = 512;

@@ -1,0 +1,358 @@
+// gfx950 line kernel for the 3D Q2 x Q1 Stokes operator and A block on
+// uniform box levels (the hot operators of the solve).
+//
+// Restates, per cell:
+//   StokesOperator::local_apply   source/simulator/solver/matrix_free_operators.cc:376-547
+//   ABlockOperator::inner_cell_operation / local_apply   :974-1052
+//
+// Work decomposition: one workgroup per 4x4x4 tile of cells, one thread per
+// (cell, line): a line is one of the 3 x 3 x-lines of a cell's nodes or
+// quadrature points, indexed (a, b) = (y index, z index).  The 64 cells of a
+// tile form one wave, so (a, b) is uniform over a wave.  The x direction of
+// the sum factorization stays in registers; the y and z directions go through
+// LDS, one exchange array [comp][q][line][cell] per step, three arrays in
+// rotation so that every step needs a single barrier.  Per-thread state is one
+// line (3 values per component), which keeps register use low enough for
+// several waves per SIMD.  Collocation form: S (Q x N) interpolates nodal
+// values to Gauss points, D (Q x Q) differentiates on the Gauss points
+// (D S = G); tests apply D^T and S^T.
+//
+// The tile write-out is the deterministic gather shared with the general cell
+// kernel (tile colours, lowest colour stores, later colours add).
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "mfs_internal.h"
+#include "tile_common.h"
+
+namespace mfs {
+namespace {
+
+constexpr int LN_THREADS = 576;  // 64 cells x 9 lines
+
+template <typename T>
+struct LinesBlocks {  // workgroups per CU the register budget is tuned for
+  static constexpr int value = std::is_same<T, double>::value ? 1 : 2;
+};
+
+// row r (or column r) of a 3x3 table, r uniform over the wave
+template <typename T>
+__device__ __forceinline__ void row3(const T (&M)[3][3], int r, T out[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = r == 0 ? M[0][k] : (r == 1 ? M[1][k] : M[2][k]);
+}
+template <typename T>
+__device__ __forceinline__ void col3(const T (&M)[3][3], int r, T out[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = r == 0 ? M[k][0] : (r == 1 ? M[k][1] : M[k][2]);
+}
+
+template <int OP, int MODE, typename T>
+__global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
+    k_lines(const LevelDesc L, const int color, const T *__restrict__ src_u, const T *__restrict__ src_p,
+            const T *__restrict__ eta, const int eta_per_qp, const T *__restrict__ rhs_u,
+            const T *__restrict__ rhs_p, T *__restrict__ dst_u, T *__restrict__ dst_p) {
+  using C = Cfg<3, 2>;
+  using B = tab::Basis<T, 2>;
+  using BD = tab::Basis<double, 2>;
+  constexpr int CT = C::CT, NL = 9;
+  constexpr int XA = 9 * NL * CT;  // [comp][q][line][cell]
+  constexpr int XP = 2 * NL * CT;  // pressure tile, then pressure shares [i][line][cell]
+  constexpr bool STOKES = (OP == OP_STOKES);
+  static_assert(CT == 64 && CT * NL == LN_THREADS, "one wave per line");
+  static_assert(3 * C::VN <= XA && C::PN <= XP, "tile load fits the exchange arrays");
+  __shared__ T sA[XA], sB[XA], sC[XA];
+  __shared__ T sP[STOKES ? XP : 1];
+
+  const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+  const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
+  int bid = blockIdx.x;
+  const int tx = 2 * (bid % hx) + px;
+  bid /= hx;
+  const int ty = 2 * (bid % hy) + py;
+  bid /= hy;
+  const int tz = 2 * bid + pz;
+  const long long tile_id = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
+  const int tid = threadIdx.x;
+  const int c = tid & (CT - 1);
+  const int ln = __builtin_amdgcn_readfirstlane(tid / CT);
+  const int a = ln % 3, b = ln / 3;
+  const int cx = c % C::TX, cy = (c / C::TX) % C::TY, cz = c / (C::TX * C::TY);
+  const int vx0 = 2 * C::TX * tx, vy0 = 2 * C::TY * ty, vz0 = 2 * C::TZ * tz;
+  const int ox0 = C::TX * tx, oy0 = C::TY * ty, oz0 = C::TZ * tz;
+  auto XI = [&](int comp, int q, int l) { return ((comp * 3 + q) * NL + l) * CT + c; };
+
+  // ---- 0. cooperative tile load (constrained velocity DoFs read as zero)
+  for (int t = tid; t < 3 * C::VN; t += LN_THREADS) {
+    const int comp = t / C::VN, r = t - comp * C::VN;
+    const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
+    const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
+    T v = 0;
+    if (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2] && !is_con(L, comp, gx, gy, gz))
+      v = src_u[comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz)];
+    sC[t] = v;
+  }
+  if constexpr (STOKES) {
+    for (int t = tid; t < C::PN; t += LN_THREADS) {
+      const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
+      const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
+      T v = 0;
+      if (gx < L.np[0] && gy < L.np[1] && gz < L.np[2])
+        v = src_p[gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz)];
+      sP[t] = v;
+    }
+  }
+  __syncthreads();
+
+  // wave-uniform rows and columns of the 1D tables
+  T Sa[3], Sb[3], Da[3], Db[3], DTa[3], DTb[3], STa[3], STb[3];
+  row3(B::S, a, Sa);
+  row3(B::S, b, Sb);
+  row3(B::D, a, Da);
+  row3(B::D, b, Db);
+  col3(B::D, a, DTa);
+  col3(B::D, b, DTb);
+  col3(B::S, a, STa);
+  col3(B::S, b, STb);
+
+  // ---- 1. x interpolation on the node line (jy, kz) = (a, b); pressure at
+  // the quadrature line (qy, qz) = (a, b)
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    T u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      u[i] = sC[comp * C::VN + ((2 * cz + b) * C::VY + (2 * cy + a)) * C::VX + 2 * cx + i];
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx)
+      sA[XI(comp, qx, ln)] = B::S[qx][0] * u[0] + B::S[qx][1] * u[1] + B::S[qx][2] * u[2];
+  }
+  T pq[3] = {0, 0, 0};
+  if constexpr (STOKES) {
+    const T SPa[2] = {a == 0 ? B::SP[0][0] : (a == 1 ? B::SP[1][0] : B::SP[2][0]),
+                      a == 0 ? B::SP[0][1] : (a == 1 ? B::SP[1][1] : B::SP[2][1])};
+    const T SPb[2] = {b == 0 ? B::SP[0][0] : (b == 1 ? B::SP[1][0] : B::SP[2][0]),
+                      b == 0 ? B::SP[0][1] : (b == 1 ? B::SP[1][1] : B::SP[2][1])};
+    T pyz[2];  // pressure nodes interpolated in y and z to (a, b), per x node
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      T acc = 0;
+#pragma unroll
+      for (int kz = 0; kz < 2; ++kz)
+#pragma unroll
+        for (int jy = 0; jy < 2; ++jy)
+          acc += SPb[kz] * SPa[jy] * sP[((cz + kz) * C::PY + (cy + jy)) * C::PX + cx + i];
+      pyz[i] = acc;
+    }
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx) pq[qx] = B::SP[qx][0] * pyz[0] + B::SP[qx][1] * pyz[1];
+  }
+  __syncthreads();
+
+  // ---- 2. y interpolation: line (qy, kz) = (a, b)
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx)
+      sB[XI(comp, qx, ln)] = Sa[0] * sA[XI(comp, qx, 3 * b)] + Sa[1] * sA[XI(comp, qx, 3 * b + 1)] +
+                             Sa[2] * sA[XI(comp, qx, 3 * b + 2)];
+  __syncthreads();
+
+  // ---- 3. z interpolation: quadrature line (qy, qz) = (a, b)
+  T v[3][3];
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx) {
+      v[comp][qx] = Sb[0] * sB[XI(comp, qx, a)] + Sb[1] * sB[XI(comp, qx, a + 3)] + Sb[2] * sB[XI(comp, qx, a + 6)];
+      sA[XI(comp, qx, ln)] = v[comp][qx];
+    }
+  __syncthreads();
+
+  // ---- 4. gradients, quadrature-point operation, x test; y and z fluxes to LDS
+  const T ih0 = (T)(1.0 / L.h[0]), ih1 = (T)(1.0 / L.h[1]), ih2 = (T)(1.0 / L.h[2]);
+  const T s = (T)L.s;
+  const double wab = L.h[0] * L.h[1] * L.h[2] * (a == 0 ? BD::W[0] : (a == 1 ? BD::W[1] : BD::W[2])) *
+                     (b == 0 ? BD::W[0] : (b == 1 ? BD::W[1] : BD::W[2]));
+  T tacc[3][3];
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) tacc[comp][r] = 0;
+  T hp0 = 0, hp1 = 0;
+#pragma unroll
+  for (int qx = 0; qx < 3; ++qx) {
+    const T JxW = (T)(wab * BD::W[qx]);
+    const T et = eta_per_qp ? eta[(tile_id * 27 + qx + 3 * ln) * CT + c] : eta[tile_id * CT + c];
+    T gr[3][3];
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      const T dx = B::D[qx][0] * v[comp][0] + B::D[qx][1] * v[comp][1] + B::D[qx][2] * v[comp][2];
+      const T dy = Da[0] * sA[XI(comp, qx, 3 * b)] + Da[1] * sA[XI(comp, qx, 3 * b + 1)] +
+                   Da[2] * sA[XI(comp, qx, 3 * b + 2)];
+      const T dz = Db[0] * sA[XI(comp, qx, a)] + Db[1] * sA[XI(comp, qx, a + 3)] + Db[2] * sA[XI(comp, qx, a + 6)];
+      gr[comp][0] = dx * ih0;
+      gr[comp][1] = dy * ih1;
+      gr[comp][2] = dz * ih2;
+    }
+    const T div = gr[0][0] + gr[1][1] + gr[2][2];
+    T flux[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        T sig = et * (gr[i][k] + gr[k][i]);
+        if (i == k) {
+          if (L.compressible) sig -= 2 * et / 3 * div;
+          if constexpr (STOKES) sig -= s * pq[qx];
+        }
+        flux[i][k] = sig * JxW;
+      }
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      const T fx = flux[comp][0] * ih0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) tacc[comp][r] += B::D[qx][r] * fx;
+      sB[XI(comp, qx, ln)] = flux[comp][1] * ih1;
+      sC[XI(comp, qx, ln)] = flux[comp][2] * ih2;
+    }
+    if constexpr (STOKES) {
+      const T fp = -s * div * JxW;
+      hp0 += B::SP[qx][0] * fp;
+      hp1 += B::SP[qx][1] * fp;
+    }
+  }
+  if constexpr (STOKES) {
+    sP[ln * CT + c] = hp0;
+    sP[(NL + ln) * CT + c] = hp1;
+  }
+  __syncthreads();
+
+  // ---- 5. y and z tests at the quadrature line (qy, qz) = (a, b)
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx)
+      sA[XI(comp, qx, ln)] = tacc[comp][qx] + DTa[0] * sB[XI(comp, qx, 3 * b)] +
+                             DTa[1] * sB[XI(comp, qx, 3 * b + 1)] + DTa[2] * sB[XI(comp, qx, 3 * b + 2)] +
+                             DTb[0] * sC[XI(comp, qx, a)] + DTb[1] * sC[XI(comp, qx, a + 3)] +
+                             DTb[2] * sC[XI(comp, qx, a + 6)];
+  __syncthreads();
+
+  // ---- 6. z integration: line (qy, kz) = (a, b)
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx)
+      sB[XI(comp, qx, ln)] =
+          STb[0] * sA[XI(comp, qx, a)] + STb[1] * sA[XI(comp, qx, a + 3)] + STb[2] * sA[XI(comp, qx, a + 6)];
+  __syncthreads();
+
+  // ---- 7. y integration: node line (jy, kz) = (a, b), then x; per-cell
+  // contributions [comp][kz][jy][i][cell]
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    T hyv[3];
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx)
+      hyv[qx] = STa[0] * sB[XI(comp, qx, 3 * b)] + STa[1] * sB[XI(comp, qx, 3 * b + 1)] +
+                STa[2] * sB[XI(comp, qx, 3 * b + 2)];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      sA[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] =
+          B::S[0][i] * hyv[0] + B::S[1][i] * hyv[1] + B::S[2][i] * hyv[2];
+  }
+  __syncthreads();
+
+  // ---- 8. tile write-out
+  tile_write_u<3, 2, OP, MODE, T, LN_THREADS>(L, sA, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+  if constexpr (STOKES) {
+    for (int t = tid; t < C::PN; t += LN_THREADS) {
+      const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
+      const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
+      if (gx >= L.np[0] || gy >= L.np[1] || gz >= L.np[2]) continue;
+      // contributing cells per direction: (l - 1, local index 1) and (l, local index 0)
+      T val = 0;
+#pragma unroll
+      for (int e3 = 0; e3 < 2; ++e3) {
+        const int ccz = lz - 1 + e3;
+        if (ccz < 0 || ccz >= C::TZ || tz * C::TZ + ccz >= L.n[2]) continue;
+#pragma unroll
+        for (int e2 = 0; e2 < 2; ++e2) {
+          const int ccy = ly - 1 + e2;
+          if (ccy < 0 || ccy >= C::TY || ty * C::TY + ccy >= L.n[1]) continue;
+#pragma unroll
+          for (int e1 = 0; e1 < 2; ++e1) {
+            const int ccx = lx - 1 + e1;
+            if (ccx < 0 || ccx >= C::TX || tx * C::TX + ccx >= L.n[0]) continue;
+            const int cell = ccx + C::TX * (ccy + C::TY * ccz);
+            // node index within the cell is 1 - e per direction
+#pragma unroll
+            for (int l = 0; l < NL; ++l)
+              val += B::SP[l % 3][1 - e2] * B::SP[l / 3][1 - e3] * sP[((1 - e1) * NL + l) * CT + cell];
+          }
+        }
+      }
+      const bool shx = (lx == 0 && tx > 0) || (lx == C::PX - 1 && tx + 1 < L.ntiles[0]);
+      const bool shy = (ly == 0 && ty > 0) || (ly == C::PY - 1 && ty + 1 < L.ntiles[1]);
+      const bool shz = (lz == 0 && tz > 0) || (lz == C::PZ - 1 && tz + 1 < L.ntiles[2]);
+      const bool first = !((shx && px) || (shy && py) || (shz && pz));
+      const long long gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
+      if constexpr (MODE == MODE_SET)
+        dst_p[gi] = first ? val : dst_p[gi] + val;
+      else if constexpr (MODE == MODE_RESID)
+        dst_p[gi] = first ? rhs_p[gi] - val : dst_p[gi] - val;
+      else
+        dst_p[gi] += val;
+    }
+  }
+}
+
+template <int OP, int MODE, typename T>
+void launch_lines_m(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru, const T *rp,
+                    T *du, T *dp, hipStream_t st) {
+  for (int color = 0; color < 8; ++color) {
+    const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+    const long long hx = (L.ntiles[0] - px + 1) / 2, hy = (L.ntiles[1] - py + 1) / 2, hz = (L.ntiles[2] - pz + 1) / 2;
+    const long long nb = hx * hy * hz;
+    if (nb <= 0) continue;
+    hipLaunchKernelGGL((k_lines<OP, MODE, T>), dim3((unsigned)nb), dim3(LN_THREADS), 0, st, L, color, su, sp, eta,
+                       pq, ru, rp, du, dp);
+  }
+  MFS_HIP(hipGetLastError());
+}
+
+template <int OP, typename T>
+void launch_lines_o(Mode mode, const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru,
+                    const T *rp, T *du, T *dp, hipStream_t st) {
+  if (mode == MODE_SET)
+    launch_lines_m<OP, MODE_SET, T>(L, su, sp, eta, pq, ru, rp, du, dp, st);
+  else if (mode == MODE_RESID)
+    launch_lines_m<OP, MODE_RESID, T>(L, su, sp, eta, pq, ru, rp, du, dp, st);
+  else
+    launch_lines_m<OP, MODE_ADD, T>(L, su, sp, eta, pq, ru, rp, du, dp, st);
+}
+
+}  // namespace
+
+template <typename T>
+bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
+                       int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st) {
+  if (L.dim != 3 || L.k != 2) return false;
+  if (op == OP_STOKES)
+    launch_lines_o<OP_STOKES, T>(mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
+  else if (op == OP_ABLOCK)
+    launch_lines_o<OP_ABLOCK, T>(mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
+  else
+    return false;
+  return true;
+}
+template bool launch_cell_lines<double>(Op, Mode, const LevelDesc &, const double *, const double *,
+                                        const double *, int, const double *, const double *, double *, double *,
+                                        hipStream_t);
+template bool launch_cell_lines<float>(Op, Mode, const LevelDesc &, const float *, const float *, const float *,
+                                       int, const float *, const float *, float *, float *, hipStream_t);
+
+}  // namespace mfs

@@ -51,6 +51,12 @@ template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st);
 
+// 3D Q2 x Q1 Stokes and A-block applies with the line kernel; returns false
+// (nothing launched) for other (dim, degree, op).
+template <typename T>
+bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
+                       int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st);
+
 // Transfers between level Lc (coarse) and Lf = refined Lc.  kind 0 velocity, 1 pressure.
 template <typename T>
 void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
@@ -76,6 +82,15 @@ void launch_eta_restrict(const LevelDesc &Lc, const LevelDesc &Lf, const T *fine
 // DGQ0 projection (quadrature-weighted mean) of a tile-blocked per-qp table
 template <typename T, typename S>
 void launch_eta_qp_to_cell(const LevelDesc &L, const S *eta_qp_tiles, T *eta_cell_tiles, hipStream_t st);
+
+// DGQ1 ("project to Q1") tables: per-cell Q1 coefficients [cell][2^dim] in
+// natural cell order, their children->parent L2 projection, and the values at
+// the quadrature points clamped to [lo, hi] as a tile-blocked per-qp table.
+void launch_eta_q1_project(const LevelDesc &L, const double *eta_nat_qp, double *coef, hipStream_t st);
+void launch_eta_q1_restrict(const LevelDesc &Lc, const LevelDesc &Lf, const double *fine, double *coarse,
+                            hipStream_t st);
+template <typename T>
+void launch_eta_q1_eval(const LevelDesc &L, const double *coef, double lo, double hi, T *out, hipStream_t st);
 
 // vector kernels
 template <typename T>
@@ -114,6 +129,8 @@ void v_mul(const T *a, const T *b, T *c, long long n, hipStream_t st);  // c = a
 void v_scale(double *x, double a, long long n, hipStream_t st);
 // y = x / a (a on device)
 void v_scaled_copy_dev(const double *x, const double *a, double *y, long long n, hipStream_t st);
+// IDR(s) shadow-space vector k (uniform in [-1, 1), counter hash of (k, i))
+void v_idr_shadow(double *q, int k, long long n, hipStream_t st);
 // zero constrained velocity entries of a level vector
 template <typename T>
 void v_zero_constrained(const LevelDesc &L, T *u, hipStream_t st);

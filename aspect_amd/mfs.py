@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(_HERE, "libaspect_mfs.so")
 MFS_OK, MFS_ERROR, MFS_NO_CONVERGENCE = 0, 1, 2
 GMG_FP64, GMG_FP32 = 0, 1
 AVERAGE = {"none": 0, "arithmetic average only viscosity": 1, "harmonic average only viscosity": 2,
-           "geometric average only viscosity": 3, "arithmetic": 1, "harmonic": 2, "geometric": 3}
+           "geometric average only viscosity": 3, "project to Q1 only viscosity": 4,
+           "arithmetic": 1, "harmonic": 2, "geometric": 3, "q1": 4}
 BC_OPEN, BC_TANGENTIAL, BC_ZERO = 0, 1, 2
 
 # every symbol include/aspect_mfs.h declares
@@ -59,7 +60,13 @@ class SolveParams(C.Structure):
         ("restart", C.c_int),
         ("a_block_tolerance", C.c_double),
         ("s_block_tolerance", C.c_double),
+        ("cheap_krylov", C.c_int),
+        ("idr_s", C.c_int),
     ]
+
+
+# "Krylov method for cheap solver steps"
+KRYLOV = {"GMRES": 0, "IDR(s)": 1}
 
 
 class SolveResult(C.Structure):
@@ -160,6 +167,7 @@ class StokesMatrixFreeHandler:
         self.box = box
         self.gmg_precision = gmg_precision
         self._h = C.c_void_p()
+        self._averaging = 0
         self.setup_dofs()
 
     # -- StokesMatrixFreeHandler interface ---------------------------------
@@ -180,16 +188,17 @@ class StokesMatrixFreeHandler:
             raise ValueError(f"viscosity table must hold {s['cells']} x {self.n_qp} values")
         mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
         _check(self.lib.mfs_set_viscosity(self._h, _ptr(eta), mode))
+        self._averaging = mode
 
     def build_preconditioner(self):
         _check(self.lib.mfs_build_preconditioner(self._h))
 
     def solve(self, rhs, x0=None, tolerance=1e-7, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
-              a_block_tolerance=1e-2, s_block_tolerance=1e-6, raise_on_failure=True):
+              a_block_tolerance=1e-2, s_block_tolerance=1e-6, raise_on_failure=True, krylov="GMRES", idr_s=2):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         sol = np.zeros_like(rhs)
         prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
-                          s_block_tolerance)
+                          s_block_tolerance, KRYLOV[krylov], idr_s)
         res = SolveResult()
         xp = None
         if x0 is not None:
@@ -268,9 +277,12 @@ class StokesMatrixFreeHandler:
         return self._unary(self.lib.mfs_restrict, (fine_level, kind), src, s["nu"] if kind == 0 else s["np"])
 
     def level_viscosity(self, level):
-        out = np.zeros(self.sizes(level)["cells"])
+        """Level table: one value per cell, or [cells, nq] with "project to Q1"."""
+        cells = self.sizes(level)["cells"]
+        nq = (self.box.degree + 1) ** self.box.dim
+        out = np.zeros(cells * nq if self._averaging == 4 else cells)
         _check(self.lib.mfs_level_viscosity(self._h, level, _ptr(out)))
-        return out
+        return out.reshape(cells, nq) if self._averaging == 4 else out
 
     def smoother_info(self):
         out = np.zeros((self.box.n_levels, 2, 5))
@@ -309,9 +321,10 @@ class StokesMatrixFreeHandler:
         _check(self.lib.mfs_assemble_buoyancy_rhs_device(self._h, C.c_void_p(d_rho), _ptr(g), C.c_void_p(d_rhs)))
 
     def solve_device(self, d_rhs: int, d_x0, d_sol: int, tolerance=1e-7, n_cheap_steps=1000,
-                     n_expensive_steps=1000, restart=100, a_block_tolerance=1e-2, s_block_tolerance=1e-6):
+                     n_expensive_steps=1000, restart=100, a_block_tolerance=1e-2, s_block_tolerance=1e-6,
+                     krylov="GMRES", idr_s=2):
         prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
-                          s_block_tolerance)
+                          s_block_tolerance, KRYLOV[krylov], idr_s)
         res = SolveResult()
         rc = _check(self.lib.mfs_solve_device(self._h, C.c_void_p(d_rhs), C.c_void_p(d_x0) if d_x0 else None,
                                               C.c_void_p(d_sol), C.byref(prm), C.byref(res)))

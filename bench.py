@@ -33,6 +33,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(size):
+    """HBM bytes per FP64 Stokes apply (all colour dispatches) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the
+    copy-kernel calibration recorded beside it; None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"stokes_fp64_{size}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get("hbm_bytes_per_apply")
+
+
 def make_config(size):
     from aspect_amd import problems as P
     cfg = P.config_sinker_128()
@@ -51,7 +63,20 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cheap-steps", type=int, default=200)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three device copies of 1 GiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(2 ** 27, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8, "copies": 3}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
@@ -124,7 +149,7 @@ def main():
                                "per-qp viscosity, free slip; step = one FP64 Stokes operator apply",
                    "cells": cells, "velocity_dofs": nu, "pressure_dofs": npd, "levels": cfg.n_levels},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.size)},
         "ablock_fp32_gdofs": nu / t_a32 / 1e9,
         "ablock_fp32_ms": t_a32 * 1e3,
         "wall_ms_per_step": wall * 1e3,

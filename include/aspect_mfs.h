@@ -32,6 +32,10 @@ extern "C" {
 #define MFS_ERROR 1
 #define MFS_NO_CONVERGENCE 2
 
+/* Krylov method of the cheap solver steps (Parameters::StokesKrylovType). */
+#define MFS_KRYLOV_GMRES 0
+#define MFS_KRYLOV_IDRS 1
+
 /* GMGNumberType of the V-cycle (stokes_matrix_free.h:40 fixes it to double). */
 #define MFS_GMG_FP64 0
 #define MFS_GMG_FP32 1
@@ -42,6 +46,10 @@ extern "C" {
 #define MFS_AVERAGE_ARITHMETIC 1
 #define MFS_AVERAGE_HARMONIC 2
 #define MFS_AVERAGE_GEOMETRIC 3
+/* "project to Q1 only viscosity": cellwise L2 projection onto Q1 (DGQ1), values
+ * at the quadrature points clamped to the evaluated range; level tables per
+ * quadrature point (matrix_free_operators.cc:200-326, local_smoothing.cc:211-294). */
+#define MFS_AVERAGE_PROJECT_Q1 4
 
 /* Boundary type per face (x-, x+, y-, y+, z-, z+). */
 #define MFS_BC_OPEN 0
@@ -70,6 +78,9 @@ typedef struct {
   int restart;              /* "GMRES solver restart length" */
   double a_block_tolerance; /* "Linear solver A block tolerance" */
   double s_block_tolerance; /* "Linear solver S block tolerance" */
+  int cheap_krylov;         /* "Krylov method for cheap solver steps": MFS_KRYLOV_GMRES or MFS_KRYLOV_IDRS
+                               (stokes_matrix_free_local_smoothing.cc:1276-1301) */
+  int idr_s;                /* "IDR(s) parameter", 1..8 (used with MFS_KRYLOV_IDRS) */
 } mfs_solve_params;
 
 /* StokesSolver::SolverOutputs (include/aspect/simulator/solver/interface.h:36-58)
@@ -135,7 +146,8 @@ int mfs_mass_diagonal(mfs_handle *h, int level, double *diag);
  * local_smoothing.cc:1843-1850); kind 0 velocity, 1 pressure. */
 int mfs_prolongate(mfs_handle *h, int fine_level, int kind, const double *src_coarse, double *dst_fine);
 int mfs_restrict(mfs_handle *h, int fine_level, int kind, const double *src_fine, double *dst_coarse);
-/* Level viscosity table (cells lexicographic), local_smoothing.cc:211-294. */
+/* Level viscosity table (cells lexicographic), local_smoothing.cc:211-294:
+ * one value per cell, or cells x nq values with MFS_AVERAGE_PROJECT_Q1. */
 int mfs_level_viscosity(mfs_handle *h, int level, double *out);
 /* Chebyshev set-up per level: out[(level*2+kind)*5 + {0..4}] =
  * lambda_min, lambda_max, theta, delta, eigenvalue-CG iterations. */

@@ -53,6 +53,8 @@ class SolveParams(C.Structure):
         ("a_block_tolerance", C.c_double),
         ("s_block_tolerance", C.c_double),
         ("level_eta_mode", C.c_int),
+        ("cheap_krylov", C.c_int),   # 0 GMRES, 1 IDR(s)
+        ("idr_s", C.c_int),
     ]
 
 
@@ -245,6 +247,21 @@ def level_viscosity(p, eta):
     return res
 
 
+def level_viscosity_q1(p, eta_qp):
+    """Per-quadrature-point viscosity tables of every level (level 0 first) for
+    "project to Q1 only viscosity"; the last entry is the active table."""
+    eta_qp = np.ascontiguousarray(eta_qp, dtype=np.float64)
+    nq = n_qp(p)
+    counts = [sizes(p, l)["cells"] for l in range(p.n_levels)]
+    out = np.zeros(sum(counts) * nq)
+    _check(lib().orc_level_viscosity_q1(C.byref(p), _p(eta_qp), _p(out)))
+    res, off = [], 0
+    for c in counts:
+        res.append(out[off:off + c * nq].reshape(c, nq).copy())
+        off += c * nq
+    return res
+
+
 def smoother_setup(p, eta):
     eta, per_qp = _eta(eta)
     out = np.zeros((p.n_levels, 2, 5))
@@ -278,7 +295,7 @@ def rhs_buoyancy(p, rho_qp, g):
 
 
 def default_params(tolerance=1e-7, n_cheap=1000, n_expensive=1000, restart=100,
-                   a_tol=1e-2, s_tol=1e-6) -> SolveParams:
+                   a_tol=1e-2, s_tol=1e-6, krylov="GMRES", idr_s=2, level_eta="DGQ0") -> SolveParams:
     prm = SolveParams()
     prm.tolerance = tolerance
     prm.n_cheap_steps = n_cheap
@@ -286,7 +303,9 @@ def default_params(tolerance=1e-7, n_cheap=1000, n_expensive=1000, restart=100,
     prm.restart = restart
     prm.a_block_tolerance = a_tol
     prm.s_block_tolerance = s_tol
-    prm.level_eta_mode = 0
+    prm.level_eta_mode = {"DGQ0": 0, "DGQ1": 1}[level_eta]
+    prm.cheap_krylov = {"GMRES": 0, "IDR(s)": 1}[krylov]
+    prm.idr_s = idr_s
     return prm
 
 

@@ -56,7 +56,9 @@ typedef struct {
   int restart;               // "GMRES solver restart length"
   double a_block_tolerance;  // "Linear solver A block tolerance"
   double s_block_tolerance;  // "Linear solver S block tolerance"
-  int level_eta_mode;        // 0: DGQ0 children-average (the only mode restated here)
+  int level_eta_mode;        // 0: DGQ0 (cell means); 1: DGQ1 ("project to Q1 only viscosity")
+  int cheap_krylov;          // "Krylov method for cheap solver steps": 0 GMRES, 1 IDR(s)
+  int idr_s;                 // "IDR(s) parameter"
 } orc_solve_params;
 
 typedef struct {
@@ -850,6 +852,111 @@ std::vector<std::vector<double>> level_viscosity(const Context &C, const double 
   return lv;
 }
 
+// ---------------------------------------------------------------- DGQ1 viscosity ("project to Q1")
+// Cellwise L2 projection of the quadrature-point viscosity onto Q1 (FE_DGQ(1),
+// Utilities::project_cellwise, utilities.cc:2738-2800, called from
+// matrix_free_operators.cc:200-270), values back at the quadrature points
+// clamped to the [min, max] of the evaluated viscosities (:308-326).  Level
+// tables (local_smoothing.cc:211-294): the children's piecewise-Q1 function
+// is projected in L2 onto the parent's Q1 space (the FE_DGQ restriction,
+// which for DGQ0 is the children mean), then evaluated at the quadrature
+// points and clamped the same way.
+// Coefficients are per cell at the 2^dim vertices, x fastest.
+struct Q1Tables {
+  double Rk[2][2][2];  // 1D child->parent projection [child][parent vertex][child vertex]
+};
+Q1Tables q1_tables() {
+  Q1Tables T{};
+  std::vector<double> gx, gw;
+  gauss_legendre(4, gx, gw);  // exact for the cubic integrands
+  for (int k = 0; k < 2; ++k)
+    for (int a = 0; a < 2; ++a)
+      for (int j = 0; j < 2; ++j) {
+        double sum = 0;
+        for (size_t q = 0; q < gx.size(); ++q) {
+          const double t = gx[q], x = 0.5 * (t + k);
+          const double psi = a == 0 ? 4.0 - 6.0 * x : 6.0 * x - 2.0;  // dual basis of Q1 on [0,1]
+          const double phi = j == 0 ? 1.0 - t : t;
+          sum += gw[q] * psi * phi;
+        }
+        T.Rk[k][a][j] = 0.5 * sum;
+      }
+  return T;
+}
+// projection of one cell's nq values: c_v = sum_q w_q eta_q prod_d psi_{v_d}(x_q,d)
+void q1_project_cell(const RefElement &re, const double *eta_q, double *coef) {
+  const int dim = re.dim, nv = 1 << dim, Q = re.Q;
+  for (int v = 0; v < nv; ++v) coef[v] = 0.0;
+  for (int q = 0; q < re.nq(); ++q) {
+    const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
+    for (int v = 0; v < nv; ++v) {
+      double w = re.qweight(q) * eta_q[q];
+      for (int d = 0; d < dim; ++d) {
+        const double x = re.qx[qi[d]];
+        w *= ((v >> d) & 1) ? 6.0 * x - 2.0 : 4.0 - 6.0 * x;
+      }
+      coef[v] += w;
+    }
+  }
+}
+void q1_eval_cell(const RefElement &re, const double *coef, double lo, double hi, double *eta_q) {
+  const int dim = re.dim, nv = 1 << dim, Q = re.Q;
+  for (int q = 0; q < re.nq(); ++q) {
+    const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
+    double val = 0;
+    for (int v = 0; v < nv; ++v) {
+      double w = coef[v];
+      for (int d = 0; d < dim; ++d) {
+        const double x = re.qx[qi[d]];
+        w *= ((v >> d) & 1) ? x : 1.0 - x;
+      }
+      val += w;
+    }
+    eta_q[q] = std::min(std::max(val, lo), hi);
+  }
+}
+// per-qp tables of every level (level 0 first) from the raw active per-qp values
+std::vector<std::vector<double>> level_viscosity_q1(const Context &C, const double *eta_qp) {
+  const int nl = C.p.n_levels, dim = C.p.dim, nv = 1 << dim;
+  const RefElement &re = C.K.re;
+  const int nq = re.nq();
+  const Level &Lf = C.levels[nl - 1];
+  double lo = eta_qp[0], hi = eta_qp[0];
+  for (long long i = 0; i < Lf.cells * nq; ++i) {
+    lo = std::min(lo, eta_qp[i]);
+    hi = std::max(hi, eta_qp[i]);
+  }
+  const Q1Tables T = q1_tables();
+  std::vector<std::vector<double>> coef(nl), out(nl);
+  coef[nl - 1].resize(Lf.cells * nv);
+  for (long long c = 0; c < Lf.cells; ++c) q1_project_cell(re, eta_qp + c * nq, &coef[nl - 1][c * nv]);
+  for (int l = nl - 2; l >= 0; --l) {
+    const Level &L = C.levels[l], &F = C.levels[l + 1];
+    coef[l].assign(L.cells * nv, 0.0);
+    for (long long c = 0; c < L.cells; ++c) {
+      int cc[3];
+      cell_coords(L, c, cc);
+      for (int ch = 0; ch < nv; ++ch) {
+        const int k[3] = {ch & 1, (ch >> 1) & 1, (ch >> 2) & 1};
+        const long long fc = (2LL * cc[0] + k[0]) +
+                             F.n[0] * ((2LL * cc[1] + k[1]) + (long long)F.n[1] * (dim == 3 ? 2LL * cc[2] + k[2] : 0));
+        for (int a = 0; a < nv; ++a)
+          for (int j = 0; j < nv; ++j) {
+            double w = 1.0;
+            for (int d = 0; d < dim; ++d) w *= T.Rk[k[d]][(a >> d) & 1][(j >> d) & 1];
+            coef[l][c * nv + a] += w * coef[l + 1][fc * nv + j];
+          }
+      }
+    }
+  }
+  for (int l = 0; l < nl; ++l) {
+    const Level &L = C.levels[l];
+    out[l].resize(L.cells * nq);
+    for (long long c = 0; c < L.cells; ++c) q1_eval_cell(re, &coef[l][c * nv], lo, hi, &out[l][c * nq]);
+  }
+  return out;
+}
+
 // ---------------------------------------------------------------- vector helpers
 using Vec = std::vector<double>;
 using Op = std::function<void(const double *, double *)>;
@@ -1134,6 +1241,135 @@ SolveStatus gmres(const Op &A, const Op &P, Vec &x, const Vec &b, double tol, in
   }
 }
 
+// Shadow-space entry k of IDR(s): uniform in [-1, 1) from a counter hash
+// (splitmix64 finalizer), so every implementation draws the same bits.
+inline double idr_shadow_entry(int k, long long i) {
+  uint64_t z = (uint64_t)(k + 1) * 0x9E3779B97F4A7C15ull + (uint64_t)i * 0xD1B54A32D192ED03ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-52 - 1.0;
+}
+
+// IDR(s) with biorthogonalization and right preconditioning (van Gijzen and
+// Sonneveld, ACM TOMS 38(1), 2011, Algorithm 913), the scheme of deal.II's
+// SolverIDR used for the cheap steps (stokes_matrix_free_local_smoothing.cc:1291-1301):
+// one step = s inner updates plus one dimension-reduction update, each with
+// one preconditioner application; convergence on the recursively updated
+// residual, checked after every update.  The shadow space is s orthonormalized
+// vectors from idr_shadow_entry (deal.II draws normal variates; any random
+// space serves the method).
+SolveStatus idrs(const Op &A, const Op &P, Vec &x, const Vec &b, double tol, int maxit, int s) {
+  SolveStatus st;
+  const size_t n = b.size();
+  Vec r(n), t(n);
+  A(x.data(), t.data());
+  for (size_t i = 0; i < n; ++i) r[i] = b[i] - t[i];
+  double res = norm(r);
+  st.res = res;
+  if (res <= tol) {
+    st.converged = true;
+    return st;
+  }
+  std::vector<Vec> Qs(s, Vec(n)), G(s, Vec(n, 0.0)), U(s, Vec(n, 0.0));
+  for (int k = 0; k < s; ++k) {
+    for (size_t i = 0; i < n; ++i) Qs[k][i] = idr_shadow_entry(k, (long long)i);
+    for (int j = 0; j < k; ++j) {  // modified Gram-Schmidt
+      const double c = dot(Qs[k], Qs[j]);
+      for (size_t i = 0; i < n; ++i) Qs[k][i] -= c * Qs[j][i];
+    }
+    const double nq = norm(Qs[k]);
+    for (size_t i = 0; i < n; ++i) Qs[k][i] /= nq;
+  }
+  std::vector<double> M(s * s, 0.0);  // M(i,k) = Q_i . G_k, lower triangular after each step
+  for (int i = 0; i < s; ++i) M[i * s + i] = 1.0;
+  double omega = 1.0;
+  Vec v(n), uhat(n), vhat(n);
+  int step = 0;
+  while (true) {
+    ++step;
+    std::vector<double> f(s);
+    for (int i = 0; i < s; ++i) f[i] = dot(Qs[i], r);
+    for (int k = 0; k < s; ++k) {
+      // solve the lower-triangular system M(k:s, k:s) c = f(k:s)
+      std::vector<double> c(s - k);
+      for (int i = k; i < s; ++i) {
+        double a = f[i];
+        for (int j = k; j < i; ++j) a -= M[i * s + j] * c[j - k];
+        c[i - k] = a / M[i * s + i];
+      }
+      for (size_t q = 0; q < n; ++q) {
+        double a = r[q];
+        for (int i = k; i < s; ++i) a -= c[i - k] * G[i][q];
+        v[q] = a;
+      }
+      P(v.data(), vhat.data());
+      ++st.prec_calls;
+      for (size_t q = 0; q < n; ++q) {
+        double a = omega * vhat[q];
+        for (int i = k; i < s; ++i) a += c[i - k] * U[i][q];
+        uhat[q] = a;
+      }
+      A(uhat.data(), G[k].data());
+      for (int i = 0; i < k; ++i) {  // make G_k orthogonal to Q_0..Q_{k-1}
+        const double alpha = dot(Qs[i], G[k]) / M[i * s + i];
+        for (size_t q = 0; q < n; ++q) {
+          G[k][q] -= alpha * G[i][q];
+          uhat[q] -= alpha * U[i][q];
+        }
+      }
+      U[k] = uhat;
+      for (int i = k; i < s; ++i) M[i * s + k] = dot(Qs[i], G[k]);
+      if (M[k * s + k] == 0.0) {  // breakdown
+        st.its = step;
+        return st;
+      }
+      const double beta = f[k] / M[k * s + k];
+      for (size_t q = 0; q < n; ++q) {
+        r[q] -= beta * G[k][q];
+        x[q] += beta * U[k][q];
+      }
+      res = norm(r);
+      st.res = res;
+      if (res <= tol) {
+        st.converged = true;
+        st.its = step;
+        return st;
+      }
+      if (step >= maxit) {
+        st.its = step;
+        return st;
+      }
+      for (int i = k + 1; i < s; ++i) f[i] -= beta * M[i * s + k];
+    }
+    // dimension reduction
+    P(r.data(), uhat.data());
+    ++st.prec_calls;
+    A(uhat.data(), v.data());
+    const double vv = dot(v, v);
+    if (vv == 0.0) {
+      st.its = step;
+      return st;
+    }
+    omega = dot(v, r) / vv;
+    for (size_t q = 0; q < n; ++q) {
+      r[q] -= omega * v[q];
+      x[q] += omega * uhat[q];
+    }
+    res = norm(r);
+    st.res = res;
+    if (res <= tol) {
+      st.converged = true;
+      st.its = step;
+      return st;
+    }
+    if (step >= maxit) {
+      st.its = step;
+      return st;
+    }
+  }
+}
+
 // preconditioned CG (deal.II SolverCG semantics: absolute tolerance, x0 given)
 SolveStatus cg(const Op &A, const Op &P, Vec &x, const Vec &b, double tol, int maxit) {
   SolveStatus st;
@@ -1178,10 +1414,20 @@ struct Solver {
   const Context &C;
   const double *eta;
   int per_qp;
+  int lq = 0;  // level tables per quadrature point (DGQ1)
   std::vector<std::vector<double>> lv;
   MG mgA, mgS;
-  Solver(const Context &c, const double *eta_, int per_qp_) : C(c), eta(eta_), per_qp(per_qp_) {
-    lv = level_viscosity(C, eta, per_qp);
+  // q1: "project to Q1 only viscosity" (eta must be per qp); the active table
+  // becomes the top level table
+  Solver(const Context &c, const double *eta_, int per_qp_, bool q1 = false) : C(c), eta(eta_), per_qp(per_qp_) {
+    if (q1) {
+      if (!per_qp) throw std::runtime_error("Q1 projection needs per-quadrature-point viscosity");
+      lv = level_viscosity_q1(C, eta);
+      lq = 1;
+      eta = lv.back().data();
+    } else {
+      lv = level_viscosity(C, eta, per_qp);
+    }
     const int nl = C.p.n_levels;
     mgA.C = mgS.C = &C;
     mgA.kind = 0;
@@ -1190,8 +1436,8 @@ struct Solver {
     mgS.steps = 4;  // mg::SmootherRelaxation mg_smoother_Schur(4), local_smoothing.cc:803-804
     for (int l = 0; l < nl; ++l) {
       const double *le = lv[l].data();
-      mgA.ops.push_back([this, l, le](const double *s, double *d) { ablock_vmult(C, l, le, 0, s, d); });
-      mgS.ops.push_back([this, l, le](const double *s, double *d) { mass_vmult(C, l, le, 0, s, d); });
+      mgA.ops.push_back([this, l, le](const double *s, double *d) { ablock_vmult(C, l, le, lq, s, d); });
+      mgS.ops.push_back([this, l, le](const double *s, double *d) { mass_vmult(C, l, le, lq, s, d); });
       mgA.sizes.push_back(C.levels[l].nu());
       mgS.sizes.push_back(C.levels[l].pnodes);
     }
@@ -1206,8 +1452,8 @@ struct Solver {
       s.A = mgS.ops[l];
       a.dinv.resize(mgA.sizes[l]);
       s.dinv.resize(mgS.sizes[l]);
-      ablock_diagonal(C, l, lv[l].data(), 0, a.dinv.data());
-      mass_diagonal(C, l, lv[l].data(), 0, s.dinv.data());
+      ablock_diagonal(C, l, lv[l].data(), lq, a.dinv.data());
+      mass_diagonal(C, l, lv[l].data(), lq, s.dinv.data());
       for (auto &v : a.dinv) v = 1.0 / v;
       for (auto &v : s.dinv) v = 1.0 / v;
       const bool coarse = (l == 0);
@@ -1374,6 +1620,19 @@ int orc_level_viscosity(const orc_problem *p, const double *eta, int per_qp, dou
     }
   })
 }
+// DGQ1 ("project to Q1") tables: out holds sum_l cells(l) * nq values, level 0
+// first; the last level is the active table
+int orc_level_viscosity_q1(const orc_problem *p, const double *eta_qp, double *out) {
+  ORC_TRY({
+    Context C(*p);
+    auto lv = level_viscosity_q1(C, eta_qp);
+    size_t off = 0;
+    for (auto &v : lv) {
+      std::memcpy(out + off, v.data(), v.size() * sizeof(double));
+      off += v.size();
+    }
+  })
+}
 // Chebyshev set-up numbers per level: [lmin, lmax, theta, delta, cg_its] for A then S
 int orc_smoother_setup(const orc_problem *p, const double *eta, int per_qp, double *out) {
   ORC_TRY({
@@ -1454,19 +1713,27 @@ int orc_rhs_buoyancy(const orc_problem *p, const double *rho_qp, const double *g
 
 // The handler's solve (stokes_matrix_free_local_smoothing.cc:758-1446) on box
 // meshes: initial guess x0 (velocity constrained entries zeroed, pressure
-// divided by s), residual-based tolerance (:977-1041), cheap GMRES with one
-// V-cycle per block (:1262-1313), FGMRES with inner CG solves on failure
+// divided by s), residual-based tolerance (:977-1041), cheap GMRES or IDR(s)
+// with one V-cycle per block (:1262-1313), FGMRES with inner CG solves on failure
 // (:1314-1397), then pressure *= s and volume pressure normalization.
 int orc_solve(const orc_problem *p, const double *eta, int per_qp, const orc_solve_params *prm,
               const double *rhs, const double *x0, double *solution, orc_solve_result *res) {
   ORC_TRY({
+    if (prm->cheap_krylov != 0 && prm->cheap_krylov != 1) throw std::runtime_error("unknown cheap Krylov method");
+    if (prm->cheap_krylov == 1 && prm->idr_s < 1) throw std::runtime_error("IDR(s) parameter must be >= 1");
     Context C(*p);
     const int top = p->n_levels - 1;
     const Level &L = C.levels[top];
     const long long nu = L.nu(), np = L.pnodes, n = nu + np;
     const double s = p->pressure_scaling;
-    Solver S(C, eta, per_qp);
+    if (prm->level_eta_mode != 0 && prm->level_eta_mode != 1) throw std::runtime_error("unknown level viscosity mode");
+    Solver S(C, eta, per_qp, prm->level_eta_mode == 1);
     S.build_preconditioner();
+    // active viscosity table: the input, or its clamped Q1 projection
+    if (S.lq) {
+      eta = S.eta;
+      per_qp = 1;
+    }
     Op Aop = [&](const double *src, double *dst) { stokes_vmult(C, top, eta, per_qp, src, dst); };
     Vec b(rhs, rhs + n), x(x0, x0 + n);
     for (long long i = 0; i < nu; ++i)
@@ -1537,7 +1804,9 @@ int orc_solve(const orc_problem *p, const double *eta, int per_qp, const orc_sol
     res->expensive_iterations = 0;
     bool ok = false;
     if (prm->n_cheap_steps > 0) {
-      SolveStatus st = gmres(Aop, Pcheap, x, b, tol, prm->n_cheap_steps, prm->restart, false);
+      SolveStatus st = prm->cheap_krylov == 1
+                           ? idrs(Aop, Pcheap, x, b, tol, prm->n_cheap_steps, prm->idr_s)
+                           : gmres(Aop, Pcheap, x, b, tol, prm->n_cheap_steps, prm->restart, false);
       res->cheap_iterations = st.its;
       res->final_linear_residual = st.res;
       ok = st.converged;

@@ -215,3 +215,79 @@ def test_no_convergence_is_reported():
                          np.array([0, 0, -1.0]))
     sol, out = h.solve(rhs, tolerance=1e-14, n_cheap_steps=2, n_expensive_steps=0, raise_on_failure=False)
     assert not out.converged and out.cheap_iterations == 2
+
+
+def _nsinker_idr(nref, prec, idr_s=2):
+    cfg = P.config_nsinker(nref, averaging="harmonic")
+    eta, rho, s = P.fields(cfg)
+    h = StokesMatrixFreeHandler(cfg.box(1.0), prec)
+    h.assemble(eta, "harmonic")
+    h.build_preconditioner()
+    rhs = h.assemble_buoyancy_rhs(rho, cfg.gravity())
+    sol, out = h.solve(rhs, tolerance=1e-6, n_cheap_steps=200, n_expensive_steps=0, raise_on_failure=False,
+                       krylov="IDR(s)", idr_s=idr_s)
+    return sol, out, eta, rho, cfg
+
+
+@pytest.mark.parametrize("idr_s", [1, 2, 4])
+def test_idr_solve_matches_oracle(idr_s):
+    # "Krylov method for cheap solver steps = IDR(s)" (local_smoothing.cc:1291-1301), 8^3 nsinker
+    sol, out, eta, rho, cfg = _nsinker_idr(3, "fp64", idr_s)
+    assert out.converged
+    p = O.make_problem(3, 2, (1, 1, 1), 4, (1, 1, 1), (2,) * 6)
+    eta_o = 1.0 / np.mean(1.0 / eta, axis=1)
+    prm = O.default_params(tolerance=1e-6, n_cheap=200, n_expensive=0, krylov="IDR(s)", idr_s=idr_s)
+    ref, res = O.solve(p, eta_o, O.rhs_buoyancy(p, rho, cfg.gravity()), params=prm)
+    assert abs(out.cheap_iterations - res["cheap_iterations"]) <= 1
+    assert abs(out.a_preconditioner_applications - res["a_preconditioner_applications"]) <= idr_s + 1
+    nu = O.sizes(p, 3)["nu"]
+    assert rel(sol[:nu], ref[:nu]) < 1e-5
+    assert rel(sol[nu:], ref[nu:]) < 1e-5
+
+
+def test_golden_idr_nsinker_64():
+    # tests/performance/performance_nsinker_gmg_idr/screen-output (ASPECT tree):
+    # 64^3, "Stokes solver (IDR(2)): 12+0 iterations", "A block preconditioner: 34+0"
+    sol, out, eta, rho, cfg = _nsinker_idr(6, "fp64")
+    assert out.converged
+    assert abs(out.cheap_iterations - 12) <= 1.2
+    assert abs(out.a_preconditioner_applications - 34) <= 3.4
+    assert abs(eta.min() - 0.01) < 1e-9 and abs(eta.max() - 100) < 1e-6
+
+
+def test_idr_fp32_vcycle_true_residual():
+    # FP32 V-cycle inside IDR(2): the recursive residual stays the true residual
+    sol, out, eta, rho, cfg = _nsinker_idr(3, "fp32")
+    assert out.converged
+    p = O.make_problem(3, 2, (1, 1, 1), 4, (1, 1, 1), (2,) * 6)
+    eta_o = 1.0 / np.mean(1.0 / eta, axis=1)
+    rhs = O.rhs_buoyancy(p, rho, cfg.gravity())
+    r = rhs - O.stokes_vmult(p, 3, eta_o, sol)
+    assert np.linalg.norm(r) <= 1.05 * out.solver_tolerance
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_q1_projection_tables_and_solve(prec):
+    # "project to Q1 only viscosity": DGQ1 tables on every level and the solve
+    p = O.make_problem(3, 2, (3, 2, 1), 3, (1.5, 1.0, 0.5), (1, 2, 0, 1, 1, 2), pressure_scaling=2.0)
+    box = make_box(3, 2, (3, 2, 1), 3, (1.5, 1.0, 0.5), (1, 2, 0, 1, 1, 2), pressure_scaling=2.0)
+    x = O.qp_points(p, 2)
+    eta = np.exp(2 * np.sin(4 * x[..., 0]) * np.cos(3 * x[..., 1]) + x[..., 2])
+    eta[(x[..., 0] - 0.7) ** 2 + (x[..., 1] - 0.5) ** 2 < 0.05] *= 100.0  # jump: clamping active
+    tabs = O.level_viscosity_q1(p, eta)
+    h = StokesMatrixFreeHandler(box, prec)
+    h.assemble(eta, "project to Q1 only viscosity")
+    tol = FP64 * 10 if prec == "fp64" else FP32
+    for l in range(3):
+        assert rel(h.level_viscosity(l), tabs[l]) < tol
+    sz = O.sizes(p, 2)
+    rng = np.random.default_rng(5)
+    xv = rng.normal(size=sz["nu"] + sz["np"])
+    assert rel(h.stokes_vmult(xv), O.stokes_vmult(p, 2, tabs[-1], xv)) < FP64
+    h.build_preconditioner()
+    rhs = O.rhs_buoyancy(p, 1.0 + rng.uniform(size=(sz["cells"], O.n_qp(p))), np.array([0, 0, -1.0]))
+    sol, out = h.solve(rhs, tolerance=1e-8)
+    ref, res = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-8, level_eta="DGQ1"))
+    assert out.converged and res["converged"]
+    assert abs(out.cheap_iterations - res["cheap_iterations"]) <= max(1, 0.1 * res["cheap_iterations"])
+    assert rel(sol, ref) < 1e-6

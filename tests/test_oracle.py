@@ -301,3 +301,112 @@ def test_level_viscosity_is_children_average():
     fine = lv[2].reshape(4, 4, 4)
     coarse = fine.reshape(2, 2, 2, 2, 2, 2).mean(axis=(1, 3, 5))
     np.testing.assert_allclose(lv[1], coarse.ravel(), rtol=1e-14)
+
+
+def test_idr_matches_gmres():
+    # IDR(s) cheap steps (local_smoothing.cc:1291-1301): same solution as GMRES
+    # to the tolerance, and the returned residual is the true residual
+    cfg = P.config_nsinker(2, averaging="harmonic")
+    eta, rho, s = P.fields(cfg)
+    p = O.make_problem(3, 2, (1, 1, 1), 3, (1, 1, 1), (2,) * 6, pressure_scaling=1.0)
+    rhs = O.rhs_buoyancy(p, rho, cfg.gravity())
+    eta = 1.0 / np.mean(1.0 / eta, axis=1)
+    ref, _ = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-10, n_cheap=200, n_expensive=0))
+    for idr_s in (1, 2, 4):
+        prm = O.default_params(tolerance=1e-6, n_cheap=200, n_expensive=0, krylov="IDR(s)", idr_s=idr_s)
+        sol, res = O.solve(p, eta, rhs, params=prm)
+        assert res["converged"]
+        # one preconditioner application per update: s + 1 per full step
+        assert res["a_preconditioner_applications"] <= (idr_s + 1) * res["cheap_iterations"]
+        assert res["a_preconditioner_applications"] > (idr_s + 1) * (res["cheap_iterations"] - 1)
+        r = rhs - O.stokes_vmult(p, 2, eta, sol)
+        assert np.linalg.norm(r) <= 1.01 * res["solver_tolerance"]
+        assert np.linalg.norm(sol - ref) / np.linalg.norm(ref) < 1e-4
+    # IDR(1) with zero iterations allowed fails cleanly
+    sol, res = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-6, n_cheap=1, n_expensive=0,
+                                                            krylov="IDR(s)", idr_s=1))
+    assert not res["converged"] and res["cheap_iterations"] == 1
+
+
+# ---- "project to Q1 only viscosity" (DGQ1 tables)
+def _q1_psi(v, x, dim):
+    # dual basis of Q1 on [0,1]^dim: psi_0 = 4 - 6x, psi_1 = 6x - 2 per direction
+    w = np.ones(x.shape[:-1])
+    for d in range(dim):
+        w = w * np.where((v >> d) & 1, 6 * x[..., d] - 2, 4 - 6 * x[..., d])
+    return w
+
+
+def _q1_phi(v, x, dim):
+    w = np.ones(x.shape[:-1])
+    for d in range(dim):
+        w = w * np.where((v >> d) & 1, x[..., d], 1 - x[..., d])
+    return w
+
+
+def _ref_qp(dim, nq1=3):
+    g, w = np.polynomial.legendre.leggauss(nq1)
+    g, w = 0.5 * (g + 1), 0.5 * w
+    idx = np.indices((nq1,) * dim).reshape(dim, -1)[::-1].T  # x fastest
+    return g[idx], np.prod(w[idx], axis=1)
+
+
+def test_q1_projection_exact_and_orthogonal():
+    p = O.make_problem(3, 2, (1, 1, 1), 3, (1, 1, 1), (1,) * 6)
+    x = O.qp_points(p, 2)
+    # a global linear field is Q1 on every cell of every level: reproduced exactly
+    lin = 1 + x[..., 0] + 2 * x[..., 1] + 3 * x[..., 2]
+    tabs = O.level_viscosity_q1(p, lin)
+    for l, t in enumerate(tabs):
+        xl = O.qp_points(p, l)
+        assert np.allclose(t, 1 + xl[..., 0] + 2 * xl[..., 1] + 3 * xl[..., 2], rtol=1e-13, atol=1e-13)
+    # smooth field: the active table is the L2 projection (residual orthogonal to Q1 per cell)
+    eta = np.exp(np.sin(3 * x[..., 0]) * np.cos(2 * x[..., 1]) + 0.5 * x[..., 2])
+    tabs = O.level_viscosity_q1(p, eta)
+    free = np.all((tabs[-1] > eta.min()) & (tabs[-1] < eta.max()), axis=1)  # cells without clamping
+    assert free.sum() > 48
+    xr, wr = _ref_qp(3)
+    for v in range(8):
+        r = ((eta - tabs[-1]) * wr[None, :] * _q1_phi(v, xr, 3)[None, :]).sum(axis=1)
+        assert np.abs(r[free]).max() < 1e-14 * np.abs(eta).max()
+    # level l from l+1: the parent function is the L2 projection of the children's
+    for l in range(len(tabs) - 1):
+        n = 2 ** l
+        par = tabs[l].reshape(n, n, n, 27)
+        chi = tabs[l + 1].reshape(2 * n, 2 * n, 2 * n, 27)
+        # parent coefficients (projection of a Q1 function is itself)
+        cp = np.stack([(par * wr * _q1_psi(v, xr, 3)).sum(-1) for v in range(8)], -1)
+        for cz in range(n):
+            for cy in range(n):
+                for cx in range(n):
+                    res = np.zeros(8)
+                    for k in range(8):
+                        kk = np.array([k & 1, (k >> 1) & 1, (k >> 2) & 1])
+                        xp = 0.5 * (xr + kk)  # child qp in parent coordinates
+                        fpar = sum(cp[cz, cy, cx, v] * _q1_phi(v, xp, 3) for v in range(8))
+                        fch = chi[2 * cz + kk[2], 2 * cy + kk[1], 2 * cx + kk[0]]
+                        for v in range(8):
+                            res[v] += (wr * (fch - fpar) * _q1_phi(v, xp, 3)).sum() / 8
+                    clamped = np.any(par[cz, cy, cx] <= eta.min()) or np.any(par[cz, cy, cx] >= eta.max()) or any(
+                        np.any(chi[2 * cz + a, 2 * cy + b, 2 * cx + c] <= eta.min()) or
+                        np.any(chi[2 * cz + a, 2 * cy + b, 2 * cx + c] >= eta.max())
+                        for a in range(2) for b in range(2) for c in range(2))
+                    if not clamped:
+                        assert np.abs(res).max() < 1e-13 * np.abs(eta).max()
+
+
+def test_q1_projection_clamped():
+    # a sharp jump overshoots in the projection: values clamp to the qp range
+    p = O.make_problem(2, 2, (1, 1), 4, (1, 1, 1), (1, 1, 1, 1, 0, 0))
+    x = O.qp_points(p, 3)
+    eta = np.where(x[..., 0] + 0.3 * x[..., 1] < 0.47, 1e3, 1.0)
+    tabs = O.level_viscosity_q1(p, eta)
+    for t in tabs:
+        assert t.min() >= 1.0 and t.max() <= 1e3
+    assert tabs[-1].min() == 1.0 and tabs[-1].max() == 1e3  # both bounds reached
+    # and the DGQ1 solve runs end to end
+    rhs = O.rhs_buoyancy(p, 1 + (eta > 1), np.array([0, -1.0]))
+    sol, res = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-8, level_eta="DGQ1"))
+    assert res["converged"]
+    r = rhs - O.stokes_vmult(p, 3, tabs[-1], sol)
+    assert np.linalg.norm(r) <= 1.01 * res["solver_tolerance"]
