@@ -129,8 +129,25 @@ struct GMG {
   std::vector<std::unique_ptr<GMGLevel<T>>> lv;
   DBuf<double> *dot_work = nullptr, *dot_out = nullptr;
   int top = 0;
+  // V-cycles replayed from a captured hipGraph (one per kind); the capture
+  // holds kernel arguments (buffers, Chebyshev coefficients), so it is dropped
+  // whenever those change
+  bool graphs = true;
+  hipGraphExec_t vexec[2] = {nullptr, nullptr};
+  void drop_graphs() {
+    for (hipGraphExec_t &e : vexec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  }
+  GMG() = default;
+  GMG(const GMG &) = delete;
+  GMG &operator=(const GMG &) = delete;
+  ~GMG() { drop_graphs(); }
 
   void setup(const mfs_box &b, hipStream_t s, DBuf<double> *work, DBuf<double> *out) {
+    drop_graphs();
     st = s;
     dot_work = work;
     dot_out = out;
@@ -160,6 +177,7 @@ struct GMG {
 
   // DGQ0 level viscosity: finest from the active table, coarser by averaging children
   void set_eta(const double *active, int active_per_qp) {
+    drop_graphs();
     eta_pq = 0;
     for (auto &g : lv)
       if (g->eta.n != g->L.ntiles_total * cells_per_tile_of(g->L)) g->eta.alloc(g->L.ntiles_total * cells_per_tile_of(g->L));
@@ -173,6 +191,7 @@ struct GMG {
 
   // DGQ1 level viscosity: per-qp tables from the Q1 coefficients of every level
   void set_eta_q1(const std::vector<DBuf<double>> &coef, double lo, double hi) {
+    drop_graphs();
     eta_pq = 1;
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
@@ -289,6 +308,7 @@ struct GMG {
 
   // diagonals (build_preconditioner) and Chebyshev set-up (solve() :771-849)
   void build() {
+    drop_graphs();
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
       launch_cell_op<T>(OP_ADIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, eta_pq, nullptr, nullptr, g.dinvA.p,
@@ -361,7 +381,26 @@ struct GMG {
     GMGLevel<T> &g = *lv[top];
     const long long n = kind == 0 ? g.nu : g.np;
     v_cast<double, T>(src, kind == 0 ? g.ub.p : g.pb.p, n, st);
-    vcycle_level(kind, top);
+    if (graphs) {
+      if (!vexec[kind]) {
+        hipGraph_t gr = nullptr;
+        MFS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+          vcycle_level(kind, top);
+        } catch (...) {
+          (void)hipStreamEndCapture(st, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        MFS_HIP(hipStreamEndCapture(st, &gr));
+        const hipError_t e = hipGraphInstantiate(&vexec[kind], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        MFS_HIP(e);
+      }
+      MFS_HIP(hipGraphLaunch(vexec[kind], st));
+    } else {
+      vcycle_level(kind, top);
+    }
     v_cast<T, double>(kind == 0 ? g.ux.p : g.px.p, dst, n, st);
   }
 };
@@ -1382,6 +1421,32 @@ static void assemble_rhs_dev(mfs_handle *h, const double *d_rho_nat, const doubl
   MFS_HIP(hipStreamSynchronize(h->st));
 }
 
+// correct_stokes_rhs (local_smoothing.cc:617-754): rhs += -A(u0), the Stokes
+// operator applied to u0 with its constrained values read as stored; the
+// constrained velocity rows of rhs are left as they are.
+void correct_rhs_dev(mfs_handle *h, const double *d_u0, double *d_rhs) {
+  need_eta(h);
+  LevelDesc Lp = h->Ltop();
+  Lp.plain_read = 1;
+  launch_cell_op<double>(OP_STOKES, MODE_SET, Lp, d_u0, d_u0 + h->nu, h->eta_active.p, h->active_per_qp, nullptr,
+                         nullptr, h->w1.p, h->w1.p + h->nu, h->st);
+  v_zero_constrained<double>(h->Ltop(), h->w1.p, h->st);
+  v_axpby<double>(-1.0, h->w1.p, 1.0, d_rhs, h->n, h->st);
+  MFS_HIP(hipStreamSynchronize(h->st));
+}
+
+int mfs_correct_stokes_rhs(mfs_handle *h, const double *u0, double *rhs) {
+  return guard([&] {
+    if (!u0 || !rhs) throw std::runtime_error("mfs_correct_stokes_rhs: null argument");
+    HostIO io{h};
+    io.in(h->w2.p, u0, h->n);
+    io.in(h->w3.p, rhs, h->n);
+    correct_rhs_dev(h, h->w2.p, h->w3.p);
+    io.out(rhs, h->w3.p, h->n);
+    return MFS_OK;
+  });
+}
+
 int mfs_assemble_buoyancy_rhs(mfs_handle *h, const double *rho_qp, const double *gravity, double *rhs) {
   return guard([&] {
     if (!rho_qp || !rhs) throw std::runtime_error("mfs_assemble_buoyancy_rhs: null argument");
@@ -1448,6 +1513,16 @@ int mfs_time_operator(mfs_handle *h, int which, int level, const void *d_src, vo
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *avg_seconds = ms * 1e-3 / reps;
+    return MFS_OK;
+  });
+}
+
+int mfs_set_graph_capture(mfs_handle *h, int enable) {
+  return guard([&] {
+    if (!h) throw std::runtime_error("mfs_set_graph_capture: null handle");
+    h->g32.drop_graphs();
+    h->g64.drop_graphs();
+    h->g32.graphs = h->g64.graphs = enable != 0;
     return MFS_OK;
   });
 }

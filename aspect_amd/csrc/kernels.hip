@@ -269,7 +269,7 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
       const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
       const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
       T v = 0;
-      if (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2] && !is_con(L, comp, gx, gy, gz))
+      if (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2] && (L.plain_read || !is_con(L, comp, gx, gy, gz)))
         v = src_u[comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz)];
       nb[t] = v;
     }

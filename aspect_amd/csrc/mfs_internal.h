@@ -38,6 +38,7 @@ struct LevelDesc {
   int bc[6];      // 0 open, 1 no normal flux, 2 zero velocity
   int compressible;
   double gravity[3];  // used by OP_RHS only
+  int plain_read;     // 1: constrained velocity DoFs are read as stored (read_dof_values_plain)
 };
 
 // Tile shapes (cells) per (dim, degree); one workgroup per tile, one thread

@@ -35,7 +35,7 @@ EXPORTED = [
     "mfs_restrict", "mfs_level_viscosity", "mfs_smoother_info", "mfs_chebyshev", "mfs_vcycle",
     "mfs_block_preconditioner", "mfs_solve", "mfs_assemble_buoyancy_rhs", "mfs_stokes_vmult_device",
     "mfs_ablock_vmult_device", "mfs_solve_device", "mfs_assemble_buoyancy_rhs_device", "mfs_synchronize",
-    "mfs_memory_consumption", "mfs_time_operator",
+    "mfs_memory_consumption", "mfs_time_operator", "mfs_set_graph_capture", "mfs_correct_stokes_rhs",
 ]
 
 
@@ -339,6 +339,17 @@ class StokesMatrixFreeHandler:
 
     def synchronize(self):
         _check(self.lib.mfs_synchronize(self._h))
+
+    def correct_stokes_rhs(self, rhs, u0):
+        """rhs - A(u0) for boundary values u0 on the constrained DoFs (correct_stokes_rhs)."""
+        out = np.array(rhs, dtype=np.float64, copy=True)
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        _check(self.lib.mfs_correct_stokes_rhs(self._h, _ptr(u0), _ptr(out)))
+        return out
+
+    def set_graph_capture(self, enable: bool):
+        """Replay V-cycles from a captured hipGraph (default) or launch them eagerly."""
+        _check(self.lib.mfs_set_graph_capture(self._h, int(bool(enable))))
 
     def __del__(self):
         try:

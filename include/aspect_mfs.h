@@ -172,6 +172,12 @@ int mfs_solve(mfs_handle *h, const double *rhs, const double *x0, double *sol, c
  * quadrature point of the active level ([cell][q]); gravity: dim components. */
 int mfs_assemble_buoyancy_rhs(mfs_handle *h, const double *rho_qp, const double *gravity, double *rhs);
 
+/* StokesMatrixFreeHandler::correct_stokes_rhs (stokes_matrix_free_local_smoothing.cc:617-754):
+ * rhs += -A(u0) where u0 holds the boundary values on constrained velocity
+ * DoFs (zero elsewhere), read as stored; constrained rows of rhs unchanged.
+ * rhs and u0: N_u + N_p values. */
+int mfs_correct_stokes_rhs(mfs_handle *h, const double *u0, double *rhs);
+
 /* Device-resident variants: all pointers are device pointers. */
 int mfs_stokes_vmult_device(mfs_handle *h, const double *d_src, double *d_dst);
 /* Level A-block apply in the GMG precision (float or double device arrays). */
@@ -180,6 +186,10 @@ int mfs_solve_device(mfs_handle *h, const double *d_rhs, const double *d_x0, dou
                      const mfs_solve_params *prm, mfs_solve_result *res);
 int mfs_assemble_buoyancy_rhs_device(mfs_handle *h, const double *d_rho_qp, const double *gravity, double *d_rhs);
 int mfs_synchronize(mfs_handle *h);
+/* V-cycles are captured once into a hipGraph and replayed (default on); 0
+ * launches them kernel by kernel.  No counterpart in the reference (deal.II
+ * runs the V-cycle on the host). */
+int mfs_set_graph_capture(mfs_handle *h, int enable);
 
 /* "Execute solver timings" harness (local_smoothing.cc:1119-1258): average
  * device time of reps applications of an operator, bracketed by hipEvents on

@@ -247,6 +247,15 @@ def level_viscosity(p, eta):
     return res
 
 
+def stokes_rhs_correction(p, eta, u0):
+    """-A(u0) with constrained values of u0 read as stored (correct_stokes_rhs)."""
+    eta, per_qp = _eta(eta)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    out = np.zeros_like(u0)
+    _check(lib().orc_stokes_rhs_correction(C.byref(p), _p(eta), per_qp, _p(u0), _p(out)))
+    return out
+
+
 def level_viscosity_q1(p, eta_qp):
     """Per-quadrature-point viscosity tables of every level (level 0 first) for
     "project to Q1 only viscosity"; the last entry is the active table."""

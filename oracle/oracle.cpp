@@ -375,6 +375,7 @@ struct OpInput {
   int eta_per_qp;
   double s;
   int compressible;
+  int plain = 0;       // read constrained DoFs as stored (read_dof_values_plain)
 };
 
 // which ops to do in the cell loop
@@ -409,7 +410,7 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
           for (int cmp = 0; cmp < dim; ++cmp)
             for (int a = 0; a < nv; ++a) {
               size_t g = (size_t)cmp * L.vnodes + vidx[a];
-              u[cmp * nv + a] = L.vcon[g] ? 0.0 : src_u[g];
+              u[cmp * nv + a] = (L.vcon[g] && !in.plain) ? 0.0 : src_u[g];
             }
         if (need_p)
           for (int a = 0; a < npl; ++a) pl[a] = src_p[pidx[a]];
@@ -486,6 +487,18 @@ void stokes_vmult(const Context &C, int l, const double *eta, int per_qp, const 
   cell_loop(in, C.K, OP_STOKES, src, src + nu, dst, dst + nu);
   for (long long i = 0; i < nu; ++i)
     if (L.vcon[i]) dst[i] = src[i];
+}
+// correct_stokes_rhs (stokes_matrix_free_local_smoothing.cc:617-754): the
+// Stokes operator applied to u0 with constrained values read as stored, negated;
+// constrained velocity rows receive nothing (integrate_scatter drops them).
+void stokes_rhs_correction(const Context &C, int l, const double *eta, int per_qp, const double *u0, double *dst) {
+  const Level &L = C.levels[l];
+  const long long nu = L.nu(), n = nu + L.pnodes;
+  std::fill(dst, dst + n, 0.0);
+  OpInput in{&L, eta, per_qp, C.p.pressure_scaling, C.p.compressible};
+  in.plain = 1;
+  cell_loop(in, C.K, OP_STOKES, u0, u0 + nu, dst, dst + nu);
+  for (long long i = 0; i < n; ++i) dst[i] = -dst[i];
 }
 void ablock_vmult(const Context &C, int l, const double *eta, int per_qp, const double *src, double *dst) {
   const Level &L = C.levels[l];
@@ -1676,6 +1689,14 @@ int orc_chebyshev(const orc_problem *p, const double *eta, int per_qp, int kind,
     Vec b(src, src + mg.size(level)), x(mg.size(level), 0.0);
     mg.smoothers[level].run(x, b, true);
     std::memcpy(dst, x.data(), x.size() * sizeof(double));
+  })
+}
+
+// rhs correction -A(u0) of correct_stokes_rhs on the active level
+int orc_stokes_rhs_correction(const orc_problem *p, const double *eta, int per_qp, const double *u0, double *out) {
+  ORC_TRY({
+    Context C(*p);
+    stokes_rhs_correction(C, p->n_levels - 1, eta, per_qp, u0, out);
   })
 }
 

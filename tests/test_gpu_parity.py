@@ -291,3 +291,42 @@ def test_q1_projection_tables_and_solve(prec):
     assert out.converged and res["converged"]
     assert abs(out.cheap_iterations - res["cheap_iterations"]) <= max(1, 0.1 * res["cheap_iterations"])
     assert rel(sol, ref) < 1e-6
+
+
+def test_vcycle_graph_replay_matches_eager():
+    # captured V-cycles replayed several times, with eager work between the
+    # replays, give the eager result bit for bit (both precisions)
+    case = CASES[2]
+    for prec in ("fp64", "fp32"):
+        p, h, eta, sz, rng = setup_pair(*case, prec=prec)
+        h.build_preconditioner()
+        u = rng.normal(size=sz["nu"])
+        q = rng.normal(size=sz["np"])
+        h.set_graph_capture(False)
+        ea, es = h.vcycle(0, u), h.vcycle(1, q)
+        h.set_graph_capture(True)
+        for _ in range(3):
+            ga, gs = h.vcycle(0, u), h.vcycle(1, q)
+            assert np.array_equal(ga, ea) and np.array_equal(gs, es)
+            h.stokes_vmult(rng.normal(size=sz["nu"] + sz["np"]))
+        # a rebuilt preconditioner drops the captured graphs
+        h.assemble(np.exp(rng.normal(size=(sz["cells"], O.n_qp(p)))), "none")
+        h.build_preconditioner()
+        h.set_graph_capture(False)
+        ea2 = h.vcycle(0, u)
+        h.set_graph_capture(True)
+        assert np.array_equal(h.vcycle(0, u), ea2) and not np.array_equal(ea2, ea)
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[4]])
+def test_correct_stokes_rhs(case):
+    # correct_stokes_rhs (local_smoothing.cc:617-754) with boundary values on the constrained DoFs
+    p, h, eta, sz, rng = setup_pair(*case, s=1.3)
+    con = O.constrained(p, case[3] - 1)
+    u0 = np.zeros(sz["nu"] + sz["np"])
+    u0[:sz["nu"]][con] = rng.normal(size=con.sum())
+    rhs = rng.normal(size=u0.size)
+    got = h.correct_stokes_rhs(rhs, u0)
+    ref = rhs + O.stokes_rhs_correction(p, eta, u0)
+    assert rel(got, ref) < FP64
+    assert np.array_equal(got[:sz["nu"]][con], rhs[:sz["nu"]][con])

@@ -410,3 +410,25 @@ def test_q1_projection_clamped():
     assert res["converged"]
     r = rhs - O.stokes_vmult(p, 3, tabs[-1], sol)
     assert np.linalg.norm(r) <= 1.01 * res["solver_tolerance"]
+
+
+@pytest.mark.parametrize("dim,deg,nl,coarse,bc", CASES[:2])
+def test_stokes_rhs_correction_matches_assembly(dim, deg, nl, coarse, bc):
+    # correct_stokes_rhs (local_smoothing.cc:617-754): -A u0 for u0 holding
+    # boundary values on constrained DoFs, constrained rows left untouched
+    p = O.make_problem(dim, deg, coarse, nl, (2.0, 1.0, 0.5), bc, pressure_scaling=1.7)
+    lev = nl - 1
+    s = O.sizes(p, lev)
+    rng = np.random.default_rng(11)
+    eta = np.exp(rng.normal(size=(s["cells"], O.n_qp(p))))
+    A, _ = assemble(p, lev, eta)
+    con = O.constrained(p, lev)
+    u0 = np.zeros(s["nu"] + s["np"])
+    u0[:s["nu"]][con] = rng.normal(size=con.sum())
+    corr = O.stokes_rhs_correction(p, eta, u0)
+    ref = -(A @ u0)
+    ref[:s["nu"]][con] = 0.0
+    assert np.linalg.norm(corr - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert np.all(corr[:s["nu"]][con] == 0.0)
+    # homogeneous boundary values: no correction
+    assert np.all(O.stokes_rhs_correction(p, eta, np.zeros_like(u0)) == 0.0)
