@@ -4,8 +4,8 @@ The HIP library ``libaspect_mfs.so`` holds every kernel and the solver; this
 package is the Python mirror of the StokesMatrixFreeHandler interface on top
 of its C-ABI (``include/aspect_mfs.h``).
 """
-from .mfs import (AVERAGE, BC_OPEN, BC_TANGENTIAL, BC_ZERO, Box, NoConvergence, SolverError, SolverOutputs,
-                  StokesMatrixFreeHandler, load_library, make_box)
+from .mfs import (AVERAGE, BC_OPEN, BC_TANGENTIAL, BC_ZERO, Box, Ensemble, NoConvergence, SolverError,
+                  SolverOutputs, StokesMatrixFreeHandler, load_library, make_box)
 
-__all__ = ["AVERAGE", "BC_OPEN", "BC_TANGENTIAL", "BC_ZERO", "Box", "NoConvergence", "SolverError",
+__all__ = ["AVERAGE", "BC_OPEN", "BC_TANGENTIAL", "BC_ZERO", "Box", "Ensemble", "NoConvergence", "SolverError",
            "SolverOutputs", "StokesMatrixFreeHandler", "load_library", "make_box"]

@@ -19,10 +19,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/aspect_mfs.h"
@@ -1105,15 +1108,19 @@ int mfs_set_pressure_scaling(mfs_handle *h, double s) {
   });
 }
 
+void build_prec(mfs_handle *h) {
+  need_eta(h);
+  if (h->prec == MFS_GMG_FP32)
+    h->g32.build();
+  else
+    h->g64.build();
+  MFS_HIP(hipStreamSynchronize(h->st));
+  h->have_prec = true;
+}
+
 int mfs_build_preconditioner(mfs_handle *h) {
   return guard([&] {
-    need_eta(h);
-    if (h->prec == MFS_GMG_FP32)
-      h->g32.build();
-    else
-      h->g64.build();
-    MFS_HIP(hipStreamSynchronize(h->st));
-    h->have_prec = true;
+    build_prec(h);
     return MFS_OK;
   });
 }
@@ -1555,3 +1562,100 @@ int mfs_memory_consumption(const mfs_handle *h, long long out[3]) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ ensembles
+// Independent problems of uneven sizes solved together: one handle (and HIP
+// stream) per problem, driven by host worker threads, so that the kernels of
+// many small solves run concurrently on the device.
+struct mfs_ensemble {
+  std::vector<mfs_handle *> members;
+  ~mfs_ensemble() {
+    for (mfs_handle *h : members) delete h;
+  }
+};
+
+namespace {
+// run f(i) for i < n on up to n_threads workers; the first error is rethrown
+template <typename F>
+void run_parallel(int n, int n_threads, F f) {
+  std::atomic<int> next{0};
+  std::string first_error;
+  std::atomic<bool> failed{false};
+  auto worker = [&]() {
+    for (int i = next++; i < n; i = next++) {
+      try {
+        f(i);
+      } catch (const std::exception &e) {
+        if (!failed.exchange(true)) first_error = "ensemble member " + std::to_string(i) + ": " + e.what();
+      }
+    }
+  };
+  const int nt = std::max(1, std::min(n_threads, n));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (std::thread &t : pool) t.join();
+  if (failed) throw std::runtime_error(first_error);
+}
+}  // namespace
+
+int mfs_ensemble_create(int n, const mfs_box *boxes, int gmg_precision, mfs_ensemble **out) {
+  return guard([&] {
+    if (n < 1 || !boxes || !out) throw std::runtime_error("mfs_ensemble_create: bad arguments");
+    auto e = std::make_unique<mfs_ensemble>();
+    e->members.assign(n, nullptr);
+    for (int i = 0; i < n; ++i) create(&boxes[i], gmg_precision, &e->members[i]);
+    *out = e.release();
+    return MFS_OK;
+  });
+}
+
+int mfs_ensemble_destroy(mfs_ensemble *e) {
+  return guard([&] {
+    delete e;
+    return MFS_OK;
+  });
+}
+
+int mfs_ensemble_member(mfs_ensemble *e, int i, mfs_handle **h) {
+  return guard([&] {
+    if (!e || !h || i < 0 || i >= (int)e->members.size()) throw std::runtime_error("mfs_ensemble_member: bad index");
+    *h = e->members[i];
+    return MFS_OK;
+  });
+}
+
+int mfs_ensemble_setup(mfs_ensemble *e, const double *const *eta_qp, int averaging, int n_threads) {
+  return guard([&] {
+    if (!e || !eta_qp) throw std::runtime_error("mfs_ensemble_setup: null argument");
+    run_parallel((int)e->members.size(), n_threads, [&](int i) {
+      set_viscosity(e->members[i], eta_qp[i], averaging);
+      build_prec(e->members[i]);
+    });
+    return MFS_OK;
+  });
+}
+
+int mfs_ensemble_solve(mfs_ensemble *e, const double *const *rhs, double *const *sol, const mfs_solve_params *prm,
+                       mfs_solve_result *res, int n_threads, double *seconds) {
+  return guard([&] {
+    if (!e || !rhs || !sol || !prm || !res) throw std::runtime_error("mfs_ensemble_solve: null argument");
+    const int n = (int)e->members.size();
+    std::vector<int> rc(n, MFS_OK);
+    for (mfs_handle *h : e->members) MFS_HIP(hipStreamSynchronize(h->st));
+    const auto t0 = std::chrono::steady_clock::now();
+    run_parallel(n, n_threads, [&](int i) {
+      mfs_handle *h = e->members[i];
+      DBuf<double> drhs, dsol;
+      drhs.alloc(h->n);
+      dsol.alloc(h->n);
+      MFS_HIP(hipMemcpyAsync(drhs.p, rhs[i], h->n * sizeof(double), hipMemcpyHostToDevice, h->st));
+      rc[i] = h->solve_dev(drhs.p, nullptr, dsol.p, *prm, res[i]);
+      HostIO{h}.out(sol[i], dsol.p, h->n);
+    });
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < n; ++i)
+      if (rc[i] != MFS_OK) return MFS_NO_CONVERGENCE;
+    return MFS_OK;
+  });
+}

@@ -36,6 +36,7 @@ EXPORTED = [
     "mfs_block_preconditioner", "mfs_solve", "mfs_assemble_buoyancy_rhs", "mfs_stokes_vmult_device",
     "mfs_ablock_vmult_device", "mfs_solve_device", "mfs_assemble_buoyancy_rhs_device", "mfs_synchronize",
     "mfs_memory_consumption", "mfs_time_operator", "mfs_set_graph_capture", "mfs_correct_stokes_rhs",
+    "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
 ]
 
 
@@ -351,10 +352,75 @@ class StokesMatrixFreeHandler:
         """Replay V-cycles from a captured hipGraph (default) or launch them eagerly."""
         _check(self.lib.mfs_set_graph_capture(self._h, int(bool(enable))))
 
+    @classmethod
+    def _member(cls, h, box: Box, gmg_precision: str):
+        """Non-owning view of an ensemble member's handle."""
+        obj = cls.__new__(cls)
+        obj.lib = load_library()
+        obj.box = box
+        obj.gmg_precision = gmg_precision
+        obj._h = h
+        obj._averaging = 0
+        obj._owned = False
+        return obj
+
     def __del__(self):
         try:
-            if self._h:
+            if self._h and getattr(self, "_owned", True):
                 self.lib.mfs_destroy(self._h)
                 self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Independent problems of uneven sizes solved together (one handle and
+    HIP stream per problem, host worker threads)."""
+
+    def __init__(self, boxes, gmg_precision: str = "fp32"):
+        self.lib = load_library()
+        self.boxes = list(boxes)
+        self.gmg_precision = gmg_precision
+        arr = (Box * len(self.boxes))(*self.boxes)
+        self._e = C.c_void_p()
+        prec = GMG_FP32 if gmg_precision == "fp32" else GMG_FP64
+        _check(self.lib.mfs_ensemble_create(len(self.boxes), arr, prec, C.byref(self._e)))
+        self.members = []
+        for i, b in enumerate(self.boxes):
+            h = C.c_void_p()
+            _check(self.lib.mfs_ensemble_member(self._e, i, C.byref(h)))
+            self.members.append(StokesMatrixFreeHandler._member(h, b, gmg_precision))
+
+    def setup(self, etas, averaging="harmonic", n_threads=16):
+        etas = [np.ascontiguousarray(e, dtype=np.float64) for e in etas]
+        ptrs = (C.c_void_p * len(etas))(*[e.ctypes.data for e in etas])
+        mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
+        _check(self.lib.mfs_ensemble_setup(self._e, ptrs, mode, n_threads))
+        for m in self.members:
+            m._averaging = mode
+
+    def solve(self, rhss, tolerance=1e-8, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
+              a_block_tolerance=1e-2, s_block_tolerance=1e-6, krylov="GMRES", idr_s=2, n_threads=16):
+        rhss = [np.ascontiguousarray(r, dtype=np.float64) for r in rhss]
+        sols = [np.zeros_like(r) for r in rhss]
+        rp = (C.c_void_p * len(rhss))(*[r.ctypes.data for r in rhss])
+        sp = (C.c_void_p * len(sols))(*[x.ctypes.data for x in sols])
+        prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
+                          s_block_tolerance, KRYLOV[krylov], idr_s)
+        res = (SolveResult * len(rhss))()
+        secs = C.c_double(0)
+        _check(self.lib.mfs_ensemble_solve(self._e, rp, sp, C.byref(prm), res, n_threads, C.byref(secs)))
+        outs = [SolverOutputs(r.initial_nonlinear_residual, r.final_linear_residual,
+                              r.pressure_normalization_adjustment, r.cheap_iterations, r.expensive_iterations,
+                              r.a_preconditioner_applications, r.s_preconditioner_applications, bool(r.converged),
+                              r.solver_tolerance, r.r0_norm, r.solve_seconds) for r in res]
+        return sols, outs, secs.value
+
+    def __del__(self):
+        try:
+            if self._e:
+                self.members = []
+                self.lib.mfs_ensemble_destroy(self._e)
+                self._e = C.c_void_p()
         except Exception:
             pass

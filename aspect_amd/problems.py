@@ -141,6 +141,28 @@ def config_sinker_128(**kw):
                      meta=dict(n_sinkers=16, contrast=1e6, seed=3), **kw)
 
 
+def config_ensemble(n_problems=48, seed=4, **kw):
+    """Ensemble of independent 2D problems: unit square, n x n cells with
+    n = 16 m, m ~ U{1..16} (coarse mesh = odd part of n, so every problem has a
+    multigrid hierarchy); 1-8 sinkers (radius 0.05, width 0.01), viscosity
+    contrast 10^U(2,5), density 1 + a chi with a ~ U(0.001, 0.1); free slip;
+    FP32 V-cycle."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_problems):
+        n = 16 * int(rng.integers(1, 17))
+        c = n
+        while c % 2 == 0:
+            c //= 2
+        levels = int(np.log2(n // c)) + 1
+        meta = dict(n_sinkers=int(rng.integers(1, 9)), contrast=float(10 ** rng.uniform(2, 5)),
+                    amplitude=float(rng.uniform(0.001, 0.1)), seed=int(rng.integers(0, 2 ** 31)), n=n)
+        kw2 = dict(gmg_precision="fp32", averaging="harmonic")
+        kw2.update(kw)
+        out.append(BoxConfig(f"ens_{i}", 2, (c, c), levels, (BC_TANGENTIAL,) * 4, meta=meta, **kw2))
+    return out
+
+
 def material(cfg: BoxConfig, x):
     """(eta, rho) at points x [..., dim] for a configuration."""
     if cfg.name.startswith("2d_box"):
@@ -156,6 +178,13 @@ def material(cfg: BoxConfig, x):
         return eta, rho
     if cfg.name.startswith("nsinker"):
         return nsinker_material(x, cfg.meta.get("viscosity_ratio", 1e4), cfg.meta.get("sinker_density", 10.0))
+    if cfg.name.startswith("ens_"):
+        rng = np.random.default_rng(cfg.meta["seed"])
+        centres = rng.uniform(0.1, 0.9, size=(cfg.meta["n_sinkers"], cfg.dim))
+        chi = smooth_sinkers(x, centres)
+        eta = 1.0 + (cfg.meta["contrast"] - 1.0) * chi
+        rho = 1.0 + cfg.meta["amplitude"] * chi
+        return eta, rho
     raise ValueError(cfg.name)
 
 

@@ -204,6 +204,23 @@ int mfs_time_operator(mfs_handle *h, int which, int level, const void *d_src, vo
  * bytes of device memory: out[0] vectors/levels, out[1] cell data, out[2] Krylov space. */
 int mfs_memory_consumption(const mfs_handle *h, long long out[3]);
 
+/* Ensembles of independent problems (uneven sizes) solved together: one
+ * handle and HIP stream per problem, host worker threads drive the solves so
+ * that the kernels of many small problems overlap on the device.  No
+ * counterpart in the reference (one ASPECT run solves one problem). */
+typedef struct mfs_ensemble mfs_ensemble;
+int mfs_ensemble_create(int n, const mfs_box *boxes, int gmg_precision, mfs_ensemble **out);
+int mfs_ensemble_destroy(mfs_ensemble *e);
+/* Handle of member i (owned by the ensemble): any per-handle call applies. */
+int mfs_ensemble_member(mfs_ensemble *e, int i, mfs_handle **h);
+/* mfs_set_viscosity + mfs_build_preconditioner on every member. */
+int mfs_ensemble_setup(mfs_ensemble *e, const double *const *eta_qp, int averaging, int n_threads);
+/* mfs_solve on every member (zero initial guess); results[i] per member;
+ * *seconds = wall time of the whole ensemble solve.  Returns MFS_NO_CONVERGENCE
+ * if any member did not converge. */
+int mfs_ensemble_solve(mfs_ensemble *e, const double *const *rhs, double *const *sol, const mfs_solve_params *prm,
+                       mfs_solve_result *results, int n_threads, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

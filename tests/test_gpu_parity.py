@@ -330,3 +330,29 @@ def test_correct_stokes_rhs(case):
     ref = rhs + O.stokes_rhs_correction(p, eta, u0)
     assert rel(got, ref) < FP64
     assert np.array_equal(got[:sz["nu"]][con], rhs[:sz["nu"]][con])
+
+
+def test_ensemble_matches_single_solves():
+    # independent problems of uneven sizes solved together (one stream each)
+    from aspect_amd import Ensemble
+    cfgs = P.config_ensemble(n_problems=5, seed=7)
+    for c in cfgs:  # keep the CPU oracle cheap: at most 64 x 64 cells
+        while c.n_cells[0] > 64:
+            c.n_levels -= 1
+    fields = [P.fields(c) for c in cfgs]
+    ens = Ensemble([c.box(f[2]) for c, f in zip(cfgs, fields)], "fp32")
+    ens.setup([f[0] for f in fields], "harmonic")
+    rhss = [m.assemble_buoyancy_rhs(f[1], c.gravity()) for m, c, f in zip(ens.members, cfgs, fields)]
+    sols, outs, secs = ens.solve(rhss, tolerance=1e-8)
+    assert secs > 0 and all(o.converged for o in outs)
+    for c, f, rhs, sol, out in zip(cfgs, fields, rhss, sols, outs):
+        h = StokesMatrixFreeHandler(c.box(f[2]), "fp32")
+        h.assemble(f[0], "harmonic")
+        h.build_preconditioner()
+        ref, o1 = h.solve(rhs, tolerance=1e-8)
+        assert np.array_equal(sol, ref) and o1.cheap_iterations == out.cheap_iterations
+        p = O.make_problem(2, 2, c.coarse, c.n_levels, (1, 1, 1), c.bc, pressure_scaling=f[2])
+        eta_h = 1.0 / np.mean(1.0 / f[0], axis=1)
+        ro, res = O.solve(p, eta_h, rhs, params=O.default_params(tolerance=1e-8))
+        assert abs(out.cheap_iterations - res["cheap_iterations"]) <= max(1, 0.1 * res["cheap_iterations"])
+        assert rel(sol, ro) < 1e-6
