@@ -14,7 +14,7 @@
 // Numerics restated (not copied) from the published algorithms: Jacobi
 // preconditioned Chebyshev semi-iteration with a CG-Lanczos eigenvalue
 // estimate, V-cycle multigrid, right-preconditioned restarted GMRES / flexible
-// GMRES with classical Gram-Schmidt and one reorthogonalization pass, and CG.
+// GMRES with classical Gram-Schmidt (second pass when the first cancelled), and CG.
 
 #include <hip/hip_runtime.h>
 
@@ -562,6 +562,8 @@ struct mfs_handle {
     MFS_HIP(hipStreamSynchronize(st));
   }
 
+  static constexpr double REORTH = 0.70710678118654752;  // reorthogonalize when |w'| < |w| / sqrt(2)
+
   struct KrylovStatus {
     bool converged = false;
     int its = 0;
@@ -600,10 +602,17 @@ struct mfs_handle {
         double *w = vec(vchunks, j + 1);
         stokes(MODE_SET, z, nullptr, w);
         std::vector<double> col(j + 2, 0.0);
+        // classical Gram-Schmidt, repeated once when the pass cancelled more
+        // than 1 - 1/sqrt(2) of the norm (Daniel-Gragg-Kaufman-Stewart test)
+        const double wn0 = std::sqrt(dot(w, w, n));
         cgs_pass(j + 1, w, h);
-        cgs_pass(j + 1, w, hh);
-        for (int i = 0; i <= j; ++i) col[i] = h[i] + hh[i];
-        const double hn = std::sqrt(dot(w, w, n));
+        for (int i = 0; i <= j; ++i) col[i] = h[i];
+        double hn = std::sqrt(dot(w, w, n));
+        if (hn < REORTH * wn0) {
+          cgs_pass(j + 1, w, hh);
+          for (int i = 0; i <= j; ++i) col[i] += hh[i];
+          hn = std::sqrt(dot(w, w, n));
+        }
         col[j + 1] = hn;
         for (int i = 0; i < j; ++i) {
           const double a = col[i], bb = col[i + 1];

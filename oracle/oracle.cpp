@@ -1149,8 +1149,8 @@ struct SolveStatus {
 };
 // Restarted GMRES with right preconditioning (non-flexible: x += P(V y) at each
 // restart) or FGMRES (flexible: x += Z y).  Convergence on the unpreconditioned
-// residual estimate from the Givens rotations.  Classical Gram-Schmidt with one
-// reorthogonalization pass.
+// residual estimate from the Givens rotations.  Classical Gram-Schmidt with a
+// second pass when the first cancelled more than 1 - 1/sqrt(2) of the norm.
 SolveStatus gmres(const Op &A, const Op &P, Vec &x, const Vec &b, double tol, int maxit, int restart,
                   bool flexible) {
   SolveStatus st;
@@ -1186,12 +1186,17 @@ SolveStatus gmres(const Op &A, const Op &P, Vec &x, const Vec &b, double tol, in
       A(z.data(), w.data());
       if (flexible) Z.push_back(z);
       Vec h(j + 2, 0.0);
+      // classical Gram-Schmidt; a second pass when the first cancelled more
+      // than 1 - 1/sqrt(2) of the norm (Daniel-Gragg-Kaufman-Stewart test)
+      const double wn0 = norm(w);
       for (int pass = 0; pass < 2; ++pass) {
+        Vec hp(j + 1);
+        for (int i = 0; i <= j; ++i) hp[i] = dot(V[i], w);
         for (int i = 0; i <= j; ++i) {
-          double hij = dot(V[i], w);
-          h[i] += hij;
-          for (size_t k = 0; k < n; ++k) w[k] -= hij * V[i][k];
+          h[i] += hp[i];
+          for (size_t k = 0; k < n; ++k) w[k] -= hp[i] * V[i][k];
         }
+        if (pass == 0 && norm(w) >= 0.70710678118654752 * wn0) break;
       }
       h[j + 1] = norm(w);
       for (int i = 0; i < j; ++i) {
