@@ -11,7 +11,8 @@
 // tile form one wave, so (a, b) is uniform over a wave.  The x direction of
 // the sum factorization stays in registers; the y and z directions go through
 // LDS, one exchange array [comp][q][line][cell] per step, three arrays in
-// rotation so that every step needs a single barrier.  Per-thread state is one
+// rotation so that every step needs a single barrier (the first step
+// interpolates in x and y at once, straight from the tile's nodes).  Per-thread state is one
 // line (3 values per component), which keeps register use low enough for
 // several waves per SIMD.  Collocation form: S (Q x N) interpolates nodal
 // values to Gauss points, D (Q x Q) differentiates on the Gauss points
@@ -117,17 +118,21 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   col3(B::S, a, STa);
   col3(B::S, b, STb);
 
-  // ---- 1. x interpolation on the node line (jy, kz) = (a, b); pressure at
-  // the quadrature line (qy, qz) = (a, b)
+  // ---- 1. x and y interpolation: line (qy, kz) = (a, b), from the three node
+  // lines (jy, kz), jy = 0..2, of the tile in LDS; pressure at the quadrature
+  // line (qy, qz) = (a, b)
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
-    T u[3];
+    T vy[3] = {0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      u[i] = sC[comp * C::VN + ((2 * cz + b) * C::VY + (2 * cy + a)) * C::VX + 2 * cx + i];
+    for (int jy = 0; jy < 3; ++jy) {
+      const T *row = sC + comp * C::VN + ((2 * cz + b) * C::VY + (2 * cy + jy)) * C::VX + 2 * cx;
+      const T u0 = row[0], u1 = row[1], u2 = row[2];
 #pragma unroll
-    for (int qx = 0; qx < 3; ++qx)
-      sA[XI(comp, qx, ln)] = B::S[qx][0] * u[0] + B::S[qx][1] * u[1] + B::S[qx][2] * u[2];
+      for (int qx = 0; qx < 3; ++qx) vy[qx] += Sa[jy] * (B::S[qx][0] * u0 + B::S[qx][1] * u1 + B::S[qx][2] * u2);
+    }
+#pragma unroll
+    for (int qx = 0; qx < 3; ++qx) sA[XI(comp, qx, ln)] = vy[qx];
   }
   T pq[3] = {0, 0, 0};
   if constexpr (STOKES) {
@@ -151,27 +156,18 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   }
   __syncthreads();
 
-  // ---- 2. y interpolation: line (qy, kz) = (a, b)
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp)
-#pragma unroll
-    for (int qx = 0; qx < 3; ++qx)
-      sB[XI(comp, qx, ln)] = Sa[0] * sA[XI(comp, qx, 3 * b)] + Sa[1] * sA[XI(comp, qx, 3 * b + 1)] +
-                             Sa[2] * sA[XI(comp, qx, 3 * b + 2)];
-  __syncthreads();
-
-  // ---- 3. z interpolation: quadrature line (qy, qz) = (a, b)
+  // ---- 2. z interpolation: quadrature line (qy, qz) = (a, b)
   T v[3][3];
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx) {
-      v[comp][qx] = Sb[0] * sB[XI(comp, qx, a)] + Sb[1] * sB[XI(comp, qx, a + 3)] + Sb[2] * sB[XI(comp, qx, a + 6)];
-      sA[XI(comp, qx, ln)] = v[comp][qx];
+      v[comp][qx] = Sb[0] * sA[XI(comp, qx, a)] + Sb[1] * sA[XI(comp, qx, a + 3)] + Sb[2] * sA[XI(comp, qx, a + 6)];
+      sB[XI(comp, qx, ln)] = v[comp][qx];
     }
   __syncthreads();
 
-  // ---- 4. gradients, quadrature-point operation, x test; y and z fluxes to LDS
+  // ---- 3. gradients, quadrature-point operation, x test; y and z fluxes to LDS
   const T ih0 = (T)(1.0 / L.h[0]), ih1 = (T)(1.0 / L.h[1]), ih2 = (T)(1.0 / L.h[2]);
   const T s = (T)L.s;
   const double wab = L.h[0] * L.h[1] * L.h[2] * (a == 0 ? BD::W[0] : (a == 1 ? BD::W[1] : BD::W[2])) *
@@ -190,9 +186,9 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
       const T dx = B::D[qx][0] * v[comp][0] + B::D[qx][1] * v[comp][1] + B::D[qx][2] * v[comp][2];
-      const T dy = Da[0] * sA[XI(comp, qx, 3 * b)] + Da[1] * sA[XI(comp, qx, 3 * b + 1)] +
-                   Da[2] * sA[XI(comp, qx, 3 * b + 2)];
-      const T dz = Db[0] * sA[XI(comp, qx, a)] + Db[1] * sA[XI(comp, qx, a + 3)] + Db[2] * sA[XI(comp, qx, a + 6)];
+      const T dy = Da[0] * sB[XI(comp, qx, 3 * b)] + Da[1] * sB[XI(comp, qx, 3 * b + 1)] +
+                   Da[2] * sB[XI(comp, qx, 3 * b + 2)];
+      const T dz = Db[0] * sB[XI(comp, qx, a)] + Db[1] * sB[XI(comp, qx, a + 3)] + Db[2] * sB[XI(comp, qx, a + 6)];
       gr[comp][0] = dx * ih0;
       gr[comp][1] = dy * ih1;
       gr[comp][2] = dz * ih2;
@@ -215,7 +211,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
       const T fx = flux[comp][0] * ih0;
 #pragma unroll
       for (int r = 0; r < 3; ++r) tacc[comp][r] += B::D[qx][r] * fx;
-      sB[XI(comp, qx, ln)] = flux[comp][1] * ih1;
+      sA[XI(comp, qx, ln)] = flux[comp][1] * ih1;
       sC[XI(comp, qx, ln)] = flux[comp][2] * ih2;
     }
     if constexpr (STOKES) {
@@ -230,44 +226,44 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   }
   __syncthreads();
 
-  // ---- 5. y and z tests at the quadrature line (qy, qz) = (a, b)
+  // ---- 4. y and z tests at the quadrature line (qy, qz) = (a, b)
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx)
-      sA[XI(comp, qx, ln)] = tacc[comp][qx] + DTa[0] * sB[XI(comp, qx, 3 * b)] +
-                             DTa[1] * sB[XI(comp, qx, 3 * b + 1)] + DTa[2] * sB[XI(comp, qx, 3 * b + 2)] +
+      sB[XI(comp, qx, ln)] = tacc[comp][qx] + DTa[0] * sA[XI(comp, qx, 3 * b)] +
+                             DTa[1] * sA[XI(comp, qx, 3 * b + 1)] + DTa[2] * sA[XI(comp, qx, 3 * b + 2)] +
                              DTb[0] * sC[XI(comp, qx, a)] + DTb[1] * sC[XI(comp, qx, a + 3)] +
                              DTb[2] * sC[XI(comp, qx, a + 6)];
   __syncthreads();
 
-  // ---- 6. z integration: line (qy, kz) = (a, b)
+  // ---- 5. z integration: line (qy, kz) = (a, b)
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx)
-      sB[XI(comp, qx, ln)] =
-          STb[0] * sA[XI(comp, qx, a)] + STb[1] * sA[XI(comp, qx, a + 3)] + STb[2] * sA[XI(comp, qx, a + 6)];
+      sA[XI(comp, qx, ln)] =
+          STb[0] * sB[XI(comp, qx, a)] + STb[1] * sB[XI(comp, qx, a + 3)] + STb[2] * sB[XI(comp, qx, a + 6)];
   __syncthreads();
 
-  // ---- 7. y integration: node line (jy, kz) = (a, b), then x; per-cell
+  // ---- 6. y integration: node line (jy, kz) = (a, b), then x; per-cell
   // contributions [comp][kz][jy][i][cell]
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     T hyv[3];
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx)
-      hyv[qx] = STa[0] * sB[XI(comp, qx, 3 * b)] + STa[1] * sB[XI(comp, qx, 3 * b + 1)] +
-                STa[2] * sB[XI(comp, qx, 3 * b + 2)];
+      hyv[qx] = STa[0] * sA[XI(comp, qx, 3 * b)] + STa[1] * sA[XI(comp, qx, 3 * b + 1)] +
+                STa[2] * sA[XI(comp, qx, 3 * b + 2)];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-      sA[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] =
+      sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] =
           B::S[0][i] * hyv[0] + B::S[1][i] * hyv[1] + B::S[2][i] * hyv[2];
   }
   __syncthreads();
 
-  // ---- 8. tile write-out
-  tile_write_u<3, 2, OP, MODE, T, LN_THREADS>(L, sA, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+  // ---- 7. tile write-out
+  tile_write_u<3, 2, OP, MODE, T, LN_THREADS>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
   if constexpr (STOKES) {
     for (int t = tid; t < C::PN; t += LN_THREADS) {
       const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);

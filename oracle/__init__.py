@@ -104,7 +104,7 @@ def make_problem(dim=3, degree=2, coarse=(1, 1, 1), n_levels=3, extent=(1.0, 1.0
         p.extent[i] = float(extent[i]) if i < len(extent) else 1.0
     p.n_levels = n_levels
     for i in range(6):
-        p.bc[i] = int(bc[i])
+        p.bc[i] = int(bc[i]) if i < len(bc) else 0
     p.compressible = int(compressible)
     p.pressure_scaling = pressure_scaling
     return p

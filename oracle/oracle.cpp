@@ -419,7 +419,7 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
         if (need_p) K.pvalues(pl.data(), pq.data(), work.data());
         // quadrature point physics
         for (int q = 0; q < nq; ++q) {
-          const double eta = in.eta[in.eta_per_qp ? c * nq + q : c];
+          const double eta = op == OP_BT ? 1.0 : in.eta[in.eta_per_qp ? c * nq + q : c];  // B^T has no eta
           const double JxW = JxW0 * re.qweight(q);
           // gu[cmp][d][q] = d u_cmp / d x_d
           double eps[3][3] = {{0}};
