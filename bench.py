@@ -132,6 +132,18 @@ def main():
     log(f"A-block FP32 fine level: {t_a32 * 1e3:.3f} ms {nu / t_a32 / 1e9:.3f} GDoF/s "
         f"{bytes_a32 / t_a32 / 1e9:.0f} GB/s algorithmic")
 
+    # one A-block and one Schur V-cycle (FP32, FP64 vectors at the boundary)
+    ud = torch.randn(nu, dtype=torch.float64, device=dev, generator=gen)
+    pd = torch.randn(npd, dtype=torch.float64, device=dev, generator=gen)
+    vd, qd = torch.empty_like(ud), torch.empty_like(pd)
+    torch.cuda.synchronize()
+    t_vc = {}
+    for which, name, a_, b_ in ((3, "a_vcycle_fp32_ms", ud, vd), (4, "s_vcycle_fp32_ms", pd, qd)):
+        h32.time_operator(which, h32.top, a_.data_ptr(), b_.data_ptr(), 1)
+        t_vc[name] = h32.time_operator(which, h32.top, a_.data_ptr(), b_.data_ptr(), 3) * 1e3
+    log(f"V-cycles FP32: A {t_vc['a_vcycle_fp32_ms']:.2f} ms, S {t_vc['s_vcycle_fp32_ms']:.2f} ms")
+    del ud, pd, vd, qd
+
     result = {
         "metric": "Stokes matvec GDoF/s; solve time to 1e-8 residual, FP32-GMG vs FP64",
         "value": gdofs,
@@ -153,6 +165,7 @@ def main():
         "ablock_fp32_gdofs": nu / t_a32 / 1e9,
         "ablock_fp32_ms": t_a32 * 1e3,
         "wall_ms_per_step": wall * 1e3,
+        **t_vc,
     }
 
     if not args.no_solve:
