@@ -24,7 +24,7 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
     flags = [
         "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
+        "-fno-signed-zeros", "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
     ]
     objs = []
     procs = []

@@ -712,6 +712,71 @@ void launch_dkm(const LevelDesc &L, const T *su, const T *sp, const T *eta, int 
   MFS_HIP(hipGetLastError());
 }
 
+// Pressure mass operator (degree 1, 3D, one viscosity value per cell),
+// evaluated per node: out(n) = s^2 |K| sum over the <= 8 cells c around n of
+// (1/eta_c) sum_m M1(n, m) p_m, M1 the Q1 reference mass matrix (tensor product
+// of [[1/3, 1/6], [1/6, 1/3]], which the 3-point Gauss rule of
+// MassMatrixOperator integrates exactly).  One thread per node and one launch
+// per apply: no tile colours, deterministic by construction.
+template <int MODE, typename T>
+__global__ void k_mass_nodes(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ eta,
+                             const T *__restrict__ rhs, T *__restrict__ dst) {
+  const long long nn = (long long)L.np[0] * L.np[1] * L.np[2];
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int g[3] = {(int)(i % L.np[0]), (int)((i / L.np[0]) % L.np[1]), (int)(i / ((long long)L.np[0] * L.np[1]))};
+  const double vol = L.h[0] * L.h[1] * L.h[2] * L.s * L.s;
+  double acc = 0;
+#pragma unroll
+  for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+      for (int ox = 0; ox < 2; ++ox) {
+        const int c[3] = {g[0] - 1 + ox, g[1] - 1 + oy, g[2] - 1 + oz};  // cell; the node is its local (1-ox, 1-oy, 1-oz)
+        if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= L.n[0] || c[1] >= L.n[1] || c[2] >= L.n[2]) continue;
+        double cs = 0;
+#pragma unroll
+        for (int jz = 0; jz < 2; ++jz)
+#pragma unroll
+          for (int jy = 0; jy < 2; ++jy)
+#pragma unroll
+            for (int jx = 0; jx < 2; ++jx) {
+              const double w = ((1 - ox) == jx ? 1.0 / 3.0 : 1.0 / 6.0) * ((1 - oy) == jy ? 1.0 / 3.0 : 1.0 / 6.0) *
+                               ((1 - oz) == jz ? 1.0 / 3.0 : 1.0 / 6.0);
+              cs += w * (double)src[(c[0] + jx) + (long long)L.np[0] * ((c[1] + jy) + (long long)L.np[1] * (c[2] + jz))];
+            }
+        const int tx = c[0] >> 2, ty = c[1] >> 2, tz = c[2] >> 2;
+        const long long ti = (tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz)) * 64 +
+                             (c[0] & 3) + 4 * ((c[1] & 3) + 4 * (c[2] & 3));
+        acc += cs / (double)eta[ti];
+      }
+  const T v = (T)(vol * acc);
+  if constexpr (MODE == MODE_SET)
+    dst[i] = v;
+  else if constexpr (MODE == MODE_RESID)
+    dst[i] = rhs[i] - v;
+  else
+    dst[i] += v;
+}
+
+template <typename T>
+bool launch_mass_nodes(Mode mode, const LevelDesc &L, const T *src, const T *eta, int eta_per_qp, const T *rhs,
+                       T *dst, hipStream_t st) {
+  if (L.dim != 3 || L.k != 2 || eta_per_qp) return false;
+  static_assert(Tile<3, 2>::TX == 4 && Tile<3, 2>::TY == 4 && Tile<3, 2>::TZ == 4, "tile-blocked eta layout");
+  const long long nn = L.pnodes;
+  const unsigned nb = (unsigned)((nn + 255) / 256);
+  if (mode == MODE_SET)
+    hipLaunchKernelGGL((k_mass_nodes<MODE_SET, T>), dim3(nb), dim3(256), 0, st, L, src, eta, rhs, dst);
+  else if (mode == MODE_RESID)
+    hipLaunchKernelGGL((k_mass_nodes<MODE_RESID, T>), dim3(nb), dim3(256), 0, st, L, src, eta, rhs, dst);
+  else
+    hipLaunchKernelGGL((k_mass_nodes<MODE_ADD, T>), dim3(nb), dim3(256), 0, st, L, src, eta, rhs, dst);
+  MFS_HIP(hipGetLastError());
+  return true;
+}
+
 template <int DIM, int K, typename T>
 void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru,
                const T *rp, T *du, T *dp, hipStream_t st) {
@@ -742,6 +807,7 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
 #if MFS_CELL_LINES
   if (launch_cell_lines<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st)) return;
 #endif
+  if (op == OP_MASS && launch_mass_nodes<T>(mode, L, src_p, eta, eta_per_qp, rhs_p, dst_p, st)) return;
   if (L.dim == 3 && L.k == 2)
     launch_dk<3, 2, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
   else if (L.dim == 3 && L.k == 3)
@@ -871,13 +937,157 @@ __global__ void k_restrict(const LevelDesc Lc, const LevelDesc Lf, const int kin
   }
 }
 
+// 3D transfers with fixed tap patterns for Q2 velocity (KK = 2, nodes 0, 1/2, 1)
+// and Q1 pressure (KK = 1).  Prolongation row of fine local node r in its
+// coarse cell: the Lagrange basis at r / (2 KK); the zero taps are skipped at
+// compile time.
+template <int KK>
+struct ProlongTaps;
+template <>
+struct ProlongTaps<2> {
+  static constexpr double W[5][3] = {
+      {1.0, 0.0, 0.0}, {0.375, 0.75, -0.125}, {0.0, 1.0, 0.0}, {-0.125, 0.75, 0.375}, {0.0, 0.0, 1.0}};
+};
+template <>
+struct ProlongTaps<1> {
+  static constexpr double W[3][2] = {{1.0, 0.0}, {0.5, 0.5}, {0.0, 1.0}};
+};
+
+__device__ __forceinline__ bool con_node(const LevelDesc &L, int comp, int gx, int gy, int gz) {
+  const int g[3] = {gx, gy, gz};
+  bool con = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int b = g[d] == 0 ? L.bc[2 * d] : (g[d] == L.nv[d] - 1 ? L.bc[2 * d + 1] : 0);
+    con = con || b == 2 || (b == 1 && comp == d);
+  }
+  return con;
+}
+
+// one thread per fine node (all components)
+template <int KK, typename T>
+__global__ void k_prolongate3(const LevelDesc Lc, const LevelDesc Lf, const T *__restrict__ src,
+                              T *__restrict__ dst, const int add) {
+  constexpr bool VEL = KK == 2;
+  const int *nf = VEL ? Lf.nv : Lf.np;
+  const int *ncn = VEL ? Lc.nv : Lc.np;
+  const long long nfine = (long long)nf[0] * nf[1] * nf[2];
+  const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (f >= nfine) return;
+  const int g[3] = {(int)(f % nf[0]), (int)((f / nf[0]) % nf[1]), (int)(f / ((long long)nf[0] * nf[1]))};
+  int base[3];
+  double w[3][KK + 1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int e = g[d] / (2 * KK), r = g[d] - 2 * KK * e;
+    if (e == Lc.n[d]) {
+      e = Lc.n[d] - 1;
+      r = 2 * KK;
+    }
+    base[d] = KK * e;
+#pragma unroll
+    for (int m = 0; m <= KK; ++m) {
+      double v = ProlongTaps<KK>::W[0][m];
+#pragma unroll
+      for (int rr = 1; rr <= 2 * KK; ++rr) v = r == rr ? ProlongTaps<KK>::W[rr][m] : v;
+      w[d][m] = v;
+    }
+  }
+  const long long ncoarse = (long long)ncn[0] * ncn[1] * ncn[2];
+  constexpr int NC = VEL ? 3 : 1;
+#pragma unroll
+  for (int comp = 0; comp < NC; ++comp) {
+    double acc = 0;
+#pragma unroll
+    for (int mz = 0; mz <= KK; ++mz)
+#pragma unroll
+      for (int my = 0; my <= KK; ++my)
+#pragma unroll
+        for (int mx = 0; mx <= KK; ++mx) {
+          const double wt = w[2][mz] * w[1][my] * w[0][mx];
+          const int cx = base[0] + mx, cy = base[1] + my, cz = base[2] + mz;
+          const bool use = wt != 0.0 && !(VEL && con_node(Lc, comp, cx, cy, cz));
+          const long long ci = comp * ncoarse + cx + (long long)ncn[0] * (cy + (long long)ncn[1] * cz);
+          if (use) acc += wt * (double)src[ci];
+        }
+    if (VEL && con_node(Lf, comp, g[0], g[1], g[2])) acc = 0.0;
+    const long long gi = comp * nfine + f;
+    dst[gi] = add ? (T)(dst[gi] + (T)acc) : (T)acc;
+  }
+}
+
+// restriction = transpose: one thread per coarse node (all components).  Per
+// direction the fine nodes 2m + o feeding coarse node m: a coarse vertex takes
+// o = 0 (1), +-1 (3/8), +-3 (-1/8) for KK = 2 and o = 0 (1), +-1 (1/2) for KK = 1;
+// a coarse cell-midpoint node (KK = 2, m odd) takes o = 0 (1), +-1 (3/4).
+template <int KK, typename T>
+__global__ void k_restrict3(const LevelDesc Lc, const LevelDesc Lf, const T *__restrict__ src,
+                            T *__restrict__ dst) {
+  constexpr bool VEL = KK == 2;
+  constexpr int NO = VEL ? 5 : 3;
+  const int *nf = VEL ? Lf.nv : Lf.np;
+  const int *ncn = VEL ? Lc.nv : Lc.np;
+  const long long ncoarse = (long long)ncn[0] * ncn[1] * ncn[2];
+  const long long cidx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (cidx >= ncoarse) return;
+  const int m[3] = {(int)(cidx % ncn[0]), (int)((cidx / ncn[0]) % ncn[1]), (int)(cidx / ((long long)ncn[0] * ncn[1]))};
+  int gi[3][NO];
+  double wi[3][NO];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const bool vertex = !VEL || (m[d] & 1) == 0;
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+      // taps ordered o = -3, -1, 0, 1, 3 (KK = 2) or -1, 0, 1 (KK = 1)
+      int o;
+      double wv;
+      if constexpr (VEL) {
+        const int oo[5] = {-3, -1, 0, 1, 3};
+        o = oo[k];
+        wv = vertex ? (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.375 : -0.125)) : (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.75 : 0.0));
+      } else {
+        o = k - 1;
+        wv = k == 1 ? 1.0 : 0.5;
+      }
+      const int g = 2 * m[d] + o;
+      const bool ok = g >= 0 && g < nf[d];
+      gi[d][k] = ok ? g : 0;
+      wi[d][k] = ok ? wv : 0.0;
+    }
+  }
+  const long long nfine = (long long)nf[0] * nf[1] * nf[2];
+  constexpr int NC = VEL ? 3 : 1;
+#pragma unroll
+  for (int comp = 0; comp < NC; ++comp) {
+    double acc = 0;
+    if (!(VEL && con_node(Lc, comp, m[0], m[1], m[2]))) {
+#pragma unroll
+      for (int a = 0; a < NO; ++a)
+#pragma unroll
+        for (int b = 0; b < NO; ++b)
+#pragma unroll
+          for (int c = 0; c < NO; ++c) {
+            const double wt = wi[2][a] * wi[1][b] * wi[0][c];
+            const bool use = wt != 0.0 && !(VEL && con_node(Lf, comp, gi[0][c], gi[1][b], gi[2][a]));
+            const long long fi = comp * nfine + gi[0][c] + (long long)nf[0] * (gi[1][b] + (long long)nf[1] * gi[2][a]);
+            if (use) acc += wt * (double)src[fi];
+          }
+    }
+    dst[comp * ncoarse + cidx] = (T)acc;
+  }
+}
+
 template <typename T>
 void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
                        hipStream_t st) {
   const long long n = kind == 0 ? Lf.vnodes : Lf.pnodes;
   const int bs = 256;
   const unsigned nb = (unsigned)((n + bs - 1) / bs);
-  if (Lf.dim == 3)
+  if (Lf.dim == 3 && Lf.k == 2 && kind == 0)
+    hipLaunchKernelGGL((k_prolongate3<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_c, dst_f, add ? 1 : 0);
+  else if (Lf.dim == 3 && Lf.k == 2 && kind == 1)
+    hipLaunchKernelGGL((k_prolongate3<1, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_c, dst_f, add ? 1 : 0);
+  else if (Lf.dim == 3)
     hipLaunchKernelGGL((k_prolongate<3, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_c, dst_f, add ? 1 : 0);
   else
     hipLaunchKernelGGL((k_prolongate<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_c, dst_f, add ? 1 : 0);
@@ -889,7 +1099,11 @@ void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T
   const long long n = kind == 0 ? Lc.vnodes : Lc.pnodes;
   const int bs = 256;
   const unsigned nb = (unsigned)((n + bs - 1) / bs);
-  if (Lf.dim == 3)
+  if (Lf.dim == 3 && Lf.k == 2 && kind == 0)
+    hipLaunchKernelGGL((k_restrict3<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_f, dst_c);
+  else if (Lf.dim == 3 && Lf.k == 2 && kind == 1)
+    hipLaunchKernelGGL((k_restrict3<1, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_f, dst_c);
+  else if (Lf.dim == 3)
     hipLaunchKernelGGL((k_restrict<3, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_f, dst_c);
   else
     hipLaunchKernelGGL((k_restrict<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_f, dst_c);

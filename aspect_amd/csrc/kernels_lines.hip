@@ -38,6 +38,17 @@ struct LinesBlocks {  // workgroups per CU the register budget is tuned for
   static constexpr int value = std::is_same<T, double>::value ? 1 : 2;
 };
 
+// acc + c x for a coefficient c that is a compile-time constant after
+// unrolling: zero and unit entries of the 1D tables cost nothing
+template <typename T>
+__device__ __forceinline__ T madd(T acc, T c, T x) {
+  return c == T(0) ? acc : (c == T(1) ? acc + x : acc + c * x);
+}
+template <typename T>
+__device__ __forceinline__ T dot3(const T (&c)[3], T x0, T x1, T x2) {
+  return madd(madd(c[0] == T(0) ? T(0) : c[0] * x0, c[1], x1), c[2], x2);
+}
+
 // row r (or column r) of a 3x3 table, r uniform over the wave
 template <typename T>
 __device__ __forceinline__ void row3(const T (&M)[3][3], int r, T out[3]) {
@@ -48,6 +59,77 @@ template <typename T>
 __device__ __forceinline__ void col3(const T (&M)[3][3], int r, T out[3]) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) out[k] = r == 0 ? M[k][0] : (r == 1 ? M[k][1] : M[k][2]);
+}
+
+// constrained velocity DoF test without branches (faces: 0 open, 1 no normal
+// flux, 2 zero velocity)
+__device__ __forceinline__ bool con_bf(const LevelDesc &L, int comp, int gx, int gy, int gz) {
+  const int g[3] = {gx, gy, gz};
+  bool con = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int b = g[d] == 0 ? L.bc[2 * d] : (g[d] == L.nv[d] - 1 ? L.bc[2 * d + 1] : 0);
+    con = con || b == 2 || (b == 1 && comp == d);
+  }
+  return con;
+}
+
+// Velocity write-out of a 4x4x4 tile, degree 2: every tile node (9^3 per
+// component) sums its at most 8 contributing cells in a fixed order, with the
+// two candidate cells per direction -- (l/2 - 1, local node 2) for even l > 0
+// and (l/2, local node l mod 2) -- evaluated as predicated reads, so all lanes
+// run the same instruction stream.  xb: per-cell contributions
+// [comp][kz][jy][i][cell].  Colours, constrained rows and modes as in
+// tile_write_u.
+template <int OP, int MODE, typename T>
+__device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, const int tid, const int tx,
+                                              const int ty, const int tz, const int px, const int py, const int pz,
+                                              const T *__restrict__ src_u, const T *__restrict__ rhs_u,
+                                              T *__restrict__ dst_u) {
+  constexpr int CT = 64, VX = 9, VN = 729, NT = 3 * VN;
+  const int ncx = min(4, L.n[0] - 4 * tx), ncy = min(4, L.n[1] - 4 * ty), ncz = min(4, L.n[2] - 4 * tz);
+#pragma unroll
+  for (int k = 0; k < (NT + LN_THREADS - 1) / LN_THREADS; ++k) {
+    const int t = tid + k * LN_THREADS;
+    if (t >= NT) break;
+    const int comp = t / VN, r = t - comp * VN;
+    const int lx = r % VX, ly = (r / VX) % VX, lz = r / (VX * VX);
+    const int gx = 8 * tx + lx, gy = 8 * ty + ly, gz = 8 * tz + lz;
+    T val = 0;
+#pragma unroll
+    for (int sz = 0; sz < 2; ++sz) {
+      const int cz = (lz >> 1) - 1 + sz, iz = sz ? (lz & 1) : 2;
+      const bool vz = (sz ? true : ((lz & 1) == 0 && lz > 0)) && cz < ncz;
+#pragma unroll
+      for (int sy = 0; sy < 2; ++sy) {
+        const int cy = (ly >> 1) - 1 + sy, iy = sy ? (ly & 1) : 2;
+        const bool vy = (sy ? true : ((ly & 1) == 0 && ly > 0)) && cy < ncy;
+#pragma unroll
+        for (int sx = 0; sx < 2; ++sx) {
+          const int cx = (lx >> 1) - 1 + sx, ix = sx ? (lx & 1) : 2;
+          const bool v = vz && vy && (sx ? true : ((lx & 1) == 0 && lx > 0)) && cx < ncx;
+          const int idx = v ? (((comp * 3 + iz) * 3 + iy) * 3 + ix) * CT + cx + 4 * (cy + 4 * cz) : 0;
+          const T c = xb[idx];
+          val += v ? c : (T)0;
+        }
+      }
+    }
+    if (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2]) continue;
+    const bool shx = (lx == 0 && tx > 0) || (lx == VX - 1 && tx + 1 < L.ntiles[0]);
+    const bool shy = (ly == 0 && ty > 0) || (ly == VX - 1 && ty + 1 < L.ntiles[1]);
+    const bool shz = (lz == 0 && tz > 0) || (lz == VX - 1 && tz + 1 < L.ntiles[2]);
+    const bool first = !((shx && px) || (shy && py) || (shz && pz));
+    const long long gi = comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz);
+    const bool con = con_bf(L, comp, gx, gy, gz);
+    if (con && !first) continue;
+    const T vv = con ? src_u[gi] : val;
+    if constexpr (MODE == MODE_SET)
+      dst_u[gi] = first ? vv : dst_u[gi] + vv;
+    else if constexpr (MODE == MODE_RESID)
+      dst_u[gi] = first ? rhs_u[gi] - vv : dst_u[gi] - vv;
+    else
+      dst_u[gi] += vv;
+  }
 }
 
 template <int OP, int MODE, typename T>
@@ -86,14 +168,18 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   auto XI = [&](int comp, int q, int l) { return ((comp * 3 + q) * NL + l) * CT + c; };
 
   // ---- 0. cooperative tile load (constrained velocity DoFs read as zero)
-  for (int t = tid; t < 3 * C::VN; t += LN_THREADS) {
+#pragma unroll
+  for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
+    const int t = tid + k * LN_THREADS;
+    if (t >= 3 * C::VN) break;
     const int comp = t / C::VN, r = t - comp * C::VN;
     const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
     const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
-    T v = 0;
-    if (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2] && (L.plain_read || !is_con(L, comp, gx, gy, gz)))
-      v = src_u[comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz)];
-    sC[t] = v;
+    const bool in = gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2];
+    const bool take = in && (L.plain_read || !con_bf(L, comp, gx, gy, gz));
+    const long long gi = in ? comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz) : 0;
+    const T v = src_u[gi];
+    sC[t] = take ? v : (T)0;
   }
   if constexpr (STOKES) {
     for (int t = tid; t < C::PN; t += LN_THREADS) {
@@ -129,7 +215,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
       const T *row = sC + comp * C::VN + ((2 * cz + b) * C::VY + (2 * cy + jy)) * C::VX + 2 * cx;
       const T u0 = row[0], u1 = row[1], u2 = row[2];
 #pragma unroll
-      for (int qx = 0; qx < 3; ++qx) vy[qx] += Sa[jy] * (B::S[qx][0] * u0 + B::S[qx][1] * u1 + B::S[qx][2] * u2);
+      for (int qx = 0; qx < 3; ++qx) vy[qx] += Sa[jy] * dot3(B::S[qx], u0, u1, u2);
     }
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx) sA[XI(comp, qx, ln)] = vy[qx];
@@ -185,7 +271,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     T gr[3][3];
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-      const T dx = B::D[qx][0] * v[comp][0] + B::D[qx][1] * v[comp][1] + B::D[qx][2] * v[comp][2];
+      const T dx = dot3(B::D[qx], v[comp][0], v[comp][1], v[comp][2]);
       const T dy = Da[0] * sB[XI(comp, qx, 3 * b)] + Da[1] * sB[XI(comp, qx, 3 * b + 1)] +
                    Da[2] * sB[XI(comp, qx, 3 * b + 2)];
       const T dz = Db[0] * sB[XI(comp, qx, a)] + Db[1] * sB[XI(comp, qx, a + 3)] + Db[2] * sB[XI(comp, qx, a + 6)];
@@ -210,7 +296,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     for (int comp = 0; comp < 3; ++comp) {
       const T fx = flux[comp][0] * ih0;
 #pragma unroll
-      for (int r = 0; r < 3; ++r) tacc[comp][r] += B::D[qx][r] * fx;
+      for (int r = 0; r < 3; ++r) tacc[comp][r] = madd(tacc[comp][r], B::D[qx][r], fx);
       sA[XI(comp, qx, ln)] = flux[comp][1] * ih1;
       sC[XI(comp, qx, ln)] = flux[comp][2] * ih2;
     }
@@ -258,50 +344,56 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] =
-          B::S[0][i] * hyv[0] + B::S[1][i] * hyv[1] + B::S[2][i] * hyv[2];
+          madd(madd(B::S[0][i] == T(0) ? T(0) : B::S[0][i] * hyv[0], B::S[1][i], hyv[1]), B::S[2][i], hyv[2]);
   }
   __syncthreads();
 
   // ---- 7. tile write-out
-  tile_write_u<3, 2, OP, MODE, T, LN_THREADS>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+  lines_write_u<OP, MODE, T>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
   if constexpr (STOKES) {
-    for (int t = tid; t < C::PN; t += LN_THREADS) {
+    const int ncx = min(C::TX, L.n[0] - C::TX * tx), ncy = min(C::TY, L.n[1] - C::TY * ty),
+              ncz = min(C::TZ, L.n[2] - C::TZ * tz);
+    if (tid < C::PN) {
+      const int t = tid;
       const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
       const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
-      if (gx >= L.np[0] || gy >= L.np[1] || gz >= L.np[2]) continue;
       // contributing cells per direction: (l - 1, local index 1) and (l, local index 0)
       T val = 0;
 #pragma unroll
       for (int e3 = 0; e3 < 2; ++e3) {
         const int ccz = lz - 1 + e3;
-        if (ccz < 0 || ccz >= C::TZ || tz * C::TZ + ccz >= L.n[2]) continue;
+        const bool vz = ccz >= 0 && ccz < ncz;
 #pragma unroll
         for (int e2 = 0; e2 < 2; ++e2) {
           const int ccy = ly - 1 + e2;
-          if (ccy < 0 || ccy >= C::TY || ty * C::TY + ccy >= L.n[1]) continue;
+          const bool vy = ccy >= 0 && ccy < ncy;
 #pragma unroll
           for (int e1 = 0; e1 < 2; ++e1) {
             const int ccx = lx - 1 + e1;
-            if (ccx < 0 || ccx >= C::TX || tx * C::TX + ccx >= L.n[0]) continue;
-            const int cell = ccx + C::TX * (ccy + C::TY * ccz);
+            const bool v = vz && vy && ccx >= 0 && ccx < ncx;
+            const int cell = v ? ccx + C::TX * (ccy + C::TY * ccz) : 0;
             // node index within the cell is 1 - e per direction
+            T acc = 0;
 #pragma unroll
             for (int l = 0; l < NL; ++l)
-              val += B::SP[l % 3][1 - e2] * B::SP[l / 3][1 - e3] * sP[((1 - e1) * NL + l) * CT + cell];
+              acc += B::SP[l % 3][1 - e2] * B::SP[l / 3][1 - e3] * sP[((1 - e1) * NL + l) * CT + cell];
+            val += v ? acc : (T)0;
           }
         }
       }
-      const bool shx = (lx == 0 && tx > 0) || (lx == C::PX - 1 && tx + 1 < L.ntiles[0]);
-      const bool shy = (ly == 0 && ty > 0) || (ly == C::PY - 1 && ty + 1 < L.ntiles[1]);
-      const bool shz = (lz == 0 && tz > 0) || (lz == C::PZ - 1 && tz + 1 < L.ntiles[2]);
-      const bool first = !((shx && px) || (shy && py) || (shz && pz));
-      const long long gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
-      if constexpr (MODE == MODE_SET)
-        dst_p[gi] = first ? val : dst_p[gi] + val;
-      else if constexpr (MODE == MODE_RESID)
-        dst_p[gi] = first ? rhs_p[gi] - val : dst_p[gi] - val;
-      else
-        dst_p[gi] += val;
+      if (gx < L.np[0] && gy < L.np[1] && gz < L.np[2]) {
+        const bool shx = (lx == 0 && tx > 0) || (lx == C::PX - 1 && tx + 1 < L.ntiles[0]);
+        const bool shy = (ly == 0 && ty > 0) || (ly == C::PY - 1 && ty + 1 < L.ntiles[1]);
+        const bool shz = (lz == 0 && tz > 0) || (lz == C::PZ - 1 && tz + 1 < L.ntiles[2]);
+        const bool first = !((shx && px) || (shy && py) || (shz && pz));
+        const long long gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
+        if constexpr (MODE == MODE_SET)
+          dst_p[gi] = first ? val : dst_p[gi] + val;
+        else if constexpr (MODE == MODE_RESID)
+          dst_p[gi] = first ? rhs_p[gi] - val : dst_p[gi] - val;
+        else
+          dst_p[gi] += val;
+      }
     }
   }
 }
