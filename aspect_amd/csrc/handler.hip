@@ -137,6 +137,7 @@ struct GMG {
   // whenever those change
   bool graphs = true;
   hipGraphExec_t vexec[2] = {nullptr, nullptr};
+  DBuf<T> xwork;  // work arrays of the separable transfers (sized for the finest pair)
   void drop_graphs() {
     for (hipGraphExec_t &e : vexec)
       if (e) {
@@ -174,6 +175,7 @@ struct GMG {
       lv.push_back(std::move(g));
     }
     top = b.n_levels - 1;
+    if (top > 0) xwork.alloc(transfer_work_size(lv[top - 1]->L, lv[top]->L));
   }
 
   int eta_pq = 0;  // level tables per quadrature point (DGQ1) or per cell (DGQ0)
@@ -373,9 +375,9 @@ struct GMG {
     smooth(kind, l, x, b, true, steps);
     apply(kind, l, MODE_RESID, x, b, r);
     GMGLevel<T> &c = *lv[l - 1];
-    launch_restrict<T>(kind, c.L, g.L, r, kind == 0 ? c.ub.p : c.pb.p, st);
+    launch_restrict<T>(kind, c.L, g.L, r, kind == 0 ? c.ub.p : c.pb.p, st, xwork.p);
     vcycle_level(kind, l - 1);
-    launch_prolongate<T>(kind, c.L, g.L, kind == 0 ? c.ux.p : c.px.p, x, true, st);
+    launch_prolongate<T>(kind, c.L, g.L, kind == 0 ? c.ux.p : c.px.p, x, true, st, xwork.p);
     smooth(kind, l, x, b, false, steps);
   }
 
@@ -1277,7 +1279,9 @@ int mfs_prolongate(mfs_handle *h, int fine_level, int kind, const double *src, d
     const long long nf = kind == 0 ? (long long)h->box.dim * Lf.vnodes : Lf.pnodes;
     HostIO io{h};
     io.in(h->w1.p, src, nc);
-    launch_prolongate<double>(kind, Lc, Lf, h->w1.p, h->w2.p, false, h->st);
+    DBuf<double> work;
+    work.alloc(transfer_work_size(Lc, Lf));
+    launch_prolongate<double>(kind, Lc, Lf, h->w1.p, h->w2.p, false, h->st, work.p);
     io.out(dst, h->w2.p, nf);
     return MFS_OK;
   });
@@ -1292,7 +1296,9 @@ int mfs_restrict(mfs_handle *h, int fine_level, int kind, const double *src, dou
     const long long nf = kind == 0 ? (long long)h->box.dim * Lf.vnodes : Lf.pnodes;
     HostIO io{h};
     io.in(h->w1.p, src, nf);
-    launch_restrict<double>(kind, Lc, Lf, h->w1.p, h->w2.p, h->st);
+    DBuf<double> work;
+    work.alloc(transfer_work_size(Lc, Lf));
+    launch_restrict<double>(kind, Lc, Lf, h->w1.p, h->w2.p, h->st, work.p);
     io.out(dst, h->w2.p, nc);
     return MFS_OK;
   });

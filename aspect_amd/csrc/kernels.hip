@@ -1077,9 +1077,171 @@ __global__ void k_restrict3(const LevelDesc Lc, const LevelDesc Lf, const T *__r
   }
 }
 
+// Separable 3D transfers: the tensor-product prolongation P = Px Py Pz (and
+// R = P^T) applied one direction per pass through two work arrays.  Each pass
+// is a 1D stencil along one axis of an (n0, n1, n2) array per component:
+// restriction, coarse node m takes the fine nodes 2m + o (weights in
+// k_restrict3), prolongation, fine node g takes its coarse cell's nodes with
+// the row of ProlongTaps.  Constrained DoFs: the first restriction pass reads
+// constrained fine nodes as zero and the last one zeroes constrained coarse
+// nodes; the first prolongation pass reads constrained coarse nodes as zero
+// and the last one zeroes constrained fine nodes.
+struct Dims3 {
+  int n[3];
+};
+template <int KK, typename T>
+__global__ void k_restrict_pass(const LevelDesc Lc, const LevelDesc Lf, const int ax, const Dims3 din, const Dims3 dout,
+                                const int ncomp, const int first, const int last, const T *__restrict__ in,
+                                T *__restrict__ out) {
+  const long long per = (long long)dout.n[0] * dout.n[1] * dout.n[2];
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= per * ncomp) return;
+  const int comp = (int)(idx / per);
+  const long long r = idx - comp * per;
+  int o[3] = {(int)(r % dout.n[0]), (int)((r / dout.n[0]) % dout.n[1]), (int)(r / ((long long)dout.n[0] * dout.n[1]))};
+  const long long stride = ax == 0 ? 1 : (ax == 1 ? (long long)din.n[0] : (long long)din.n[0] * din.n[1]);
+  const long long inper = (long long)din.n[0] * din.n[1] * din.n[2];
+  const int m = o[ax];
+  int base[3] = {o[0], o[1], o[2]};
+  base[ax] = 0;
+  const long long b0 = comp * inper + base[0] + (long long)din.n[0] * (base[1] + (long long)din.n[1] * base[2]);
+  constexpr bool VEL = KK == 2;
+  constexpr int NO = VEL ? 5 : 3;
+  const bool vertex = !VEL || (m & 1) == 0;
+  double acc = 0;
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    int off;
+    double wv;
+    if constexpr (VEL) {
+      off = k == 0 ? -3 : (k == 1 ? -1 : (k == 2 ? 0 : (k == 3 ? 1 : 3)));
+      wv = vertex ? (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.375 : -0.125)) : (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.75 : 0.0));
+    } else {
+      off = k - 1;
+      wv = k == 1 ? 1.0 : 0.5;
+    }
+    const int g = 2 * m + off;
+    bool ok = wv != 0.0 && g >= 0 && g < din.n[ax];
+    if (ok && first && VEL) {
+      int f[3] = {o[0], o[1], o[2]};
+      f[ax] = g;
+      ok = !con_node(Lf, comp, f[0], f[1], f[2]);
+    }
+    if (ok) acc += wv * (double)in[b0 + g * stride];
+  }
+  if (last && VEL && con_node(Lc, comp, o[0], o[1], o[2])) acc = 0.0;
+  out[idx] = (T)acc;
+}
+template <int KK, typename T>
+__global__ void k_prolong_pass(const LevelDesc Lc, const LevelDesc Lf, const int ax, const Dims3 din, const Dims3 dout,
+                               const int ncomp, const int first, const int last, const int add,
+                               const T *__restrict__ in, T *__restrict__ out) {
+  const long long per = (long long)dout.n[0] * dout.n[1] * dout.n[2];
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= per * ncomp) return;
+  const int comp = (int)(idx / per);
+  const long long r = idx - comp * per;
+  const int o[3] = {(int)(r % dout.n[0]), (int)((r / dout.n[0]) % dout.n[1]),
+                    (int)(r / ((long long)dout.n[0] * dout.n[1]))};
+  const long long stride = ax == 0 ? 1 : (ax == 1 ? (long long)din.n[0] : (long long)din.n[0] * din.n[1]);
+  const long long inper = (long long)din.n[0] * din.n[1] * din.n[2];
+  const int g = o[ax];
+  const int ncell = Lc.n[ax];
+  int e = g / (2 * KK), rr = g - 2 * KK * e;
+  if (e == ncell) {
+    e = ncell - 1;
+    rr = 2 * KK;
+  }
+  int base[3] = {o[0], o[1], o[2]};
+  base[ax] = 0;
+  const long long b0 = comp * inper + base[0] + (long long)din.n[0] * (base[1] + (long long)din.n[1] * base[2]);
+  constexpr bool VEL = KK == 2;
+  double acc = 0;
+#pragma unroll
+  for (int mm = 0; mm <= KK; ++mm) {
+    double wv = ProlongTaps<KK>::W[0][mm];
+#pragma unroll
+    for (int q = 1; q <= 2 * KK; ++q) wv = rr == q ? ProlongTaps<KK>::W[q][mm] : wv;
+    const int cidx = KK * e + mm;
+    bool ok = wv != 0.0;
+    if (ok && first && VEL) {
+      int c[3] = {o[0], o[1], o[2]};
+      c[ax] = cidx;
+      ok = !con_node(Lc, comp, c[0], c[1], c[2]);
+    }
+    if (ok) acc += wv * (double)in[b0 + cidx * stride];
+  }
+  if (last && VEL && con_node(Lf, comp, o[0], o[1], o[2])) acc = 0.0;
+  out[idx] = add ? (T)(out[idx] + (T)acc) : (T)acc;
+}
+
+template <typename T>
+static void run_sep_transfer(bool prolong, int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src, T *dst,
+                             bool add, T *work, hipStream_t st) {
+  const bool vel = kind == 0;
+  const int ncomp = vel ? 3 : 1;
+  Dims3 dc, df;
+  for (int d = 0; d < 3; ++d) {
+    dc.n[d] = vel ? Lc.nv[d] : Lc.np[d];
+    df.n[d] = vel ? Lf.nv[d] : Lf.np[d];
+  }
+  auto blocks = [&](const Dims3 &d) {
+    return (unsigned)(((long long)d.n[0] * d.n[1] * d.n[2] * ncomp + 255) / 256);
+  };
+  if (!prolong) {
+    // x, y, z: fine -> (c, f, f) -> (c, c, f) -> coarse
+    Dims3 d1 = df, d2 = df;
+    d1.n[0] = dc.n[0];
+    d2.n[0] = dc.n[0];
+    d2.n[1] = dc.n[1];
+    T *t1 = work, *t2 = work + (long long)d1.n[0] * d1.n[1] * d1.n[2] * ncomp;
+    const T *ins[3] = {src, t1, t2};
+    T *outs[3] = {t1, t2, dst};
+    const Dims3 di[3] = {df, d1, d2}, dd[3] = {d1, d2, dc};
+    for (int ax = 0; ax < 3; ++ax) {
+      if (vel)
+        hipLaunchKernelGGL((k_restrict_pass<2, T>), dim3(blocks(dd[ax])), dim3(256), 0, st, Lc, Lf, ax, di[ax], dd[ax],
+                           ncomp, ax == 0 ? 1 : 0, ax == 2 ? 1 : 0, ins[ax], outs[ax]);
+      else
+        hipLaunchKernelGGL((k_restrict_pass<1, T>), dim3(blocks(dd[ax])), dim3(256), 0, st, Lc, Lf, ax, di[ax], dd[ax],
+                           ncomp, 0, 0, ins[ax], outs[ax]);
+    }
+  } else {
+    // z, y, x: coarse -> (c, c, f) -> (c, f, f) -> fine
+    Dims3 d1 = dc, d2 = dc;
+    d1.n[2] = df.n[2];
+    d2.n[2] = df.n[2];
+    d2.n[1] = df.n[1];
+    T *t1 = work, *t2 = work + (long long)d1.n[0] * d1.n[1] * d1.n[2] * ncomp;
+    const T *ins[3] = {src, t1, t2};
+    T *outs[3] = {t1, t2, dst};
+    const Dims3 di[3] = {dc, d1, d2}, dd[3] = {d1, d2, df};
+    const int axes[3] = {2, 1, 0};
+    for (int p = 0; p < 3; ++p) {
+      if (vel)
+        hipLaunchKernelGGL((k_prolong_pass<2, T>), dim3(blocks(dd[p])), dim3(256), 0, st, Lc, Lf, axes[p], di[p], dd[p],
+                           ncomp, p == 0 ? 1 : 0, p == 2 ? 1 : 0, p == 2 && add ? 1 : 0, ins[p], outs[p]);
+      else
+        hipLaunchKernelGGL((k_prolong_pass<1, T>), dim3(blocks(dd[p])), dim3(256), 0, st, Lc, Lf, axes[p], di[p], dd[p],
+                           ncomp, 0, 0, p == 2 && add ? 1 : 0, ins[p], outs[p]);
+    }
+  }
+  MFS_HIP(hipGetLastError());
+}
+long long transfer_work_size(const LevelDesc &Lc, const LevelDesc &Lf) {
+  // two work arrays: velocity (c, f, f) and (c, c, f) per component (restriction) or
+  // (c, c, f) and (c, f, f) (prolongation); sized for the larger of the two orders
+  const long long a = (long long)Lc.nv[0] * Lf.nv[1] * Lf.nv[2], b = (long long)Lc.nv[0] * Lc.nv[1] * Lf.nv[2];
+  return 3 * (a + b);
+}
+
 template <typename T>
 void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
-                       hipStream_t st) {
+                       hipStream_t st, T *work) {
+  if (work && Lf.dim == 3 && Lf.k == 2) {
+    run_sep_transfer<T>(true, kind, Lc, Lf, src_c, dst_f, add, work, st);
+    return;
+  }
   const long long n = kind == 0 ? Lf.vnodes : Lf.pnodes;
   const int bs = 256;
   const unsigned nb = (unsigned)((n + bs - 1) / bs);
@@ -1095,7 +1257,11 @@ void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const
 }
 template <typename T>
 void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_f, T *dst_c,
-                     hipStream_t st) {
+                     hipStream_t st, T *work) {
+  if (work && Lf.dim == 3 && Lf.k == 2) {
+    run_sep_transfer<T>(false, kind, Lc, Lf, src_f, dst_c, false, work, st);
+    return;
+  }
   const long long n = kind == 0 ? Lc.vnodes : Lc.pnodes;
   const int bs = 256;
   const unsigned nb = (unsigned)((n + bs - 1) / bs);
@@ -1110,13 +1276,13 @@ void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T
   MFS_HIP(hipGetLastError());
 }
 template void launch_prolongate<double>(int, const LevelDesc &, const LevelDesc &, const double *, double *, bool,
-                                        hipStream_t);
+                                        hipStream_t, double *);
 template void launch_prolongate<float>(int, const LevelDesc &, const LevelDesc &, const float *, float *, bool,
-                                       hipStream_t);
+                                       hipStream_t, float *);
 template void launch_restrict<double>(int, const LevelDesc &, const LevelDesc &, const double *, double *,
-                                      hipStream_t);
+                                      hipStream_t, double *);
 template void launch_restrict<float>(int, const LevelDesc &, const LevelDesc &, const float *, float *,
-                                     hipStream_t);
+                                     hipStream_t, float *);
 
 // ------------------------------------------------------------------ viscosity tables
 __device__ __forceinline__ long long tile_index(const LevelDesc &L, int gcx, int gcy, int gcz, int TX, int TY,

@@ -59,12 +59,15 @@ bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, con
                        int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st);
 
 // Transfers between level Lc (coarse) and Lf = refined Lc.  kind 0 velocity, 1 pressure.
+// work: nullptr, or transfer_work_size(Lc, Lf) elements for the separable
+// (one direction per pass) 3D degree-2 path
 template <typename T>
 void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
-                       hipStream_t st);
+                       hipStream_t st, T *work = nullptr);
 template <typename T>
 void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_f, T *dst_c,
-                     hipStream_t st);
+                     hipStream_t st, T *work = nullptr);
+long long transfer_work_size(const LevelDesc &Lc, const LevelDesc &Lf);
 
 // viscosity tables
 // natural layout eta[cell*nq + q] (or eta[cell]) -> tile-blocked layout of L
