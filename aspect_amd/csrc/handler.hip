@@ -138,6 +138,12 @@ struct GMG {
   bool graphs = true;
   hipGraphExec_t vexec[2] = {nullptr, nullptr};
   DBuf<T> xwork;  // work arrays of the separable transfers (sized for the finest pair)
+  // A-block applies on levels of at most SPLIT_TILES tiles run as two launches
+  // over all tiles through this work array instead of eight tile colours
+  // (those levels are bound by the latency of a tile, not by throughput)
+  static constexpr long long SPLIT_TILES = 4096;
+  DBuf<T> swork;
+  T *split_for(int l) { return lv[l]->L.ntiles_total <= SPLIT_TILES && swork.p ? swork.p : nullptr; }
   void drop_graphs() {
     for (hipGraphExec_t &e : vexec)
       if (e) {
@@ -176,6 +182,10 @@ struct GMG {
     }
     top = b.n_levels - 1;
     if (top > 0) xwork.alloc(transfer_work_size(lv[top - 1]->L, lv[top]->L));
+    long long sw = 0;
+    for (auto &g : lv)
+      if (g->L.dim == 3 && g->L.k == 2 && g->L.ntiles_total <= SPLIT_TILES) sw = std::max(sw, split_work_size(g->L));
+    if (sw > 0) swork.alloc(sw);
   }
 
   int eta_pq = 0;  // level tables per quadrature point (DGQ1) or per cell (DGQ0)
@@ -211,7 +221,7 @@ struct GMG {
 
   void applyA(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];
-    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst, nullptr, st);
+    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst, nullptr, st, split_for(l));
   }
   void applyS(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];

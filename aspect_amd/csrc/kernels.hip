@@ -803,9 +803,11 @@ void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, c
 
 template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
-                    int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st) {
+                    int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
+                    T *split_work) {
 #if MFS_CELL_LINES
-  if (launch_cell_lines<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st)) return;
+  if (launch_cell_lines<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st, split_work))
+    return;
 #endif
   if (op == OP_MASS && launch_mass_nodes<T>(mode, L, src_p, eta, eta_per_qp, rhs_p, dst_p, st)) return;
   if (L.dim == 3 && L.k == 2)
@@ -820,9 +822,9 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
     throw std::runtime_error("unsupported (dim, degree)");
 }
 template void launch_cell_op<double>(Op, Mode, const LevelDesc &, const double *, const double *, const double *,
-                                     int, const double *, const double *, double *, double *, hipStream_t);
+                                     int, const double *, const double *, double *, double *, hipStream_t, double *);
 template void launch_cell_op<float>(Op, Mode, const LevelDesc &, const float *, const float *, const float *, int,
-                                    const float *, const float *, float *, float *, hipStream_t);
+                                    const float *, const float *, float *, float *, hipStream_t, float *);
 
 // ------------------------------------------------------------------ transfers
 // prolongation: one thread per fine node (all components)

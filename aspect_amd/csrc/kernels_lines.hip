@@ -32,6 +32,9 @@ namespace mfs {
 namespace {
 
 constexpr int LN_THREADS = 576;  // 64 cells x 9 lines
+// internal mode of k_lines: every tile in one launch, per-cell contributions
+// written to a work array [tile][comp][kz][jy][i][cell], summed by k_gather_lines
+constexpr int LN_MODE_SPLIT = 3;
 
 template <typename T>
 struct LinesBlocks {  // workgroups per CU the register budget is tuned for
@@ -149,14 +152,23 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   __shared__ T sA[XA], sB[XA], sC[XA];
   __shared__ T sP[STOKES ? XP : 1];
 
+  constexpr bool SPLIT = (MODE == LN_MODE_SPLIT);  // all tiles in one launch, contributions to dst_u
   const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
-  const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
-  int bid = blockIdx.x;
-  const int tx = 2 * (bid % hx) + px;
-  bid /= hx;
-  const int ty = 2 * (bid % hy) + py;
-  bid /= hy;
-  const int tz = 2 * bid + pz;
+  int tx, ty, tz;
+  if constexpr (SPLIT) {
+    const int bid = blockIdx.x;
+    tx = bid % L.ntiles[0];
+    ty = (bid / L.ntiles[0]) % L.ntiles[1];
+    tz = bid / (L.ntiles[0] * L.ntiles[1]);
+  } else {
+    const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
+    int bid = blockIdx.x;
+    tx = 2 * (bid % hx) + px;
+    bid /= hx;
+    ty = 2 * (bid % hy) + py;
+    bid /= hy;
+    tz = 2 * bid + pz;
+  }
   const long long tile_id = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
   const int tid = threadIdx.x;
   const int c = tid & (CT - 1);
@@ -342,10 +354,16 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
       hyv[qx] = STa[0] * sA[XI(comp, qx, 3 * b)] + STa[1] * sA[XI(comp, qx, 3 * b + 1)] +
                 STa[2] * sA[XI(comp, qx, 3 * b + 2)];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] =
+    for (int i = 0; i < 3; ++i) {
+      const T o =
           madd(madd(B::S[0][i] == T(0) ? T(0) : B::S[0][i] * hyv[0], B::S[1][i], hyv[1]), B::S[2][i], hyv[2]);
+      if constexpr (SPLIT)
+        dst_u[tile_id * XA + (((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
+      else
+        sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
+    }
   }
+  if constexpr (SPLIT) return;
   __syncthreads();
 
   // ---- 7. tile write-out
@@ -398,6 +416,50 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   }
 }
 
+// Second pass of the split A-block apply: one thread per velocity DoF sums
+// the contributions of its <= 8 cells from the work array in a fixed order
+// (deterministic) and applies the mode; constrained rows take src (identity).
+template <int MODE, typename T>
+__global__ void k_gather_lines(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
+                               const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
+  constexpr int CT = 64, XA = 81 * CT;
+  const long long nn = L.vnodes;
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= 3 * nn) return;
+  const int comp = (int)(idx / nn);
+  const long long r = idx - comp * nn;
+  const int g[3] = {(int)(r % L.nv[0]), (int)((r / L.nv[0]) % L.nv[1]), (int)(r / ((long long)L.nv[0] * L.nv[1]))};
+  T val = 0;
+#pragma unroll
+  for (int sz = 0; sz < 2; ++sz) {
+    const int cz = (g[2] >> 1) - 1 + sz, iz = sz ? (g[2] & 1) : 2;
+    const bool vz = (sz ? true : ((g[2] & 1) == 0 && g[2] > 0)) && cz < L.n[2];
+#pragma unroll
+    for (int sy = 0; sy < 2; ++sy) {
+      const int cy = (g[1] >> 1) - 1 + sy, iy = sy ? (g[1] & 1) : 2;
+      const bool vy = (sy ? true : ((g[1] & 1) == 0 && g[1] > 0)) && cy < L.n[1];
+#pragma unroll
+      for (int sx = 0; sx < 2; ++sx) {
+        const int cx = (g[0] >> 1) - 1 + sx, ix = sx ? (g[0] & 1) : 2;
+        const bool v = vz && vy && (sx ? true : ((g[0] & 1) == 0 && g[0] > 0)) && cx < L.n[0];
+        const long long tile = (cx >> 2) + (long long)L.ntiles[0] * ((cy >> 2) + (long long)L.ntiles[1] * (cz >> 2));
+        const int cell = (cx & 3) + 4 * ((cy & 3) + 4 * (cz & 3));
+        const long long wi = v ? tile * XA + (long long)((((comp * 3 + iz) * 3 + iy) * 3 + ix) * CT + cell) : 0;
+        const T cval = work[wi];
+        val += v ? cval : (T)0;
+      }
+    }
+  }
+  const long long gi = comp * nn + r;
+  const T vv = con_bf(L, comp, g[0], g[1], g[2]) ? src_u[gi] : val;
+  if constexpr (MODE == MODE_SET)
+    dst_u[gi] = vv;
+  else if constexpr (MODE == MODE_RESID)
+    dst_u[gi] = rhs_u[gi] - vv;
+  else
+    dst_u[gi] += vv;
+}
+
 template <int OP, int MODE, typename T>
 void launch_lines_m(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru, const T *rp,
                     T *du, T *dp, hipStream_t st) {
@@ -409,6 +471,22 @@ void launch_lines_m(const LevelDesc &L, const T *su, const T *sp, const T *eta, 
     hipLaunchKernelGGL((k_lines<OP, MODE, T>), dim3((unsigned)nb), dim3(LN_THREADS), 0, st, L, color, su, sp, eta,
                        pq, ru, rp, du, dp);
   }
+  MFS_HIP(hipGetLastError());
+}
+
+// split A-block apply: per-cell contributions of all tiles in one launch, then the node-wise sum
+template <typename T>
+void launch_ablock_split(Mode mode, const LevelDesc &L, const T *su, const T *eta, int pq, const T *ru, T *du,
+                         T *work, hipStream_t st) {
+  hipLaunchKernelGGL((k_lines<OP_ABLOCK, LN_MODE_SPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(LN_THREADS), 0, st,
+                     L, 0, su, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work, (T *)nullptr);
+  const unsigned nb = (unsigned)((3 * L.vnodes + 255) / 256);
+  if (mode == MODE_SET)
+    hipLaunchKernelGGL((k_gather_lines<MODE_SET, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+  else if (mode == MODE_RESID)
+    hipLaunchKernelGGL((k_gather_lines<MODE_RESID, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+  else
+    hipLaunchKernelGGL((k_gather_lines<MODE_ADD, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
   MFS_HIP(hipGetLastError());
 }
 
@@ -425,10 +503,17 @@ void launch_lines_o(Mode mode, const LevelDesc &L, const T *su, const T *sp, con
 
 }  // namespace
 
+long long split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * 81 * 64; }
+
 template <typename T>
 bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
-                       int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st) {
+                       int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
+                       T *split_work) {
   if (L.dim != 3 || L.k != 2) return false;
+  if (op == OP_ABLOCK && split_work) {
+    launch_ablock_split<T>(mode, L, src_u, eta, eta_per_qp, rhs_u, dst_u, split_work, st);
+    return true;
+  }
   if (op == OP_STOKES)
     launch_lines_o<OP_STOKES, T>(mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
   else if (op == OP_ABLOCK)
@@ -439,8 +524,8 @@ bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, con
 }
 template bool launch_cell_lines<double>(Op, Mode, const LevelDesc &, const double *, const double *,
                                         const double *, int, const double *, const double *, double *, double *,
-                                        hipStream_t);
+                                        hipStream_t, double *);
 template bool launch_cell_lines<float>(Op, Mode, const LevelDesc &, const float *, const float *, const float *,
-                                       int, const float *, const float *, float *, float *, hipStream_t);
+                                       int, const float *, const float *, float *, float *, hipStream_t, float *);
 
 }  // namespace mfs

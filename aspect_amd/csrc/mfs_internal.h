@@ -50,13 +50,19 @@ void upload_basis_tables();
 
 template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
-                    int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st);
+                    int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
+                    T *split_work = nullptr);
 
 // 3D Q2 x Q1 Stokes and A-block applies with the line kernel; returns false
 // (nothing launched) for other (dim, degree, op).
+// split_work (A block only): split_work_size(L) elements; the apply then runs
+// as two launches over all tiles (per-cell contributions, node-wise sums)
+// instead of eight tile colours.
 template <typename T>
 bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
-                       int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st);
+                       int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
+                       T *split_work = nullptr);
+long long split_work_size(const LevelDesc &L);
 
 // Transfers between level Lc (coarse) and Lf = refined Lc.  kind 0 velocity, 1 pressure.
 // work: nullptr, or transfer_work_size(Lc, Lf) elements for the separable

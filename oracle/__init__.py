@@ -160,10 +160,10 @@ def _eta(eta):
     return eta, per_qp
 
 
-def stokes_vmult(p, level, eta, src):
+def stokes_vmult(p, level, eta, src, out=None):
     eta, per_qp = _eta(eta)
     src = np.ascontiguousarray(src, dtype=np.float64)
-    dst = np.zeros_like(src)
+    dst = np.zeros_like(src) if out is None else out
     _check(lib().orc_stokes_vmult(C.byref(p), level, _p(eta), per_qp, _p(src), _p(dst)))
     return dst
 
