@@ -558,17 +558,28 @@ struct mfs_handle {
     return chunks[c] + (long long)(i % 8) * n;
   }
 
+  // vectors [i0, i0 + MV_MAX) of a chunked basis
+  VecSet vecset(std::vector<double *> &chunks, int i0, int m) {
+    VecSet V{};
+    for (int c = 0; c * 8 < m; ++c) V.p[c] = vec(chunks, i0 + 8 * c);
+    return V;
+  }
+  // w += sign * sum_{i<m} hdev[i] B_i over a chunked basis
+  void basis_axpy(std::vector<double *> &chunks, int m, const double *hdev, double *w, double sign) {
+    for (int i0 = 0; i0 < m; i0 += MV_MAX) {
+      const int mm = std::min(MV_MAX, m - i0);
+      v_maxpy_all(vecset(chunks, i0, mm), n, mm, hdev + i0, w, n, sign, st);
+    }
+  }
+
   // h = V[0..m)^T w ; w -= V h   (one classical Gram-Schmidt pass), h to host
   void cgs_pass(int m, double *w, std::vector<double> &h) {
-    for (int c = 0; c * 8 < m; ++c) {
-      const int mg = std::min(8, m - 8 * c);
-      v_multidot_partial(vec(vchunks, 8 * c), n, 8 * c, mg, w, n, dot_work.p, st);
+    for (int i0 = 0; i0 < m; i0 += MV_MAX) {
+      const int mm = std::min(MV_MAX, m - i0);
+      v_mdot_all(vecset(vchunks, i0, mm), n, mm, w, n, dot_work.p + (long long)i0 * dot_work_size(), st);
     }
     v_sum_partials(dot_work.p, m, hbuf.p, st);
-    for (int c = 0; c * 8 < m; ++c) {
-      const int mg = std::min(8, m - 8 * c);
-      v_multiaxpy(vec(vchunks, 8 * c), n, mg, hbuf.p + 8 * c, w, n, -1.0, st);
-    }
+    basis_axpy(vchunks, m, hbuf.p, w, -1.0);
     h.resize(m);
     MFS_HIP(hipMemcpyAsync(h.data(), hbuf.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
     MFS_HIP(hipStreamSynchronize(st));
@@ -664,12 +675,10 @@ struct mfs_handle {
       }
       MFS_HIP(hipMemcpyAsync(hbuf.p, y.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
       if (flexible) {
-        for (int c = 0; c * 8 < m; ++c)
-          v_multiaxpy(vec(zchunks, 8 * c), n, std::min(8, m - 8 * c), hbuf.p + 8 * c, x, n, 1.0, st);
+        basis_axpy(zchunks, m, hbuf.p, x, 1.0);
       } else {
         v_fill<double>(w2.p, 0.0, n, st);
-        for (int c = 0; c * 8 < m; ++c)
-          v_multiaxpy(vec(vchunks, 8 * c), n, std::min(8, m - 8 * c), hbuf.p + 8 * c, w2.p, n, 1.0, st);
+        basis_axpy(vchunks, m, hbuf.p, w2.p, 1.0);
         P(w2.p, w3.p);
         v_axpby<double>(1.0, w3.p, 1.0, x, n, st);
       }

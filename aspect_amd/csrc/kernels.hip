@@ -1706,47 +1706,73 @@ void v_dot(const T *x, const T *y, long long n, double *work, double *out, hipSt
 template void v_dot<double>(const double *, const double *, long long, double *, double *, hipStream_t);
 template void v_dot<float>(const float *, const float *, long long, double *, double *, hipStream_t);
 
-// h[i] = V_i . w for i < m, processed in groups of 8 vectors per pass over w
-constexpr int MD_GROUP = 8;
-__global__ void __launch_bounds__(DOT_THREADS) k_multidot_partial(const double *__restrict__ V, long long ld, int m0,
-                                                                  int mg, const double *__restrict__ w, long long n,
-                                                                  double *__restrict__ part) {
-  __shared__ double sh[DOT_THREADS];
-  double acc[MD_GROUP];
-#pragma unroll
-  for (int g = 0; g < MD_GROUP; ++g) acc[g] = 0;
-  GRID_STRIDE(i, n) {
-    const double wi = w[i];
-#pragma unroll
-    for (int g = 0; g < MD_GROUP; ++g)
-      if (g < mg) acc[g] += V[g * ld + i] * wi;
-  }
-  for (int g = 0; g < mg; ++g) {
-    const double s = block_sum(acc[g], sh);
-    if (threadIdx.x == 0) part[(long long)(m0 + g) * DOT_BLOCKS + blockIdx.x] = s;
-    __syncthreads();
-  }
-}
-void v_multidot_partial(const double *Vchunk, long long ld, int m0, int mg, const double *w, long long n,
-                        double *work, hipStream_t st) {
-  hipLaunchKernelGGL(k_multidot_partial, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, st, Vchunk, ld, m0, mg, w,
-                     n, work);
-}
 void v_sum_partials(const double *work, int m, double *h, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_partials, dim3(m), dim3(DOT_THREADS), 0, st, work, m, DOT_BLOCKS, h);
 }
-__global__ void k_multiaxpy(const double *__restrict__ V, long long ld, int m, const double *__restrict__ h,
-                            double *__restrict__ w, long long n, double sign) {
+// Fused Gram-Schmidt kernels over up to MV_MAX Krylov vectors held in chunks of
+// 8 (vector i = p[i / 8] + (i % 8) * ld).  k_mdot_all: partial dots of every
+// vector with w, 16 vectors per sweep over w, wave-shuffle reductions in a
+// fixed order; k_maxpy_all: w += sign * sum_i h_i V_i, one read of each
+// vector and one read-modify-write of w.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__global__ void __launch_bounds__(DOT_THREADS) k_mdot_all(VecSet V, long long ld, int m, const double *__restrict__ w,
+                                                          long long n, double *__restrict__ part) {
+  constexpr int G = 16;
+  __shared__ double sh[G][DOT_THREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int g0 = 0; g0 < m; g0 += G) {
+    double acc[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) acc[k] = 0;
+    GRID_STRIDE(i, n) {
+      const double wi = w[i];
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+        if (g0 + k < m) acc[k] += V.p[(g0 + k) >> 3][((g0 + k) & 7) * ld + i] * wi;
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const double r = wave_sum(acc[k]);
+      if (lane == 0) sh[k][wv] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < G && g0 + (int)threadIdx.x < m) {
+      double t = 0;
+#pragma unroll
+      for (int q = 0; q < DOT_THREADS / 64; ++q) t += sh[threadIdx.x][q];
+      part[(long long)(g0 + threadIdx.x) * DOT_BLOCKS + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+__global__ void k_maxpy_all(VecSet V, long long ld, int m, const double *__restrict__ h, double *__restrict__ w,
+                            long long n, double sign) {
+  __shared__ double hs[MV_MAX];
+  for (int k = threadIdx.x; k < m; k += blockDim.x) hs[k] = h[k];
+  __syncthreads();
   GRID_STRIDE(i, n) {
     double acc = 0;
-    for (int g = 0; g < m; ++g) acc += h[g] * V[g * ld + i];
+    for (int k = 0; k < m; ++k) acc += hs[k] * V.p[k >> 3][(k & 7) * ld + i];
     w[i] += sign * acc;
   }
 }
-void v_multiaxpy(const double *Vchunk, long long ld, int mg, const double *h, double *w, long long n, double sign,
-                 hipStream_t st) {
-  hipLaunchKernelGGL(k_multiaxpy, dim3(nblocks(n)), dim3(256), 0, st, Vchunk, ld, mg, h, w, n, sign);
+void v_mdot_all(const VecSet &V, long long ld, int m, const double *w, long long n, double *work, hipStream_t st) {
+  if (m > MV_MAX) throw std::runtime_error("v_mdot_all: too many vectors");
+  hipLaunchKernelGGL(k_mdot_all, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, st, V, ld, m, w, n, work);
+  MFS_HIP(hipGetLastError());
 }
+void v_maxpy_all(const VecSet &V, long long ld, int m, const double *h, double *w, long long n, double sign,
+                 hipStream_t st) {
+  if (m > MV_MAX) throw std::runtime_error("v_maxpy_all: too many vectors");
+  hipLaunchKernelGGL(k_maxpy_all, dim3(std::min<long long>(nblocks(n), 8192)), dim3(256), 0, st, V, ld, m, h, w, n,
+                     sign);
+  MFS_HIP(hipGetLastError());
+}
+
 template <typename T>
 __global__ void k_mul(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ c, long long n) {
   GRID_STRIDE(i, n) c[i] = a[i] * b[i];

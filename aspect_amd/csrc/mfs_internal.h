@@ -125,13 +125,17 @@ void v_scale_neg_copy(const T *x, T *y, long long n, hipStream_t st);  // y = -x
 int dot_work_size();
 template <typename T>
 void v_dot(const T *x, const T *y, long long n, double *work, double *out, hipStream_t st);
-// Multi-dot over a chunk of mg vectors (stride ld): partial sums into
-// work[(m0+g)*dot_work_size() + block]; v_sum_partials reduces m of them into h.
-void v_multidot_partial(const double *Vchunk, long long ld, int m0, int mg, const double *w, long long n,
-                        double *work, hipStream_t st);
+// v_sum_partials reduces m partial-dot rows of work into h (device).
 void v_sum_partials(const double *work, int m, double *h, hipStream_t st);
-// w += sign * sum_{g<mg} h[g] Vchunk[g]
-void v_multiaxpy(const double *Vchunk, long long ld, int mg, const double *h, double *w, long long n, double sign,
+// Krylov vectors in chunks of 8: vector i = p[i / 8] + (i % 8) * ld, i < MV_MAX
+constexpr int MV_MAX = 128;
+struct VecSet {
+  const double *p[MV_MAX / 8];
+};
+// partial dots (V_i . w) for i < m into work[i * dot_work_size() + block]
+void v_mdot_all(const VecSet &V, long long ld, int m, const double *w, long long n, double *work, hipStream_t st);
+// w += sign * sum_{i<m} h[i] V_i (h on device)
+void v_maxpy_all(const VecSet &V, long long ld, int m, const double *h, double *w, long long n, double sign,
                  hipStream_t st);
 template <typename T>
 void v_mul(const T *a, const T *b, T *c, long long n, hipStream_t st);  // c = a * b
