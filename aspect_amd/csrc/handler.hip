@@ -690,6 +690,152 @@ struct mfs_handle {
     }
   }
 
+  // Restarted flexible GMRES (x += Z y) whose Arnoldi process uses delayed
+  // classical Gram-Schmidt (Swirydowicz et al., Numer. Linear Algebra Appl.
+  // 28(2), 2021; Bielich et al., Parallel Comput. 112, 2022): step j computes
+  // w = A P v_j with v_j still provisional (one projection pass, approximate
+  // norm), then ONE reduction gives V^T v_j, V^T w and the three inner
+  // products of v_j and w; from them v_j is reorthogonalized (the second
+  // Gram-Schmidt pass, one step late), column j-1 of the Hessenberg matrix is
+  // corrected and finalized, and w is projected; ONE sweep updates v_j and w.
+  // Same iterates as GMRES with classical Gram-Schmidt twice, at half its
+  // basis traffic.  Restart lengths up to MV_MAX - 1.
+  template <typename Prec>
+  KrylovStatus fgmres_dcgs(const double *b, double *x, double tol, int maxit, int restart, Prec P) {
+    KrylovStatus ks;
+    int acc = 0;
+    std::vector<double> red, coef;
+    while (true) {
+      double *v0 = vec(vchunks, 0);
+      stokes(MODE_RESID, x, b, v0);
+      double rho = std::sqrt(dot(v0, v0, n));
+      ks.res = rho;
+      if (rho <= tol) {
+        ks.converged = true;
+        ks.its = acc;
+        return ks;
+      }
+      if (acc >= maxit) {
+        ks.its = acc;
+        return ks;
+      }
+      v_scale(v0, 1.0 / rho, n, st);
+      std::vector<std::vector<double>> H;  // H[c]: column c, rows 0..c+1 (finalized columns are rotated)
+      std::vector<double> cs, sn, g(1, rho);
+      // Givens rotations of a column against the first c rotations, then its own
+      auto rotate = [&](std::vector<double> col, int c, std::vector<double> &cs_, std::vector<double> &sn_,
+                        std::vector<double> &g_) {
+        for (int i = 0; i < c; ++i) {
+          const double a = col[i], bb = col[i + 1];
+          col[i] = cs_[i] * a + sn_[i] * bb;
+          col[i + 1] = -sn_[i] * a + cs_[i] * bb;
+        }
+        const double den = std::hypot(col[c], col[c + 1]);
+        const double cc = den > 0 ? col[c] / den : 1.0, ss = den > 0 ? col[c + 1] / den : 0.0;
+        cs_.push_back(cc);
+        sn_.push_back(ss);
+        col[c] = den;
+        col[c + 1] = 0.0;
+        g_.push_back(-ss * g_[c]);
+        g_[c] = cc * g_[c];
+        return col;
+      };
+      bool done = false, provisional_conv = false;
+      int m = 0;  // finalized columns
+      for (int j = 0;; ++j) {
+        double *vj = vec(vchunks, j);
+        const bool at_limit = j == restart || acc >= maxit;
+        const bool finalize_only = at_limit || provisional_conv;
+        double *w = vj;
+        if (!finalize_only) {
+          ++acc;
+          double *z = vec(zchunks, j);
+          P(vj, z);
+          w = vec(vchunks, j + 1);
+          stokes(MODE_SET, z, nullptr, w);
+        }
+        const VecSet V = vecset(vchunks, 0, j);
+        v_dcgs_reduce(V, n, j, vj, w, n, dot_work.p, st);
+        const int rows = 2 * j + 3;
+        v_sum_partials(dot_work.p, rows, hbuf.p, st);
+        red.resize(rows);
+        MFS_HIP(hipMemcpyAsync(red.data(), hbuf.p, rows * sizeof(double), hipMemcpyDeviceToHost, st));
+        MFS_HIP(hipStreamSynchronize(st));
+        const double *sv = red.data(), *tv = red.data() + j;
+        const double alpha = red[2 * j], beta = red[2 * j + 1], gamma = red[2 * j + 2];
+        double ss = 0, st_ = 0, tt = 0;
+        for (int k = 0; k < j; ++k) {
+          ss += sv[k] * sv[k];
+          st_ += sv[k] * tv[k];
+          tt += tv[k] * tv[k];
+        }
+        // the first basis vector is normalized exactly
+        const double nu = j == 0 ? 1.0 : std::sqrt(std::max(alpha - ss, 0.0));
+        if (j > 0 && m < j) {
+          // finalize column j-1: its component along the provisional v_j
+          // splits into V s and nu v_j(final)
+          std::vector<double> &col = H[j - 1];
+          const double hj = col[j];
+          for (int k = 0; k < j; ++k) col[k] += hj * sv[k];
+          col[j] = hj * nu;
+          col = rotate(col, j - 1, cs, sn, g);
+          ks.res = std::fabs(g[j]);
+          m = j;
+          if (ks.res <= tol || nu == 0.0) {
+            ks.converged = true;
+            done = true;
+          }
+        }
+        if (done || at_limit) break;
+        if (finalize_only) {  // the provisional estimate was too optimistic: do the full step now
+          provisional_conv = false;
+          --j;
+          continue;
+        }
+        // provisional column j and the next provisional basis vector
+        const double hjj = (beta - st_) / nu;
+        std::vector<double> col(j + 2);
+        for (int k = 0; k < j; ++k) col[k] = tv[k];
+        col[j] = hjj;
+        const double r2 = gamma - tt - hjj * hjj;
+        const double rn = std::sqrt(std::max(r2, 1e-30 * gamma));
+        col[j + 1] = rn;
+        H.push_back(col);
+        coef.assign(rows, 0.0);
+        for (int k = 0; k < j; ++k) {
+          coef[k] = -sv[k] / nu;
+          coef[j + k] = -(tv[k] - sv[k] * hjj / nu) / rn;
+        }
+        coef[2 * j] = 1.0 / nu;
+        coef[2 * j + 1] = 1.0 / rn;
+        coef[2 * j + 2] = -hjj / (nu * rn);
+        MFS_HIP(hipMemcpyAsync(hbuf.p, coef.data(), rows * sizeof(double), hipMemcpyHostToDevice, st));
+        v_dcgs_update(V, n, j, hbuf.p, vj, w, n, st);
+        // residual estimate with the provisional column decides whether the
+        // next step only finalizes
+        std::vector<double> cs2 = cs, sn2 = sn, g2 = g;
+        rotate(col, j, cs2, sn2, g2);
+        provisional_conv = std::fabs(g2[j + 1]) <= tol;
+      }
+      // least squares with the m finalized columns, x += Z y
+      std::vector<double> y(m, 0.0);
+      for (int i = m - 1; i >= 0; --i) {
+        double sum = g[i];
+        for (int k = i + 1; k < m; ++k) sum -= H[k][i] * y[k];
+        y[i] = sum / H[i][i];
+      }
+      if (m > 0) {
+        MFS_HIP(hipMemcpyAsync(hbuf.p, y.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+        basis_axpy(zchunks, m, hbuf.p, x, 1.0);
+      }
+      MFS_HIP(hipStreamSynchronize(st));
+      if (done) {
+        ks.its = acc;
+        return ks;
+      }
+    }
+  }
+
   // IDR(s) with biorthogonalization (van Gijzen and Sonneveld, ACM TOMS 38(1),
   // 2011, Algorithm 913), deal.II's SolverIDR scheme for the cheap steps
   // (stokes_matrix_free_local_smoothing.cc:1291-1301).  One step = s inner
@@ -831,14 +977,14 @@ struct mfs_handle {
     MFS_HIP(hipEventRecord(e0, st));
     bool ok = false;
     if (prm.n_cheap_steps > 0) {
-      // With an FP32 V-cycle the preconditioner is linear only up to FP32
-      // rounding, so the cheap phase stores the preconditioned vectors
-      // (flexible update x += Z y) to keep the residual estimate exact.
+      // The cheap phase stores the preconditioned vectors (flexible update
+      // x += Z y): with an FP32 V-cycle the preconditioner is linear only up to
+      // FP32 rounding, and the flexible form keeps the residual estimate exact.
       const bool flexible = prec == MFS_GMG_FP32;
       auto pc = [&](const double *a, double *b) { block_prec_cheap(a, b); };
-      KrylovStatus ks = prm.cheap_krylov == MFS_KRYLOV_IDRS
-                            ? idrs(rhs, sol, tol, prm.n_cheap_steps, prm.idr_s, pc)
-                            : gmres(rhs, sol, tol, prm.n_cheap_steps, prm.restart, flexible, pc);
+      KrylovStatus ks = prm.cheap_krylov == MFS_KRYLOV_IDRS ? idrs(rhs, sol, tol, prm.n_cheap_steps, prm.idr_s, pc)
+                        : prm.restart < MV_MAX ? fgmres_dcgs(rhs, sol, tol, prm.n_cheap_steps, prm.restart, pc)
+                                               : gmres(rhs, sol, tol, prm.n_cheap_steps, prm.restart, flexible, pc);
       res.cheap_iterations = ks.its;
       res.final_linear_residual = ks.res;
       ok = ks.converged;
@@ -848,8 +994,9 @@ struct mfs_handle {
         if (v->n != nu) v->alloc(nu);
       for (DBuf<double> *v : {&cp1, &cp2, &cp3, &cp4})
         if (v->n != np) v->alloc(np);
-      KrylovStatus ks = gmres(rhs, sol, tol, prm.n_expensive_steps, prm.restart, true,
-                              [&](const double *a, double *b) { block_prec_expensive(a, b); });
+      auto pe = [&](const double *a, double *b) { block_prec_expensive(a, b); };
+      KrylovStatus ks = prm.restart < MV_MAX ? fgmres_dcgs(rhs, sol, tol, prm.n_expensive_steps, prm.restart, pe)
+                                             : gmres(rhs, sol, tol, prm.n_expensive_steps, prm.restart, true, pe);
       res.expensive_iterations = ks.its;
       res.final_linear_residual = ks.res;
       ok = ks.converged;

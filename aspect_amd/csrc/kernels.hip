@@ -1662,7 +1662,7 @@ void v_idr_shadow(double *q, int k, long long n, hipStream_t st) {
 }
 
 // ---- deterministic reductions: fixed grid of DOT_BLOCKS blocks, fixed-order tree
-constexpr int DOT_BLOCKS = 512;
+constexpr int DOT_BLOCKS = 2048;  // 8 workgroups per CU: enough loads in flight to stream HBM
 constexpr int DOT_THREADS = 256;
 int dot_work_size() { return DOT_BLOCKS; }
 
@@ -1759,6 +1759,99 @@ __global__ void k_maxpy_all(VecSet V, long long ld, int m, const double *__restr
     for (int k = 0; k < m; ++k) acc += hs[k] * V.p[k >> 3][(k & 7) * ld + i];
     w[i] += sign * acc;
   }
+}
+// Delayed classical Gram-Schmidt (one reduction and one update sweep per Arnoldi
+// step).  Reduction rows: s_k = V_k . v and t_k = V_k . w for k < j, then
+// v . v, v . w, w . w; 8 basis vectors per sweep over v and w.
+__global__ void __launch_bounds__(DOT_THREADS) k_dcgs_reduce(VecSet V, long long ld, int j, const double *__restrict__ v,
+                                                             const double *__restrict__ w, long long n,
+                                                             double *__restrict__ part) {
+  constexpr int G = 8;
+  __shared__ double sh[2 * G + 3][DOT_THREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int sweeps = j > 0 ? (j + G - 1) / G : 1;
+  for (int sw = 0; sw < sweeps; ++sw) {
+    const int g0 = sw * G;
+    double as[G], at[G], sa = 0, sb = 0, sc = 0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) as[k] = at[k] = 0;
+    GRID_STRIDE(i, n) {
+      const double vi = v[i], wi = w[i];
+      if (sw == 0) {
+        sa += vi * vi;
+        sb += vi * wi;
+        sc += wi * wi;
+      }
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+        if (g0 + k < j) {
+          const double xk = V.p[(g0 + k) >> 3][((g0 + k) & 7) * ld + i];
+          as[k] += xk * vi;
+          at[k] += xk * wi;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const double r1 = wave_sum(as[k]), r2 = wave_sum(at[k]);
+      if (lane == 0) {
+        sh[k][wv] = r1;
+        sh[G + k][wv] = r2;
+      }
+    }
+    {
+      const double r1 = wave_sum(sa), r2 = wave_sum(sb), r3 = wave_sum(sc);
+      if (lane == 0) {
+        sh[2 * G][wv] = r1;
+        sh[2 * G + 1][wv] = r2;
+        sh[2 * G + 2][wv] = r3;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * G + 3) {
+      const int q = threadIdx.x;
+      double t = 0;
+#pragma unroll
+      for (int u = 0; u < DOT_THREADS / 64; ++u) t += sh[q][u];
+      long long row = -1;
+      if (q < G && g0 + q < j) row = g0 + q;
+      else if (q >= G && q < 2 * G && g0 + q - G < j) row = j + g0 + q - G;
+      else if (q >= 2 * G && sw == 0) row = 2 * j + (q - 2 * G);
+      if (row >= 0) part[row * DOT_BLOCKS + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+// v <- c1 v + sum_k a1_k V_k;  w <- c2 w + c3 v_old + sum_k a2_k V_k; coef = [a1 (j), a2 (j), c1, c2, c3]
+__global__ void k_dcgs_update(VecSet V, long long ld, int j, const double *__restrict__ coef, double *__restrict__ v,
+                              double *__restrict__ w, long long n) {
+  __shared__ double cf[2 * MV_MAX + 3];
+  for (int k = threadIdx.x; k < 2 * j + 3; k += blockDim.x) cf[k] = coef[k];
+  __syncthreads();
+  const double c1 = cf[2 * j], c2 = cf[2 * j + 1], c3 = cf[2 * j + 2];
+  GRID_STRIDE(i, n) {
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < j; ++k) {
+      const double xk = V.p[k >> 3][(k & 7) * ld + i];
+      s1 += cf[k] * xk;
+      s2 += cf[j + k] * xk;
+    }
+    const double vo = v[i];
+    v[i] = c1 * vo + s1;
+    w[i] = c2 * w[i] + c3 * vo + s2;
+  }
+}
+void v_dcgs_reduce(const VecSet &V, long long ld, int j, const double *v, const double *w, long long n, double *work,
+                   hipStream_t st) {
+  if (j > MV_MAX) throw std::runtime_error("v_dcgs_reduce: too many vectors");
+  hipLaunchKernelGGL(k_dcgs_reduce, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, st, V, ld, j, v, w, n, work);
+  MFS_HIP(hipGetLastError());
+}
+void v_dcgs_update(const VecSet &V, long long ld, int j, const double *coef, double *v, double *w, long long n,
+                   hipStream_t st) {
+  if (j > MV_MAX) throw std::runtime_error("v_dcgs_update: too many vectors");
+  hipLaunchKernelGGL(k_dcgs_update, dim3(std::min<long long>(nblocks(n), 8192)), dim3(256), 0, st, V, ld, j, coef, v,
+                     w, n);
+  MFS_HIP(hipGetLastError());
 }
 void v_mdot_all(const VecSet &V, long long ld, int m, const double *w, long long n, double *work, hipStream_t st) {
   if (m > MV_MAX) throw std::runtime_error("v_mdot_all: too many vectors");
