@@ -138,12 +138,13 @@ struct GMG {
   bool graphs = true;
   hipGraphExec_t vexec[2] = {nullptr, nullptr};
   DBuf<T> xwork;  // work arrays of the separable transfers (sized for the finest pair)
-  // A-block applies on levels of at most SPLIT_TILES tiles run as two launches
-  // over all tiles through this work array instead of eight tile colours
-  // (those levels are bound by the latency of a tile, not by throughput)
-  static constexpr long long SPLIT_TILES = 4096;
+  // A-block applies in the V-cycle run as two launches over all tiles through
+  // this work array (per-cell contributions, then node-wise sums) instead of
+  // eight tile colours: on coarse levels a colour launch is bound by the latency
+  // of one tile, and on the fine level the work-array traffic costs less than
+  // the colour tails and the in-kernel gather
   DBuf<T> swork;
-  T *split_for(int l) { return lv[l]->L.ntiles_total <= SPLIT_TILES && swork.p ? swork.p : nullptr; }
+  T *split_for(int) { return swork.p; }
   void drop_graphs() {
     for (hipGraphExec_t &e : vexec)
       if (e) {
@@ -184,7 +185,7 @@ struct GMG {
     if (top > 0) xwork.alloc(transfer_work_size(lv[top - 1]->L, lv[top]->L));
     long long sw = 0;
     for (auto &g : lv)
-      if (g->L.dim == 3 && g->L.k == 2 && g->L.ntiles_total <= SPLIT_TILES) sw = std::max(sw, split_work_size(g->L));
+      if (g->L.dim == 3 && g->L.k == 2) sw = std::max(sw, split_work_size(g->L));
     if (sw > 0) swork.alloc(sw);
   }
 

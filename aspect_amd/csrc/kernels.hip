@@ -966,124 +966,12 @@ __device__ __forceinline__ bool con_node(const LevelDesc &L, int comp, int gx, i
   return con;
 }
 
-// one thread per fine node (all components)
-template <int KK, typename T>
-__global__ void k_prolongate3(const LevelDesc Lc, const LevelDesc Lf, const T *__restrict__ src,
-                              T *__restrict__ dst, const int add) {
-  constexpr bool VEL = KK == 2;
-  const int *nf = VEL ? Lf.nv : Lf.np;
-  const int *ncn = VEL ? Lc.nv : Lc.np;
-  const long long nfine = (long long)nf[0] * nf[1] * nf[2];
-  const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (f >= nfine) return;
-  const int g[3] = {(int)(f % nf[0]), (int)((f / nf[0]) % nf[1]), (int)(f / ((long long)nf[0] * nf[1]))};
-  int base[3];
-  double w[3][KK + 1];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    int e = g[d] / (2 * KK), r = g[d] - 2 * KK * e;
-    if (e == Lc.n[d]) {
-      e = Lc.n[d] - 1;
-      r = 2 * KK;
-    }
-    base[d] = KK * e;
-#pragma unroll
-    for (int m = 0; m <= KK; ++m) {
-      double v = ProlongTaps<KK>::W[0][m];
-#pragma unroll
-      for (int rr = 1; rr <= 2 * KK; ++rr) v = r == rr ? ProlongTaps<KK>::W[rr][m] : v;
-      w[d][m] = v;
-    }
-  }
-  const long long ncoarse = (long long)ncn[0] * ncn[1] * ncn[2];
-  constexpr int NC = VEL ? 3 : 1;
-#pragma unroll
-  for (int comp = 0; comp < NC; ++comp) {
-    double acc = 0;
-#pragma unroll
-    for (int mz = 0; mz <= KK; ++mz)
-#pragma unroll
-      for (int my = 0; my <= KK; ++my)
-#pragma unroll
-        for (int mx = 0; mx <= KK; ++mx) {
-          const double wt = w[2][mz] * w[1][my] * w[0][mx];
-          const int cx = base[0] + mx, cy = base[1] + my, cz = base[2] + mz;
-          const bool use = wt != 0.0 && !(VEL && con_node(Lc, comp, cx, cy, cz));
-          const long long ci = comp * ncoarse + cx + (long long)ncn[0] * (cy + (long long)ncn[1] * cz);
-          if (use) acc += wt * (double)src[ci];
-        }
-    if (VEL && con_node(Lf, comp, g[0], g[1], g[2])) acc = 0.0;
-    const long long gi = comp * nfine + f;
-    dst[gi] = add ? (T)(dst[gi] + (T)acc) : (T)acc;
-  }
-}
-
-// restriction = transpose: one thread per coarse node (all components).  Per
-// direction the fine nodes 2m + o feeding coarse node m: a coarse vertex takes
-// o = 0 (1), +-1 (3/8), +-3 (-1/8) for KK = 2 and o = 0 (1), +-1 (1/2) for KK = 1;
-// a coarse cell-midpoint node (KK = 2, m odd) takes o = 0 (1), +-1 (3/4).
-template <int KK, typename T>
-__global__ void k_restrict3(const LevelDesc Lc, const LevelDesc Lf, const T *__restrict__ src,
-                            T *__restrict__ dst) {
-  constexpr bool VEL = KK == 2;
-  constexpr int NO = VEL ? 5 : 3;
-  const int *nf = VEL ? Lf.nv : Lf.np;
-  const int *ncn = VEL ? Lc.nv : Lc.np;
-  const long long ncoarse = (long long)ncn[0] * ncn[1] * ncn[2];
-  const long long cidx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (cidx >= ncoarse) return;
-  const int m[3] = {(int)(cidx % ncn[0]), (int)((cidx / ncn[0]) % ncn[1]), (int)(cidx / ((long long)ncn[0] * ncn[1]))};
-  int gi[3][NO];
-  double wi[3][NO];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const bool vertex = !VEL || (m[d] & 1) == 0;
-#pragma unroll
-    for (int k = 0; k < NO; ++k) {
-      // taps ordered o = -3, -1, 0, 1, 3 (KK = 2) or -1, 0, 1 (KK = 1)
-      int o;
-      double wv;
-      if constexpr (VEL) {
-        const int oo[5] = {-3, -1, 0, 1, 3};
-        o = oo[k];
-        wv = vertex ? (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.375 : -0.125)) : (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.75 : 0.0));
-      } else {
-        o = k - 1;
-        wv = k == 1 ? 1.0 : 0.5;
-      }
-      const int g = 2 * m[d] + o;
-      const bool ok = g >= 0 && g < nf[d];
-      gi[d][k] = ok ? g : 0;
-      wi[d][k] = ok ? wv : 0.0;
-    }
-  }
-  const long long nfine = (long long)nf[0] * nf[1] * nf[2];
-  constexpr int NC = VEL ? 3 : 1;
-#pragma unroll
-  for (int comp = 0; comp < NC; ++comp) {
-    double acc = 0;
-    if (!(VEL && con_node(Lc, comp, m[0], m[1], m[2]))) {
-#pragma unroll
-      for (int a = 0; a < NO; ++a)
-#pragma unroll
-        for (int b = 0; b < NO; ++b)
-#pragma unroll
-          for (int c = 0; c < NO; ++c) {
-            const double wt = wi[2][a] * wi[1][b] * wi[0][c];
-            const bool use = wt != 0.0 && !(VEL && con_node(Lf, comp, gi[0][c], gi[1][b], gi[2][a]));
-            const long long fi = comp * nfine + gi[0][c] + (long long)nf[0] * (gi[1][b] + (long long)nf[1] * gi[2][a]);
-            if (use) acc += wt * (double)src[fi];
-          }
-    }
-    dst[comp * ncoarse + cidx] = (T)acc;
-  }
-}
-
 // Separable 3D transfers: the tensor-product prolongation P = Px Py Pz (and
 // R = P^T) applied one direction per pass through two work arrays.  Each pass
 // is a 1D stencil along one axis of an (n0, n1, n2) array per component:
-// restriction, coarse node m takes the fine nodes 2m + o (weights in
-// k_restrict3), prolongation, fine node g takes its coarse cell's nodes with
+// restriction, coarse node m takes the fine nodes 2m + o (a coarse vertex: o = 0
+// (1), +-1 (3/8), +-3 (-1/8) for KK = 2, o = 0 (1), +-1 (1/2) for KK = 1; a cell
+// midpoint node: o = 0 (1), +-1 (3/4)), prolongation, fine node g takes its coarse cell's nodes with
 // the row of ProlongTaps.  Constrained DoFs: the first restriction pass reads
 // constrained fine nodes as zero and the last one zeroes constrained coarse
 // nodes; the first prolongation pass reads constrained coarse nodes as zero
@@ -1247,11 +1135,7 @@ void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const
   const long long n = kind == 0 ? Lf.vnodes : Lf.pnodes;
   const int bs = 256;
   const unsigned nb = (unsigned)((n + bs - 1) / bs);
-  if (Lf.dim == 3 && Lf.k == 2 && kind == 0)
-    hipLaunchKernelGGL((k_prolongate3<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_c, dst_f, add ? 1 : 0);
-  else if (Lf.dim == 3 && Lf.k == 2 && kind == 1)
-    hipLaunchKernelGGL((k_prolongate3<1, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_c, dst_f, add ? 1 : 0);
-  else if (Lf.dim == 3)
+  if (Lf.dim == 3)
     hipLaunchKernelGGL((k_prolongate<3, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_c, dst_f, add ? 1 : 0);
   else
     hipLaunchKernelGGL((k_prolongate<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_c, dst_f, add ? 1 : 0);
@@ -1267,11 +1151,7 @@ void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T
   const long long n = kind == 0 ? Lc.vnodes : Lc.pnodes;
   const int bs = 256;
   const unsigned nb = (unsigned)((n + bs - 1) / bs);
-  if (Lf.dim == 3 && Lf.k == 2 && kind == 0)
-    hipLaunchKernelGGL((k_restrict3<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_f, dst_c);
-  else if (Lf.dim == 3 && Lf.k == 2 && kind == 1)
-    hipLaunchKernelGGL((k_restrict3<1, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, src_f, dst_c);
-  else if (Lf.dim == 3)
+  if (Lf.dim == 3)
     hipLaunchKernelGGL((k_restrict<3, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_f, dst_c);
   else
     hipLaunchKernelGGL((k_restrict<2, T>), dim3(nb), dim3(bs), 0, st, Lc, Lf, kind, src_f, dst_c);
