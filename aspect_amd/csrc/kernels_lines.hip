@@ -48,20 +48,8 @@ __device__ __forceinline__ T madd(T acc, T c, T x) {
   return c == T(0) ? acc : (c == T(1) ? acc + x : acc + c * x);
 }
 template <typename T>
-__device__ __forceinline__ T dot3(const T (&c)[3], T x0, T x1, T x2) {
-  return madd(madd(c[0] == T(0) ? T(0) : c[0] * x0, c[1], x1), c[2], x2);
-}
-
-// row r (or column r) of a 3x3 table, r uniform over the wave
-template <typename T>
-__device__ __forceinline__ void row3(const T (&M)[3][3], int r, T out[3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) out[k] = r == 0 ? M[0][k] : (r == 1 ? M[1][k] : M[2][k]);
-}
-template <typename T>
-__device__ __forceinline__ void col3(const T (&M)[3][3], int r, T out[3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) out[k] = r == 0 ? M[k][0] : (r == 1 ? M[k][1] : M[k][2]);
+__device__ __forceinline__ T dot3(T c0, T c1, T c2, T x0, T x1, T x2) {
+  return madd(madd(c0 == T(0) ? T(0) : c0 * x0, c1, x1), c2, x2);
 }
 
 // constrained velocity DoF test without branches (faces: 0 open, 1 no normal
@@ -207,37 +195,39 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 
   // wave-uniform rows and columns of the 1D tables
   T Sa[3], Sb[3], Da[3], Db[3], DTa[3], DTb[3], STa[3], STb[3];
-  row3(B::S, a, Sa);
-  row3(B::S, b, Sb);
-  row3(B::D, a, Da);
-  row3(B::D, b, Db);
-  col3(B::D, a, DTa);
-  col3(B::D, b, DTb);
-  col3(B::S, a, STa);
-  col3(B::S, b, STb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Sa[k] = B::S(a, k);
+    Sb[k] = B::S(b, k);
+    Da[k] = B::D(a, k);
+    Db[k] = B::D(b, k);
+    DTa[k] = B::D(k, a);
+    DTb[k] = B::D(k, b);
+    STa[k] = B::S(k, a);
+    STb[k] = B::S(k, b);
+  }
 
   // ---- 1. x and y interpolation: line (qy, kz) = (a, b), from the three node
   // lines (jy, kz), jy = 0..2, of the tile in LDS; pressure at the quadrature
   // line (qy, qz) = (a, b)
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
-    T vy[3] = {0, 0, 0};
+    T wy[3] = {0, 0, 0};  // y-interpolated node line
 #pragma unroll
     for (int jy = 0; jy < 3; ++jy) {
       const T *row = sC + comp * C::VN + ((2 * cz + b) * C::VY + (2 * cy + jy)) * C::VX + 2 * cx;
-      const T u0 = row[0], u1 = row[1], u2 = row[2];
 #pragma unroll
-      for (int qx = 0; qx < 3; ++qx) vy[qx] += Sa[jy] * dot3(B::S[qx], u0, u1, u2);
+      for (int i = 0; i < 3; ++i) wy[i] += Sa[jy] * row[i];
     }
 #pragma unroll
-    for (int qx = 0; qx < 3; ++qx) sA[XI(comp, qx, ln)] = vy[qx];
+    for (int qx = 0; qx < 3; ++qx) sA[XI(comp, qx, ln)] = dot3(B::S(qx, 0), B::S(qx, 1), B::S(qx, 2), wy[0], wy[1], wy[2]);
   }
   T pq[3] = {0, 0, 0};
   if constexpr (STOKES) {
-    const T SPa[2] = {a == 0 ? B::SP[0][0] : (a == 1 ? B::SP[1][0] : B::SP[2][0]),
-                      a == 0 ? B::SP[0][1] : (a == 1 ? B::SP[1][1] : B::SP[2][1])};
-    const T SPb[2] = {b == 0 ? B::SP[0][0] : (b == 1 ? B::SP[1][0] : B::SP[2][0]),
-                      b == 0 ? B::SP[0][1] : (b == 1 ? B::SP[1][1] : B::SP[2][1])};
+    const T SPa[2] = {a == 0 ? B::SP(0, 0) : (a == 1 ? B::SP(1, 0) : B::SP(2, 0)),
+                      a == 0 ? B::SP(0, 1) : (a == 1 ? B::SP(1, 1) : B::SP(2, 1))};
+    const T SPb[2] = {b == 0 ? B::SP(0, 0) : (b == 1 ? B::SP(1, 0) : B::SP(2, 0)),
+                      b == 0 ? B::SP(0, 1) : (b == 1 ? B::SP(1, 1) : B::SP(2, 1))};
     T pyz[2];  // pressure nodes interpolated in y and z to (a, b), per x node
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -250,7 +240,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
       pyz[i] = acc;
     }
 #pragma unroll
-    for (int qx = 0; qx < 3; ++qx) pq[qx] = B::SP[qx][0] * pyz[0] + B::SP[qx][1] * pyz[1];
+    for (int qx = 0; qx < 3; ++qx) pq[qx] = B::SP(qx, 0) * pyz[0] + B::SP(qx, 1) * pyz[1];
   }
   __syncthreads();
 
@@ -268,8 +258,8 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   // ---- 3. gradients, quadrature-point operation, x test; y and z fluxes to LDS
   const T ih0 = (T)(1.0 / L.h[0]), ih1 = (T)(1.0 / L.h[1]), ih2 = (T)(1.0 / L.h[2]);
   const T s = (T)L.s;
-  const double wab = L.h[0] * L.h[1] * L.h[2] * (a == 0 ? BD::W[0] : (a == 1 ? BD::W[1] : BD::W[2])) *
-                     (b == 0 ? BD::W[0] : (b == 1 ? BD::W[1] : BD::W[2]));
+  const double wab = L.h[0] * L.h[1] * L.h[2] * (a == 0 ? BD::W(0) : (a == 1 ? BD::W(1) : BD::W(2))) *
+                     (b == 0 ? BD::W(0) : (b == 1 ? BD::W(1) : BD::W(2)));
   T tacc[3][3];
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
@@ -278,12 +268,12 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   T hp0 = 0, hp1 = 0;
 #pragma unroll
   for (int qx = 0; qx < 3; ++qx) {
-    const T JxW = (T)(wab * BD::W[qx]);
+    const T JxW = (T)(wab * BD::W(qx));
     const T et = eta_per_qp ? eta[(tile_id * 27 + qx + 3 * ln) * CT + c] : eta[tile_id * CT + c];
     T gr[3][3];
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-      const T dx = dot3(B::D[qx], v[comp][0], v[comp][1], v[comp][2]);
+      const T dx = dot3(B::D(qx, 0), B::D(qx, 1), B::D(qx, 2), v[comp][0], v[comp][1], v[comp][2]);
       const T dy = Da[0] * sB[XI(comp, qx, 3 * b)] + Da[1] * sB[XI(comp, qx, 3 * b + 1)] +
                    Da[2] * sB[XI(comp, qx, 3 * b + 2)];
       const T dz = Db[0] * sB[XI(comp, qx, a)] + Db[1] * sB[XI(comp, qx, a + 3)] + Db[2] * sB[XI(comp, qx, a + 6)];
@@ -308,14 +298,14 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     for (int comp = 0; comp < 3; ++comp) {
       const T fx = flux[comp][0] * ih0;
 #pragma unroll
-      for (int r = 0; r < 3; ++r) tacc[comp][r] = madd(tacc[comp][r], B::D[qx][r], fx);
+      for (int r = 0; r < 3; ++r) tacc[comp][r] = madd(tacc[comp][r], B::D(qx, r), fx);
       sA[XI(comp, qx, ln)] = flux[comp][1] * ih1;
       sC[XI(comp, qx, ln)] = flux[comp][2] * ih2;
     }
     if constexpr (STOKES) {
       const T fp = -s * div * JxW;
-      hp0 += B::SP[qx][0] * fp;
-      hp1 += B::SP[qx][1] * fp;
+      hp0 += B::SP(qx, 0) * fp;
+      hp1 += B::SP(qx, 1) * fp;
     }
   }
   if constexpr (STOKES) {
@@ -356,7 +346,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const T o =
-          madd(madd(B::S[0][i] == T(0) ? T(0) : B::S[0][i] * hyv[0], B::S[1][i], hyv[1]), B::S[2][i], hyv[2]);
+          madd(madd(B::S(0, i) == T(0) ? T(0) : B::S(0, i) * hyv[0], B::S(1, i), hyv[1]), B::S(2, i), hyv[2]);
       if constexpr (SPLIT)
         dst_u[tile_id * XA + (((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
       else
@@ -394,7 +384,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
             T acc = 0;
 #pragma unroll
             for (int l = 0; l < NL; ++l)
-              acc += B::SP[l % 3][1 - e2] * B::SP[l / 3][1 - e3] * sP[((1 - e1) * NL + l) * CT + cell];
+              acc += B::SP(l % 3, 1 - e2) * B::SP(l / 3, 1 - e3) * sP[((1 - e1) * NL + l) * CT + cell];
             val += v ? acc : (T)0;
           }
         }
