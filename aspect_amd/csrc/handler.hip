@@ -96,6 +96,11 @@ LevelDesc make_level_desc(const mfs_box &b, int l) {
     L.pnodes *= L.np[d];
     L.ntiles_total *= L.ntiles[d];
   }
+  L.vol = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    L.ih[d] = 1.0 / L.h[d];
+    L.vol *= L.h[d];
+  }
   L.s = b.pressure_scaling;
   for (int f = 0; f < 6; ++f) L.bc[f] = (f < 2 * b.dim) ? b.bc[f] : 0;
   L.compressible = b.compressible;

@@ -65,26 +65,58 @@ __device__ __forceinline__ bool con_bf(const LevelDesc &L, int comp, int gx, int
   return con;
 }
 
+// r / 9 for 0 <= r < 81 as a multiply and shift
+__device__ __forceinline__ int div9(int r) { return (r * 57) >> 9; }
+constexpr bool div9_exact() {
+  for (int r = 0; r < 81; ++r)
+    if (((r * 57) >> 9) != r / 9) return false;
+  return true;
+}
+static_assert(div9_exact(), "div9 is exact on 0..80");
+
+// Node of a 4x4x4 degree-2 tile (9^3 nodes per component) handled by thread
+// tid in pass k of a sweep of the 576 threads over the 3 x 729 tile nodes:
+// t = tid + 576 k = 9 row + lx with row = comp * 81 + lz * 9 + ly, so
+// consecutive threads take consecutive x nodes.
+struct TileNode {
+  int comp, lx, ly, lz;
+  bool valid;
+};
+__device__ __forceinline__ TileNode tile_node(int tid, int k) {
+  TileNode n;
+  const int row = tid / 9 + 64 * k;
+  n.valid = row < 243;
+  n.comp = row >= 162 ? 2 : (row >= 81 ? 1 : 0);
+  const int rr = row - 81 * n.comp;
+  n.lz = div9(rr);
+  n.ly = rr - 9 * n.lz;
+  n.lx = tid - 9 * (tid / 9);
+  return n;
+}
+
 // Velocity write-out of a 4x4x4 tile, degree 2: every tile node (9^3 per
 // component) sums its at most 8 contributing cells in a fixed order, with the
 // two candidate cells per direction -- (l/2 - 1, local node 2) for even l > 0
 // and (l/2, local node l mod 2) -- evaluated as predicated reads, so all lanes
 // run the same instruction stream.  xb: per-cell contributions
 // [comp][kz][jy][i][cell].  Colours, constrained rows and modes as in
-// tile_write_u.
-template <int OP, int MODE, typename T>
+// tile_write_u.  INTERIOR: the tile holds no node of the domain boundary
+// (no constrained node, no partial cell range).
+template <int OP, int MODE, typename T, bool INTERIOR>
 __device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, const int tid, const int tx,
                                               const int ty, const int tz, const int px, const int py, const int pz,
                                               const T *__restrict__ src_u, const T *__restrict__ rhs_u,
                                               T *__restrict__ dst_u) {
-  constexpr int CT = 64, VX = 9, VN = 729, NT = 3 * VN;
-  const int ncx = min(4, L.n[0] - 4 * tx), ncy = min(4, L.n[1] - 4 * ty), ncz = min(4, L.n[2] - 4 * tz);
+  constexpr int CT = 64, VX = 9, NT = 3 * VX * VX * VX;
+  const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
+            ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
+  const long long tb = 8 * tx + (long long)L.nv[0] * (8 * ty + (long long)L.nv[1] * (8 * tz));
+  const long long plane = (long long)L.nv[0] * L.nv[1];
 #pragma unroll
   for (int k = 0; k < (NT + LN_THREADS - 1) / LN_THREADS; ++k) {
-    const int t = tid + k * LN_THREADS;
-    if (t >= NT) break;
-    const int comp = t / VN, r = t - comp * VN;
-    const int lx = r % VX, ly = (r / VX) % VX, lz = r / (VX * VX);
+    const TileNode n = tile_node(tid, k);
+    if (!n.valid) continue;
+    const int comp = n.comp, lx = n.lx, ly = n.ly, lz = n.lz;
     const int gx = 8 * tx + lx, gy = 8 * ty + ly, gz = 8 * tz + lz;
     T val = 0;
 #pragma unroll
@@ -105,13 +137,17 @@ __device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, c
         }
       }
     }
-    if (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2]) continue;
-    const bool shx = (lx == 0 && tx > 0) || (lx == VX - 1 && tx + 1 < L.ntiles[0]);
-    const bool shy = (ly == 0 && ty > 0) || (ly == VX - 1 && ty + 1 < L.ntiles[1]);
-    const bool shz = (lz == 0 && tz > 0) || (lz == VX - 1 && tz + 1 < L.ntiles[2]);
+    if (!INTERIOR && (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2])) continue;
+    const bool shx = INTERIOR ? (lx == 0 || lx == VX - 1)
+                              : ((lx == 0 && tx > 0) || (lx == VX - 1 && tx + 1 < L.ntiles[0]));
+    const bool shy = INTERIOR ? (ly == 0 || ly == VX - 1)
+                              : ((ly == 0 && ty > 0) || (ly == VX - 1 && ty + 1 < L.ntiles[1]));
+    const bool shz = INTERIOR ? (lz == 0 || lz == VX - 1)
+                              : ((lz == 0 && tz > 0) || (lz == VX - 1 && tz + 1 < L.ntiles[2]));
     const bool first = !((shx && px) || (shy && py) || (shz && pz));
-    const long long gi = comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz);
-    const bool con = con_bf(L, comp, gx, gy, gz);
+    const long long gi =
+        (comp == 0 ? 0 : (comp == 1 ? L.vnodes : 2 * L.vnodes)) + tb + lx + L.nv[0] * ly + plane * lz;
+    const bool con = INTERIOR ? false : con_bf(L, comp, gx, gy, gz);
     if (con && !first) continue;
     const T vv = con ? src_u[gi] : val;
     if constexpr (MODE == MODE_SET)
@@ -167,19 +203,40 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   const int ox0 = C::TX * tx, oy0 = C::TY * ty, oz0 = C::TZ * tz;
   auto XI = [&](int comp, int q, int l) { return ((comp * 3 + q) * NL + l) * CT + c; };
 
-  // ---- 0. cooperative tile load (constrained velocity DoFs read as zero)
+  // ---- 0. cooperative tile load (constrained velocity DoFs read as zero);
+  // interior tiles (no node on the domain boundary) skip the bounds and
+  // constraint tests
+  const bool interior = vx0 > 0 && vy0 > 0 && vz0 > 0 && vx0 + C::VX < L.nv[0] && vy0 + C::VY < L.nv[1] &&
+                        vz0 + C::VZ < L.nv[2];
+  {
+    const long long tb = vx0 + (long long)L.nv[0] * (vy0 + (long long)L.nv[1] * vz0);
+    const long long plane = (long long)L.nv[0] * L.nv[1];
+    static_assert(C::VX == 9 && C::VY == 9 && C::VZ == 9, "tile_node assumes 9^3 nodes");
+    if (interior) {
 #pragma unroll
-  for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
-    const int t = tid + k * LN_THREADS;
-    if (t >= 3 * C::VN) break;
-    const int comp = t / C::VN, r = t - comp * C::VN;
-    const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
-    const int gx = vx0 + lx, gy = vy0 + ly, gz = vz0 + lz;
-    const bool in = gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2];
-    const bool take = in && (L.plain_read || !con_bf(L, comp, gx, gy, gz));
-    const long long gi = in ? comp * L.vnodes + gx + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz) : 0;
-    const T v = src_u[gi];
-    sC[t] = take ? v : (T)0;
+      for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
+        const TileNode n = tile_node(tid, k);
+        if (!n.valid) continue;
+        const long long gi =
+            (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + tb + n.lx + L.nv[0] * n.ly + plane * n.lz;
+        sC[tid + k * LN_THREADS] = src_u[gi];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
+        const TileNode n = tile_node(tid, k);
+        if (!n.valid) continue;
+        const int gx = vx0 + n.lx, gy = vy0 + n.ly, gz = vz0 + n.lz;
+        const bool in = gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2];
+        const bool take = in && (L.plain_read || !con_bf(L, n.comp, gx, gy, gz));
+        const long long gi =
+            in ? (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + tb + n.lx + L.nv[0] * n.ly +
+                     plane * n.lz
+               : 0;
+        const T v = src_u[gi];
+        sC[tid + k * LN_THREADS] = take ? v : (T)0;
+      }
+    }
   }
   if constexpr (STOKES) {
     for (int t = tid; t < C::PN; t += LN_THREADS) {
@@ -256,9 +313,9 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   __syncthreads();
 
   // ---- 3. gradients, quadrature-point operation, x test; y and z fluxes to LDS
-  const T ih0 = (T)(1.0 / L.h[0]), ih1 = (T)(1.0 / L.h[1]), ih2 = (T)(1.0 / L.h[2]);
+  const T ih0 = (T)L.ih[0], ih1 = (T)L.ih[1], ih2 = (T)L.ih[2];
   const T s = (T)L.s;
-  const double wab = L.h[0] * L.h[1] * L.h[2] * (a == 0 ? BD::W(0) : (a == 1 ? BD::W(1) : BD::W(2))) *
+  const double wab = L.vol * (a == 0 ? BD::W(0) : (a == 1 ? BD::W(1) : BD::W(2))) *
                      (b == 0 ? BD::W(0) : (b == 1 ? BD::W(1) : BD::W(2)));
   T tacc[3][3];
 #pragma unroll
@@ -357,7 +414,10 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   __syncthreads();
 
   // ---- 7. tile write-out
-  lines_write_u<OP, MODE, T>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+  if (interior)
+    lines_write_u<OP, MODE, T, true>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+  else
+    lines_write_u<OP, MODE, T, false>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
   if constexpr (STOKES) {
     const int ncx = min(C::TX, L.n[0] - C::TX * tx), ncy = min(C::TY, L.n[1] - C::TY * ty),
               ncz = min(C::TZ, L.n[2] - C::TZ * tz);

@@ -34,6 +34,8 @@ struct LevelDesc {
   int ntiles[3];
   long long vnodes, pnodes, cells, ntiles_total;
   double h[3];
+  double ih[3];   // 1 / h
+  double vol;     // cell volume h0 h1 h2 (h0 h1 in 2D)
   double s;       // pressure scaling
   int bc[6];      // 0 open, 1 no normal flux, 2 zero velocity
   int compressible;
