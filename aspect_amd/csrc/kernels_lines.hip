@@ -94,49 +94,55 @@ __device__ __forceinline__ TileNode tile_node(int tid, int k) {
   return n;
 }
 
+// Contributing cells of tile node l along one direction (local cell stride
+// sc, local-node stride si in the per-cell contribution layout): candidate 0
+// is (l/2 - 1, local node 2), present for even l > 0; candidate 1 is
+// (l/2, local node l mod 2), present while l/2 < nc.
+__device__ __forceinline__ void node_cands(int l, int sc, int si, int nc, int off[2], bool ok[2]) {
+  off[0] = 2 * si + ((l >> 1) - 1) * sc;
+  ok[0] = (l & 1) == 0 && l > 0;
+  off[1] = (l & 1) * si + (l >> 1) * sc;
+  ok[1] = (l >> 1) < nc;
+}
+
 // Velocity write-out of a 4x4x4 tile, degree 2: every tile node (9^3 per
-// component) sums its at most 8 contributing cells in a fixed order, with the
-// two candidate cells per direction -- (l/2 - 1, local node 2) for even l > 0
-// and (l/2, local node l mod 2) -- evaluated as predicated reads, so all lanes
-// run the same instruction stream.  xb: per-cell contributions
-// [comp][kz][jy][i][cell].  Colours, constrained rows and modes as in
-// tile_write_u.  INTERIOR: the tile holds no node of the domain boundary
-// (no constrained node, no partial cell range).
+// component) sums its at most 8 contributing cells in a fixed order.  The 8
+// candidate reads are unconditional: a missing candidate reads xb[ZS], a slot
+// that holds zero, so all lanes run the same instruction stream.  xb:
+// per-cell contributions [comp][kz][jy][i][cell] followed by the zero slot.
+// Colours, constrained rows and modes as in tile_write_u.  INTERIOR: the
+// tile holds no node of the domain boundary (no constrained node, no partial
+// cell range).
 template <int OP, int MODE, typename T, bool INTERIOR>
 __device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, const int tid, const int tx,
                                               const int ty, const int tz, const int px, const int py, const int pz,
                                               const T *__restrict__ src_u, const T *__restrict__ rhs_u,
                                               T *__restrict__ dst_u) {
-  constexpr int CT = 64, VX = 9, NT = 3 * VX * VX * VX;
+  constexpr int CT = 64, VX = 9, NT = 3 * VX * VX * VX, ZS = 81 * CT;
   const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
             ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
   const long long tb = 8 * tx + (long long)L.nv[0] * (8 * ty + (long long)L.nv[1] * (8 * tz));
-  const long long plane = (long long)L.nv[0] * L.nv[1];
+  T *const dtile = dst_u + tb;
+  const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];  // node strides (a tile spans < 2^31 entries)
 #pragma unroll
   for (int k = 0; k < (NT + LN_THREADS - 1) / LN_THREADS; ++k) {
     const TileNode n = tile_node(tid, k);
     if (!n.valid) continue;
     const int comp = n.comp, lx = n.lx, ly = n.ly, lz = n.lz;
-    const int gx = 8 * tx + lx, gy = 8 * ty + ly, gz = 8 * tz + lz;
+    int ox[2], oy[2], oz[2];
+    bool wx[2], wy[2], wz[2];
+    node_cands(lx, 1, CT, ncx, ox, wx);
+    node_cands(ly, 4, 3 * CT, ncy, oy, wy);
+    node_cands(lz, 16, 9 * CT, ncz, oz, wz);
+    const int cb = comp * 27 * CT;
     T val = 0;
 #pragma unroll
-    for (int sz = 0; sz < 2; ++sz) {
-      const int cz = (lz >> 1) - 1 + sz, iz = sz ? (lz & 1) : 2;
-      const bool vz = (sz ? true : ((lz & 1) == 0 && lz > 0)) && cz < ncz;
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int sy = 0; sy < 2; ++sy) {
-        const int cy = (ly >> 1) - 1 + sy, iy = sy ? (ly & 1) : 2;
-        const bool vy = (sy ? true : ((ly & 1) == 0 && ly > 0)) && cy < ncy;
+      for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int sx = 0; sx < 2; ++sx) {
-          const int cx = (lx >> 1) - 1 + sx, ix = sx ? (lx & 1) : 2;
-          const bool v = vz && vy && (sx ? true : ((lx & 1) == 0 && lx > 0)) && cx < ncx;
-          const int idx = v ? (((comp * 3 + iz) * 3 + iy) * 3 + ix) * CT + cx + 4 * (cy + 4 * cz) : 0;
-          const T c = xb[idx];
-          val += v ? c : (T)0;
-        }
-      }
-    }
+        for (int e = 0; e < 2; ++e) val += xb[wz[a] && wy[b] && wx[e] ? cb + oz[a] + oy[b] + ox[e] : ZS];
+    const int gx = 8 * tx + lx, gy = 8 * ty + ly, gz = 8 * tz + lz;
     if (!INTERIOR && (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2])) continue;
     const bool shx = INTERIOR ? (lx == 0 || lx == VX - 1)
                               : ((lx == 0 && tx > 0) || (lx == VX - 1 && tx + 1 < L.ntiles[0]));
@@ -145,17 +151,16 @@ __device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, c
     const bool shz = INTERIOR ? (lz == 0 || lz == VX - 1)
                               : ((lz == 0 && tz > 0) || (lz == VX - 1 && tz + 1 < L.ntiles[2]));
     const bool first = !((shx && px) || (shy && py) || (shz && pz));
-    const long long gi =
-        (comp == 0 ? 0 : (comp == 1 ? L.vnodes : 2 * L.vnodes)) + tb + lx + L.nv[0] * ly + plane * lz;
+    const long long gi = (comp == 0 ? 0 : (comp == 1 ? L.vnodes : 2 * L.vnodes)) + (lx + sy * ly + sz * lz);
     const bool con = INTERIOR ? false : con_bf(L, comp, gx, gy, gz);
     if (con && !first) continue;
-    const T vv = con ? src_u[gi] : val;
+    const T vv = con ? src_u[tb + gi] : val;
     if constexpr (MODE == MODE_SET)
-      dst_u[gi] = first ? vv : dst_u[gi] + vv;
+      dtile[gi] = first ? vv : dtile[gi] + vv;
     else if constexpr (MODE == MODE_RESID)
-      dst_u[gi] = first ? rhs_u[gi] - vv : dst_u[gi] - vv;
+      dtile[gi] = first ? rhs_u[tb + gi] - vv : dtile[gi] - vv;
     else
-      dst_u[gi] += vv;
+      dtile[gi] += vv;
   }
 }
 
@@ -173,7 +178,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   constexpr bool STOKES = (OP == OP_STOKES);
   static_assert(CT == 64 && CT * NL == LN_THREADS, "one wave per line");
   static_assert(3 * C::VN <= XA && C::PN <= XP, "tile load fits the exchange arrays");
-  __shared__ T sA[XA], sB[XA], sC[XA];
+  __shared__ T sA[XA], sB[XA + 1], sC[XA];  // sB[XA]: zero slot of the write-out
   __shared__ T sP[STOKES ? XP : 1];
 
   constexpr bool SPLIT = (MODE == LN_MODE_SPLIT);  // all tiles in one launch, contributions to dst_u
@@ -206,6 +211,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   // ---- 0. cooperative tile load (constrained velocity DoFs read as zero);
   // interior tiles (no node on the domain boundary) skip the bounds and
   // constraint tests
+  if (tid == 0) sB[XA] = 0;
   const bool interior = vx0 > 0 && vy0 > 0 && vz0 > 0 && vx0 + C::VX < L.nv[0] && vy0 + C::VY < L.nv[1] &&
                         vz0 + C::VZ < L.nv[2];
   {
@@ -213,13 +219,14 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     const long long plane = (long long)L.nv[0] * L.nv[1];
     static_assert(C::VX == 9 && C::VY == 9 && C::VZ == 9, "tile_node assumes 9^3 nodes");
     if (interior) {
+      const T *const stile = src_u + tb;
 #pragma unroll
       for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
         const TileNode n = tile_node(tid, k);
         if (!n.valid) continue;
-        const long long gi =
-            (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + tb + n.lx + L.nv[0] * n.ly + plane * n.lz;
-        sC[tid + k * LN_THREADS] = src_u[gi];
+        const long long gi = (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) +
+                             (n.lx + L.nv[0] * n.ly + L.nv[0] * L.nv[1] * n.lz);
+        sC[tid + k * LN_THREADS] = stile[gi];
       }
     } else {
 #pragma unroll
