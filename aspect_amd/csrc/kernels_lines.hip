@@ -65,6 +65,15 @@ __device__ __forceinline__ bool con_bf(const LevelDesc &L, int comp, int gx, int
   return con;
 }
 
+// Workgroup barrier for LDS exchanges only: the fences order LDS accesses
+// (wait for this wave's LDS operations), not global ones, so global loads
+// issued ahead of the barrier (prefetches for later steps) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // r / 9 for 0 <= r < 81 as a multiply and shift
 __device__ __forceinline__ int div9(int r) { return (r * 57) >> 9; }
 constexpr bool div9_exact() {
@@ -105,62 +114,133 @@ __device__ __forceinline__ void node_cands(int l, int sc, int si, int nc, int of
   ok[1] = (l >> 1) < nc;
 }
 
+// Output node of the write-out sweep: offset from the tile origin, whether
+// it lies in the level, whether this tile is the first (lowest colour) to
+// write it, whether it is a constrained row.  INTERIOR: the tile holds no
+// node of the domain boundary (no constrained node, no partial cell range).
+struct OutNode {
+  long long gi;
+  bool in, first, con;
+};
+template <bool INTERIOR>
+__device__ __forceinline__ OutNode out_node(const LevelDesc &L, const TileNode &n, int tx, int ty, int tz, int px,
+                                            int py, int pz) {
+  constexpr int VX = 9;
+  OutNode o;
+  const int gx = 8 * tx + n.lx, gy = 8 * ty + n.ly, gz = 8 * tz + n.lz;
+  o.in = INTERIOR || (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2]);
+  const bool shx = INTERIOR ? (n.lx == 0 || n.lx == VX - 1)
+                            : ((n.lx == 0 && tx > 0) || (n.lx == VX - 1 && tx + 1 < L.ntiles[0]));
+  const bool shy = INTERIOR ? (n.ly == 0 || n.ly == VX - 1)
+                            : ((n.ly == 0 && ty > 0) || (n.ly == VX - 1 && ty + 1 < L.ntiles[1]));
+  const bool shz = INTERIOR ? (n.lz == 0 || n.lz == VX - 1)
+                            : ((n.lz == 0 && tz > 0) || (n.lz == VX - 1 && tz + 1 < L.ntiles[2]));
+  o.first = !((shx && px) || (shy && py) || (shz && pz));
+  const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];  // node strides (a tile spans < 2^31 entries)
+  o.gi = (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + (n.lx + sy * n.ly + sz * n.lz);
+  o.con = INTERIOR ? false : (o.in && con_bf(L, n.comp, gx, gy, gz));
+  return o;
+}
+
+constexpr int LN_WPASSES = (3 * 729 + LN_THREADS - 1) / LN_THREADS;  // write-out passes per thread
+
+// Pressure node t (of the 5^3 of a tile) in the write-out: global index,
+// whether it lies in the level, whether this tile is its first writer.
+struct PNode {
+  int lx, ly, lz;
+  long long gi;
+  bool in, first;
+};
+__device__ __forceinline__ PNode p_node(const LevelDesc &L, int t, int tx, int ty, int tz, int px, int py, int pz) {
+  constexpr int PX = 5;
+  PNode o;
+  o.lx = t % PX;
+  o.ly = (t / PX) % PX;
+  o.lz = t / (PX * PX);
+  const int gx = 4 * tx + o.lx, gy = 4 * ty + o.ly, gz = 4 * tz + o.lz;
+  o.in = gx < L.np[0] && gy < L.np[1] && gz < L.np[2];
+  const bool shx = (o.lx == 0 && tx > 0) || (o.lx == PX - 1 && tx + 1 < L.ntiles[0]);
+  const bool shy = (o.ly == 0 && ty > 0) || (o.ly == PX - 1 && ty + 1 < L.ntiles[1]);
+  const bool shz = (o.lz == 0 && tz > 0) || (o.lz == PX - 1 && tz + 1 < L.ntiles[2]);
+  o.first = !((shx && px) || (shy && py) || (shz && pz));
+  o.gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
+  return o;
+}
+
+// Global operands of the velocity write-out, loaded before the tile's
+// computation so that their latency overlaps it (earlier colours are earlier
+// launches, so dst is final for them).  Only loads happen here -- a value is
+// first used in the write-out, so no wait for these loads is needed before
+// then.  Per node: old = dst (SET, not the first writer), rhs (RESID, first
+// writer) or dst (otherwise); sold = src on constrained rows (boundary tiles).
+// Every load is issued: a lane with nothing to load reads the tile origin
+// (one cached line).
+template <int MODE, typename T, bool INTERIOR>
+__device__ __forceinline__ void lines_prefetch_u(const LevelDesc &L, const int tid, const int tx, const int ty,
+                                                 const int tz, const int px, const int py, const int pz,
+                                                 const T *__restrict__ src_u, const T *__restrict__ rhs_u,
+                                                 const T *__restrict__ dst_u, T (&old)[LN_WPASSES],
+                                                 T (&sold)[LN_WPASSES]) {
+  const long long tb = 8 * tx + (long long)L.nv[0] * (8 * ty + (long long)L.nv[1] * (8 * tz));
+#pragma unroll
+  for (int k = 0; k < LN_WPASSES; ++k) {
+    const TileNode n = tile_node(tid, k);
+    const OutNode o = out_node<INTERIOR>(L, n, tx, ty, tz, px, py, pz);
+    const bool act = n.valid && o.in;
+    const long long g = tb + (act ? o.gi : 0);
+    if constexpr (MODE == MODE_SET)
+      old[k] = dst_u[act && !o.first ? g : tb];
+    else if constexpr (MODE == MODE_RESID)
+      old[k] = (o.first ? rhs_u : dst_u)[g];
+    else
+      old[k] = dst_u[g];
+    if constexpr (!INTERIOR) sold[k] = src_u[act && o.con && o.first ? g : tb];
+  }
+}
+
 // Velocity write-out of a 4x4x4 tile, degree 2: every tile node (9^3 per
 // component) sums its at most 8 contributing cells in a fixed order.  The 8
 // candidate reads are unconditional: a missing candidate reads xb[ZS], a slot
 // that holds zero, so all lanes run the same instruction stream.  xb:
-// per-cell contributions [comp][kz][jy][i][cell] followed by the zero slot.
-// Colours, constrained rows and modes as in tile_write_u.  INTERIOR: the
-// tile holds no node of the domain boundary (no constrained node, no partial
-// cell range).
-template <int OP, int MODE, typename T, bool INTERIOR>
+// per-cell contributions [comp][kz][jy][i][cell] followed by the zero slot;
+// old: the prefetched operands.  Colours, constrained rows and modes as in
+// tile_write_u.
+template <int MODE, typename T, bool INTERIOR>
 __device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, const int tid, const int tx,
                                               const int ty, const int tz, const int px, const int py, const int pz,
-                                              const T *__restrict__ src_u, const T *__restrict__ rhs_u,
+                                              const T (&old)[LN_WPASSES], const T (&sold)[LN_WPASSES],
                                               T *__restrict__ dst_u) {
-  constexpr int CT = 64, VX = 9, NT = 3 * VX * VX * VX, ZS = 81 * CT;
+  constexpr int CT = 64, ZS = 81 * CT;
   const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
             ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
   const long long tb = 8 * tx + (long long)L.nv[0] * (8 * ty + (long long)L.nv[1] * (8 * tz));
   T *const dtile = dst_u + tb;
-  const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];  // node strides (a tile spans < 2^31 entries)
 #pragma unroll
-  for (int k = 0; k < (NT + LN_THREADS - 1) / LN_THREADS; ++k) {
+  for (int k = 0; k < LN_WPASSES; ++k) {
     const TileNode n = tile_node(tid, k);
-    if (!n.valid) continue;
-    const int comp = n.comp, lx = n.lx, ly = n.ly, lz = n.lz;
     int ox[2], oy[2], oz[2];
     bool wx[2], wy[2], wz[2];
-    node_cands(lx, 1, CT, ncx, ox, wx);
-    node_cands(ly, 4, 3 * CT, ncy, oy, wy);
-    node_cands(lz, 16, 9 * CT, ncz, oz, wz);
-    const int cb = comp * 27 * CT;
+    node_cands(n.lx, 1, CT, ncx, ox, wx);
+    node_cands(n.ly, 4, 3 * CT, ncy, oy, wy);
+    node_cands(n.lz, 16, 9 * CT, ncz, oz, wz);
+    const int cb = n.comp * 27 * CT;
     T val = 0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int e = 0; e < 2; ++e) val += xb[wz[a] && wy[b] && wx[e] ? cb + oz[a] + oy[b] + ox[e] : ZS];
-    const int gx = 8 * tx + lx, gy = 8 * ty + ly, gz = 8 * tz + lz;
-    if (!INTERIOR && (gx >= L.nv[0] || gy >= L.nv[1] || gz >= L.nv[2])) continue;
-    const bool shx = INTERIOR ? (lx == 0 || lx == VX - 1)
-                              : ((lx == 0 && tx > 0) || (lx == VX - 1 && tx + 1 < L.ntiles[0]));
-    const bool shy = INTERIOR ? (ly == 0 || ly == VX - 1)
-                              : ((ly == 0 && ty > 0) || (ly == VX - 1 && ty + 1 < L.ntiles[1]));
-    const bool shz = INTERIOR ? (lz == 0 || lz == VX - 1)
-                              : ((lz == 0 && tz > 0) || (lz == VX - 1 && tz + 1 < L.ntiles[2]));
-    const bool first = !((shx && px) || (shy && py) || (shz && pz));
-    const long long gi = (comp == 0 ? 0 : (comp == 1 ? L.vnodes : 2 * L.vnodes)) + (lx + sy * ly + sz * lz);
-    const bool con = INTERIOR ? false : con_bf(L, comp, gx, gy, gz);
-    if (con && !first) continue;
-    const T vv = con ? src_u[tb + gi] : val;
-    if constexpr (MODE == MODE_SET)
-      dtile[gi] = first ? vv : dtile[gi] + vv;
-    else if constexpr (MODE == MODE_RESID)
-      dtile[gi] = first ? rhs_u[tb + gi] - vv : dtile[gi] - vv;
-    else
-      dtile[gi] += vv;
+        for (int e = 0; e < 2; ++e)
+          val += xb[n.valid && wz[a] && wy[b] && wx[e] ? cb + oz[a] + oy[b] + ox[e] : ZS];
+    const OutNode o = out_node<INTERIOR>(L, n, tx, ty, tz, px, py, pz);
+    // constrained rows (first writer): SET src, RESID rhs - src, ADD dst + src
+    const T base = (MODE == MODE_SET && o.first) ? (T)0 : old[k];
+    T out = MODE == MODE_RESID ? base - val : base + val;
+    if constexpr (!INTERIOR) {
+      const T cv = MODE == MODE_SET ? sold[k] : (MODE == MODE_RESID ? old[k] - sold[k] : old[k] + sold[k]);
+      out = o.con ? cv : out;
+    }
+    if (n.valid && o.in && !(o.con && !o.first)) dtile[o.gi] = out;
   }
 }
 
@@ -215,47 +295,75 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   const bool interior = vx0 > 0 && vy0 > 0 && vz0 > 0 && vx0 + C::VX < L.nv[0] && vy0 + C::VY < L.nv[1] &&
                         vz0 + C::VZ < L.nv[2];
   {
-    const long long tb = vx0 + (long long)L.nv[0] * (vy0 + (long long)L.nv[1] * vz0);
-    const long long plane = (long long)L.nv[0] * L.nv[1];
+    // the loads of all passes are issued before the LDS stores (a lane past
+    // the last node, or at a node outside the level, reads the tile origin);
+    // rows t in [3 * 729, 4 * 576) of sC receive unused values
     static_assert(C::VX == 9 && C::VY == 9 && C::VZ == 9, "tile_node assumes 9^3 nodes");
+    static_assert(LN_WPASSES * LN_THREADS <= XA, "load passes fit sC");
+    const long long tb = vx0 + (long long)L.nv[0] * (vy0 + (long long)L.nv[1] * vz0);
+    const T *const stile = src_u + tb;
+    const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];
+    T nd[LN_WPASSES];
+    bool take[LN_WPASSES];
     if (interior) {
-      const T *const stile = src_u + tb;
 #pragma unroll
-      for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
+      for (int k = 0; k < LN_WPASSES; ++k) {
         const TileNode n = tile_node(tid, k);
-        if (!n.valid) continue;
-        const long long gi = (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) +
-                             (n.lx + L.nv[0] * n.ly + L.nv[0] * L.nv[1] * n.lz);
-        sC[tid + k * LN_THREADS] = stile[gi];
+        const long long gi =
+            n.valid ? (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + (n.lx + sy * n.ly + sz * n.lz)
+                    : 0;
+        nd[k] = stile[gi];
+        take[k] = true;
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < (3 * C::VN + LN_THREADS - 1) / LN_THREADS; ++k) {
+      for (int k = 0; k < LN_WPASSES; ++k) {
         const TileNode n = tile_node(tid, k);
-        if (!n.valid) continue;
         const int gx = vx0 + n.lx, gy = vy0 + n.ly, gz = vz0 + n.lz;
-        const bool in = gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2];
-        const bool take = in && (L.plain_read || !con_bf(L, n.comp, gx, gy, gz));
+        const bool in = n.valid && gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2];
+        take[k] = in && (L.plain_read || !con_bf(L, n.comp, gx, gy, gz));
         const long long gi =
-            in ? (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + tb + n.lx + L.nv[0] * n.ly +
-                     plane * n.lz
-               : 0;
-        const T v = src_u[gi];
-        sC[tid + k * LN_THREADS] = take ? v : (T)0;
+            in ? (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + (n.lx + sy * n.ly + sz * n.lz) : 0;
+        nd[k] = stile[gi];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < LN_WPASSES; ++k) sC[tid + k * LN_THREADS] = take[k] ? nd[k] : (T)0;
+  }
+  if constexpr (STOKES) {
+    const int t = min(tid, C::PN - 1);
+    const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
+    const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
+    const bool in = gx < L.np[0] && gy < L.np[1] && gz < L.np[2];
+    const T v = src_p[in ? gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz) : 0];
+    if (tid < C::PN) sP[tid] = in ? v : (T)0;
+  }
+  T wold[LN_WPASSES], wsold[LN_WPASSES];
+  T pold[2] = {0, 0};  // pressure write-out operands of wave 8 (nodes tid - 512, tid - 448)
+  if constexpr (!SPLIT) {
+    if (interior)
+      lines_prefetch_u<MODE, T, true>(L, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u, wold, wsold);
+    else
+      lines_prefetch_u<MODE, T, false>(L, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u, wold, wsold);
+    if constexpr (STOKES) {
+      if (ln == 8) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int t = tid - 8 * CT + j * CT;
+          const PNode o = p_node(L, min(t, C::PN - 1), tx, ty, tz, px, py, pz);
+          const bool act = t < C::PN && o.in;
+          if constexpr (MODE == MODE_SET) {
+            pold[j] = dst_p[act && !o.first ? o.gi : 0];  // used only when not the first writer
+          } else if constexpr (MODE == MODE_RESID) {
+            pold[j] = (o.first ? rhs_p : dst_p)[act ? o.gi : 0];
+          } else {
+            pold[j] = dst_p[act ? o.gi : 0];
+          }
+        }
       }
     }
   }
-  if constexpr (STOKES) {
-    for (int t = tid; t < C::PN; t += LN_THREADS) {
-      const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
-      const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
-      T v = 0;
-      if (gx < L.np[0] && gy < L.np[1] && gz < L.np[2])
-        v = src_p[gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz)];
-      sP[t] = v;
-    }
-  }
-  __syncthreads();
+  lds_barrier();
 
   // wave-uniform rows and columns of the 1D tables
   T Sa[3], Sb[3], Da[3], Db[3], DTa[3], DTb[3], STa[3], STb[3];
@@ -306,7 +414,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx) pq[qx] = B::SP(qx, 0) * pyz[0] + B::SP(qx, 1) * pyz[1];
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- 2. z interpolation: quadrature line (qy, qz) = (a, b)
   T v[3][3];
@@ -317,7 +425,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
       v[comp][qx] = Sb[0] * sA[XI(comp, qx, a)] + Sb[1] * sA[XI(comp, qx, a + 3)] + Sb[2] * sA[XI(comp, qx, a + 6)];
       sB[XI(comp, qx, ln)] = v[comp][qx];
     }
-  __syncthreads();
+  lds_barrier();
 
   // ---- 3. gradients, quadrature-point operation, x test; y and z fluxes to LDS
   const T ih0 = (T)L.ih[0], ih1 = (T)L.ih[1], ih2 = (T)L.ih[2];
@@ -376,9 +484,22 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     sP[ln * CT + c] = hp0;
     sP[(NL + ln) * CT + c] = hp1;
   }
-  __syncthreads();
+  lds_barrier();
 
-  // ---- 4. y and z tests at the quadrature line (qy, qz) = (a, b)
+  // ---- 4. y and z tests at the quadrature line (qy, qz) = (a, b); pressure:
+  // waves 0..7 integrate the x-integrated shares over y and z to the cell's
+  // pressure node (i, j, k) = (ln & 1, (ln >> 1) & 1, ln >> 2)
+  T pnode = 0;
+  if constexpr (STOKES) {
+    if (ln < 8) {
+      const int pi = ln & 1, pj = (ln >> 1) & 1, pk = ln >> 2;
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        const T wy = pj ? B::SP(l % 3, 1) : B::SP(l % 3, 0), wz = pk ? B::SP(l / 3, 1) : B::SP(l / 3, 0);
+        pnode += wy * wz * sP[(pi * NL + l) * CT + c];
+      }
+    }
+  }
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
@@ -387,16 +508,20 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
                              DTa[1] * sA[XI(comp, qx, 3 * b + 1)] + DTa[2] * sA[XI(comp, qx, 3 * b + 2)] +
                              DTb[0] * sC[XI(comp, qx, a)] + DTb[1] * sC[XI(comp, qx, a + 3)] +
                              DTb[2] * sC[XI(comp, qx, a + 6)];
-  __syncthreads();
+  lds_barrier();
 
-  // ---- 5. z integration: line (qy, kz) = (a, b)
+  // ---- 5. z integration: line (qy, kz) = (a, b); pressure node values to
+  // sP[node][cell] (no thread reads sP in this step)
+  if constexpr (STOKES) {
+    if (ln < 8) sP[ln * CT + c] = pnode;
+  }
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx)
       sA[XI(comp, qx, ln)] =
           STb[0] * sB[XI(comp, qx, a)] + STb[1] * sB[XI(comp, qx, a + 3)] + STb[2] * sB[XI(comp, qx, a + 6)];
-  __syncthreads();
+  lds_barrier();
 
   // ---- 6. y integration: node line (jy, kz) = (a, b), then x; per-cell
   // contributions [comp][kz][jy][i][cell]
@@ -418,57 +543,45 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     }
   }
   if constexpr (SPLIT) return;
-  __syncthreads();
+  lds_barrier();
 
   // ---- 7. tile write-out
   if (interior)
-    lines_write_u<OP, MODE, T, true>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+    lines_write_u<MODE, T, true>(L, sB, tid, tx, ty, tz, px, py, pz, wold, wsold, dst_u);
   else
-    lines_write_u<OP, MODE, T, false>(L, sB, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u);
+    lines_write_u<MODE, T, false>(L, sB, tid, tx, ty, tz, px, py, pz, wold, wsold, dst_u);
   if constexpr (STOKES) {
     const int ncx = min(C::TX, L.n[0] - C::TX * tx), ncy = min(C::TY, L.n[1] - C::TY * ty),
               ncz = min(C::TZ, L.n[2] - C::TZ * tz);
-    if (tid < C::PN) {
-      const int t = tid;
-      const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
-      const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
-      // contributing cells per direction: (l - 1, local index 1) and (l, local index 0)
+    // wave 8 (its velocity write-out ends a pass early): two pressure nodes per thread
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (ln != 8) break;
+      const int t = tid - 8 * CT + j * CT;
+      const PNode o = p_node(L, min(t, C::PN - 1), tx, ty, tz, px, py, pz);
+      // contributing cells per direction: (l - 1, local node 1) and (l, local node 0);
+      // sP[node][cell] with node = i + 2 j + 4 k
       T val = 0;
 #pragma unroll
       for (int e3 = 0; e3 < 2; ++e3) {
-        const int ccz = lz - 1 + e3;
+        const int ccz = o.lz - 1 + e3;
         const bool vz = ccz >= 0 && ccz < ncz;
 #pragma unroll
         for (int e2 = 0; e2 < 2; ++e2) {
-          const int ccy = ly - 1 + e2;
+          const int ccy = o.ly - 1 + e2;
           const bool vy = ccy >= 0 && ccy < ncy;
 #pragma unroll
           for (int e1 = 0; e1 < 2; ++e1) {
-            const int ccx = lx - 1 + e1;
+            const int ccx = o.lx - 1 + e1;
             const bool v = vz && vy && ccx >= 0 && ccx < ncx;
             const int cell = v ? ccx + C::TX * (ccy + C::TY * ccz) : 0;
-            // node index within the cell is 1 - e per direction
-            T acc = 0;
-#pragma unroll
-            for (int l = 0; l < NL; ++l)
-              acc += B::SP(l % 3, 1 - e2) * B::SP(l / 3, 1 - e3) * sP[((1 - e1) * NL + l) * CT + cell];
-            val += v ? acc : (T)0;
+            const T pv = sP[((1 - e1) + 2 * (1 - e2) + 4 * (1 - e3)) * CT + cell];
+            val += v ? pv : (T)0;
           }
         }
       }
-      if (gx < L.np[0] && gy < L.np[1] && gz < L.np[2]) {
-        const bool shx = (lx == 0 && tx > 0) || (lx == C::PX - 1 && tx + 1 < L.ntiles[0]);
-        const bool shy = (ly == 0 && ty > 0) || (ly == C::PY - 1 && ty + 1 < L.ntiles[1]);
-        const bool shz = (lz == 0 && tz > 0) || (lz == C::PZ - 1 && tz + 1 < L.ntiles[2]);
-        const bool first = !((shx && px) || (shy && py) || (shz && pz));
-        const long long gi = gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz);
-        if constexpr (MODE == MODE_SET)
-          dst_p[gi] = first ? val : dst_p[gi] + val;
-        else if constexpr (MODE == MODE_RESID)
-          dst_p[gi] = first ? rhs_p[gi] - val : dst_p[gi] - val;
-        else
-          dst_p[gi] += val;
-      }
+      const T base = (MODE == MODE_SET && o.first) ? (T)0 : pold[j];
+      if (t < C::PN && o.in) dst_p[o.gi] = MODE == MODE_RESID ? base - val : base + val;
     }
   }
 }
