@@ -167,20 +167,17 @@ __device__ __forceinline__ PNode p_node(const LevelDesc &L, int t, int tx, int t
   return o;
 }
 
-// Global operands of the velocity write-out, loaded before the tile's
-// computation so that their latency overlaps it (earlier colours are earlier
-// launches, so dst is final for them).  Only loads happen here -- a value is
-// first used in the write-out, so no wait for these loads is needed before
-// then.  Per node: old = dst (SET, not the first writer), rhs (RESID, first
-// writer) or dst (otherwise); sold = src on constrained rows (boundary tiles).
-// Every load is issued: a lane with nothing to load reads the tile origin
-// (one cached line).
+// Addresses of the global operands of the velocity write-out, which are
+// loaded before the tile's computation so that their latency overlaps it
+// (earlier colours are earlier launches, so dst is final for them).  Per
+// node: op -> dst (SET, not the first writer), rhs (RESID, first writer) or
+// dst (otherwise); sp -> src on constrained rows (boundary tiles).  A lane
+// with nothing to load points at the tile origin (one cached line).
 template <int MODE, typename T, bool INTERIOR>
-__device__ __forceinline__ void lines_prefetch_u(const LevelDesc &L, const int tid, const int tx, const int ty,
-                                                 const int tz, const int px, const int py, const int pz,
-                                                 const T *__restrict__ src_u, const T *__restrict__ rhs_u,
-                                                 const T *__restrict__ dst_u, T (&old)[LN_WPASSES],
-                                                 T (&sold)[LN_WPASSES]) {
+__device__ __forceinline__ void lines_prefetch_addr(const LevelDesc &L, const int tid, const int tx, const int ty,
+                                                    const int tz, const int px, const int py, const int pz,
+                                                    const T *src_u, const T *rhs_u, const T *dst_u,
+                                                    const T *(&op)[LN_WPASSES], const T *(&sp)[LN_WPASSES]) {
   const long long tb = 8 * tx + (long long)L.nv[0] * (8 * ty + (long long)L.nv[1] * (8 * tz));
 #pragma unroll
   for (int k = 0; k < LN_WPASSES; ++k) {
@@ -189,12 +186,12 @@ __device__ __forceinline__ void lines_prefetch_u(const LevelDesc &L, const int t
     const bool act = n.valid && o.in;
     const long long g = tb + (act ? o.gi : 0);
     if constexpr (MODE == MODE_SET)
-      old[k] = dst_u[act && !o.first ? g : tb];
+      op[k] = dst_u + (act && !o.first ? g : tb);
     else if constexpr (MODE == MODE_RESID)
-      old[k] = (o.first ? rhs_u : dst_u)[g];
+      op[k] = (o.first ? rhs_u : dst_u) + g;
     else
-      old[k] = dst_u[g];
-    if constexpr (!INTERIOR) sold[k] = src_u[act && o.con && o.first ? g : tb];
+      op[k] = dst_u + g;
+    sp[k] = src_u + (!INTERIOR && act && o.con && o.first ? g : tb);
   }
 }
 
@@ -294,6 +291,10 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   if (tid == 0) sB[XA] = 0;
   const bool interior = vx0 > 0 && vy0 > 0 && vz0 > 0 && vx0 + C::VX < L.nv[0] && vy0 + C::VY < L.nv[1] &&
                         vz0 + C::VZ < L.nv[2];
+  // tile values, stored to LDS after the later loads have been issued
+  T nd[LN_WPASSES];
+  bool take[LN_WPASSES];
+  T pnd = 0;
   {
     // the loads of all passes are issued before the LDS stores (a lane past
     // the last node, or at a node outside the level, reads the tile origin);
@@ -303,16 +304,14 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     const long long tb = vx0 + (long long)L.nv[0] * (vy0 + (long long)L.nv[1] * vz0);
     const T *const stile = src_u + tb;
     const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];
-    T nd[LN_WPASSES];
-    bool take[LN_WPASSES];
+    long long gofs[LN_WPASSES];
     if (interior) {
 #pragma unroll
       for (int k = 0; k < LN_WPASSES; ++k) {
         const TileNode n = tile_node(tid, k);
-        const long long gi =
+        gofs[k] =
             n.valid ? (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + (n.lx + sy * n.ly + sz * n.lz)
                     : 0;
-        nd[k] = stile[gi];
         take[k] = true;
       }
     } else {
@@ -322,29 +321,38 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
         const int gx = vx0 + n.lx, gy = vy0 + n.ly, gz = vz0 + n.lz;
         const bool in = n.valid && gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2];
         take[k] = in && (L.plain_read || !con_bf(L, n.comp, gx, gy, gz));
-        const long long gi =
+        gofs[k] =
             in ? (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + (n.lx + sy * n.ly + sz * n.lz) : 0;
-        nd[k] = stile[gi];
       }
     }
 #pragma unroll
-    for (int k = 0; k < LN_WPASSES; ++k) sC[tid + k * LN_THREADS] = take[k] ? nd[k] : (T)0;
+    for (int k = 0; k < LN_WPASSES; ++k) nd[k] = stile[gofs[k]];
   }
+  bool pin = false;
   if constexpr (STOKES) {
     const int t = min(tid, C::PN - 1);
     const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
     const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
-    const bool in = gx < L.np[0] && gy < L.np[1] && gz < L.np[2];
-    const T v = src_p[in ? gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz) : 0];
-    if (tid < C::PN) sP[tid] = in ? v : (T)0;
+    pin = gx < L.np[0] && gy < L.np[1] && gz < L.np[2];
+    pnd = src_p[pin ? gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz) : 0];
   }
+  // keep the tile loads ahead of the operand loads below: memory returns in
+  // issue order, so the LDS stores of the tile then wait for the tile only
+  __builtin_amdgcn_sched_barrier(0);
   T wold[LN_WPASSES], wsold[LN_WPASSES];
   T pold[2] = {0, 0};  // pressure write-out operands of wave 8 (nodes tid - 512, tid - 448)
   if constexpr (!SPLIT) {
+    const T *op[LN_WPASSES], *sp[LN_WPASSES];
     if (interior)
-      lines_prefetch_u<MODE, T, true>(L, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u, wold, wsold);
+      lines_prefetch_addr<MODE, T, true>(L, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u, op, sp);
     else
-      lines_prefetch_u<MODE, T, false>(L, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u, wold, wsold);
+      lines_prefetch_addr<MODE, T, false>(L, tid, tx, ty, tz, px, py, pz, src_u, rhs_u, dst_u, op, sp);
+#pragma unroll
+    for (int k = 0; k < LN_WPASSES; ++k) wold[k] = *op[k];
+    if (!interior) {
+#pragma unroll
+      for (int k = 0; k < LN_WPASSES; ++k) wsold[k] = *sp[k];
+    }
     if constexpr (STOKES) {
       if (ln == 8) {
 #pragma unroll
@@ -362,6 +370,16 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
         }
       }
     }
+  }
+  // viscosity at this thread's quadrature points (used in the gradient step)
+  T et_q[3];
+#pragma unroll
+  for (int qx = 0; qx < 3; ++qx)
+    et_q[qx] = eta[eta_per_qp ? (tile_id * 27 + qx + 3 * ln) * CT + c : tile_id * CT + c];
+#pragma unroll
+  for (int k = 0; k < LN_WPASSES; ++k) sC[tid + k * LN_THREADS] = take[k] ? nd[k] : (T)0;
+  if constexpr (STOKES) {
+    if (tid < C::PN) sP[tid] = pin ? pnd : (T)0;
   }
   lds_barrier();
 
@@ -441,7 +459,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
   for (int qx = 0; qx < 3; ++qx) {
     const T JxW = (T)(wab * BD::W(qx));
-    const T et = eta_per_qp ? eta[(tile_id * 27 + qx + 3 * ln) * CT + c] : eta[tile_id * CT + c];
+    const T et = et_q[qx];
     T gr[3][3];
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
