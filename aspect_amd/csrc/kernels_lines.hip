@@ -32,8 +32,8 @@ namespace mfs {
 namespace {
 
 constexpr int LN_THREADS = 576;  // 64 cells x 9 lines
-// internal mode of k_lines: every tile in one launch, per-cell contributions
-// written to a work array [tile][comp][kz][jy][i][cell], summed by k_gather_lines
+// internal mode of k_lines: every tile in one launch, in-tile node sums
+// written to a work array [tile][comp][lz][ly][lx], summed by k_gather_tiles
 constexpr int LN_MODE_SPLIT = 3;
 
 template <typename T>
@@ -195,40 +195,48 @@ __device__ __forceinline__ void lines_prefetch_addr(const LevelDesc &L, const in
   }
 }
 
-// Velocity write-out of a 4x4x4 tile, degree 2: every tile node (9^3 per
-// component) sums its at most 8 contributing cells in a fixed order.  The 8
-// candidate reads are unconditional: a missing candidate reads xb[ZS], a slot
+// Sum over the at most 8 cells of a 4x4x4 degree-2 tile that contribute to
+// tile node n, in a fixed order.  The 8 candidate reads are unconditional: a
+// missing candidate (or a thread past the last node) reads xb[ZS], a slot
 // that holds zero, so all lanes run the same instruction stream.  xb:
-// per-cell contributions [comp][kz][jy][i][cell] followed by the zero slot;
-// old: the prefetched operands.  Colours, constrained rows and modes as in
+// per-cell contributions [comp][kz][jy][i][cell] followed by the zero slot.
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ T tile_node_sum(const LevelDesc &L, const T *xb, const TileNode &n, const int tx,
+                                           const int ty, const int tz) {
+  constexpr int CT = 64, ZS = 81 * CT;
+  const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
+            ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
+  int ox[2], oy[2], oz[2];
+  bool wx[2], wy[2], wz[2];
+  node_cands(n.lx, 1, CT, ncx, ox, wx);
+  node_cands(n.ly, 4, 3 * CT, ncy, oy, wy);
+  node_cands(n.lz, 16, 9 * CT, ncz, oz, wz);
+  const int cb = n.comp * 27 * CT;
+  T val = 0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) val += xb[n.valid && wz[a] && wy[b] && wx[e] ? cb + oz[a] + oy[b] + ox[e] : ZS];
+  return val;
+}
+
+// Velocity write-out of a 4x4x4 tile, degree 2: every tile node sums its
+// contributing cells (tile_node_sum) and is written to dst; old/sold: the
+// prefetched operands.  Colours, constrained rows and modes as in
 // tile_write_u.
 template <int MODE, typename T, bool INTERIOR>
 __device__ __forceinline__ void lines_write_u(const LevelDesc &L, const T *xb, const int tid, const int tx,
                                               const int ty, const int tz, const int px, const int py, const int pz,
                                               const T (&old)[LN_WPASSES], const T (&sold)[LN_WPASSES],
                                               T *__restrict__ dst_u) {
-  constexpr int CT = 64, ZS = 81 * CT;
-  const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
-            ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
   const long long tb = 8 * tx + (long long)L.nv[0] * (8 * ty + (long long)L.nv[1] * (8 * tz));
   T *const dtile = dst_u + tb;
 #pragma unroll
   for (int k = 0; k < LN_WPASSES; ++k) {
     const TileNode n = tile_node(tid, k);
-    int ox[2], oy[2], oz[2];
-    bool wx[2], wy[2], wz[2];
-    node_cands(n.lx, 1, CT, ncx, ox, wx);
-    node_cands(n.ly, 4, 3 * CT, ncy, oy, wy);
-    node_cands(n.lz, 16, 9 * CT, ncz, oz, wz);
-    const int cb = n.comp * 27 * CT;
-    T val = 0;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int e = 0; e < 2; ++e)
-          val += xb[n.valid && wz[a] && wy[b] && wx[e] ? cb + oz[a] + oy[b] + ox[e] : ZS];
+    const T val = tile_node_sum<T, INTERIOR>(L, xb, n, tx, ty, tz);
     const OutNode o = out_node<INTERIOR>(L, n, tx, ty, tz, px, py, pz);
     // constrained rows (first writer): SET src, RESID rhs - src, ADD dst + src
     const T base = (MODE == MODE_SET && o.first) ? (T)0 : old[k];
@@ -554,14 +562,25 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
     for (int i = 0; i < 3; ++i) {
       const T o =
           madd(madd(B::S(0, i) == T(0) ? T(0) : B::S(0, i) * hyv[0], B::S(1, i), hyv[1]), B::S(2, i), hyv[2]);
-      if constexpr (SPLIT)
-        dst_u[tile_id * XA + (((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
-      else
-        sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
+      sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
     }
   }
-  if constexpr (SPLIT) return;
   lds_barrier();
+
+  if constexpr (SPLIT) {
+    // in-tile node sums of all 3 x 9^3 tile nodes to the work array
+    // [tile][comp][lz][ly][lx] (node t = tid + 576 k); k_gather_tiles adds
+    // the tiles sharing a node
+    T *const wt = dst_u + tile_id * (3 * C::VN);
+#pragma unroll
+    for (int k = 0; k < LN_WPASSES; ++k) {
+      const TileNode n = tile_node(tid, k);
+      const T val = interior ? tile_node_sum<T, true>(L, sB, n, tx, ty, tz)
+                             : tile_node_sum<T, false>(L, sB, n, tx, ty, tz);
+      if (n.valid) wt[tid + k * LN_THREADS] = val;
+    }
+    return;
+  }
 
   // ---- 7. tile write-out
   if (interior)
@@ -605,39 +624,45 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 }
 
 // Second pass of the split A-block apply: one thread per velocity DoF sums
-// the contributions of its <= 8 cells from the work array in a fixed order
-// (deterministic) and applies the mode; constrained rows take src (identity).
+// the in-tile sums of the at most 8 tiles sharing the node from the work
+// array [tile][comp][lz][ly][lx] in a fixed order (deterministic) and applies
+// the mode; constrained rows take src (identity).  Per direction the
+// candidates of node g are (tile g/8 - 1, local node 8) for g mod 8 == 0,
+// g > 0, and (tile g/8, local node g mod 8) while g/8 < ntiles.
 template <int MODE, typename T>
-__global__ void k_gather_lines(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
+__global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
                                const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
-  constexpr int CT = 64, XA = 81 * CT;
+  constexpr int TN = 3 * 729;
   const long long nn = L.vnodes;
   const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= 3 * nn) return;
   const int comp = (int)(idx / nn);
   const long long r = idx - comp * nn;
   const int g[3] = {(int)(r % L.nv[0]), (int)((r / L.nv[0]) % L.nv[1]), (int)(r / ((long long)L.nv[0] * L.nv[1]))};
+  int tt[3][2], ll[3][2];
+  bool ok[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    tt[d][0] = (g[d] >> 3) - 1;
+    ll[d][0] = 8;
+    ok[d][0] = (g[d] & 7) == 0 && g[d] > 0;
+    tt[d][1] = g[d] >> 3;
+    ll[d][1] = g[d] & 7;
+    ok[d][1] = (g[d] >> 3) < L.ntiles[d];
+  }
   T val = 0;
 #pragma unroll
-  for (int sz = 0; sz < 2; ++sz) {
-    const int cz = (g[2] >> 1) - 1 + sz, iz = sz ? (g[2] & 1) : 2;
-    const bool vz = (sz ? true : ((g[2] & 1) == 0 && g[2] > 0)) && cz < L.n[2];
+  for (int sz = 0; sz < 2; ++sz)
 #pragma unroll
-    for (int sy = 0; sy < 2; ++sy) {
-      const int cy = (g[1] >> 1) - 1 + sy, iy = sy ? (g[1] & 1) : 2;
-      const bool vy = (sy ? true : ((g[1] & 1) == 0 && g[1] > 0)) && cy < L.n[1];
+    for (int sy = 0; sy < 2; ++sy)
 #pragma unroll
       for (int sx = 0; sx < 2; ++sx) {
-        const int cx = (g[0] >> 1) - 1 + sx, ix = sx ? (g[0] & 1) : 2;
-        const bool v = vz && vy && (sx ? true : ((g[0] & 1) == 0 && g[0] > 0)) && cx < L.n[0];
-        const long long tile = (cx >> 2) + (long long)L.ntiles[0] * ((cy >> 2) + (long long)L.ntiles[1] * (cz >> 2));
-        const int cell = (cx & 3) + 4 * ((cy & 3) + 4 * (cz & 3));
-        const long long wi = v ? tile * XA + (long long)((((comp * 3 + iz) * 3 + iy) * 3 + ix) * CT + cell) : 0;
-        const T cval = work[wi];
-        val += v ? cval : (T)0;
+        const bool v = ok[2][sz] && ok[1][sy] && ok[0][sx];
+        const long long tile = tt[0][sx] + (long long)L.ntiles[0] * (tt[1][sy] + (long long)L.ntiles[1] * tt[2][sz]);
+        const long long wi = v ? tile * TN + ((comp * 9 + ll[2][sz]) * 9 + ll[1][sy]) * 9 + ll[0][sx] : 0;
+        const T c = work[wi];
+        val += v ? c : (T)0;
       }
-    }
-  }
   const long long gi = comp * nn + r;
   const T vv = con_bf(L, comp, g[0], g[1], g[2]) ? src_u[gi] : val;
   if constexpr (MODE == MODE_SET)
@@ -670,11 +695,11 @@ void launch_ablock_split(Mode mode, const LevelDesc &L, const T *su, const T *et
                      L, 0, su, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work, (T *)nullptr);
   const unsigned nb = (unsigned)((3 * L.vnodes + 255) / 256);
   if (mode == MODE_SET)
-    hipLaunchKernelGGL((k_gather_lines<MODE_SET, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_SET, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
   else if (mode == MODE_RESID)
-    hipLaunchKernelGGL((k_gather_lines<MODE_RESID, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
   else
-    hipLaunchKernelGGL((k_gather_lines<MODE_ADD, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
   MFS_HIP(hipGetLastError());
 }
 
@@ -691,7 +716,7 @@ void launch_lines_o(Mode mode, const LevelDesc &L, const T *su, const T *sp, con
 
 }  // namespace
 
-long long split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * 81 * 64; }
+long long split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * 3 * 729; }
 
 template <typename T>
 bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
