@@ -96,12 +96,8 @@ LevelDesc make_level_desc(const mfs_box &b, int l) {
     L.pnodes *= L.np[d];
     L.ntiles_total *= L.ntiles[d];
   }
-  L.vol = 1.0;
-  for (int d = 0; d < 3; ++d) {
-    L.ih[d] = 1.0 / L.h[d];
-    L.vol *= L.h[d];
-  }
   L.s = b.pressure_scaling;
+  level_derive(L);
   for (int f = 0; f < 6; ++f) L.bc[f] = (f < 2 * b.dim) ? b.bc[f] : 0;
   L.compressible = b.compressible;
   return L;
@@ -1089,6 +1085,7 @@ void create(const mfs_box *b, int prec, mfs_handle **out) {
   {
     LevelDesc Lu = h->Ltop();
     Lu.s = 1.0;
+    level_derive(Lu);
     h->ones_cell.alloc(Lu.ntiles_total * h->ct);
     v_fill<double>(h->ones_cell.p, 1.0, h->ones_cell.n, h->st);
     h->pnorm_w.alloc(h->np);
@@ -1280,9 +1277,15 @@ int mfs_set_pressure_scaling(mfs_handle *h, double s) {
   return guard([&] {
     if (!(s > 0)) throw std::runtime_error("pressure scaling must be positive");
     h->box.pressure_scaling = s;
-    for (auto &L : h->L) L.s = s;
+    for (auto &L : h->L) {
+      L.s = s;
+      level_derive(L);
+    }
     auto upd = [&](auto &g) {
-      for (auto &lv : g.lv) lv->L.s = s;
+      for (auto &lv : g.lv) {
+        lv->L.s = s;
+        level_derive(lv->L);
+      }
     };
     upd(h->g32);
     upd(h->g64);

@@ -138,6 +138,23 @@ void prolong_table(int kk, double *tab, int ld) {
 }
 }  // namespace
 
+void level_derive(LevelDesc &L) {
+  L.vol = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    L.ih[d] = 1.0 / L.h[d];
+    L.vol *= L.h[d];
+  }
+  for (int i = 0; i < 3; ++i) {
+    L.ih_f[i] = (float)L.ih[i];
+    for (int k = 0; k < 3; ++k) {
+      L.ihw[i][k] = L.ih[i] * L.ih[k];
+      L.ihw_f[i][k] = (float)L.ihw[i][k];
+    }
+  }
+  L.vol_f = (float)L.vol;
+  L.s_f = (float)L.s;
+}
+
 void upload_basis_tables() {
   static bool done = false;
   if (done) return;

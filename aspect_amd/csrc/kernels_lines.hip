@@ -454,10 +454,15 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   lds_barrier();
 
   // ---- 3. gradients, quadrature-point operation, x test; y and z fluxes to LDS
-  const T ih0 = (T)L.ih[0], ih1 = (T)L.ih[1], ih2 = (T)L.ih[2];
-  const T s = (T)L.s;
-  const double wab = L.vol * (a == 0 ? BD::W(0) : (a == 1 ? BD::W(1) : BD::W(2))) *
-                     (b == 0 ? BD::W(0) : (b == 1 ? BD::W(1) : BD::W(2)));
+  // Physical gradient g_ik = d_ik ih_k (d: derivative on the unit cell), stress
+  // sigma = eta (g + g^T) - (s p [+ 2/3 eta div u]) I, and the fluxes that the
+  // tests take, f_ik = sigma_ik JxW ih_k, folded into
+  //   f_ik = c (w_kk d_ik + w_ik d_ki) (i != k),  f_ii = 2 c w_ii d_ii - P JxW ih_i
+  // with c = eta JxW and w_ik = ih_i ih_k.
+  using LC = LevelConst<T>;
+  const T s = LC::s(L);
+  const T wab = LC::vol(L) * (a == 0 ? B::W(0) : (a == 1 ? B::W(1) : B::W(2))) *
+                (b == 0 ? B::W(0) : (b == 1 ? B::W(1) : B::W(2)));
   T tacc[3][3];
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp)
@@ -466,39 +471,49 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   T hp0 = 0, hp1 = 0;
 #pragma unroll
   for (int qx = 0; qx < 3; ++qx) {
-    const T JxW = (T)(wab * BD::W(qx));
-    const T et = et_q[qx];
-    T gr[3][3];
+    const T JxW = wab * B::W(qx);
+    const T cq = et_q[qx] * JxW;
+    T d[3][3];  // d[comp][direction]
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-      const T dx = dot3(B::D(qx, 0), B::D(qx, 1), B::D(qx, 2), v[comp][0], v[comp][1], v[comp][2]);
-      const T dy = Da[0] * sB[XI(comp, qx, 3 * b)] + Da[1] * sB[XI(comp, qx, 3 * b + 1)] +
+      d[comp][0] = dot3(B::D(qx, 0), B::D(qx, 1), B::D(qx, 2), v[comp][0], v[comp][1], v[comp][2]);
+      d[comp][1] = Da[0] * sB[XI(comp, qx, 3 * b)] + Da[1] * sB[XI(comp, qx, 3 * b + 1)] +
                    Da[2] * sB[XI(comp, qx, 3 * b + 2)];
-      const T dz = Db[0] * sB[XI(comp, qx, a)] + Db[1] * sB[XI(comp, qx, a + 3)] + Db[2] * sB[XI(comp, qx, a + 6)];
-      gr[comp][0] = dx * ih0;
-      gr[comp][1] = dy * ih1;
-      gr[comp][2] = dz * ih2;
+      d[comp][2] = Db[0] * sB[XI(comp, qx, a)] + Db[1] * sB[XI(comp, qx, a + 3)] + Db[2] * sB[XI(comp, qx, a + 6)];
     }
-    const T div = gr[0][0] + gr[1][1] + gr[2][2];
+    const T div = d[0][0] * LC::ih(L, 0) + d[1][1] * LC::ih(L, 1) + d[2][2] * LC::ih(L, 2);
+    T pdiag = 0;  // P JxW
+    if (L.compressible) pdiag = (T)(2.0 / 3.0) * cq * div;
+    if constexpr (STOKES) pdiag += s * pq[qx] * JxW;
     T flux[3][3];
+    if constexpr (std::is_same<T, float>::value) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        T sig = et * (gr[i][k] + gr[k][i]);
-        if (i == k) {
-          if (L.compressible) sig -= 2 * et / 3 * div;
-          if constexpr (STOKES) sig -= s * pq[qx];
+        for (int k = 0; k < 3; ++k) {
+          if (i == k)
+            flux[i][i] = (T)2 * (cq * LC::ihw(L, i, i)) * d[i][i] - pdiag * LC::ih(L, i);
+          else
+            flux[i][k] = (cq * LC::ihw(L, k, k)) * d[i][k] + (cq * LC::ihw(L, i, k)) * d[k][i];
         }
-        flux[i][k] = sig * JxW;
-      }
+    } else {
+      // same fluxes with fewer live values (the FP64 kernel is register-bound)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[i][k] *= LC::ih(L, k);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          flux[i][k] = ((i == k ? (T)2 * cq * d[i][i] - pdiag : cq * (d[i][k] + d[k][i]))) * LC::ih(L, k);
+    }
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-      const T fx = flux[comp][0] * ih0;
 #pragma unroll
-      for (int r = 0; r < 3; ++r) tacc[comp][r] = madd(tacc[comp][r], B::D(qx, r), fx);
-      sA[XI(comp, qx, ln)] = flux[comp][1] * ih1;
-      sC[XI(comp, qx, ln)] = flux[comp][2] * ih2;
+      for (int r = 0; r < 3; ++r) tacc[comp][r] = madd(tacc[comp][r], B::D(qx, r), flux[comp][0]);
+      sA[XI(comp, qx, ln)] = flux[comp][1];
+      sC[XI(comp, qx, ln)] = flux[comp][2];
     }
     if constexpr (STOKES) {
       const T fp = -s * div * JxW;

@@ -34,14 +34,37 @@ struct LevelDesc {
   int ntiles[3];
   long long vnodes, pnodes, cells, ntiles_total;
   double h[3];
-  double ih[3];   // 1 / h
-  double vol;     // cell volume h0 h1 h2 (h0 h1 in 2D)
-  double s;       // pressure scaling
+  double ih[3];      // 1 / h
+  double ihw[3][3];  // ih_i ih_k
+  double vol;        // cell volume h0 h1 h2 (h0 h1 in 2D)
+  float ih_f[3], ihw_f[3][3], vol_f, s_f;  // the same in single precision
+  double s;          // pressure scaling
   int bc[6];      // 0 open, 1 no normal flux, 2 zero velocity
   int compressible;
   double gravity[3];  // used by OP_RHS only
   int plain_read;     // 1: constrained velocity DoFs are read as stored (read_dof_values_plain)
 };
+
+// Mesh constants of a level in the arithmetic type of a kernel (kernel
+// arguments, so they stay scalar operands)
+template <typename T>
+struct LevelConst;
+template <>
+struct LevelConst<double> {
+  __host__ __device__ static double ih(const LevelDesc &L, int d) { return L.ih[d]; }
+  __host__ __device__ static double ihw(const LevelDesc &L, int i, int k) { return L.ihw[i][k]; }
+  __host__ __device__ static double vol(const LevelDesc &L) { return L.vol; }
+  __host__ __device__ static double s(const LevelDesc &L) { return L.s; }
+};
+template <>
+struct LevelConst<float> {
+  __host__ __device__ static float ih(const LevelDesc &L, int d) { return L.ih_f[d]; }
+  __host__ __device__ static float ihw(const LevelDesc &L, int i, int k) { return L.ihw_f[i][k]; }
+  __host__ __device__ static float vol(const LevelDesc &L) { return L.vol_f; }
+  __host__ __device__ static float s(const LevelDesc &L) { return L.s_f; }
+};
+// fill the derived mesh constants (ih, ihw, vol and the single-precision copies)
+void level_derive(LevelDesc &L);
 
 // Tile shapes (cells) per (dim, degree); one workgroup per tile, one thread
 // per (cell, y-slice).
