@@ -140,6 +140,26 @@ def test_many_tiles_ragged():
     assert rel(h.ablock_vmult(2, u), O.ablock_vmult(p, 2, lv[2], u)) < FP64
 
 
+def test_large_ragged_level():
+    # 3D 70x38x50 cells: 18x10x13 tiles with partial tiles in every direction
+    # and more tiles per colour (>= 270) and per level (2340) than the device
+    # runs at once (colour path in FP64; split path of the V-cycle in FP32 and
+    # FP64)
+    case = (3, 2, (35, 19, 25), 2, (1.4, 0.76, 1.0), (1, 2, 0, 1, 2, 1))
+    p, h, eta, sz, rng = setup_pair(*case, s=1.7)
+    x = rng.normal(size=sz["nu"] + sz["np"])
+    assert rel(h.stokes_vmult(x), O.stokes_vmult(p, 1, eta, x)) < FP64
+    lv = O.level_viscosity(p, eta)
+    u = rng.normal(size=sz["nu"])
+    assert rel(h.ablock_vmult(1, u), O.ablock_vmult(p, 1, lv[1], u)) < FP64
+    h.build_preconditioner()
+    assert rel(h.vcycle(0, u), O.vcycle(p, eta, 0, u)) < 1e-10
+    p32, h32, eta32, _, _ = setup_pair(*case, s=1.7, prec="fp32")
+    assert rel(h32.ablock_vmult(1, u), O.ablock_vmult(p32, 1, lv[1], u)) < FP32
+    h32.build_preconditioner()
+    assert rel(h32.vcycle(0, u), O.vcycle(p32, eta32, 0, u)) < 1e-4
+
+
 def test_determinism():
     p, h, eta, sz, rng = setup_pair(*CASES[2])
     x = rng.normal(size=sz["nu"] + sz["np"])
