@@ -738,37 +738,54 @@ void launch_dkm(const LevelDesc &L, const T *su, const T *sp, const T *eta, int 
 template <int MODE, typename T>
 __global__ void k_mass_nodes(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ eta,
                              const T *__restrict__ rhs, T *__restrict__ dst) {
-  const long long nn = (long long)L.np[0] * L.np[1] * L.np[2];
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  const int g[3] = {(int)(i % L.np[0]), (int)((i / L.np[0]) % L.np[1]), (int)(i / ((long long)L.np[0] * L.np[1]))};
-  const double vol = L.h[0] * L.h[1] * L.h[2] * L.s * L.s;
-  double acc = 0;
+  // grid: x over the nodes of an xy plane, y over z (32-bit offsets, checked
+  // by the launcher); arithmetic in T
+  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int gz = blockIdx.y;
+  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
+  const int g[3] = {gx, gy, gz};
+  // the 3^3 pressure values around the node (zero outside the level)
+  T pv[3][3][3];
+#pragma unroll
+  for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const int x = gx - 1 + dx, y = gy - 1 + dy, z = gz - 1 + dz;
+        const bool in = x >= 0 && y >= 0 && z >= 0 && x < L.np[0] && y < L.np[1] && z < L.np[2];
+        const T v = src[in ? x + L.np[0] * (y + L.np[1] * z) : 0];
+        pv[dz][dy][dx] = in ? v : (T)0;
+      }
+  constexpr T m1[2] = {T(1) / T(3), T(1) / T(6)};  // Q1 mass on [0,1]: 1/3 on the diagonal, 1/6 off it
+  T acc = 0;
 #pragma unroll
   for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
     for (int oy = 0; oy < 2; ++oy)
 #pragma unroll
       for (int ox = 0; ox < 2; ++ox) {
-        const int c[3] = {g[0] - 1 + ox, g[1] - 1 + oy, g[2] - 1 + oz};  // cell; the node is its local (1-ox, 1-oy, 1-oz)
-        if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= L.n[0] || c[1] >= L.n[1] || c[2] >= L.n[2]) continue;
-        double cs = 0;
+        // cell g - 1 + o; the node is its local node 1 - o, its nodes are pv[o + j]
+        const int c[3] = {g[0] - 1 + ox, g[1] - 1 + oy, g[2] - 1 + oz};
+        const bool valid = c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < L.n[0] && c[1] < L.n[1] && c[2] < L.n[2];
+        T cs = 0;
 #pragma unroll
         for (int jz = 0; jz < 2; ++jz)
 #pragma unroll
           for (int jy = 0; jy < 2; ++jy)
 #pragma unroll
-            for (int jx = 0; jx < 2; ++jx) {
-              const double w = ((1 - ox) == jx ? 1.0 / 3.0 : 1.0 / 6.0) * ((1 - oy) == jy ? 1.0 / 3.0 : 1.0 / 6.0) *
-                               ((1 - oz) == jz ? 1.0 / 3.0 : 1.0 / 6.0);
-              cs += w * (double)src[(c[0] + jx) + (long long)L.np[0] * ((c[1] + jy) + (long long)L.np[1] * (c[2] + jz))];
-            }
+            for (int jx = 0; jx < 2; ++jx)
+              cs += m1[(1 - ox) != jx] * m1[(1 - oy) != jy] * m1[(1 - oz) != jz] * pv[oz + jz][oy + jy][ox + jx];
         const int tx = c[0] >> 2, ty = c[1] >> 2, tz = c[2] >> 2;
-        const long long ti = (tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz)) * 64 +
-                             (c[0] & 3) + 4 * ((c[1] & 3) + 4 * (c[2] & 3));
-        acc += cs / (double)eta[ti];
+        const int ti = (tx + L.ntiles[0] * (ty + L.ntiles[1] * tz)) * 64 + (c[0] & 3) + 4 * ((c[1] & 3) + 4 * (c[2] & 3));
+        const T e = eta[valid ? ti : 0];
+        acc += valid ? cs / e : (T)0;
       }
-  const T v = (T)(vol * acc);
+  using LC = LevelConst<T>;
+  const T v = LC::vol(L) * LC::s(L) * LC::s(L) * acc;
+  const long long i = pidx + (long long)plane * gz;
   if constexpr (MODE == MODE_SET)
     dst[i] = v;
   else if constexpr (MODE == MODE_RESID)
@@ -782,14 +799,15 @@ bool launch_mass_nodes(Mode mode, const LevelDesc &L, const T *src, const T *eta
                        T *dst, hipStream_t st) {
   if (L.dim != 3 || L.k != 2 || eta_per_qp) return false;
   static_assert(Tile<3, 2>::TX == 4 && Tile<3, 2>::TY == 4 && Tile<3, 2>::TZ == 4, "tile-blocked eta layout");
-  const long long nn = L.pnodes;
-  const unsigned nb = (unsigned)((nn + 255) / 256);
+  if (L.pnodes >= (1LL << 31) || L.ntiles_total * 64 >= (1LL << 31))
+    throw std::runtime_error("level too large for the node-wise mass apply (32-bit offsets)");
+  const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
   if (mode == MODE_SET)
-    hipLaunchKernelGGL((k_mass_nodes<MODE_SET, T>), dim3(nb), dim3(256), 0, st, L, src, eta, rhs, dst);
+    hipLaunchKernelGGL((k_mass_nodes<MODE_SET, T>), grid, dim3(256), 0, st, L, src, eta, rhs, dst);
   else if (mode == MODE_RESID)
-    hipLaunchKernelGGL((k_mass_nodes<MODE_RESID, T>), dim3(nb), dim3(256), 0, st, L, src, eta, rhs, dst);
+    hipLaunchKernelGGL((k_mass_nodes<MODE_RESID, T>), grid, dim3(256), 0, st, L, src, eta, rhs, dst);
   else
-    hipLaunchKernelGGL((k_mass_nodes<MODE_ADD, T>), dim3(nb), dim3(256), 0, st, L, src, eta, rhs, dst);
+    hipLaunchKernelGGL((k_mass_nodes<MODE_ADD, T>), grid, dim3(256), 0, st, L, src, eta, rhs, dst);
   MFS_HIP(hipGetLastError());
   return true;
 }

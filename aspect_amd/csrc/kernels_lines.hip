@@ -643,17 +643,21 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 // array [tile][comp][lz][ly][lx] in a fixed order (deterministic) and applies
 // the mode; constrained rows take src (identity).  Per direction the
 // candidates of node g are (tile g/8 - 1, local node 8) for g mod 8 == 0,
-// g > 0, and (tile g/8, local node g mod 8) while g/8 < ntiles.
+// g > 0, and (tile g/8, local node g mod 8) while g/8 < ntiles.  Grid: x over
+// the nodes of an xy plane, y over (z, comp); 32-bit offsets (checked by the
+// launcher).
 template <int MODE, typename T>
 __global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
                                const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
   constexpr int TN = 3 * 729;
-  const long long nn = L.vnodes;
-  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= 3 * nn) return;
-  const int comp = (int)(idx / nn);
-  const long long r = idx - comp * nn;
-  const int g[3] = {(int)(r % L.nv[0]), (int)((r / L.nv[0]) % L.nv[1]), (int)(r / ((long long)L.nv[0] * L.nv[1]))};
+  const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int row = blockIdx.y;  // gz + nv2 * comp
+  const int comp = row / L.nv[2], gz = row - comp * L.nv[2];
+  const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
+  const int g[3] = {gx, gy, gz};
+  // per direction: tile offset (in tiles) and local node of both candidates
   int tt[3][2], ll[3][2];
   bool ok[3][2];
 #pragma unroll
@@ -665,6 +669,7 @@ __global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, co
     ll[d][1] = g[d] & 7;
     ok[d][1] = (g[d] >> 3) < L.ntiles[d];
   }
+  const int sty = L.ntiles[0], stz = L.ntiles[0] * L.ntiles[1];
   T val = 0;
 #pragma unroll
   for (int sz = 0; sz < 2; ++sz)
@@ -673,13 +678,14 @@ __global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, co
 #pragma unroll
       for (int sx = 0; sx < 2; ++sx) {
         const bool v = ok[2][sz] && ok[1][sy] && ok[0][sx];
-        const long long tile = tt[0][sx] + (long long)L.ntiles[0] * (tt[1][sy] + (long long)L.ntiles[1] * tt[2][sz]);
-        const long long wi = v ? tile * TN + ((comp * 9 + ll[2][sz]) * 9 + ll[1][sy]) * 9 + ll[0][sx] : 0;
+        const int tile = tt[0][sx] + sty * tt[1][sy] + stz * tt[2][sz];
+        const int wi = v ? tile * TN + ((comp * 9 + ll[2][sz]) * 9 + ll[1][sy]) * 9 + ll[0][sx] : 0;
         const T c = work[wi];
         val += v ? c : (T)0;
       }
-  const long long gi = comp * nn + r;
-  const T vv = con_bf(L, comp, g[0], g[1], g[2]) ? src_u[gi] : val;
+  const long long gi = comp * L.vnodes + pidx + (long long)plane * gz;
+  T vv = val;
+  if (con_bf(L, comp, gx, gy, gz)) vv = src_u[gi];
   if constexpr (MODE == MODE_SET)
     dst_u[gi] = vv;
   else if constexpr (MODE == MODE_RESID)
@@ -708,13 +714,15 @@ void launch_ablock_split(Mode mode, const LevelDesc &L, const T *su, const T *et
                          T *work, hipStream_t st) {
   hipLaunchKernelGGL((k_lines<OP_ABLOCK, LN_MODE_SPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(LN_THREADS), 0, st,
                      L, 0, su, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work, (T *)nullptr);
-  const unsigned nb = (unsigned)((3 * L.vnodes + 255) / 256);
+  if (split_work_size(L) >= (1LL << 31) || (long long)L.nv[0] * L.nv[1] >= (1LL << 31))
+    throw std::runtime_error("level too large for the split A-block apply (32-bit work offsets)");
+  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(3 * L.nv[2]));
   if (mode == MODE_SET)
-    hipLaunchKernelGGL((k_gather_tiles<MODE_SET, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_SET, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
   else if (mode == MODE_RESID)
-    hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
   else
-    hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), dim3(nb), dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
   MFS_HIP(hipGetLastError());
 }
 
