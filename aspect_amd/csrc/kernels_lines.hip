@@ -584,15 +584,39 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 
   if constexpr (SPLIT) {
     // in-tile node sums of all 3 x 9^3 tile nodes to the work array
-    // [tile][comp][lz][ly][lx] (node t = tid + 576 k); k_gather_tiles adds
-    // the tiles sharing a node
-    T *const wt = dst_u + tile_id * (3 * C::VN);
+    // [tile][comp][lz][ly][lx]; k_gather_tiles adds the tiles sharing a node.
+    // One thread per node row (comp, lz, ly): the (y, z) cell pairs are set
+    // up once per row and the x pairs are compile-time offsets, so the 12
+    // contributions of each (y, z) cell row are read once for all 9 nodes.
+    constexpr int ZS = 81 * CT;
+    if (tid < 3 * 81) {
+      const int comp = tid >= 162 ? 2 : (tid >= 81 ? 1 : 0);
+      const int rr = tid - 81 * comp, lz = div9(rr), ly = rr - 9 * lz;
+      const int ncx = interior ? 4 : min(4, L.n[0] - 4 * tx), ncy = interior ? 4 : min(4, L.n[1] - 4 * ty),
+                ncz = interior ? 4 : min(4, L.n[2] - 4 * tz);
+      int oy[2], oz[2], ox[2];
+      bool wy[2], wz[2], wx[2];
+      node_cands(ly, 4, 3 * CT, ncy, oy, wy);
+      node_cands(lz, 16, 9 * CT, ncz, oz, wz);
+      T val[9];
 #pragma unroll
-    for (int k = 0; k < LN_WPASSES; ++k) {
-      const TileNode n = tile_node(tid, k);
-      const T val = interior ? tile_node_sum<T, true>(L, sB, n, tx, ty, tz)
-                             : tile_node_sum<T, false>(L, sB, n, tx, ty, tz);
-      if (n.valid) wt[tid + k * LN_THREADS] = val;
+      for (int lx = 0; lx < 9; ++lx) val[lx] = 0;
+#pragma unroll
+      for (int za = 0; za < 2; ++za)
+#pragma unroll
+        for (int yb = 0; yb < 2; ++yb) {
+          const bool v = wz[za] && wy[yb];
+          const int base = comp * 27 * CT + oz[za] + oy[yb];
+#pragma unroll
+          for (int lx = 0; lx < 9; ++lx) {
+            node_cands(lx, 1, CT, ncx, ox, wx);  // compile-time offsets; ncx only masks
+#pragma unroll
+            for (int e = 0; e < 2; ++e) val[lx] += sB[v && wx[e] ? base + ox[e] : ZS];
+          }
+        }
+      T *const wt = dst_u + tile_id * (3 * C::VN) + tid * 9;
+#pragma unroll
+      for (int lx = 0; lx < 9; ++lx) wt[lx] = val[lx];
     }
     return;
   }
