@@ -353,22 +353,30 @@ struct GMG {
     T *dinv = kind == 0 ? g.dinvA.p : g.dinvS.p;
     T *r = kind == 0 ? g.ur.p : g.pr.p;
     T *d = kind == 0 ? g.ud.p : g.pd.p;
+    // residual + update in one pass where the A-block apply is split
+    auto fused = [&](T f1, T f2, int first) {
+      return kind == 0 && launch_ablock_cheb<T>(g.L, x, b, g.eta.p, eta_pq, dinv, d, f1, f2, first, split_for(l), st);
+    };
     for (int s = 0; s < steps; ++s) {
       const bool zero = x_zero && s == 0;
       const T *res = b;
-      if (!zero) {
+      if (zero) {
+        v_cheb_first<T>(dinv, res, d, x, (T)(1.0 / ch.theta), true, n, st);
+      } else if (!fused((T)0, (T)(1.0 / ch.theta), 1)) {
         apply(kind, l, MODE_RESID, x, b, r);
         res = r;
+        v_cheb_first<T>(dinv, res, d, x, (T)(1.0 / ch.theta), false, n, st);
       }
-      v_cheb_first<T>(dinv, res, d, x, (T)(1.0 / ch.theta), zero, n, st);
       if (ch.degree < 2 || std::fabs(ch.delta) < 1e-40) continue;
       const double sigma = ch.theta / ch.delta;
       double rho_old = ch.delta / ch.theta;
       for (int k = 1; k < ch.degree; ++k) {
-        apply(kind, l, MODE_RESID, x, b, r);
         const double rho = 1.0 / (2.0 * sigma - rho_old);
         const double f1 = rho * rho_old, f2 = 2.0 * rho / ch.delta;
-        v_cheb_step<T>(dinv, r, d, x, (T)f1, (T)f2, n, st);
+        if (!fused((T)f1, (T)f2, 0)) {
+          apply(kind, l, MODE_RESID, x, b, r);
+          v_cheb_step<T>(dinv, r, d, x, (T)f1, (T)f2, n, st);
+        }
         rho_old = rho;
       }
     }

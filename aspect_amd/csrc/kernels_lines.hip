@@ -662,26 +662,15 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   }
 }
 
-// Second pass of the split A-block apply: one thread per velocity DoF sums
-// the in-tile sums of the at most 8 tiles sharing the node from the work
-// array [tile][comp][lz][ly][lx] in a fixed order (deterministic) and applies
-// the mode; constrained rows take src (identity).  Per direction the
-// candidates of node g are (tile g/8 - 1, local node 8) for g mod 8 == 0,
-// g > 0, and (tile g/8, local node g mod 8) while g/8 < ntiles.  Grid: x over
-// the nodes of an xy plane, y over (z, comp); 32-bit offsets (checked by the
-// launcher).
-template <int MODE, typename T>
-__global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
-                               const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
+// Sum over the at most 8 tiles sharing velocity node g of component comp,
+// from the work array [tile][comp][lz][ly][lx] of in-tile sums, in a fixed
+// order (deterministic).  Per direction the candidates of node g are
+// (tile g/8 - 1, local node 8) for g mod 8 == 0, g > 0, and (tile g/8, local
+// node g mod 8) while g/8 < ntiles.  32-bit offsets (checked by the launcher).
+template <typename T>
+__device__ __forceinline__ T tile_gather_sum(const LevelDesc &L, const T *__restrict__ work, int comp,
+                                             const int (&g)[3]) {
   constexpr int TN = 3 * 729;
-  const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
-  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pidx >= plane) return;
-  const int row = blockIdx.y;  // gz + nv2 * comp
-  const int comp = row / L.nv[2], gz = row - comp * L.nv[2];
-  const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
-  const int g[3] = {gx, gy, gz};
-  // per direction: tile offset (in tiles) and local node of both candidates
   int tt[3][2], ll[3][2];
   bool ok[3][2];
 #pragma unroll
@@ -707,6 +696,23 @@ __global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, co
         const T c = work[wi];
         val += v ? c : (T)0;
       }
+  return val;
+}
+
+// Second pass of the split A-block apply: one thread per velocity DoF adds
+// the tiles sharing the node and applies the mode; constrained rows take src
+// (identity).  Grid: x over the nodes of an xy plane, y over (z, comp).
+template <int MODE, typename T>
+__global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
+                               const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
+  const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int row = blockIdx.y;  // gz + nv2 * comp
+  const int comp = row / L.nv[2], gz = row - comp * L.nv[2];
+  const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
+  const int g[3] = {gx, gy, gz};
+  const T val = tile_gather_sum<T>(L, work, comp, g);
   const long long gi = comp * L.vnodes + pidx + (long long)plane * gz;
   T vv = val;
   if (con_bf(L, comp, gx, gy, gz)) vv = src_u[gi];
@@ -716,6 +722,30 @@ __global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, co
     dst_u[gi] = rhs_u[gi] - vv;
   else
     dst_u[gi] += vv;
+}
+
+// The same second pass fused with a Jacobi-Chebyshev update: r = b - A x at
+// each DoF (not stored), then d = dinv r inv_theta (first step of a sweep
+// from a nonzero x) or d = f1 d + f2 dinv r, and x += d -- the arithmetic of
+// v_cheb_first / v_cheb_step, so the result is that of the unfused sequence.
+template <typename T>
+__global__ void k_gather_cheb(const LevelDesc L, const T *__restrict__ work, T *x, const T *__restrict__ b,
+                              const T *__restrict__ dinv, T *__restrict__ d, const T f1, const T f2,
+                              const int first) {
+  const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int row = blockIdx.y;
+  const int comp = row / L.nv[2], gz = row - comp * L.nv[2];
+  const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
+  const int g[3] = {gx, gy, gz};
+  const T val = tile_gather_sum<T>(L, work, comp, g);
+  const long long gi = comp * L.vnodes + pidx + (long long)plane * gz;
+  const T xi = x[gi];
+  const T r = b[gi] - (con_bf(L, comp, gx, gy, gz) ? xi : val);
+  const T dd = first ? dinv[gi] * r * f2 : f1 * d[gi] + f2 * dinv[gi] * r;
+  d[gi] = dd;
+  x[gi] = xi + dd;
 }
 
 template <int OP, int MODE, typename T>
@@ -747,6 +777,19 @@ void launch_ablock_split(Mode mode, const LevelDesc &L, const T *su, const T *et
     hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
   else
     hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
+  MFS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_ablock_cheb_split(const LevelDesc &L, T *x, const T *b, const T *eta, int pq, const T *dinv, T *d, T f1,
+                              T f2, int first, T *work, hipStream_t st) {
+  hipLaunchKernelGGL((k_lines<OP_ABLOCK, LN_MODE_SPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(LN_THREADS), 0, st,
+                     L, 0, (const T *)x, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work,
+                     (T *)nullptr);
+  if (split_work_size(L) >= (1LL << 31) || (long long)L.nv[0] * L.nv[1] >= (1LL << 31))
+    throw std::runtime_error("level too large for the split A-block apply (32-bit work offsets)");
+  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(3 * L.nv[2]));
+  hipLaunchKernelGGL((k_gather_cheb<T>), grid, dim3(256), 0, st, L, (const T *)work, x, b, dinv, d, f1, f2, first);
   MFS_HIP(hipGetLastError());
 }
 
@@ -782,6 +825,17 @@ bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, con
     return false;
   return true;
 }
+template <typename T>
+bool launch_ablock_cheb(const LevelDesc &L, T *x, const T *b, const T *eta, int eta_per_qp, const T *dinv, T *d,
+                        T f1, T f2, int first, T *split_work, hipStream_t st) {
+  if (L.dim != 3 || L.k != 2 || !split_work) return false;
+  launch_ablock_cheb_split<T>(L, x, b, eta, eta_per_qp, dinv, d, f1, f2, first, split_work, st);
+  return true;
+}
+template bool launch_ablock_cheb<double>(const LevelDesc &, double *, const double *, const double *, int,
+                                         const double *, double *, double, double, int, double *, hipStream_t);
+template bool launch_ablock_cheb<float>(const LevelDesc &, float *, const float *, const float *, int, const float *,
+                                        float *, float, float, int, float *, hipStream_t);
 template bool launch_cell_lines<double>(Op, Mode, const LevelDesc &, const double *, const double *,
                                         const double *, int, const double *, const double *, double *, double *,
                                         hipStream_t, double *);

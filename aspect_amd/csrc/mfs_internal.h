@@ -88,6 +88,12 @@ bool launch_cell_lines(Op op, Mode mode, const LevelDesc &L, const T *src_u, con
                        int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                        T *split_work = nullptr);
 long long split_work_size(const LevelDesc &L);
+// Split A-block apply fused with a Jacobi-Chebyshev update (3D degree 2 with
+// split_work only; returns false otherwise): r = b - A x (not stored), then
+// d = dinv r f2 (first != 0) or d = f1 d + f2 dinv r, and x += d
+template <typename T>
+bool launch_ablock_cheb(const LevelDesc &L, T *x, const T *b, const T *eta, int eta_per_qp, const T *dinv, T *d,
+                        T f1, T f2, int first, T *split_work, hipStream_t st);
 
 // Transfers between level Lc (coarse) and Lf = refined Lc.  kind 0 velocity, 1 pressure.
 // work: nullptr, or transfer_work_size(Lc, Lf) elements for the separable
