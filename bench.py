@@ -9,9 +9,11 @@ A step is one FP64 application of the fine-level Stokes operator (the metric's
 average device time per step, timed with hipEvents on the library's stream.
 After the timed steps the script reports, outside ``value``: the FP32 level
 A-block apply rate, and time-to-solve to 1e-8 relative residual with the FP32
-V-cycle and with the FP64 V-cycle (both inside FP64 GMRES/FGMRES).
+V-cycle and with the FP64 V-cycle (both inside FP64 GMRES/FGMRES), and the
+time to solve an ensemble of 48 independent 2D problems together.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-solve] [--no-cpu]
+                      [--no-ensemble]
 """
 from __future__ import annotations
 
@@ -62,6 +64,8 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble of 48 independent 2D problems")
     ap.add_argument("--cheap-steps", type=int, default=1000,
                     help="Number of cheap Stokes solver steps (ASPECT's default 1000)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -213,6 +217,27 @@ def main():
         del sols
     else:
         del h32
+
+    if not args.no_ensemble and not args.no_solve:
+        # Ensemble of 48 independent 2D problems of uneven sizes (16^2 .. 256^2
+        # cells, each with its own sinker field), FP32 V-cycle, solved together:
+        # one handle and HIP stream per problem, 16 host worker threads.
+        from aspect_amd import Ensemble
+        cfgs = P.config_ensemble(48, seed=4)
+        fields = [P.fields(c) for c in cfgs]
+        ens = Ensemble([c.box(f[2]) for c, f in zip(cfgs, fields)], "fp32")
+        t0 = time.perf_counter()
+        ens.setup([f[0] for f in fields], "harmonic")
+        t_set = time.perf_counter() - t0
+        rhss = [m.assemble_buoyancy_rhs(f[1], c.gravity()) for m, c, f in zip(ens.members, cfgs, fields)]
+        _, outs, secs = ens.solve(rhss, tolerance=1e-8)
+        its = sum(o.cheap_iterations + o.expensive_iterations for o in outs)
+        result["ensemble"] = {"problems": len(cfgs), "dofs": int(sum(r.size for r in rhss)), "seconds": secs,
+                              "setup_seconds": t_set, "iterations": its,
+                              "converged": all(o.converged for o in outs), "tolerance": 1e-8}
+        log(f"ensemble of {len(cfgs)} 2D problems: {secs:.2f}s, {its} iterations, "
+            f"converged={result['ensemble']['converged']}")
+        del ens
 
     if not args.no_cpu:
         # CPU baseline: the oracle's OpenMP sum-factorized FP64 Stokes apply
