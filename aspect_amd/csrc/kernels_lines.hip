@@ -263,7 +263,14 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   constexpr bool STOKES = (OP == OP_STOKES);
   static_assert(CT == 64 && CT * NL == LN_THREADS, "one wave per line");
   static_assert(3 * C::VN <= XA && C::PN <= XP, "tile load fits the exchange arrays");
-  __shared__ T sA[XA], sB[XA + 1], sC[XA];  // sB[XA]: zero slot of the write-out
+  // FP32: two exchange arrays (one more barrier in the test steps) keep the
+  // workgroup at <= 53 KB of LDS, the most two 576-thread workgroups per CU
+  // get; FP64 (one workgroup per CU in any case): three arrays, one barrier
+  // per step.  [XA]: zero slot of the write-out gathers.
+  constexpr bool TWO = std::is_same<T, float>::value;
+  __shared__ T sA[XA + 1], sB[XA + 1];
+  __shared__ T sC3[TWO ? 1 : XA];
+  T *const sC = TWO ? sB : sC3;  // tile nodes (FP32: in sB), FP64: also the z fluxes
   __shared__ T sP[STOKES ? XP : 1];
 
   constexpr bool SPLIT = (MODE == LN_MODE_SPLIT);  // all tiles in one launch, contributions to dst_u
@@ -296,7 +303,10 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   // ---- 0. cooperative tile load (constrained velocity DoFs read as zero);
   // interior tiles (no node on the domain boundary) skip the bounds and
   // constraint tests
-  if (tid == 0) sB[XA] = 0;
+  if (tid == 0) {
+    sA[XA] = 0;
+    sB[XA] = 0;
+  }
   const bool interior = vx0 > 0 && vy0 > 0 && vz0 > 0 && vx0 + C::VX < L.nv[0] && vy0 + C::VY < L.nv[1] &&
                         vz0 + C::VZ < L.nv[2];
   // tile values, stored to LDS after the later loads have been issued
@@ -469,6 +479,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
     for (int r = 0; r < 3; ++r) tacc[comp][r] = 0;
   T hp0 = 0, hp1 = 0;
+  T fz[3][3];  // FP32: z fluxes, to LDS after the next barrier
 #pragma unroll
   for (int qx = 0; qx < 3; ++qx) {
     const T JxW = wab * B::W(qx);
@@ -513,7 +524,10 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 #pragma unroll
       for (int r = 0; r < 3; ++r) tacc[comp][r] = madd(tacc[comp][r], B::D(qx, r), flux[comp][0]);
       sA[XI(comp, qx, ln)] = flux[comp][1];
-      sC[XI(comp, qx, ln)] = flux[comp][2];
+      if constexpr (TWO)
+        fz[comp][qx] = flux[comp][2];
+      else
+        sC[XI(comp, qx, ln)] = flux[comp][2];
     }
     if constexpr (STOKES) {
       const T fp = -s * div * JxW;
@@ -541,14 +555,35 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
       }
     }
   }
+  if constexpr (TWO) {
+    // 4a: y test from sA; the z fluxes go to sB (read last in step 3)
+    T tq[3][3];
 #pragma unroll
-  for (int comp = 0; comp < 3; ++comp)
+    for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
-    for (int qx = 0; qx < 3; ++qx)
-      sB[XI(comp, qx, ln)] = tacc[comp][qx] + DTa[0] * sA[XI(comp, qx, 3 * b)] +
-                             DTa[1] * sA[XI(comp, qx, 3 * b + 1)] + DTa[2] * sA[XI(comp, qx, 3 * b + 2)] +
-                             DTb[0] * sC[XI(comp, qx, a)] + DTb[1] * sC[XI(comp, qx, a + 3)] +
-                             DTb[2] * sC[XI(comp, qx, a + 6)];
+      for (int qx = 0; qx < 3; ++qx) {
+        sB[XI(comp, qx, ln)] = fz[comp][qx];
+        tq[comp][qx] = tacc[comp][qx] + DTa[0] * sA[XI(comp, qx, 3 * b)] + DTa[1] * sA[XI(comp, qx, 3 * b + 1)] +
+                       DTa[2] * sA[XI(comp, qx, 3 * b + 2)];
+      }
+    lds_barrier();
+    // 4b: z test from sB; the sums go to sA (read last in 4a)
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+      for (int qx = 0; qx < 3; ++qx)
+        sA[XI(comp, qx, ln)] = tq[comp][qx] + DTb[0] * sB[XI(comp, qx, a)] + DTb[1] * sB[XI(comp, qx, a + 3)] +
+                               DTb[2] * sB[XI(comp, qx, a + 6)];
+  } else {
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+      for (int qx = 0; qx < 3; ++qx)
+        sB[XI(comp, qx, ln)] = tacc[comp][qx] + DTa[0] * sA[XI(comp, qx, 3 * b)] +
+                               DTa[1] * sA[XI(comp, qx, 3 * b + 1)] + DTa[2] * sA[XI(comp, qx, 3 * b + 2)] +
+                               DTb[0] * sC[XI(comp, qx, a)] + DTb[1] * sC[XI(comp, qx, a + 3)] +
+                               DTb[2] * sC[XI(comp, qx, a + 6)];
+  }
   lds_barrier();
 
   // ---- 5. z integration: line (qy, kz) = (a, b); pressure node values to
@@ -556,28 +591,35 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
   if constexpr (STOKES) {
     if (ln < 8) sP[ln * CT + c] = pnode;
   }
+  {
+    // FP32: test sums in sA, result to sB; FP64: the other way round
+    const T *const in5 = TWO ? sA : sB;
+    T *const out5 = TWO ? sB : sA;
 #pragma unroll
-  for (int comp = 0; comp < 3; ++comp)
+    for (int comp = 0; comp < 3; ++comp)
 #pragma unroll
-    for (int qx = 0; qx < 3; ++qx)
-      sA[XI(comp, qx, ln)] =
-          STb[0] * sB[XI(comp, qx, a)] + STb[1] * sB[XI(comp, qx, a + 3)] + STb[2] * sB[XI(comp, qx, a + 6)];
+      for (int qx = 0; qx < 3; ++qx)
+        out5[XI(comp, qx, ln)] =
+            STb[0] * in5[XI(comp, qx, a)] + STb[1] * in5[XI(comp, qx, a + 3)] + STb[2] * in5[XI(comp, qx, a + 6)];
+  }
   lds_barrier();
 
   // ---- 6. y integration: node line (jy, kz) = (a, b), then x; per-cell
-  // contributions [comp][kz][jy][i][cell]
+  // contributions [comp][kz][jy][i][cell] to xb (FP32: sA, FP64: sB)
+  const T *const in6 = TWO ? sB : sA;
+  T *const xb = TWO ? sA : sB;
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     T hyv[3];
 #pragma unroll
     for (int qx = 0; qx < 3; ++qx)
-      hyv[qx] = STa[0] * sA[XI(comp, qx, 3 * b)] + STa[1] * sA[XI(comp, qx, 3 * b + 1)] +
-                STa[2] * sA[XI(comp, qx, 3 * b + 2)];
+      hyv[qx] = STa[0] * in6[XI(comp, qx, 3 * b)] + STa[1] * in6[XI(comp, qx, 3 * b + 1)] +
+                STa[2] * in6[XI(comp, qx, 3 * b + 2)];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const T o =
           madd(madd(B::S(0, i) == T(0) ? T(0) : B::S(0, i) * hyv[0], B::S(1, i), hyv[1]), B::S(2, i), hyv[2]);
-      sB[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
+      xb[(((comp * 3 + b) * 3 + a) * 3 + i) * CT + c] = o;
     }
   }
   lds_barrier();
@@ -611,7 +653,7 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
           for (int lx = 0; lx < 9; ++lx) {
             node_cands(lx, 1, CT, ncx, ox, wx);  // compile-time offsets; ncx only masks
 #pragma unroll
-            for (int e = 0; e < 2; ++e) val[lx] += sB[v && wx[e] ? base + ox[e] : ZS];
+            for (int e = 0; e < 2; ++e) val[lx] += xb[v && wx[e] ? base + ox[e] : ZS];
           }
         }
       T *const wt = dst_u + tile_id * (3 * C::VN) + tid * 9;
@@ -623,9 +665,9 @@ __global__ void __launch_bounds__(LN_THREADS, LinesBlocks<T>::value)
 
   // ---- 7. tile write-out
   if (interior)
-    lines_write_u<MODE, T, true>(L, sB, tid, tx, ty, tz, px, py, pz, wold, wsold, dst_u);
+    lines_write_u<MODE, T, true>(L, xb, tid, tx, ty, tz, px, py, pz, wold, wsold, dst_u);
   else
-    lines_write_u<MODE, T, false>(L, sB, tid, tx, ty, tz, px, py, pz, wold, wsold, dst_u);
+    lines_write_u<MODE, T, false>(L, xb, tid, tx, ty, tz, px, py, pz, wold, wsold, dst_u);
   if constexpr (STOKES) {
     const int ncx = min(C::TX, L.n[0] - C::TX * tx), ncy = min(C::TY, L.n[1] - C::TY * ty),
               ncz = min(C::TZ, L.n[2] - C::TZ * tz);
