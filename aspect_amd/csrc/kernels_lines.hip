@@ -10,16 +10,19 @@
 // quadrature points, indexed (a, b) = (y index, z index).  The 64 cells of a
 // tile form one wave, so (a, b) is uniform over a wave.  The x direction of
 // the sum factorization stays in registers; the y and z directions go through
-// LDS, one exchange array [comp][q][line][cell] per step, three arrays in
-// rotation so that every step needs a single barrier (the first step
-// interpolates in x and y at once, straight from the tile's nodes).  Per-thread state is one
-// line (3 values per component), which keeps register use low enough for
-// several waves per SIMD.  Collocation form: S (Q x N) interpolates nodal
-// values to Gauss points, D (Q x Q) differentiates on the Gauss points
-// (D S = G); tests apply D^T and S^T.
+// LDS, one exchange array [comp][q][line][cell] per step.  FP64: three arrays
+// in rotation, one barrier per step (a workgroup needs 134 KB of LDS, one per
+// CU).  FP32: two arrays and one more barrier (41.5 KB, two workgroups per
+// CU).  The first step interpolates in x and y at once, straight from the
+// tile's nodes.  Per-thread state is one line (3 values per component), which
+// keeps register use low enough for several waves per SIMD.  Collocation form:
+// S (Q x N) interpolates nodal values to Gauss points, D (Q x Q)
+// differentiates on the Gauss points (D S = G); tests apply D^T and S^T.
 //
-// The tile write-out is the deterministic gather shared with the general cell
-// kernel (tile colours, lowest colour stores, later colours add).
+// Tile write-out: every tile node sums its cells from LDS in a fixed order
+// (deterministic); tile colours (lowest colour stores, later colours add), or
+// in the split form a work array of in-tile sums that a node-wise pass adds
+// over the tiles sharing a node.
 
 #include <hip/hip_runtime.h>
 
