@@ -1014,62 +1014,65 @@ __device__ __forceinline__ bool con_node(const LevelDesc &L, int comp, int gx, i
 struct Dims3 {
   int n[3];
 };
+// Grid of the passes: x over the nodes of an xy plane of the output, y over
+// (z, comp); 32-bit offsets (the launcher checks the sizes).  Arithmetic in T.
 template <int KK, typename T>
 __global__ void k_restrict_pass(const LevelDesc Lc, const LevelDesc Lf, const int ax, const Dims3 din, const Dims3 dout,
                                 const int ncomp, const int first, const int last, const T *__restrict__ in,
                                 T *__restrict__ out) {
-  const long long per = (long long)dout.n[0] * dout.n[1] * dout.n[2];
-  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= per * ncomp) return;
-  const int comp = (int)(idx / per);
-  const long long r = idx - comp * per;
-  int o[3] = {(int)(r % dout.n[0]), (int)((r / dout.n[0]) % dout.n[1]), (int)(r / ((long long)dout.n[0] * dout.n[1]))};
-  const long long stride = ax == 0 ? 1 : (ax == 1 ? (long long)din.n[0] : (long long)din.n[0] * din.n[1]);
-  const long long inper = (long long)din.n[0] * din.n[1] * din.n[2];
+  const unsigned plane = (unsigned)(dout.n[0] * dout.n[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int row = blockIdx.y;  // z + nz * comp
+  const int comp = row / dout.n[2];
+  const int o1 = (int)(pidx / (unsigned)dout.n[0]);
+  const int o[3] = {(int)pidx - o1 * dout.n[0], o1, row - comp * dout.n[2]};
+  const int stride = ax == 0 ? 1 : (ax == 1 ? din.n[0] : din.n[0] * din.n[1]);
   const int m = o[ax];
   int base[3] = {o[0], o[1], o[2]};
   base[ax] = 0;
-  const long long b0 = comp * inper + base[0] + (long long)din.n[0] * (base[1] + (long long)din.n[1] * base[2]);
+  const int b0 = comp * din.n[0] * din.n[1] * din.n[2] + base[0] + din.n[0] * (base[1] + din.n[1] * base[2]);
   constexpr bool VEL = KK == 2;
   constexpr int NO = VEL ? 5 : 3;
   const bool vertex = !VEL || (m & 1) == 0;
-  double acc = 0;
+  T acc = 0;
 #pragma unroll
   for (int k = 0; k < NO; ++k) {
     int off;
-    double wv;
+    T wv;
     if constexpr (VEL) {
       off = k == 0 ? -3 : (k == 1 ? -1 : (k == 2 ? 0 : (k == 3 ? 1 : 3)));
-      wv = vertex ? (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.375 : -0.125)) : (k == 2 ? 1.0 : (k == 1 || k == 3 ? 0.75 : 0.0));
+      wv = vertex ? (k == 2 ? T(1) : (k == 1 || k == 3 ? T(0.375) : T(-0.125)))
+                  : (k == 2 ? T(1) : (k == 1 || k == 3 ? T(0.75) : T(0)));
     } else {
       off = k - 1;
-      wv = k == 1 ? 1.0 : 0.5;
+      wv = k == 1 ? T(1) : T(0.5);
     }
     const int g = 2 * m + off;
-    bool ok = wv != 0.0 && g >= 0 && g < din.n[ax];
-    if (ok && first && VEL) {
+    bool ok = wv != T(0) && g >= 0 && g < din.n[ax];
+    if (VEL && first) {
       int f[3] = {o[0], o[1], o[2]};
       f[ax] = g;
-      ok = !con_node(Lf, comp, f[0], f[1], f[2]);
+      ok = ok && !con_node(Lf, comp, f[0], f[1], f[2]);
     }
-    if (ok) acc += wv * (double)in[b0 + g * stride];
+    const T v = in[ok ? b0 + g * stride : b0];
+    acc += ok ? wv * v : T(0);
   }
-  if (last && VEL && con_node(Lc, comp, o[0], o[1], o[2])) acc = 0.0;
-  out[idx] = (T)acc;
+  if (last && VEL && con_node(Lc, comp, o[0], o[1], o[2])) acc = 0;
+  out[(long long)row * plane + pidx] = acc;
 }
 template <int KK, typename T>
 __global__ void k_prolong_pass(const LevelDesc Lc, const LevelDesc Lf, const int ax, const Dims3 din, const Dims3 dout,
                                const int ncomp, const int first, const int last, const int add,
                                const T *__restrict__ in, T *__restrict__ out) {
-  const long long per = (long long)dout.n[0] * dout.n[1] * dout.n[2];
-  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= per * ncomp) return;
-  const int comp = (int)(idx / per);
-  const long long r = idx - comp * per;
-  const int o[3] = {(int)(r % dout.n[0]), (int)((r / dout.n[0]) % dout.n[1]),
-                    (int)(r / ((long long)dout.n[0] * dout.n[1]))};
-  const long long stride = ax == 0 ? 1 : (ax == 1 ? (long long)din.n[0] : (long long)din.n[0] * din.n[1]);
-  const long long inper = (long long)din.n[0] * din.n[1] * din.n[2];
+  const unsigned plane = (unsigned)(dout.n[0] * dout.n[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int row = blockIdx.y;  // z + nz * comp
+  const int comp = row / dout.n[2];
+  const int o1 = (int)(pidx / (unsigned)dout.n[0]);
+  const int o[3] = {(int)pidx - o1 * dout.n[0], o1, row - comp * dout.n[2]};
+  const int stride = ax == 0 ? 1 : (ax == 1 ? din.n[0] : din.n[0] * din.n[1]);
   const int g = o[ax];
   const int ncell = Lc.n[ax];
   int e = g / (2 * KK), rr = g - 2 * KK * e;
@@ -1079,25 +1082,27 @@ __global__ void k_prolong_pass(const LevelDesc Lc, const LevelDesc Lf, const int
   }
   int base[3] = {o[0], o[1], o[2]};
   base[ax] = 0;
-  const long long b0 = comp * inper + base[0] + (long long)din.n[0] * (base[1] + (long long)din.n[1] * base[2]);
+  const int b0 = comp * din.n[0] * din.n[1] * din.n[2] + base[0] + din.n[0] * (base[1] + din.n[1] * base[2]);
   constexpr bool VEL = KK == 2;
-  double acc = 0;
+  T acc = 0;
 #pragma unroll
   for (int mm = 0; mm <= KK; ++mm) {
-    double wv = ProlongTaps<KK>::W[0][mm];
+    T wv = (T)ProlongTaps<KK>::W[0][mm];
 #pragma unroll
-    for (int q = 1; q <= 2 * KK; ++q) wv = rr == q ? ProlongTaps<KK>::W[q][mm] : wv;
+    for (int q = 1; q <= 2 * KK; ++q) wv = rr == q ? (T)ProlongTaps<KK>::W[q][mm] : wv;
     const int cidx = KK * e + mm;
-    bool ok = wv != 0.0;
-    if (ok && first && VEL) {
+    bool ok = wv != T(0);
+    if (VEL && first) {
       int c[3] = {o[0], o[1], o[2]};
       c[ax] = cidx;
-      ok = !con_node(Lc, comp, c[0], c[1], c[2]);
+      ok = ok && !con_node(Lc, comp, c[0], c[1], c[2]);
     }
-    if (ok) acc += wv * (double)in[b0 + cidx * stride];
+    const T v = in[b0 + cidx * stride];
+    acc += ok ? wv * v : T(0);
   }
-  if (last && VEL && con_node(Lf, comp, o[0], o[1], o[2])) acc = 0.0;
-  out[idx] = add ? (T)(out[idx] + (T)acc) : (T)acc;
+  if (last && VEL && con_node(Lf, comp, o[0], o[1], o[2])) acc = 0;
+  const long long oi = (long long)row * plane + pidx;
+  out[oi] = add ? (T)(out[oi] + acc) : acc;
 }
 
 template <typename T>
@@ -1110,8 +1115,10 @@ static void run_sep_transfer(bool prolong, int kind, const LevelDesc &Lc, const 
     dc.n[d] = vel ? Lc.nv[d] : Lc.np[d];
     df.n[d] = vel ? Lf.nv[d] : Lf.np[d];
   }
+  if ((long long)df.n[0] * df.n[1] * df.n[2] * ncomp >= (1LL << 31))
+    throw std::runtime_error("level too large for the separable transfers (32-bit offsets)");
   auto blocks = [&](const Dims3 &d) {
-    return (unsigned)(((long long)d.n[0] * d.n[1] * d.n[2] * ncomp + 255) / 256);
+    return dim3((unsigned)(((long long)d.n[0] * d.n[1] + 255) / 256), (unsigned)(d.n[2] * ncomp));
   };
   if (!prolong) {
     // x, y, z: fine -> (c, f, f) -> (c, c, f) -> coarse
@@ -1125,10 +1132,10 @@ static void run_sep_transfer(bool prolong, int kind, const LevelDesc &Lc, const 
     const Dims3 di[3] = {df, d1, d2}, dd[3] = {d1, d2, dc};
     for (int ax = 0; ax < 3; ++ax) {
       if (vel)
-        hipLaunchKernelGGL((k_restrict_pass<2, T>), dim3(blocks(dd[ax])), dim3(256), 0, st, Lc, Lf, ax, di[ax], dd[ax],
+        hipLaunchKernelGGL((k_restrict_pass<2, T>), blocks(dd[ax]), dim3(256), 0, st, Lc, Lf, ax, di[ax], dd[ax],
                            ncomp, ax == 0 ? 1 : 0, ax == 2 ? 1 : 0, ins[ax], outs[ax]);
       else
-        hipLaunchKernelGGL((k_restrict_pass<1, T>), dim3(blocks(dd[ax])), dim3(256), 0, st, Lc, Lf, ax, di[ax], dd[ax],
+        hipLaunchKernelGGL((k_restrict_pass<1, T>), blocks(dd[ax]), dim3(256), 0, st, Lc, Lf, ax, di[ax], dd[ax],
                            ncomp, 0, 0, ins[ax], outs[ax]);
     }
   } else {
@@ -1144,10 +1151,10 @@ static void run_sep_transfer(bool prolong, int kind, const LevelDesc &Lc, const 
     const int axes[3] = {2, 1, 0};
     for (int p = 0; p < 3; ++p) {
       if (vel)
-        hipLaunchKernelGGL((k_prolong_pass<2, T>), dim3(blocks(dd[p])), dim3(256), 0, st, Lc, Lf, axes[p], di[p], dd[p],
+        hipLaunchKernelGGL((k_prolong_pass<2, T>), blocks(dd[p]), dim3(256), 0, st, Lc, Lf, axes[p], di[p], dd[p],
                            ncomp, p == 0 ? 1 : 0, p == 2 ? 1 : 0, p == 2 && add ? 1 : 0, ins[p], outs[p]);
       else
-        hipLaunchKernelGGL((k_prolong_pass<1, T>), dim3(blocks(dd[p])), dim3(256), 0, st, Lc, Lf, axes[p], di[p], dd[p],
+        hipLaunchKernelGGL((k_prolong_pass<1, T>), blocks(dd[p]), dim3(256), 0, st, Lc, Lf, axes[p], di[p], dd[p],
                            ncomp, 0, 0, p == 2 && add ? 1 : 0, ins[p], outs[p]);
     }
   }
