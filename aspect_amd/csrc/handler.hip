@@ -123,6 +123,7 @@ struct GMGLevel {
   LevelDesc L;
   long long nu = 0, np = 0;
   DBuf<T> eta, dinvA, dinvS;
+  DBuf<T> ietan;  // 1 / cellwise viscosity in natural cell order (Schur level operator), set by build()
   DBuf<T> ux, ub, ur, ud, px, pb, pr, pd;
   ChebInfo chA, chS;
 };
@@ -146,6 +147,7 @@ struct GMG {
   // the colour tails and the in-kernel gather
   DBuf<T> swork;
   T *split_for(int) { return swork.p; }
+  bool ietan_ok = false;  // GMGLevel::ietan matches the current level viscosity
   void drop_graphs() {
     for (hipGraphExec_t &e : vexec)
       if (e) {
@@ -195,6 +197,7 @@ struct GMG {
   // DGQ0 level viscosity: finest from the active table, coarser by averaging children
   void set_eta(const double *active, int active_per_qp) {
     drop_graphs();
+    ietan_ok = false;  // recomputed by build()
     eta_pq = 0;
     for (auto &g : lv)
       if (g->eta.n != g->L.ntiles_total * cells_per_tile_of(g->L)) g->eta.alloc(g->L.ntiles_total * cells_per_tile_of(g->L));
@@ -209,6 +212,7 @@ struct GMG {
   // DGQ1 level viscosity: per-qp tables from the Q1 coefficients of every level
   void set_eta_q1(const std::vector<DBuf<double>> &coef, double lo, double hi) {
     drop_graphs();
+    ietan_ok = false;
     eta_pq = 1;
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
@@ -227,6 +231,7 @@ struct GMG {
   }
   void applyS(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];
+    if (ietan_ok && !eta_pq && g.ietan.p && launch_mass_inv<T>(mode, g.L, x, g.ietan.p, rhs, dst, st)) return;
     launch_cell_op<T>(OP_MASS, mode, g.L, nullptr, x, g.eta.p, eta_pq, nullptr, rhs, nullptr, dst, st);
   }
   void apply(int kind, int l, Mode mode, const T *x, const T *rhs, T *dst) {
@@ -328,6 +333,11 @@ struct GMG {
     drop_graphs();
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
+      if (!eta_pq && g.L.dim == 3 && g.L.k == 2) {
+        if (g.ietan.n != g.L.cells) g.ietan.alloc(g.L.cells);
+        launch_inv_eta_nat<T>(g.L, g.eta.p, g.ietan.p, st);
+      }
+      ietan_ok = true;
       launch_cell_op<T>(OP_ADIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, eta_pq, nullptr, nullptr, g.dinvA.p,
                         nullptr, st);
       launch_cell_op<T>(OP_MDIAG, MODE_SET, g.L, nullptr, nullptr, g.eta.p, eta_pq, nullptr, nullptr, nullptr,

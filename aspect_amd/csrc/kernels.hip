@@ -794,6 +794,117 @@ __global__ void k_mass_nodes(const LevelDesc L, const T *__restrict__ src, const
     dst[i] += v;
 }
 
+// Pressure mass apply from a natural-order table of inverse cell viscosities
+// (the level operator of the Schur V-cycle): as k_mass_nodes, with the cell
+// weights multiplied instead of divided and, for nodes away from the domain
+// boundary, neighbour and cell offsets that are constants.
+template <int MODE, typename T>
+__global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ ieta,
+                           const T *__restrict__ rhs, T *__restrict__ dst) {
+  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int gz = blockIdx.y;
+  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
+  const int sy = L.np[0], sz = L.np[0] * L.np[1];      // pressure node strides
+  const int cy = L.n[0], cz = L.n[0] * L.n[1];         // cell strides
+  const bool inner = gx > 0 && gy > 0 && gz > 0 && gx + 1 < L.np[0] && gy + 1 < L.np[1] && gz + 1 < L.np[2];
+  const int n0 = gx + sy * gy + sz * gz;               // this node
+  const int c0 = (gx - 1) + cy * (gy - 1) + cz * (gz - 1);  // cell (gx-1, gy-1, gz-1)
+  T pv[3][3][3];
+  if (inner) {
+    const T *const p0 = src + n0;
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) pv[dz][dy][dx] = p0[(dx - 1) + sy * (dy - 1) + sz * (dz - 1)];
+  } else {
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const int x = gx - 1 + dx, y = gy - 1 + dy, z = gz - 1 + dz;
+          const bool in = x >= 0 && y >= 0 && z >= 0 && x < L.np[0] && y < L.np[1] && z < L.np[2];
+          const T v = src[in ? n0 + (dx - 1) + sy * (dy - 1) + sz * (dz - 1) : 0];
+          pv[dz][dy][dx] = in ? v : (T)0;
+        }
+  }
+  constexpr T m1[2] = {T(1) / T(3), T(1) / T(6)};  // Q1 mass on [0,1]: 1/3 on the diagonal, 1/6 off it
+  T acc = 0;
+#pragma unroll
+  for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+      for (int ox = 0; ox < 2; ++ox) {
+        const int c[3] = {gx - 1 + ox, gy - 1 + oy, gz - 1 + oz};
+        const bool valid =
+            inner || (c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < L.n[0] && c[1] < L.n[1] && c[2] < L.n[2]);
+        T cs = 0;
+#pragma unroll
+        for (int jz = 0; jz < 2; ++jz)
+#pragma unroll
+          for (int jy = 0; jy < 2; ++jy)
+#pragma unroll
+            for (int jx = 0; jx < 2; ++jx)
+              cs += m1[(1 - ox) != jx] * m1[(1 - oy) != jy] * m1[(1 - oz) != jz] * pv[oz + jz][oy + jy][ox + jx];
+        const T w = ieta[valid ? c0 + ox + cy * oy + cz * oz : 0];
+        acc += valid ? cs * w : (T)0;
+      }
+  using LC = LevelConst<T>;
+  const T v = LC::vol(L) * LC::s(L) * LC::s(L) * acc;
+  const long long i = pidx + (long long)plane * gz;
+  if constexpr (MODE == MODE_SET)
+    dst[i] = v;
+  else if constexpr (MODE == MODE_RESID)
+    dst[i] = rhs[i] - v;
+  else
+    dst[i] += v;
+}
+
+// natural-order inverse cell viscosity from a tile-blocked cellwise table (3D, degree 2)
+template <typename T>
+__global__ void k_inv_eta_nat(const LevelDesc L, const T *__restrict__ eta_tiles, T *__restrict__ out) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= L.cells) return;
+  const int cx = (int)(c % L.n[0]), cy = (int)((c / L.n[0]) % L.n[1]), cz = (int)(c / ((long long)L.n[0] * L.n[1]));
+  const long long ti = (cx / 4 + (long long)L.ntiles[0] * (cy / 4 + (long long)L.ntiles[1] * (cz / 4))) * 64 +
+                       (cx & 3) + 4 * ((cy & 3) + 4 * (cz & 3));
+  out[c] = (T)1 / eta_tiles[ti];
+}
+
+template <typename T>
+bool launch_mass_inv(Mode mode, const LevelDesc &L, const T *src, const T *ieta, const T *rhs, T *dst,
+                     hipStream_t st) {
+  if (L.dim != 3 || L.k != 2 || !ieta) return false;
+  if (L.pnodes >= (1LL << 31)) throw std::runtime_error("level too large for the node-wise mass apply");
+  const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
+  if (mode == MODE_SET)
+    hipLaunchKernelGGL((k_mass_inv<MODE_SET, T>), grid, dim3(256), 0, st, L, src, ieta, rhs, dst);
+  else if (mode == MODE_RESID)
+    hipLaunchKernelGGL((k_mass_inv<MODE_RESID, T>), grid, dim3(256), 0, st, L, src, ieta, rhs, dst);
+  else
+    hipLaunchKernelGGL((k_mass_inv<MODE_ADD, T>), grid, dim3(256), 0, st, L, src, ieta, rhs, dst);
+  MFS_HIP(hipGetLastError());
+  return true;
+}
+template bool launch_mass_inv<double>(Mode, const LevelDesc &, const double *, const double *, const double *,
+                                      double *, hipStream_t);
+template bool launch_mass_inv<float>(Mode, const LevelDesc &, const float *, const float *, const float *, float *,
+                                     hipStream_t);
+template <typename T>
+void launch_inv_eta_nat(const LevelDesc &L, const T *eta_tiles, T *out, hipStream_t st) {
+  const unsigned nb = (unsigned)((L.cells + 255) / 256);
+  hipLaunchKernelGGL((k_inv_eta_nat<T>), dim3(nb), dim3(256), 0, st, L, eta_tiles, out);
+  MFS_HIP(hipGetLastError());
+}
+template void launch_inv_eta_nat<double>(const LevelDesc &, const double *, double *, hipStream_t);
+template void launch_inv_eta_nat<float>(const LevelDesc &, const float *, float *, hipStream_t);
+
 template <typename T>
 bool launch_mass_nodes(Mode mode, const LevelDesc &L, const T *src, const T *eta, int eta_per_qp, const T *rhs,
                        T *dst, hipStream_t st) {
