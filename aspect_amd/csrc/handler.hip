@@ -775,9 +775,8 @@ struct mfs_handle {
           stokes(MODE_SET, z, nullptr, w);
         }
         const VecSet V = vecset(vchunks, 0, j);
-        v_dcgs_reduce(V, n, j, vj, w, n, dot_work.p, st);
+        v_dcgs_reduce(V, n, j, vj, w, n, dot_work.p, hbuf.p, st);
         const int rows = 2 * j + 3;
-        v_sum_partials(dot_work.p, rows, hbuf.p, st);
         red.resize(rows);
         MFS_HIP(hipMemcpyAsync(red.data(), hbuf.p, rows * sizeof(double), hipMemcpyDeviceToHost, st));
         MFS_HIP(hipStreamSynchronize(st));

@@ -1794,96 +1794,170 @@ __global__ void k_maxpy_all(VecSet V, long long ld, int m, const double *__restr
   }
 }
 // Delayed classical Gram-Schmidt (one reduction and one update sweep per Arnoldi
-// step).  Reduction rows: s_k = V_k . v and t_k = V_k . w for k < j, then
-// v . v, v . w, w . w; 8 basis vectors per sweep over v and w.
-__global__ void __launch_bounds__(DOT_THREADS) k_dcgs_reduce(VecSet V, long long ld, int j, const double *__restrict__ v,
-                                                             const double *__restrict__ w, long long n,
-                                                             double *__restrict__ part) {
-  constexpr int G = 8;
-  __shared__ double sh[2 * G + 3][DOT_THREADS / 64];
+// step).  Both kernels give every wave chunks of DCGS_E * 64 consecutive entries
+// (lane l owns entries base + e * 64 + l, e < DCGS_E), chunk c = wave, wave +
+// waves, ...: the chunk's entries of v and w stay in registers while the wave
+// streams every basis vector over the chunk, so each vector is read once per
+// sweep with eight 512-byte loads in flight per wave.
+constexpr int DCGS_E = 8;
+constexpr int DCGS_BLOCKS = 1024;  // 4 workgroups of 4 waves per CU (86 VGPRs)
+__device__ __forceinline__ double wave_allsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// Reduction rows s_k = V_k . v and t_k = V_k . w for k < j, then v . v, v . w,
+// w . w.  Per chunk and k the wave sums its lanes' partial products (butterfly,
+// fixed order) and lane k % 64 accumulates the total, so a wave keeps all 2 j
+// rows in four registers per lane; the four waves of a workgroup are added in
+// a fixed order into part[row * DCGS_BLOCKS + block].
+__global__ void __launch_bounds__(256) k_dcgs_reduce(VecSet V, long long ld, int j, const double *__restrict__ v,
+                                                     const double *__restrict__ w, long long n,
+                                                     double *__restrict__ part) {
+  constexpr int E = DCGS_E;
+  __shared__ double sh[4][2 * MV_MAX + 3];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int sweeps = j > 0 ? (j + G - 1) / G : 1;
-  for (int sw = 0; sw < sweeps; ++sw) {
-    const int g0 = sw * G;
-    double as[G], at[G], sa = 0, sb = 0, sc = 0;
+  const long long CH = 64LL * E;
+  const long long nch = (n + CH - 1) / CH;
+  const long long waves = (long long)gridDim.x * 4;
+  double aS0 = 0, aS1 = 0, aT0 = 0, aT1 = 0, vv = 0, vw = 0, ww = 0;
+  for (long long c = blockIdx.x * 4LL + wv; c < nch; c += waves) {
+    const long long base = c * CH + lane;
+    const bool full = (c + 1) * CH <= n;
+    double vr[E], wr[E];
 #pragma unroll
-    for (int k = 0; k < G; ++k) as[k] = at[k] = 0;
-    GRID_STRIDE(i, n) {
-      const double vi = v[i], wi = w[i];
-      if (sw == 0) {
-        sa += vi * vi;
-        sb += vi * wi;
-        sc += wi * wi;
-      }
+    for (int e = 0; e < E; ++e) {
+      const long long idx = base + e * 64;
+      const bool ok = full || idx < n;
+      vr[e] = ok ? v[idx] : 0.0;
+      wr[e] = ok ? w[idx] : 0.0;
+    }
 #pragma unroll
-      for (int k = 0; k < G; ++k)
-        if (g0 + k < j) {
-          const double xk = V.p[(g0 + k) >> 3][((g0 + k) & 7) * ld + i];
-          as[k] += xk * vi;
-          at[k] += xk * wi;
+    for (int e = 0; e < E; ++e) {
+      vv += vr[e] * vr[e];
+      vw += vr[e] * wr[e];
+      ww += wr[e] * wr[e];
+    }
+    for (int k = 0; k < j; ++k) {
+      const double *x = V.p[k >> 3] + (long long)(k & 7) * ld + base;
+      double s = 0, t = 0;
+      if (full) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double xe = x[e * 64];
+          s += xe * vr[e];
+          t += xe * wr[e];
         }
-    }
+      } else {
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
-      const double r1 = wave_sum(as[k]), r2 = wave_sum(at[k]);
-      if (lane == 0) {
-        sh[k][wv] = r1;
-        sh[G + k][wv] = r2;
+        for (int e = 0; e < E; ++e) {
+          const double xe = base + e * 64 < n ? x[e * 64] : 0.0;
+          s += xe * vr[e];
+          t += xe * wr[e];
+        }
+      }
+      s = wave_allsum(s);
+      t = wave_allsum(t);
+      if (lane == (k & 63)) {
+        if (k < 64) {
+          aS0 += s;
+          aT0 += t;
+        } else {
+          aS1 += s;
+          aT1 += t;
+        }
       }
     }
-    {
-      const double r1 = wave_sum(sa), r2 = wave_sum(sb), r3 = wave_sum(sc);
-      if (lane == 0) {
-        sh[2 * G][wv] = r1;
-        sh[2 * G + 1][wv] = r2;
-        sh[2 * G + 2][wv] = r3;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < 2 * G + 3) {
-      const int q = threadIdx.x;
-      double t = 0;
-#pragma unroll
-      for (int u = 0; u < DOT_THREADS / 64; ++u) t += sh[q][u];
-      long long row = -1;
-      if (q < G && g0 + q < j) row = g0 + q;
-      else if (q >= G && q < 2 * G && g0 + q - G < j) row = j + g0 + q - G;
-      else if (q >= 2 * G && sw == 0) row = 2 * j + (q - 2 * G);
-      if (row >= 0) part[row * DOT_BLOCKS + blockIdx.x] = t;
-    }
-    __syncthreads();
   }
+  if (lane < j) {
+    sh[wv][lane] = aS0;
+    sh[wv][j + lane] = aT0;
+  }
+  if (lane + 64 < j) {
+    sh[wv][lane + 64] = aS1;
+    sh[wv][j + lane + 64] = aT1;
+  }
+  vv = wave_allsum(vv);
+  vw = wave_allsum(vw);
+  ww = wave_allsum(ww);
+  if (lane == 0) {
+    sh[wv][2 * j] = vv;
+    sh[wv][2 * j + 1] = vw;
+    sh[wv][2 * j + 2] = ww;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < 2 * j + 3; r += 256)
+    part[(long long)r * DCGS_BLOCKS + blockIdx.x] = ((sh[0][r] + sh[1][r]) + sh[2][r]) + sh[3][r];
 }
 // v <- c1 v + sum_k a1_k V_k;  w <- c2 w + c3 v_old + sum_k a2_k V_k; coef = [a1 (j), a2 (j), c1, c2, c3]
-__global__ void k_dcgs_update(VecSet V, long long ld, int j, const double *__restrict__ coef, double *__restrict__ v,
-                              double *__restrict__ w, long long n) {
+__global__ void __launch_bounds__(256) k_dcgs_update(VecSet V, long long ld, int j, const double *__restrict__ coef,
+                                                     double *__restrict__ v, double *__restrict__ w, long long n) {
+  constexpr int E = DCGS_E;
   __shared__ double cf[2 * MV_MAX + 3];
   for (int k = threadIdx.x; k < 2 * j + 3; k += blockDim.x) cf[k] = coef[k];
   __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double c1 = cf[2 * j], c2 = cf[2 * j + 1], c3 = cf[2 * j + 2];
-  GRID_STRIDE(i, n) {
-    double s1 = 0, s2 = 0;
-    for (int k = 0; k < j; ++k) {
-      const double xk = V.p[k >> 3][(k & 7) * ld + i];
-      s1 += cf[k] * xk;
-      s2 += cf[j + k] * xk;
+  const long long CH = 64LL * E;
+  const long long nch = (n + CH - 1) / CH;
+  const long long waves = (long long)gridDim.x * 4;
+  for (long long c = blockIdx.x * 4LL + wv; c < nch; c += waves) {
+    const long long base = c * CH + lane;
+    const bool full = (c + 1) * CH <= n;
+    double vr[E], wr[E], s1[E], s2[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const long long idx = base + e * 64;
+      const bool ok = full || idx < n;
+      vr[e] = ok ? v[idx] : 0.0;
+      wr[e] = ok ? w[idx] : 0.0;
+      s1[e] = s2[e] = 0;
     }
-    const double vo = v[i];
-    v[i] = c1 * vo + s1;
-    w[i] = c2 * w[i] + c3 * vo + s2;
+    if (full) {
+#pragma unroll 2
+      for (int k = 0; k < j; ++k) {
+        const double *x = V.p[k >> 3] + (long long)(k & 7) * ld + base;
+        const double a1 = cf[k], a2 = cf[j + k];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double xe = x[e * 64];
+          s1[e] += a1 * xe;
+          s2[e] += a2 * xe;
+        }
+      }
+    } else {
+      for (int k = 0; k < j; ++k) {
+        const double *x = V.p[k >> 3] + (long long)(k & 7) * ld + base;
+        const double a1 = cf[k], a2 = cf[j + k];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double xe = base + e * 64 < n ? x[e * 64] : 0.0;
+          s1[e] += a1 * xe;
+          s2[e] += a2 * xe;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const long long idx = base + e * 64;
+      if (full || idx < n) {
+        v[idx] = c1 * vr[e] + s1[e];
+        w[idx] = c2 * wr[e] + c3 * vr[e] + s2[e];
+      }
+    }
   }
 }
 void v_dcgs_reduce(const VecSet &V, long long ld, int j, const double *v, const double *w, long long n, double *work,
-                   hipStream_t st) {
-  if (j > MV_MAX) throw std::runtime_error("v_dcgs_reduce: too many vectors");
-  hipLaunchKernelGGL(k_dcgs_reduce, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, st, V, ld, j, v, w, n, work);
+                   double *rows, hipStream_t st) {
+  if (j < 0 || j > MV_MAX) throw std::runtime_error("v_dcgs_reduce: too many vectors");
+  hipLaunchKernelGGL(k_dcgs_reduce, dim3(DCGS_BLOCKS), dim3(256), 0, st, V, ld, j, v, w, n, work);
+  hipLaunchKernelGGL(k_sum_partials, dim3(2 * j + 3), dim3(DOT_THREADS), 0, st, work, 2 * j + 3, DCGS_BLOCKS, rows);
   MFS_HIP(hipGetLastError());
 }
 void v_dcgs_update(const VecSet &V, long long ld, int j, const double *coef, double *v, double *w, long long n,
                    hipStream_t st) {
-  if (j > MV_MAX) throw std::runtime_error("v_dcgs_update: too many vectors");
-  hipLaunchKernelGGL(k_dcgs_update, dim3(std::min<long long>(nblocks(n), 8192)), dim3(256), 0, st, V, ld, j, coef, v,
-                     w, n);
+  if (j < 0 || j > MV_MAX) throw std::runtime_error("v_dcgs_update: too many vectors");
+  hipLaunchKernelGGL(k_dcgs_update, dim3(DCGS_BLOCKS), dim3(256), 0, st, V, ld, j, coef, v, w, n);
   MFS_HIP(hipGetLastError());
 }
 void v_mdot_all(const VecSet &V, long long ld, int m, const double *w, long long n, double *work, hipStream_t st) {

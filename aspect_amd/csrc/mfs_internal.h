@@ -175,10 +175,11 @@ struct VecSet {
 // partial dots (V_i . w) for i < m into work[i * dot_work_size() + block]
 void v_mdot_all(const VecSet &V, long long ld, int m, const double *w, long long n, double *work, hipStream_t st);
 // delayed classical Gram-Schmidt: reduction rows s (j), t (j), v.v, v.w, w.w
-// into work[row * dot_work_size() + block]; update v <- c1 v + V a1,
+// summed into rows[0 .. 2j+3) on device (work holds the per-workgroup
+// partials, at most (2j+3) * dot_work_size() doubles); update v <- c1 v + V a1,
 // w <- c2 w + c3 v_old + V a2 with coef = [a1, a2, c1, c2, c3] on device
 void v_dcgs_reduce(const VecSet &V, long long ld, int j, const double *v, const double *w, long long n, double *work,
-                   hipStream_t st);
+                   double *rows, hipStream_t st);
 void v_dcgs_update(const VecSet &V, long long ld, int j, const double *coef, double *v, double *w, long long n,
                    hipStream_t st);
 // w += sign * sum_{i<m} h[i] V_i (h on device)
