@@ -947,6 +947,115 @@ void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, c
 #undef MFS_CASE
 }
 
+// B^T block for Q2 x Q1 (the cell operator with test gradient and flux
+// -s p I), evaluated per velocity node.  On a uniform box level every cell has
+// the same Jacobian, and the 3-point Gauss rule integrates the products of a
+// Q2 basis function (or its derivative) with a Q1 one exactly, so the operator
+// is a tensor-product stencil over the pressure nodes:
+//   (B^T p)_{a}(g) = -s |K| / h_a  sum_m p_m  prod_d T_d(g_d, m_d),
+// T_d the 1D "derivative" table for d = a and the 1D "mass" table otherwise,
+// summed over the (at most two) cells around g_d.  On [0,1] with Q2 nodes
+// 0, 1/2, 1 and Q1 nodes 0, 1:
+//   mass  int phi_i psi_m : [[1/6, 0], [1/3, 1/3], [0, 1/6]],
+//   deriv int phi_i' psi_m: [[-5/6, -1/6], [2/3, -2/3], [1/6, 5/6]].
+// A cell-midpoint node (g odd, cell c = (g-1)/2) takes pressure nodes c, c+1;
+// a vertex node (g even, P = g/2) takes P-1 .. P+1 from the cells P-1 (local
+// node 2) and P (local node 0) that exist.  Constrained velocity rows are zero.
+// One thread per velocity node (all components), one launch, fixed order.
+template <typename T>
+__device__ __forceinline__ void bt_taps(int g, int n, int &q0, T cm[3], T cd[3]) {
+  if (g & 1) {
+    q0 = (g - 1) >> 1;
+    cm[0] = T(1) / T(3);
+    cm[1] = T(1) / T(3);
+    cm[2] = 0;
+    cd[0] = T(2) / T(3);
+    cd[1] = -T(2) / T(3);
+    cd[2] = 0;
+  } else {
+    const int P = g >> 1;
+    const bool left = P >= 1, right = P < n;
+    q0 = P - 1;
+    cm[0] = 0;
+    cm[1] = (left ? T(1) / T(6) : T(0)) + (right ? T(1) / T(6) : T(0));
+    cm[2] = 0;
+    cd[0] = left ? T(1) / T(6) : T(0);
+    cd[1] = (left ? T(5) / T(6) : T(0)) - (right ? T(5) / T(6) : T(0));
+    cd[2] = right ? -T(1) / T(6) : T(0);
+  }
+}
+template <int DIM, int MODE, typename T>
+__global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__restrict__ p,
+                                                  const T *__restrict__ rhs, T *__restrict__ dst) {
+  // grid: x over the velocity nodes of an xy plane, y over z (32-bit offsets,
+  // checked by the launcher)
+  const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int gz = blockIdx.y;
+  const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
+  const int g[3] = {gx, gy, gz};
+  int q0[3];
+  T cm[3][3], cd[3][3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) bt_taps<T>(g[d], L.n[d], q0[d], cm[d], cd[d]);
+  T val[3] = {0, 0, 0};
+#pragma unroll
+  for (int tz = 0; tz < (DIM == 3 ? 3 : 1); ++tz) {
+    const int z = DIM == 3 ? min(max(q0[2] + tz, 0), L.np[2] - 1) : 0;
+    const T mz = DIM == 3 ? cm[2][tz] : T(1), dz = DIM == 3 ? cd[2][tz] : T(0);
+#pragma unroll
+    for (int ty = 0; ty < 3; ++ty) {
+      const int y = min(max(q0[1] + ty, 0), L.np[1] - 1);
+      T rm = 0, rd = 0;  // x contractions with the mass and derivative taps
+#pragma unroll
+      for (int tx = 0; tx < 3; ++tx) {
+        const int x = min(max(q0[0] + tx, 0), L.np[0] - 1);
+        const T pv = p[x + L.np[0] * (y + L.np[1] * z)];
+        rm += cm[0][tx] * pv;
+        rd += cd[0][tx] * pv;
+      }
+      val[0] += rd * cm[1][ty] * mz;
+      val[1] += rm * cd[1][ty] * mz;
+      if constexpr (DIM == 3) val[2] += rm * cm[1][ty] * dz;
+    }
+  }
+  using LC = LevelConst<T>;
+  const T sv = -LC::s(L) * LC::vol(L);
+  const long long node = pidx + (long long)plane * gz;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    const T v = is_con(L, a, gx, gy, gz) ? T(0) : sv * LC::ih(L, a) * val[a];
+    const long long i = a * L.vnodes + node;
+    if constexpr (MODE == MODE_SET)
+      dst[i] = v;
+    else if constexpr (MODE == MODE_RESID)
+      dst[i] = rhs[i] - v;
+    else
+      dst[i] += v;
+  }
+}
+template <typename T>
+bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs_u, T *dst_u, hipStream_t st) {
+  if (L.k != 2) return false;
+  if ((long long)L.nv[0] * L.nv[1] >= (1LL << 31) || L.pnodes >= (1LL << 31))
+    throw std::runtime_error("level too large for the node-wise B^T apply (32-bit offsets)");
+  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(L.dim == 3 ? L.nv[2] : 1));
+#define MFS_BT(D, M) hipLaunchKernelGGL((k_bt_nodes<D, M, T>), grid, dim3(256), 0, st, L, src_p, rhs_u, dst_u)
+  if (L.dim == 3) {
+    if (mode == MODE_SET) MFS_BT(3, MODE_SET);
+    else if (mode == MODE_RESID) MFS_BT(3, MODE_RESID);
+    else MFS_BT(3, MODE_ADD);
+  } else {
+    if (mode == MODE_SET) MFS_BT(2, MODE_SET);
+    else if (mode == MODE_RESID) MFS_BT(2, MODE_RESID);
+    else MFS_BT(2, MODE_ADD);
+  }
+#undef MFS_BT
+  MFS_HIP(hipGetLastError());
+  return true;
+}
+
 template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
@@ -956,6 +1065,7 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
     return;
 #endif
   if (op == OP_MASS && launch_mass_nodes<T>(mode, L, src_p, eta, eta_per_qp, rhs_p, dst_p, st)) return;
+  if (op == OP_BT && launch_bt_nodes<T>(mode, L, src_p, rhs_u, dst_u, st)) return;
   if (L.dim == 3 && L.k == 2)
     launch_dk<3, 2, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
   else if (L.dim == 3 && L.k == 3)
