@@ -13,7 +13,10 @@ V-cycle and with the FP64 V-cycle (both inside FP64 GMRES/FGMRES), and the
 time to solve an ensemble of 48 independent 2D problems together.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-solve] [--no-cpu]
-                      [--no-ensemble]
+                      [--no-ensemble] [--fgmres-profile K]
+
+``--fgmres-profile K`` runs only K cheap iterations of the FP32-GMG solve and
+prints their time (the command the per-iteration kernel profile is taken on).
 """
 from __future__ import annotations
 
@@ -56,6 +59,31 @@ def make_config(size):
     return cfg
 
 
+def fgmres_profile(cfg, eta, rho, s, its):
+    """K cheap iterations of the FP32-GMG solve (no expensive steps): one
+    Arnoldi step = block preconditioner + Stokes apply + DCGS reduce/update."""
+    import torch
+    from aspect_amd import StokesMatrixFreeHandler
+
+    h = StokesMatrixFreeHandler(cfg.box(s), "fp32")
+    h.assemble(eta, "harmonic")
+    h.build_preconditioner()
+    sz = h.sizes(h.top)
+    n = sz["nu"] + sz["np"]
+    dev = torch.device("cuda:0")
+    rho_d = torch.from_numpy(np.ascontiguousarray(rho.reshape(-1))).to(dev)
+    rhs = torch.empty(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    h.assemble_buoyancy_rhs_device(rho_d.data_ptr(), cfg.gravity(), rhs.data_ptr())
+    sol = torch.zeros(n, dtype=torch.float64, device=dev)
+    rc, r = h.solve_device(rhs.data_ptr(), 0, sol.data_ptr(), tolerance=cfg.tolerance, n_cheap_steps=its,
+                           n_expensive_steps=0)
+    k = max(1, r["cheap_iterations"])
+    print(json.dumps({"fgmres_iterations": r["cheap_iterations"], "solve_seconds": r["solve_seconds"],
+                      "ms_per_iteration": r["solve_seconds"] / k * 1e3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +96,8 @@ def main():
                     help="skip the ensemble of 48 independent 2D problems")
     ap.add_argument("--cheap-steps", type=int, default=1000,
                     help="Number of cheap Stokes solver steps (ASPECT's default 1000)")
+    ap.add_argument("--fgmres-profile", type=int, default=0, metavar="K",
+                    help="only run K cheap FP32-GMG FGMRES iterations of the solve (per-iteration kernel profile)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 1 GiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -94,6 +124,8 @@ def main():
     t0 = time.perf_counter()
     eta, rho, s = P.fields(cfg)
     log(f"fields {cfg.n_cells} cells in {time.perf_counter() - t0:.1f}s, pressure scaling {s:.4g}")
+    if args.fgmres_profile:
+        return fgmres_profile(cfg, eta, rho, s, args.fgmres_profile)
 
     h32 = StokesMatrixFreeHandler(cfg.box(s), "fp32")
     h32.assemble(eta, "none")
