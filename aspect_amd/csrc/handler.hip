@@ -363,6 +363,10 @@ struct GMG {
     T *dinv = kind == 0 ? g.dinvA.p : g.dinvS.p;
     T *r = kind == 0 ? g.ur.p : g.pr.p;
     T *d = kind == 0 ? g.ud.p : g.pd.p;
+    if (kind == 1 && ietan_ok && !eta_pq && g.ietan.p && g.L.dim == 3 && g.L.k == 2) {
+      smooth_schur_fused(g, ch, x, b, x_zero, steps);
+      return;
+    }
     // residual + update in one pass where the A-block apply is split
     auto fused = [&](T f1, T f2, int first) {
       return kind == 0 && launch_ablock_cheb<T>(g.L, x, b, g.eta.p, eta_pq, dinv, d, f1, f2, first, split_for(l), st);
@@ -390,6 +394,39 @@ struct GMG {
         rho_old = rho;
       }
     }
+  }
+
+  // Schur smoother with the residual fused into every Chebyshev step
+  // (launch_mass_cheb).  The step reads x around each node, so it writes the
+  // other of two buffers: x and the level's residual buffer, which is free
+  // during smoothing.  The first buffer is chosen so that the last step lands
+  // in x.
+  void smooth_schur_fused(GMGLevel<T> &g, const ChebInfo &ch, T *x, const T *b, bool x_zero, int steps) {
+    const bool inner = ch.degree >= 2 && std::fabs(ch.delta) >= 1e-40;
+    int swaps = 0;
+    for (int s = 0; s < steps; ++s) swaps += (x_zero && s == 0 ? 0 : 1) + (inner ? ch.degree - 1 : 0);
+    T *cur = x, *alt = g.pr.p;
+    if (x_zero && (swaps & 1)) std::swap(cur, alt);
+    const T inv_theta = (T)(1.0 / ch.theta);
+    for (int s = 0; s < steps; ++s) {
+      if (x_zero && s == 0) {
+        v_cheb_first<T>(g.dinvS.p, b, g.pd.p, cur, inv_theta, true, g.np, st);
+      } else {
+        launch_mass_cheb<T>(g.L, cur, g.ietan.p, b, g.dinvS.p, g.pd.p, alt, (T)0, inv_theta, 1, st);
+        std::swap(cur, alt);
+      }
+      if (!inner) continue;
+      const double sigma = ch.theta / ch.delta;
+      double rho_old = ch.delta / ch.theta;
+      for (int k = 1; k < ch.degree; ++k) {
+        const double rho = 1.0 / (2.0 * sigma - rho_old);
+        const double f1 = rho * rho_old, f2 = 2.0 * rho / ch.delta;
+        launch_mass_cheb<T>(g.L, cur, g.ietan.p, b, g.dinvS.p, g.pd.p, alt, (T)f1, (T)f2, 0, st);
+        std::swap(cur, alt);
+        rho_old = rho;
+      }
+    }
+    if (cur != x) MFS_HIP(hipMemcpyAsync(x, cur, g.np * sizeof(T), hipMemcpyDeviceToDevice, st));
   }
 
   void vcycle_level(int kind, int l) {

@@ -798,14 +798,9 @@ __global__ void k_mass_nodes(const LevelDesc L, const T *__restrict__ src, const
 // (the level operator of the Schur V-cycle): as k_mass_nodes, with the cell
 // weights multiplied instead of divided and, for nodes away from the domain
 // boundary, neighbour and cell offsets that are constants.
-template <int MODE, typename T>
-__global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ ieta,
-                           const T *__restrict__ rhs, T *__restrict__ dst) {
-  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
-  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pidx >= plane) return;
-  const int gz = blockIdx.y;
-  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
+template <typename T>
+__device__ __forceinline__ T mass_inv_node(const LevelDesc &L, const T *__restrict__ src, const T *__restrict__ ieta,
+                                           const int gx, const int gy, const int gz) {
   const int sy = L.np[0], sz = L.np[0] * L.np[1];      // pressure node strides
   const int cy = L.n[0], cz = L.n[0] * L.n[1];         // cell strides
   const bool inner = gx > 0 && gy > 0 && gz > 0 && gx + 1 < L.np[0] && gy + 1 < L.np[1] && gz + 1 < L.np[2];
@@ -856,7 +851,19 @@ __global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T
         acc += valid ? cs * w : (T)0;
       }
   using LC = LevelConst<T>;
-  const T v = LC::vol(L) * LC::s(L) * LC::s(L) * acc;
+  return LC::vol(L) * LC::s(L) * LC::s(L) * acc;
+}
+template <int MODE, typename T>
+__global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ ieta,
+                           const T *__restrict__ rhs, T *__restrict__ dst) {
+  // grid: x over the nodes of an xy plane, y over z (32-bit offsets, checked
+  // by the launcher)
+  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int gz = blockIdx.y;
+  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
+  const T v = mass_inv_node<T>(L, src, ieta, gx, gy, gz);
   const long long i = pidx + (long long)plane * gz;
   if constexpr (MODE == MODE_SET)
     dst[i] = v;
@@ -865,7 +872,27 @@ __global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T
   else
     dst[i] += v;
 }
-
+// One Chebyshev step of the Schur smoother fused with its residual:
+//   r = b - M x_in,  d = (first ? 0 : f1 d) + f2 D^-1 r,  x_out = x_in + d.
+// x_in is read around every node, so x_out is a different buffer (the caller
+// alternates the two).
+template <typename T>
+__global__ void k_mass_cheb(const LevelDesc L, const T *__restrict__ xin, const T *__restrict__ ieta,
+                            const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ d,
+                            T *__restrict__ xout, T f1, T f2, int first) {
+  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
+  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int gz = blockIdx.y;
+  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
+  const long long i = pidx + (long long)plane * gz;
+  const T bi = b[i], di = dinv[i], dold = first ? T(0) : d[i];
+  const T v = mass_inv_node<T>(L, xin, ieta, gx, gy, gz);
+  const T r = bi - v;
+  const T dd = first ? di * r * f2 : f1 * dold + f2 * di * r;
+  d[i] = dd;
+  xout[i] = xin[i] + dd;
+}
 // natural-order inverse cell viscosity from a tile-blocked cellwise table (3D, degree 2)
 template <typename T>
 __global__ void k_inv_eta_nat(const LevelDesc L, const T *__restrict__ eta_tiles, T *__restrict__ out) {
@@ -892,6 +919,21 @@ bool launch_mass_inv(Mode mode, const LevelDesc &L, const T *src, const T *ieta,
   MFS_HIP(hipGetLastError());
   return true;
 }
+template <typename T>
+bool launch_mass_cheb(const LevelDesc &L, const T *xin, const T *ieta, const T *b, const T *dinv, T *d, T *xout,
+                      T f1, T f2, int first, hipStream_t st) {
+  if (L.dim != 3 || L.k != 2 || !ieta) return false;
+  if (L.pnodes >= (1LL << 31)) throw std::runtime_error("level too large for the node-wise mass apply");
+  if (xin == xout) throw std::runtime_error("launch_mass_cheb: x_in and x_out must differ");
+  const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
+  hipLaunchKernelGGL((k_mass_cheb<T>), grid, dim3(256), 0, st, L, xin, ieta, b, dinv, d, xout, f1, f2, first);
+  MFS_HIP(hipGetLastError());
+  return true;
+}
+template bool launch_mass_cheb<double>(const LevelDesc &, const double *, const double *, const double *,
+                                       const double *, double *, double *, double, double, int, hipStream_t);
+template bool launch_mass_cheb<float>(const LevelDesc &, const float *, const float *, const float *, const float *,
+                                      float *, float *, float, float, int, hipStream_t);
 template bool launch_mass_inv<double>(Mode, const LevelDesc &, const double *, const double *, const double *,
                                       double *, hipStream_t);
 template bool launch_mass_inv<float>(Mode, const LevelDesc &, const float *, const float *, const float *, float *,
@@ -964,25 +1006,17 @@ void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, c
 // One thread per velocity node (all components), one launch, fixed order.
 template <typename T>
 __device__ __forceinline__ void bt_taps(int g, int n, int &q0, T cm[3], T cd[3]) {
-  if (g & 1) {
-    q0 = (g - 1) >> 1;
-    cm[0] = T(1) / T(3);
-    cm[1] = T(1) / T(3);
-    cm[2] = 0;
-    cd[0] = T(2) / T(3);
-    cd[1] = -T(2) / T(3);
-    cd[2] = 0;
-  } else {
-    const int P = g >> 1;
-    const bool left = P >= 1, right = P < n;
-    q0 = P - 1;
-    cm[0] = 0;
-    cm[1] = (left ? T(1) / T(6) : T(0)) + (right ? T(1) / T(6) : T(0));
-    cm[2] = 0;
-    cd[0] = left ? T(1) / T(6) : T(0);
-    cd[1] = (left ? T(5) / T(6) : T(0)) - (right ? T(5) / T(6) : T(0));
-    cd[2] = right ? -T(1) / T(6) : T(0);
-  }
+  // branch-free: neighbouring lanes alternate between midpoint and vertex nodes
+  const bool odd = g & 1;
+  const int P = g >> 1;
+  const T l = P >= 1 ? T(1) : T(0), r = P < n ? T(1) : T(0);
+  q0 = odd ? P : P - 1;
+  cm[0] = odd ? T(1) / T(3) : T(0);
+  cm[1] = odd ? T(1) / T(3) : (l + r) * (T(1) / T(6));
+  cm[2] = 0;
+  cd[0] = odd ? T(2) / T(3) : l * (T(1) / T(6));
+  cd[1] = odd ? -T(2) / T(3) : (l - r) * (T(5) / T(6));
+  cd[2] = odd ? T(0) : -r * (T(1) / T(6));
 }
 template <int DIM, int MODE, typename T>
 __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__restrict__ p,
@@ -995,44 +1029,76 @@ __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__
   const int gz = blockIdx.y;
   const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
   const int g[3] = {gx, gy, gz};
+  const long long node = pidx + (long long)plane * gz;
+  // the streamed operand first, so its latency overlaps the pressure gathers
+  T prev[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    if constexpr (MODE == MODE_RESID)
+      prev[a] = rhs[a * L.vnodes + node];
+    else if constexpr (MODE == MODE_ADD)
+      prev[a] = dst[a * L.vnodes + node];
+    else
+      prev[a] = 0;
+  }
   int q0[3];
   T cm[3][3], cd[3][3];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) bt_taps<T>(g[d], L.n[d], q0[d], cm[d], cd[d]);
+  // clamped pressure offsets (a clamped tap has a zero coefficient)
+  unsigned xo[3], yo[3], zo[3];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    xo[t] = (unsigned)min(max(q0[0] + t, 0), L.np[0] - 1);
+    yo[t] = (unsigned)(min(max(q0[1] + t, 0), L.np[1] - 1) * L.np[0]);
+    zo[t] = DIM == 3 ? (unsigned)(min(max(q0[2] + t, 0), L.np[2] - 1)) * (unsigned)(L.np[0] * L.np[1]) : 0u;
+  }
+  T pv[DIM == 3 ? 3 : 1][3][3];
+#pragma unroll
+  for (int tz = 0; tz < (DIM == 3 ? 3 : 1); ++tz)
+#pragma unroll
+    for (int ty = 0; ty < 3; ++ty)
+#pragma unroll
+      for (int tx = 0; tx < 3; ++tx) pv[tz][ty][tx] = p[zo[tz] + yo[ty] + xo[tx]];
   T val[3] = {0, 0, 0};
 #pragma unroll
   for (int tz = 0; tz < (DIM == 3 ? 3 : 1); ++tz) {
-    const int z = DIM == 3 ? min(max(q0[2] + tz, 0), L.np[2] - 1) : 0;
     const T mz = DIM == 3 ? cm[2][tz] : T(1), dz = DIM == 3 ? cd[2][tz] : T(0);
 #pragma unroll
     for (int ty = 0; ty < 3; ++ty) {
-      const int y = min(max(q0[1] + ty, 0), L.np[1] - 1);
       T rm = 0, rd = 0;  // x contractions with the mass and derivative taps
 #pragma unroll
       for (int tx = 0; tx < 3; ++tx) {
-        const int x = min(max(q0[0] + tx, 0), L.np[0] - 1);
-        const T pv = p[x + L.np[0] * (y + L.np[1] * z)];
-        rm += cm[0][tx] * pv;
-        rd += cd[0][tx] * pv;
+        rm += cm[0][tx] * pv[tz][ty][tx];
+        rd += cd[0][tx] * pv[tz][ty][tx];
       }
       val[0] += rd * cm[1][ty] * mz;
       val[1] += rm * cd[1][ty] * mz;
       if constexpr (DIM == 3) val[2] += rm * cm[1][ty] * dz;
     }
   }
+  // boundary faces the node lies on: bit 2d (low face), bit 2d+1 (high face)
+  int on = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) on |= (g[d] == 0 ? 1 << (2 * d) : 0) | (g[d] == L.nv[d] - 1 ? 2 << (2 * d) : 0);
   using LC = LevelConst<T>;
   const T sv = -LC::s(L) * LC::vol(L);
-  const long long node = pidx + (long long)plane * gz;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    const T v = is_con(L, a, gx, gy, gz) ? T(0) : sv * LC::ih(L, a) * val[a];
+    bool con = false;
+    if (on) {
+#pragma unroll
+      for (int f = 0; f < 2 * DIM; ++f)
+        con = con || (((on >> f) & 1) && (L.bc[f] == 2 || (L.bc[f] == 1 && (f >> 1) == a)));
+    }
+    const T v = con ? T(0) : sv * LC::ih(L, a) * val[a];
     const long long i = a * L.vnodes + node;
     if constexpr (MODE == MODE_SET)
       dst[i] = v;
     else if constexpr (MODE == MODE_RESID)
-      dst[i] = rhs[i] - v;
+      dst[i] = prev[a] - v;
     else
-      dst[i] += v;
+      dst[i] = prev[a] + v;
   }
 }
 template <typename T>

@@ -112,6 +112,11 @@ long long transfer_work_size(const LevelDesc &Lc, const LevelDesc &Lf);
 template <typename T>
 bool launch_mass_inv(Mode mode, const LevelDesc &L, const T *src, const T *ieta, const T *rhs, T *dst,
                      hipStream_t st);
+// one fused Chebyshev step of the Schur smoother on such a level:
+// d = (first ? 0 : f1 d) + f2 D^-1 (b - M x_in), x_out = x_in + d (x_out != x_in)
+template <typename T>
+bool launch_mass_cheb(const LevelDesc &L, const T *xin, const T *ieta, const T *b, const T *dinv, T *d, T *xout,
+                      T f1, T f2, int first, hipStream_t st);
 template <typename T>
 void launch_inv_eta_nat(const LevelDesc &L, const T *eta_tiles, T *out, hipStream_t st);
 
