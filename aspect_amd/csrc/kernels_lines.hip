@@ -815,13 +815,15 @@ void launch_ablock_split(Mode mode, const LevelDesc &L, const T *su, const T *et
                      L, 0, su, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work, (T *)nullptr);
   if (split_work_size(L) >= (1LL << 31) || (long long)L.nv[0] * L.nv[1] >= (1LL << 31))
     throw std::runtime_error("level too large for the split A-block apply (32-bit work offsets)");
-  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(3 * L.nv[2]));
+  // 512 nodes (two rows of x) per workgroup: a 128-byte line of the work
+  // array holds parts of several node rows, and one CU then consumes more of it
+  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 511) / 512), (unsigned)(3 * L.nv[2]));
   if (mode == MODE_SET)
-    hipLaunchKernelGGL((k_gather_tiles<MODE_SET, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_SET, T>), grid, dim3(512), 0, st, L, work, su, ru, du);
   else if (mode == MODE_RESID)
-    hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), grid, dim3(512), 0, st, L, work, su, ru, du);
   else
-    hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), grid, dim3(256), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), grid, dim3(512), 0, st, L, work, su, ru, du);
   MFS_HIP(hipGetLastError());
 }
 
@@ -833,8 +835,8 @@ void launch_ablock_cheb_split(const LevelDesc &L, T *x, const T *b, const T *eta
                      (T *)nullptr);
   if (split_work_size(L) >= (1LL << 31) || (long long)L.nv[0] * L.nv[1] >= (1LL << 31))
     throw std::runtime_error("level too large for the split A-block apply (32-bit work offsets)");
-  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(3 * L.nv[2]));
-  hipLaunchKernelGGL((k_gather_cheb<T>), grid, dim3(256), 0, st, L, (const T *)work, x, b, dinv, d, f1, f2, first);
+  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 511) / 512), (unsigned)(3 * L.nv[2]));
+  hipLaunchKernelGGL((k_gather_cheb<T>), grid, dim3(512), 0, st, L, (const T *)work, x, b, dinv, d, f1, f2, first);
   MFS_HIP(hipGetLastError());
 }
 
