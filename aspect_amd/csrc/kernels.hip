@@ -1974,7 +1974,10 @@ __global__ void k_maxpy_all(VecSet V, long long ld, int m, const double *__restr
 // (lane l owns entries base + e * 64 + l, e < DCGS_E), chunk c = wave, wave +
 // waves, ...: the chunk's entries of v and w stay in registers while the wave
 // streams every basis vector over the chunk, so each vector is read once per
-// sweep with eight 512-byte loads in flight per wave.
+// sweep with eight 512-byte loads in flight per wave.  The basis vectors are
+// streamed with nontemporal loads (and the update's results written with
+// nontemporal stores): nothing re-reads them before the next sweep, and
+// keeping them out of the caches makes both kernels 4-10% faster.
 constexpr int DCGS_E = 8;
 constexpr int DCGS_BLOCKS = 1024;  // 4 workgroups of 4 waves per CU (86 VGPRs)
 __device__ __forceinline__ double wave_allsum(double v) {
@@ -2020,7 +2023,7 @@ __global__ void __launch_bounds__(256) k_dcgs_reduce(VecSet V, long long ld, int
       if (full) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const double xe = x[e * 64];
+          const double xe = __builtin_nontemporal_load(x + e * 64);
           s += xe * vr[e];
           t += xe * wr[e];
         }
@@ -2096,7 +2099,7 @@ __global__ void __launch_bounds__(256) k_dcgs_update(VecSet V, long long ld, int
         const double a1 = cf[k], a2 = cf[j + k];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const double xe = x[e * 64];
+          const double xe = __builtin_nontemporal_load(x + e * 64);
           s1[e] += a1 * xe;
           s2[e] += a2 * xe;
         }
@@ -2117,8 +2120,8 @@ __global__ void __launch_bounds__(256) k_dcgs_update(VecSet V, long long ld, int
     for (int e = 0; e < E; ++e) {
       const long long idx = base + e * 64;
       if (full || idx < n) {
-        v[idx] = c1 * vr[e] + s1[e];
-        w[idx] = c2 * wr[e] + c3 * vr[e] + s2[e];
+        __builtin_nontemporal_store(c1 * vr[e] + s1[e], v + idx);
+        __builtin_nontemporal_store(c2 * wr[e] + c3 * vr[e] + s2[e], w + idx);
       }
     }
   }
