@@ -6,7 +6,10 @@ the reference files it follows).  Only ``tests/``, ``__graft_entry__.smoke()``
 and the ``cpu_baseline`` leg of ``bench.py`` import this package; the product
 in ``aspect_amd/`` never does.
 
-Parity status of each function (how it is pinned) is listed in DESIGN.md.
+How each function is pinned: the brute-force dense element-matrix path and
+the sum-factorized path are checked against each other, both against closed
+forms, and the solves against the iteration counts of the reference's own
+test outputs (``tests/test_oracle.py``).
 """
 from __future__ import annotations
 
