@@ -452,7 +452,15 @@ struct GMG {
   void vmult(int kind, const double *src, double *dst) {
     GMGLevel<T> &g = *lv[top];
     const long long n = kind == 0 ? g.nu : g.np;
-    v_cast<double, T>(src, kind == 0 ? g.ub.p : g.pb.p, n, st);
+    v_cast<double, T>(src, input(kind), n, st);
+    vmult_from_input(kind, dst);
+  }
+  // the fine-level right-hand side of a V-cycle (a caller may fill it directly)
+  T *input(int kind) { return kind == 0 ? lv[top]->ub.p : lv[top]->pb.p; }
+  // V-cycle on the right-hand side already in input(kind), result cast to dst
+  void vmult_from_input(int kind, double *dst) {
+    GMGLevel<T> &g = *lv[top];
+    const long long n = kind == 0 ? g.nu : g.np;
     if (graphs) {
       if (!vexec[kind]) {
         hipGraph_t gr = nullptr;
@@ -542,9 +550,23 @@ struct mfs_handle {
   }
 
   // BlockSchurPreconditioner::vmult, cheap: p = -S^{-1} r_p; t = r_u - B^T p; u = A^{-1} t
+  // (degree 2: t is written straight into the V-cycle's fine-level right-hand
+  // side in the V-cycle precision, which is the FP64 t rounded once, as the
+  // cast at the V-cycle entry would)
   void block_prec_cheap(const double *src, double *dst) {
     gmg_vmult(1, src + nu, w5.p);
     v_scale_neg_copy<double>(w5.p, dst + nu, np, st);
+    const bool fused = prec == MFS_GMG_FP32
+                           ? launch_bt_nodes<double, float>(MODE_RESID, Ltop(), dst + nu, src, g32.input(0), st)
+                           : launch_bt_nodes<double, double>(MODE_RESID, Ltop(), dst + nu, src, g64.input(0), st);
+    if (fused) {
+      if (prec == MFS_GMG_FP32)
+        g32.vmult_from_input(0, dst);
+      else
+        g64.vmult_from_input(0, dst);
+      ++a_apps;
+      return;
+    }
     launch_cell_op<double>(OP_BT, MODE_RESID, Ltop(), nullptr, dst + nu, nullptr, 0, src, nullptr, w5.p, nullptr,
                            st);
     gmg_vmult(0, w5.p, dst);

@@ -1018,9 +1018,10 @@ __device__ __forceinline__ void bt_taps(int g, int n, int &q0, T cm[3], T cd[3])
   cd[1] = odd ? -T(2) / T(3) : (l - r) * (T(5) / T(6));
   cd[2] = odd ? T(0) : -r * (T(1) / T(6));
 }
-template <int DIM, int MODE, typename T>
+template <int DIM, int MODE, typename T, typename TO>
 __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__restrict__ p,
-                                                  const T *__restrict__ rhs, T *__restrict__ dst) {
+                                                  const T *__restrict__ rhs, TO *__restrict__ dst) {
+  // arithmetic in T; the result is rounded to TO once, at the store
   // grid: x over the velocity nodes of an xy plane, y over z (32-bit offsets,
   // checked by the launcher)
   const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
@@ -1037,7 +1038,7 @@ __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__
     if constexpr (MODE == MODE_RESID)
       prev[a] = rhs[a * L.vnodes + node];
     else if constexpr (MODE == MODE_ADD)
-      prev[a] = dst[a * L.vnodes + node];
+      prev[a] = (T)dst[a * L.vnodes + node];
     else
       prev[a] = 0;
   }
@@ -1094,20 +1095,20 @@ __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__
     const T v = con ? T(0) : sv * LC::ih(L, a) * val[a];
     const long long i = a * L.vnodes + node;
     if constexpr (MODE == MODE_SET)
-      dst[i] = v;
+      dst[i] = (TO)v;
     else if constexpr (MODE == MODE_RESID)
-      dst[i] = prev[a] - v;
+      dst[i] = (TO)(prev[a] - v);
     else
-      dst[i] = prev[a] + v;
+      dst[i] = (TO)(prev[a] + v);
   }
 }
-template <typename T>
-bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs_u, T *dst_u, hipStream_t st) {
+template <typename T, typename TO>
+bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs_u, TO *dst_u, hipStream_t st) {
   if (L.k != 2) return false;
   if ((long long)L.nv[0] * L.nv[1] >= (1LL << 31) || L.pnodes >= (1LL << 31))
     throw std::runtime_error("level too large for the node-wise B^T apply (32-bit offsets)");
   const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(L.dim == 3 ? L.nv[2] : 1));
-#define MFS_BT(D, M) hipLaunchKernelGGL((k_bt_nodes<D, M, T>), grid, dim3(256), 0, st, L, src_p, rhs_u, dst_u)
+#define MFS_BT(D, M) hipLaunchKernelGGL((k_bt_nodes<D, M, T, TO>), grid, dim3(256), 0, st, L, src_p, rhs_u, dst_u)
   if (L.dim == 3) {
     if (mode == MODE_SET) MFS_BT(3, MODE_SET);
     else if (mode == MODE_RESID) MFS_BT(3, MODE_RESID);
@@ -1122,6 +1123,11 @@ bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs
   return true;
 }
 
+template bool launch_bt_nodes<double, float>(Mode, const LevelDesc &, const double *, const double *, float *,
+                                             hipStream_t);
+template bool launch_bt_nodes<double, double>(Mode, const LevelDesc &, const double *, const double *, double *,
+                                              hipStream_t);
+
 template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
@@ -1131,7 +1137,7 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
     return;
 #endif
   if (op == OP_MASS && launch_mass_nodes<T>(mode, L, src_p, eta, eta_per_qp, rhs_p, dst_p, st)) return;
-  if (op == OP_BT && launch_bt_nodes<T>(mode, L, src_p, rhs_u, dst_u, st)) return;
+  if (op == OP_BT && launch_bt_nodes<T, T>(mode, L, src_p, rhs_u, dst_u, st)) return;
   if (L.dim == 3 && L.k == 2)
     launch_dk<3, 2, T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);
   else if (L.dim == 3 && L.k == 3)

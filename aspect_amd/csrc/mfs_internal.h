@@ -119,6 +119,10 @@ bool launch_mass_cheb(const LevelDesc &L, const T *xin, const T *ieta, const T *
                       T f1, T f2, int first, hipStream_t st);
 template <typename T>
 void launch_inv_eta_nat(const LevelDesc &L, const T *eta_tiles, T *out, hipStream_t st);
+// B^T for degree 2 as a node-wise stencil (dst_u = op(B^T src_p) by mode, rhs_u
+// for RESID), computed in T and stored as TO; false for degree 3
+template <typename T, typename TO>
+bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs_u, TO *dst_u, hipStream_t st);
 
 // viscosity tables
 // natural layout eta[cell*nq + q] (or eta[cell]) -> tile-blocked layout of L
