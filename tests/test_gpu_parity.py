@@ -121,6 +121,38 @@ def test_vcycle_fp32_close_to_fp64():
         assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < 1e-4
 
 
+# FP64: against the oracle's composition to 1e-8 (the Chebyshev set-up, i.e.
+# the eigenvalue estimates, agrees to 1e-9 and the V-cycles inherit that).
+# FP32: the smoother set-up itself runs in FP32, so its Chebyshev intervals
+# differ from the FP64 oracle's by up to ~1 % on small levels and the V-cycles
+# by up to ~3e-3 (the 4^3 no-slip case); the bar there is the library's own
+# FP32 V-cycles.  Both precisions: the velocity part equals the library's
+# V-cycle of r_u - B^T p (the preconditioner writes that residual straight
+# into the V-cycle's right-hand side), 1e-12 in FP64, 1e-6 in FP32 (a
+# last-bit difference of the FP64 residual may round to a neighbouring FP32
+# value).
+@pytest.mark.parametrize("prec,self_tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+@pytest.mark.parametrize("case", CASES[:3])
+def test_cheap_block_preconditioner(case, prec, self_tol):
+    # BlockSchurPreconditioner::vmult, cheap (block_stokes_preconditioner.h:220-255):
+    # p = -S^-1 r_p with the Schur V-cycle, u = A^-1 (r_u - B^T p) with the A V-cycle
+    p, h, eta, sz, rng = setup_pair(*case, s=1.9, prec=prec)
+    h.build_preconditioner()
+    top = case[3] - 1
+    nu = sz["nu"]
+    x = rng.normal(size=nu + sz["np"])
+    out = h.block_preconditioner(x)
+    if prec == "fp64":
+        pp = -O.vcycle(p, eta, 1, x[nu:])
+        uu = O.vcycle(p, eta, 0, x[:nu] - O.bt_vmult(p, top, pp))
+        assert rel(out[nu:], pp) < 1e-8
+        assert rel(out[:nu], uu) < 1e-8
+    else:
+        assert rel(out[nu:], -h.vcycle(1, x[nu:])) < self_tol
+    own = h.vcycle(0, x[:nu] - h.bt_vmult(out[nu:]))
+    assert rel(out[:nu], own) < self_tol
+
+
 @pytest.mark.parametrize("case", CASES[:3])
 def test_buoyancy_rhs(case):
     p, h, eta, sz, rng = setup_pair(*case)
