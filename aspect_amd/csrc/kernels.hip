@@ -1003,7 +1003,8 @@ void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, c
 // A cell-midpoint node (g odd, cell c = (g-1)/2) takes pressure nodes c, c+1;
 // a vertex node (g even, P = g/2) takes P-1 .. P+1 from the cells P-1 (local
 // node 2) and P (local node 0) that exist.  Constrained velocity rows are zero.
-// One thread per velocity node (all components), one launch, fixed order.
+// One thread per pair of velocity nodes along x (all components), one launch,
+// fixed order.
 template <typename T>
 __device__ __forceinline__ void bt_taps(int g, int n, int &q0, T cm[3], T cd[3]) {
   // branch-free: neighbouring lanes alternate between midpoint and vertex nodes
@@ -1021,36 +1022,49 @@ __device__ __forceinline__ void bt_taps(int g, int n, int &q0, T cm[3], T cd[3])
 template <int DIM, int MODE, typename T, typename TO>
 __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__restrict__ p,
                                                   const T *__restrict__ rhs, TO *__restrict__ dst) {
-  // arithmetic in T; the result is rounded to TO once, at the store
-  // grid: x over the velocity nodes of an xy plane, y over z (32-bit offsets,
-  // checked by the launcher)
-  const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
+  // arithmetic in T; the result is rounded to TO once, at the store.  One
+  // thread per pair of nodes along x: the vertex node 2P and the midpoint node
+  // 2P+1 (none for P = n0) share the pressure columns P-1 .. P+1.
+  // grid: x over the pairs of an xy plane, y over z (32-bit offsets, checked
+  // by the launcher)
+  const int npair = L.n[0] + 1;
+  const unsigned plane = (unsigned)(npair * L.nv[1]);
   const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (pidx >= plane) return;
   const int gz = blockIdx.y;
-  const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
-  const int g[3] = {gx, gy, gz};
-  const long long node = pidx + (long long)plane * gz;
+  const int gy = (int)(pidx / (unsigned)npair), P = (int)pidx - gy * npair;
+  const bool has_odd = P < L.n[0];
+  const long long nodeE = 2 * P + (long long)L.nv[0] * (gy + (long long)L.nv[1] * gz);
   // the streamed operand first, so its latency overlaps the pressure gathers
-  T prev[DIM];
+  T prevE[DIM], prevO[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    if constexpr (MODE == MODE_RESID)
-      prev[a] = rhs[a * L.vnodes + node];
-    else if constexpr (MODE == MODE_ADD)
-      prev[a] = (T)dst[a * L.vnodes + node];
-    else
-      prev[a] = 0;
+    const long long i = a * L.vnodes + nodeE;
+    if constexpr (MODE == MODE_RESID) {
+      prevE[a] = rhs[i];
+      prevO[a] = has_odd ? rhs[i + 1] : T(0);
+    } else if constexpr (MODE == MODE_ADD) {
+      prevE[a] = (T)dst[i];
+      prevO[a] = has_odd ? (T)dst[i + 1] : T(0);
+    } else {
+      prevE[a] = prevO[a] = 0;
+    }
   }
+  // x taps over the columns P-1, P, P+1
+  const T l = P >= 1 ? T(1) : T(0), r = P < L.n[0] ? T(1) : T(0);
+  const T cmE[3] = {T(0), (l + r) * (T(1) / T(6)), T(0)};
+  const T cdE[3] = {l * (T(1) / T(6)), (l - r) * (T(5) / T(6)), -r * (T(1) / T(6))};
+  const T cmO[3] = {T(0), T(1) / T(3), T(1) / T(3)};
+  const T cdO[3] = {T(0), T(2) / T(3), -T(2) / T(3)};
   int q0[3];
   T cm[3][3], cd[3][3];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) bt_taps<T>(g[d], L.n[d], q0[d], cm[d], cd[d]);
+  bt_taps<T>(gy, L.n[1], q0[1], cm[1], cd[1]);
+  if constexpr (DIM == 3) bt_taps<T>(gz, L.n[2], q0[2], cm[2], cd[2]);
   // clamped pressure offsets (a clamped tap has a zero coefficient)
   unsigned xo[3], yo[3], zo[3];
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
-    xo[t] = (unsigned)min(max(q0[0] + t, 0), L.np[0] - 1);
+    xo[t] = (unsigned)min(max(P - 1 + t, 0), L.np[0] - 1);
     yo[t] = (unsigned)(min(max(q0[1] + t, 0), L.np[1] - 1) * L.np[0]);
     zo[t] = DIM == 3 ? (unsigned)(min(max(q0[2] + t, 0), L.np[2] - 1)) * (unsigned)(L.np[0] * L.np[1]) : 0u;
   }
@@ -1061,45 +1075,61 @@ __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__
     for (int ty = 0; ty < 3; ++ty)
 #pragma unroll
       for (int tx = 0; tx < 3; ++tx) pv[tz][ty][tx] = p[zo[tz] + yo[ty] + xo[tx]];
-  T val[3] = {0, 0, 0};
+  T valE[3] = {0, 0, 0}, valO[3] = {0, 0, 0};
 #pragma unroll
   for (int tz = 0; tz < (DIM == 3 ? 3 : 1); ++tz) {
     const T mz = DIM == 3 ? cm[2][tz] : T(1), dz = DIM == 3 ? cd[2][tz] : T(0);
 #pragma unroll
     for (int ty = 0; ty < 3; ++ty) {
-      T rm = 0, rd = 0;  // x contractions with the mass and derivative taps
+      T rmE = 0, rdE = 0, rmO = 0, rdO = 0;  // x contractions with the mass and derivative taps
 #pragma unroll
       for (int tx = 0; tx < 3; ++tx) {
-        rm += cm[0][tx] * pv[tz][ty][tx];
-        rd += cd[0][tx] * pv[tz][ty][tx];
+        rmE += cmE[tx] * pv[tz][ty][tx];
+        rdE += cdE[tx] * pv[tz][ty][tx];
+        rmO += cmO[tx] * pv[tz][ty][tx];
+        rdO += cdO[tx] * pv[tz][ty][tx];
       }
-      val[0] += rd * cm[1][ty] * mz;
-      val[1] += rm * cd[1][ty] * mz;
-      if constexpr (DIM == 3) val[2] += rm * cm[1][ty] * dz;
+      valE[0] += rdE * cm[1][ty] * mz;
+      valE[1] += rmE * cd[1][ty] * mz;
+      valO[0] += rdO * cm[1][ty] * mz;
+      valO[1] += rmO * cd[1][ty] * mz;
+      if constexpr (DIM == 3) {
+        valE[2] += rmE * cm[1][ty] * dz;
+        valO[2] += rmO * cm[1][ty] * dz;
+      }
     }
   }
-  // boundary faces the node lies on: bit 2d (low face), bit 2d+1 (high face)
-  int on = 0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) on |= (g[d] == 0 ? 1 << (2 * d) : 0) | (g[d] == L.nv[d] - 1 ? 2 << (2 * d) : 0);
+  // boundary faces a node lies on: bit 2d (low face), bit 2d+1 (high face);
+  // the midpoint node is never on an x face
+  int onyz = 0;
+  onyz |= (gy == 0 ? 1 << 2 : 0) | (gy == L.nv[1] - 1 ? 2 << 2 : 0);
+  if constexpr (DIM == 3) onyz |= (gz == 0 ? 1 << 4 : 0) | (gz == L.nv[2] - 1 ? 2 << 4 : 0);
+  const int onE = onyz | (P == 0 ? 1 : 0) | (2 * P == L.nv[0] - 1 ? 2 : 0);
   using LC = LevelConst<T>;
   const T sv = -LC::s(L) * LC::vol(L);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    bool con = false;
-    if (on) {
+    bool conE = false, conO = false;
 #pragma unroll
-      for (int f = 0; f < 2 * DIM; ++f)
-        con = con || (((on >> f) & 1) && (L.bc[f] == 2 || (L.bc[f] == 1 && (f >> 1) == a)));
+    for (int f = 0; f < 2 * DIM; ++f) {
+      const bool hit = L.bc[f] == 2 || (L.bc[f] == 1 && (f >> 1) == a);
+      conE = conE || (((onE >> f) & 1) && hit);
+      conO = conO || (((onyz >> f) & 1) && hit);
     }
-    const T v = con ? T(0) : sv * LC::ih(L, a) * val[a];
-    const long long i = a * L.vnodes + node;
-    if constexpr (MODE == MODE_SET)
-      dst[i] = (TO)v;
-    else if constexpr (MODE == MODE_RESID)
-      dst[i] = (TO)(prev[a] - v);
-    else
-      dst[i] = (TO)(prev[a] + v);
+    const T ih = LC::ih(L, a);
+    const T vE = conE ? T(0) : sv * ih * valE[a];
+    const T vO = conO ? T(0) : sv * ih * valO[a];
+    const long long i = a * L.vnodes + nodeE;
+    if constexpr (MODE == MODE_SET) {
+      dst[i] = (TO)vE;
+      if (has_odd) dst[i + 1] = (TO)vO;
+    } else if constexpr (MODE == MODE_RESID) {
+      dst[i] = (TO)(prevE[a] - vE);
+      if (has_odd) dst[i + 1] = (TO)(prevO[a] - vO);
+    } else {
+      dst[i] = (TO)(prevE[a] + vE);
+      if (has_odd) dst[i + 1] = (TO)(prevO[a] + vO);
+    }
   }
 }
 template <typename T, typename TO>
@@ -1107,7 +1137,7 @@ bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs
   if (L.k != 2) return false;
   if ((long long)L.nv[0] * L.nv[1] >= (1LL << 31) || L.pnodes >= (1LL << 31))
     throw std::runtime_error("level too large for the node-wise B^T apply (32-bit offsets)");
-  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 255) / 256), (unsigned)(L.dim == 3 ? L.nv[2] : 1));
+  const dim3 grid((unsigned)(((long long)(L.n[0] + 1) * L.nv[1] + 255) / 256), (unsigned)(L.dim == 3 ? L.nv[2] : 1));
 #define MFS_BT(D, M) hipLaunchKernelGGL((k_bt_nodes<D, M, T, TO>), grid, dim3(256), 0, st, L, src_p, rhs_u, dst_u)
   if (L.dim == 3) {
     if (mode == MODE_SET) MFS_BT(3, MODE_SET);
