@@ -22,16 +22,43 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/aspect_mfs.h"
 #include "mfs_internal.h"
 
 using namespace mfs;
+
+namespace mfs {
+// Pointers each operator kernel dereferences, by (op, mode): the velocity
+// operators read src_u and write dst_u, the pressure ones src_p / dst_p, the
+// Stokes operator both; every operator except B^T reads a cell table (the
+// density for OP_RHS); RESID reads the right-hand side of what it writes.
+void check_op_args(Op op, Mode mode, const void *src_u, const void *src_p, const void *eta, const void *rhs_u,
+                   const void *rhs_p, const void *dst_u, const void *dst_p) {
+  if (op < OP_STOKES || op > OP_RHS) throw std::invalid_argument("unknown operator code");
+  if (mode < MODE_SET || mode > MODE_ADD) throw std::invalid_argument("unknown operator mode");
+  const bool rd_u = op == OP_STOKES || op == OP_ABLOCK, rd_p = op == OP_STOKES || op == OP_BT || op == OP_MASS;
+  const bool wr_u = op == OP_STOKES || op == OP_ABLOCK || op == OP_BT || op == OP_ADIAG || op == OP_RHS;
+  const bool wr_p = op == OP_STOKES || op == OP_MASS || op == OP_MDIAG;
+  const bool resid = mode == MODE_RESID && op != OP_ADIAG && op != OP_MDIAG && op != OP_RHS;
+  const char *names[7] = {"src_u", "src_p", "eta", "rhs_u", "rhs_p", "dst_u", "dst_p"};
+  const void *ptrs[7] = {src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p};
+  const bool need[7] = {rd_u, rd_p, op != OP_BT, resid && wr_u, resid && wr_p, wr_u, wr_p};
+  static const char *opn[7] = {"Stokes", "A block", "B^T", "mass", "A diagonal", "mass diagonal", "buoyancy rhs"};
+  for (int i = 0; i < 7; ++i)
+    if (need[i] && !ptrs[i])
+      throw std::invalid_argument(std::string(opn[op]) + " operator: null " + names[i] + " pointer");
+}
+}  // namespace mfs
 
 namespace {
 
@@ -148,6 +175,15 @@ struct GMG {
   DBuf<T> swork;
   T *split_for(int) { return swork.p; }
   bool ietan_ok = false;  // GMGLevel::ietan matches the current level viscosity
+  // Where the level tables came from (the FP64 copy below is rebuilt from them)
+  const double *eta_src = nullptr;
+  int eta_src_pq = 0;
+  const std::vector<DBuf<double>> *q1_src = nullptr;
+  double q1_lo = 0, q1_hi = 0;
+  // FP32 hierarchies: an FP64 copy of the level tables runs the Chebyshev
+  // set-up (diagonals and eigenvalue estimates) and, kept for level 0 only,
+  // the coarse-grid solve of every V-cycle
+  std::unique_ptr<GMG<double>> shadow;
   void drop_graphs() {
     for (hipGraphExec_t &e : vexec)
       if (e) {
@@ -199,6 +235,8 @@ struct GMG {
     drop_graphs();
     ietan_ok = false;  // recomputed by build()
     eta_pq = 0;
+    eta_src = active;
+    eta_src_pq = active_per_qp;
     for (auto &g : lv)
       if (g->eta.n != g->L.ntiles_total * cells_per_tile_of(g->L)) g->eta.alloc(g->L.ntiles_total * cells_per_tile_of(g->L));
     GMGLevel<T> &f = *lv[top];
@@ -214,6 +252,9 @@ struct GMG {
     drop_graphs();
     ietan_ok = false;
     eta_pq = 1;
+    q1_src = &coef;
+    q1_lo = lo;
+    q1_hi = hi;
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
       const long long na = g.L.ntiles_total * cells_per_tile_of(g.L) * qp_per_cell_of(g.L);
@@ -329,7 +370,7 @@ struct GMG {
   }
 
   // diagonals (build_preconditioner) and Chebyshev set-up (solve() :771-849)
-  void build() {
+  void build(const mfs_box &b) {
     drop_graphs();
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
@@ -345,6 +386,11 @@ struct GMG {
       v_invert<T>(g.dinvA.p, g.nu, st);
       v_invert<T>(g.dinvS.p, g.np, st);
     }
+    if constexpr (std::is_same<T, float>::value) {
+      build_shadow(b);
+      return;
+    }
+    (void)b;
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
       const bool coarse = l == 0;
@@ -352,6 +398,43 @@ struct GMG {
       estimate(0, l, g.chA, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
       estimate(1, l, g.chS, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
     }
+  }
+
+  // FP64 copy of the hierarchy: full set-up, Chebyshev intervals copied to
+  // this hierarchy's levels, then everything but level 0 released
+  void build_shadow(const mfs_box &b) {
+    shadow.reset();
+    auto sh = std::make_unique<GMG<double>>();
+    sh->graphs = false;
+    sh->setup(b, st, dot_work, dot_out);
+    if (eta_pq)
+      sh->set_eta_q1(*q1_src, q1_lo, q1_hi);
+    else
+      sh->set_eta(eta_src, eta_src_pq);
+    sh->build(b);
+    for (int l = 0; l <= top; ++l) {
+      lv[l]->chA = sh->lv[l]->chA;
+      lv[l]->chS = sh->lv[l]->chS;
+    }
+    MFS_HIP(hipStreamSynchronize(st));
+    sh->lv.resize(1);
+    sh->top = 0;
+    sh->xwork.release();
+    const LevelDesc &L0 = sh->lv[0]->L;
+    sh->swork.release();
+    if (L0.dim == 3 && L0.k == 2) sh->swork.alloc(split_work_size(L0));
+    shadow = std::move(sh);
+  }
+
+  // level-0 solve (the degree-8 Chebyshev coarse smoother, :855-860) in FP64
+  void coarse_fp64(int kind, int steps) {
+    GMGLevel<T> &g = *lv[0];
+    GMGLevel<double> &c = *shadow->lv[0];
+    const long long n = kind == 0 ? g.nu : g.np;
+    double *cx = kind == 0 ? c.ux.p : c.px.p, *cb = kind == 0 ? c.ub.p : c.pb.p;
+    v_cast<T, double>(kind == 0 ? g.ub.p : g.pb.p, cb, n, st);
+    shadow->smooth(kind, 0, cx, cb, true, steps);
+    v_cast<double, T>(cx, kind == 0 ? g.ux.p : g.px.p, n, st);
   }
 
   // Chebyshev semi-iteration (Saad, Alg. 12.1) with Jacobi preconditioning;
@@ -436,6 +519,12 @@ struct GMG {
     T *r = kind == 0 ? g.ur.p : g.pr.p;
     const int steps = kind == 0 ? 1 : 4;
     if (l == 0) {
+      if constexpr (std::is_same<T, float>::value) {
+        if (shadow) {
+          coarse_fp64(kind, steps);
+          return;
+        }
+      }
       smooth(kind, 0, x, b, true, steps);
       return;
     }
@@ -683,6 +772,7 @@ struct mfs_handle {
       stokes(MODE_RESID, x, b, v0);
       double rho = std::sqrt(dot(v0, v0, n));
       ks.res = rho;
+      if (acc == 0) record(rho);
       if (rho <= tol) {
         ks.converged = true;
         ks.its = acc;
@@ -732,6 +822,7 @@ struct mfs_handle {
         H.push_back(col);
         rho = std::fabs(g[j + 1]);
         ks.res = rho;
+        record(rho);
         if (hn > 0) v_scale(w, 1.0 / hn, n, st);
         if (rho <= tol || hn == 0.0) {
           ks.converged = true;
@@ -789,6 +880,7 @@ struct mfs_handle {
       stokes(MODE_RESID, x, b, v0);
       double rho = std::sqrt(dot(v0, v0, n));
       ks.res = rho;
+      if (acc == 0) record(rho);
       if (rho <= tol) {
         ks.converged = true;
         ks.its = acc;
@@ -858,6 +950,7 @@ struct mfs_handle {
           col[j] = hj * nu;
           col = rotate(col, j - 1, cs, sn, g);
           ks.res = std::fabs(g[j]);
+          record(ks.res);
           m = j;
           if (ks.res <= tol || nu == 0.0) {
             ks.converged = true;
@@ -933,6 +1026,7 @@ struct mfs_handle {
     stokes(MODE_RESID, x, b, r);
     double res = std::sqrt(dot(r, r, n));
     ks.res = res;
+    record(res);
     if (res <= tol) {
       ks.converged = true;
       return ks;
@@ -980,6 +1074,7 @@ struct mfs_handle {
         v_axpby<double>(beta, U(k), 1.0, x, n, st);
         res = std::sqrt(dot(r, r, n));
         ks.res = res;
+        record(res);
         if (res <= tol) {
           ks.converged = true;
           ks.its = step;
@@ -1004,6 +1099,7 @@ struct mfs_handle {
       v_axpby<double>(omega, uhat, 1.0, x, n, st);
       res = std::sqrt(dot(r, r, n));
       ks.res = res;
+      record(res);
       if (res <= tol) {
         ks.converged = true;
         ks.its = step;
@@ -1031,22 +1127,33 @@ struct mfs_handle {
     a_apps = s_apps = 0;
     a_tol = prm.a_block_tolerance;
     s_tol = prm.s_block_tolerance;
-    // initial guess: constrained velocity entries zeroed, pressure / s (:944-975)
-    if (x0)
-      MFS_HIP(hipMemcpyAsync(sol, x0, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    else
+    if (prm.failure_strategy != MFS_FAILURE_ABORT && prm.failure_strategy != MFS_FAILURE_CONTINUE)
+      throw std::runtime_error("mfs_solve: unknown linear solver failure strategy");
+    const bool newton = prm.solve_newton_system != 0;
+    if (newton || !x0) {
+      // Newton update: the best guess is zero (:950-969)
       v_fill<double>(sol, 0.0, n, st);
-    v_zero_constrained<double>(Ltop(), sol, st);
-    v_scale(sol + nu, 1.0 / s, np, st);
-    // residuals for the nonlinear residual and the solver tolerance (:977-1041)
-    stokes(MODE_RESID, sol, rhs, w1.p);
-    res.initial_nonlinear_residual = std::sqrt(dot(w1.p, w1.p, n));
-    MFS_HIP(hipMemcpyAsync(w2.p, sol, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    v_fill<double>(w2.p, 0.0, nu, st);
-    stokes(MODE_RESID, w2.p, rhs, w1.p);
-    const double ru2 = dot(w1.p, w1.p, nu);
-    const double rp2 = dot(rhs + nu, rhs + nu, np);
-    res.r0_norm = std::sqrt(ru2 + rp2);
+    } else {
+      // initial guess: constrained velocity entries zeroed, pressure / s (:944-975)
+      MFS_HIP(hipMemcpyAsync(sol, x0, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      v_zero_constrained<double>(Ltop(), sol, st);
+      v_scale(sol + nu, 1.0 / s, np, st);
+    }
+    if (newton) {
+      // tolerance and initial nonlinear residual from the Newton right-hand side (:1028-1041)
+      res.r0_norm = std::sqrt(dot(rhs, rhs, nu) + dot(rhs + nu, rhs + nu, np));
+      res.initial_nonlinear_residual = res.r0_norm;
+    } else {
+      // residuals for the nonlinear residual and the solver tolerance (:977-1025)
+      stokes(MODE_RESID, sol, rhs, w1.p);
+      res.initial_nonlinear_residual = std::sqrt(dot(w1.p, w1.p, n));
+      MFS_HIP(hipMemcpyAsync(w2.p, sol, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      v_fill<double>(w2.p, 0.0, nu, st);
+      stokes(MODE_RESID, w2.p, rhs, w1.p);
+      const double ru2 = dot(w1.p, w1.p, nu);
+      const double rp2 = dot(rhs + nu, rhs + nu, np);
+      res.r0_norm = std::sqrt(ru2 + rp2);
+    }
     const double tol = prm.tolerance * res.r0_norm;
     res.solver_tolerance = tol;
     hipEvent_t e0, e1;
@@ -1054,7 +1161,9 @@ struct mfs_handle {
     MFS_HIP(hipEventCreate(&e1));
     MFS_HIP(hipEventRecord(e0, st));
     bool ok = false;
+    std::vector<double> hist_cheap, hist_exp;  // residual history per phase (SolverControl history data)
     if (prm.n_cheap_steps > 0) {
+      history = &hist_cheap;
       // The cheap phase stores the preconditioned vectors (flexible update
       // x += Z y): with an FP32 V-cycle the preconditioner is linear only up to
       // FP32 rounding, and the flexible form keeps the residual estimate exact.
@@ -1066,6 +1175,7 @@ struct mfs_handle {
       res.cheap_iterations = ks.its;
       res.final_linear_residual = ks.res;
       ok = ks.converged;
+      history = nullptr;
     }
     if (!ok && prm.n_expensive_steps > 0) {
       for (DBuf<double> *v : {&cu1, &cu2, &cu3, &cu4})
@@ -1073,11 +1183,13 @@ struct mfs_handle {
       for (DBuf<double> *v : {&cp1, &cp2, &cp3, &cp4})
         if (v->n != np) v->alloc(np);
       auto pe = [&](const double *a, double *b) { block_prec_expensive(a, b); };
+      history = &hist_exp;
       KrylovStatus ks = prm.restart < MV_MAX ? fgmres_dcgs(rhs, sol, tol, prm.n_expensive_steps, prm.restart, pe)
                                              : gmres(rhs, sol, tol, prm.n_expensive_steps, prm.restart, true, pe);
       res.expensive_iterations = ks.its;
       res.final_linear_residual = ks.res;
       ok = ks.converged;
+      history = nullptr;
     }
     MFS_HIP(hipEventRecord(e1, st));
     MFS_HIP(hipEventSynchronize(e1));
@@ -1089,16 +1201,70 @@ struct mfs_handle {
     res.a_preconditioner_applications = a_apps;
     res.s_preconditioner_applications = s_apps;
     res.converged = ok ? 1 : 0;
-    // post-steps (:1406-1445): pressure *= s, volume normalization
+    // post-steps (:1406-1445): pressure *= s, volume normalization (not for a Newton update)
     v_scale(sol + nu, s, np, st);
-    const double ip = dot(sol + nu, pnorm_w.p, np);
-    const double vol = dot(pnorm_w.p, ones_p(), np);
-    const double adj = ip / vol;
-    v_fill<double>(w3.p, 1.0, np, st);
-    v_axpby<double>(-adj, w3.p, 1.0, sol + nu, np, st);
-    res.pressure_normalization_adjustment = -adj;
+    if (!newton) {
+      const double ip = dot(sol + nu, pnorm_w.p, np);
+      const double vol = dot(pnorm_w.p, ones_p(), np);
+      const double adj = ip / vol;
+      v_fill<double>(w3.p, 1.0, np, st);
+      v_axpby<double>(-adj, w3.p, 1.0, sol + nu, np, st);
+      res.pressure_normalization_adjustment = -adj;
+    }
     MFS_HIP(hipStreamSynchronize(st));
-    return ok ? MFS_OK : MFS_NO_CONVERGENCE;
+    if (ok) return MFS_OK;
+    // Linear solver failure strategy (:1353-1395)
+    if (prm.failure_strategy == MFS_FAILURE_CONTINUE) return MFS_OK;
+    g_err = failure_message(prm, res, hist_cheap, hist_exp);
+    return MFS_NO_CONVERGENCE;
+  }
+
+  // Utilities::throw_linear_solver_failure_exception (source/utilities.cc:2828-2886):
+  // the residual history of every phase that ran goes to history_file, the
+  // message states the initial, final and required residuals
+  static std::string failure_message(const mfs_solve_params &prm, const mfs_solve_result &res,
+                                     const std::vector<double> &hc, const std::vector<double> &he) {
+    std::vector<const std::vector<double> *> phases;
+    if (prm.n_cheap_steps > 0) phases.push_back(&hc);
+    if (prm.n_expensive_steps > 0) phases.push_back(&he);
+    char buf[512];
+    std::string msg =
+        "The iterative Stokes solver in StokesMatrixFreeHandler::solve did not converge.\n\n";
+    if (!phases.empty() && !phases.front()->empty()) {
+      std::snprintf(buf, sizeof(buf), "The initial residual was: %e\n", phases.front()->front());
+      msg += buf;
+    }
+    if (!phases.empty() && !phases.back()->empty()) {
+      std::snprintf(buf, sizeof(buf), "The final residual is: %e\n", phases.back()->back());
+      msg += buf;
+    }
+    std::snprintf(buf, sizeof(buf), "The required residual for convergence is: %e\n", res.solver_tolerance);
+    msg += buf;
+    if (prm.history_file && prm.history_file[0]) {
+      if (FILE *f = std::fopen(prm.history_file, "w")) {
+        bool first = true;
+        for (const std::vector<double> *ph : phases) {
+          if (ph->empty()) continue;
+          if (!first) std::fputc('\n', f);
+          first = false;
+          for (size_t j = 0; j < ph->size(); ++j) std::fprintf(f, "%zu %.17g\n", j, (*ph)[j]);
+        }
+        std::fclose(f);
+        msg += std::string("See ") + prm.history_file + " for the full convergence history.\n";
+      } else {
+        msg += std::string("The convergence history could not be written to ") + prm.history_file + ".\n";
+      }
+    }
+    std::snprintf(buf, sizeof(buf),
+                  "\nThe solver reported the following error:\nIterative method reported convergence failure in "
+                  "step %d. The residual in the last step was %e.",
+                  res.cheap_iterations + res.expensive_iterations, res.final_linear_residual);
+    msg += buf;
+    return msg;
+  }
+  std::vector<double> *history = nullptr;
+  void record(double r) {
+    if (history) history->push_back(r);
   }
   DBuf<double> ones_p_buf;
   const double *ones_p() {
@@ -1174,6 +1340,7 @@ void create(const mfs_box *b, int prec, mfs_handle **out) {
 }
 
 void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
+  if (!h) throw std::invalid_argument("null mfs_handle");
   if (!eta_qp) throw std::runtime_error("mfs_set_viscosity: null viscosity");
   if (averaging < 0 || averaging > MFS_AVERAGE_PROJECT_Q1)
     throw std::runtime_error("mfs_set_viscosity: invalid averaging");
@@ -1260,20 +1427,36 @@ void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
 struct HostIO {
   mfs_handle *h;
   void in(double *d, const double *hsrc, long long n) {
+    if (!hsrc) throw std::invalid_argument("null host input vector");
     MFS_HIP(hipMemcpyAsync(d, hsrc, n * sizeof(double), hipMemcpyHostToDevice, h->st));
   }
   void out(double *hdst, const double *d, long long n) {
+    if (!hdst) throw std::invalid_argument("null host output vector");
     MFS_HIP(hipMemcpyAsync(hdst, d, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
     MFS_HIP(hipStreamSynchronize(h->st));
   }
 };
 
+void need_handle(const mfs_handle *h) {
+  if (!h) throw std::invalid_argument("null mfs_handle");
+}
 void check_level(const mfs_handle *h, int level, bool allow_active) {
+  need_handle(h);
   if (level == -1 && allow_active) return;
   if (level < 0 || level > h->top) throw std::runtime_error("level out of range");
 }
 void need_eta(const mfs_handle *h) {
+  need_handle(h);
   if (!h->have_eta) throw std::runtime_error("viscosity not set (call mfs_set_viscosity)");
+}
+void need_prec(const mfs_handle *h) {
+  need_handle(h);
+  if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+}
+// host buffers of a host-vector entry point
+void need_ptrs(std::initializer_list<const void *> ps, const char *fn) {
+  for (const void *p : ps)
+    if (!p) throw std::invalid_argument(std::string(fn) + ": null argument");
 }
 
 template <typename F>
@@ -1308,6 +1491,14 @@ extern "C" {
 
 const char *mfs_last_error(void) { return g_err.c_str(); }
 
+int mfs_validate_operator_arguments(int op, int mode, const void *src_u, const void *src_p, const void *eta,
+                                    const void *rhs_u, const void *rhs_p, const void *dst_u, const void *dst_p) {
+  return guard([&] {
+    check_op_args((Op)op, (Mode)mode, src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p);
+    return MFS_OK;
+  });
+}
+
 int mfs_create(const mfs_box *box, int gmg_precision, mfs_handle **out) {
   return guard([&] {
     create(box, gmg_precision, out);
@@ -1333,6 +1524,7 @@ int mfs_name(const mfs_handle *h, char *buf, int len) {
 int mfs_sizes(const mfs_handle *h, int level, long long out[4]) {
   return guard([&] {
     check_level(h, level, false);
+    need_ptrs({out}, "mfs_sizes");
     const LevelDesc &L = h->L[level];
     out[0] = L.cells;
     out[1] = (long long)L.dim * L.vnodes;
@@ -1351,6 +1543,7 @@ int mfs_set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
 
 int mfs_set_pressure_scaling(mfs_handle *h, double s) {
   return guard([&] {
+    need_handle(h);
     if (!(s > 0)) throw std::runtime_error("pressure scaling must be positive");
     h->box.pressure_scaling = s;
     for (auto &L : h->L) {
@@ -1373,9 +1566,9 @@ int mfs_set_pressure_scaling(mfs_handle *h, double s) {
 void build_prec(mfs_handle *h) {
   need_eta(h);
   if (h->prec == MFS_GMG_FP32)
-    h->g32.build();
+    h->g32.build(h->box);
   else
-    h->g64.build();
+    h->g64.build(h->box);
   MFS_HIP(hipStreamSynchronize(h->st));
   h->have_prec = true;
 }
@@ -1401,6 +1594,7 @@ int mfs_stokes_vmult(mfs_handle *h, const double *src, double *dst) {
 int mfs_stokes_vmult_device(mfs_handle *h, const double *d_src, double *d_dst) {
   return guard([&] {
     need_eta(h);
+    need_ptrs({d_src, d_dst}, "mfs_stokes_vmult_device");
     h->stokes(MODE_SET, d_src, nullptr, d_dst);
     MFS_HIP(hipGetLastError());
     return MFS_OK;
@@ -1434,6 +1628,7 @@ int mfs_ablock_vmult_device(mfs_handle *h, int level, const void *d_src, void *d
   return guard([&] {
     need_eta(h);
     check_level(h, level, false);
+    need_ptrs({d_src, d_dst}, "mfs_ablock_vmult_device");
     if (h->prec == MFS_GMG_FP32)
       h->g32.applyA(level, MODE_SET, (const float *)d_src, nullptr, (float *)d_dst);
     else
@@ -1468,6 +1663,7 @@ int mfs_mass_vmult(mfs_handle *h, int level, const double *src, double *dst) {
 
 int mfs_bt_vmult(mfs_handle *h, const double *src_p, double *dst_u) {
   return guard([&] {
+    need_handle(h);
     HostIO io{h};
     io.in(h->w1.p, src_p, h->np);
     launch_cell_op<double>(OP_BT, MODE_SET, h->Ltop(), nullptr, h->w1.p, nullptr, 0, nullptr, nullptr, h->w2.p,
@@ -1523,6 +1719,7 @@ int mfs_mass_diagonal(mfs_handle *h, int level, double *diag) {
 
 int mfs_prolongate(mfs_handle *h, int fine_level, int kind, const double *src, double *dst) {
   return guard([&] {
+    need_handle(h);
     if (fine_level < 1 || fine_level > h->top) throw std::runtime_error("fine_level out of range");
     if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 or 1");
     const LevelDesc &Lc = h->L[fine_level - 1], &Lf = h->L[fine_level];
@@ -1540,6 +1737,7 @@ int mfs_prolongate(mfs_handle *h, int fine_level, int kind, const double *src, d
 
 int mfs_restrict(mfs_handle *h, int fine_level, int kind, const double *src, double *dst) {
   return guard([&] {
+    need_handle(h);
     if (fine_level < 1 || fine_level > h->top) throw std::runtime_error("fine_level out of range");
     if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 or 1");
     const LevelDesc &Lc = h->L[fine_level - 1], &Lf = h->L[fine_level];
@@ -1587,7 +1785,8 @@ int mfs_level_viscosity(mfs_handle *h, int level, double *out) {
 
 int mfs_smoother_info(mfs_handle *h, double *out) {
   return guard([&] {
-    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    need_prec(h);
+    need_ptrs({out}, "mfs_smoother_info");
     auto run = [&](auto &g) {
       for (int l = 0; l <= h->top; ++l) {
         const ChebInfo *cs[2] = {&g.lv[l]->chA, &g.lv[l]->chS};
@@ -1611,8 +1810,9 @@ int mfs_smoother_info(mfs_handle *h, double *out) {
 
 int mfs_chebyshev(mfs_handle *h, int kind, int level, const double *src, double *dst) {
   return guard([&] {
-    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    need_prec(h);
     check_level(h, level, false);
+    if (kind != 0 && kind != 1) throw std::invalid_argument("kind must be 0 (A block) or 1 (Schur)");
     const long long nn = kind == 0 ? (long long)h->box.dim * h->L[level].vnodes : h->L[level].pnodes;
     auto run = [&](auto &g) {
       level_op_host(h, g, nn, nn, src, dst, [&](auto *a, auto *b) {
@@ -1631,7 +1831,8 @@ int mfs_chebyshev(mfs_handle *h, int kind, int level, const double *src, double 
 
 int mfs_vcycle(mfs_handle *h, int kind, const double *src, double *dst) {
   return guard([&] {
-    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    need_prec(h);
+    if (kind != 0 && kind != 1) throw std::invalid_argument("kind must be 0 (A block) or 1 (Schur)");
     const long long nn = kind == 0 ? h->nu : h->np;
     HostIO io{h};
     io.in(h->w1.p, src, nn);
@@ -1646,7 +1847,7 @@ int mfs_vcycle(mfs_handle *h, int kind, const double *src, double *dst) {
 
 int mfs_block_preconditioner(mfs_handle *h, const double *src, double *dst) {
   return guard([&] {
-    if (!h->have_prec) throw std::runtime_error("preconditioner not built");
+    need_prec(h);
     HostIO io{h};
     io.in(h->w3.p, src, h->n);
     h->block_prec_cheap(h->w3.p, h->w4.p);
@@ -1658,7 +1859,8 @@ int mfs_block_preconditioner(mfs_handle *h, const double *src, double *dst) {
 int mfs_solve_device(mfs_handle *h, const double *d_rhs, const double *d_x0, double *d_sol,
                      const mfs_solve_params *prm, mfs_solve_result *res) {
   return guard([&] {
-    if (!prm || !res) throw std::runtime_error("mfs_solve: null params/result");
+    need_handle(h);
+    need_ptrs({prm, res, d_rhs, d_sol}, "mfs_solve_device");
     return h->solve_dev(d_rhs, d_x0, d_sol, *prm, *res);
   });
 }
@@ -1666,7 +1868,8 @@ int mfs_solve_device(mfs_handle *h, const double *d_rhs, const double *d_x0, dou
 int mfs_solve(mfs_handle *h, const double *rhs, const double *x0, double *sol, const mfs_solve_params *prm,
               mfs_solve_result *res) {
   return guard([&] {
-    if (!prm || !res || !rhs || !sol) throw std::runtime_error("mfs_solve: null argument");
+    need_handle(h);
+    need_ptrs({prm, res, rhs, sol}, "mfs_solve");
     DBuf<double> drhs, dx0, dsol;
     drhs.alloc(h->n);
     dsol.alloc(h->n);
@@ -1710,7 +1913,8 @@ void correct_rhs_dev(mfs_handle *h, const double *d_u0, double *d_rhs) {
 
 int mfs_correct_stokes_rhs(mfs_handle *h, const double *u0, double *rhs) {
   return guard([&] {
-    if (!u0 || !rhs) throw std::runtime_error("mfs_correct_stokes_rhs: null argument");
+    need_handle(h);
+    need_ptrs({u0, rhs}, "mfs_correct_stokes_rhs");
     HostIO io{h};
     io.in(h->w2.p, u0, h->n);
     io.in(h->w3.p, rhs, h->n);
@@ -1722,7 +1926,8 @@ int mfs_correct_stokes_rhs(mfs_handle *h, const double *u0, double *rhs) {
 
 int mfs_assemble_buoyancy_rhs(mfs_handle *h, const double *rho_qp, const double *gravity, double *rhs) {
   return guard([&] {
-    if (!rho_qp || !rhs) throw std::runtime_error("mfs_assemble_buoyancy_rhs: null argument");
+    need_handle(h);
+    need_ptrs({rho_qp, rhs}, "mfs_assemble_buoyancy_rhs");
     const long long nqv = h->Ltop().cells * h->nq;
     DBuf<double> rho, out;
     rho.alloc(nqv);
@@ -1736,6 +1941,8 @@ int mfs_assemble_buoyancy_rhs(mfs_handle *h, const double *rho_qp, const double 
 
 int mfs_assemble_buoyancy_rhs_device(mfs_handle *h, const double *d_rho_qp, const double *gravity, double *d_rhs) {
   return guard([&] {
+    need_handle(h);
+    need_ptrs({d_rho_qp, d_rhs}, "mfs_assemble_buoyancy_rhs_device");
     assemble_rhs_dev(h, d_rho_qp, gravity, d_rhs);
     return MFS_OK;
   });
@@ -1745,7 +1952,8 @@ int mfs_time_operator(mfs_handle *h, int which, int level, const void *d_src, vo
                       double *avg_seconds) {
   return guard([&] {
     need_eta(h);
-    if (reps < 1 || !avg_seconds) throw std::runtime_error("mfs_time_operator: bad arguments");
+    if (reps < 1) throw std::runtime_error("mfs_time_operator: reps must be >= 1");
+    need_ptrs({d_src, d_dst, avg_seconds}, "mfs_time_operator");
     if ((which >= 3) && !h->have_prec) throw std::runtime_error("preconditioner not built");
     if (which == 1 || which == 2) check_level(h, level, false);
     auto body = [&]() {
@@ -1802,6 +2010,7 @@ int mfs_set_graph_capture(mfs_handle *h, int enable) {
 
 int mfs_synchronize(mfs_handle *h) {
   return guard([&] {
+    need_handle(h);
     MFS_HIP(hipStreamSynchronize(h->st));
     return MFS_OK;
   });
@@ -1809,6 +2018,8 @@ int mfs_synchronize(mfs_handle *h) {
 
 int mfs_memory_consumption(const mfs_handle *h, long long out[3]) {
   return guard([&] {
+    need_handle(h);
+    need_ptrs({out}, "mfs_memory_consumption");
     long long vec = 0, cell = h->eta_active.bytes() + h->eta_nat.bytes();
     for (const DBuf<double> *v : {&h->w1, &h->w2, &h->w3, &h->w4, &h->w5, &h->pnorm_w}) vec += v->bytes();
     auto acc = [&](const auto &g) {

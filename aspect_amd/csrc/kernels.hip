@@ -1162,6 +1162,7 @@ template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                     T *split_work) {
+  check_op_args(op, mode, src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p);
 #if MFS_CELL_LINES
   if (launch_cell_lines<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st, split_work))
     return;

@@ -73,6 +73,11 @@ int tile_threads(int dim, int k);
 
 void upload_basis_tables();
 
+// Host-side argument check of an operator launch: throws std::invalid_argument
+// naming the first null pointer the (op, mode) kernel would read or write.
+void check_op_args(Op op, Mode mode, const void *src_u, const void *src_p, const void *eta, const void *rhs_u,
+                   const void *rhs_p, const void *dst_u, const void *dst_p);
+
 template <typename T>
 void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,

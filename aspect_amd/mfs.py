@@ -37,6 +37,7 @@ EXPORTED = [
     "mfs_ablock_vmult_device", "mfs_solve_device", "mfs_assemble_buoyancy_rhs_device", "mfs_synchronize",
     "mfs_memory_consumption", "mfs_time_operator", "mfs_set_graph_capture", "mfs_correct_stokes_rhs",
     "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
+    "mfs_validate_operator_arguments",
 ]
 
 
@@ -63,11 +64,44 @@ class SolveParams(C.Structure):
         ("s_block_tolerance", C.c_double),
         ("cheap_krylov", C.c_int),
         ("idr_s", C.c_int),
+        ("solve_newton_system", C.c_int),
+        ("failure_strategy", C.c_int),
+        ("history_file", C.c_char_p),
     ]
 
 
 # "Krylov method for cheap solver steps"
 KRYLOV = {"GMRES": 0, "IDR(s)": 1}
+# "Linear solver failure strategy"
+FAILURE = {"abort": 0, "continue with nonlinear solver": 1}
+# operator codes and modes of mfs_validate_operator_arguments
+OPS = {"stokes": 0, "ablock": 1, "bt": 2, "mass": 3, "adiag": 4, "mdiag": 5, "rhs": 6}
+MODES = {"set": 0, "resid": 1, "add": 2}
+
+
+def _params(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance, krylov,
+            idr_s, solve_newton_system=False, failure_strategy="abort", history_file=None):
+    hf = None if history_file is None else os.fsencode(history_file)
+    return SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance,
+                       KRYLOV[krylov], idr_s, int(bool(solve_newton_system)), FAILURE[failure_strategy], hf)
+
+
+def _vec(a, n, what):
+    """Contiguous FP64 copy of a host vector, checked against the length the library reads or writes."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 1 or a.size != n:
+        raise ValueError(f"{what} must be a vector of {n} values (got shape {a.shape})")
+    return a
+
+
+def validate_operator_arguments(op, mode, src_u=None, src_p=None, eta=None, rhs_u=None, rhs_p=None, dst_u=None,
+                                dst_p=None):
+    """Host-side pointer check the library runs before every operator launch
+    (no device needed); arguments are addresses (ints) or None.  Raises
+    SolverError naming the first null pointer the (op, mode) kernel uses."""
+    lib = load_library()
+    ptrs = [C.c_void_p(x) if x else None for x in (src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p)]
+    _check(lib.mfs_validate_operator_arguments(OPS[op], MODES[mode], *ptrs))
 
 
 class SolveResult(C.Structure):
@@ -195,15 +229,22 @@ class StokesMatrixFreeHandler:
         _check(self.lib.mfs_build_preconditioner(self._h))
 
     def solve(self, rhs, x0=None, tolerance=1e-7, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
-              a_block_tolerance=1e-2, s_block_tolerance=1e-6, raise_on_failure=True, krylov="GMRES", idr_s=2):
-        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+              a_block_tolerance=1e-2, s_block_tolerance=1e-6, raise_on_failure=True, krylov="GMRES", idr_s=2,
+              solve_newton_system=False, failure_strategy="abort", history_file=None):
+        """solve() (interface.h:89-94): rhs and x0 hold N_u + N_p values.  A
+        failed solve raises NoConvergence (with the library's message) unless
+        raise_on_failure is False or failure_strategy is "continue with
+        nonlinear solver"."""
+        s = self.sizes(self.top)
+        n = s["nu"] + s["np"]
+        rhs = _vec(rhs, n, "rhs")
         sol = np.zeros_like(rhs)
-        prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
-                          s_block_tolerance, KRYLOV[krylov], idr_s)
+        prm = _params(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance,
+                      krylov, idr_s, solve_newton_system, failure_strategy, history_file)
         res = SolveResult()
         xp = None
         if x0 is not None:
-            xp = np.ascontiguousarray(x0, dtype=np.float64)
+            xp = _vec(x0, n, "x0")
         rc = _check(self.lib.mfs_solve(self._h, _ptr(rhs), _ptr(xp) if xp is not None else None, _ptr(sol),
                                        C.byref(prm), C.byref(res)))
         out = SolverOutputs(res.initial_nonlinear_residual, res.final_linear_residual,
@@ -211,7 +252,7 @@ class StokesMatrixFreeHandler:
                             res.a_preconditioner_applications, res.s_preconditioner_applications,
                             bool(res.converged), res.solver_tolerance, res.r0_norm, res.solve_seconds)
         if rc == MFS_NO_CONVERGENCE and raise_on_failure:
-            raise NoConvergence("Stokes solver did not converge", out)
+            raise NoConvergence(self.lib.mfs_last_error().decode(), out)
         return sol, out
 
     def name(self) -> str:
@@ -238,8 +279,8 @@ class StokesMatrixFreeHandler:
         _check(self.lib.mfs_sizes(self._h, level, out))
         return dict(cells=out[0], nu=out[1], np=out[2], vnodes=out[3])
 
-    def _unary(self, fn, level_args, src, nout):
-        src = np.ascontiguousarray(src, dtype=np.float64)
+    def _unary(self, fn, level_args, src, nout, nin=None):
+        src = _vec(src, nout if nin is None else nin, "source vector")
         dst = np.zeros(nout)
         _check(fn(self._h, *level_args, _ptr(src), _ptr(dst)))
         return dst
@@ -257,7 +298,8 @@ class StokesMatrixFreeHandler:
         return self._unary(self.lib.mfs_mass_vmult, (level,), src, s["np"])
 
     def bt_vmult(self, src_p):
-        return self._unary(self.lib.mfs_bt_vmult, (), src_p, self.sizes(self.top)["nu"])
+        s = self.sizes(self.top)
+        return self._unary(self.lib.mfs_bt_vmult, (), src_p, s["nu"], s["np"])
 
     def ablock_diagonal(self, level):
         out = np.zeros(self.sizes(level)["nu"])
@@ -270,12 +312,14 @@ class StokesMatrixFreeHandler:
         return out
 
     def prolongate(self, fine_level, kind, src):
-        s = self.sizes(fine_level)
-        return self._unary(self.lib.mfs_prolongate, (fine_level, kind), src, s["nu"] if kind == 0 else s["np"])
+        s, c = self.sizes(fine_level), self.sizes(fine_level - 1)
+        key = "nu" if kind == 0 else "np"
+        return self._unary(self.lib.mfs_prolongate, (fine_level, kind), src, s[key], c[key])
 
     def restrict(self, fine_level, kind, src):
-        s = self.sizes(fine_level - 1)
-        return self._unary(self.lib.mfs_restrict, (fine_level, kind), src, s["nu"] if kind == 0 else s["np"])
+        s, f = self.sizes(fine_level - 1), self.sizes(fine_level)
+        key = "nu" if kind == 0 else "np"
+        return self._unary(self.lib.mfs_restrict, (fine_level, kind), src, s[key], f[key])
 
     def level_viscosity(self, level):
         """Level table: one value per cell, or [cells, nq] with "project to Q1"."""
@@ -306,6 +350,8 @@ class StokesMatrixFreeHandler:
         rho = np.ascontiguousarray(rho_qp, dtype=np.float64)
         g = np.ascontiguousarray(np.resize(np.asarray(gravity, dtype=np.float64), 3))
         s = self.sizes(self.top)
+        if rho.size != s["cells"] * self.n_qp:
+            raise ValueError(f"density table must hold {s['cells']} x {self.n_qp} values")
         out = np.zeros(s["nu"] + s["np"])
         _check(self.lib.mfs_assemble_buoyancy_rhs(self._h, _ptr(rho), _ptr(g), _ptr(out)))
         return out
@@ -323,9 +369,10 @@ class StokesMatrixFreeHandler:
 
     def solve_device(self, d_rhs: int, d_x0, d_sol: int, tolerance=1e-7, n_cheap_steps=1000,
                      n_expensive_steps=1000, restart=100, a_block_tolerance=1e-2, s_block_tolerance=1e-6,
-                     krylov="GMRES", idr_s=2):
-        prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
-                          s_block_tolerance, KRYLOV[krylov], idr_s)
+                     krylov="GMRES", idr_s=2, solve_newton_system=False, failure_strategy="abort",
+                     history_file=None):
+        prm = _params(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance,
+                      krylov, idr_s, solve_newton_system, failure_strategy, history_file)
         res = SolveResult()
         rc = _check(self.lib.mfs_solve_device(self._h, C.c_void_p(d_rhs), C.c_void_p(d_x0) if d_x0 else None,
                                               C.c_void_p(d_sol), C.byref(prm), C.byref(res)))
@@ -343,8 +390,10 @@ class StokesMatrixFreeHandler:
 
     def correct_stokes_rhs(self, rhs, u0):
         """rhs - A(u0) for boundary values u0 on the constrained DoFs (correct_stokes_rhs)."""
-        out = np.array(rhs, dtype=np.float64, copy=True)
-        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        s = self.sizes(self.top)
+        n = s["nu"] + s["np"]
+        out = _vec(rhs, n, "rhs").copy()
+        u0 = _vec(u0, n, "u0")
         _check(self.lib.mfs_correct_stokes_rhs(self._h, _ptr(u0), _ptr(out)))
         return out
 
@@ -392,7 +441,12 @@ class Ensemble:
             self.members.append(StokesMatrixFreeHandler._member(h, b, gmg_precision))
 
     def setup(self, etas, averaging="harmonic", n_threads=16):
+        if len(etas) != len(self.members):
+            raise ValueError(f"ensemble of {len(self.members)} problems: got {len(etas)} viscosity tables")
         etas = [np.ascontiguousarray(e, dtype=np.float64) for e in etas]
+        for i, (e, m) in enumerate(zip(etas, self.members)):
+            if e.size != m.sizes(m.top)["cells"] * m.n_qp:
+                raise ValueError(f"viscosity table of member {i} has the wrong size")
         ptrs = (C.c_void_p * len(etas))(*[e.ctypes.data for e in etas])
         mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
         _check(self.lib.mfs_ensemble_setup(self._e, ptrs, mode, n_threads))
@@ -401,12 +455,15 @@ class Ensemble:
 
     def solve(self, rhss, tolerance=1e-8, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
               a_block_tolerance=1e-2, s_block_tolerance=1e-6, krylov="GMRES", idr_s=2, n_threads=16):
-        rhss = [np.ascontiguousarray(r, dtype=np.float64) for r in rhss]
+        if len(rhss) != len(self.members):
+            raise ValueError(f"ensemble of {len(self.members)} problems: got {len(rhss)} right-hand sides")
+        rhss = [_vec(r, m.sizes(m.top)["nu"] + m.sizes(m.top)["np"], f"rhs of member {i}")
+                for i, (r, m) in enumerate(zip(rhss, self.members))]
         sols = [np.zeros_like(r) for r in rhss]
         rp = (C.c_void_p * len(rhss))(*[r.ctypes.data for r in rhss])
         sp = (C.c_void_p * len(sols))(*[x.ctypes.data for x in sols])
-        prm = SolveParams(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance,
-                          s_block_tolerance, KRYLOV[krylov], idr_s)
+        prm = _params(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance,
+                      krylov, idr_s)
         res = (SolveResult * len(rhss))()
         secs = C.c_double(0)
         _check(self.lib.mfs_ensemble_solve(self._e, rp, sp, C.byref(prm), res, n_threads, C.byref(secs)))

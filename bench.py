@@ -76,6 +76,7 @@ def fgmres_profile(cfg, eta, rho, s, its):
     torch.cuda.synchronize()
     h.assemble_buoyancy_rhs_device(rho_d.data_ptr(), cfg.gravity(), rhs.data_ptr())
     sol = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # the library runs on its own stream
     rc, r = h.solve_device(rhs.data_ptr(), 0, sol.data_ptr(), tolerance=cfg.tolerance, n_cheap_steps=its,
                            n_expensive_steps=0)
     k = max(1, r["cheap_iterations"])
@@ -228,6 +229,7 @@ def main():
             hs.build_preconditioner()
             t_setup = time.perf_counter() - t0
             sol = torch.zeros(n, dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()  # the library runs on its own stream
             rc, r = hs.solve_device(rhs.data_ptr(), 0, sol.data_ptr(), tolerance=cfg.tolerance,
                                     n_cheap_steps=args.cheap_steps)
             hs.synchronize()

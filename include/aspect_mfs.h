@@ -71,6 +71,11 @@ typedef struct {
   double pressure_scaling; /* Simulator::get_pressure_scaling() */
 } mfs_box;
 
+/* "Linear solver failure strategy" (source/simulator/parameters.cc:287-292),
+ * applied when the expensive steps fail (local_smoothing.cc:1353-1395). */
+#define MFS_FAILURE_ABORT 0    /* status MFS_NO_CONVERGENCE, message of Utilities::throw_linear_solver_failure_exception */
+#define MFS_FAILURE_CONTINUE 1 /* "continue with nonlinear solver": status MFS_OK, result.converged = 0 */
+
 typedef struct {
   double tolerance;         /* "Linear solver tolerance" */
   int n_cheap_steps;        /* "Number of cheap Stokes solver steps" */
@@ -81,6 +86,13 @@ typedef struct {
   int cheap_krylov;         /* "Krylov method for cheap solver steps": MFS_KRYLOV_GMRES or MFS_KRYLOV_IDRS
                                (stokes_matrix_free_local_smoothing.cc:1276-1301) */
   int idr_s;                /* "IDR(s) parameter", 1..8 (used with MFS_KRYLOV_IDRS) */
+  int solve_newton_system;  /* solve()'s argument (interface.h:89-94): zero initial guess, tolerance relative
+                               to ||rhs||, initial_nonlinear_residual = ||rhs||, no pressure normalization
+                               (local_smoothing.cc:950-969, 1028-1041, 1443-1444) */
+  int failure_strategy;     /* MFS_FAILURE_ABORT or MFS_FAILURE_CONTINUE */
+  const char *history_file; /* MFS_FAILURE_ABORT: residual history of every solver phase that ran
+                               ("step residual" lines, a blank line between phases), as ASPECT writes
+                               solver_history.txt (source/utilities.cc:2854-2876); NULL: none */
 } mfs_solve_params;
 
 /* StokesSolver::SolverOutputs (include/aspect/simulator/solver/interface.h:36-58)
@@ -101,6 +113,27 @@ typedef struct {
 
 /* Error message of the last failing call on this thread. */
 const char *mfs_last_error(void);
+
+/* Operator codes of mfs_validate_operator_arguments. */
+#define MFS_OP_STOKES 0
+#define MFS_OP_ABLOCK 1
+#define MFS_OP_BT 2
+#define MFS_OP_MASS 3
+#define MFS_OP_ADIAG 4
+#define MFS_OP_MDIAG 5
+#define MFS_OP_RHS 6
+/* Operator modes: 0 dst = op(src), 1 dst = rhs - op(src), 2 dst += op(src). */
+#define MFS_MODE_SET 0
+#define MFS_MODE_RESID 1
+#define MFS_MODE_ADD 2
+
+/* Host-side check that every pointer the (op, mode) kernel reads or writes is
+ * non-null; the library runs it before every operator launch and the *_device
+ * entry points return MFS_ERROR instead of launching on a null pointer.
+ * Needs no device.  No counterpart in the reference (deal.II's FEEvaluation
+ * takes its vectors by reference). */
+int mfs_validate_operator_arguments(int op, int mode, const void *src_u, const void *src_p, const void *eta,
+                                    const void *rhs_u, const void *rhs_p, const void *dst_u, const void *dst_p);
 
 /* setup_dofs(): stokes_matrix_free_local_smoothing.cc:1450-1851 — level meshes,
  * DoF layout, level constraints and device storage. */

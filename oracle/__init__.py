@@ -58,6 +58,7 @@ class SolveParams(C.Structure):
         ("level_eta_mode", C.c_int),
         ("cheap_krylov", C.c_int),   # 0 GMRES, 1 IDR(s)
         ("idr_s", C.c_int),
+        ("solve_newton_system", C.c_int),
     ]
 
 
@@ -307,7 +308,8 @@ def rhs_buoyancy(p, rho_qp, g):
 
 
 def default_params(tolerance=1e-7, n_cheap=1000, n_expensive=1000, restart=100,
-                   a_tol=1e-2, s_tol=1e-6, krylov="GMRES", idr_s=2, level_eta="DGQ0") -> SolveParams:
+                   a_tol=1e-2, s_tol=1e-6, krylov="GMRES", idr_s=2, level_eta="DGQ0",
+                   newton=False) -> SolveParams:
     prm = SolveParams()
     prm.tolerance = tolerance
     prm.n_cheap_steps = n_cheap
@@ -318,6 +320,7 @@ def default_params(tolerance=1e-7, n_cheap=1000, n_expensive=1000, restart=100,
     prm.level_eta_mode = {"DGQ0": 0, "DGQ1": 1}[level_eta]
     prm.cheap_krylov = {"GMRES": 0, "IDR(s)": 1}[krylov]
     prm.idr_s = idr_s
+    prm.solve_newton_system = int(bool(newton))
     return prm
 
 

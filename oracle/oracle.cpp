@@ -59,6 +59,8 @@ typedef struct {
   int level_eta_mode;        // 0: DGQ0 (cell means); 1: DGQ1 ("project to Q1 only viscosity")
   int cheap_krylov;          // "Krylov method for cheap solver steps": 0 GMRES, 1 IDR(s)
   int idr_s;                 // "IDR(s) parameter"
+  int solve_newton_system;   // zero initial guess, tolerance from ||rhs||, no pressure normalization
+                             // (stokes_matrix_free_local_smoothing.cc:950-969, 1028-1041, 1443-1444)
 } orc_solve_params;
 
 typedef struct {
@@ -1761,22 +1763,31 @@ int orc_solve(const orc_problem *p, const double *eta, int per_qp, const orc_sol
       per_qp = 1;
     }
     Op Aop = [&](const double *src, double *dst) { stokes_vmult(C, top, eta, per_qp, src, dst); };
+    const bool newton = prm->solve_newton_system != 0;
     Vec b(rhs, rhs + n), x(x0, x0 + n);
+    if (newton) std::fill(x.begin(), x.end(), 0.0);
     for (long long i = 0; i < nu; ++i)
       if (L.vcon[i]) x[i] = 0.0;
     for (long long i = 0; i < np; ++i) x[nu + i] /= s;
     Vec t(n);
-    Aop(x.data(), t.data());
-    double r2 = 0;
-    for (long long i = 0; i < n; ++i) r2 += (b[i] - t[i]) * (b[i] - t[i]);
-    res->initial_nonlinear_residual = std::sqrt(r2);
-    Vec xu0 = x;
-    std::fill(xu0.begin(), xu0.begin() + nu, 0.0);
-    Aop(xu0.data(), t.data());
-    double ru = 0, rp = 0;
-    for (long long i = 0; i < nu; ++i) ru += (b[i] - t[i]) * (b[i] - t[i]);
-    for (long long i = 0; i < np; ++i) rp += b[nu + i] * b[nu + i];
-    res->r0_norm = std::sqrt(ru + rp);
+    if (newton) {
+      // Newton update: tolerance and initial residual from ||rhs|| (:1028-1041)
+      double r2 = 0;
+      for (long long i = 0; i < n; ++i) r2 += b[i] * b[i];
+      res->r0_norm = res->initial_nonlinear_residual = std::sqrt(r2);
+    } else {
+      Aop(x.data(), t.data());
+      double r2 = 0;
+      for (long long i = 0; i < n; ++i) r2 += (b[i] - t[i]) * (b[i] - t[i]);
+      res->initial_nonlinear_residual = std::sqrt(r2);
+      Vec xu0 = x;
+      std::fill(xu0.begin(), xu0.begin() + nu, 0.0);
+      Aop(xu0.data(), t.data());
+      double ru = 0, rp = 0;
+      for (long long i = 0; i < nu; ++i) ru += (b[i] - t[i]) * (b[i] - t[i]);
+      for (long long i = 0; i < np; ++i) rp += b[nu + i] * b[nu + i];
+      res->r0_norm = std::sqrt(ru + rp);
+    }
     const double tol = prm->tolerance * res->r0_norm;
     res->solver_tolerance = tol;
     int a_apps = 0, s_apps = 0;
@@ -1850,8 +1861,9 @@ int orc_solve(const orc_problem *p, const double *eta, int per_qp, const orc_sol
     }
     res->converged = ok ? 1 : 0;
     for (long long i = 0; i < np; ++i) x[nu + i] *= s;
-    // volume pressure normalization: subtract (int p)/|Omega|
-    {
+    // volume pressure normalization: subtract (int p)/|Omega| (not for a Newton update)
+    res->pressure_normalization_adjustment = 0;
+    if (!newton) {
       Vec ones(np, 1.0), w(np);
       std::vector<double> unit(L.cells, 1.0);
       // int p = p . (M_1 1) with the unweighted pressure mass (s=1, eta=1)

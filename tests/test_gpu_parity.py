@@ -97,7 +97,7 @@ def test_transfers(case):
             assert rel(h.restrict(fl, kind, xf), O.restrict(p, fl, kind, xf)) < FP64
 
 
-@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("case", CASES)
 def test_smoother_setup_and_vcycle_fp64(case):
     p, h, eta, sz, rng = setup_pair(*case, s=1.9)
     h.build_preconditioner()
@@ -112,28 +112,30 @@ def test_smoother_setup_and_vcycle_fp64(case):
         assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < 1e-10
 
 
-def test_vcycle_fp32_close_to_fp64():
-    case = CASES[2]
+@pytest.mark.parametrize("case", CASES)
+def test_vcycle_fp32_matches_oracle(case):
+    # FP32 V-cycle (level operators, smoothers, transfers in FP32; Chebyshev
+    # set-up and the level-0 coarse solve in FP64) against the FP64 oracle
     p, h, eta, sz, rng = setup_pair(*case, s=1.9, prec="fp32")
     h.build_preconditioner()
+    info = h.smoother_info()
+    ref = O.smoother_setup(p, eta)
+    np.testing.assert_allclose(info[:, :, 4], ref[:, :, 4])
+    np.testing.assert_allclose(info[:, :, :4], ref[:, :, :4], rtol=1e-6)
     for kind, key in ((0, "nu"), (1, "np")):
         x = rng.normal(size=sz[key])
-        assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < 1e-4
+        assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < FP32
 
 
-# FP64: against the oracle's composition to 1e-8 (the Chebyshev set-up, i.e.
-# the eigenvalue estimates, agrees to 1e-9 and the V-cycles inherit that).
-# FP32: the smoother set-up itself runs in FP32, so its Chebyshev intervals
-# differ from the FP64 oracle's by up to ~1 % on small levels and the V-cycles
-# by up to ~3e-3 (the 4^3 no-slip case); the bar there is the library's own
-# FP32 V-cycles.  Both precisions: the velocity part equals the library's
-# V-cycle of r_u - B^T p (the preconditioner writes that residual straight
-# into the V-cycle's right-hand side), 1e-12 in FP64, 1e-6 in FP32 (a
-# last-bit difference of the FP64 residual may round to a neighbouring FP32
-# value).
-@pytest.mark.parametrize("prec,self_tol", [("fp64", 1e-12), ("fp32", 1e-6)])
-@pytest.mark.parametrize("case", CASES[:3])
-def test_cheap_block_preconditioner(case, prec, self_tol):
+# Against the oracle's composition: 1e-8 in FP64 (the Chebyshev set-up agrees
+# to 1e-9 and the V-cycles inherit that), 1e-5 in FP32.  Both precisions: the
+# velocity part equals the library's V-cycle of r_u - B^T p (the
+# preconditioner writes that residual straight into the V-cycle's right-hand
+# side), 1e-12 in FP64, 1e-6 in FP32 (a last-bit difference of the FP64
+# residual may round to a neighbouring FP32 value).
+@pytest.mark.parametrize("prec,tol,self_tol", [("fp64", 1e-8, 1e-12), ("fp32", FP32, 1e-6)])
+@pytest.mark.parametrize("case", CASES)
+def test_cheap_block_preconditioner(case, prec, tol, self_tol):
     # BlockSchurPreconditioner::vmult, cheap (block_stokes_preconditioner.h:220-255):
     # p = -S^-1 r_p with the Schur V-cycle, u = A^-1 (r_u - B^T p) with the A V-cycle
     p, h, eta, sz, rng = setup_pair(*case, s=1.9, prec=prec)
@@ -142,13 +144,10 @@ def test_cheap_block_preconditioner(case, prec, self_tol):
     nu = sz["nu"]
     x = rng.normal(size=nu + sz["np"])
     out = h.block_preconditioner(x)
-    if prec == "fp64":
-        pp = -O.vcycle(p, eta, 1, x[nu:])
-        uu = O.vcycle(p, eta, 0, x[:nu] - O.bt_vmult(p, top, pp))
-        assert rel(out[nu:], pp) < 1e-8
-        assert rel(out[:nu], uu) < 1e-8
-    else:
-        assert rel(out[nu:], -h.vcycle(1, x[nu:])) < self_tol
+    pp = -O.vcycle(p, eta, 1, x[nu:])
+    uu = O.vcycle(p, eta, 0, x[:nu] - O.bt_vmult(p, top, pp))
+    assert rel(out[nu:], pp) < tol
+    assert rel(out[:nu], uu) < tol
     own = h.vcycle(0, x[:nu] - h.bt_vmult(out[nu:]))
     assert rel(out[:nu], own) < self_tol
 
@@ -408,3 +407,219 @@ def test_ensemble_matches_single_solves():
         ro, res = O.solve(p, eta_h, rhs, params=O.default_params(tolerance=1e-8))
         assert abs(out.cheap_iterations - res["cheap_iterations"]) <= max(1, 0.1 * res["cheap_iterations"])
         assert rel(sol, ro) < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", [(2, 3, (1, 2, 1), 4, (1.0, 2.0, 1.0), (1, 2, 0, 1, 0, 0)),  # 2D Q3 8x16
+                                  (3, 3, (1, 1, 1), 3, (1.0, 1.0, 1.0), (2, 2, 1, 1, 2, 0))])  # 3D Q3 4^3
+def test_degree3_solve(case, prec):
+    # Q3 x Q2 ("Stokes velocity polynomial degree = 3"): the whole GMG path
+    # (degree-3 level operators, transfers, smoothers, block preconditioner)
+    p, h, eta, sz, rng = setup_pair(*case, s=1.4, prec=prec)
+    h.build_preconditioner()
+    rhs = O.rhs_buoyancy(p, 1.0 + rng.uniform(size=(sz["cells"], O.n_qp(p))), np.array([0.2, -1.0, 0.1])[:case[0]])
+    sol, out = h.solve(rhs, tolerance=1e-8)
+    ref, res = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-8))
+    assert out.converged and res["converged"]
+    assert abs(out.cheap_iterations - res["cheap_iterations"]) <= max(1, 0.1 * res["cheap_iterations"])
+    nu = sz["nu"]
+    assert rel(sol[:nu], ref[:nu]) < 1e-6
+    assert rel(sol[nu:], ref[nu:]) < 1e-6
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[2], CASES[4], (2, 2, (2, 1, 1), 4, (2.0, 1.0, 1.0), (0,) * 4)])
+def test_compressible_operators(case):
+    # compressible: 2 eta eps(u) - 2/3 eta div(u) I (matrix_free_operators.cc:466-479, :994-1000)
+    dim, deg, coarse, nl, ext, bc = case
+    p = O.make_problem(dim, deg, coarse, nl, ext, bc, compressible=True, pressure_scaling=1.6)
+    box = make_box(dim, deg, coarse, nl, ext, bc, compressible=True, pressure_scaling=1.6)
+    h = StokesMatrixFreeHandler(box, "fp64")
+    sz = O.sizes(p, nl - 1)
+    rng = np.random.default_rng(11)
+    eta = np.exp(rng.normal(size=(sz["cells"], O.n_qp(p))))
+    h.assemble(eta, "none")
+    x = rng.normal(size=sz["nu"] + sz["np"])
+    assert rel(h.stokes_vmult(x), O.stokes_vmult(p, nl - 1, eta, x)) < FP64
+    assert rel(h.ablock_vmult(-1, x[:sz["nu"]]), O.ablock_vmult(p, nl - 1, eta, x[:sz["nu"]])) < FP64
+    lv = O.level_viscosity(p, eta)
+    for lev in range(nl):
+        u = rng.normal(size=O.sizes(p, lev)["nu"])
+        assert rel(h.ablock_vmult(lev, u), O.ablock_vmult(p, lev, lv[lev], u)) < FP64
+        assert rel(h.ablock_diagonal(lev), O.ablock_diagonal(p, lev, lv[lev])) < FP64
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[2]])
+def test_geometric_averaging(case):
+    # "geometric average only viscosity": prod_q eta_q^(1/N) per cell
+    # (source/material_model/interface.cc:598-611), level tables by children means
+    p, h, eta, sz, rng = setup_pair(*case)
+    h.assemble(eta, "geometric")
+    eta_g = np.exp(np.mean(np.log(eta), axis=1))
+    lv = O.level_viscosity(p, eta_g)
+    for lev in range(case[3]):
+        np.testing.assert_allclose(h.level_viscosity(lev), lv[lev], rtol=1e-13)
+    x = rng.normal(size=sz["nu"] + sz["np"])
+    assert rel(h.stokes_vmult(x), O.stokes_vmult(p, case[3] - 1, eta_g, x)) < FP64
+
+
+def test_config2_field_apply_64():
+    # the 64^3 workload's own field (8 seeded sinkers, 1e3 contrast, per-qp
+    # viscosity, free slip): one FP64 Stokes apply against the oracle
+    cfg = P.config_sinker_64()
+    eta, rho, s = P.fields(cfg)
+    h = StokesMatrixFreeHandler(cfg.box(s), "fp64")
+    h.assemble(eta, "none")
+    p = O.make_problem(3, 2, cfg.coarse, cfg.n_levels, cfg.extent, cfg.bc, pressure_scaling=s)
+    sz = O.sizes(p, cfg.n_levels - 1)
+    x = np.random.default_rng(64).normal(size=sz["nu"] + sz["np"])
+    assert rel(h.stokes_vmult(x), O.stokes_vmult(p, cfg.n_levels - 1, eta, x)) < FP64
+    rhs = h.assemble_buoyancy_rhs(rho, cfg.gravity())
+    assert rel(rhs, O.rhs_buoyancy(p, rho, cfg.gravity())) < FP64
+
+
+def _config3(n_levels):
+    cfg = P.config_sinker_128()
+    cfg.n_levels = n_levels
+    eta, rho, s = P.fields(cfg)
+    p = O.make_problem(3, 2, cfg.coarse, n_levels, cfg.extent, cfg.bc, pressure_scaling=s)
+    return cfg, eta, rho, s, p
+
+
+def test_config3_field_fp32_gmg_16():
+    # the 128^3 workload's field (16 sinkers, 1e6 contrast, 2x2x2 coarse box,
+    # harmonic averaging) at 16^3: FP32 V-cycle inside FP64 FGMRES against the
+    # oracle's FP64-GMG solve to 1e-8
+    cfg, eta, rho, s, p = _config3(4)
+    h = StokesMatrixFreeHandler(cfg.box(s), "fp32")
+    h.assemble(eta, "harmonic")
+    h.build_preconditioner()
+    rhs = h.assemble_buoyancy_rhs(rho, cfg.gravity())
+    sol, out = h.solve(rhs, tolerance=1e-8)
+    eta_h = 1.0 / np.mean(1.0 / eta, axis=1)
+    ref, res = O.solve(p, eta_h, rhs, params=O.default_params(tolerance=1e-8))
+    assert out.converged and res["converged"]
+    assert abs(out.cheap_iterations - res["cheap_iterations"]) <= 0.1 * res["cheap_iterations"]
+    nu = O.sizes(p, 3)["nu"]
+    assert rel(sol[:nu], ref[:nu]) < 1e-6
+    assert rel(sol[nu:], ref[nu:]) < 1e-6
+    # residual of the GPU solution under the oracle's operator
+    x = sol.copy()
+    x[nu:] = (x[nu:] - out.pressure_normalization_adjustment) / s
+    r = rhs - O.stokes_vmult(p, 3, eta_h, x)
+    assert np.linalg.norm(r) <= 1.05e-8 * res["r0_norm"]
+
+
+def test_config3_field_fp32_gmg_32_golden():
+    # the same field at 32^3 against the oracle's fixture (tests/golden/make_golden.py)
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "config3_sinker_32.npz"))
+    cfg, eta, rho, s, p = _config3(5)
+    assert abs(s - float(g["pressure_scaling"])) <= 1e-12 * s
+    h = StokesMatrixFreeHandler(cfg.box(s), "fp32")
+    h.assemble(eta, "harmonic")
+    h.build_preconditioner()
+    rhs = h.assemble_buoyancy_rhs(rho, cfg.gravity())
+    sol, out = h.solve(rhs, tolerance=1e-8)
+    assert out.converged
+    assert abs(out.r0_norm - float(g["r0"])) <= 1e-10 * float(g["r0"])
+    assert abs(out.cheap_iterations - int(g["cheap"])) <= 0.1 * int(g["cheap"])
+    idx = g["idx"]
+    nu = O.sizes(p, 4)["nu"]
+    vu, vp = idx < nu, idx >= nu
+    assert rel(sol[idx[vu]], g["values"][vu]) < 1e-6
+    assert rel(sol[idx[vp]], g["values"][vp]) < 1e-6
+    assert abs(np.linalg.norm(sol[:nu]) - float(g["norm_u"])) <= 1e-6 * float(g["norm_u"])
+    assert abs(np.linalg.norm(sol[nu:]) - float(g["norm_p"])) <= 1e-6 * float(g["norm_p"])
+    eta_h = 1.0 / np.mean(1.0 / eta, axis=1)
+    x = sol.copy()
+    x[nu:] = (x[nu:] - out.pressure_normalization_adjustment) / s
+    r = rhs - O.stokes_vmult(p, 4, eta_h, x)
+    assert np.linalg.norm(r) <= 1.05e-8 * float(g["r0"])
+
+
+def test_ensemble48_matches_oracle_golden():
+    # configuration 4 at its real sizes (48 problems of 16^2 .. 256^2 cells)
+    # against the oracle's fixture: iteration counts, sampled solutions, norms
+    import os
+    from aspect_amd import Ensemble
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ensemble48.npz"))
+    cfgs = P.config_ensemble(48, seed=4)
+    fields = [P.fields(c) for c in cfgs]
+    ens = Ensemble([c.box(f[2]) for c, f in zip(cfgs, fields)], "fp32")
+    ens.setup([f[0] for f in fields], "harmonic")
+    rhss = [m.assemble_buoyancy_rhs(f[1], c.gravity()) for m, c, f in zip(ens.members, cfgs, fields)]
+    sols, outs, secs = ens.solve(rhss, tolerance=1e-8)
+    off = 0
+    for i, (c, sol, out) in enumerate(zip(cfgs, sols, outs)):
+        n_i = int(g["idx_len"][i])
+        idx, val = g["idx"][off:off + n_i], g["values"][off:off + n_i]
+        off += n_i
+        assert out.converged, i
+        assert abs(out.cheap_iterations - int(g["cheap"][i])) <= max(1, 0.1 * int(g["cheap"][i])), i
+        nu = c.box().dim * (2 * c.n_cells[0] + 1) * (2 * c.n_cells[1] + 1)
+        vu, vp = idx < nu, idx >= nu
+        assert rel(sol[idx[vu]], val[vu]) < 1e-6, i
+        assert rel(sol[idx[vp]], val[vp]) < 1e-6, i
+
+
+def test_newton_update_solve():
+    # solve_newton_system = true: zero initial guess whatever x0 holds,
+    # tolerance relative to ||rhs||, no pressure normalization
+    # (local_smoothing.cc:950-969, 1028-1041, 1443-1444)
+    p, h, eta, sz, rng = setup_pair(*CASES[2], s=1.3)
+    h.build_preconditioner()
+    rhs = rng.normal(size=sz["nu"] + sz["np"])
+    rhs[:sz["nu"]][O.constrained(p, 2)] = 0
+    x0 = rng.normal(size=rhs.size)
+    sol, out = h.solve(rhs, x0=x0, tolerance=1e-8, solve_newton_system=True)
+    ref, res = O.solve(p, eta, rhs, x0=x0, params=O.default_params(tolerance=1e-8, newton=True))
+    assert out.converged and res["converged"]
+    assert abs(out.r0_norm - np.linalg.norm(rhs)) <= 1e-12 * np.linalg.norm(rhs)
+    assert abs(out.initial_nonlinear_residual - np.linalg.norm(rhs)) <= 1e-12 * np.linalg.norm(rhs)
+    assert out.pressure_normalization_adjustment == 0.0
+    assert out.cheap_iterations == res["cheap_iterations"]
+    assert rel(sol, ref) < 1e-6
+
+
+def test_linear_solver_failure_strategies(tmp_path):
+    # "Linear solver failure strategy" (parameters.cc:287-292; local_smoothing.cc:1353-1395)
+    from aspect_amd.mfs import NoConvergence
+    p, h, eta, sz, rng = setup_pair(*CASES[1])
+    h.build_preconditioner()
+    rhs = O.rhs_buoyancy(p, 1.0 + rng.uniform(size=(sz["cells"], O.n_qp(p))), np.array([0, 0, -1.0]))
+    hist = tmp_path / "solver_history.txt"
+    with pytest.raises(NoConvergence) as exc:
+        h.solve(rhs, tolerance=1e-14, n_cheap_steps=3, n_expensive_steps=2, history_file=str(hist))
+    msg = str(exc.value)
+    assert "did not converge" in msg and "The required residual for convergence is" in msg
+    assert str(hist) in msg
+    out = exc.value.outputs
+    assert not out.converged and out.cheap_iterations == 3 and out.expensive_iterations == 2
+    lines = hist.read_text().split("\n\n")
+    assert len(lines) == 2  # cheap phase, then expensive phase
+    cheap = [ln.split() for ln in lines[0].strip().splitlines()]
+    assert [int(a) for a, _ in cheap] == list(range(len(cheap))) and len(cheap) == 4
+    assert abs(float(cheap[0][1]) - out.r0_norm) <= 1e-12 * out.r0_norm
+    # continue with nonlinear solver: no error, unconverged result returned
+    sol, out2 = h.solve(rhs, tolerance=1e-14, n_cheap_steps=3, n_expensive_steps=0,
+                        failure_strategy="continue with nonlinear solver")
+    assert not out2.converged and out2.cheap_iterations == 3 and np.all(np.isfinite(sol))
+
+
+def test_device_entry_points_reject_null_pointers():
+    import ctypes
+    from aspect_amd.mfs import SolverError
+    p, h, eta, sz, rng = setup_pair(*CASES[1])
+    lib = h.lib
+    for call in (lambda: lib.mfs_stokes_vmult_device(h._h, None, None),
+                 lambda: lib.mfs_ablock_vmult_device(h._h, 0, None, None),
+                 lambda: lib.mfs_time_operator(h._h, 0, 0, None, None, 1, None)):
+        assert call() == 1
+        assert b"null" in lib.mfs_last_error()
+    with pytest.raises(ValueError):
+        h.stokes_vmult(np.zeros(3))
+    with pytest.raises(ValueError):
+        h.solve(np.zeros(5))
+    with pytest.raises(SolverError):
+        h.vcycle(2, np.zeros(sz["np"]))
+    del ctypes

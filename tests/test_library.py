@@ -54,3 +54,66 @@ def test_pressure_scaling_rule():
     # geometric mean of midpoint viscosities over the box length scale 0.01 * x extent
     eta = np.array([1.0, 100.0])
     assert abs(P.pressure_scaling(eta, 0.01) - 10.0 / 0.01) < 1e-9
+
+
+# pointers each (op, mode) kernel dereferences: src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p
+_NEEDS = {
+    "stokes": ("src_u", "src_p", "eta", "dst_u", "dst_p"),
+    "ablock": ("src_u", "eta", "dst_u"),
+    "bt": ("src_p", "dst_u"),
+    "mass": ("src_p", "eta", "dst_p"),
+    "adiag": ("eta", "dst_u"),
+    "mdiag": ("eta", "dst_p"),
+    "rhs": ("eta", "dst_u"),
+}
+_RESID = {"stokes": ("rhs_u", "rhs_p"), "ablock": ("rhs_u",), "bt": ("rhs_u",), "mass": ("rhs_p",)}
+_ALL = ("src_u", "src_p", "eta", "rhs_u", "rhs_p", "dst_u", "dst_p")
+
+
+@pytest.mark.parametrize("op", sorted(_NEEDS))
+@pytest.mark.parametrize("mode", ["set", "resid", "add"])
+def test_operator_argument_validation(op, mode):
+    # the host-side check every operator launch runs (no device involved)
+    if not os.path.exists(mfs.LIB_PATH):
+        pytest.skip("libaspect_mfs.so not built")
+    need = set(_NEEDS[op]) | (set(_RESID.get(op, ())) if mode == "resid" else set())
+    # exactly the needed pointers: accepted
+    mfs.validate_operator_arguments(op, mode, **{k: 4096 for k in need})
+    for k in need:  # each needed pointer missing: rejected, naming it
+        args = {j: 4096 for j in need if j != k}
+        with pytest.raises(mfs.SolverError, match=f"null {k} pointer"):
+            mfs.validate_operator_arguments(op, mode, **args)
+    for k in set(_ALL) - need:  # pointers the kernel does not use may be null
+        mfs.validate_operator_arguments(op, mode, **{j: 4096 for j in need})
+
+
+def test_null_handle_is_an_error_not_a_launch():
+    if not os.path.exists(mfs.LIB_PATH):
+        pytest.skip("libaspect_mfs.so not built")
+    lib = mfs.load_library()
+    buf = (ctypes.c_double * 4)()
+    out = (ctypes.c_longlong * 4)()
+    calls = [
+        lambda: lib.mfs_stokes_vmult_device(None, buf, buf),
+        lambda: lib.mfs_ablock_vmult_device(None, 0, buf, buf),
+        lambda: lib.mfs_stokes_vmult(None, buf, buf),
+        lambda: lib.mfs_sizes(None, 0, out),
+        lambda: lib.mfs_set_viscosity(None, buf, 0),
+        lambda: lib.mfs_build_preconditioner(None),
+        lambda: lib.mfs_vcycle(None, 0, buf, buf),
+        lambda: lib.mfs_solve_device(None, buf, None, buf, None, None),
+        lambda: lib.mfs_synchronize(None),
+        lambda: lib.mfs_bt_vmult(None, buf, buf),
+        lambda: lib.mfs_prolongate(None, 1, 0, buf, buf),
+    ]
+    for c in calls:
+        assert c() == mfs.MFS_ERROR
+        assert b"null" in lib.mfs_last_error()
+
+
+def test_vector_length_check():
+    with pytest.raises(ValueError):
+        mfs._vec(np.zeros(5), 6, "rhs")
+    with pytest.raises(ValueError):
+        mfs._vec(np.zeros((2, 3)), 6, "rhs")
+    assert mfs._vec([1, 2, 3], 3, "rhs").dtype == np.float64

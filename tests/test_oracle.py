@@ -432,3 +432,58 @@ def test_stokes_rhs_correction_matches_assembly(dim, deg, nl, coarse, bc):
     assert np.all(corr[:s["nu"]][con] == 0.0)
     # homogeneous boundary values: no correction
     assert np.all(O.stokes_rhs_correction(p, eta, np.zeros_like(u0)) == 0.0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_compressible_term_closed_form(dim):
+    # compressible A block: a(u, v) = int 2 eta eps(u):eps(v) - 2/3 eta div u div v
+    # (matrix_free_operators.cc:466-479, :994-1000).  With u = (x, 0[, 0]),
+    # v = (0, y[, 0]) and constant eta on an open box: a(u, u) = 4/3 eta |Omega|
+    # (2 eta |Omega| without the term), a(u, v) = -2/3 eta |Omega| (0 without)
+    ext = (1.5, 0.8, 0.6)
+    vol = np.prod(ext[:dim])
+    eta0 = 2.5
+    for comp in (False, True):
+        p = O.make_problem(dim, 2, (2, 1, 1), 2, ext, (0,) * 6, compressible=comp)
+        s = O.sizes(p, 1)
+        X = O.velocity_support_points(p, 1)
+        nv = s["vnodes"]
+        u = np.zeros((dim, nv))
+        u[0] = X[:, 0]
+        v = np.zeros((dim, nv))
+        v[1] = X[:, 1]
+        eta = np.full(s["cells"], eta0)
+        Au = O.ablock_vmult(p, 1, eta, u.ravel())
+        uu, uv = u.ravel() @ Au, v.ravel() @ Au
+        assert abs(uu - (4.0 / 3.0 if comp else 2.0) * eta0 * vol) < 1e-12 * vol
+        assert abs(uv - (-2.0 / 3.0 if comp else 0.0) * eta0 * vol) < 1e-12 * vol
+        # the diagonal is the diagonal of the assembled (brute-force) operator
+        A, _ = assemble(p, 1, np.full((s["cells"], O.n_qp(p)), eta0))
+        con = O.constrained(p, 1)
+        Au_m = constrained_operator(A, con, s["nu"])[: s["nu"], : s["nu"]]
+        assert np.allclose(O.ablock_diagonal(p, 1, eta), Au_m.diagonal(), rtol=1e-13, atol=0)
+        x = np.random.default_rng(3).normal(size=s["nu"] + s["np"])
+        y = O.stokes_vmult(p, 1, eta, x)
+        ref = constrained_operator(A, con, s["nu"]) @ x
+        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_newton_update_solve_semantics():
+    # solve_newton_system: x0 ignored (zero guess), r0 = ||rhs||, no pressure
+    # normalization (local_smoothing.cc:950-969, 1028-1041, 1443-1444); the
+    # update equals the Picard solve of the same system from a zero guess up
+    # to the pressure constant
+    p = O.make_problem(2, 2, (1, 1), 4, (1, 1, 1), (1, 1, 1, 1, 0, 0), pressure_scaling=1.5)
+    s = O.sizes(p, 3)
+    rng = np.random.default_rng(9)
+    eta = np.exp(rng.normal(size=(s["cells"], O.n_qp(p))))
+    rhs = O.rhs_buoyancy(p, 1 + rng.uniform(size=(s["cells"], O.n_qp(p))), np.array([0.0, -1.0]))
+    x0 = rng.normal(size=rhs.size)
+    xn, rn = O.solve(p, eta, rhs, x0=x0, params=O.default_params(tolerance=1e-10, newton=True))
+    assert abs(rn["r0_norm"] - np.linalg.norm(rhs)) < 1e-14 * np.linalg.norm(rhs)
+    assert rn["initial_nonlinear_residual"] == rn["r0_norm"] and rn["pressure_normalization_adjustment"] == 0
+    xp, rp = O.solve(p, eta, rhs, params=O.default_params(tolerance=1e-10))
+    nu = s["nu"]
+    assert np.linalg.norm(xn[:nu] - xp[:nu]) < 1e-8 * np.linalg.norm(xp[:nu])
+    dp = xn[nu:] - xp[nu:]
+    assert np.std(dp) < 1e-8 * np.linalg.norm(xp[nu:]) / np.sqrt(len(dp))
