@@ -26,6 +26,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -266,9 +267,11 @@ struct GMG {
     }
   }
 
-  void applyA(int l, Mode mode, const T *x, const T *rhs, T *dst) {
+  void applyA(int l, Mode mode, const T *x, const T *rhs, T *dst) { applyA_work(l, mode, x, rhs, dst, split_for(l)); }
+  // the same apply with the split path's per-cell work array supplied by the caller
+  void applyA_work(int l, Mode mode, const T *x, const T *rhs, T *dst, T *work) {
     GMGLevel<T> &g = *lv[l];
-    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst, nullptr, st, split_for(l));
+    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst, nullptr, st, work);
   }
   void applyS(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];
@@ -1278,6 +1281,33 @@ struct mfs_handle {
 
 namespace {
 
+// Work array of the allocation-free A-block apply.  The name below is a
+// host-side stand-in for the device array: host code uses only the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch()).
+constexpr long long kNoAllocScratchBytes = MFS_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of scratch on the current device, resolved once per device
+// (at mfs_create, or at the first allocation-free call on a device).
+void *noalloc_scratch() {
+  int dev = 0;
+  MFS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 void create(const mfs_box *b, int prec, mfs_handle **out) {
   if (!b || !out) throw std::runtime_error("mfs_create: null argument");
   if (b->dim != 2 && b->dim != 3) throw std::runtime_error("mfs_create: dim must be 2 or 3");
@@ -1297,6 +1327,7 @@ void create(const mfs_box *b, int prec, mfs_handle **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     throw std::runtime_error("mfs_create: no HIP device available");
   upload_basis_tables();
+  (void)noalloc_scratch();
   auto h = std::make_unique<mfs_handle>();
   h->box = *b;
   for (int d = b->dim; d < 3; ++d) {
@@ -1633,6 +1664,28 @@ int mfs_ablock_vmult_device(mfs_handle *h, int level, const void *d_src, void *d
       h->g32.applyA(level, MODE_SET, (const float *)d_src, nullptr, (float *)d_dst);
     else
       h->g64.applyA(level, MODE_SET, (const double *)d_src, nullptr, (double *)d_dst);
+    MFS_HIP(hipGetLastError());
+    return MFS_OK;
+  });
+}
+
+int mfs_ablock_vmult_device_noalloc(mfs_handle *h, int level, const void *d_src, void *d_dst) {
+  return guard([&] {
+    need_eta(h);
+    check_level(h, level, false);
+    need_ptrs({d_src, d_dst}, "mfs_ablock_vmult_device_noalloc");
+    const LevelDesc &L = h->L[level];
+    const long long esz = h->prec == MFS_GMG_FP32 ? (long long)sizeof(float) : (long long)sizeof(double);
+    const long long need = (L.dim == 3 && L.k == 2) ? split_work_size(L) * esz : 0;
+    if (need > kNoAllocScratchBytes)
+      throw std::invalid_argument("mfs_ablock_vmult_device_noalloc: level needs " + std::to_string(need) +
+                                  " bytes of work array, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                  "-byte scratch array");
+    void *work = noalloc_scratch();
+    if (h->prec == MFS_GMG_FP32)
+      h->g32.applyA_work(level, MODE_SET, (const float *)d_src, nullptr, (float *)d_dst, (float *)work);
+    else
+      h->g64.applyA_work(level, MODE_SET, (const double *)d_src, nullptr, (double *)d_dst, (double *)work);
     MFS_HIP(hipGetLastError());
     return MFS_OK;
   });

@@ -37,7 +37,7 @@ EXPORTED = [
     "mfs_ablock_vmult_device", "mfs_solve_device", "mfs_assemble_buoyancy_rhs_device", "mfs_synchronize",
     "mfs_memory_consumption", "mfs_time_operator", "mfs_set_graph_capture", "mfs_correct_stokes_rhs",
     "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
-    "mfs_validate_operator_arguments",
+    "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc",
 ]
 
 
@@ -362,6 +362,11 @@ class StokesMatrixFreeHandler:
 
     def ablock_vmult_device(self, level: int, d_src: int, d_dst: int):
         _check(self.lib.mfs_ablock_vmult_device(self._h, level, C.c_void_p(d_src), C.c_void_p(d_dst)))
+
+    def ablock_vmult_device_noalloc(self, level: int, d_src: int, d_dst: int):
+        """The same apply without any device allocation: the work array is the
+        library's fixed per-GPU scratch array (calls on one GPU must not overlap)."""
+        _check(self.lib.mfs_ablock_vmult_device_noalloc(self._h, level, C.c_void_p(d_src), C.c_void_p(d_dst)))
 
     def assemble_buoyancy_rhs_device(self, d_rho: int, gravity, d_rhs: int):
         g = np.ascontiguousarray(np.resize(np.asarray(gravity, dtype=np.float64), 3))

@@ -1,0 +1,62 @@
+"""The allocation-free A-block apply (mfs_ablock_vmult_device_noalloc) against
+mfs_ablock_vmult_device through the C-ABI: bit-identical results (same
+kernels, only the work array differs), on two calls in a row with the second
+on a smaller level, in both GMG precisions, and the size limit of the scratch
+array reported as an error before any launch."""
+import numpy as np
+import pytest
+import torch
+
+from aspect_amd import StokesMatrixFreeHandler, make_box
+from aspect_amd.mfs import SolverError
+
+pytestmark = pytest.mark.gpu
+
+
+def _handler(prec, coarse, n_levels, seed):
+    box = make_box(3, 2, coarse, n_levels, (1.0, 1.0, 1.0), (1, 2, 0, 1, 1, 2), pressure_scaling=1.3)
+    h = StokesMatrixFreeHandler(box, prec)
+    s = h.sizes(h.top)
+    rng = np.random.default_rng(seed)
+    h.assemble(np.exp(rng.normal(size=(s["cells"], h.n_qp))), "harmonic")
+    return h, rng
+
+
+def _pair(h, rng, level, dtype):
+    n = h.sizes(level)["nu"]
+    src = torch.tensor(rng.normal(size=n), dtype=dtype, device="cuda:0")
+    ref = torch.full((n,), float("nan"), dtype=dtype, device="cuda:0")
+    out = torch.full((n,), float("nan"), dtype=dtype, device="cuda:0")
+    torch.cuda.synchronize()
+    h.ablock_vmult_device(level, src.data_ptr(), ref.data_ptr())
+    h.lib.mfs_synchronize(h._h)
+    h.ablock_vmult_device_noalloc(level, src.data_ptr(), out.data_ptr())
+    h.lib.mfs_synchronize(h._h)
+    return ref.cpu().numpy(), out.cpu().numpy()
+
+
+@pytest.mark.parametrize("prec,dtype", [("fp32", torch.float32), ("fp64", torch.float64)])
+def test_noalloc_matches_allocating_apply(prec, dtype):
+    # 24x16x8 cells on the finest level (partial tiles), then a smaller level
+    h, rng = _handler(prec, (3, 2, 1), 4, 7)
+    for level in (h.top, h.top - 2):
+        print(f"noalloc {prec} level {level}", flush=True)
+        ref, out = _pair(h, rng, level, dtype)
+        assert np.all(np.isfinite(ref)) and np.array_equal(ref, out), level
+
+
+def test_noalloc_size_limit():
+    # 80x64x64 cells = 5120 tiles: 5120 x 3 x 729 FP64 values exceed the
+    # 72 MiB array (rejected before any launch); the FP32 work array fits
+    h64, rng = _handler("fp64", (5, 4, 4), 5, 3)
+    n = h64.sizes(h64.top)["nu"]
+    src = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    with pytest.raises(SolverError, match="scratch array"):
+        h64.ablock_vmult_device_noalloc(h64.top, src.data_ptr(), dst.data_ptr())
+    del h64, src, dst
+    h32, rng = _handler("fp32", (5, 4, 4), 5, 3)
+    print("noalloc fp32 80x64x64", flush=True)
+    ref, out = _pair(h32, rng, h32.top, torch.float32)
+    assert np.array_equal(ref, out)
