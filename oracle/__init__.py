@@ -25,10 +25,10 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 def build(force: bool = False) -> str:
     """Compile the oracle with g++ (OpenMP).  Returns the library path."""
-    src = os.path.join(_HERE, "oracle.cpp")
-    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(src) > os.path.getmtime(_LIB_PATH):
+    srcs = [os.path.join(_HERE, f) for f in ("oracle.cpp", "cpu_port.cpp")]
+    if force or not os.path.exists(_LIB_PATH) or max(map(os.path.getmtime, srcs)) > os.path.getmtime(_LIB_PATH):
         cmd = ["g++", "-O3", "-march=x86-64-v2", "-fopenmp", "-shared", "-fPIC", "-std=c++17",
-               "-o", _LIB_PATH + ".tmp", src]
+               "-o", _LIB_PATH + ".tmp", *srcs]
         subprocess.run(cmd, check=True)
         os.replace(_LIB_PATH + ".tmp", _LIB_PATH)
     return _LIB_PATH
@@ -86,6 +86,8 @@ def lib():
         build()
         _lib = C.CDLL(_LIB_PATH)
         _lib.orc_last_error.restype = C.c_char_p
+        _lib.orc_port_last_error.restype = C.c_char_p
+        _lib.orc_port_isa.restype = C.c_char_p
     return _lib
 
 
@@ -290,6 +292,14 @@ def vcycle(p, eta, kind, src):
     return dst
 
 
+def time_vcycle(p, eta, kind, reps=1):
+    """(set-up seconds, seconds per V-cycle) of the oracle's multigrid on the CPU."""
+    eta, per_qp = _eta(eta)
+    out = np.zeros(2)
+    _check(lib().orc_time_vcycle(C.byref(p), _p(eta), per_qp, kind, int(reps), _p(out)))
+    return float(out[0]), float(out[1])
+
+
 def chebyshev(p, eta, kind, level, src):
     eta, per_qp = _eta(eta)
     src = np.ascontiguousarray(src, dtype=np.float64)
@@ -334,3 +344,35 @@ def solve(p, eta, rhs, x0=None, params: SolveParams | None = None):
     _check(lib().orc_solve(C.byref(p), _p(eta), per_qp, C.byref(params), _p(rhs), _p(x0), _p(sol),
                            C.byref(res)))
     return sol, {f: getattr(res, f) for f, _ in SolveResult._fields_}
+
+
+class CpuPort:
+    """The SIMD-across-cells CPU Stokes apply (``cpu_port.cpp``) for the finest
+    level of a 3D degree-2 box problem: the CPU baseline's timed operator."""
+
+    def __init__(self, p: Problem):
+        self._plan = C.c_void_p()
+        if lib().orc_port_create(C.byref(p), C.byref(self._plan)) != 0:
+            raise RuntimeError(lib().orc_port_last_error().decode())
+
+    def stokes_vmult(self, eta, src, out=None, threads=0):
+        eta, per_qp = _eta(eta)
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.empty_like(src) if out is None else out
+        lib().orc_port_stokes_vmult(self._plan, _p(eta), per_qp, _p(src), _p(dst), int(threads))
+        return dst
+
+    def __del__(self):
+        if getattr(self, "_plan", None):
+            lib().orc_port_destroy(self._plan)
+            self._plan = None
+
+
+def simd_width_name() -> str:
+    """Instruction set the CPU port's batch kernel runs with on this host."""
+    return lib().orc_port_isa().decode()
+
+
+def set_threads(n: int) -> None:
+    """OpenMP thread count of the oracle's parallel loops."""
+    lib().orc_set_threads(int(n))

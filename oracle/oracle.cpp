@@ -26,6 +26,7 @@
 // component-major, nodes lexicographic with x fastest; pressure block follows.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1683,6 +1684,26 @@ int orc_vcycle(const orc_problem *p, const double *eta, int per_qp, int kind, co
     Vec b(src, src + mg.size(top)), x;
     mg.vmult(x, b);
     std::memcpy(dst, x.data(), x.size() * sizeof(double));
+  })
+}
+// CPU timing of the multigrid preconditioner: set-up (diagonals, eigenvalue
+// estimates) once, then reps V-cycles of kind 0 (A block) or 1 (Schur mass);
+// seconds[0] = set-up, seconds[1] = mean V-cycle time
+int orc_time_vcycle(const orc_problem *p, const double *eta, int per_qp, int kind, int reps, double *seconds) {
+  ORC_TRY({
+    Context C(*p);
+    const auto t0 = std::chrono::steady_clock::now();
+    Solver S(C, eta, per_qp);
+    S.build_preconditioner();
+    const auto t1 = std::chrono::steady_clock::now();
+    const MG &mg = kind == 0 ? S.mgA : S.mgS;
+    const int top = p->n_levels - 1;
+    Vec b(mg.size(top)), x;
+    for (size_t i = 0; i < b.size(); ++i) b[i] = std::sin(0.37 * (double)i);
+    for (int r = 0; r < std::max(1, reps); ++r) mg.vmult(x, b);
+    const auto t2 = std::chrono::steady_clock::now();
+    seconds[0] = std::chrono::duration<double>(t1 - t0).count();
+    seconds[1] = std::chrono::duration<double>(t2 - t1).count() / std::max(1, reps);
   })
 }
 // single Chebyshev smoother application (x0 = 0) on a level

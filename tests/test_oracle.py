@@ -487,3 +487,33 @@ def test_newton_update_solve_semantics():
     assert np.linalg.norm(xn[:nu] - xp[:nu]) < 1e-8 * np.linalg.norm(xp[:nu])
     dp = xn[nu:] - xp[nu:]
     assert np.std(dp) < 1e-8 * np.linalg.norm(xp[nu:]) / np.sqrt(len(dp))
+
+
+@pytest.mark.parametrize("coarse,nl,ext,bc,comp", [
+    ((3, 2, 1), 3, (1.5, 1.0, 0.5), (1, 2, 0, 1, 1, 2), 0),   # partial 8-cell batches, mixed walls
+    ((1, 1, 1), 3, (1.0, 1.0, 1.0), (2,) * 6, 1),             # compressible, no slip
+    ((5, 3, 2), 2, (1.0, 2.0, 3.0), (1,) * 6, 0),             # 10 cells in x: 8 + 2 lanes
+])
+def test_cpu_port_matches_oracle_apply(coarse, nl, ext, bc, comp):
+    # the CPU baseline's SIMD-across-cells apply (oracle/cpu_port.cpp) against
+    # the oracle's scalar sum-factorized apply (itself pinned by the
+    # brute-force element matrices above), per-qp and per-cell viscosity, 1 and
+    # 3 threads
+    p = O.make_problem(3, 2, coarse, nl, ext, bc, compressible=comp, pressure_scaling=1.7)
+    s = O.sizes(p, nl - 1)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=s["nu"] + s["np"])
+    port = O.CpuPort(p)
+    for eta in (np.exp(rng.normal(size=(s["cells"], 27))), np.exp(rng.normal(size=s["cells"]))):
+        ref = O.stokes_vmult(p, nl - 1, eta, x)
+        for threads in (1, 3):
+            y = port.stokes_vmult(eta, x, threads=threads)
+            assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_oracle_vcycle_timing_entry():
+    # the CPU V-cycle timing runs the same V-cycle as orc_vcycle
+    p = O.make_problem(3, 2, (1, 1, 1), 3, (1.0, 1.0, 1.0), (1,) * 6)
+    s = O.sizes(p, 2)
+    setup, per = O.time_vcycle(p, np.ones(s["cells"]), 0, reps=2)
+    assert setup > 0 and per > 0
