@@ -231,9 +231,6 @@ int tile_threads(int dim, int k) {
 // quadrature-point values with S (Q x N) along every direction, and
 // derivatives are taken on the Gauss points with D (Q x Q), D S = G.  Only the y
 // direction crosses threads (LDS); x and z stay in registers.
-#ifndef MFS_CELL_LINES
-#define MFS_CELL_LINES 1  // 3D Q2 Stokes / A block through the line kernel
-#endif
 #ifndef MFS_MINW_F32
 #define MFS_MINW_F32 1
 #endif
@@ -1163,10 +1160,8 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                     T *split_work) {
   check_op_args(op, mode, src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p);
-#if MFS_CELL_LINES
-  if (launch_cell_lines<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st, split_work))
+  if (launch_cell_q2<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st, split_work))
     return;
-#endif
   if (op == OP_MASS && launch_mass_nodes<T>(mode, L, src_p, eta, eta_per_qp, rhs_p, dst_p, st)) return;
   if (op == OP_BT && launch_bt_nodes<T, T>(mode, L, src_p, rhs_u, dst_u, st)) return;
   if (L.dim == 3 && L.k == 2)

@@ -132,4 +132,100 @@ __device__ __forceinline__ void tile_write_u(const LevelDesc &L, const T *xb, co
   }
 }
 
+// ------------------------------------------------------------------ 3D degree-2 tiles (4x4x4 cells, 9^3 nodes)
+// constrained velocity DoF test without branches (faces: 0 open, 1 no normal
+// flux, 2 zero velocity)
+__device__ __forceinline__ bool con_bf(const LevelDesc &L, int comp, int gx, int gy, int gz) {
+  const int g[3] = {gx, gy, gz};
+  bool con = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int b = g[d] == 0 ? L.bc[2 * d] : (g[d] == L.nv[d] - 1 ? L.bc[2 * d + 1] : 0);
+    con = con || b == 2 || (b == 1 && comp == d);
+  }
+  return con;
+}
+
+// Workgroup barrier for LDS exchanges only: the fences order LDS accesses
+// (wait for this wave's LDS operations), not global ones, so global loads
+// issued ahead of the barrier (prefetches for later steps) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Node of a 4x4x4 degree-2 tile: component and local indices (9 per direction).
+struct TileNode {
+  int comp, lx, ly, lz;
+  bool valid;
+};
+
+// Contributing cells of tile node l along one direction (local cell stride
+// sc, local-node stride si in the per-cell contribution layout): candidate 0
+// is (l/2 - 1, local node 2), present for even l > 0; candidate 1 is
+// (l/2, local node l mod 2), present while l/2 < nc.
+__device__ __forceinline__ void node_cands(int l, int sc, int si, int nc, int off[2], bool ok[2]) {
+  off[0] = 2 * si + ((l >> 1) - 1) * sc;
+  ok[0] = (l & 1) == 0 && l > 0;
+  off[1] = (l & 1) * si + (l >> 1) * sc;
+  ok[1] = (l >> 1) < nc;
+}
+
+// Output node of the write-out sweep: offset from the tile origin, whether
+// it lies in the level, whether this tile is the first (lowest colour) to
+// write it, whether it is a constrained row.  INTERIOR: the tile holds no
+// node of the domain boundary (no constrained node, no partial cell range).
+struct OutNode {
+  long long gi;
+  bool in, first, con;
+};
+template <bool INTERIOR>
+__device__ __forceinline__ OutNode out_node(const LevelDesc &L, const TileNode &n, int tx, int ty, int tz, int px,
+                                            int py, int pz) {
+  constexpr int VX = 9;
+  OutNode o;
+  const int gx = 8 * tx + n.lx, gy = 8 * ty + n.ly, gz = 8 * tz + n.lz;
+  o.in = INTERIOR || (gx < L.nv[0] && gy < L.nv[1] && gz < L.nv[2]);
+  const bool shx = INTERIOR ? (n.lx == 0 || n.lx == VX - 1)
+                            : ((n.lx == 0 && tx > 0) || (n.lx == VX - 1 && tx + 1 < L.ntiles[0]));
+  const bool shy = INTERIOR ? (n.ly == 0 || n.ly == VX - 1)
+                            : ((n.ly == 0 && ty > 0) || (n.ly == VX - 1 && ty + 1 < L.ntiles[1]));
+  const bool shz = INTERIOR ? (n.lz == 0 || n.lz == VX - 1)
+                            : ((n.lz == 0 && tz > 0) || (n.lz == VX - 1 && tz + 1 < L.ntiles[2]));
+  o.first = !((shx && px) || (shy && py) || (shz && pz));
+  const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];  // node strides (a tile spans < 2^31 entries)
+  o.gi = (n.comp == 0 ? 0 : (n.comp == 1 ? L.vnodes : 2 * L.vnodes)) + (n.lx + sy * n.ly + sz * n.lz);
+  o.con = INTERIOR ? false : (o.in && con_bf(L, n.comp, gx, gy, gz));
+  return o;
+}
+
+// Sum over the at most 8 cells of a 4x4x4 degree-2 tile that contribute to
+// tile node n, in a fixed order.  The 8 candidate reads are unconditional: a
+// missing candidate (or a thread past the last node) reads xb[ZS], a slot
+// that holds zero, so all lanes run the same instruction stream.  xb:
+// per-cell contributions [comp][kz][jy][i][cell] followed by the zero slot.
+template <typename T, bool INTERIOR>
+__device__ __forceinline__ T tile_node_sum(const LevelDesc &L, const T *xb, const TileNode &n, const int tx,
+                                           const int ty, const int tz) {
+  constexpr int CT = 64, ZS = 81 * CT;
+  const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
+            ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
+  int ox[2], oy[2], oz[2];
+  bool wx[2], wy[2], wz[2];
+  node_cands(n.lx, 1, CT, ncx, ox, wx);
+  node_cands(n.ly, 4, 3 * CT, ncy, oy, wy);
+  node_cands(n.lz, 16, 9 * CT, ncz, oz, wz);
+  const int cb = n.comp * 27 * CT;
+  T val = 0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) val += xb[n.valid && wz[a] && wy[b] && wx[e] ? cb + oz[a] + oy[b] + ox[e] : ZS];
+  return val;
+}
+
+
 }  // namespace mfs
