@@ -45,16 +45,21 @@ namespace mfs {
 // density for OP_RHS); RESID reads the right-hand side of what it writes.
 void check_op_args(Op op, Mode mode, const void *src_u, const void *src_p, const void *eta, const void *rhs_u,
                    const void *rhs_p, const void *dst_u, const void *dst_p) {
-  if (op < OP_STOKES || op > OP_RHS) throw std::invalid_argument("unknown operator code");
+  if (op < OP_STOKES || op > OP_LDIAG) throw std::invalid_argument("unknown operator code");
   if (mode < MODE_SET || mode > MODE_ADD) throw std::invalid_argument("unknown operator mode");
-  const bool rd_u = op == OP_STOKES || op == OP_ABLOCK, rd_p = op == OP_STOKES || op == OP_BT || op == OP_MASS;
-  const bool wr_u = op == OP_STOKES || op == OP_ABLOCK || op == OP_BT || op == OP_ADIAG || op == OP_RHS;
+  const bool rd_u = op == OP_STOKES || op == OP_ABLOCK || op == OP_LAPLACE,
+             rd_p = op == OP_STOKES || op == OP_BT || op == OP_MASS;
+  const bool wr_u = op == OP_STOKES || op == OP_ABLOCK || op == OP_BT || op == OP_ADIAG || op == OP_RHS ||
+                    op == OP_LAPLACE || op == OP_LDIAG;
   const bool wr_p = op == OP_STOKES || op == OP_MASS || op == OP_MDIAG;
-  const bool resid = mode == MODE_RESID && op != OP_ADIAG && op != OP_MDIAG && op != OP_RHS;
+  const bool diag = op == OP_ADIAG || op == OP_MDIAG || op == OP_RHS || op == OP_LDIAG;
+  const bool resid = mode == MODE_RESID && !diag;
+  const bool coef = op != OP_BT && op != OP_LAPLACE && op != OP_LDIAG;  // operators with a cell table
   const char *names[7] = {"src_u", "src_p", "eta", "rhs_u", "rhs_p", "dst_u", "dst_p"};
   const void *ptrs[7] = {src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p};
-  const bool need[7] = {rd_u, rd_p, op != OP_BT, resid && wr_u, resid && wr_p, wr_u, wr_p};
-  static const char *opn[7] = {"Stokes", "A block", "B^T", "mass", "A diagonal", "mass diagonal", "buoyancy rhs"};
+  const bool need[7] = {rd_u, rd_p, coef, resid && wr_u, resid && wr_p, wr_u, wr_p};
+  static const char *opn[9] = {"Stokes", "A block", "B^T", "mass", "A diagonal", "mass diagonal", "buoyancy rhs",
+                               "Laplace", "Laplace diagonal"};
   for (int i = 0; i < 7; ++i)
     if (need[i] && !ptrs[i])
       throw std::invalid_argument(std::string(opn[op]) + " operator: null " + names[i] + " pointer");
@@ -176,6 +181,10 @@ struct GMG {
   DBuf<T> swork;
   T *split_for(int) { return swork.p; }
   bool ietan_ok = false;  // GMGLevel::ietan matches the current level viscosity
+  // vector Laplace hierarchy of the mesh-deformation solve (kind 0 only): the
+  // level operator is the Laplace operator, its diagonal the Laplace diagonal,
+  // and the smoothers follow interface.cc:1263-1290
+  bool laplace = false;
   // Where the level tables came from (the FP64 copy below is rebuilt from them)
   const double *eta_src = nullptr;
   int eta_src_pq = 0;
@@ -226,7 +235,7 @@ struct GMG {
     long long sw = 0;
     for (auto &g : lv)
       if (g->L.dim == 3 && g->L.k == 2) sw = std::max(sw, split_work_size(g->L));
-    if (sw > 0) swork.alloc(sw);
+    if (sw > 0 && !laplace) swork.alloc(sw);
   }
 
   int eta_pq = 0;  // level tables per quadrature point (DGQ1) or per cell (DGQ0)
@@ -271,7 +280,8 @@ struct GMG {
   // the same apply with the split path's per-cell work array supplied by the caller
   void applyA_work(int l, Mode mode, const T *x, const T *rhs, T *dst, T *work) {
     GMGLevel<T> &g = *lv[l];
-    launch_cell_op<T>(OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst, nullptr, st, work);
+    launch_cell_op<T>(laplace ? OP_LAPLACE : OP_ABLOCK, mode, g.L, x, nullptr, g.eta.p, eta_pq, rhs, nullptr, dst,
+                      nullptr, st, laplace ? nullptr : work);
   }
   void applyS(int l, Mode mode, const T *x, const T *rhs, T *dst) {
     GMGLevel<T> &g = *lv[l];
@@ -375,6 +385,10 @@ struct GMG {
   // diagonals (build_preconditioner) and Chebyshev set-up (solve() :771-849)
   void build(const mfs_box &b) {
     drop_graphs();
+    if (laplace) {
+      build_laplace();
+      return;
+    }
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
       if (!eta_pq && g.L.dim == 3 && g.L.k == 2) {
@@ -400,6 +414,37 @@ struct GMG {
       g.chA.degree = g.chS.degree = coarse ? 8 : 4;
       estimate(0, l, g.chA, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
       estimate(1, l, g.chS, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
+    }
+  }
+
+  // Mesh-deformation smoothers (interface.cc:1263-1290): Laplace diagonals;
+  // Chebyshev degree 5 over [1.2 lambda_max / 15, 1.2 lambda_max] from 10
+  // eigenvalue iterations on levels > 0; on level 0 Chebyshev as a solver:
+  // 50 iterations, range 1e-3, the degree from the Chebyshev error bound
+  // (deal.II's degree = invalid_unsigned_int): with r = lambda_max / alpha,
+  // sigma = (1 - r^-1/2) / (1 + r^-1/2), degree = 1 + floor(log(1/eps +
+  // sqrt(1/eps^2 - 1)) / log(1/sigma)), eps = 1e-3
+  void build_laplace() {
+    for (int l = 0; l <= top; ++l) {
+      GMGLevel<T> &g = *lv[l];
+      launch_cell_op<T>(OP_LDIAG, MODE_SET, g.L, nullptr, nullptr, nullptr, 0, nullptr, nullptr, g.dinvA.p, nullptr,
+                        st);
+      v_invert<T>(g.dinvA.p, g.nu, st);
+    }
+    for (int l = 0; l <= top; ++l) {
+      GMGLevel<T> &g = *lv[l];
+      if (l > 0) {
+        g.chA.degree = 5;
+        estimate(0, l, g.chA, 10, 15.0);
+      } else {
+        estimate(0, 0, g.chA, 50, 1e-3);
+        const double alpha = g.chA.theta - g.chA.delta;
+        const double r = g.chA.lmax / alpha;
+        const double sigma = (1.0 - std::sqrt(1.0 / r)) / (1.0 + std::sqrt(1.0 / r));
+        const double eps = 1e-3;
+        g.chA.degree =
+            sigma > 0 ? 1 + (int)(std::log(1.0 / eps + std::sqrt(1.0 / eps / eps - 1.0)) / std::log(1.0 / sigma)) : 1;
+      }
     }
   }
 
@@ -455,7 +500,8 @@ struct GMG {
     }
     // residual + update in one pass where the A-block apply is split
     auto fused = [&](T f1, T f2, int first) {
-      return kind == 0 && launch_ablock_cheb<T>(g.L, x, b, g.eta.p, eta_pq, dinv, d, f1, f2, first, split_for(l), st);
+      return kind == 0 && !laplace &&
+             launch_ablock_cheb<T>(g.L, x, b, g.eta.p, eta_pq, dinv, d, f1, f2, first, split_for(l), st);
     };
     for (int s = 0; s < steps; ++s) {
       const bool zero = x_zero && s == 0;
@@ -599,6 +645,13 @@ struct mfs_handle {
   DBuf<double> w1, w2, w3, w4, w5, pnorm_w;
   std::vector<double *> vchunks, zchunks;
   int a_apps = 0, s_apps = 0;
+  // free-surface stabilization of the active-level Stokes operator
+  int fs_faces[6] = {0, 0, 0, 0, 0, 0};
+  bool fs_on = false;
+  DBuf<double> fs_stab, fs_sv;
+  // mesh-deformation hierarchy (built for the mesh constraints of the last call)
+  std::unique_ptr<GMG<double>> gmesh;
+  int gmesh_bc[6] = {-1, -1, -1, -1, -1, -1};
 
   ~mfs_handle() {
     for (double *p : vchunks) (void)hipFree(p);
@@ -631,6 +684,7 @@ struct mfs_handle {
   void stokes(Mode mode, const double *x, const double *rhs, double *dst) {
     launch_cell_op<double>(OP_STOKES, mode, Ltop(), x, x + nu, eta_active.p, active_per_qp, rhs,
                            rhs ? rhs + nu : nullptr, dst, dst + nu, st);
+    if (fs_on) launch_free_surface(Ltop(), fs_faces, fs_stab.p, x, fs_sv.p, mode == MODE_RESID ? -1.0 : 1.0, dst, st);
   }
   void ablock_active(Mode mode, const double *x, const double *rhs, double *dst) {
     launch_cell_op<double>(OP_ABLOCK, mode, Ltop(), x, nullptr, eta_active.p, active_per_qp, rhs, nullptr, dst,
@@ -1594,6 +1648,116 @@ int mfs_set_pressure_scaling(mfs_handle *h, double s) {
   });
 }
 
+void set_free_surface(mfs_handle *h, const int *faces, const double *stab) {
+  need_handle(h);
+  if (!faces) throw std::invalid_argument("mfs_set_free_surface: null faces");
+  const LevelDesc &L = h->Ltop();
+  const int nfq = L.dim == 3 ? (L.k + 1) * (L.k + 1) : L.k + 1;
+  long long tot = 0, mx = 0;
+  int f6[6] = {0, 0, 0, 0, 0, 0};
+  for (int f = 0; f < 6; ++f) {
+    if (faces[f] != 0 && faces[f] != 1) throw std::invalid_argument("mfs_set_free_surface: face flags must be 0 or 1");
+    if (f >= 2 * L.dim && faces[f]) throw std::invalid_argument("mfs_set_free_surface: face beyond the dimension");
+    f6[f] = faces[f];
+    if (f6[f]) {
+      const long long c = free_surface_cells(L, f) * nfq;
+      tot += c;
+      mx = std::max(mx, c);
+    }
+  }
+  h->fs_on = false;
+  std::memcpy(h->fs_faces, f6, sizeof(f6));
+  if (tot == 0) return;
+  if (!stab) throw std::invalid_argument("mfs_set_free_surface: null stabilization table");
+  h->fs_stab.alloc(tot * L.dim);
+  h->fs_sv.alloc(mx);
+  MFS_HIP(hipMemcpyAsync(h->fs_stab.p, stab, tot * L.dim * sizeof(double), hipMemcpyHostToDevice, h->st));
+  MFS_HIP(hipStreamSynchronize(h->st));
+  h->fs_on = true;
+}
+
+// MeshDeformationHandler::compute_mesh_displacements_gmg_for_degree
+// (source/mesh_deformation/interface.cc:1076-1360) on the handle's levels:
+// rhs = -A u0c (u0 read as stored, constrained rows zero) with u0c the
+// constrained part of u0; CG with the Laplace V-cycle to tolerance * ||rhs||
+// within 5 * N iterations; solution = x + u0c.
+void mesh_setup(mfs_handle *h, const int *bc) {
+  need_handle(h);
+  if (!bc) throw std::invalid_argument("mesh deformation: null boundary types");
+  mfs_box mb = h->box;
+  for (int f = 0; f < 6; ++f) {
+    if (bc[f] < 0 || bc[f] > 3) throw std::invalid_argument("mesh deformation: boundary type must be 0..3");
+    mb.bc[f] = f < 2 * mb.dim ? (bc[f] == 3 ? 2 : bc[f]) : 0;
+  }
+  mb.compressible = 0;
+  if (h->gmesh && std::memcmp(h->gmesh_bc, mb.bc, sizeof(mb.bc)) == 0) return;
+  auto g = std::make_unique<GMG<double>>();
+  g->laplace = true;
+  g->setup(mb, h->st, &h->dot_work, &h->dot_out);
+  g->build(mb);
+  MFS_HIP(hipStreamSynchronize(h->st));
+  std::memcpy(h->gmesh_bc, mb.bc, sizeof(mb.bc));
+  h->gmesh = std::move(g);
+}
+
+int mesh_solve_dev(mfs_handle *h, const int *bc, const double *d_u0, double tolerance, double *d_sol, int *its,
+                   double *res) {
+  mesh_setup(h, bc);
+  GMG<double> &g = *h->gmesh;
+  const int top = g.top;
+  const LevelDesc &L = g.lv[top]->L;
+  const long long n = g.lv[top]->nu;
+  hipStream_t st = h->st;
+  DBuf<double> u0c, rhs, x, r, z, pv, q;
+  for (DBuf<double> *v : {&u0c, &rhs, &x, &r, &z, &pv, &q}) v->alloc(n);
+  // constrained part of u0: u0 - (u0 with constrained entries zeroed)
+  MFS_HIP(hipMemcpyAsync(z.p, d_u0, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  v_zero_constrained<double>(L, z.p, st);
+  MFS_HIP(hipMemcpyAsync(u0c.p, d_u0, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  v_axpby<double>(-1.0, z.p, 1.0, u0c.p, n, st);
+  LevelDesc Lp = L;
+  Lp.plain_read = 1;
+  launch_cell_op<double>(OP_LAPLACE, MODE_SET, Lp, u0c.p, nullptr, nullptr, 0, nullptr, nullptr, rhs.p, nullptr, st);
+  v_zero_constrained<double>(L, rhs.p, st);
+  v_scale(rhs.p, -1.0, n, st);
+  const double rn = std::sqrt(g.dot(rhs.p, rhs.p, n));
+  const double tol = tolerance * rn;
+  v_fill<double>(x.p, 0.0, n, st);
+  MFS_HIP(hipMemcpyAsync(r.p, rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  double rr = rn;
+  int k = 0;
+  const long long max_its = std::min<long long>(5 * n, 1000000);
+  if (rr > tol) {
+    g.vmult(0, r.p, z.p);
+    MFS_HIP(hipMemcpyAsync(pv.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double rz = g.dot(r.p, z.p, n);
+    while (k < max_its) {
+      g.applyA(top, MODE_SET, pv.p, nullptr, q.p);
+      const double alpha = rz / g.dot(pv.p, q.p, n);
+      v_axpby<double>(alpha, pv.p, 1.0, x.p, n, st);
+      v_axpby<double>(-alpha, q.p, 1.0, r.p, n, st);
+      ++k;
+      rr = std::sqrt(g.dot(r.p, r.p, n));
+      if (rr <= tol) break;
+      g.vmult(0, r.p, z.p);
+      const double rz2 = g.dot(r.p, z.p, n);
+      const double beta = rz2 / rz;
+      rz = rz2;
+      v_axpby<double>(1.0, z.p, beta, pv.p, n, st);
+    }
+  }
+  // solution = x + u0c (x is zero on the constrained DoFs)
+  v_axpby<double>(1.0, x.p, 1.0, u0c.p, n, st);
+  MFS_HIP(hipMemcpyAsync(d_sol, u0c.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  MFS_HIP(hipStreamSynchronize(st));
+  if (its) *its = k;
+  if (res) {
+    res[0] = rr;
+    res[1] = tol;
+  }
+  return rr <= tol ? MFS_OK : MFS_NO_CONVERGENCE;
+}
+
 void build_prec(mfs_handle *h) {
   need_eta(h);
   if (h->prec == MFS_GMG_FP32)
@@ -1602,6 +1766,77 @@ void build_prec(mfs_handle *h) {
     h->g64.build(h->box);
   MFS_HIP(hipStreamSynchronize(h->st));
   h->have_prec = true;
+}
+
+int mfs_set_free_surface(mfs_handle *h, const int faces[6], const double *stab) {
+  return guard([&] {
+    set_free_surface(h, faces, stab);
+    return MFS_OK;
+  });
+}
+
+int mfs_mesh_deformation_solve(mfs_handle *h, const int bc[6], const double *u0, double tolerance, double *sol,
+                               int *iterations, double *residuals) {
+  return guard([&] {
+    need_handle(h);
+    need_ptrs({u0, sol}, "mfs_mesh_deformation_solve");
+    const long long n = h->nu;
+    DBuf<double> du0, dsol;
+    du0.alloc(n);
+    dsol.alloc(n);
+    MFS_HIP(hipMemcpyAsync(du0.p, u0, n * sizeof(double), hipMemcpyHostToDevice, h->st));
+    const int rc = mesh_solve_dev(h, bc, du0.p, tolerance, dsol.p, iterations, residuals);
+    MFS_HIP(hipMemcpyAsync(sol, dsol.p, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    MFS_HIP(hipStreamSynchronize(h->st));
+    if (rc != MFS_OK) g_err = "mesh deformation: CG did not reach the tolerance";
+    return rc;
+  });
+}
+
+int mfs_mesh_laplace_vmult(mfs_handle *h, const int bc[6], int level, const double *src, double *dst) {
+  return guard([&] {
+    mesh_setup(h, bc);
+    check_level(h, level, false);
+    need_ptrs({src, dst}, "mfs_mesh_laplace_vmult");
+    GMG<double> &g = *h->gmesh;
+    const long long n = g.lv[level]->nu;
+    DBuf<double> a, b;
+    a.alloc(n);
+    b.alloc(n);
+    MFS_HIP(hipMemcpyAsync(a.p, src, n * sizeof(double), hipMemcpyHostToDevice, h->st));
+    g.applyA(level, MODE_SET, a.p, nullptr, b.p);
+    MFS_HIP(hipMemcpyAsync(dst, b.p, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    MFS_HIP(hipStreamSynchronize(h->st));
+    return MFS_OK;
+  });
+}
+
+int mfs_mesh_vcycle(mfs_handle *h, const int bc[6], const double *src, double *dst, double *smoother_info) {
+  return guard([&] {
+    mesh_setup(h, bc);
+    need_ptrs({src, dst}, "mfs_mesh_vcycle");
+    GMG<double> &g = *h->gmesh;
+    const long long n = g.lv[g.top]->nu;
+    DBuf<double> a, b;
+    a.alloc(n);
+    b.alloc(n);
+    MFS_HIP(hipMemcpyAsync(a.p, src, n * sizeof(double), hipMemcpyHostToDevice, h->st));
+    g.vmult(0, a.p, b.p);
+    MFS_HIP(hipMemcpyAsync(dst, b.p, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    MFS_HIP(hipStreamSynchronize(h->st));
+    if (smoother_info)
+      for (int l = 0; l <= g.top; ++l) {
+        const ChebInfo &c = g.lv[l]->chA;
+        double *o = smoother_info + 6 * l;
+        o[0] = c.lmin;
+        o[1] = c.lmax;
+        o[2] = c.theta;
+        o[3] = c.delta;
+        o[4] = c.degree;
+        o[5] = c.its;
+      }
+    return MFS_OK;
+  });
 }
 
 int mfs_build_preconditioner(mfs_handle *h) {
@@ -1959,6 +2194,8 @@ void correct_rhs_dev(mfs_handle *h, const double *d_u0, double *d_rhs) {
   Lp.plain_read = 1;
   launch_cell_op<double>(OP_STOKES, MODE_SET, Lp, d_u0, d_u0 + h->nu, h->eta_active.p, h->active_per_qp, nullptr,
                          nullptr, h->w1.p, h->w1.p + h->nu, h->st);
+  // free-surface term with plain reads of u0 (local_smoothing.cc:709-744)
+  if (h->fs_on) launch_free_surface(Lp, h->fs_faces, h->fs_stab.p, d_u0, h->fs_sv.p, 1.0, h->w1.p, h->st);
   v_zero_constrained<double>(h->Ltop(), h->w1.p, h->st);
   v_axpby<double>(-1.0, h->w1.p, 1.0, d_rhs, h->n, h->st);
   MFS_HIP(hipStreamSynchronize(h->st));

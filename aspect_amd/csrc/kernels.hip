@@ -248,11 +248,13 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
            const T *__restrict__ rhs_p, T *__restrict__ dst_u, T *__restrict__ dst_p) {
   using C = Cfg<DIM, K>;
   constexpr int N = C::N, Q = C::Q, NP = C::NP, ZN = C::ZN, ZQ = C::ZQ, ZNP = C::ZNP, CT = C::CT;
-  constexpr bool NEED_U = (OP == OP_STOKES || OP == OP_ABLOCK);
+  constexpr bool NEED_U = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_LAPLACE);
   constexpr bool NEED_P = (OP == OP_STOKES || OP == OP_BT || OP == OP_MASS);
-  constexpr bool OUT_U = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT || OP == OP_ADIAG || OP == OP_RHS);
+  constexpr bool OUT_U = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT || OP == OP_ADIAG || OP == OP_RHS ||
+                          OP == OP_LAPLACE || OP == OP_LDIAG);
   constexpr bool OUT_P = (OP == OP_STOKES || OP == OP_MASS || OP == OP_MDIAG);
-  constexpr bool GRAD_TEST = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT);
+  constexpr bool GRAD_TEST = (OP == OP_STOKES || OP == OP_ABLOCK || OP == OP_BT || OP == OP_LAPLACE);
+  constexpr bool SQDIAG = (OP == OP_ADIAG || OP == OP_LDIAG);  // closed-form velocity diagonals
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
   T *nb = sm;                // [DIM][VZ][VY][VX] tile velocity nodes (load phase)
@@ -407,7 +409,8 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
       const double wq = c_basis[K - 2].w[qx] * c_basis[K - 2].w[j] * (DIM == 3 ? c_basis[K - 2].w[qz] : 1.0);
       const T JxW = (T)(jac * wq);
       T et = 1;
-      if constexpr (OP != OP_BT) et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
+      if constexpr (OP != OP_BT && OP != OP_LAPLACE && OP != OP_LDIAG)
+        et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
       T pqv = 0;
       if constexpr (NEED_P) {
 #pragma unroll
@@ -455,6 +458,11 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
               flux[a][bb] = sig * JxW;
             }
           if constexpr (OP == OP_STOKES) fp = -s * div * JxW;
+        } else if constexpr (OP == OP_LAPLACE) {  // grad u : grad v, component by component
+#pragma unroll
+          for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int bb = 0; bb < DIM; ++bb) flux[a][bb] = gr[a][bb] * JxW;
         } else {  // B^T: -s p I
 #pragma unroll
           for (int a = 0; a < DIM; ++a)
@@ -475,8 +483,8 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
         fp = s * s / et * pqv * JxW;
       } else if constexpr (OP == OP_MDIAG) {
         fp = s * s / et * JxW;
-      } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
-        fz_or_val[qx] = et * JxW;  // rho JxW for the right-hand side, eta JxW for the diagonal
+      } else if constexpr (OP == OP_RHS || SQDIAG) {
+        fz_or_val[qx] = et * JxW;  // rho JxW for the right-hand side, eta JxW (or JxW) for the diagonals
       }
       if constexpr (OUT_P) {
         constexpr bool SQ = (OP == OP_MDIAG);
@@ -499,7 +507,7 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
       for (int comp = 0; comp < DIM; ++comp)
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) X(comp, qz, qx, j) = fy[comp][qx];
-    } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
+    } else if constexpr (OP == OP_RHS || SQDIAG) {
 #pragma unroll
       for (int qx = 0; qx < Q; ++qx) X(0, qz, qx, j) = fz_or_val[qx];
     }
@@ -563,9 +571,9 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
         }
       }
     }
-  } else if constexpr (OP == OP_RHS || OP == OP_ADIAG) {
+  } else if constexpr (OP == OP_RHS || SQDIAG) {
     __syncthreads();
-    constexpr bool SQ = (OP == OP_ADIAG);
+    constexpr bool SQ = SQDIAG;
     T bs[ZQ][Q], bg[ZQ][Q];
 #pragma unroll
     for (int qz = 0; qz < ZQ; ++qz)
@@ -623,7 +631,8 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
           const T Dx = a0 * ih[0] * ih[0], Dy = a1 * ih[1] * ih[1], Dz = a2 * ih[2] * ih[2];
           const T sum = Dx + Dy + (DIM == 3 ? Dz : (T)0);
           // 2 eta eps(phi):eps(phi) = eta (|grad N|^2 + (dN/dx_c)^2)  [- 2/3 eta (dN/dx_c)^2 compressible]
-          const T fc = L.compressible ? (T)(1.0 / 3.0) : (T)1;
+          // the Laplace diagonal: |grad N|^2 for every component
+          const T fc = OP == OP_LDIAG ? (T)0 : (L.compressible ? (T)(1.0 / 3.0) : (T)1);
           out[0][kz][i] = sum + fc * Dx;
           out[1][kz][i] = sum + fc * Dy;
           if constexpr (DIM == 3) out[2][kz][i] = sum + fc * Dz;
@@ -982,6 +991,8 @@ void launch_dk(Op op, Mode mode, const LevelDesc &L, const T *su, const T *sp, c
     MFS_CASE(OP_ADIAG)
     MFS_CASE(OP_MDIAG)
     MFS_CASE(OP_RHS)
+    MFS_CASE(OP_LAPLACE)
+    MFS_CASE(OP_LDIAG)
   }
 #undef MFS_CASE
 }
@@ -1179,6 +1190,137 @@ template void launch_cell_op<double>(Op, Mode, const LevelDesc &, const double *
                                      int, const double *, const double *, double *, double *, hipStream_t, double *);
 template void launch_cell_op<float>(Op, Mode, const LevelDesc &, const float *, const float *, const float *, int,
                                     const float *, const float *, float *, float *, hipStream_t, float *);
+
+// ------------------------------------------------------------------ free-surface face term
+// StokesOperator::local_apply_boundary_face (matrix_free_operators.cc:563-602):
+// on a free-surface face (normal direction d, side), -(t . u)(phi . n) over the
+// face.  Pass 1, one thread per (face cell, face quadrature point): the value
+// s = JxW (t . u)(x_q) with u read from the face's nodes (constrained DoFs as
+// zero unless plain_read).  Pass 2, one thread per face node: the rows of the
+// normal component add sign * (-n_d) sum over the <= 2^(dim-1) face cells at
+// the node of sum_q phi_node(x_q) s_q, in a fixed order; constrained rows are
+// skipped.  Face cells are lexicographic over the tangential directions t1 < t2
+// (t1 fastest), face quadrature points likewise.
+template <int DIM, int K>
+__global__ void k_fs_values(const LevelDesc L, const int face, const double *__restrict__ stab,
+                            const double *__restrict__ src_u, double *__restrict__ sv) {
+  using B = tab::Basis<double, K>;
+  constexpr int N = K + 1, Q = K + 1, NFQ = DIM == 3 ? Q * Q : Q, NFN = DIM == 3 ? N * N : N;
+  const int d = face / 2, side = face % 2;
+  const int t1 = d == 0 ? 1 : 0, t2 = DIM == 3 ? (d == 2 ? 1 : 2) : 0;
+  const long long n1 = L.n[t1], n2 = DIM == 3 ? L.n[t2] : 1;
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n1 * n2 * NFQ) return;
+  const int q = (int)(idx % NFQ);
+  const long long cell = idx / NFQ;
+  const int c1 = (int)(cell % n1), c2 = (int)(cell / n1);
+  const int q1 = q % Q, q2 = q / Q;
+  const double area = L.h[t1] * (DIM == 3 ? L.h[t2] : 1.0);
+  const double w = area * B::W(q1) * (DIM == 3 ? B::W(q2) : 1.0);
+  int g[3] = {0, 0, 0};
+  g[d] = side == 0 ? 0 : L.nv[d] - 1;
+  double tu = 0;
+#pragma unroll
+  for (int a = 0; a < NFN; ++a) {
+    const int a1 = a % N, a2 = a / N;
+    g[t1] = K * c1 + a1;
+    if (DIM == 3) g[t2] = K * c2 + a2;
+    const double phi = B::S(q1, a1) * (DIM == 3 ? B::S(q2, a2) : 1.0);
+    const long long node = g[0] + (long long)L.nv[0] * (g[1] + (long long)L.nv[1] * g[2]);
+#pragma unroll
+    for (int comp = 0; comp < DIM; ++comp) {
+      const bool con = !L.plain_read && is_con(L, comp, g[0], g[1], g[2]);
+      const double uv = con ? 0.0 : src_u[comp * L.vnodes + node];
+      tu += stab[idx * DIM + comp] * phi * uv;
+    }
+  }
+  sv[idx] = w * tu;
+}
+
+template <int DIM, int K>
+__global__ void k_fs_apply(const LevelDesc L, const int face, const double *__restrict__ sv, const double sign,
+                           double *__restrict__ dst_u) {
+  using B = tab::Basis<double, K>;
+  constexpr int Q = K + 1, NFQ = DIM == 3 ? Q * Q : Q;
+  const int d = face / 2, side = face % 2;
+  const int t1 = d == 0 ? 1 : 0, t2 = DIM == 3 ? (d == 2 ? 1 : 2) : 0;
+  const long long m1 = L.nv[t1], m2 = DIM == 3 ? L.nv[t2] : 1;
+  const long long n1 = L.n[t1], n2 = DIM == 3 ? L.n[t2] : 1;
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= m1 * m2) return;
+  const int g1 = (int)(idx % m1), g2 = (int)(idx / m1);
+  int g[3] = {0, 0, 0};
+  g[d] = side == 0 ? 0 : L.nv[d] - 1;
+  g[t1] = g1;
+  if (DIM == 3) g[t2] = g2;
+  if (is_con(L, d, g[0], g[1], g[2])) return;
+  double val = 0;
+  // contributing face cells per tangential direction: (g/K - 1, local K) when g % K == 0 and g > 0, (g/K, g % K)
+#pragma unroll
+  for (int e2 = 0; e2 < (DIM == 3 ? 2 : 1); ++e2) {
+    int c2 = 0, a2 = 0;
+    if (DIM == 3) {
+      c2 = e2 == 0 ? g2 / K - 1 : g2 / K;
+      a2 = e2 == 0 ? K : g2 % K;
+      if (e2 == 0 && !(g2 % K == 0 && g2 > 0)) continue;
+      if (c2 >= n2) continue;
+    }
+#pragma unroll
+    for (int e1 = 0; e1 < 2; ++e1) {
+      const int c1 = e1 == 0 ? g1 / K - 1 : g1 / K, a1 = e1 == 0 ? K : g1 % K;
+      if (e1 == 0 && !(g1 % K == 0 && g1 > 0)) continue;
+      if (c1 >= n1) continue;
+      const double *s = sv + ((long long)c1 + n1 * c2) * NFQ;
+#pragma unroll
+      for (int q = 0; q < NFQ; ++q) {
+        const int q1 = q % Q, q2 = q / Q;
+        val += B::S(q1, a1) * (DIM == 3 ? B::S(q2, a2) : 1.0) * s[q];
+      }
+    }
+  }
+  const double nsign = side == 0 ? -1.0 : 1.0;
+  dst_u[d * L.vnodes + g[0] + (long long)L.nv[0] * (g[1] + (long long)L.nv[1] * g[2])] += sign * (-nsign) * val;
+}
+
+long long free_surface_cells(const LevelDesc &L, int face) {
+  const int d = face / 2;
+  long long c = 1;
+  for (int e = 0; e < L.dim; ++e)
+    if (e != d) c *= L.n[e];
+  return c;
+}
+
+void launch_free_surface(const LevelDesc &L, const int faces[6], const double *stab, const double *src_u,
+                         double *sv, double sign, double *dst_u, hipStream_t st) {
+  const int nfq = L.dim == 3 ? (L.k + 1) * (L.k + 1) : L.k + 1;
+  long long off = 0;
+  for (int f = 0; f < 2 * L.dim; ++f) {
+    if (!faces[f]) continue;
+    const int d = f / 2;
+    const long long cells = free_surface_cells(L, f);
+    long long nodes = 1;
+    for (int e = 0; e < L.dim; ++e)
+      if (e != d) nodes *= L.nv[e];
+    const long long nv = cells * nfq;
+    const unsigned g1 = (unsigned)((nv + 255) / 256), g2 = (unsigned)((nodes + 255) / 256);
+    const double *tb = stab + off * L.dim;
+    if (L.dim == 3 && L.k == 2) {
+      hipLaunchKernelGGL((k_fs_values<3, 2>), dim3(g1), dim3(256), 0, st, L, f, tb, src_u, sv);
+      hipLaunchKernelGGL((k_fs_apply<3, 2>), dim3(g2), dim3(256), 0, st, L, f, sv, sign, dst_u);
+    } else if (L.dim == 3 && L.k == 3) {
+      hipLaunchKernelGGL((k_fs_values<3, 3>), dim3(g1), dim3(256), 0, st, L, f, tb, src_u, sv);
+      hipLaunchKernelGGL((k_fs_apply<3, 3>), dim3(g2), dim3(256), 0, st, L, f, sv, sign, dst_u);
+    } else if (L.dim == 2 && L.k == 2) {
+      hipLaunchKernelGGL((k_fs_values<2, 2>), dim3(g1), dim3(256), 0, st, L, f, tb, src_u, sv);
+      hipLaunchKernelGGL((k_fs_apply<2, 2>), dim3(g2), dim3(256), 0, st, L, f, sv, sign, dst_u);
+    } else {
+      hipLaunchKernelGGL((k_fs_values<2, 3>), dim3(g1), dim3(256), 0, st, L, f, tb, src_u, sv);
+      hipLaunchKernelGGL((k_fs_apply<2, 3>), dim3(g2), dim3(256), 0, st, L, f, sv, sign, dst_u);
+    }
+    off += nv;
+  }
+  MFS_HIP(hipGetLastError());
+}
 
 // ------------------------------------------------------------------ transfers
 // prolongation: one thread per fine node (all components)

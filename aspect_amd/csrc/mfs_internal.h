@@ -18,7 +18,18 @@ namespace mfs {
   } while (0)
 
 // OP_RHS: buoyancy right-hand side  sum_q rho g . phi JxW  (rho per quadrature point in the eta slot)
-enum Op { OP_STOKES = 0, OP_ABLOCK = 1, OP_BT = 2, OP_MASS = 3, OP_ADIAG = 4, OP_MDIAG = 5, OP_RHS = 6 };
+// OP_LAPLACE: vector Laplace grad u : grad v (coefficient 1) of the mesh-deformation solve; OP_LDIAG its diagonal
+enum Op {
+  OP_STOKES = 0,
+  OP_ABLOCK = 1,
+  OP_BT = 2,
+  OP_MASS = 3,
+  OP_ADIAG = 4,
+  OP_MDIAG = 5,
+  OP_RHS = 6,
+  OP_LAPLACE = 7,
+  OP_LDIAG = 8
+};
 // SET: dst = A src;  RESID: dst = rhs - A src;  ADD: dst += A src
 enum Mode { MODE_SET = 0, MODE_RESID = 1, MODE_ADD = 2 };
 
@@ -217,5 +228,14 @@ void v_idr_shadow(double *q, int k, long long n, hipStream_t st);
 // zero constrained velocity entries of a level vector
 template <typename T>
 void v_zero_constrained(const LevelDesc &L, T *u, hipStream_t st);
+// Free-surface face term of the Stokes operator (active level): dst_u +=
+// sign * F src_u on the faces marked in faces[6]; stab holds dim values per
+// face quadrature point of the marked faces (face order, face cells and
+// points lexicographic over the tangential directions); sv: work array of
+// one value per face quadrature point of the marked faces.  plain_read of L
+// selects reads of constrained DoFs as stored.
+long long free_surface_cells(const LevelDesc &L, int face);
+void launch_free_surface(const LevelDesc &L, const int faces[6], const double *stab, const double *src_u,
+                         double *sv, double sign, double *dst_u, hipStream_t st);
 
 }  // namespace mfs

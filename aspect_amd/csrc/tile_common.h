@@ -114,7 +114,7 @@ __device__ __forceinline__ void tile_write_u(const LevelDesc &L, const T *xb, co
     T vv;
     if (is_con(L, comp, gx, gy, gz)) {
       if (!first) continue;
-      if constexpr (OP == OP_ADIAG)
+      if constexpr (OP == OP_ADIAG || OP == OP_LDIAG)
         vv = 1;
       else if constexpr (OP == OP_BT || OP == OP_RHS)
         vv = 0;

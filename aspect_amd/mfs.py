@@ -37,7 +37,8 @@ EXPORTED = [
     "mfs_ablock_vmult_device", "mfs_solve_device", "mfs_assemble_buoyancy_rhs_device", "mfs_synchronize",
     "mfs_memory_consumption", "mfs_time_operator", "mfs_set_graph_capture", "mfs_correct_stokes_rhs",
     "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
-    "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc",
+    "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
+    "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle",
 ]
 
 
@@ -75,7 +76,7 @@ KRYLOV = {"GMRES": 0, "IDR(s)": 1}
 # "Linear solver failure strategy"
 FAILURE = {"abort": 0, "continue with nonlinear solver": 1}
 # operator codes and modes of mfs_validate_operator_arguments
-OPS = {"stokes": 0, "ablock": 1, "bt": 2, "mass": 3, "adiag": 4, "mdiag": 5, "rhs": 6}
+OPS = {"stokes": 0, "ablock": 1, "bt": 2, "mass": 3, "adiag": 4, "mdiag": 5, "rhs": 6, "laplace": 7, "ldiag": 8}
 MODES = {"set": 0, "resid": 1, "add": 2}
 
 
@@ -224,6 +225,55 @@ class StokesMatrixFreeHandler:
         mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
         _check(self.lib.mfs_set_viscosity(self._h, _ptr(eta), mode))
         self._averaging = mode
+
+    def set_free_surface(self, faces, stab=None):
+        """Free-surface stabilization on the marked faces (x-, x+, y-, y+, z-, z+):
+        stab holds dim values per face quadrature point of the marked faces."""
+        f = (C.c_int * 6)(*[int(v) for v in list(faces) + [0] * (6 - len(faces))])
+        if stab is None:
+            _check(self.lib.mfs_set_free_surface(self._h, f, None))
+            return
+        st = np.ascontiguousarray(stab, dtype=np.float64).reshape(-1)
+        d = self.box.dim
+        nq = (self.box.degree + 1) ** (d - 1)
+        n = [self.box.coarse[e] << (self.box.n_levels - 1) for e in range(d)]
+        need = sum(int(np.prod([n[e] for e in range(d) if e != fi // 2])) * nq * d for fi in range(2 * d) if f[fi])
+        if st.size != need:
+            raise ValueError(f"stabilization table must hold {need} values")
+        _check(self.lib.mfs_set_free_surface(self._h, f, _ptr(st)))
+
+    # -- mesh deformation (MeshDeformationHandler::compute_mesh_displacements_gmg)
+    def _mesh_bc(self, bc):
+        return (C.c_int * 6)(*[int(v) for v in list(bc) + [0] * (6 - len(bc))])
+
+    def mesh_deformation_solve(self, bc, u0, tolerance=1e-10):
+        """Vector Laplace displacement solve; bc per face: 0 none, 1 no normal
+        displacement, 2 zero, 3 prescribed (from u0).  Returns (solution,
+        iterations, (residual, required residual))."""
+        nu = self.sizes(self.top)["nu"]
+        u0 = _vec(u0, nu, "u0")
+        sol = np.zeros(nu)
+        its = C.c_int()
+        res = np.zeros(2)
+        rc = _check(self.lib.mfs_mesh_deformation_solve(self._h, self._mesh_bc(bc), _ptr(u0), C.c_double(tolerance),
+                                                        _ptr(sol), C.byref(its), _ptr(res)))
+        if rc == MFS_NO_CONVERGENCE:
+            raise NoConvergence(self.lib.mfs_last_error().decode(), (its.value, res))
+        return sol, its.value, (res[0], res[1])
+
+    def mesh_laplace_vmult(self, bc, level, src):
+        n = self.sizes(level)["nu"]
+        return self._unary(self.lib.mfs_mesh_laplace_vmult, (self._mesh_bc(bc), level), src, n)
+
+    def mesh_vcycle(self, bc, src):
+        """One V-cycle of the mesh multigrid and the smoother set-up
+        ([level][lambda_min, lambda_max, theta, delta, degree, eigen-CG its])."""
+        n = self.sizes(self.top)["nu"]
+        src = _vec(src, n, "source vector")
+        dst = np.zeros(n)
+        info = np.zeros(6 * (self.top + 1))
+        _check(self.lib.mfs_mesh_vcycle(self._h, self._mesh_bc(bc), _ptr(src), _ptr(dst), _ptr(info)))
+        return dst, info.reshape(self.top + 1, 6)
 
     def build_preconditioner(self):
         _check(self.lib.mfs_build_preconditioner(self._h))

@@ -122,6 +122,8 @@ const char *mfs_last_error(void);
 #define MFS_OP_ADIAG 4
 #define MFS_OP_MDIAG 5
 #define MFS_OP_RHS 6
+#define MFS_OP_LAPLACE 7 /* vector Laplace of the mesh-deformation solve */
+#define MFS_OP_LDIAG 8   /* its diagonal */
 /* Operator modes: 0 dst = op(src), 1 dst = rhs - op(src), 2 dst += op(src). */
 #define MFS_MODE_SET 0
 #define MFS_MODE_RESID 1
@@ -155,6 +157,44 @@ int mfs_sizes(const mfs_handle *h, int level, long long out[4]);
  * projection restricted by averaging children. */
 int mfs_set_viscosity(mfs_handle *h, const double *eta_qp, int averaging);
 int mfs_set_pressure_scaling(mfs_handle *h, double pressure_scaling);
+
+/* Free-surface stabilization of the active-level Stokes operator,
+ * StokesOperator::local_apply_boundary_face (matrix_free_operators.cc:563-602):
+ * on every face with faces[f] = 1 (order x-, x+, y-, y+, z-, z+) the operator
+ * adds -(t . u)(phi . n) integrated over the face (QGauss(k+1) on the face).
+ * stab holds t: for each marked face in that order, for each face cell
+ * (lexicographic over the two tangential directions, the lower direction
+ * fastest), for each face quadrature point (same order), dim values (ASPECT
+ * fills rho dt theta |g| g_hat, local_smoothing.cc:527-608).  It enters
+ * mfs_stokes_vmult, the solver's Stokes operator and mfs_correct_stokes_rhs,
+ * not the multigrid level operators (as in the reference).  All faces 0
+ * switches it off; stab is copied. */
+int mfs_set_free_surface(mfs_handle *h, const int faces[6], const double *stab);
+
+/* Mesh-deformation displacement solve on the handle's mesh,
+ * MeshDeformationHandler::compute_mesh_displacements_gmg_for_degree
+ * (source/mesh_deformation/interface.cc:1076-1360): the vector Laplace
+ * operator (dim components of FE_Q(k), k the Stokes velocity degree,
+ * coefficient 1) with geometric multigrid (Chebyshev degree 5 over
+ * [1.2 lambda_max / 15, 1.2 lambda_max] from 10 eigenvalue iterations; on
+ * level 0 Chebyshev as a solver, 50 eigenvalue iterations, range 1e-3, degree
+ * from the Chebyshev error bound) inside CG to tolerance * ||rhs|| within
+ * 5 N iterations.  bc per face (x-, x+, y-, y+, z-, z+): 0 none, 1 no normal
+ * displacement, 2 zero displacement, 3 prescribed displacement (values taken
+ * from u0 on those DoFs; the inhomogeneous constraints are moved to the
+ * right-hand side, rhs = -A u0, as the reference does at :1116-1156).  u0 and
+ * sol hold dim * velocity-nodes values (the velocity layout); only the
+ * constrained entries of u0 are read.  iterations: CG steps; residuals[0..1]:
+ * final residual and required residual.  Returns MFS_NO_CONVERGENCE (with the
+ * result filled) when CG stops at the iteration limit. */
+int mfs_mesh_deformation_solve(mfs_handle *h, const int bc[6], const double *u0, double tolerance, double *sol,
+                               int *iterations, double *residuals);
+/* Its pieces for parity checks: the level Laplace operator (Base::vmult
+ * semantics: constrained rows copy src) and one V-cycle of the mesh multigrid;
+ * smoother_info (or NULL): per level lambda_min, lambda_max, theta, delta,
+ * Chebyshev degree, eigenvalue-CG iterations. */
+int mfs_mesh_laplace_vmult(mfs_handle *h, const int bc[6], int level, const double *src, double *dst);
+int mfs_mesh_vcycle(mfs_handle *h, const int bc[6], const double *src, double *dst, double *smoother_info);
 
 /* build_preconditioner(): local_smoothing.cc:1855-1867 (level diagonals) and the
  * Chebyshev eigenvalue estimates of solve() (:827-849). */

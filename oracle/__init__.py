@@ -292,6 +292,30 @@ def vcycle(p, eta, kind, src):
     return dst
 
 
+def free_surface_stab_size(p, faces):
+    """Number of values of the stabilization table for the marked faces."""
+    nq = (p.degree + 1) ** (p.dim - 1)
+    n = [p.coarse[d] << (p.n_levels - 1) for d in range(p.dim)]
+    tot = 0
+    for f in range(2 * p.dim):
+        if faces[f]:
+            d = f // 2
+            cells = int(np.prod([n[e] for e in range(p.dim) if e != d]))
+            tot += cells * nq * p.dim
+    return tot
+
+
+def free_surface_apply(p, faces, stab, src_u, sign=1.0, plain=False, out=None):
+    """The free-surface face term on the active level added to out (N_u values)."""
+    faces = np.ascontiguousarray(np.resize(np.asarray(faces, dtype=np.int32), 6))
+    stab = np.ascontiguousarray(stab, dtype=np.float64)
+    src_u = np.ascontiguousarray(src_u, dtype=np.float64)
+    dst = np.zeros_like(src_u) if out is None else out
+    _check(lib().orc_free_surface_apply(C.byref(p), _p(faces), _p(stab), _p(src_u), _p(dst), C.c_double(sign),
+                                        int(plain)))
+    return dst
+
+
 def time_vcycle(p, eta, kind, reps=1):
     """(set-up seconds, seconds per V-cycle) of the oracle's multigrid on the CPU."""
     eta, per_qp = _eta(eta)
@@ -376,3 +400,52 @@ def simd_width_name() -> str:
 def set_threads(n: int) -> None:
     """OpenMP thread count of the oracle's parallel loops."""
     lib().orc_set_threads(int(n))
+
+
+# ------------------------------------------------------------------ mesh deformation
+def mesh_problem(p: Problem, mesh_bc) -> Problem:
+    """The problem with the mesh-deformation constraints: per face 0 none,
+    1 no normal displacement, 2 zero displacement, 3 prescribed displacement
+    (constrained like 2, values from u0)."""
+    q = Problem.from_buffer_copy(p)
+    for i in range(6):
+        b = int(mesh_bc[i]) if i < len(mesh_bc) else 0
+        q.bc[i] = 2 if b == 3 else b
+    return q
+
+
+def laplace_vmult(p, level, src):
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_laplace_vmult(C.byref(p), level, _p(src), _p(dst)))
+    return dst
+
+
+def laplace_diagonal(p, level):
+    out = np.zeros(sizes(p, level)["nu"])
+    _check(lib().orc_laplace_diagonal(C.byref(p), level, _p(out)))
+    return out
+
+
+def mesh_smoother_info(p):
+    """[level][lambda_min, lambda_max, theta, delta, degree] and eigen-CG iterations per level."""
+    nl = p.n_levels
+    out = np.zeros(nl * 6)
+    _check(lib().orc_mesh_smoother_info(C.byref(p), _p(out)))
+    return out[:nl * 5].reshape(nl, 5), out[nl * 5:]
+
+
+def mesh_vcycle(p, src):
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.zeros_like(src)
+    _check(lib().orc_mesh_vcycle(C.byref(p), _p(src), _p(dst)))
+    return dst
+
+
+def mesh_solve(p, u0, tolerance=1e-10):
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    sol = np.zeros_like(u0)
+    its = C.c_int()
+    res = np.zeros(2)
+    _check(lib().orc_mesh_solve(C.byref(p), _p(u0), C.c_double(tolerance), _p(sol), C.byref(its), _p(res)))
+    return sol, its.value, res

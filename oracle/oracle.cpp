@@ -382,7 +382,7 @@ struct OpInput {
 };
 
 // which ops to do in the cell loop
-enum { OP_STOKES = 0, OP_ABLOCK = 1, OP_BT = 2, OP_MASS = 3 };
+enum { OP_STOKES = 0, OP_ABLOCK = 1, OP_BT = 2, OP_MASS = 3, OP_LAPLACE = 4 };
 
 // Generic cell loop with coloring over 2^dim parities; dst accumulates (constrained rows untouched).
 void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *src_u, const double *src_p,
@@ -407,7 +407,7 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
         if (col != color) continue;
         for (int a = 0; a < nv; ++a) vidx[a] = vnode(L, cc, a, re.N);
         for (int a = 0; a < npl; ++a) pidx[a] = pnode(L, cc, a, re.NP);
-        const bool need_u = (op == OP_STOKES || op == OP_ABLOCK);
+        const bool need_u = (op == OP_STOKES || op == OP_ABLOCK || op == OP_LAPLACE);
         const bool need_p = (op == OP_STOKES || op == OP_BT || op == OP_MASS);
         if (need_u)
           for (int cmp = 0; cmp < dim; ++cmp)
@@ -422,7 +422,8 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
         if (need_p) K.pvalues(pl.data(), pq.data(), work.data());
         // quadrature point physics
         for (int q = 0; q < nq; ++q) {
-          const double eta = op == OP_BT ? 1.0 : in.eta[in.eta_per_qp ? c * nq + q : c];  // B^T has no eta
+          // B^T and the Laplace operator have no coefficient
+          const double eta = (op == OP_BT || op == OP_LAPLACE) ? 1.0 : in.eta[in.eta_per_qp ? c * nq + q : c];
           const double JxW = JxW0 * re.qweight(q);
           // gu[cmp][d][q] = d u_cmp / d x_d
           double eps[3][3] = {{0}};
@@ -440,6 +441,9 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
             if (in.compressible)
               for (int a = 0; a < dim; ++a) sig[a][a] -= 2.0 * eta / 3.0 * div;
           }
+          if (op == OP_LAPLACE)  // grad u : grad v, component by component
+            for (int a = 0; a < dim; ++a)
+              for (int b = 0; b < dim; ++b) sig[a][b] = gu[(a * dim + b) * nq + q];
           if (op == OP_STOKES || op == OP_BT)
             for (int a = 0; a < dim; ++a) sig[a][a] -= in.s * pq[q];
           if (op == OP_STOKES) fp[q] = -in.s * div * JxW;
@@ -529,6 +533,51 @@ void mass_vmult(const Context &C, int l, const double *eta, int per_qp, const do
   cell_loop(in, C.K, OP_MASS, nullptr, src, nullptr, dst);
 }
 
+// deal.II MatrixFreeOperators::LaplaceOperator<dim, k, k+1, dim> (vector
+// Laplace, coefficient 1) of the mesh-deformation solve
+// (source/mesh_deformation/interface.cc:1094-1095), Base::vmult semantics
+void laplace_vmult(const Context &C, int l, const double *src, double *dst) {
+  const Level &L = C.levels[l];
+  const long long nu = L.nu();
+  std::fill(dst, dst + nu, 0.0);
+  OpInput in{&L, nullptr, 0, C.p.pressure_scaling, 0};
+  cell_loop(in, C.K, OP_LAPLACE, src, nullptr, dst, nullptr);
+  for (long long i = 0; i < nu; ++i)
+    if (L.vcon[i]) dst[i] = src[i];
+}
+// its diagonal: |grad N|^2 integrated over the cells of the node (the same
+// for every component), 1 on constrained DoFs
+void laplace_diagonal(const Context &C, int l, double *diag) {
+  const Level &L = C.levels[l];
+  const RefElement &re = C.K.re;
+  const int dim = L.dim, nq = re.nq(), nv = re.nvloc(), N = re.N, Q = re.Q;
+  std::fill(diag, diag + L.nu(), 0.0);
+  double JxW0 = 1.0;
+  for (int d = 0; d < dim; ++d) JxW0 *= L.h[d];
+  for (long long c = 0; c < L.cells; ++c) {
+    int cc[3];
+    cell_coords(L, c, cc);
+    for (int a = 0; a < nv; ++a) {
+      const int ai[3] = {a % N, (a / N) % N, a / (N * N)};
+      double acc = 0;
+      for (int q = 0; q < nq; ++q) {
+        const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
+        double g2 = 0;
+        for (int d = 0; d < dim; ++d) {
+          double v = 1.0;
+          for (int e = 0; e < dim; ++e)
+            v *= (e == d) ? re.G[qi[e] * N + ai[e]] / L.h[e] : re.S[qi[e] * N + ai[e]];
+          g2 += v * v;
+        }
+        acc += g2 * JxW0 * re.qweight(q);
+      }
+      const long long gn = vnode(L, cc, a, N);
+      for (int cmp = 0; cmp < dim; ++cmp) diag[(size_t)cmp * L.vnodes + gn] += acc;
+    }
+  }
+  for (long long i = 0; i < L.nu(); ++i)
+    if (L.vcon[i]) diag[i] = 1.0;
+}
 
 // Diagonal of the A block from the closed form of 2 eta eps(phi):eps(phi) for
 // phi = N e_c on an axis-aligned cell:  eta (|grad N|^2 + (d_c N)^2)
@@ -1686,6 +1735,158 @@ int orc_vcycle(const orc_problem *p, const double *eta, int per_qp, int kind, co
     std::memcpy(dst, x.data(), x.size() * sizeof(double));
   })
 }
+// Mesh-deformation displacement solve (MeshDeformationHandler::
+// compute_mesh_displacements_gmg_for_degree, source/mesh_deformation/interface.cc:1076-1360):
+// Chebyshev smoothers of degree 5 over [1.2 lambda_max / 15, 1.2 lambda_max]
+// from 10 eigenvalue-CG iterations on levels > 0; on level 0 Chebyshev as a
+// solver: 50 eigenvalue iterations, range 1e-3 and the degree chosen by the
+// Chebyshev error bound (deal.II's degree = invalid_unsigned_int):
+//   sigma = (1 - sqrt(1/r)) / (1 + sqrt(1/r)), r = lambda_max / alpha,
+//   degree = 1 + floor(log(1/eps + sqrt(1/eps^2 - 1)) / log(1/sigma)), eps = 1e-3;
+// the coarse grid applies that smoother (MGCoarseGridApplySmoother).
+struct MeshMG {
+  MG mg;
+  void build(const Context &C) {
+    const int nl = C.p.n_levels;
+    mg.C = &C;
+    mg.kind = 0;
+    mg.steps = 1;
+    for (int l = 0; l < nl; ++l) {
+      mg.ops.push_back([&C, l](const double *s, double *d) { laplace_vmult(C, l, s, d); });
+      mg.sizes.push_back(C.levels[l].nu());
+    }
+    mg.smoothers.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+      Chebyshev &a = mg.smoothers[l];
+      a.A = mg.ops[l];
+      a.dinv.resize(mg.sizes[l]);
+      laplace_diagonal(C, l, a.dinv.data());
+      for (auto &v : a.dinv) v = 1.0 / v;
+      if (l > 0) {
+        a.degree = 5;
+        a.estimate(10, 15.0);
+      } else {
+        a.estimate(50, 1e-3);
+        const double alpha = a.theta - a.delta;
+        const double r = a.lmax / alpha;
+        const double sigma = (1.0 - std::sqrt(1.0 / r)) / (1.0 + std::sqrt(1.0 / r));
+        const double eps = 1e-3;
+        a.degree = sigma > 0 ? 1 + (int)(std::log(1.0 / eps + std::sqrt(1.0 / eps / eps - 1.0)) / std::log(1.0 / sigma))
+                             : 1;
+      }
+    }
+  }
+};
+// CG with the V-cycle (SolverCG, interface.cc:1316-1340); rhs = -A u0 with u0
+// read as stored and constrained rows zero (:1130-1156); solution = x + u0 on
+// the constrained DoFs (constraints.distribute, :1342).  Returns iterations.
+int mesh_solve(const Context &C, const double *u0, double tolerance, double *sol, double *res_out) {
+  const int top = C.p.n_levels - 1;
+  const Level &L = C.levels[top];
+  const size_t n = (size_t)L.nu();
+  MeshMG M;
+  M.build(C);
+  Vec rhs(n, 0.0);
+  {
+    OpInput in{&L, nullptr, 0, 1.0, 0};
+    in.plain = 1;
+    cell_loop(in, C.K, OP_LAPLACE, u0, nullptr, rhs.data(), nullptr);
+    for (size_t i = 0; i < n; ++i) rhs[i] = L.vcon[i] ? 0.0 : -rhs[i];
+  }
+  const double tol = tolerance * norm(rhs);
+  Vec x(n, 0.0), r = rhs, z, p, q(n);
+  M.mg.vmult(z, r);
+  p = z;
+  double rz = dot(r, z), res = norm(r);
+  int its = 0;
+  const int max_its = (int)std::min<size_t>(5 * n, 1000000);
+  while (res > tol && its < max_its) {
+    laplace_vmult(C, top, p.data(), q.data());
+    const double alpha = rz / dot(p, q);
+    for (size_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    ++its;
+    res = norm(r);
+    if (res <= tol) break;
+    M.mg.vmult(z, r);
+    const double rz2 = dot(r, z);
+    const double beta = rz2 / rz;
+    rz = rz2;
+    for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+  }
+  for (size_t i = 0; i < n; ++i) sol[i] = L.vcon[i] ? u0[i] : x[i];
+  if (res_out) {
+    res_out[0] = res;
+    res_out[1] = tol;
+  }
+  return its;
+}
+
+// Free-surface stabilization term of the Stokes operator on the active level
+// (StokesOperator::local_apply_boundary_face, matrix_free_operators.cc:563-602):
+// on every face marked in faces[6] (x-, x+, y-, y+, z-, z+),
+//   dst_u[i] += sign * ( - sum_q JxW_q (t_q . u(x_q)) (phi_i(x_q) . n) ),
+// constrained source DoFs read as zero (as stored when plain), constrained rows
+// skipped (integrate_scatter).  stab: per marked face in face order, per face
+// cell (lexicographic over the two other directions, the lower one fastest),
+// per face quadrature point (QGauss(k+1) tensor, lower direction fastest), dim
+// values t (ASPECT: rho dt theta |g| g_hat, local_smoothing.cc:555-608).
+// Written as a plain quadrature loop with the Lagrange basis evaluated pointwise.
+void free_surface_apply(const Context &C, int l, const int faces[6], const double *stab, const double *src_u,
+                        double *dst_u, double sign, int plain) {
+  const Level &L = C.levels[l];
+  const RefElement &re = C.K.re;
+  const int dim = L.dim, N = re.N, Q = re.Q;
+  const double *tab = stab;
+  for (int f = 0; f < 2 * dim; ++f) {
+    if (!faces[f]) continue;
+    const int d = f / 2, side = f % 2;
+    int t1 = -1, t2 = -1;  // tangential directions, lower first
+    for (int e = 0; e < dim; ++e)
+      if (e != d) (t1 < 0 ? t1 : t2) = e;
+    const int n1 = L.n[t1], n2 = dim == 3 ? L.n[t2] : 1;
+    const double area = L.h[t1] * (dim == 3 ? L.h[t2] : 1.0);
+    const int nfq = dim == 3 ? Q * Q : Q, nfn = dim == 3 ? N * N : N;
+    const double nsign = side == 0 ? -1.0 : 1.0;  // outward normal n = nsign e_d
+    for (int c2 = 0; c2 < n2; ++c2)
+      for (int c1 = 0; c1 < n1; ++c1) {
+        // global node of face-local node (a1, a2)
+        auto gnode = [&](int a1, int a2) {
+          int g[3] = {0, 0, 0};
+          g[d] = side == 0 ? 0 : L.nv[d] - 1;
+          g[t1] = L.k * c1 + a1;
+          if (dim == 3) g[t2] = L.k * c2 + a2;
+          return (long long)g[0] + L.nv[0] * ((long long)g[1] + (long long)L.nv[1] * g[2]);
+        };
+        const double *tq = tab + (size_t)(c1 + (size_t)n1 * c2) * nfq * dim;
+        for (int q = 0; q < nfq; ++q) {
+          const int q1 = q % Q, q2 = q / Q;
+          const double w = area * re.qw[q1] * (dim == 3 ? re.qw[q2] : 1.0);
+          double tu = 0;
+          for (int a = 0; a < nfn; ++a) {
+            const int a1 = a % N, a2 = a / N;
+            const double phi = re.vb.value(a1, re.qx[q1]) * (dim == 3 ? re.vb.value(a2, re.qx[q2]) : 1.0);
+            const long long gn = gnode(a1, a2);
+            for (int comp = 0; comp < dim; ++comp) {
+              const size_t gi = (size_t)comp * L.vnodes + gn;
+              const double uv = (L.vcon[gi] && !plain) ? 0.0 : src_u[gi];
+              tu += tq[q * dim + comp] * phi * uv;
+            }
+          }
+          for (int a = 0; a < nfn; ++a) {
+            const int a1 = a % N, a2 = a / N;
+            const double phi = re.vb.value(a1, re.qx[q1]) * (dim == 3 ? re.vb.value(a2, re.qx[q2]) : 1.0);
+            const size_t gi = (size_t)d * L.vnodes + gnode(a1, a2);
+            if (!L.vcon[gi]) dst_u[gi] += sign * (-w * tu * phi * nsign);
+          }
+        }
+      }
+    tab += (size_t)n1 * n2 * nfq * dim;
+  }
+}
+
 // CPU timing of the multigrid preconditioner: set-up (diagonals, eigenvalue
 // estimates) once, then reps V-cycles of kind 0 (A block) or 1 (Schur mass);
 // seconds[0] = set-up, seconds[1] = mean V-cycle time
@@ -1704,6 +1905,67 @@ int orc_time_vcycle(const orc_problem *p, const double *eta, int per_qp, int kin
     const auto t2 = std::chrono::steady_clock::now();
     seconds[0] = std::chrono::duration<double>(t1 - t0).count();
     seconds[1] = std::chrono::duration<double>(t2 - t1).count() / std::max(1, reps);
+  })
+}
+// mesh-deformation pieces: the problem's bc are the mesh constraints
+// (0 none, 1 no normal displacement, 2 zero / prescribed displacement)
+int orc_laplace_vmult(const orc_problem *p, int level, const double *src, double *dst) {
+  ORC_TRY({
+    Context C(*p);
+    laplace_vmult(C, level, src, dst);
+  })
+}
+int orc_laplace_diagonal(const orc_problem *p, int level, double *diag) {
+  ORC_TRY({
+    Context C(*p);
+    laplace_diagonal(C, level, diag);
+  })
+}
+// out[l*5 + {0..4}] = lambda_min, lambda_max, theta, delta, degree (eigenvalue-CG
+// iterations in out[nl*5 + l])
+int orc_mesh_smoother_info(const orc_problem *p, double *out) {
+  ORC_TRY({
+    Context C(*p);
+    MeshMG M;
+    M.build(C);
+    const int nl = p->n_levels;
+    for (int l = 0; l < nl; ++l) {
+      const Chebyshev &a = M.mg.smoothers[l];
+      out[l * 5 + 0] = a.lmin;
+      out[l * 5 + 1] = a.lmax;
+      out[l * 5 + 2] = a.theta;
+      out[l * 5 + 3] = a.delta;
+      out[l * 5 + 4] = a.degree;
+      out[nl * 5 + l] = a.cg_its;
+    }
+  })
+}
+int orc_mesh_vcycle(const orc_problem *p, const double *src, double *dst) {
+  ORC_TRY({
+    Context C(*p);
+    MeshMG M;
+    M.build(C);
+    const int top = p->n_levels - 1;
+    Vec b(src, src + M.mg.size(top)), x;
+    M.mg.vmult(x, b);
+    std::memcpy(dst, x.data(), x.size() * sizeof(double));
+  })
+}
+int orc_mesh_solve(const orc_problem *p, const double *u0, double tolerance, double *sol, int *iterations,
+                   double *res) {
+  ORC_TRY({
+    Context C(*p);
+    *iterations = mesh_solve(C, u0, tolerance, sol, res);
+  })
+}
+// the free-surface face term alone, added to dst (N_u values): sign +1 for the
+// operator, -1 with plain reads for the right-hand-side correction of
+// correct_stokes_rhs (local_smoothing.cc:709-744)
+int orc_free_surface_apply(const orc_problem *p, const int *faces, const double *stab, const double *src_u,
+                           double *dst_u, double sign, int plain) {
+  ORC_TRY({
+    Context C(*p);
+    free_surface_apply(C, p->n_levels - 1, faces, stab, src_u, dst_u, sign, plain);
   })
 }
 // single Chebyshev smoother application (x0 = 0) on a level

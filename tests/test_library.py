@@ -65,8 +65,11 @@ _NEEDS = {
     "adiag": ("eta", "dst_u"),
     "mdiag": ("eta", "dst_p"),
     "rhs": ("eta", "dst_u"),
+    "laplace": ("src_u", "dst_u"),
+    "ldiag": ("dst_u",),
 }
-_RESID = {"stokes": ("rhs_u", "rhs_p"), "ablock": ("rhs_u",), "bt": ("rhs_u",), "mass": ("rhs_p",)}
+_RESID = {"stokes": ("rhs_u", "rhs_p"), "ablock": ("rhs_u",), "bt": ("rhs_u",), "mass": ("rhs_p",),
+          "laplace": ("rhs_u",)}
 _ALL = ("src_u", "src_p", "eta", "rhs_u", "rhs_p", "dst_u", "dst_p")
 
 

@@ -517,3 +517,129 @@ def test_oracle_vcycle_timing_entry():
     s = O.sizes(p, 2)
     setup, per = O.time_vcycle(p, np.ones(s["cells"]), 0, reps=2)
     assert setup > 0 and per > 0
+
+
+def _mass_1d_q2(n, h):
+    """Assembled 1D Q2 mass matrix (nodes 0, 1/2, 1 per cell): h/30 [[4,2,-1],[2,16,2],[-1,2,4]]."""
+    M = np.zeros((2 * n + 1, 2 * n + 1))
+    loc = h / 30.0 * np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]], dtype=float)
+    for c in range(n):
+        M[2 * c:2 * c + 3, 2 * c:2 * c + 3] += loc
+    return M
+
+
+@pytest.mark.parametrize("dim,face", [(3, 5), (3, 0), (3, 2), (2, 3), (2, 1)])
+def test_free_surface_term_closed_form(dim, face):
+    # StokesOperator::local_apply_boundary_face (matrix_free_operators.cc:563-602):
+    # with a constant stabilization vector t the face term is
+    # -(t . u)(phi . n) integrated over the face, i.e. -n_d * M_face (sum_c t_c u_c)
+    # in the rows of the normal component, M_face the exact tensor-product Q2
+    # face mass matrix (QGauss(3) is exact for it); open boundaries, so no DoF is
+    # constrained
+    coarse, ext = ((2, 3, 1), (1.0, 1.5, 0.5)) if dim == 3 else ((3, 2), (1.5, 1.0))
+    nl = 2
+    p = O.make_problem(dim, 2, coarse, nl, ext, (0,) * 6, pressure_scaling=1.0)
+    s = O.sizes(p, nl - 1)
+    n = [coarse[d] << (nl - 1) for d in range(dim)]
+    h = [ext[d] / n[d] for d in range(dim)]
+    nv = [2 * n[d] + 1 for d in range(dim)]
+    d, side = face // 2, face % 2
+    faces = [0] * 6
+    faces[face] = 1
+    rng = np.random.default_rng(face)
+    tconst = rng.normal(size=dim)
+    size = O.free_surface_stab_size(p, faces)
+    stab = np.tile(tconst, size // dim)
+    u = rng.normal(size=s["nu"])
+    out = O.free_surface_apply(p, faces, stab, u)
+    # expected: face nodes of the marked face
+    tang = [e for e in range(dim) if e != d]
+    Mf = _mass_1d_q2(n[tang[0]], h[tang[0]])
+    if dim == 3:
+        Mf = np.kron(_mass_1d_q2(n[tang[1]], h[tang[1]]), Mf)  # lower tangential direction fastest
+    grids = np.meshgrid(*[np.arange(nv[e]) for e in tang[::-1]], indexing="ij")
+    g = [None] * dim
+    for e, gr in zip(tang[::-1], grids):
+        g[e] = gr.ravel()
+    g[d] = np.full(g[tang[0]].shape, 0 if side == 0 else nv[d] - 1)
+    node = g[0] + nv[0] * g[1] + (nv[0] * nv[1] * g[2] if dim == 3 else 0)
+    tu = sum(tconst[c] * u[c * s["vnodes"] + node] for c in range(dim))
+    nsign = -1.0 if side == 0 else 1.0
+    expect = np.zeros(s["nu"])
+    expect[d * s["vnodes"] + node] = -nsign * (Mf @ tu)
+    assert np.linalg.norm(out - expect) <= 1e-13 * np.linalg.norm(expect)
+    # the right-hand-side form (plain reads, sign -1) is the negative on unconstrained data
+    assert np.allclose(O.free_surface_apply(p, faces, stab, u, sign=-1.0, plain=True), -out, rtol=1e-14, atol=0)
+
+
+def _stiff_1d_q2(n, h):
+    """Assembled 1D Q2 stiffness matrix: 1/(3h) [[7,-8,1],[-8,16,-8],[1,-8,7]] per cell."""
+    K = np.zeros((2 * n + 1, 2 * n + 1))
+    loc = 1.0 / (3.0 * h) * np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]], dtype=float)
+    for c in range(n):
+        K[2 * c:2 * c + 3, 2 * c:2 * c + 3] += loc
+    return K
+
+
+def _laplace_closed_form(dim, n, h):
+    """Scalar Q2 Laplace matrix as a sum of Kronecker products of exact 1D matrices (x fastest)."""
+    Ks = [_stiff_1d_q2(n[d], h[d]) for d in range(dim)]
+    Ms = [_mass_1d_q2(n[d], h[d]) for d in range(dim)]
+    A = 0
+    for d in range(dim):
+        term = np.ones((1, 1))
+        for e in reversed(range(dim)):
+            term = np.kron(term, Ks[e] if e == d else Ms[e])
+        A = A + term
+    return A
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_laplace_operator_closed_form(dim):
+    # LaplaceOperator<dim, 2, 3, dim> of the mesh-deformation solve: every
+    # component is the scalar Q2 Laplace matrix, K(x)M(x)M + ... with exact 1D
+    # matrices (QGauss(3) integrates them exactly); constrained rows identity,
+    # constrained columns dropped; the diagonal of that matrix
+    coarse, ext = ((2, 1, 1), (1.0, 0.5, 0.75)) if dim == 3 else ((2, 3), (1.0, 1.5))
+    nl = 2
+    bc = (1, 2, 0, 1, 2, 0) if dim == 3 else (1, 2, 0, 1)
+    p = O.mesh_problem(O.make_problem(dim, 2, coarse, nl, ext, (0,) * 6), bc)
+    s = O.sizes(p, nl - 1)
+    n = [coarse[d] << (nl - 1) for d in range(dim)]
+    h = [ext[d] / n[d] for d in range(dim)]
+    A1 = _laplace_closed_form(dim, n, h)
+    A = np.kron(np.eye(dim), A1)
+    con = O.constrained(p, nl - 1)
+    Ac = A.copy()
+    Ac[con, :] = 0
+    Ac[:, con] = 0
+    Ac[con, con] = 1.0
+    x = np.random.default_rng(dim).normal(size=s["nu"])
+    assert np.linalg.norm(O.laplace_vmult(p, nl - 1, x) - Ac @ x) <= 1e-13 * np.linalg.norm(Ac @ x)
+    np.testing.assert_allclose(O.laplace_diagonal(p, nl - 1), np.diag(Ac), rtol=1e-13)
+
+
+def test_mesh_solve_against_direct_solve():
+    # CG with the mesh multigrid against the direct solve of the constrained
+    # system: prescribed displacement on the top face (bc 3), zero on the
+    # bottom, no normal displacement on the sides
+    p0 = O.make_problem(3, 2, (1, 1, 1), 3, (1.0, 1.0, 1.0), (0,) * 6)
+    p = O.mesh_problem(p0, (1, 1, 1, 1, 2, 3))
+    s = O.sizes(p, 2)
+    n = [4, 4, 4]
+    A = np.kron(np.eye(3), _laplace_closed_form(3, n, [0.25] * 3))
+    con = O.constrained(p, 2)
+    pts = O.velocity_support_points(p, 2)
+    u0 = np.zeros(s["nu"])
+    top = np.isclose(pts[:, 2], 1.0)
+    u0[2 * s["vnodes"]:][top] = 0.1 * np.sin(np.pi * pts[top, 0]) * np.sin(np.pi * pts[top, 1])
+    u0[~con] = 0
+    sol, its, res = O.mesh_solve(p, u0, tolerance=1e-12)
+    free = ~con
+    ref = u0.copy()
+    ref[free] = np.linalg.solve(A[np.ix_(free, free)], -A[np.ix_(free, con)] @ u0[con])
+    assert np.linalg.norm(sol - ref) <= 1e-9 * np.linalg.norm(ref)
+    assert 0 < its < 30
+    info, cg_its = O.mesh_smoother_info(p)
+    assert list(info[1:, 4]) == [5, 5] and info[0, 4] >= 1
+    assert list(cg_its[1:]) == [10, 10]
