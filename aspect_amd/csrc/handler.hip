@@ -391,7 +391,7 @@ struct GMG {
     }
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
-      if (!eta_pq && g.L.dim == 3 && g.L.k == 2) {
+      if (!eta_pq && g.L.dim == 3 && g.L.k == 2 && !g.L.geo) {
         if (g.ietan.n != g.L.cells) g.ietan.alloc(g.L.cells);
         launch_inv_eta_nat<T>(g.L, g.eta.p, g.ietan.p, st);
       }
@@ -455,6 +455,7 @@ struct GMG {
     auto sh = std::make_unique<GMG<double>>();
     sh->graphs = false;
     sh->setup(b, st, dot_work, dot_out);
+    for (int l = 0; l <= top; ++l) sh->lv[l]->L.geo = lv[l]->L.geo;
     if (eta_pq)
       sh->set_eta_q1(*q1_src, q1_lo, q1_hi);
     else
@@ -652,6 +653,8 @@ struct mfs_handle {
   // mesh-deformation hierarchy (built for the mesh constraints of the last call)
   std::unique_ptr<GMG<double>> gmesh;
   int gmesh_bc[6] = {-1, -1, -1, -1, -1, -1};
+  // general geometry: tile-blocked tables per level (empty: the uniform box)
+  std::vector<DBuf<double>> geo_lv;
 
   ~mfs_handle() {
     for (double *p : vchunks) (void)hipFree(p);
@@ -1439,6 +1442,9 @@ void set_viscosity(mfs_handle *h, const double *eta_qp, int averaging) {
   if (h->eta_nat.n != nqv) h->eta_nat.alloc(nqv);
   MFS_HIP(hipMemcpyAsync(h->eta_nat.p, eta_qp, nqv * sizeof(double), hipMemcpyHostToDevice, h->st));
   const bool q1 = averaging == MFS_AVERAGE_PROJECT_Q1;
+  if (q1)
+    for (const LevelDesc &Ll : h->L)
+      if (Ll.geo) throw std::runtime_error("mfs_set_viscosity: project-to-Q1 averaging needs the box geometry");
   h->active_per_qp = (averaging == MFS_AVERAGE_NONE || q1) ? 1 : 0;
   const long long na = L.ntiles_total * h->ct * (h->active_per_qp ? h->nq : 1);
   if (h->eta_active.n != na) h->eta_active.alloc(na);
@@ -1648,8 +1654,63 @@ int mfs_set_pressure_scaling(mfs_handle *h, double s) {
   });
 }
 
+// volume weights int psi_i of the pressure normalization on the active level
+void pressure_weights(mfs_handle *h) {
+  LevelDesc Lu = h->Ltop();
+  Lu.s = 1.0;
+  level_derive(Lu);
+  Lu.geo = h->Ltop().geo;
+  v_fill<double>(h->w3.p, 1.0, h->np, h->st);
+  launch_cell_op<double>(OP_MASS, MODE_SET, Lu, nullptr, h->w3.p, h->ones_cell.p, 0, nullptr, nullptr, nullptr,
+                         h->pnorm_w.p, h->st);
+}
+
+void set_geometry(mfs_handle *h, int level, const double *geo) {
+  need_handle(h);
+  check_level(h, level, false);
+  if (!geo) throw std::invalid_argument("mfs_set_geometry: null table");
+  for (int f = 0; f < 2 * h->box.dim; ++f)
+    if (h->box.bc[f] == MFS_BC_TANGENTIAL)
+      throw std::invalid_argument("mfs_set_geometry: no-normal-flux faces need axis-aligned normals (box geometry)");
+  if (h->fs_on) throw std::invalid_argument("mfs_set_geometry: the free-surface term needs the box geometry");
+  const LevelDesc &L = h->L[level];
+  const int nq = h->nq;
+  const long long n = L.cells * nq * (L.dim * L.dim + 1);
+  if ((int)h->geo_lv.size() != h->top + 1) h->geo_lv.resize(h->top + 1);
+  DBuf<double> nat;
+  nat.alloc(n);
+  MFS_HIP(hipMemcpyAsync(nat.p, geo, n * sizeof(double), hipMemcpyHostToDevice, h->st));
+  DBuf<double> &t = h->geo_lv[level];
+  if (t.n != geo_table_size(L)) t.alloc(geo_table_size(L));
+  launch_geo_to_tiles(L, nat.p, t.p, h->st);
+  MFS_HIP(hipStreamSynchronize(h->st));
+  // every descriptor of this level: the handler's, the multigrid's (graphs hold
+  // them by value, so they are dropped), the mesh-deformation hierarchy's
+  h->L[level].geo = t.p;
+  auto upd = [&](auto &g) {
+    if ((int)g.lv.size() > level) {
+      g.lv[level]->L.geo = t.p;
+      g.drop_graphs();
+      g.shadow.reset();
+    }
+  };
+  if (h->prec == MFS_GMG_FP32)
+    upd(h->g32);
+  else
+    upd(h->g64);
+  h->gmesh.reset();
+  h->have_prec = false;
+  h->have_eta = false;  // the level tables depend on JxW: mfs_set_viscosity again
+  if (level == h->top) {
+    pressure_weights(h);
+    MFS_HIP(hipStreamSynchronize(h->st));
+  }
+}
+
 void set_free_surface(mfs_handle *h, const int *faces, const double *stab) {
   need_handle(h);
+  for (const auto &L : h->L)
+    if (L.geo) throw std::invalid_argument("mfs_set_free_surface: box geometry only");
   if (!faces) throw std::invalid_argument("mfs_set_free_surface: null faces");
   const LevelDesc &L = h->Ltop();
   const int nfq = L.dim == 3 ? (L.k + 1) * (L.k + 1) : L.k + 1;
@@ -1688,12 +1749,15 @@ void mesh_setup(mfs_handle *h, const int *bc) {
   for (int f = 0; f < 6; ++f) {
     if (bc[f] < 0 || bc[f] > 3) throw std::invalid_argument("mesh deformation: boundary type must be 0..3");
     mb.bc[f] = f < 2 * mb.dim ? (bc[f] == 3 ? 2 : bc[f]) : 0;
+    if (mb.bc[f] == MFS_BC_TANGENTIAL && h->Ltop().geo)
+      throw std::invalid_argument("mesh deformation: no-normal-displacement faces need the box geometry");
   }
   mb.compressible = 0;
   if (h->gmesh && std::memcmp(h->gmesh_bc, mb.bc, sizeof(mb.bc)) == 0) return;
   auto g = std::make_unique<GMG<double>>();
   g->laplace = true;
   g->setup(mb, h->st, &h->dot_work, &h->dot_out);
+  for (int l = 0; l <= g->top; ++l) g->lv[l]->L.geo = h->L[l].geo;
   g->build(mb);
   MFS_HIP(hipStreamSynchronize(h->st));
   std::memcpy(h->gmesh_bc, mb.bc, sizeof(mb.bc));
@@ -1766,6 +1830,13 @@ void build_prec(mfs_handle *h) {
     h->g64.build(h->box);
   MFS_HIP(hipStreamSynchronize(h->st));
   h->have_prec = true;
+}
+
+int mfs_set_geometry(mfs_handle *h, int level, const double *geo) {
+  return guard([&] {
+    set_geometry(h, level, geo);
+    return MFS_OK;
+  });
 }
 
 int mfs_set_free_surface(mfs_handle *h, const int faces[6], const double *stab) {

@@ -407,7 +407,16 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
     for (int qx = 0; qx < Q; ++qx) {
       const int qn = qx + Q * (j + Q * qz);
       const double wq = c_basis[K - 2].w[qx] * c_basis[K - 2].w[j] * (DIM == 3 ? c_basis[K - 2].w[qz] : 1.0);
-      const T JxW = (T)(jac * wq);
+      // general geometry: J^{-1} (row-major, e * DIM + d) and JxW per quadrature point
+      T Jv[DIM * DIM];
+      T JxW = (T)(jac * wq);
+      if (L.geo) {
+        constexpr int G = DIM * DIM + 1;
+        const double *gq = L.geo + (tile_id * C::NQ + qn) * (long long)G * CT + c;
+#pragma unroll
+        for (int e = 0; e < DIM * DIM; ++e) Jv[e] = (T)gq[e * CT];
+        JxW = (T)gq[DIM * DIM * CT];
+      }
       T et = 1;
       if constexpr (OP != OP_BT && OP != OP_LAPLACE && OP != OP_LDIAG)
         et = eta_per_qp ? eta[(tile_id * C::NQ + qn) * CT + c] : eta[tile_id * CT + c];
@@ -434,9 +443,20 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
             dy += CB.D[j][r] * X(comp, qz, qx, r);
             if constexpr (DIM == 3) dz += CB.D[qz][r] * v[comp][r][qx];
           }
-          gr[comp][0] = dx * ih[0];
-          gr[comp][1] = dy * ih[1];
-          if constexpr (DIM == 3) gr[comp][2] = dz * ih[2];
+          if (L.geo) {
+            const T rd[3] = {dx, dy, dz};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+              T a = 0;
+#pragma unroll
+              for (int e = 0; e < DIM; ++e) a += Jv[e * DIM + d] * rd[e];
+              gr[comp][d] = a;
+            }
+          } else {
+            gr[comp][0] = dx * ih[0];
+            gr[comp][1] = dy * ih[1];
+            if constexpr (DIM == 3) gr[comp][2] = dz * ih[2];
+          }
         }
       }
       T fp = 0;
@@ -469,15 +489,30 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
 #pragma unroll
             for (int bb = 0; bb < DIM; ++bb) flux[a][bb] = a == bb ? -s * pqv * JxW : (T)0;
         }
-        // x and z tests accumulate in registers, y test is stashed
+        // fluxes against the reference derivatives of the test functions:
+        // f_e = sum_d J^{-1}[e][d] f_d (box: f_e / h_e)
 #pragma unroll
         for (int comp = 0; comp < DIM; ++comp) {
+          T fr[DIM];
+          if (L.geo) {
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+              T a = 0;
+#pragma unroll
+              for (int d = 0; d < DIM; ++d) a += Jv[e * DIM + d] * flux[comp][d];
+              fr[e] = a;
+            }
+          } else {
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) fr[e] = flux[comp][e] * ih[e];
+          }
+          // x and z tests accumulate in registers, y test is stashed
 #pragma unroll
           for (int r = 0; r < Q; ++r) {
-            g[comp][qz][r] += CB.D[qx][r] * flux[comp][0] * ih[0];
-            if constexpr (DIM == 3) g[comp][r][qx] += CB.D[qz][r] * flux[comp][2] * ih[2];
+            g[comp][qz][r] += CB.D[qx][r] * fr[0];
+            if constexpr (DIM == 3) g[comp][r][qx] += CB.D[qz][r] * fr[2];
           }
-          fy[comp][qx] = flux[comp][1] * ih[1];
+          fy[comp][qx] = fr[1];
         }
       } else if constexpr (OP == OP_MASS) {
         fp = s * s / et * pqv * JxW;
@@ -913,7 +948,7 @@ __global__ void k_inv_eta_nat(const LevelDesc L, const T *__restrict__ eta_tiles
 template <typename T>
 bool launch_mass_inv(Mode mode, const LevelDesc &L, const T *src, const T *ieta, const T *rhs, T *dst,
                      hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || !ieta) return false;
+  if (L.dim != 3 || L.k != 2 || !ieta || L.geo) return false;
   if (L.pnodes >= (1LL << 31)) throw std::runtime_error("level too large for the node-wise mass apply");
   const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
   if (mode == MODE_SET)
@@ -956,7 +991,7 @@ template void launch_inv_eta_nat<float>(const LevelDesc &, const float *, float 
 template <typename T>
 bool launch_mass_nodes(Mode mode, const LevelDesc &L, const T *src, const T *eta, int eta_per_qp, const T *rhs,
                        T *dst, hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || eta_per_qp) return false;
+  if (L.dim != 3 || L.k != 2 || eta_per_qp || L.geo) return false;
   static_assert(Tile<3, 2>::TX == 4 && Tile<3, 2>::TY == 4 && Tile<3, 2>::TZ == 4, "tile-blocked eta layout");
   if (L.pnodes >= (1LL << 31) || L.ntiles_total * 64 >= (1LL << 31))
     throw std::runtime_error("level too large for the node-wise mass apply (32-bit offsets)");
@@ -1142,7 +1177,7 @@ __global__ void __launch_bounds__(256) k_bt_nodes(const LevelDesc L, const T *__
 }
 template <typename T, typename TO>
 bool launch_bt_nodes(Mode mode, const LevelDesc &L, const T *src_p, const T *rhs_u, TO *dst_u, hipStream_t st) {
-  if (L.k != 2) return false;
+  if (L.k != 2 || L.geo) return false;
   if ((long long)L.nv[0] * L.nv[1] >= (1LL << 31) || L.pnodes >= (1LL << 31))
     throw std::runtime_error("level too large for the node-wise B^T apply (32-bit offsets)");
   const dim3 grid((unsigned)(((long long)(L.n[0] + 1) * L.nv[1] + 255) / 256), (unsigned)(L.dim == 3 ? L.nv[2] : 1));
@@ -1173,6 +1208,10 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
   check_op_args(op, mode, src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p);
   if (launch_cell_q2<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st, split_work))
     return;
+  if ((op == OP_ADIAG || op == OP_LDIAG) && L.geo) {
+    launch_vdiag_geo<T>(op, L, eta, eta_per_qp, dst_u, st);
+    return;
+  }
   if (op == OP_MASS && launch_mass_nodes<T>(mode, L, src_p, eta, eta_per_qp, rhs_p, dst_p, st)) return;
   if (op == OP_BT && launch_bt_nodes<T, T>(mode, L, src_p, rhs_u, dst_u, st)) return;
   if (L.dim == 3 && L.k == 2)
@@ -1678,6 +1717,148 @@ __device__ __forceinline__ long long tile_index(const LevelDesc &L, int gcx, int
   return tile * (TX * TY * TZ) + cx + TX * (cy + TY * cz);
 }
 
+// ------------------------------------------------------------------ general geometry
+long long geo_table_size(const LevelDesc &L) {
+  int t[3];
+  const int ct = tile_cells(L.dim, L.k, t);
+  const int nq = L.dim == 3 ? (L.k + 1) * (L.k + 1) * (L.k + 1) : (L.k + 1) * (L.k + 1);
+  return L.ntiles_total * nq * (L.dim * L.dim + 1) * ct;
+}
+
+// natural [cell][q][g] -> [((tile * NQ + q) * G + g) * CT + c]; one thread per (cell, q)
+__global__ void k_geo_to_tiles(const LevelDesc L, const double *__restrict__ nat, const int nq, const int TX,
+                               const int TY, const int TZ, double *__restrict__ out) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= L.cells * nq) return;
+  const long long cell = idx / nq;
+  const int q = (int)(idx - cell * nq);
+  const int gcx = (int)(cell % L.n[0]), gcy = (int)((cell / L.n[0]) % L.n[1]),
+            gcz = (int)(cell / ((long long)L.n[0] * L.n[1]));
+  const long long CT = TX * TY * TZ;
+  const long long ti = tile_index(L, gcx, gcy, gcz, TX, TY, TZ);
+  const long long tile = ti / CT, c = ti - tile * CT;
+  const int G = L.dim * L.dim + 1;
+  for (int g = 0; g < G; ++g) out[((tile * nq + q) * G + g) * CT + c] = nat[idx * G + g];
+}
+
+void launch_geo_to_tiles(const LevelDesc &L, const double *nat, double *tiled, hipStream_t st) {
+  int t[3];
+  tile_cells(L.dim, L.k, t);
+  const int nq = L.dim == 3 ? (L.k + 1) * (L.k + 1) * (L.k + 1) : (L.k + 1) * (L.k + 1);
+  v_fill<double>(tiled, 0.0, geo_table_size(L), st);
+  const long long n = L.cells * nq;
+  hipLaunchKernelGGL(k_geo_to_tiles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, nat, nq, t[0], t[1],
+                     t[2], tiled);
+  MFS_HIP(hipGetLastError());
+}
+
+// Velocity diagonals under a general geometry (A block: eta (|grad N|^2 +
+// (d N / d x_c)^2) [- 2/3 eta (d N / d x_c)^2 compressible]; Laplace:
+// |grad N|^2), grad N = J^{-T} grad_ref N at every quadrature point.  One
+// thread per (cell of one parity colour, local node): cells of a colour share
+// no node, so the launch adds to dst without conflicts.
+template <int DIM, int K, int OP, typename T>
+__global__ void k_vdiag_geo(const LevelDesc L, const int color, const T *__restrict__ eta, const int eta_per_qp,
+                            T *__restrict__ dst) {
+  using B = tab::Basis<double, K>;
+  using C = Cfg<DIM, K>;
+  constexpr int N = K + 1, Q = K + 1, NV = DIM == 3 ? N * N * N : N * N, NQ = C::NQ, G = DIM * DIM + 1;
+  const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+  const long long hx = (L.n[0] - px + 1) / 2, hy = (L.n[1] - py + 1) / 2, hz = DIM == 3 ? (L.n[2] - pz + 1) / 2 : 1;
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= hx * hy * hz * NV) return;
+  const int a = (int)(idx % NV);
+  const long long ci = idx / NV;
+  const int cx = (int)(2 * (ci % hx) + px), cy = (int)(2 * ((ci / hx) % hy) + py),
+            cz = DIM == 3 ? (int)(2 * (ci / (hx * hy)) + pz) : 0;
+  const long long ti = tile_index(L, cx, cy, cz, C::TX, C::TY, C::TZ);
+  const long long tile = ti / C::CT, c = ti - tile * C::CT;
+  const int ai[3] = {a % N, (a / N) % N, a / (N * N)};
+  double acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) acc[d] = 0;
+  for (int q = 0; q < NQ; ++q) {
+    const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
+    double r[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+      double v = 1.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) v *= d == e ? B::G(qi[d], ai[d]) : B::S(qi[d], ai[d]);
+      r[e] = v;
+    }
+    const double *gq = L.geo + (tile * NQ + q) * (long long)G * C::CT + c;
+    double gr[DIM], g2 = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      double v = 0;
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) v += gq[(e * DIM + d) * C::CT] * r[e];
+      gr[d] = v;
+      g2 += v * v;
+    }
+    const double JxW = gq[(G - 1) * C::CT];
+    if constexpr (OP == OP_LDIAG) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) acc[d] += g2 * JxW;
+    } else {
+      const double et = (double)(eta_per_qp ? eta[(tile * NQ + q) * C::CT + c] : eta[tile * C::CT + c]);
+      const double fc = L.compressible ? 1.0 / 3.0 : 1.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) acc[d] += et * (g2 + fc * gr[d] * gr[d]) * JxW;
+    }
+  }
+  const int g[3] = {K * cx + ai[0], K * cy + ai[1], DIM == 3 ? K * cz + ai[2] : 0};
+  const long long node = g[0] + (long long)L.nv[0] * (g[1] + (long long)L.nv[1] * g[2]);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+    if (!is_con(L, d, g[0], g[1], g[2])) dst[d * L.vnodes + node] += (T)acc[d];
+}
+
+// constrained velocity DoFs of a diagonal: 1
+template <typename T>
+__global__ void k_con_one(const LevelDesc L, T *__restrict__ dst) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= L.dim * L.vnodes) return;
+  const int comp = (int)(idx / L.vnodes);
+  const long long node = idx - comp * L.vnodes;
+  const int gx = (int)(node % L.nv[0]), gy = (int)((node / L.nv[0]) % L.nv[1]),
+            gz = (int)(node / ((long long)L.nv[0] * L.nv[1]));
+  if (is_con(L, comp, gx, gy, gz)) dst[idx] = (T)1;
+}
+
+template <int DIM, int K, typename T>
+void launch_vdiag_geo_dk(Op op, const LevelDesc &L, const T *eta, int pq, T *dst, hipStream_t st) {
+  constexpr int N = K + 1, NV = DIM == 3 ? N * N * N : N * N;
+  v_fill<T>(dst, (T)0, L.dim * L.vnodes, st);
+  for (int color = 0; color < (1 << DIM); ++color) {
+    const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+    const long long n = (long long)((L.n[0] - px + 1) / 2) * ((L.n[1] - py + 1) / 2) *
+                        (DIM == 3 ? (L.n[2] - pz + 1) / 2 : 1) * NV;
+    if (n <= 0) continue;
+    const dim3 grid((unsigned)((n + 127) / 128));
+    if (op == OP_LDIAG)
+      hipLaunchKernelGGL((k_vdiag_geo<DIM, K, OP_LDIAG, T>), grid, dim3(128), 0, st, L, color, eta, pq, dst);
+    else
+      hipLaunchKernelGGL((k_vdiag_geo<DIM, K, OP_ADIAG, T>), grid, dim3(128), 0, st, L, color, eta, pq, dst);
+  }
+  const long long nu = L.dim * L.vnodes;
+  hipLaunchKernelGGL(k_con_one<T>, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, st, L, dst);
+  MFS_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_vdiag_geo(Op op, const LevelDesc &L, const T *eta, int pq, T *dst, hipStream_t st) {
+  if (L.dim == 3 && L.k == 2)
+    launch_vdiag_geo_dk<3, 2, T>(op, L, eta, pq, dst, st);
+  else if (L.dim == 3 && L.k == 3)
+    launch_vdiag_geo_dk<3, 3, T>(op, L, eta, pq, dst, st);
+  else if (L.dim == 2 && L.k == 2)
+    launch_vdiag_geo_dk<2, 2, T>(op, L, eta, pq, dst, st);
+  else
+    launch_vdiag_geo_dk<2, 3, T>(op, L, eta, pq, dst, st);
+}
+
+
 // natural [cell][q] (or [cell]) -> tile-blocked; one thread per (cell, q)
 template <typename T>
 __global__ void k_eta_to_tiles(const LevelDesc L, const double *__restrict__ eta_nat, const int per_qp, const int nq,
@@ -1747,9 +1928,12 @@ __global__ void k_eta_qp_to_cell(const LevelDesc L, const S *__restrict__ in, co
   const long long ti = tile_index(L, gcx, gcy, gcz, TX, TY, TZ);
   const long long tile = ti / CT, c = ti - tile * CT;
   double acc = 0, wsum = 0;
+  // cellwise L2 projection onto constants: JxW-weighted mean (box: weights w_q)
+  const int G = L.dim * L.dim + 1;
   for (int q = 0; q < nq; ++q) {
     const int qx = q % Q, qy = (q / Q) % Q, qz = q / (Q * Q);
-    const double w = c_basis[kidx].w[qx] * c_basis[kidx].w[qy] * (L.dim == 3 ? c_basis[kidx].w[qz] : 1.0);
+    const double w = L.geo ? L.geo[((tile * nq + q) * G + G - 1) * CT + c]
+                           : c_basis[kidx].w[qx] * c_basis[kidx].w[qy] * (L.dim == 3 ? c_basis[kidx].w[qz] : 1.0);
     acc += w * (double)in[(tile * nq + q) * CT + c];
     wsum += w;
   }

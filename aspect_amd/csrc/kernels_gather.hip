@@ -146,7 +146,7 @@ template <typename T>
 bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                        int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                        T *split_work) {
-  if (L.dim != 3 || L.k != 2) return false;
+  if (L.dim != 3 || L.k != 2 || L.geo) return false;  // the plane kernel assumes the uniform box
   if (op == OP_ABLOCK && split_work) {
     launch_ablock_split<T>(mode, L, src_u, eta, eta_per_qp, rhs_u, dst_u, split_work, st);
     return true;
@@ -156,7 +156,7 @@ bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
 template <typename T>
 bool launch_ablock_cheb(const LevelDesc &L, T *x, const T *b, const T *eta, int eta_per_qp, const T *dinv, T *d,
                         T f1, T f2, int first, T *split_work, hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || !split_work) return false;
+  if (L.dim != 3 || L.k != 2 || !split_work || L.geo) return false;
   launch_ablock_cheb_split<T>(L, x, b, eta, eta_per_qp, dinv, d, f1, f2, first, split_work, st);
   return true;
 }

@@ -54,6 +54,10 @@ struct LevelDesc {
   int compressible;
   double gravity[3];  // used by OP_RHS only
   int plain_read;     // 1: constrained velocity DoFs are read as stored (read_dof_values_plain)
+  // general geometry (nullptr: the uniform box): per quadrature point J^{-1}
+  // (row-major, d xi_e / d x_d at e * dim + d) and JxW, tile-blocked
+  // [((tile * NQ + q) * (dim * dim + 1) + g) * CT + cell_in_tile]
+  const double *geo;
 };
 
 // Mesh constants of a level in the arithmetic type of a kernel (kernel
@@ -228,6 +232,12 @@ void v_idr_shadow(double *q, int k, long long n, hipStream_t st);
 // zero constrained velocity entries of a level vector
 template <typename T>
 void v_zero_constrained(const LevelDesc &L, T *u, hipStream_t st);
+// natural [cell][q][dim*dim+1] geometry table -> the tile-blocked layout of LevelDesc::geo
+void launch_geo_to_tiles(const LevelDesc &L, const double *nat, double *tiled, hipStream_t st);
+long long geo_table_size(const LevelDesc &L);
+// velocity diagonals (OP_ADIAG, OP_LDIAG) under a general geometry, constrained entries 1
+template <typename T>
+void launch_vdiag_geo(Op op, const LevelDesc &L, const T *eta, int pq, T *dst, hipStream_t st);
 // Free-surface face term of the Stokes operator (active level): dst_u +=
 // sign * F src_u on the faces marked in faces[6]; stab holds dim values per
 // face quadrature point of the marked faces (face order, face cells and

@@ -38,7 +38,7 @@ EXPORTED = [
     "mfs_memory_consumption", "mfs_time_operator", "mfs_set_graph_capture", "mfs_correct_stokes_rhs",
     "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
-    "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle",
+    "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
 ]
 
 
@@ -225,6 +225,16 @@ class StokesMatrixFreeHandler:
         mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
         _check(self.lib.mfs_set_viscosity(self._h, _ptr(eta), mode))
         self._averaging = mode
+
+    def set_geometry(self, level, geo):
+        """Per-quadrature-point geometry of a level: [cells, nq, dim*dim + 1]
+        (J^{-1} row-major, then JxW)."""
+        s = self.sizes(level)
+        d = self.box.dim
+        g = np.ascontiguousarray(geo, dtype=np.float64).reshape(-1)
+        if g.size != s["cells"] * self.n_qp * (d * d + 1):
+            raise ValueError(f"geometry table must hold {s['cells']} x {self.n_qp} x {d * d + 1} values")
+        _check(self.lib.mfs_set_geometry(self._h, level, _ptr(g)))
 
     def set_free_surface(self, faces, stab=None):
         """Free-surface stabilization on the marked faces (x-, x+, y-, y+, z-, z+):

@@ -206,3 +206,113 @@ def fields(cfg: BoxConfig):
         eta_mid, _ = material(cfg, mid)
         s = pressure_scaling(eta_mid, 0.01 * cfg.extent[0])
     return eta, rho, s
+
+
+# ------------------------------------------------------------------ general geometry (mapped boxes)
+def _lobatto(k):
+    """Support points of FE_Q(k) on [0, 1] (Gauss-Lobatto)."""
+    if k == 1:
+        return np.array([0.0, 1.0])
+    inner = np.sort(np.polynomial.legendre.Legendre.basis(k).deriv().roots())
+    return np.concatenate([[0.0], 0.5 * (inner + 1.0), [1.0]])
+
+
+def _lagrange_tables(k):
+    """S[q][i] and G[q][i]: FE_Q(k) basis values and derivatives at the QGauss(k+1) points on [0, 1]."""
+    pts = _lobatto(k)
+    xq, _ = gauss_points(k + 1)
+    S = np.ones((k + 1, k + 1))
+    G = np.zeros((k + 1, k + 1))
+    for i in range(k + 1):
+        for q in range(k + 1):
+            v = 1.0
+            for j in range(k + 1):
+                if j != i:
+                    v *= (xq[q] - pts[j]) / (pts[i] - pts[j])
+            S[q, i] = v
+            d = 0.0
+            for m in range(k + 1):
+                if m == i:
+                    continue
+                t = 1.0 / (pts[i] - pts[m])
+                for j in range(k + 1):
+                    if j != i and j != m:
+                        t *= (xq[q] - pts[j]) / (pts[i] - pts[j])
+                d += t
+            G[q, i] = d
+    return S, G
+
+
+def chunk_map(dim, r_range=(0.55, 1.0), lon_range=(0.0, 0.8), lat_range=(-0.4, 0.4)):
+    """Map of box coordinates xi in [0, 1]^dim onto a chunk in the chart
+    (radius, longitude[, latitude]) of ASPECT's chunk geometry
+    (source/geometry_model/chunk.cc:219-241, push_forward_sphere): 2D annulus
+    sector, 3D spherical chunk; positively oriented."""
+    def f(xi):
+        r = r_range[0] + (r_range[1] - r_range[0]) * xi[..., 0]
+        lon = lon_range[0] + (lon_range[1] - lon_range[0]) * xi[..., 1]
+        if dim == 2:
+            return np.stack([r * np.cos(lon), r * np.sin(lon)], axis=-1)
+        lat = lat_range[0] + (lat_range[1] - lat_range[0]) * xi[..., 2]
+        return np.stack([r * np.cos(lat) * np.cos(lon), r * np.cos(lat) * np.sin(lon), r * np.sin(lat)], axis=-1)
+    return f
+
+
+def affine_map(B, c=None):
+    """x = B xi + c."""
+    B = np.asarray(B, dtype=float)
+    c = np.zeros(B.shape[0]) if c is None else np.asarray(c, dtype=float)
+    return lambda xi: xi @ B.T + c
+
+
+def mapped_nodes(dim, degree, n_cells, mapping):
+    """Physical positions of the FE_Q(k) nodes of every cell under the
+    iso-parametric map with nodes on mapping(xi): [cells, nv, dim]."""
+    pts = _lobatto(degree)
+    ax = [(np.arange(n_cells[d])[:, None] + pts[None, :]) / n_cells[d] for d in range(dim)]
+    N = degree + 1
+    if dim == 2:
+        X = ax[0][None, :, None, :]
+        Y = ax[1][:, None, :, None]
+        X, Y = np.broadcast_arrays(X, Y)
+        xi = np.stack([X, Y], axis=-1).reshape(n_cells[1] * n_cells[0], N * N, 2)
+    else:
+        X = ax[0][None, None, :, None, None, :]
+        Y = ax[1][None, :, None, None, :, None]
+        Z = ax[2][:, None, None, :, None, None]
+        X, Y, Z = np.broadcast_arrays(X, Y, Z)
+        xi = np.stack([X, Y, Z], axis=-1).reshape(n_cells[2] * n_cells[1] * n_cells[0], N ** 3, 3)
+    return mapping(xi)
+
+
+def mapped_geometry(dim, degree, n_cells, mapping):
+    """Per-quadrature-point geometry of the iso-parametric FE_Q(k) map with
+    nodes on mapping(xi), the data MatrixFree stores for a MappingQ:
+    [cells, nq, dim*dim + 1] = J^{-1} row-major (d xi_e / d x_d) and JxW."""
+    S, G = _lagrange_tables(degree)
+    _, w1 = gauss_points(degree + 1)
+    Xn = mapped_nodes(dim, degree, n_cells, mapping)  # [cells, nv, dim]
+    # D_j[q, a] = d phi_a / d xi_j at q (tensor products, x fastest)
+    mats = []
+    for j in range(dim):
+        M = np.ones((1, 1))
+        for d in reversed(range(dim)):
+            M = np.kron(M, G if d == j else S)
+        mats.append(M)
+    wq = np.ones(1)
+    for d in reversed(range(dim)):
+        wq = np.kron(wq, w1)
+    J = np.stack([np.einsum("qa,cai->cqi", Dj, Xn) for Dj in mats], axis=-1)  # [c, q, i(x), j(xi)]
+    Jinv = np.linalg.inv(J)  # [c, q, e(xi), d(x)]
+    jxw = np.linalg.det(J) * wq[None, :]
+    cells, nq = jxw.shape
+    return np.concatenate([Jinv.reshape(cells, nq, dim * dim), jxw[..., None]], axis=-1)
+
+
+def mapped_qp_coordinates(dim, degree, n_cells, mapping):
+    """Physical coordinates of the quadrature points under the iso-parametric map: [cells, nq, dim]."""
+    S, _ = _lagrange_tables(degree)
+    M = np.ones((1, 1))
+    for d in reversed(range(dim)):
+        M = np.kron(M, S)
+    return np.einsum("qa,cai->cqi", M, mapped_nodes(dim, degree, n_cells, mapping))

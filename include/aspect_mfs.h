@@ -158,6 +158,20 @@ int mfs_sizes(const mfs_handle *h, int level, long long out[4]);
 int mfs_set_viscosity(mfs_handle *h, const double *eta_qp, int averaging);
 int mfs_set_pressure_scaling(mfs_handle *h, double pressure_scaling);
 
+/* General geometry of one level: the MappingQ data that MatrixFree stores per
+ * quadrature point (update_gradients | update_JxW_values; active level
+ * stokes_matrix_free_local_smoothing.cc:1643-1656, multigrid levels with the
+ * level mapping :1699-1760): geo
+ * holds, for every cell of the level (lexicographic) and every QGauss(k+1)
+ * point (lexicographic), dim*dim + 1 values: J^{-1} row-major (entry
+ * e*dim + d is d xi_e / d x_d, xi the unit cell coordinates) and JxW (det J
+ * times the quadrature weight).  Every operator, diagonal, right-hand side and
+ * level viscosity projection of that level then uses them (the uniform-box
+ * fast paths are bypassed).  Set every level, then call mfs_set_viscosity;
+ * faces of type MFS_BC_TANGENTIAL and the free-surface term need the box
+ * geometry and are refused. */
+int mfs_set_geometry(mfs_handle *h, int level, const double *geo);
+
 /* Free-surface stabilization of the active-level Stokes operator,
  * StokesOperator::local_apply_boundary_face (matrix_free_operators.cc:563-602):
  * on every face with faces[f] = 1 (order x-, x+, y-, y+, z-, z+) the operator

@@ -402,6 +402,11 @@ def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
 
 
+def get_threads() -> int:
+    """OpenMP thread count the oracle's parallel loops run with."""
+    return int(lib().orc_get_threads())
+
+
 # ------------------------------------------------------------------ mesh deformation
 def mesh_problem(p: Problem, mesh_bc) -> Problem:
     """The problem with the mesh-deformation constraints: per face 0 none,
@@ -449,3 +454,16 @@ def mesh_solve(p, u0, tolerance=1e-10):
     res = np.zeros(2)
     _check(lib().orc_mesh_solve(C.byref(p), _p(u0), C.c_double(tolerance), _p(sol), C.byref(its), _p(res)))
     return sol, its.value, res
+
+
+# ------------------------------------------------------------------ general geometry
+def set_geometry(tables):
+    """Geometry tables of the levels for every following call: a list (level 0
+    first) of [cells, nq, dim*dim + 1] arrays (J^{-1} row-major, JxW), or
+    None to return to the box."""
+    _check(lib().orc_clear_geometry())
+    if tables is None:
+        return
+    for l, t in enumerate(tables):
+        t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+        _check(lib().orc_set_geometry(l, C.c_longlong(t.size), _p(t)))

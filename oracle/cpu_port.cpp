@@ -265,6 +265,13 @@ int orc_set_threads(int n) {
 #endif
   return 0;
 }
+int orc_get_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 // Plan for the finest level of a 3D degree-2 box problem.
 int orc_port_create(const orc_port_problem *pp, void **out) {

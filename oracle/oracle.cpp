@@ -170,9 +170,16 @@ struct Level {
   int np[3] = {1, 1, 1};   // pressure nodes per direction ((k-1)*n+1)
   long long cells = 0, vnodes = 0, pnodes = 0;
   std::vector<uint8_t> vcon;  // dim*vnodes flags: constrained velocity DoF
+  // general geometry (empty: the uniform box): per cell, per quadrature point,
+  // J^{-1} row-major (d xi_e / d x_d at [e * dim + d]) and JxW (with the weight)
+  std::vector<double> geo;
+  int G() const { return dim * dim + 1; }
   long long nu() const { return (long long)dim * vnodes; }
   long long np_() const { return pnodes; }
 };
+
+// per-level geometry tables registered by orc_set_geometry (test infrastructure)
+std::vector<std::vector<double>> g_geometry;
 
 Level make_level(const orc_problem &p, int l) {
   Level L;
@@ -192,6 +199,12 @@ Level make_level(const orc_problem &p, int l) {
     }
   }
   L.cells = (long long)L.n[0] * L.n[1] * L.n[2];
+  if (l < (int)g_geometry.size() && !g_geometry[l].empty()) {
+    const int nq1 = p.degree + 1;
+    const long long need = L.cells * (p.dim == 3 ? nq1 * nq1 * nq1 : nq1 * nq1) * (p.dim * p.dim + 1);
+    if ((long long)g_geometry[l].size() != need) throw std::runtime_error("geometry table size mismatch");
+    L.geo = g_geometry[l];
+  }
   L.vnodes = (long long)L.nv[0] * L.nv[1] * L.nv[2];
   L.pnodes = (long long)L.np[0] * L.np[1] * L.np[2];
   L.vcon.assign((size_t)(L.dim * L.vnodes), 0);
@@ -257,6 +270,30 @@ inline void cell_coords(const Level &L, long long c, int cc[3]) {
   cc[1] = (int)((c / L.n[0]) % L.n[1]);
   cc[2] = (int)(c / ((long long)L.n[0] * L.n[1]));
 }
+// J^{-1} (row-major, e * dim + d) and JxW of cell c at quadrature point q:
+// the general table, or the box values diag(1/h) and |K| w_q
+inline double geometry(const Level &L, const RefElement &re, long long c, int q, double Jinv[9]) {
+  const int dim = L.dim;
+  if (!L.geo.empty()) {
+    const double *g = &L.geo[((size_t)c * re.nq() + q) * L.G()];
+    for (int i = 0; i < dim * dim; ++i) Jinv[i] = g[i];
+    return g[dim * dim];
+  }
+  double vol = 1.0;
+  for (int d = 0; d < dim; ++d) vol *= L.h[d];
+  for (int e = 0; e < dim; ++e)
+    for (int d = 0; d < dim; ++d) Jinv[e * dim + d] = e == d ? 1.0 / L.h[d] : 0.0;
+  return vol * re.qweight(q);
+}
+// physical gradient from reference derivatives: g_d = sum_e Jinv[e][d] r_e
+inline void to_physical(int dim, const double Jinv[9], const double *r, double *g) {
+  for (int d = 0; d < dim; ++d) {
+    double a = 0;
+    for (int e = 0; e < dim; ++e) a += Jinv[e * dim + d] * r[e];
+    g[d] = a;
+  }
+}
+
 inline long long vnode(const Level &L, const int cc[3], int a, int N) {
   int i = a % N, j = (a / N) % N, kk = a / (N * N);
   if (L.dim == 2) kk = 0;
@@ -393,6 +430,7 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
   double JxW0 = 1.0;
   for (int d = 0; d < dim; ++d) JxW0 *= L.h[d];
   const int ncolors = 1 << dim;
+  static const double unit_h[3] = {1.0, 1.0, 1.0};
   for (int color = 0; color < ncolors; ++color) {
 #pragma omp parallel
     {
@@ -417,14 +455,24 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
             }
         if (need_p)
           for (int a = 0; a < npl; ++a) pl[a] = src_p[pidx[a]];
+        // reference derivatives (unit h) under a general geometry, transformed per point below
+        const double *hh = L.geo.empty() ? L.h : unit_h;
         if (need_u)
-          for (int cmp = 0; cmp < dim; ++cmp) K.grad(&u[cmp * nv], L.h, &gu[cmp * dim * nq], work.data());
+          for (int cmp = 0; cmp < dim; ++cmp) K.grad(&u[cmp * nv], hh, &gu[cmp * dim * nq], work.data());
         if (need_p) K.pvalues(pl.data(), pq.data(), work.data());
         // quadrature point physics
         for (int q = 0; q < nq; ++q) {
           // B^T and the Laplace operator have no coefficient
           const double eta = (op == OP_BT || op == OP_LAPLACE) ? 1.0 : in.eta[in.eta_per_qp ? c * nq + q : c];
-          const double JxW = JxW0 * re.qweight(q);
+          double Jinv[9];
+          const double JxW = L.geo.empty() ? JxW0 * re.qweight(q) : geometry(L, re, c, q, Jinv);
+          if (!L.geo.empty() && need_u)
+            for (int cmp = 0; cmp < dim; ++cmp) {
+              double r[3], g[3];
+              for (int e = 0; e < dim; ++e) r[e] = gu[(cmp * dim + e) * nq + q];
+              to_physical(dim, Jinv, r, g);
+              for (int d = 0; d < dim; ++d) gu[(cmp * dim + d) * nq + q] = g[d];
+            }
           // gu[cmp][d][q] = d u_cmp / d x_d
           double eps[3][3] = {{0}};
           double div = 0;
@@ -448,14 +496,23 @@ void cell_loop(const OpInput &in, const CellKernels &K, int op, const double *sr
             for (int a = 0; a < dim; ++a) sig[a][a] -= in.s * pq[q];
           if (op == OP_STOKES) fp[q] = -in.s * div * JxW;
           if (op == OP_MASS) fp[q] = in.s * in.s / eta * pq[q] * JxW;
-          if (op != OP_MASS)
+          if (op != OP_MASS) {
             for (int a = 0; a < dim; ++a)
               for (int b = 0; b < dim; ++b) f[(a * dim + b) * nq + q] = sig[a][b] * JxW;
+            if (!L.geo.empty())  // test with reference derivatives: f_e = sum_d Jinv[e][d] f_d
+              for (int a = 0; a < dim; ++a) {
+                double fd[3], fe[3] = {0, 0, 0};
+                for (int d = 0; d < dim; ++d) fd[d] = f[(a * dim + d) * nq + q];
+                for (int e = 0; e < dim; ++e)
+                  for (int d = 0; d < dim; ++d) fe[e] += Jinv[e * dim + d] * fd[d];
+                for (int e = 0; e < dim; ++e) f[(a * dim + e) * nq + q] = fe[e];
+              }
+          }
         }
         if (op != OP_MASS) {
           std::fill(out_u.begin(), out_u.end(), 0.0);
           for (int cmp = 0; cmp < dim; ++cmp)
-            K.grad_T_add(&f[cmp * dim * nq], L.h, &out_u[cmp * nv], work.data());
+            K.grad_T_add(&f[cmp * dim * nq], hh, &out_u[cmp * nv], work.data());
           for (int cmp = 0; cmp < dim; ++cmp)
             for (int a = 0; a < nv; ++a) {
               size_t g = (size_t)cmp * L.vnodes + vidx[a];
@@ -562,14 +619,17 @@ void laplace_diagonal(const Context &C, int l, double *diag) {
       double acc = 0;
       for (int q = 0; q < nq; ++q) {
         const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
-        double g2 = 0;
+        double r[3] = {0, 0, 0}, gr[3] = {0, 0, 0}, Jinv[9];
         for (int d = 0; d < dim; ++d) {
           double v = 1.0;
-          for (int e = 0; e < dim; ++e)
-            v *= (e == d) ? re.G[qi[e] * N + ai[e]] / L.h[e] : re.S[qi[e] * N + ai[e]];
-          g2 += v * v;
+          for (int e = 0; e < dim; ++e) v *= (e == d) ? re.G[qi[e] * N + ai[e]] : re.S[qi[e] * N + ai[e]];
+          r[d] = v;
         }
-        acc += g2 * JxW0 * re.qweight(q);
+        const double JxW = geometry(L, re, c, q, Jinv);
+        to_physical(dim, Jinv, r, gr);
+        double g2 = 0;
+        for (int d = 0; d < dim; ++d) g2 += gr[d] * gr[d];
+        acc += g2 * JxW;
       }
       const long long gn = vnode(L, cc, a, N);
       for (int cmp = 0; cmp < dim; ++cmp) diag[(size_t)cmp * L.vnodes + gn] += acc;
@@ -600,17 +660,17 @@ void ablock_diagonal(const Context &C, int l, const double *eta, int per_qp, dou
       double acc[3] = {0, 0, 0};
       for (int q = 0; q < nq; ++q) {
         const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
-        double gr[3] = {0, 0, 0};
+        double r[3] = {0, 0, 0}, gr[3] = {0, 0, 0}, Jinv[9];
         for (int d = 0; d < dim; ++d) {
           double v = 1.0;
-          for (int e = 0; e < dim; ++e)
-            v *= (e == d) ? re.G[qi[e] * N + ai[e]] / L.h[e] : re.S[qi[e] * N + ai[e]];
-          gr[d] = v;
+          for (int e = 0; e < dim; ++e) v *= (e == d) ? re.G[qi[e] * N + ai[e]] : re.S[qi[e] * N + ai[e]];
+          r[d] = v;
         }
+        const double JxW = geometry(L, re, c, q, Jinv);
+        to_physical(dim, Jinv, r, gr);
         double g2 = 0;
         for (int d = 0; d < dim; ++d) g2 += gr[d] * gr[d];
         const double et = eta[per_qp ? c * nq + q : c];
-        const double JxW = JxW0 * re.qweight(q);
         for (int cmp = 0; cmp < dim; ++cmp) {
           double v = et * (g2 + gr[cmp] * gr[cmp]);
           if (C.p.compressible) v -= 2.0 / 3.0 * et * gr[cmp] * gr[cmp];
@@ -645,7 +705,8 @@ void mass_diagonal(const Context &C, int l, const double *eta, int per_qp, doubl
         double v = 1.0;
         for (int e = 0; e < dim; ++e) v *= re.SP[qi[e] * NP + ai[e]];
         const double et = eta[per_qp ? c * nq + q : c];
-        acc += s * s / et * v * v * JxW0 * re.qweight(q);
+        double Jinv[9];
+        acc += s * s / et * v * v * geometry(L, re, c, q, Jinv);
       }
       diag[pnode(L, cc, a, NP)] += acc;
     }
@@ -677,16 +738,19 @@ void cell_matrices(const Context &C, int l, const double *eta, int per_qp, doubl
     std::fill(Km, Km + (size_t)npl * npl, 0.0);
     for (int q = 0; q < nq; ++q) {
       const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
-      const double JxW = JxW0 * re.qweight(q);
+      double Jinv[9];
+      const double JxW = geometry(L, re, c, q, Jinv);
       const double et = eta[per_qp ? c * nq + q : c];
       for (int a = 0; a < nv; ++a) {
         const int ai[3] = {a % N, (a / N) % N, a / (N * N)};
+        double r[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) {
           double v = 1.0;
           for (int e = 0; e < dim; ++e)
-            v *= (e == d) ? re.vb.deriv(ai[e], re.qx[qi[e]]) / L.h[e] : re.vb.value(ai[e], re.qx[qi[e]]);
-          gN[a * 3 + d] = v;
+            v *= (e == d) ? re.vb.deriv(ai[e], re.qx[qi[e]]) : re.vb.value(ai[e], re.qx[qi[e]]);
+          r[d] = v;
         }
+        to_physical(dim, Jinv, r, &gN[a * 3]);
       }
       for (int a = 0; a < npl; ++a) {
         const int ai[3] = {a % NP, (a / NP) % NP, a / (NP * NP)};
@@ -889,10 +953,12 @@ std::vector<std::vector<double>> level_viscosity(const Context &C, const double 
       lv[nl - 1][c] = eta[c];
       continue;
     }
-    double s = 0, w = 0;
+    // cellwise L2 projection onto constants: JxW-weighted mean
+    double s = 0, w = 0, Jinv[9];
     for (int q = 0; q < nq; ++q) {
-      s += re.qweight(q) * eta[c * nq + q];
-      w += re.qweight(q);
+      const double wq = Lf.geo.empty() ? re.qweight(q) : geometry(Lf, re, c, q, Jinv);
+      s += wq * eta[c * nq + q];
+      w += wq;
     }
     lv[nl - 1][c] = s / w;
   }
@@ -986,6 +1052,8 @@ std::vector<std::vector<double>> level_viscosity_q1(const Context &C, const doub
   const RefElement &re = C.K.re;
   const int nq = re.nq();
   const Level &Lf = C.levels[nl - 1];
+  for (const Level &L : C.levels)
+    if (!L.geo.empty()) throw std::runtime_error("Q1 viscosity projection: box geometry only");
   double lo = eta_qp[0], hi = eta_qp[0];
   for (long long i = 0; i < Lf.cells * nq; ++i) {
     lo = std::min(lo, eta_qp[i]);
@@ -1837,6 +1905,7 @@ int mesh_solve(const Context &C, const double *u0, double tolerance, double *sol
 void free_surface_apply(const Context &C, int l, const int faces[6], const double *stab, const double *src_u,
                         double *dst_u, double sign, int plain) {
   const Level &L = C.levels[l];
+  if (!L.geo.empty()) throw std::runtime_error("free-surface term: box geometry only");
   const RefElement &re = C.K.re;
   const int dim = L.dim, N = re.N, Q = re.Q;
   const double *tab = stab;
@@ -1906,6 +1975,18 @@ int orc_time_vcycle(const orc_problem *p, const double *eta, int per_qp, int kin
     seconds[0] = std::chrono::duration<double>(t1 - t0).count();
     seconds[1] = std::chrono::duration<double>(t2 - t1).count() / std::max(1, reps);
   })
+}
+// geometry tables of the levels used by every following call (natural order:
+// cells lexicographic, quadrature points lexicographic, dim*dim + 1 values)
+int orc_set_geometry(int level, long long n, const double *geo) {
+  ORC_TRY({
+    if (level < 0 || level > 30) throw std::runtime_error("level out of range");
+    if ((int)g_geometry.size() <= level) g_geometry.resize(level + 1);
+    g_geometry[level].assign(geo, geo + n);
+  })
+}
+int orc_clear_geometry() {
+  ORC_TRY({ g_geometry.clear(); })
 }
 // mesh-deformation pieces: the problem's bc are the mesh constraints
 // (0 none, 1 no normal displacement, 2 zero / prescribed displacement)
@@ -2011,7 +2092,8 @@ int orc_rhs_buoyancy(const orc_problem *p, const double *rho_qp, const double *g
           const int qi[3] = {q % Q, (q / Q) % Q, q / (Q * Q)};
           double v = 1.0;
           for (int e = 0; e < dim; ++e) v *= re.S[qi[e] * N + ai[e]];
-          acc += rho_qp[c * nq + q] * v * J * re.qweight(q);
+          double Jinv[9];
+          acc += rho_qp[c * nq + q] * v * (L.geo.empty() ? J * re.qweight(q) : geometry(L, re, c, q, Jinv));
         }
         const long long gn = vnode(L, cc, a, N);
         for (int cmp = 0; cmp < dim; ++cmp) rhs[(size_t)cmp * L.vnodes + gn] += gvec[cmp] * acc;
