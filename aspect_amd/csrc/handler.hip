@@ -655,6 +655,8 @@ struct mfs_handle {
   int gmesh_bc[6] = {-1, -1, -1, -1, -1, -1};
   // general geometry: tile-blocked tables per level (empty: the uniform box)
   std::vector<DBuf<double>> geo_lv;
+  // work array of the split FP64 Stokes apply on the active level (3D degree 2)
+  DBuf<double> swork_stokes;
 
   ~mfs_handle() {
     for (double *p : vchunks) (void)hipFree(p);
@@ -684,9 +686,18 @@ struct mfs_handle {
   }
 
   // ---- active-level FP64 operators
+  // 3D degree 2 on the box: the Stokes apply runs split (one plane-kernel
+  // launch over all tiles, then node-wise sums) with this work array
+  double *stokes_work() {
+    const LevelDesc &T = Ltop();
+    if (T.dim != 3 || T.k != 2 || T.geo) return nullptr;
+    const long long n = stokes_split_work_size(T);
+    if (swork_stokes.n != n) swork_stokes.alloc(n);
+    return swork_stokes.p;
+  }
   void stokes(Mode mode, const double *x, const double *rhs, double *dst) {
     launch_cell_op<double>(OP_STOKES, mode, Ltop(), x, x + nu, eta_active.p, active_per_qp, rhs,
-                           rhs ? rhs + nu : nullptr, dst, dst + nu, st);
+                           rhs ? rhs + nu : nullptr, dst, dst + nu, st, stokes_work());
     if (fs_on) launch_free_surface(Ltop(), fs_faces, fs_stab.p, x, fs_sv.p, mode == MODE_RESID ? -1.0 : 1.0, dst, st);
   }
   void ablock_active(Mode mode, const double *x, const double *rhs, double *dst) {

@@ -66,9 +66,12 @@ template <int MODE, typename T>
 __global__ void k_gather_tiles(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
                                const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
   const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
-  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware order of the (x, y) blocks: node rows that share lines of the work array meet in one L2
+  const unsigned m = xcd_order(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const unsigned by = m / gridDim.x, bx = m - by * gridDim.x;
+  const unsigned pidx = bx * blockDim.x + threadIdx.x;
   if (pidx >= plane) return;
-  const int row = blockIdx.y;  // gz + nv2 * comp
+  const int row = (int)by;  // gz + nv2 * comp
   const int comp = row / L.nv[2], gz = row - comp * L.nv[2];
   const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
   const int g[3] = {gx, gy, gz};
@@ -93,9 +96,12 @@ __global__ void k_gather_cheb(const LevelDesc L, const T *__restrict__ work, T *
                               const T *__restrict__ dinv, T *__restrict__ d, const T f1, const T f2,
                               const int first) {
   const unsigned plane = (unsigned)(L.nv[0] * L.nv[1]);
-  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware order of the (x, y) blocks: node rows that share lines of the work array meet in one L2
+  const unsigned m = xcd_order(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const unsigned by = m / gridDim.x, bx = m - by * gridDim.x;
+  const unsigned pidx = bx * blockDim.x + threadIdx.x;
   if (pidx >= plane) return;
-  const int row = blockIdx.y;
+  const int row = (int)by;
   const int comp = row / L.nv[2], gz = row - comp * L.nv[2];
   const int gy = (int)(pidx / (unsigned)L.nv[0]), gx = (int)pidx - gy * L.nv[0];
   const int g[3] = {gx, gy, gz};
@@ -106,6 +112,79 @@ __global__ void k_gather_cheb(const LevelDesc L, const T *__restrict__ work, T *
   const T dd = first ? dinv[gi] * r * f2 : f1 * d[gi] + f2 * dinv[gi] * r;
   d[gi] = dd;
   x[gi] = xi + dd;
+}
+
+// Second pass of the split Stokes apply for the pressure: one thread per
+// pressure node sums the at most 8 tiles sharing it from work_p
+// [tile][plz][ply][plx] (candidates per direction: tile g/4 - 1 with local
+// node 4 for g mod 4 == 0, g > 0, and tile g/4 with local node g mod 4) in a
+// fixed order, then applies the mode.
+template <int MODE, typename T>
+__global__ void k_gather_p(const LevelDesc L, const T *__restrict__ work_p, const T *__restrict__ rhs_p,
+                           T *__restrict__ dst_p) {
+  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
+  const unsigned m = xcd_order(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const unsigned by = m / gridDim.x, bx = m - by * gridDim.x;
+  const unsigned pidx = bx * blockDim.x + threadIdx.x;
+  if (pidx >= plane) return;
+  const int gz = (int)by;
+  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
+  const int g[3] = {gx, gy, gz};
+  int tt[3][2], ll[3][2];
+  bool ok[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    tt[d][0] = (g[d] >> 2) - 1;
+    ll[d][0] = 4;
+    ok[d][0] = (g[d] & 3) == 0 && g[d] > 0;
+    tt[d][1] = g[d] >> 2;
+    ll[d][1] = g[d] & 3;
+    ok[d][1] = (g[d] >> 2) < L.ntiles[d];
+  }
+  const int sty = L.ntiles[0], stz = L.ntiles[0] * L.ntiles[1];
+  T val = 0;
+#pragma unroll
+  for (int sz = 0; sz < 2; ++sz)
+#pragma unroll
+    for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+      for (int sx = 0; sx < 2; ++sx) {
+        const bool v = ok[2][sz] && ok[1][sy] && ok[0][sx];
+        const int tile = tt[0][sx] + sty * tt[1][sy] + stz * tt[2][sz];
+        const T c = work_p[v ? tile * 125 + (ll[2][sz] * 5 + ll[1][sy]) * 5 + ll[0][sx] : 0];
+        val += v ? c : (T)0;
+      }
+  const long long gi = pidx + (long long)plane * gz;
+  if constexpr (MODE == MODE_SET)
+    dst_p[gi] = val;
+  else if constexpr (MODE == MODE_RESID)
+    dst_p[gi] = rhs_p[gi] - val;
+  else
+    dst_p[gi] += val;
+}
+
+// split Stokes apply: one launch of the plane kernel over all tiles, then the
+// node-wise sums of the velocity and the pressure
+template <typename T>
+void launch_stokes_split(Mode mode, const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, const T *ru,
+                         const T *rp, T *du, T *dp, T *work, hipStream_t st) {
+  if (split_work_size(L) >= (1LL << 31) || (long long)L.nv[0] * L.nv[1] >= (1LL << 31))
+    throw std::runtime_error("level too large for the split Stokes apply (32-bit work offsets)");
+  T *const work_p = work + split_work_size(L);
+  launch_planes_split_stokes<T>(L, su, sp, eta, pq, work, work_p, st);
+  const dim3 grid((unsigned)(((long long)L.nv[0] * L.nv[1] + 511) / 512), (unsigned)(3 * L.nv[2]));
+  const dim3 gridp((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
+  if (mode == MODE_SET) {
+    hipLaunchKernelGGL((k_gather_tiles<MODE_SET, T>), grid, dim3(512), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_p<MODE_SET, T>), gridp, dim3(256), 0, st, L, work_p, rp, dp);
+  } else if (mode == MODE_RESID) {
+    hipLaunchKernelGGL((k_gather_tiles<MODE_RESID, T>), grid, dim3(512), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_p<MODE_RESID, T>), gridp, dim3(256), 0, st, L, work_p, rp, dp);
+  } else {
+    hipLaunchKernelGGL((k_gather_tiles<MODE_ADD, T>), grid, dim3(512), 0, st, L, work, su, ru, du);
+    hipLaunchKernelGGL((k_gather_p<MODE_ADD, T>), gridp, dim3(256), 0, st, L, work_p, rp, dp);
+  }
+  MFS_HIP(hipGetLastError());
 }
 
 // split A-block apply: per-cell contributions of all tiles in one launch, then the node-wise sum
@@ -141,6 +220,7 @@ void launch_ablock_cheb_split(const LevelDesc &L, T *x, const T *b, const T *eta
 }  // namespace
 
 long long split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * 3 * 729; }
+long long stokes_split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * (3 * 729 + 125); }
 
 template <typename T>
 bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
@@ -149,6 +229,10 @@ bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
   if (L.dim != 3 || L.k != 2 || L.geo) return false;  // the plane kernel assumes the uniform box
   if (op == OP_ABLOCK && split_work) {
     launch_ablock_split<T>(mode, L, src_u, eta, eta_per_qp, rhs_u, dst_u, split_work, st);
+    return true;
+  }
+  if (op == OP_STOKES && split_work) {
+    launch_stokes_split<T>(mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, split_work, st);
     return true;
   }
   return launch_cell_planes<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st);

@@ -212,7 +212,8 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
   const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
   int tx, ty, tz;
   if constexpr (SPLIT) {
-    const int bid = blockIdx.x;
+    // XCD-aware tile order: each XCD runs a contiguous range of tiles
+    const int bid = (int)xcd_order(blockIdx.x, gridDim.x);
     tx = bid % L.ntiles[0];
     ty = (bid / L.ntiles[0]) % L.ntiles[1];
     tz = bid / (L.ntiles[0] * L.ntiles[1]);
@@ -497,8 +498,10 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
       const bool shz = (plz == 0 && tz > 0) || (plz == 4 && tz + 1 < L.ntiles[2]);
       const bool first = !((shx && px) || (shy && py) || (shz && pz));
       const long long gi = in ? gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz) : 0;
-      T pold;
-      if constexpr (MODE == MODE_SET)
+      T pold = 0;
+      if constexpr (SPLIT)
+        ;
+      else if constexpr (MODE == MODE_SET)
         pold = first ? (T)0 : dst_p[gi];
       else if constexpr (MODE == MODE_RESID)
         pold = (first ? rhs_p : dst_p)[gi];
@@ -525,7 +528,10 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
           }
         }
       }
-      if (in) dst_p[gi] = MODE == MODE_RESID ? pold - val : pold + val;
+      if constexpr (SPLIT)
+        dst_p[tile_id * 125 + t] = val;  // in-tile sums [tile][plz][ply][plx]
+      else if (in)
+        dst_p[gi] = MODE == MODE_RESID ? pold - val : pold + val;
     }
   }
 }
@@ -555,6 +561,19 @@ void launch_planes_split(const LevelDesc &L, const T *su, const T *eta, int pq, 
                      (T *)nullptr);
   MFS_HIP(hipGetLastError());
 }
+// The same for the Stokes operator: velocity in-tile sums to work_u as above,
+// pressure in-tile sums to work_p[tile][plz][ply][plx] (5^3 per tile)
+template <typename T>
+void launch_planes_split_stokes(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, T *work_u,
+                                T *work_p, hipStream_t st) {
+  hipLaunchKernelGGL((k_planes<OP_STOKES, PL_MODE_SPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(PL_THREADS), 0, st,
+                     L, 0, su, sp, eta, pq, (const T *)nullptr, (const T *)nullptr, work_u, work_p);
+  MFS_HIP(hipGetLastError());
+}
+template void launch_planes_split_stokes<double>(const LevelDesc &, const double *, const double *, const double *,
+                                                 int, double *, double *, hipStream_t);
+template void launch_planes_split_stokes<float>(const LevelDesc &, const float *, const float *, const float *, int,
+                                                float *, float *, hipStream_t);
 template void launch_planes_split<double>(const LevelDesc &, const double *, const double *, int, double *,
                                           hipStream_t);
 template void launch_planes_split<float>(const LevelDesc &, const float *, const float *, int, float *, hipStream_t);

@@ -100,17 +100,22 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
 
 // 3D Q2 x Q1 Stokes and A-block applies with the plane kernel; returns false
 // (nothing launched) for other (dim, degree, op).
-// split_work (A block only): split_work_size(L) elements; the apply then runs
-// as two launches over all tiles (in-tile node sums to the work array, then
-// node-wise sums over the tiles) instead of eight tile colours.
+// split_work (A block: split_work_size(L) elements; Stokes:
+// stokes_split_work_size(L)): the apply then runs as one launch over all tiles
+// (in-tile node sums to the work array) and node-wise sums over the tiles
+// instead of eight tile colours.
 template <typename T>
 bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                     T *split_work = nullptr);
 long long split_work_size(const LevelDesc &L);
+long long stokes_split_work_size(const LevelDesc &L);
 // First pass of the split apply: every tile in one launch of the plane kernel
 template <typename T>
 void launch_planes_split(const LevelDesc &L, const T *su, const T *eta, int pq, T *work, hipStream_t st);
+template <typename T>
+void launch_planes_split_stokes(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, T *work_u,
+                                T *work_p, hipStream_t st);
 // The plane kernel in tile colours (Stokes or A block, any mode)
 template <typename T>
 bool launch_cell_planes(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,

@@ -155,6 +155,17 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// XCD-aware block order.  The hardware deals the workgroups of a launch
+// round-robin to the 8 XCDs, each with its own L2; block b of nb is renumbered
+// so that XCD x runs the contiguous range [x q + min(x, r), (x + 1) q +
+// min(x + 1, r)) of nb = 8 q + r (a bijection of [0, nb)).  Blocks that touch
+// neighbouring data (tiles sharing the cache lines of their common nodes,
+// node rows sharing lines of a work array) then mostly meet in one L2.
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned nb) {
+  const unsigned q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
 // Node of a 4x4x4 degree-2 tile: component and local indices (9 per direction).
 struct TileNode {
   int comp, lx, ly, lz;
