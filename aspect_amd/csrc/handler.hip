@@ -1993,7 +1993,7 @@ int mfs_ablock_vmult_device_noalloc(mfs_handle *h, int level, const void *d_src,
     need_ptrs({d_src, d_dst}, "mfs_ablock_vmult_device_noalloc");
     const LevelDesc &L = h->L[level];
     const long long esz = h->prec == MFS_GMG_FP32 ? (long long)sizeof(float) : (long long)sizeof(double);
-    const long long need = (L.dim == 3 && L.k == 2) ? split_work_size(L) * esz : 0;
+    const long long need = (L.dim == 3 && L.k == 2) ? split_apply_work_size(L) * esz : 0;
     if (need > kNoAllocScratchBytes)
       throw std::invalid_argument("mfs_ablock_vmult_device_noalloc: level needs " + std::to_string(need) +
                                   " bytes of work array, more than the " + std::to_string(kNoAllocScratchBytes) +

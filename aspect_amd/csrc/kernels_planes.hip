@@ -43,31 +43,20 @@ namespace mfs {
 namespace {
 
 constexpr int PL_THREADS = 192;  // 64 cells x 3 planes
-// internal mode: every tile in one launch, in-tile node sums written to a
-// work array [tile][comp][lz][ly][lx] (summed over tiles by the gather pass)
-constexpr int PL_MODE_SPLIT = 3;
-constexpr int PL_WPASSES = (3 * 729 + PL_THREADS - 1) / PL_THREADS;  // 12 write-out passes
+// internal split modes (every tile in one launch).  PL_SPLIT + m (m =
+// MODE_SET, MODE_RESID, MODE_ADD): the tile-interior nodes (no other tile
+// touches them) are finished with mode m, the in-tile sums of the
+// tile-boundary nodes go to the compact work array [tile][comp][386]
+// (tile_bnd_index).  PL_SPLIT_FULL: every in-tile node sum goes to the work
+// array [tile][comp][lz][ly][lx] (the Chebyshev-fused A-block step, whose
+// node-wise pass streams every DoF anyway).
+constexpr int PL_SPLIT = 3;
+constexpr int PL_SPLIT_FULL = 6;
 
 template <typename T>
 struct PlanesBlocks {  // workgroups per CU the register budget is tuned for
   static constexpr int value = std::is_same<T, double>::value ? 2 : 4;
 };
-
-// tile node handled by thread tid in write-out pass k: t = tid + 192 k =
-// 9 row + lx with row = comp * 81 + lz * 9 + ly (consecutive threads take
-// consecutive x nodes)
-__device__ __forceinline__ TileNode plane_node(int tid, int k) {
-  TileNode n;
-  const int t = tid + PL_THREADS * k;
-  const int row = t / 9;
-  n.valid = t < 3 * 729;
-  n.comp = row >= 162 ? 2 : (row >= 81 ? 1 : 0);
-  const int rr = row - 81 * n.comp;
-  n.lz = rr / 9;
-  n.ly = rr - 9 * n.lz;
-  n.lx = t - 9 * row;
-  return n;
-}
 
 // LDS barrier that also fences the instruction scheduler: without the
 // sched_barriers the compiler sinks the arithmetic on values read before the
@@ -96,20 +85,64 @@ __device__ __forceinline__ T dot3c(T c0, T c1, T c2, T x0, T x1, T x2) {
   return r;
 }
 
-// Tile write-out by node rows.  Row r = tid + 192 pass (r < 243): comp = r / 81,
+// Write-out row r of a tile (243 rows (comp, lz, ly) of 9 nodes along x):
+// the 147 rows with 1 <= ly, lz <= 7 first (comp, lz, ly lexicographic), so
+// that they are the first rows of threads 0..146, then the 96 rows on a z or
+// y face (per comp: lz = 0 and lz = 8 with every ly, then ly = 0 and ly = 8
+// with 1 <= lz <= 7)
+__device__ __forceinline__ void wrow(int r, int &comp, int &lz, int &ly) {
+  if (r < 147) {
+    comp = r / 49;
+    const int q = r - 49 * comp;
+    lz = 1 + q / 7;
+    ly = 1 + q % 7;
+  } else {
+    const int f = r - 147;
+    comp = f / 32;
+    const int q = f - 32 * comp;
+    if (q < 18) {
+      lz = q < 9 ? 0 : 8;
+      ly = q < 9 ? q : q - 9;
+    } else {
+      lz = 1 + ((q - 18) >> 1);
+      ly = (q & 1) ? 8 : 0;
+    }
+  }
+}
+
+// Compact split modes in FP32: the operand of the 7 tile-interior nodes lx =
+// 1..7 of this thread's first write-out row (rows 0..146 are the interior
+// rows, wrow), rhs (RESID) or dst (ADD), is loaded at the start of the kernel
+// so that its latency overlaps the tile's arithmetic.  In FP64 the registers
+// would spill: the write-out loads it.
+template <int MODE, typename T>
+struct PlanePrefetch {
+  static constexpr int N =
+      std::is_same<T, float>::value && (MODE == PL_SPLIT + MODE_RESID || MODE == PL_SPLIT + MODE_ADD) ? 1 : 0;
+};
+
+// Tile write-out by node rows.  Row r = tid + 192 pass (r < 243, wrow): comp,
 // lz, ly; the 9 nodes lx of the row sum their contributing cells from the
 // per-cell contributions xb [comp][kz][jy][i][cell] (zero slot at 81 * 64):
 // per (z, y) candidate pair (node_cands) the cell row's 4 x 3 values go to
-// node lx = 2 cx + i.  Colours, modes and constrained rows as in the line
-// kernel's write-out; SPLIT writes the in-tile sums to the work array
-// [tile][comp][lz][ly][lx] instead.
+// node lx = 2 cx + i.  Colour modes: the lowest colour touching a node
+// stores, later colours add; constrained rows take src (SET), rhs - src
+// (RESID), dst + src (ADD) from their first writer.  Compact split modes:
+// rows on a z or y face, and the two x-face nodes of the other rows, go to the
+// compact work array; the 7 tile-interior nodes of those rows are finished
+// here with the mode (constrained DoFs, only at the upper end of a partial
+// tile, as above).  Full split: every row to the full work array.
 template <int MODE, typename T, bool INTERIOR>
 __device__ __forceinline__ void planes_write_rows(const LevelDesc &L, const T *xb, const int tid, const int tx,
                                                   const int ty, const int tz, const int px, const int py,
                                                   const int pz, const long long tile_id, const T *__restrict__ src_u,
-                                                  const T *__restrict__ rhs_u, T *__restrict__ dst_u) {
+                                                  const T *__restrict__ rhs_u, T *__restrict__ dst_u,
+                                                  T *__restrict__ work_u, const T (&pf)[1][7]) {
+  constexpr int NPF = PlanePrefetch<MODE, T>::N;
   constexpr int CT = 64, ZS = 81 * CT;
-  constexpr bool SPLIT = (MODE == PL_MODE_SPLIT);
+  constexpr bool SPLIT = (MODE >= PL_SPLIT && MODE < PL_SPLIT_FULL);
+  constexpr bool FULL = (MODE == PL_SPLIT_FULL);
+  constexpr int IM = MODE - PL_SPLIT;  // compact split: mode of the tile-interior nodes
   const int ncx = INTERIOR ? 4 : min(4, L.n[0] - 4 * tx), ncy = INTERIOR ? 4 : min(4, L.n[1] - 4 * ty),
             ncz = INTERIOR ? 4 : min(4, L.n[2] - 4 * tz);
   const int sy = L.nv[0], sz = L.nv[0] * L.nv[1];
@@ -117,8 +150,8 @@ __device__ __forceinline__ void planes_write_rows(const LevelDesc &L, const T *x
   for (int pass = 0; pass < 2; ++pass) {
     const int row = tid + PL_THREADS * pass;
     if (row >= 243) break;
-    const int comp = row >= 162 ? 2 : (row >= 81 ? 1 : 0);
-    const int rr = row - 81 * comp, lz = rr / 9, ly = rr - 9 * lz;
+    int comp, lz, ly;
+    wrow(row, comp, lz, ly);
     int oy[2], oz[2];
     bool wy[2], wz[2];
     node_cands(ly, 4, 3 * CT, ncy, oy, wy);
@@ -147,10 +180,40 @@ __device__ __forceinline__ void planes_write_rows(const LevelDesc &L, const T *x
             for (int i = 0; i < 3; ++i) val[2 * ccx + i] += xb[v && ccx < ncx ? base + i * CT + ccx : ZS];
         }
       }
-    if constexpr (SPLIT) {
-      T *const wt = dst_u + tile_id * (3 * 729) + row * 9;
+    if constexpr (FULL) {
+      T *const wt = work_u + tile_id * (3 * 729) + ((comp * 9 + lz) * 9 + ly) * 9;
 #pragma unroll
       for (int lx = 0; lx < 9; ++lx) wt[lx] = val[lx];
+    } else if constexpr (SPLIT) {
+      T *const wt = work_u + tile_id * (3 * kTileBnd) + comp * kTileBnd;
+      if (lz == 0 || lz == 8 || ly == 0 || ly == 8) {
+        const int i0 = tile_bnd_index(0, ly, lz);
+#pragma unroll
+        for (int lx = 0; lx < 9; ++lx) wt[i0 + lx] = val[lx];
+      } else {  // an interior row (only in pass 0: row = tid < 147, wrow)
+        const int i0 = tile_bnd_index(0, ly, lz);
+        wt[i0] = val[0];
+        wt[i0 + 49] = val[8];
+        // tile-interior nodes lx = 1..7: no other tile touches them
+        const int gy = 8 * ty + ly, gz = 8 * tz + lz;
+        const bool rowin = INTERIOR || (gy < L.nv[1] && gz < L.nv[2]);
+        const long long rb =
+            (comp == 0 ? 0 : (comp == 1 ? L.vnodes : 2 * L.vnodes)) + 8 * tx + ((long long)sy * gy + (long long)sz * gz);
+#pragma unroll
+        for (int lx = 1; lx < 8; ++lx) {
+          const int gx = 8 * tx + lx;
+          if (!INTERIOR && !(rowin && gx < L.nv[0])) continue;
+          const long long gi = rb + lx;
+          const bool con = !INTERIOR && con_bf(L, comp, gx, gy, gz);
+          const T add = con ? src_u[gi] : val[lx];
+          if constexpr (IM == MODE_SET)
+            dst_u[gi] = add;
+          else if constexpr (IM == MODE_RESID)
+            dst_u[gi] = (NPF ? pf[0][lx - 1] : rhs_u[gi]) - add;
+          else
+            dst_u[gi] = (NPF ? pf[0][lx - 1] : dst_u[gi]) + add;
+        }
+      }
     } else {
       const int gy = 8 * ty + ly, gz = 8 * tz + lz;
       const bool shy = INTERIOR ? (ly == 0 || ly == 8) : ((ly == 0 && ty > 0) || (ly == 8 && ty + 1 < L.ntiles[1]));
@@ -198,11 +261,13 @@ template <int OP, int MODE, typename T>
 __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
     k_planes(const LevelDesc L, const int color, const T *__restrict__ src_u, const T *__restrict__ src_p,
              const T *__restrict__ eta, const int eta_per_qp, const T *__restrict__ rhs_u,
-             const T *__restrict__ rhs_p, T *__restrict__ dst_u, T *__restrict__ dst_p) {
+             const T *__restrict__ rhs_p, T *__restrict__ dst_u, T *__restrict__ dst_p, T *__restrict__ work_u,
+             T *__restrict__ work_p) {
   using B = tab::Basis<T, 2>;
   constexpr int CT = 64;
   constexpr bool STOKES = (OP == OP_STOKES);
-  constexpr bool SPLIT = (MODE == PL_MODE_SPLIT);
+  constexpr bool SPLIT = (MODE >= PL_SPLIT);  // compact or full split: every tile in one launch
+  constexpr int IM = MODE - PL_SPLIT;
   constexpr int XN = 3 * 9 * 3 * CT;  // [comp][k][plane][cell]; [XN]: zero slot of the write-out gathers
   constexpr int WN = 9 * 3 * CT;      // one component's planes [k][plane][cell]
   __shared__ T sX[XN + 1];            // z fluxes, then x/y test sums, then per-cell node contributions
@@ -281,6 +346,24 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
   T et[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) et[k] = eta[eta_per_qp ? (tile_id * 27 + k + 9 * b) * CT + c : tile_id * CT + c];
+  // compact split modes (FP32): the operand of the tile-interior nodes of
+  // this thread's first write-out row (PlanePrefetch)
+  constexpr int NPF = PlanePrefetch<MODE, T>::N;
+  T pf[1][7];
+  if constexpr (NPF > 0) {
+    int pcomp = 0, plz = 1, ply = 1;
+    wrow(tid, pcomp, plz, ply);
+    const int gy = vy0 + ply, gz = vz0 + plz;
+    const bool rowok = tid < 147 && (interior || (gy < L.nv[1] && gz < L.nv[2]));
+    const long long rb = (pcomp == 0 ? 0 : (pcomp == 1 ? L.vnodes : 2 * L.vnodes)) + vx0 +
+                         ((long long)sy * gy + (long long)sz * gz);
+#pragma unroll
+    for (int lx = 1; lx < 8; ++lx) {
+      const bool ok = rowok && (interior || vx0 + lx < L.nv[0]);
+      const long long gi = ok ? rb + lx : 0;
+      pf[0][lx - 1] = IM == MODE_RESID ? rhs_u[gi] : dst_u[gi];
+    }
+  }
 
   // wave-uniform coefficients: rows and columns of the 1D tables for plane b,
   // and the differentiation matrix with the inverse mesh sizes folded in
@@ -484,9 +567,9 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
   // cells x 3 local x nodes) are read once, each into its node.  Pressure:
   // the threads without a second row (64..188) take one tile pressure node.
   if (interior)
-    planes_write_rows<MODE, T, true>(L, sX, tid, tx, ty, tz, px, py, pz, tile_id, src_u, rhs_u, dst_u);
+    planes_write_rows<MODE, T, true>(L, sX, tid, tx, ty, tz, px, py, pz, tile_id, src_u, rhs_u, dst_u, work_u, pf);
   else
-    planes_write_rows<MODE, T, false>(L, sX, tid, tx, ty, tz, px, py, pz, tile_id, src_u, rhs_u, dst_u);
+    planes_write_rows<MODE, T, false>(L, sX, tid, tx, ty, tz, px, py, pz, tile_id, src_u, rhs_u, dst_u, work_u, pf);
   if constexpr (STOKES) {
     if (tid >= 64 && tid < 64 + 125) {
       const int t = tid - 64;
@@ -500,7 +583,7 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
       const long long gi = in ? gx + (long long)L.np[0] * (gy + (long long)L.np[1] * gz) : 0;
       T pold = 0;
       if constexpr (SPLIT)
-        ;
+        (void)IM;
       else if constexpr (MODE == MODE_SET)
         pold = first ? (T)0 : dst_p[gi];
       else if constexpr (MODE == MODE_RESID)
@@ -529,7 +612,7 @@ __global__ void __launch_bounds__(PL_THREADS, PlanesBlocks<T>::value)
         }
       }
       if constexpr (SPLIT)
-        dst_p[tile_id * 125 + t] = val;  // in-tile sums [tile][plz][ply][plx]
+        work_p[tile_id * 125 + t] = val;  // in-tile sums [tile][plz][ply][plx]
       else if (in)
         dst_p[gi] = MODE == MODE_RESID ? pold - val : pold + val;
     }
@@ -545,38 +628,68 @@ void launch_planes_m(const LevelDesc &L, const T *su, const T *sp, const T *eta,
     const long long nb = hx * hy * hz;
     if (nb <= 0) continue;
     hipLaunchKernelGGL((k_planes<OP, MODE, T>), dim3((unsigned)nb), dim3(PL_THREADS), 0, st, L, color, su, sp, eta,
-                       pq, ru, rp, du, dp);
+                       pq, ru, rp, du, dp, (T *)nullptr, (T *)nullptr);
   }
   MFS_HIP(hipGetLastError());
 }
 
 }  // namespace
 
-// First pass of the split A-block apply with the plane kernel: every tile in
-// one launch, in-tile node sums to work[tile][comp][lz][ly][lx]
+// First pass of the split A-block apply with the plane kernel, every tile in
+// one launch: mode MODE_SET / RESID / ADD (compact split: tile-interior nodes
+// finished, tile-boundary in-tile sums to work[tile][comp][386]) or
+// SPLIT_FULL_SUMS (all in-tile sums to work[tile][comp][lz][ly][lx])
 template <typename T>
-void launch_planes_split(const LevelDesc &L, const T *su, const T *eta, int pq, T *work, hipStream_t st) {
-  hipLaunchKernelGGL((k_planes<OP_ABLOCK, PL_MODE_SPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(PL_THREADS), 0, st,
-                     L, 0, su, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work,
-                     (T *)nullptr);
+void launch_planes_split(int mode, const LevelDesc &L, const T *su, const T *eta, int pq, const T *ru, T *du,
+                         T *work, hipStream_t st) {
+  const dim3 grid((unsigned)L.ntiles_total), block(PL_THREADS);
+  const T *np_ = nullptr;
+  T *nw = nullptr;
+  switch (mode) {
+    case MODE_SET:
+      hipLaunchKernelGGL((k_planes<OP_ABLOCK, PL_SPLIT + MODE_SET, T>), grid, block, 0, st, L, 0, su, np_, eta, pq,
+                         ru, np_, du, nw, work, nw);
+      break;
+    case MODE_RESID:
+      hipLaunchKernelGGL((k_planes<OP_ABLOCK, PL_SPLIT + MODE_RESID, T>), grid, block, 0, st, L, 0, su, np_, eta, pq,
+                         ru, np_, du, nw, work, nw);
+      break;
+    case MODE_ADD:
+      hipLaunchKernelGGL((k_planes<OP_ABLOCK, PL_SPLIT + MODE_ADD, T>), grid, block, 0, st, L, 0, su, np_, eta, pq,
+                         ru, np_, du, nw, work, nw);
+      break;
+    case SPLIT_FULL_SUMS:
+      hipLaunchKernelGGL((k_planes<OP_ABLOCK, PL_SPLIT_FULL, T>), grid, block, 0, st, L, 0, su, np_, eta, pq, np_,
+                         np_, nw, nw, work, nw);
+      break;
+    default:
+      throw std::runtime_error("launch_planes_split: unknown mode");
+  }
   MFS_HIP(hipGetLastError());
 }
-// The same for the Stokes operator: velocity in-tile sums to work_u as above,
-// pressure in-tile sums to work_p[tile][plz][ply][plx] (5^3 per tile)
-template <typename T>
-void launch_planes_split_stokes(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, T *work_u,
-                                T *work_p, hipStream_t st) {
-  hipLaunchKernelGGL((k_planes<OP_STOKES, PL_MODE_SPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(PL_THREADS), 0, st,
-                     L, 0, su, sp, eta, pq, (const T *)nullptr, (const T *)nullptr, work_u, work_p);
+template void launch_planes_split<double>(int, const LevelDesc &, const double *, const double *, int,
+                                          const double *, double *, double *, hipStream_t);
+template void launch_planes_split<float>(int, const LevelDesc &, const float *, const float *, int, const float *,
+                                         float *, float *, hipStream_t);
+
+// The same for the FP64 Stokes operator (compact split): velocity as above,
+// pressure in-tile sums to work_p[tile][plz][ply][plx] (5^3 per tile, all nodes)
+void launch_planes_split_stokes(int mode, const LevelDesc &L, const double *su, const double *sp,
+                                const double *eta, int pq, const double *ru, const double *rp, double *du,
+                                double *work_u, double *work_p, hipStream_t st) {
+  const dim3 grid((unsigned)L.ntiles_total), block(PL_THREADS);
+  double *nd = nullptr;
+  if (mode == MODE_SET)
+    hipLaunchKernelGGL((k_planes<OP_STOKES, PL_SPLIT + MODE_SET, double>), grid, block, 0, st, L, 0, su, sp, eta, pq,
+                       ru, rp, du, nd, work_u, work_p);
+  else if (mode == MODE_RESID)
+    hipLaunchKernelGGL((k_planes<OP_STOKES, PL_SPLIT + MODE_RESID, double>), grid, block, 0, st, L, 0, su, sp, eta,
+                       pq, ru, rp, du, nd, work_u, work_p);
+  else
+    hipLaunchKernelGGL((k_planes<OP_STOKES, PL_SPLIT + MODE_ADD, double>), grid, block, 0, st, L, 0, su, sp, eta, pq,
+                       ru, rp, du, nd, work_u, work_p);
   MFS_HIP(hipGetLastError());
 }
-template void launch_planes_split_stokes<double>(const LevelDesc &, const double *, const double *, const double *,
-                                                 int, double *, double *, hipStream_t);
-template void launch_planes_split_stokes<float>(const LevelDesc &, const float *, const float *, const float *, int,
-                                                float *, float *, hipStream_t);
-template void launch_planes_split<double>(const LevelDesc &, const double *, const double *, int, double *,
-                                          hipStream_t);
-template void launch_planes_split<float>(const LevelDesc &, const float *, const float *, int, float *, hipStream_t);
 
 template <typename T>
 bool launch_cell_planes(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,

@@ -33,6 +33,22 @@ enum Op {
 // SET: dst = A src;  RESID: dst = rhs - A src;  ADD: dst += A src
 enum Mode { MODE_SET = 0, MODE_RESID = 1, MODE_ADD = 2 };
 
+// mode value of launch_planes_split: every in-tile node sum to the full work
+// array [tile][comp][lz][ly][lx] (the Chebyshev-fused A-block step)
+constexpr int SPLIT_FULL_SUMS = 3;
+// Operands of a Jacobi-Chebyshev step fused into the split A-block apply:
+// r = b - A x, d = f2 dinv r (first step) or d = f1 d + f2 dinv r, x += d
+// (constrained rows: r = b - x)
+template <typename T>
+struct ChebArgs {
+  T *x;
+  const T *b;
+  const T *dinv;
+  T *d;
+  T f1, f2;
+  int first;
+};
+
 // Geometry of one uniform box level.  Velocity nodes: (k*n+1) per direction,
 // pressure nodes ((k-1)*n+1); both lexicographic with x fastest; the velocity
 // block is component-major.  Cell data (viscosity) is tile-blocked:
@@ -102,20 +118,27 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
 // (nothing launched) for other (dim, degree, op).
 // split_work (A block: split_work_size(L) elements; Stokes:
 // stokes_split_work_size(L)): the apply then runs as one launch over all tiles
-// (in-tile node sums to the work array) and node-wise sums over the tiles
-// instead of eight tile colours.
+// instead of eight tile colours: nodes inside a tile (no other tile touches
+// them) are finished by that launch, the in-tile sums of the tile-boundary
+// nodes go to the work array and a node-wise pass over the tile-boundary
+// nodes adds the at most 8 tiles sharing each.  split_work_size covers the
+// full in-tile sums of the Chebyshev-fused step (3 x 729 per tile).
 template <typename T>
 bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                     T *split_work = nullptr);
-long long split_work_size(const LevelDesc &L);
+long long split_work_size(const LevelDesc &L);        // A block, also the Chebyshev-fused step
+long long split_apply_work_size(const LevelDesc &L);  // A-block applies (SET / RESID / ADD) only
 long long stokes_split_work_size(const LevelDesc &L);
 // First pass of the split apply: every tile in one launch of the plane kernel
+// (mode: MODE_SET / RESID / ADD for the tile-interior nodes, or
+// SPLIT_FULL_SUMS; Stokes: FP64, SET / RESID / ADD)
 template <typename T>
-void launch_planes_split(const LevelDesc &L, const T *su, const T *eta, int pq, T *work, hipStream_t st);
-template <typename T>
-void launch_planes_split_stokes(const LevelDesc &L, const T *su, const T *sp, const T *eta, int pq, T *work_u,
-                                T *work_p, hipStream_t st);
+void launch_planes_split(int mode, const LevelDesc &L, const T *su, const T *eta, int pq, const T *ru, T *du,
+                         T *work, hipStream_t st);
+void launch_planes_split_stokes(int mode, const LevelDesc &L, const double *su, const double *sp,
+                                const double *eta, int pq, const double *ru, const double *rp, double *du,
+                                double *work_u, double *work_p, hipStream_t st);
 // The plane kernel in tile colours (Stokes or A block, any mode)
 template <typename T>
 bool launch_cell_planes(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,

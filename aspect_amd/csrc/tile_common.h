@@ -166,6 +166,18 @@ __device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned nb) {
   return x * q + min(x, r) + k;
 }
 
+// Compact index of a tile-boundary node (at least one local index 0 or 8) of
+// a 4x4x4 degree-2 tile, per component: the two z faces [lz][ly][lx] (162),
+// then the two y faces without their z-face rows [ly][lz - 1][lx] (126), then
+// the two x faces without the other faces' nodes [lx][lz - 1][ly - 1] (98):
+// 386 values.  Rows along x stay contiguous on the z and y faces.
+constexpr int kTileBnd = 386;
+__device__ __forceinline__ int tile_bnd_index(int lx, int ly, int lz) {
+  if (lz == 0 || lz == 8) return (lz == 8 ? 81 : 0) + 9 * ly + lx;
+  if (ly == 0 || ly == 8) return 162 + (ly == 8 ? 63 : 0) + 9 * (lz - 1) + lx;
+  return 288 + (lx == 8 ? 49 : 0) + 7 * (lz - 1) + (ly - 1);
+}
+
 // Node of a 4x4x4 degree-2 tile: component and local indices (9 per direction).
 struct TileNode {
   int comp, lx, ly, lz;

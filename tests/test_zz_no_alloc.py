@@ -46,9 +46,9 @@ def test_noalloc_matches_allocating_apply(prec, dtype):
 
 
 def test_noalloc_size_limit():
-    # 80x64x64 cells = 5120 tiles: 5120 x 3 x 729 FP64 values exceed the
-    # 72 MiB array (rejected before any launch); the FP32 work array fits
-    h64, rng = _handler("fp64", (5, 4, 4), 5, 3)
+    # 96x96x64 cells = 9216 tiles: 9216 x 3 x 386 FP64 values (85 MB) exceed
+    # the 72 MiB array (rejected before any launch); the FP32 work array fits
+    h64, rng = _handler("fp64", (3, 3, 2), 6, 3)
     n = h64.sizes(h64.top)["nu"]
     src = torch.zeros(n, dtype=torch.float64, device="cuda:0")
     dst = torch.zeros(n, dtype=torch.float64, device="cuda:0")
@@ -56,7 +56,7 @@ def test_noalloc_size_limit():
     with pytest.raises(SolverError, match="scratch array"):
         h64.ablock_vmult_device_noalloc(h64.top, src.data_ptr(), dst.data_ptr())
     del h64, src, dst
-    h32, rng = _handler("fp32", (5, 4, 4), 5, 3)
-    print("noalloc fp32 80x64x64", flush=True)
+    h32, rng = _handler("fp32", (3, 3, 2), 6, 3)
+    print("noalloc fp32 96x96x64", flush=True)
     ref, out = _pair(h32, rng, h32.top, torch.float32)
     assert np.array_equal(ref, out)
