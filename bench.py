@@ -39,7 +39,8 @@ def log(msg):
 
 
 def measured_traffic(size):
-    """HBM bytes per FP64 Stokes apply (all colour dispatches) from the committed
+    """HBM bytes per FP64 Stokes apply (the four dispatches of the split apply:
+    plane kernel, two velocity gathers, pressure gather) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the
     copy-kernel calibration recorded beside it; None when no summary exists."""
     path = os.path.join(ROOT, "profiles", f"stokes_fp64_{size}_traffic.json")
