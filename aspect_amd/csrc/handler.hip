@@ -1245,9 +1245,13 @@ struct mfs_handle {
                                                : gmres(rhs, sol, tol, prm.n_cheap_steps, prm.restart, flexible, pc);
       res.cheap_iterations = ks.its;
       res.final_linear_residual = ks.res;
+      res.cheap_final_residual = ks.res;
       ok = ks.converged;
       history = nullptr;
     }
+    hipEvent_t em;
+    MFS_HIP(hipEventCreate(&em));
+    MFS_HIP(hipEventRecord(em, st));
     if (!ok && prm.n_expensive_steps > 0) {
       for (DBuf<double> *v : {&cu1, &cu2, &cu3, &cu4})
         if (v->n != nu) v->alloc(nu);
@@ -1264,11 +1268,14 @@ struct mfs_handle {
     }
     MFS_HIP(hipEventRecord(e1, st));
     MFS_HIP(hipEventSynchronize(e1));
-    float ms = 0;
+    float ms = 0, msc = 0;
     MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    MFS_HIP(hipEventElapsedTime(&msc, e0, em));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(em);
     res.solve_seconds = ms * 1e-3;
+    res.cheap_seconds = msc * 1e-3;
     res.a_preconditioner_applications = a_apps;
     res.s_preconditioner_applications = s_apps;
     res.converged = ok ? 1 : 0;
@@ -1374,6 +1381,30 @@ void *noalloc_scratch() {
     g_scratch_ptr[dev] = p;
   }
   return g_scratch_ptr[dev];
+}
+
+// Support points of FE_Q(k) on [0, 1] (Gauss-Lobatto; k <= 3 in closed form)
+std::vector<double> fe_q_support(int k) {
+  if (k == 1) return {0.0, 1.0};
+  if (k == 2) return {0.0, 0.5, 1.0};
+  const double a = 0.5 * (1.0 - 1.0 / std::sqrt(5.0));
+  return {0.0, a, 1.0 - a, 1.0};
+}
+
+// Lattice index along one direction of coordinate x on nc cells of size h
+// with FE_Q(k) nodes pts: k c + i for the node at cell c, local point i
+// (points within 1e-6 cell sizes); -1 if x is not a node
+long long lattice_node(double x, double h, long long nc, const std::vector<double> &pts) {
+  const double t = x / h;
+  if (!(t > -1e-6 && t < nc + 1e-6)) return -1;
+  long long c = (long long)std::floor(t + 1e-6);
+  if (c >= nc) c = nc - 1;
+  if (c < 0) c = 0;
+  const double xi = t - (double)c;
+  const int k = (int)pts.size() - 1;
+  for (int i = 0; i <= k; ++i)
+    if (std::fabs(xi - pts[i]) < 1e-6) return k * c + i;
+  return -1;
 }
 
 void create(const mfs_box *b, int prec, mfs_handle **out) {
@@ -1597,6 +1628,53 @@ int mfs_validate_operator_arguments(int op, int mode, const void *src_u, const v
                                     const void *rhs_u, const void *rhs_p, const void *dst_u, const void *dst_p) {
   return guard([&] {
     check_op_args((Op)op, (Mode)mode, src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p);
+    return MFS_OK;
+  });
+}
+
+int mfs_box_dof_index(const mfs_box *box, long long n, const double *points, const int *component,
+                      long long *index) {
+  return guard([&] {
+    if (!box || (n > 0 && (!points || !component || !index)))
+      throw std::invalid_argument("mfs_box_dof_index: null argument");
+    const mfs_box &b = *box;
+    if (b.dim != 2 && b.dim != 3) throw std::invalid_argument("mfs_box_dof_index: dim must be 2 or 3");
+    if (b.degree != 2 && b.degree != 3) throw std::invalid_argument("mfs_box_dof_index: degree must be 2 or 3");
+    if (b.n_levels < 1 || b.n_levels > 16) throw std::invalid_argument("mfs_box_dof_index: n_levels out of range");
+    const int k = b.degree;
+    long long nc[3] = {1, 1, 1}, nv[3] = {1, 1, 1}, npn[3] = {1, 1, 1};
+    double h[3] = {1, 1, 1};
+    for (int d = 0; d < b.dim; ++d) {
+      if (b.coarse[d] < 1 || !(b.extent[d] > 0)) throw std::invalid_argument("mfs_box_dof_index: invalid box");
+      nc[d] = (long long)b.coarse[d] << (b.n_levels - 1);
+      h[d] = b.extent[d] / (double)nc[d];
+      nv[d] = k * nc[d] + 1;
+      npn[d] = (k - 1) * nc[d] + 1;
+    }
+    const long long vnodes = nv[0] * nv[1] * nv[2], nu = b.dim * vnodes, np = npn[0] * npn[1] * npn[2];
+    const std::vector<double> pv = fe_q_support(k), pp = fe_q_support(k - 1);
+    std::vector<uint8_t> seen((size_t)(nu + np), 0);
+    for (long long i = 0; i < n; ++i) {
+      const int c = component[i];
+      if (c < 0 || c > b.dim)
+        throw std::invalid_argument("mfs_box_dof_index: component " + std::to_string(c) + " of DoF " +
+                                    std::to_string(i) + " out of range");
+      const bool vel = c < b.dim;
+      long long g[3] = {0, 0, 0};
+      for (int d = 0; d < b.dim; ++d) {
+        g[d] = lattice_node(points[3 * i + d], h[d], nc[d], vel ? pv : pp);
+        if (g[d] < 0)
+          throw std::invalid_argument("mfs_box_dof_index: support point of DoF " + std::to_string(i) +
+                                      " is not a node of the box");
+      }
+      const long long *m = vel ? nv : npn;
+      const long long lex = g[0] + m[0] * (g[1] + m[1] * g[2]);
+      const long long idx = vel ? c * vnodes + lex : nu + lex;
+      if (seen[(size_t)idx]) throw std::invalid_argument("mfs_box_dof_index: two DoFs at one node (DoF " +
+                                                         std::to_string(i) + ")");
+      seen[(size_t)idx] = 1;
+      index[i] = idx;
+    }
     return MFS_OK;
   });
 }

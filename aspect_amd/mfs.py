@@ -39,6 +39,7 @@ EXPORTED = [
     "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
+    "mfs_box_dof_index",
 ]
 
 
@@ -95,6 +96,26 @@ def _vec(a, n, what):
     return a
 
 
+def box_dof_index(box, points, component):
+    """Position of each DoF (support point [n, dim] or [n, 3], vector component
+    0..dim-1 velocity, dim pressure) in the library's block vector of the
+    box's finest level: the permutation from a deal.II DoF numbering (no
+    device needed).  Raises SolverError for a point off the box's nodes or two
+    DoFs at one node."""
+    lib = load_library()
+    pts = np.asarray(points, dtype=np.float64)
+    n = pts.shape[0]
+    p3 = np.zeros((n, 3))
+    p3[:, : pts.shape[1]] = pts
+    comp = np.ascontiguousarray(component, dtype=np.int32)
+    if comp.shape != (n,):
+        raise ValueError("component must hold one entry per support point")
+    out = np.zeros(n, dtype=np.int64)
+    _check(lib.mfs_box_dof_index(C.byref(box), C.c_longlong(n), _ptr(np.ascontiguousarray(p3)),
+                                 comp.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
 def validate_operator_arguments(op, mode, src_u=None, src_p=None, eta=None, rhs_u=None, rhs_p=None, dst_u=None,
                                 dst_p=None):
     """Host-side pointer check the library runs before every operator launch
@@ -118,6 +139,8 @@ class SolveResult(C.Structure):
         ("r0_norm", C.c_double),
         ("pressure_normalization_adjustment", C.c_double),
         ("solve_seconds", C.c_double),
+        ("cheap_final_residual", C.c_double),
+        ("cheap_seconds", C.c_double),
     ]
 
 
@@ -175,6 +198,8 @@ class SolverOutputs:
     solver_tolerance: float
     r0_norm: float
     solve_seconds: float
+    cheap_final_residual: float = 0.0
+    cheap_seconds: float = 0.0
 
 
 def make_box(dim=3, degree=2, coarse=(1, 1, 1), n_levels=3, extent=(1.0, 1.0, 1.0), bc=(1,) * 6,
@@ -310,7 +335,8 @@ class StokesMatrixFreeHandler:
         out = SolverOutputs(res.initial_nonlinear_residual, res.final_linear_residual,
                             res.pressure_normalization_adjustment, res.cheap_iterations, res.expensive_iterations,
                             res.a_preconditioner_applications, res.s_preconditioner_applications,
-                            bool(res.converged), res.solver_tolerance, res.r0_norm, res.solve_seconds)
+                            bool(res.converged), res.solver_tolerance, res.r0_norm, res.solve_seconds,
+                            res.cheap_final_residual, res.cheap_seconds)
         if rc == MFS_NO_CONVERGENCE and raise_on_failure:
             raise NoConvergence(self.lib.mfs_last_error().decode(), out)
         return sol, out
@@ -535,7 +561,8 @@ class Ensemble:
         outs = [SolverOutputs(r.initial_nonlinear_residual, r.final_linear_residual,
                               r.pressure_normalization_adjustment, r.cheap_iterations, r.expensive_iterations,
                               r.a_preconditioner_applications, r.s_preconditioner_applications, bool(r.converged),
-                              r.solver_tolerance, r.r0_norm, r.solve_seconds) for r in res]
+                              r.solver_tolerance, r.r0_norm, r.solve_seconds, r.cheap_final_residual,
+                              r.cheap_seconds) for r in res]
         return sols, outs, secs.value
 
     def __del__(self):

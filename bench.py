@@ -257,6 +257,8 @@ def main():
                                 cheap_iterations=r["cheap_iterations"],
                                 expensive_iterations=r["expensive_iterations"], converged=bool(r["converged"]),
                                 final_residual=r["final_linear_residual"], r0=r["r0_norm"],
+                                cheap_phase_seconds=r["cheap_seconds"],
+                                cheap_phase_final_residual=r["cheap_final_residual"],
                                 pressure_normalization_adjustment=r["pressure_normalization_adjustment"])
             sols[prec] = sol
             del hs

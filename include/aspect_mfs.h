@@ -109,6 +109,8 @@ typedef struct {
   double r0_norm;
   double pressure_normalization_adjustment;
   double solve_seconds; /* device time of the Krylov solve (hipEvent) */
+  double cheap_final_residual; /* residual norm at the end of the cheap phase */
+  double cheap_seconds;        /* device time of the cheap phase (hipEvent) */
 } mfs_solve_result;
 
 /* Error message of the last failing call on this thread. */
@@ -136,6 +138,20 @@ const char *mfs_last_error(void);
  * takes its vectors by reference). */
 int mfs_validate_operator_arguments(int op, int mode, const void *src_u, const void *src_p, const void *eta,
                                     const void *rhs_u, const void *rhs_p, const void *dst_u, const void *dst_p);
+
+/* The DoF map between a deal.II DoF numbering and this library's vectors
+ * (the reference's setup_dofs distributes and renumbers the velocity and
+ * pressure DoFs hierarchically, local_smoothing.cc:1487-1491 and :1557-1561):
+ * for n DoFs with support points points[3 i + d] (DoFTools::
+ * map_dofs_to_support_points; z ignored in 2D) and vector component
+ * component[i] (0..dim-1 velocity, dim pressure), index[i] = position of the
+ * DoF in the block vector of the box's finest level (velocity component-major,
+ * nodes lexicographic with x fastest, then pressure).  FE_Q(k) support points
+ * are Gauss-Lobatto per cell.  Returns MFS_ERROR if a point is not a node of
+ * the box or two DoFs map to one position.  Host only, needs no device.
+ * No counterpart in the reference. */
+int mfs_box_dof_index(const mfs_box *box, long long n, const double *points, const int *component,
+                      long long *index);
 
 /* setup_dofs(): stokes_matrix_free_local_smoothing.cc:1450-1851 — level meshes,
  * DoF layout, level constraints and device storage. */

@@ -120,3 +120,32 @@ def test_vector_length_check():
     with pytest.raises(ValueError):
         mfs._vec(np.zeros((2, 3)), 6, "rhs")
     assert mfs._vec([1, 2, 3], 3, "rhs").dtype == np.float64
+
+
+@pytest.mark.parametrize("dim,deg,coarse,nl", [(2, 2, (2, 3), 3), (3, 2, (2, 1, 1), 2), (2, 3, (1, 2), 3),
+                                               (3, 3, (1, 1, 1), 2)])
+def test_box_dof_index_recovers_permutation(dim, deg, coarse, nl):
+    # deal.II-style DoF numbering (a random order of support points, the FE_Q
+    # Gauss-Lobatto points of every cell) mapped back to the library's layout
+    import oracle as O
+    from aspect_amd import make_box
+    from aspect_amd.mfs import SolverError, box_dof_index
+    ext = (1.5, 0.75, 2.0)[:dim]
+    box = make_box(dim, deg, coarse, nl, ext, (0,) * 6)
+    p = O.make_problem(dim, deg, coarse, nl, ext, (0,) * 6)
+    s = O.sizes(p, nl - 1)
+    vp = O.velocity_support_points(p, nl - 1)
+    pp = O.pressure_support_points(p, nl - 1)
+    pts = np.concatenate([np.tile(vp, (dim, 1)), pp])
+    comp = np.concatenate([np.repeat(np.arange(dim), s["vnodes"]), np.full(s["np"], dim)])
+    perm = np.random.default_rng(dim * deg).permutation(len(comp))
+    idx = box_dof_index(box, pts[perm], comp[perm])
+    np.testing.assert_array_equal(idx, perm)
+    bad = pts[perm].copy()
+    bad[5, 0] += 0.1 * ext[0] / (coarse[0] << (nl - 1))
+    with pytest.raises(SolverError, match="not a node"):
+        box_dof_index(box, bad, comp[perm])
+    dup = perm.copy()
+    dup[1] = dup[0]
+    with pytest.raises(SolverError, match="two DoFs"):
+        box_dof_index(box, pts[dup], comp[dup])
