@@ -391,7 +391,7 @@ struct GMG {
     }
     for (int l = 0; l <= top; ++l) {
       GMGLevel<T> &g = *lv[l];
-      if (!eta_pq && g.L.dim == 3 && g.L.k == 2 && !g.L.geo) {
+      if (!eta_pq && g.L.k == 2 && !g.L.geo) {
         if (g.ietan.n != g.L.cells) g.ietan.alloc(g.L.cells);
         launch_inv_eta_nat<T>(g.L, g.eta.p, g.ietan.p, st);
       }
@@ -495,7 +495,7 @@ struct GMG {
     T *dinv = kind == 0 ? g.dinvA.p : g.dinvS.p;
     T *r = kind == 0 ? g.ur.p : g.pr.p;
     T *d = kind == 0 ? g.ud.p : g.pd.p;
-    if (kind == 1 && ietan_ok && !eta_pq && g.ietan.p && g.L.dim == 3 && g.L.k == 2) {
+    if (kind == 1 && ietan_ok && !eta_pq && g.ietan.p && g.L.k == 2 && !g.L.geo) {
       smooth_schur_fused(g, ch, x, b, x_zero, steps);
       return;
     }
@@ -2496,6 +2496,7 @@ int mfs_memory_consumption(const mfs_handle *h, long long out[3]) {
 // many small solves run concurrently on the device.
 struct mfs_ensemble {
   std::vector<mfs_handle *> members;
+  std::vector<int> last_its;  // iterations of each member in the previous ensemble solve (scheduling)
   ~mfs_ensemble() {
     for (mfs_handle *h : members) delete h;
   }
@@ -2570,8 +2571,19 @@ int mfs_ensemble_solve(mfs_ensemble *e, const double *const *rhs, double *const 
     const int n = (int)e->members.size();
     std::vector<int> rc(n, MFS_OK);
     for (mfs_handle *h : e->members) MFS_HIP(hipStreamSynchronize(h->st));
+    // longest-first order (LPT): the workers take members by decreasing
+    // expected cost, DoFs times the member's iteration count of the previous
+    // ensemble solve (DoFs alone on the first), so that the member on the
+    // critical path does not start last
+    if ((int)e->last_its.size() != n) e->last_its.assign(n, 1);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (double)e->members[a]->n * e->last_its[a] > (double)e->members[b]->n * e->last_its[b];
+    });
     const auto t0 = std::chrono::steady_clock::now();
-    run_parallel(n, n_threads, [&](int i) {
+    run_parallel(n, n_threads, [&](int k) {
+      const int i = order[k];
       mfs_handle *h = e->members[i];
       DBuf<double> drhs, dsol;
       drhs.alloc(h->n);
@@ -2581,6 +2593,7 @@ int mfs_ensemble_solve(mfs_ensemble *e, const double *const *rhs, double *const 
       HostIO{h}.out(sol[i], dsol.p, h->n);
     });
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < n; ++i) e->last_its[i] = std::max(1, res[i].cheap_iterations + res[i].expensive_iterations);
     for (int i = 0; i < n; ++i)
       if (rc[i] != MFS_OK) return MFS_NO_CONVERGENCE;
     return MFS_OK;

@@ -770,141 +770,92 @@ void launch_dkm(const LevelDesc &L, const T *su, const T *sp, const T *eta, int 
   MFS_HIP(hipGetLastError());
 }
 
-// Pressure mass operator (degree 1, 3D, one viscosity value per cell),
-// evaluated per node: out(n) = s^2 |K| sum over the <= 8 cells c around n of
-// (1/eta_c) sum_m M1(n, m) p_m, M1 the Q1 reference mass matrix (tensor product
-// of [[1/3, 1/6], [1/6, 1/3]], which the 3-point Gauss rule of
+// Pressure mass operator (degree 1, one viscosity value per cell), evaluated
+// per node: out(n) = s^2 |K| sum over the <= 2^dim cells c around n of
+// (1/eta_c) sum_m M1(n, m) p_m, M1 the Q1 reference mass matrix (tensor
+// product of [[1/3, 1/6], [1/6, 1/3]], which the Gauss rule of
 // MassMatrixOperator integrates exactly).  One thread per node and one launch
-// per apply: no tile colours, deterministic by construction.
-template <int MODE, typename T>
-__global__ void k_mass_nodes(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ eta,
-                             const T *__restrict__ rhs, T *__restrict__ dst) {
-  // grid: x over the nodes of an xy plane, y over z (32-bit offsets, checked
-  // by the launcher); arithmetic in T
-  const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
-  const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pidx >= plane) return;
-  const int gz = blockIdx.y;
-  const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
-  const int g[3] = {gx, gy, gz};
-  // the 3^3 pressure values around the node (zero outside the level)
-  T pv[3][3][3];
+// per apply: no tile colours, deterministic by construction.  The cell weight
+// w_c is 1/eta_c: eta from the tile-blocked cellwise table (MASS_TILES) or
+// 1/eta from a natural-order table (MASS_INV).
+constexpr int MASS_TILES = 0, MASS_INV = 1;
+template <int DIM, int SRC, typename T>
+__device__ __forceinline__ T mass_node(const LevelDesc &L, const T *__restrict__ src, const T *__restrict__ ceta,
+                                       const int gx, const int gy, const int gz) {
+  constexpr int NZ = DIM == 3 ? 3 : 1, OZ = DIM == 3 ? 2 : 1;
+  const int sy = L.np[0], sz = L.np[0] * L.np[1];  // pressure node strides
+  const int cy = L.n[0], cz = L.n[0] * L.n[1];     // cell strides
+  const bool inner = gx > 0 && gy > 0 && gx + 1 < L.np[0] && gy + 1 < L.np[1] &&
+                     (DIM == 2 || (gz > 0 && gz + 1 < L.np[2]));
+  const int n0 = gx + sy * gy + sz * gz;  // this node
+  // the 3^dim pressure values around the node (zero outside the level)
+  T pv[NZ][3][3];
 #pragma unroll
-  for (int dz = 0; dz < 3; ++dz)
+  for (int dz = 0; dz < NZ; ++dz)
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx) {
-        const int x = gx - 1 + dx, y = gy - 1 + dy, z = gz - 1 + dz;
-        const bool in = x >= 0 && y >= 0 && z >= 0 && x < L.np[0] && y < L.np[1] && z < L.np[2];
-        const T v = src[in ? x + L.np[0] * (y + L.np[1] * z) : 0];
-        pv[dz][dy][dx] = in ? v : (T)0;
+        const int oz = DIM == 3 ? dz - 1 : 0;
+        const int off = (dx - 1) + sy * (dy - 1) + sz * oz;
+        if (inner) {
+          pv[dz][dy][dx] = src[n0 + off];
+        } else {
+          const int x = gx - 1 + dx, y = gy - 1 + dy, z = gz + oz;
+          const bool in = x >= 0 && y >= 0 && x < L.np[0] && y < L.np[1] && (DIM == 2 || (z >= 0 && z < L.np[2]));
+          const T v = src[in ? n0 + off : 0];
+          pv[dz][dy][dx] = in ? v : (T)0;
+        }
       }
   constexpr T m1[2] = {T(1) / T(3), T(1) / T(6)};  // Q1 mass on [0,1]: 1/3 on the diagonal, 1/6 off it
   T acc = 0;
 #pragma unroll
-  for (int oz = 0; oz < 2; ++oz)
+  for (int oz = 0; oz < OZ; ++oz)
 #pragma unroll
     for (int oy = 0; oy < 2; ++oy)
 #pragma unroll
       for (int ox = 0; ox < 2; ++ox) {
         // cell g - 1 + o; the node is its local node 1 - o, its nodes are pv[o + j]
-        const int c[3] = {g[0] - 1 + ox, g[1] - 1 + oy, g[2] - 1 + oz};
-        const bool valid = c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < L.n[0] && c[1] < L.n[1] && c[2] < L.n[2];
+        const int c[3] = {gx - 1 + ox, gy - 1 + oy, DIM == 3 ? gz - 1 + oz : 0};
+        const bool valid = inner || (c[0] >= 0 && c[1] >= 0 && c[0] < L.n[0] && c[1] < L.n[1] &&
+                                     (DIM == 2 || (c[2] >= 0 && c[2] < L.n[2])));
         T cs = 0;
 #pragma unroll
-        for (int jz = 0; jz < 2; ++jz)
+        for (int jz = 0; jz < OZ; ++jz)
 #pragma unroll
           for (int jy = 0; jy < 2; ++jy)
 #pragma unroll
-            for (int jx = 0; jx < 2; ++jx)
-              cs += m1[(1 - ox) != jx] * m1[(1 - oy) != jy] * m1[(1 - oz) != jz] * pv[oz + jz][oy + jy][ox + jx];
-        const int tx = c[0] >> 2, ty = c[1] >> 2, tz = c[2] >> 2;
-        const int ti = (tx + L.ntiles[0] * (ty + L.ntiles[1] * tz)) * 64 + (c[0] & 3) + 4 * ((c[1] & 3) + 4 * (c[2] & 3));
-        const T e = eta[valid ? ti : 0];
-        acc += valid ? cs / e : (T)0;
-      }
-  using LC = LevelConst<T>;
-  const T v = LC::vol(L) * LC::s(L) * LC::s(L) * acc;
-  const long long i = pidx + (long long)plane * gz;
-  if constexpr (MODE == MODE_SET)
-    dst[i] = v;
-  else if constexpr (MODE == MODE_RESID)
-    dst[i] = rhs[i] - v;
-  else
-    dst[i] += v;
-}
-
-// Pressure mass apply from a natural-order table of inverse cell viscosities
-// (the level operator of the Schur V-cycle): as k_mass_nodes, with the cell
-// weights multiplied instead of divided and, for nodes away from the domain
-// boundary, neighbour and cell offsets that are constants.
-template <typename T>
-__device__ __forceinline__ T mass_inv_node(const LevelDesc &L, const T *__restrict__ src, const T *__restrict__ ieta,
-                                           const int gx, const int gy, const int gz) {
-  const int sy = L.np[0], sz = L.np[0] * L.np[1];      // pressure node strides
-  const int cy = L.n[0], cz = L.n[0] * L.n[1];         // cell strides
-  const bool inner = gx > 0 && gy > 0 && gz > 0 && gx + 1 < L.np[0] && gy + 1 < L.np[1] && gz + 1 < L.np[2];
-  const int n0 = gx + sy * gy + sz * gz;               // this node
-  const int c0 = (gx - 1) + cy * (gy - 1) + cz * (gz - 1);  // cell (gx-1, gy-1, gz-1)
-  T pv[3][3][3];
-  if (inner) {
-    const T *const p0 = src + n0;
-#pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-      for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-        for (int dx = 0; dx < 3; ++dx) pv[dz][dy][dx] = p0[(dx - 1) + sy * (dy - 1) + sz * (dz - 1)];
-  } else {
-#pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-      for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-          const int x = gx - 1 + dx, y = gy - 1 + dy, z = gz - 1 + dz;
-          const bool in = x >= 0 && y >= 0 && z >= 0 && x < L.np[0] && y < L.np[1] && z < L.np[2];
-          const T v = src[in ? n0 + (dx - 1) + sy * (dy - 1) + sz * (dz - 1) : 0];
-          pv[dz][dy][dx] = in ? v : (T)0;
+            for (int jx = 0; jx < 2; ++jx) {
+              const T wz = DIM == 3 ? m1[(1 - oz) != jz] : (T)1;
+              cs += m1[(1 - ox) != jx] * m1[(1 - oy) != jy] * wz * pv[DIM == 3 ? oz + jz : 0][oy + jy][ox + jx];
+            }
+        if constexpr (SRC == MASS_INV) {
+          const T w = ceta[valid ? c[0] + cy * c[1] + cz * c[2] : 0];
+          acc += valid ? cs * w : (T)0;
+        } else {
+          // tile-blocked cellwise viscosity: 4x4x4 tiles in 3D, 8x8 in 2D (64 cells)
+          constexpr int TS = DIM == 3 ? 4 : 8, TSH = DIM == 3 ? 2 : 3;
+          const int tx = c[0] >> TSH, ty = c[1] >> TSH, tz = DIM == 3 ? c[2] >> TSH : 0;
+          const int ti = (tx + L.ntiles[0] * (ty + L.ntiles[1] * tz)) * 64 + (c[0] & (TS - 1)) +
+                         TS * ((c[1] & (TS - 1)) + (DIM == 3 ? TS * (c[2] & (TS - 1)) : 0));
+          const T e = ceta[valid ? ti : 0];
+          acc += valid ? cs / e : (T)0;
         }
-  }
-  constexpr T m1[2] = {T(1) / T(3), T(1) / T(6)};  // Q1 mass on [0,1]: 1/3 on the diagonal, 1/6 off it
-  T acc = 0;
-#pragma unroll
-  for (int oz = 0; oz < 2; ++oz)
-#pragma unroll
-    for (int oy = 0; oy < 2; ++oy)
-#pragma unroll
-      for (int ox = 0; ox < 2; ++ox) {
-        const int c[3] = {gx - 1 + ox, gy - 1 + oy, gz - 1 + oz};
-        const bool valid =
-            inner || (c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < L.n[0] && c[1] < L.n[1] && c[2] < L.n[2]);
-        T cs = 0;
-#pragma unroll
-        for (int jz = 0; jz < 2; ++jz)
-#pragma unroll
-          for (int jy = 0; jy < 2; ++jy)
-#pragma unroll
-            for (int jx = 0; jx < 2; ++jx)
-              cs += m1[(1 - ox) != jx] * m1[(1 - oy) != jy] * m1[(1 - oz) != jz] * pv[oz + jz][oy + jy][ox + jx];
-        const T w = ieta[valid ? c0 + ox + cy * oy + cz * oz : 0];
-        acc += valid ? cs * w : (T)0;
       }
   using LC = LevelConst<T>;
   return LC::vol(L) * LC::s(L) * LC::s(L) * acc;
 }
-template <int MODE, typename T>
-__global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ ieta,
-                           const T *__restrict__ rhs, T *__restrict__ dst) {
-  // grid: x over the nodes of an xy plane, y over z (32-bit offsets, checked
-  // by the launcher)
+
+// grid: x over the nodes of an xy plane, y over z (32-bit offsets, checked by the launcher)
+template <int DIM, int SRC, int MODE, typename T>
+__global__ void k_mass_nodes(const LevelDesc L, const T *__restrict__ src, const T *__restrict__ ceta,
+                             const T *__restrict__ rhs, T *__restrict__ dst) {
   const unsigned plane = (unsigned)(L.np[0] * L.np[1]);
   const unsigned pidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (pidx >= plane) return;
   const int gz = blockIdx.y;
   const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
-  const T v = mass_inv_node<T>(L, src, ieta, gx, gy, gz);
+  const T v = mass_node<DIM, SRC, T>(L, src, ceta, gx, gy, gz);
   const long long i = pidx + (long long)plane * gz;
   if constexpr (MODE == MODE_SET)
     dst[i] = v;
@@ -917,7 +868,7 @@ __global__ void k_mass_inv(const LevelDesc L, const T *__restrict__ src, const T
 //   r = b - M x_in,  d = (first ? 0 : f1 d) + f2 D^-1 r,  x_out = x_in + d.
 // x_in is read around every node, so x_out is a different buffer (the caller
 // alternates the two).
-template <typename T>
+template <int DIM, typename T>
 __global__ void k_mass_cheb(const LevelDesc L, const T *__restrict__ xin, const T *__restrict__ ieta,
                             const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ d,
                             T *__restrict__ xout, T f1, T f2, int first) {
@@ -928,46 +879,72 @@ __global__ void k_mass_cheb(const LevelDesc L, const T *__restrict__ xin, const 
   const int gy = (int)(pidx / (unsigned)L.np[0]), gx = (int)pidx - gy * L.np[0];
   const long long i = pidx + (long long)plane * gz;
   const T bi = b[i], di = dinv[i], dold = first ? T(0) : d[i];
-  const T v = mass_inv_node<T>(L, xin, ieta, gx, gy, gz);
+  const T v = mass_node<DIM, MASS_INV, T>(L, xin, ieta, gx, gy, gz);
   const T r = bi - v;
   const T dd = first ? di * r * f2 : f1 * dold + f2 * di * r;
   d[i] = dd;
   xout[i] = xin[i] + dd;
 }
-// natural-order inverse cell viscosity from a tile-blocked cellwise table (3D, degree 2)
+// natural-order inverse cell viscosity from a tile-blocked cellwise table
+// (degree 2: 4x4x4 tiles in 3D, 8x8 in 2D)
 template <typename T>
 __global__ void k_inv_eta_nat(const LevelDesc L, const T *__restrict__ eta_tiles, T *__restrict__ out) {
   const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (c >= L.cells) return;
   const int cx = (int)(c % L.n[0]), cy = (int)((c / L.n[0]) % L.n[1]), cz = (int)(c / ((long long)L.n[0] * L.n[1]));
-  const long long ti = (cx / 4 + (long long)L.ntiles[0] * (cy / 4 + (long long)L.ntiles[1] * (cz / 4))) * 64 +
-                       (cx & 3) + 4 * ((cy & 3) + 4 * (cz & 3));
+  const int TS = L.dim == 3 ? 4 : 8;
+  const long long ti = (cx / TS + (long long)L.ntiles[0] * (cy / TS + (long long)L.ntiles[1] * (cz / TS))) * 64 +
+                       (cx % TS) + TS * ((cy % TS) + TS * (cz % TS));
   out[c] = (T)1 / eta_tiles[ti];
+}
+
+// node-wise mass kernels: degree 2 (Q1 pressure), box geometry, 32-bit offsets
+inline bool mass_nodes_ok(const LevelDesc &L) {
+  if (L.k != 2 || L.geo) return false;
+  if (L.pnodes >= (1LL << 31) || L.ntiles_total * 64 >= (1LL << 31) || L.cells >= (1LL << 31))
+    throw std::runtime_error("level too large for the node-wise mass apply (32-bit offsets)");
+  return true;
+}
+inline dim3 mass_grid(const LevelDesc &L) {
+  return dim3((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
+}
+template <int SRC, typename T>
+void launch_mass_nodes_src(Mode mode, const LevelDesc &L, const T *src, const T *ceta, const T *rhs, T *dst,
+                           hipStream_t st) {
+  const dim3 grid = mass_grid(L);
+  auto go = [&](auto dimc) {
+    constexpr int D = decltype(dimc)::value;
+    if (mode == MODE_SET)
+      hipLaunchKernelGGL((k_mass_nodes<D, SRC, MODE_SET, T>), grid, dim3(256), 0, st, L, src, ceta, rhs, dst);
+    else if (mode == MODE_RESID)
+      hipLaunchKernelGGL((k_mass_nodes<D, SRC, MODE_RESID, T>), grid, dim3(256), 0, st, L, src, ceta, rhs, dst);
+    else
+      hipLaunchKernelGGL((k_mass_nodes<D, SRC, MODE_ADD, T>), grid, dim3(256), 0, st, L, src, ceta, rhs, dst);
+  };
+  if (L.dim == 3)
+    go(std::integral_constant<int, 3>());
+  else
+    go(std::integral_constant<int, 2>());
+  MFS_HIP(hipGetLastError());
 }
 
 template <typename T>
 bool launch_mass_inv(Mode mode, const LevelDesc &L, const T *src, const T *ieta, const T *rhs, T *dst,
                      hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || !ieta || L.geo) return false;
-  if (L.pnodes >= (1LL << 31)) throw std::runtime_error("level too large for the node-wise mass apply");
-  const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
-  if (mode == MODE_SET)
-    hipLaunchKernelGGL((k_mass_inv<MODE_SET, T>), grid, dim3(256), 0, st, L, src, ieta, rhs, dst);
-  else if (mode == MODE_RESID)
-    hipLaunchKernelGGL((k_mass_inv<MODE_RESID, T>), grid, dim3(256), 0, st, L, src, ieta, rhs, dst);
-  else
-    hipLaunchKernelGGL((k_mass_inv<MODE_ADD, T>), grid, dim3(256), 0, st, L, src, ieta, rhs, dst);
-  MFS_HIP(hipGetLastError());
+  if (!ieta || !mass_nodes_ok(L)) return false;
+  launch_mass_nodes_src<MASS_INV, T>(mode, L, src, ieta, rhs, dst, st);
   return true;
 }
 template <typename T>
 bool launch_mass_cheb(const LevelDesc &L, const T *xin, const T *ieta, const T *b, const T *dinv, T *d, T *xout,
                       T f1, T f2, int first, hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || !ieta) return false;
-  if (L.pnodes >= (1LL << 31)) throw std::runtime_error("level too large for the node-wise mass apply");
+  if (!ieta || !mass_nodes_ok(L)) return false;
   if (xin == xout) throw std::runtime_error("launch_mass_cheb: x_in and x_out must differ");
-  const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
-  hipLaunchKernelGGL((k_mass_cheb<T>), grid, dim3(256), 0, st, L, xin, ieta, b, dinv, d, xout, f1, f2, first);
+  const dim3 grid = mass_grid(L);
+  if (L.dim == 3)
+    hipLaunchKernelGGL((k_mass_cheb<3, T>), grid, dim3(256), 0, st, L, xin, ieta, b, dinv, d, xout, f1, f2, first);
+  else
+    hipLaunchKernelGGL((k_mass_cheb<2, T>), grid, dim3(256), 0, st, L, xin, ieta, b, dinv, d, xout, f1, f2, first);
   MFS_HIP(hipGetLastError());
   return true;
 }
@@ -991,18 +968,10 @@ template void launch_inv_eta_nat<float>(const LevelDesc &, const float *, float 
 template <typename T>
 bool launch_mass_nodes(Mode mode, const LevelDesc &L, const T *src, const T *eta, int eta_per_qp, const T *rhs,
                        T *dst, hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || eta_per_qp || L.geo) return false;
+  if (eta_per_qp || !mass_nodes_ok(L)) return false;
   static_assert(Tile<3, 2>::TX == 4 && Tile<3, 2>::TY == 4 && Tile<3, 2>::TZ == 4, "tile-blocked eta layout");
-  if (L.pnodes >= (1LL << 31) || L.ntiles_total * 64 >= (1LL << 31))
-    throw std::runtime_error("level too large for the node-wise mass apply (32-bit offsets)");
-  const dim3 grid((unsigned)(((long long)L.np[0] * L.np[1] + 255) / 256), (unsigned)L.np[2]);
-  if (mode == MODE_SET)
-    hipLaunchKernelGGL((k_mass_nodes<MODE_SET, T>), grid, dim3(256), 0, st, L, src, eta, rhs, dst);
-  else if (mode == MODE_RESID)
-    hipLaunchKernelGGL((k_mass_nodes<MODE_RESID, T>), grid, dim3(256), 0, st, L, src, eta, rhs, dst);
-  else
-    hipLaunchKernelGGL((k_mass_nodes<MODE_ADD, T>), grid, dim3(256), 0, st, L, src, eta, rhs, dst);
-  MFS_HIP(hipGetLastError());
+  static_assert(Tile<2, 2>::TX == 8 && Tile<2, 2>::TY == 8, "tile-blocked eta layout");
+  launch_mass_nodes_src<MASS_TILES, T>(mode, L, src, eta, rhs, dst, st);
   return true;
 }
 

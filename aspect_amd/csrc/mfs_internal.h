@@ -161,9 +161,9 @@ void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T
                      hipStream_t st, T *work = nullptr);
 long long transfer_work_size(const LevelDesc &Lc, const LevelDesc &Lf);
 
-// Pressure mass apply (3D, degree 2, cellwise viscosity) from a natural-order
-// table of inverse cell viscosities (launch_inv_eta_nat); false if not 3D
-// degree 2 or ieta is null
+// Pressure mass apply (degree 2, 2D or 3D box, cellwise viscosity) from a
+// natural-order table of inverse cell viscosities (launch_inv_eta_nat); false
+// for degree 3, a general geometry, or a null ieta
 template <typename T>
 bool launch_mass_inv(Mode mode, const LevelDesc &L, const T *src, const T *ieta, const T *rhs, T *dst,
                      hipStream_t st);
