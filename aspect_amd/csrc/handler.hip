@@ -234,7 +234,7 @@ struct GMG {
     if (top > 0) xwork.alloc(transfer_work_size(lv[top - 1]->L, lv[top]->L));
     long long sw = 0;
     for (auto &g : lv)
-      if (g->L.dim == 3 && g->L.k == 2) sw = std::max(sw, split_work_size(g->L));
+      sw = std::max(sw, split_work_size(g->L));
     if (sw > 0 && !laplace) swork.alloc(sw);
   }
 
@@ -471,7 +471,7 @@ struct GMG {
     sh->xwork.release();
     const LevelDesc &L0 = sh->lv[0]->L;
     sh->swork.release();
-    if (L0.dim == 3 && L0.k == 2) sh->swork.alloc(split_work_size(L0));
+    if (split_work_size(L0) > 0) sh->swork.alloc(split_work_size(L0));
     shadow = std::move(sh);
   }
 
@@ -2071,7 +2071,7 @@ int mfs_ablock_vmult_device_noalloc(mfs_handle *h, int level, const void *d_src,
     need_ptrs({d_src, d_dst}, "mfs_ablock_vmult_device_noalloc");
     const LevelDesc &L = h->L[level];
     const long long esz = h->prec == MFS_GMG_FP32 ? (long long)sizeof(float) : (long long)sizeof(double);
-    const long long need = (L.dim == 3 && L.k == 2) ? split_apply_work_size(L) * esz : 0;
+    const long long need = split_apply_work_size(L) * esz;
     if (need > kNoAllocScratchBytes)
       throw std::invalid_argument("mfs_ablock_vmult_device_noalloc: level needs " + std::to_string(need) +
                                   " bytes of work array, more than the " + std::to_string(kNoAllocScratchBytes) +

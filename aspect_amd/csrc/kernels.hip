@@ -262,13 +262,22 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
   T *xb = sm;                // exchange buffer, then per-cell contributions
 
   const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
-  const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
-  int b = blockIdx.x;
-  const int tx = 2 * (b % hx) + px;
-  b /= hx;
-  const int ty = 2 * (b % hy) + py;
-  b /= hy;
-  const int tz = 2 * b + pz;
+  int tx, ty, tz;
+  if constexpr (MODE == MODE_TSPLIT) {  // every tile in one launch
+    int b = blockIdx.x;
+    tx = b % L.ntiles[0];
+    b /= L.ntiles[0];
+    ty = b % L.ntiles[1];
+    tz = b / L.ntiles[1];
+  } else {
+    const int hx = (L.ntiles[0] - px + 1) >> 1, hy = (L.ntiles[1] - py + 1) >> 1;
+    int b = blockIdx.x;
+    tx = 2 * (b % hx) + px;
+    b /= hx;
+    ty = 2 * (b % hy) + py;
+    b /= hy;
+    tz = 2 * b + pz;
+  }
   const long long tile_id = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
   const int tid = threadIdx.x;
   const int c = tid % CT;
@@ -770,6 +779,91 @@ void launch_dkm(const LevelDesc &L, const T *su, const T *sp, const T *eta, int 
   MFS_HIP(hipGetLastError());
 }
 
+// Split A-block apply with the general cell kernel (2D degree 2): every tile
+// in one launch writing its in-tile node sums to work[tile][comp][ly][lx]
+// (2 x 17^2 per 8x8-cell tile), then one thread per velocity DoF sums the at
+// most 4 tiles sharing it in a fixed order (per direction: tile g/16 - 1,
+// local node 16 for g mod 16 == 0, g > 0; tile g/16, local node g mod 16 while
+// g/16 < ntiles) and applies the mode (constrained rows take src), or a
+// Jacobi-Chebyshev step (ChebArgs; constrained rows r = b - x), the
+// arithmetic of v_cheb_first / v_cheb_step.
+constexpr int GM2_CHEB = SPLIT_CHEB;
+template <int MODE, typename T>
+__global__ void k_gather_tiles2(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
+                                const T *__restrict__ rhs_u, T *__restrict__ dst_u, const ChebArgs<T> ch) {
+  using C = Cfg<2, 2>;
+  constexpr int SX = 2 * C::TX, SY = 2 * C::TY;  // tile node spans
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= 2 * L.vnodes) return;
+  const int comp = (int)(i / L.vnodes);
+  const long long node = i - comp * L.vnodes;
+  const int gy = (int)(node / L.nv[0]), gx = (int)(node - (long long)gy * L.nv[0]);
+  int tt[2][2], ll[2][2];
+  bool ok[2][2];
+  const int g[2] = {gx, gy}, sp[2] = {SX, SY};
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    tt[d][0] = g[d] / sp[d] - 1;
+    ll[d][0] = sp[d];
+    ok[d][0] = g[d] % sp[d] == 0 && g[d] > 0;
+    tt[d][1] = g[d] / sp[d];
+    ll[d][1] = g[d] % sp[d];
+    ok[d][1] = g[d] / sp[d] < L.ntiles[d];
+  }
+  T val = 0;
+#pragma unroll
+  for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx) {
+      const bool v = ok[1][sy] && ok[0][sx];
+      const long long tile = tt[0][sx] + (long long)L.ntiles[0] * tt[1][sy];
+      const long long wi = v ? (tile * 2 + comp) * C::VN + ll[0][sx] + C::VX * ll[1][sy] : 0;
+      const T c = work[wi];
+      val += v ? c : (T)0;
+    }
+  const bool con = is_con(L, comp, gx, gy, 0);
+  if constexpr (MODE == GM2_CHEB) {
+    const T xi = ch.x[i];
+    const T r = ch.b[i] - (con ? xi : val);
+    const T dd = ch.first ? ch.dinv[i] * r * ch.f2 : ch.f1 * ch.d[i] + ch.f2 * ch.dinv[i] * r;
+    ch.d[i] = dd;
+    ch.x[i] = xi + dd;
+  } else {
+    const T vv = con ? src_u[i] : val;
+    if constexpr (MODE == MODE_SET)
+      dst_u[i] = vv;
+    else if constexpr (MODE == MODE_RESID)
+      dst_u[i] = rhs_u[i] - vv;
+    else
+      dst_u[i] += vv;
+  }
+}
+
+template <typename T>
+void launch_ablock_split2(int mode, const LevelDesc &L, const T *su, const T *eta, int pq, const T *ru, T *du,
+                          T *work, const ChebArgs<T> &ch, hipStream_t st) {
+  using C = Cfg<2, 2>;
+  if (L.ntiles_total * 2 * C::VN >= (1LL << 40)) throw std::runtime_error("level too large for the split apply");
+  const size_t smem = (size_t)C::SMEM * sizeof(T);
+  hipLaunchKernelGGL((k_cell<2, 2, OP_ABLOCK, MODE_TSPLIT, T>), dim3((unsigned)L.ntiles_total), dim3(C::THREADS),
+                     smem, st, L, 0, su, (const T *)nullptr, eta, pq, (const T *)nullptr, (const T *)nullptr, work,
+                     (T *)nullptr);
+  const dim3 grid((unsigned)((2 * L.vnodes + 255) / 256));
+  if (mode == MODE_SET)
+    hipLaunchKernelGGL((k_gather_tiles2<MODE_SET, T>), grid, dim3(256), 0, st, L, work, su, ru, du, ch);
+  else if (mode == MODE_RESID)
+    hipLaunchKernelGGL((k_gather_tiles2<MODE_RESID, T>), grid, dim3(256), 0, st, L, work, su, ru, du, ch);
+  else if (mode == MODE_ADD)
+    hipLaunchKernelGGL((k_gather_tiles2<MODE_ADD, T>), grid, dim3(256), 0, st, L, work, su, ru, du, ch);
+  else
+    hipLaunchKernelGGL((k_gather_tiles2<GM2_CHEB, T>), grid, dim3(256), 0, st, L, work, su, ru, du, ch);
+  MFS_HIP(hipGetLastError());
+}
+template void launch_ablock_split2<double>(int, const LevelDesc &, const double *, const double *, int,
+                                           const double *, double *, double *, const ChebArgs<double> &, hipStream_t);
+template void launch_ablock_split2<float>(int, const LevelDesc &, const float *, const float *, int, const float *,
+                                          float *, float *, const ChebArgs<float> &, hipStream_t);
+
 // Pressure mass operator (degree 1, one viscosity value per cell), evaluated
 // per node: out(n) = s^2 |K| sum over the <= 2^dim cells c around n of
 // (1/eta_c) sum_m M1(n, m) p_m, M1 the Q1 reference mass matrix (tensor
@@ -1177,6 +1271,10 @@ void launch_cell_op(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
   check_op_args(op, mode, src_u, src_p, eta, rhs_u, rhs_p, dst_u, dst_p);
   if (launch_cell_q2<T>(op, mode, L, src_u, src_p, eta, eta_per_qp, rhs_u, rhs_p, dst_u, dst_p, st, split_work))
     return;
+  if (op == OP_ABLOCK && split_work && L.dim == 2 && L.k == 2 && !L.geo) {
+    launch_ablock_split2<T>(mode, L, src_u, eta, eta_per_qp, rhs_u, dst_u, split_work, ChebArgs<T>{}, st);
+    return;
+  }
   if ((op == OP_ADIAG || op == OP_LDIAG) && L.geo) {
     launch_vdiag_geo<T>(op, L, eta, eta_per_qp, dst_u, st);
     return;

@@ -305,8 +305,17 @@ void launch_ablock_cheb_split(const LevelDesc &L, T *x, const T *b, const T *eta
 
 }  // namespace
 
-long long split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * 3 * 729; }
-long long split_apply_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * 3 * kTileBnd; }
+// work arrays of the split A-block applies (0: the level has none): 3D
+// degree 2 full (Chebyshev-fused step) and compact (applies), 2D degree 2
+// full in-tile sums (2 x 17^2 per tile, both uses); none on a mapped level
+long long split_work_size(const LevelDesc &L) {
+  if (L.k != 2 || L.geo) return 0;
+  return L.dim == 3 ? (long long)L.ntiles_total * 3 * 729 : (long long)L.ntiles_total * 2 * 289;
+}
+long long split_apply_work_size(const LevelDesc &L) {
+  if (L.k != 2 || L.geo) return 0;
+  return L.dim == 3 ? (long long)L.ntiles_total * 3 * kTileBnd : (long long)L.ntiles_total * 2 * 289;
+}
 long long stokes_split_work_size(const LevelDesc &L) { return (long long)L.ntiles_total * (3 * kTileBnd + 125); }
 
 template <typename T>
@@ -329,7 +338,12 @@ bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const 
 template <typename T>
 bool launch_ablock_cheb(const LevelDesc &L, T *x, const T *b, const T *eta, int eta_per_qp, const T *dinv, T *d,
                         T f1, T f2, int first, T *split_work, hipStream_t st) {
-  if (L.dim != 3 || L.k != 2 || !split_work || L.geo) return false;
+  if (L.k != 2 || !split_work || L.geo) return false;
+  if (L.dim == 2) {
+    const ChebArgs<T> ch{x, b, dinv, d, f1, f2, first};
+    launch_ablock_split2<T>(SPLIT_CHEB, L, (const T *)x, eta, eta_per_qp, b, x, split_work, ch, st);
+    return true;
+  }
   launch_ablock_cheb_split<T>(L, x, b, eta, eta_per_qp, dinv, d, f1, f2, first, split_work, st);
   return true;
 }

@@ -33,6 +33,11 @@ enum Op {
 // SET: dst = A src;  RESID: dst = rhs - A src;  ADD: dst += A src
 enum Mode { MODE_SET = 0, MODE_RESID = 1, MODE_ADD = 2 };
 
+// mode value of launch_ablock_split2: a fused Jacobi-Chebyshev step
+constexpr int SPLIT_CHEB = 3;
+// internal mode of the general cell kernel (A block, 2D degree 2): every tile
+// in one launch, in-tile node sums to work[tile][comp][lz][ly][lx]
+constexpr int MODE_TSPLIT = 3;
 // mode value of launch_planes_split: every in-tile node sum to the full work
 // array [tile][comp][lz][ly][lx] (the Chebyshev-fused A-block step)
 constexpr int SPLIT_FULL_SUMS = 3;
@@ -127,6 +132,12 @@ template <typename T>
 bool launch_cell_q2(Op op, Mode mode, const LevelDesc &L, const T *src_u, const T *src_p, const T *eta,
                     int eta_per_qp, const T *rhs_u, const T *rhs_p, T *dst_u, T *dst_p, hipStream_t st,
                     T *split_work = nullptr);
+// Split A-block apply on a 2D degree-2 level (general cell kernel, one launch
+// over all tiles, then a node-wise pass): mode MODE_SET / RESID / ADD, or
+// SPLIT_CHEB with ch (a fused Jacobi-Chebyshev step; su = ch.x)
+template <typename T>
+void launch_ablock_split2(int mode, const LevelDesc &L, const T *su, const T *eta, int pq, const T *ru, T *du,
+                          T *work, const ChebArgs<T> &ch, hipStream_t st);
 long long split_work_size(const LevelDesc &L);        // A block, also the Chebyshev-fused step
 long long split_apply_work_size(const LevelDesc &L);  // A-block applies (SET / RESID / ADD) only
 long long stokes_split_work_size(const LevelDesc &L);

@@ -67,6 +67,8 @@ __device__ __forceinline__ bool is_con(const LevelDesc &L, int comp, int gx, int
 // contributing cells in a fixed order (deterministic) and is written out:
 // the lowest tile colour touching a node stores, later colours add.
 // Constrained rows follow the identity semantics of the operator.
+// MODE_TSPLIT: the in-tile sums go to the work array dst_u[tile][comp][node]
+// instead (constraints applied by the node-wise pass).
 template <int DIM, int K, int OP, int MODE, typename T, int NTH>
 __device__ __forceinline__ void tile_write_u(const LevelDesc &L, const T *xb, const int tid, const int tx,
                                              const int ty, const int tz, const int px, const int py, const int pz,
@@ -105,6 +107,10 @@ __device__ __forceinline__ void tile_write_u(const LevelDesc &L, const T *xb, co
           val += xb[(((comp * ZN + iiz[a]) * N + iiy[bb]) * N + iix[e]) * CT + cell];
         }
       }
+    }
+    if constexpr (MODE == MODE_TSPLIT) {  // in-tile sum to the work array (dst_u)
+      dst_u[(tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz)) * (DIM * C::VN) + t] = val;
+      continue;
     }
     const bool shx = (lx == 0 && tx > 0) || (lx == C::VX - 1 && tx + 1 < L.ntiles[0]);
     const bool shy = (ly == 0 && ty > 0) || (ly == C::VY - 1 && ty + 1 < L.ntiles[1]);
