@@ -1568,8 +1568,9 @@ __device__ __forceinline__ bool con_node(const LevelDesc &L, int comp, int gx, i
   return con;
 }
 
-// Separable 3D transfers: the tensor-product prolongation P = Px Py Pz (and
-// R = P^T) applied one direction per pass through two work arrays.  Each pass
+// Separable transfers (degree 2): the tensor-product prolongation P = Px Py Pz
+// (and R = P^T) applied one direction per pass through two work arrays (2D:
+// Px Py, one work array).  Each pass
 // is a 1D stencil along one axis of an (n0, n1, n2) array per component:
 // restriction, coarse node m takes the fine nodes 2m + o (a coarse vertex: o = 0
 // (1), +-1 (3/8), +-3 (-1/8) for KK = 2, o = 0 (1), +-1 (1/2) for KK = 1; a cell
@@ -1676,7 +1677,51 @@ template <typename T>
 static void run_sep_transfer(bool prolong, int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src, T *dst,
                              bool add, T *work, hipStream_t st) {
   const bool vel = kind == 0;
-  const int ncomp = vel ? 3 : 1;
+  const int ncomp = vel ? Lf.dim : 1;
+  if (Lf.dim == 2) {  // two passes through one work array (c, f)
+    Dims3 dc, df, d1;
+    for (int d = 0; d < 3; ++d) {
+      dc.n[d] = vel ? Lc.nv[d] : Lc.np[d];
+      df.n[d] = vel ? Lf.nv[d] : Lf.np[d];
+    }
+    if ((long long)df.n[0] * df.n[1] * ncomp >= (1LL << 31))
+      throw std::runtime_error("level too large for the separable transfers (32-bit offsets)");
+    d1 = prolong ? dc : df;
+    if (prolong)
+      d1.n[1] = df.n[1];  // coarse -> (c, f) -> fine
+    else
+      d1.n[0] = dc.n[0];  // fine -> (c, f) -> coarse
+    auto blocks = [&](const Dims3 &d) {
+      return dim3((unsigned)(((long long)d.n[0] * d.n[1] + 255) / 256), (unsigned)ncomp);
+    };
+    if (!prolong) {
+      if (vel) {
+        hipLaunchKernelGGL((k_restrict_pass<2, T>), blocks(d1), dim3(256), 0, st, Lc, Lf, 0, df, d1, ncomp, 1, 0, src,
+                           work);
+        hipLaunchKernelGGL((k_restrict_pass<2, T>), blocks(dc), dim3(256), 0, st, Lc, Lf, 1, d1, dc, ncomp, 0, 1,
+                           (const T *)work, dst);
+      } else {
+        hipLaunchKernelGGL((k_restrict_pass<1, T>), blocks(d1), dim3(256), 0, st, Lc, Lf, 0, df, d1, ncomp, 0, 0, src,
+                           work);
+        hipLaunchKernelGGL((k_restrict_pass<1, T>), blocks(dc), dim3(256), 0, st, Lc, Lf, 1, d1, dc, ncomp, 0, 0,
+                           (const T *)work, dst);
+      }
+    } else {
+      if (vel) {
+        hipLaunchKernelGGL((k_prolong_pass<2, T>), blocks(d1), dim3(256), 0, st, Lc, Lf, 1, dc, d1, ncomp, 1, 0, 0, src,
+                           work);
+        hipLaunchKernelGGL((k_prolong_pass<2, T>), blocks(df), dim3(256), 0, st, Lc, Lf, 0, d1, df, ncomp, 0, 1,
+                           add ? 1 : 0, (const T *)work, dst);
+      } else {
+        hipLaunchKernelGGL((k_prolong_pass<1, T>), blocks(d1), dim3(256), 0, st, Lc, Lf, 1, dc, d1, ncomp, 0, 0, 0, src,
+                           work);
+        hipLaunchKernelGGL((k_prolong_pass<1, T>), blocks(df), dim3(256), 0, st, Lc, Lf, 0, d1, df, ncomp, 0, 0,
+                           add ? 1 : 0, (const T *)work, dst);
+      }
+    }
+    MFS_HIP(hipGetLastError());
+    return;
+  }
   Dims3 dc, df;
   for (int d = 0; d < 3; ++d) {
     dc.n[d] = vel ? Lc.nv[d] : Lc.np[d];
@@ -1737,7 +1782,7 @@ long long transfer_work_size(const LevelDesc &Lc, const LevelDesc &Lf) {
 template <typename T>
 void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_c, T *dst_f, bool add,
                        hipStream_t st, T *work) {
-  if (work && Lf.dim == 3 && Lf.k == 2) {
+  if (work && Lf.k == 2) {
     run_sep_transfer<T>(true, kind, Lc, Lf, src_c, dst_f, add, work, st);
     return;
   }
@@ -1753,7 +1798,7 @@ void launch_prolongate(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const
 template <typename T>
 void launch_restrict(int kind, const LevelDesc &Lc, const LevelDesc &Lf, const T *src_f, T *dst_c,
                      hipStream_t st, T *work) {
-  if (work && Lf.dim == 3 && Lf.k == 2) {
+  if (work && Lf.k == 2) {
     run_sep_transfer<T>(false, kind, Lc, Lf, src_f, dst_c, false, work, st);
     return;
   }
