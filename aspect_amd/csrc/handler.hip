@@ -536,6 +536,32 @@ struct GMG {
   // in x.
   void smooth_schur_fused(GMGLevel<T> &g, const ChebInfo &ch, T *x, const T *b, bool x_zero, int steps) {
     const bool inner = ch.degree >= 2 && std::fabs(ch.delta) >= 1e-40;
+    {  // a level that fits one workgroup: the whole sequence in one launch
+      MassChebSeq seq{};
+      const double inv_theta = 1.0 / ch.theta;
+      bool fits = true;
+      for (int s = 0; s < steps && fits; ++s) {
+        auto push = [&](double f1, double f2, int first) {
+          if (seq.n >= MassChebSeq::MAX) {
+            fits = false;
+            return;
+          }
+          seq.f1[seq.n] = f1;
+          seq.f2[seq.n] = f2;
+          seq.first[seq.n++] = first;
+        };
+        push(0.0, inv_theta, 1);
+        if (!inner) continue;
+        const double sigma = ch.theta / ch.delta;
+        double rho_old = ch.delta / ch.theta;
+        for (int k = 1; k < ch.degree; ++k) {
+          const double rho = 1.0 / (2.0 * sigma - rho_old);
+          push(rho * rho_old, 2.0 * rho / ch.delta, 0);
+          rho_old = rho;
+        }
+      }
+      if (fits && launch_mass_smooth_small<T>(g.L, x, b, g.dinvS.p, g.ietan.p, seq, x_zero, st)) return;
+    }
     int swaps = 0;
     for (int s = 0; s < steps; ++s) swaps += (x_zero && s == 0 ? 0 : 1) + (inner ? ch.degree - 1 : 0);
     T *cur = x, *alt = g.pr.p;
@@ -1427,6 +1453,8 @@ void create(const mfs_box *b, int prec, mfs_handle **out) {
     throw std::runtime_error("mfs_create: no HIP device available");
   upload_basis_tables();
   (void)noalloc_scratch();
+  static std::once_flag smoother_once;
+  std::call_once(smoother_once, init_small_smoother);
   auto h = std::make_unique<mfs_handle>();
   h->box = *b;
   for (int d = b->dim; d < 3; ++d) {

@@ -884,23 +884,28 @@ __device__ __forceinline__ T mass_node(const LevelDesc &L, const T *__restrict__
   const int n0 = gx + sy * gy + sz * gz;  // this node
   // the 3^dim pressure values around the node (zero outside the level)
   T pv[NZ][3][3];
+  if (inner) {  // away from the level boundary: constant offsets from the node
+    const T *const p0 = src + n0;
 #pragma unroll
-  for (int dz = 0; dz < NZ; ++dz)
+    for (int dz = 0; dz < NZ; ++dz)
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
+      for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
-        const int oz = DIM == 3 ? dz - 1 : 0;
-        const int off = (dx - 1) + sy * (dy - 1) + sz * oz;
-        if (inner) {
-          pv[dz][dy][dx] = src[n0 + off];
-        } else {
+        for (int dx = 0; dx < 3; ++dx) pv[dz][dy][dx] = p0[(dx - 1) + sy * (dy - 1) + sz * (DIM == 3 ? dz - 1 : 0)];
+  } else {
+#pragma unroll
+    for (int dz = 0; dz < NZ; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const int oz = DIM == 3 ? dz - 1 : 0;
           const int x = gx - 1 + dx, y = gy - 1 + dy, z = gz + oz;
           const bool in = x >= 0 && y >= 0 && x < L.np[0] && y < L.np[1] && (DIM == 2 || (z >= 0 && z < L.np[2]));
-          const T v = src[in ? n0 + off : 0];
+          const T v = src[in ? n0 + (dx - 1) + sy * (dy - 1) + sz * oz : 0];
           pv[dz][dy][dx] = in ? v : (T)0;
         }
-      }
+  }
   constexpr T m1[2] = {T(1) / T(3), T(1) / T(6)};  // Q1 mass on [0,1]: 1/3 on the diagonal, 1/6 off it
   T acc = 0;
 #pragma unroll
@@ -924,7 +929,8 @@ __device__ __forceinline__ T mass_node(const LevelDesc &L, const T *__restrict__
               cs += m1[(1 - ox) != jx] * m1[(1 - oy) != jy] * wz * pv[DIM == 3 ? oz + jz : 0][oy + jy][ox + jx];
             }
         if constexpr (SRC == MASS_INV) {
-          const T w = ceta[valid ? c[0] + cy * c[1] + cz * c[2] : 0];
+          const int c0 = (gx - 1) + cy * (gy - 1) + (DIM == 3 ? cz * (gz - 1) : 0);  // cell (g - 1)
+          const T w = ceta[valid ? c0 + ox + cy * oy + cz * oz : 0];
           acc += valid ? cs * w : (T)0;
         } else {
           // tile-blocked cellwise viscosity: 4x4x4 tiles in 3D, 8x8 in 2D (64 cells)
@@ -979,6 +985,105 @@ __global__ void k_mass_cheb(const LevelDesc L, const T *__restrict__ xin, const 
   d[i] = dd;
   xout[i] = xin[i] + dd;
 }
+// The whole Schur smoothing of a small level in one workgroup: two x buffers
+// and the inverse cell viscosities in LDS, b, D^-1 and d in registers, the steps of
+// the Chebyshev sequence (seq: per step f1, f2, first) separated by workgroup
+// barriers, x written back at the end.  Each step is the arithmetic of
+// k_mass_cheb on the LDS copies; a sweep from x = 0 is a step with x_in = 0
+// (r = b).  One launch instead of one per step on levels that are
+// launch-bound.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(1024) k_mass_smooth_small(const LevelDesc L, T *__restrict__ x,
+                                                            const T *__restrict__ b, const T *__restrict__ dinv,
+                                                            const T *__restrict__ ieta, const MassChebSeq seq,
+                                                            const int x_zero) {
+  // each thread owns the nodes tid + 1024 k (k < NPT): b, D^-1 and d of its
+  // nodes stay in registers, their coordinates are computed once; x (two
+  // buffers) and 1/eta live in LDS
+  constexpr int NPT = DIM == 2 ? 4 : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *const sm = reinterpret_cast<T *>(smem_raw);
+  const int n = (int)L.pnodes, nc = (int)L.cells;
+  T *xa = sm, *xb = sm + n, *sie = sm + 2 * n;
+  int gxyz[NPT][3];
+  T bi[NPT], di[NPT], dd[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int i = threadIdx.x + 1024 * k;
+    const int ii = i < n ? i : 0;
+    const int np01 = L.np[0] * L.np[1];
+    gxyz[k][2] = ii / np01;
+    const int rem = ii - gxyz[k][2] * np01;
+    gxyz[k][1] = rem / L.np[0];
+    gxyz[k][0] = rem - gxyz[k][1] * L.np[0];
+    bi[k] = b[ii];
+    di[k] = dinv[ii];
+    dd[k] = 0;
+    if (i < n) xa[i] = x_zero ? (T)0 : x[i];
+  }
+  for (int c = threadIdx.x; c < nc; c += 1024) sie[c] = ieta[c];
+  __syncthreads();
+  for (int s = 0; s < seq.n; ++s) {
+    const T f1 = (T)seq.f1[s], f2 = (T)seq.f2[s];
+    const int first = seq.first[s];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = threadIdx.x + 1024 * k;
+      if (i < n) {
+        const T v = mass_node<DIM, MASS_INV, T>(L, xa, sie, gxyz[k][0], gxyz[k][1], gxyz[k][2]);
+        const T r = bi[k] - v;
+        dd[k] = first ? di[k] * r * f2 : f1 * dd[k] + f2 * di[k] * r;
+        xb[i] = xa[i] + dd[k];
+      }
+    }
+    __syncthreads();
+    T *t = xa;
+    xa = xb;
+    xb = t;
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int i = threadIdx.x + 1024 * k;
+    if (i < n) x[i] = xa[i];
+  }
+}
+
+template <typename T>
+bool launch_mass_smooth_small(const LevelDesc &L, T *x, const T *b, const T *dinv, const T *ieta,
+                              const MassChebSeq &seq, bool x_zero, hipStream_t st) {
+  if (!ieta || L.k != 2 || L.geo || seq.n < 1 || seq.n > MassChebSeq::MAX) return false;
+  // where one workgroup beats a launch per step: 2D levels up to 64^2
+  // pressure nodes, 3D levels up to 9^3 (a 3D step costs 27 reads per node)
+  if (L.pnodes > (L.dim == 2 ? 4096 : 729)) return false;
+  const size_t bytes = (size_t)(2 * L.pnodes + L.cells) * sizeof(T);
+  if (bytes > kSmallSmoothLds) return false;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(1), dim3(1024), bytes, st, L, x, b, dinv, ieta, seq, x_zero ? 1 : 0);
+  };
+  if (L.dim == 3)
+    go(k_mass_smooth_small<3, T>);
+  else
+    go(k_mass_smooth_small<2, T>);
+  MFS_HIP(hipGetLastError());
+  return true;
+}
+// dynamic LDS above 64 KB for the single-workgroup smoother (set once per
+// device, before any launch or stream capture: mfs_create)
+void init_small_smoother() {
+  MFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_mass_smooth_small<2, float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallSmoothLds));
+  MFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_mass_smooth_small<3, float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallSmoothLds));
+  MFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_mass_smooth_small<2, double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallSmoothLds));
+  MFS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_mass_smooth_small<3, double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmallSmoothLds));
+}
+template bool launch_mass_smooth_small<double>(const LevelDesc &, double *, const double *, const double *,
+                                               const double *, const MassChebSeq &, bool, hipStream_t);
+template bool launch_mass_smooth_small<float>(const LevelDesc &, float *, const float *, const float *, const float *,
+                                              const MassChebSeq &, bool, hipStream_t);
+
 // natural-order inverse cell viscosity from a tile-blocked cellwise table
 // (degree 2: 4x4x4 tiles in 3D, 8x8 in 2D)
 template <typename T>

@@ -185,6 +185,21 @@ bool launch_mass_cheb(const LevelDesc &L, const T *xin, const T *ieta, const T *
                       T f1, T f2, int first, hipStream_t st);
 template <typename T>
 void launch_inv_eta_nat(const LevelDesc &L, const T *eta_tiles, T *out, hipStream_t st);
+// The Chebyshev steps of one Schur smoothing: d = (first ? 0 : f1 d) + f2 D^-1 r
+struct MassChebSeq {
+  static constexpr int MAX = 40;
+  int n;
+  double f1[MAX], f2[MAX];
+  int first[MAX];
+};
+// LDS budget of the single-workgroup smoother (2 pressure vectors + cells)
+constexpr size_t kSmallSmoothLds = 144 * 1024;
+// The whole Schur smoothing of a degree-2 level that fits one workgroup's LDS
+// in one launch (x from zero when x_zero); false if the level does not fit
+template <typename T>
+bool launch_mass_smooth_small(const LevelDesc &L, T *x, const T *b, const T *dinv, const T *ieta,
+                              const MassChebSeq &seq, bool x_zero, hipStream_t st);
+void init_small_smoother();
 // B^T for degree 2 as a node-wise stencil (dst_u = op(B^T src_p) by mode, rhs_u
 // for RESID), computed in T and stored as TO; false for degree 3
 template <typename T, typename TO>
