@@ -288,13 +288,14 @@ int mfs_ablock_vmult_device(mfs_handle *h, int level, const void *d_src, void *d
 /* Allocation-free variant of mfs_ablock_vmult_device.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
- * The 3D Q2 split apply's work array (in-tile sums of the tile-boundary
+ * The degree-2 split apply's work array (in-tile sums of the tile-boundary
  * nodes) lives in a fixed device array of the library,
  * MFS_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), instead of
  * the handle's buffer; the call allocates and frees nothing.  Levels whose
- * work array (3 x 386 values per 4^3-cell tile) exceeds it return MFS_ERROR:
- * at most 8149 tiles in FP64 and 16299 in FP32 (80^3 and 100^3 cells).  2D
- * and degree-3 levels use no work array.  Every call on one GPU
+ * work array (3 x 386 values per 4^3-cell tile in 3D, 2 x 289 per 8^2-cell
+ * tile in 2D) exceeds it return MFS_ERROR: at most 8149 tiles in FP64 and
+ * 16299 in FP32 (80^3 and 100^3 cells) in 3D.  Degree-3 levels use no work
+ * array.  Every call on one GPU
  * shares that array (from any handle): calls that overlap on one GPU race. */
 #define MFS_NOALLOC_SCRATCH_BYTES (72LL << 20)
 int mfs_ablock_vmult_device_noalloc(mfs_handle *h, int level, const void *d_src, void *d_dst);

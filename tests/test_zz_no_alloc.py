@@ -45,6 +45,22 @@ def test_noalloc_matches_allocating_apply(prec, dtype):
         assert np.all(np.isfinite(ref)) and np.array_equal(ref, out), level
 
 
+@pytest.mark.parametrize("prec,dtype", [("fp32", torch.float32), ("fp64", torch.float64)])
+def test_noalloc_2d_split_apply(prec, dtype):
+    # 2D degree 2: the split A-block apply's work array (2 x 17^2 per 8x8-cell
+    # tile) also comes from the scratch array; 40x24 cells (partial tiles),
+    # then a smaller level
+    box = make_box(2, 2, (5, 3), 4, (2.0, 1.0), (1, 2, 0, 1), pressure_scaling=1.1)
+    h = StokesMatrixFreeHandler(box, prec)
+    s = h.sizes(h.top)
+    rng = np.random.default_rng(11)
+    h.assemble(np.exp(rng.normal(size=(s["cells"], h.n_qp))), "harmonic")
+    for level in (h.top, h.top - 2):
+        print(f"noalloc 2D {prec} level {level}", flush=True)
+        ref, out = _pair(h, rng, level, dtype)
+        assert np.all(np.isfinite(ref)) and np.array_equal(ref, out), level
+
+
 def test_noalloc_size_limit():
     # 96x96x64 cells = 9216 tiles: 9216 x 3 x 386 FP64 values (85 MB) exceed
     # the 72 MiB array (rejected before any launch); the FP32 work array fits
