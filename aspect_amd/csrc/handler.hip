@@ -69,6 +69,13 @@ void check_op_args(Op op, Mode mode, const void *src_u, const void *src_p, const
 namespace {
 
 thread_local std::string g_err;
+}  // namespace
+
+// error message of the calling thread's last failed C-ABI call (shared with
+// the general-mesh entry points)
+void mfs::set_last_error(const char *msg) { g_err = msg ? msg : ""; }
+
+namespace {
 
 template <typename T>
 struct DBuf {

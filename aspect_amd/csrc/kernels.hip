@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
@@ -263,7 +264,9 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
 
   const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
   int tx, ty, tz;
-  if constexpr (MODE == MODE_TSPLIT) {  // every tile in one launch
+  if constexpr (MODE == MODE_CELLS) {  // a batch of cells of a general mesh
+    tx = ty = tz = 0;
+  } else if constexpr (MODE == MODE_TSPLIT) {  // every tile in one launch
     int b = blockIdx.x;
     tx = b % L.ntiles[0];
     b /= L.ntiles[0];
@@ -278,7 +281,8 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
     b /= hy;
     tz = 2 * b + pz;
   }
-  const long long tile_id = tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
+  const long long tile_id =
+      MODE == MODE_CELLS ? (long long)blockIdx.x : tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
   const int tid = threadIdx.x;
   const int c = tid % CT;
   // the y-slice index is uniform over a wave when a slice spans whole waves
@@ -287,8 +291,21 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
   const int vx0 = K * C::TX * tx, vy0 = K * C::TY * ty, vz0 = DIM == 3 ? K * C::TZ * tz : 0;
   const int ox0 = (K - 1) * C::TX * tx, oy0 = (K - 1) * C::TY * ty, oz0 = DIM == 3 ? (K - 1) * C::TZ * tz : 0;
 
-  // ---- 1. cooperative tile load (constrained velocity DoFs read as zero)
-  if constexpr (NEED_U) {
+  // ---- 1. cooperative tile load (constrained velocity DoFs read as zero);
+  // general mesh: the cells' nodes through the cell lists, [comp][a][CT] and [a][CT]
+  constexpr int NV = ZN * N * N;
+  if constexpr (MODE == MODE_CELLS) {
+    if constexpr (NEED_U) {
+      for (int t = tid; t < DIM * NV * CT; t += C::THREADS) {
+        const int comp = t / (NV * CT), r = t - comp * NV * CT;
+        nb[t] = src_u[comp * L.vnodes + L.cell_u[tile_id * NV * CT + r]];
+      }
+    }
+    if constexpr (NEED_P) {
+      T *pc = sm + DIM * NV * CT;
+      for (int t = tid; t < C::NPL * CT; t += C::THREADS) pc[t] = src_p[L.cell_p[tile_id * C::NPL * CT + t]];
+    }
+  } else if constexpr (NEED_U) {
     for (int t = tid; t < DIM * C::VN; t += C::THREADS) {
       const int comp = t / C::VN, r = t - comp * C::VN;
       const int lx = r % C::VX, ly = (r / C::VX) % C::VY, lz = r / (C::VX * C::VY);
@@ -299,7 +316,7 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
       nb[t] = v;
     }
   }
-  if constexpr (NEED_P) {
+  if constexpr (NEED_P && MODE != MODE_CELLS) {
     for (int t = tid; t < C::PN; t += C::THREADS) {
       const int lx = t % C::PX, ly = (t / C::PX) % C::PY, lz = t / (C::PX * C::PY);
       const int gx = ox0 + lx, gy = oy0 + ly, gz = oz0 + lz;
@@ -320,7 +337,9 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
       for (int kz = 0; kz < ZN; ++kz)
 #pragma unroll
         for (int i = 0; i < N; ++i)
-          u[comp][kz][i] = nb[comp * C::VN + ((K * cz + kz) * C::VY + (K * cy + j)) * C::VX + K * cx + i];
+          u[comp][kz][i] = MODE == MODE_CELLS
+                               ? nb[(comp * NV + i + N * (j + N * kz)) * CT + c]
+                               : nb[comp * C::VN + ((K * cz + kz) * C::VY + (K * cy + j)) * C::VX + K * cx + i];
   }
   if constexpr (NEED_P) {
 #pragma unroll
@@ -329,7 +348,9 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
       for (int jy = 0; jy < NP; ++jy)
 #pragma unroll
         for (int i = 0; i < NP; ++i)
-          p[kz][jy][i] = pb[(((K - 1) * cz + kz) * C::PY + ((K - 1) * cy + jy)) * C::PX + (K - 1) * cx + i];
+          p[kz][jy][i] = MODE == MODE_CELLS
+                             ? sm[DIM * NV * CT + (i + NP * (jy + NP * kz)) * CT + c]
+                             : pb[(((K - 1) * cz + kz) * C::PY + ((K - 1) * cy + jy)) * C::PX + (K - 1) * cx + i];
   }
   __syncthreads();  // nb/pb now reused as exchange buffer
 
@@ -692,6 +713,35 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
   // contributors in a fixed order (deterministic) and is written out
   constexpr int CU_OFF = 0;                         // [comp][kz][j][i][CT]
   constexpr int CP_OFF = DIM * ZN * N * N * CT;     // [kz][jy][i][Q][CT] pressure shares per slice
+  if constexpr (MODE == MODE_CELLS) {
+    if constexpr (OUT_U) {
+#pragma unroll
+      for (int comp = 0; comp < DIM; ++comp)
+#pragma unroll
+        for (int kz = 0; kz < ZN; ++kz)
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            dst_u[((tile_id * DIM + comp) * NV + i + N * (j + N * kz)) * CT + c] = out[comp][kz][i];
+    }
+    if constexpr (OUT_P) {
+      __syncthreads();
+#pragma unroll
+      for (int kz = 0; kz < ZNP; ++kz)
+#pragma unroll
+        for (int jy = 0; jy < NP; ++jy)
+#pragma unroll
+          for (int i = 0; i < NP; ++i) xb[CP_OFF + ((((kz * NP + jy) * NP + i) * Q + j) * CT) + c] = pacc[kz][jy][i];
+      __syncthreads();
+      for (int t = tid; t < C::NPL * CT; t += C::THREADS) {
+        const int a = t / CT, cc = t - a * CT;
+        T val = 0;
+#pragma unroll
+        for (int jj = 0; jj < Q; ++jj) val += xb[CP_OFF + (a * Q + jj) * CT + cc];
+        dst_p[tile_id * C::NPL * CT + t] = val;
+      }
+    }
+    return;
+  }
   __syncthreads();
   if constexpr (OUT_U) {
 #pragma unroll
@@ -788,6 +838,41 @@ void launch_dkm(const LevelDesc &L, const T *su, const T *sp, const T *eta, int 
 // Jacobi-Chebyshev step (ChebArgs; constrained rows r = b - x), the
 // arithmetic of v_cheb_first / v_cheb_step.
 constexpr int GM2_CHEB = SPLIT_CHEB;
+// General meshes: one launch of the cell kernel over all batches of 64 cells
+// (MODE_CELLS), degree 2; the per-cell contributions go to cu / cp.
+template <int DIM, int OP, typename T>
+static void launch_cells_d(const LevelDesc &L, long long nbatch, const T *su, const T *sp, const T *eta, T *cu,
+                           T *cp, hipStream_t st) {
+  using C = Cfg<DIM, 2>;
+  const size_t smem = (size_t)C::SMEM * sizeof(T);
+  hipLaunchKernelGGL((k_cell<DIM, 2, OP, MODE_CELLS, T>), dim3((unsigned)nbatch), dim3(C::THREADS), smem, st, L, 0,
+                     su, sp, eta, 1, (const T *)nullptr, (const T *)nullptr, cu, cp);
+  MFS_HIP(hipGetLastError());
+}
+template <int DIM, typename T>
+static void launch_cells_dim(Op op, const LevelDesc &L, long long nbatch, const T *su, const T *sp, const T *eta,
+                             T *cu, T *cp, hipStream_t st) {
+  switch (op) {
+    case OP_STOKES: launch_cells_d<DIM, OP_STOKES, T>(L, nbatch, su, sp, eta, cu, cp, st); break;
+    case OP_ABLOCK: launch_cells_d<DIM, OP_ABLOCK, T>(L, nbatch, su, sp, eta, cu, cp, st); break;
+    case OP_BT: launch_cells_d<DIM, OP_BT, T>(L, nbatch, su, sp, eta, cu, cp, st); break;
+    case OP_MASS: launch_cells_d<DIM, OP_MASS, T>(L, nbatch, su, sp, eta, cu, cp, st); break;
+    default: throw std::invalid_argument("general mesh: unsupported operator");
+  }
+}
+template <typename T>
+void launch_cells(Op op, const LevelDesc &L, long long nbatch, const T *su, const T *sp, const T *eta, T *cu, T *cp,
+                  hipStream_t st) {
+  if (nbatch <= 0) return;
+  if (L.k != 2 || !L.geo || !L.cell_u || !L.cell_p) throw std::invalid_argument("general mesh: degree 2 cell lists and geometry required");
+  if (L.dim == 3)
+    launch_cells_dim<3, T>(op, L, nbatch, su, sp, eta, cu, cp, st);
+  else
+    launch_cells_dim<2, T>(op, L, nbatch, su, sp, eta, cu, cp, st);
+}
+template void launch_cells<double>(Op, const LevelDesc &, long long, const double *, const double *, const double *,
+                                   double *, double *, hipStream_t);
+
 template <int MODE, typename T>
 __global__ void k_gather_tiles2(const LevelDesc L, const T *__restrict__ work, const T *__restrict__ src_u,
                                 const T *__restrict__ rhs_u, T *__restrict__ dst_u, const ChebArgs<T> ch) {

@@ -38,6 +38,12 @@ constexpr int SPLIT_CHEB = 3;
 // internal mode of the general cell kernel (A block, 2D degree 2): every tile
 // in one launch, in-tile node sums to work[tile][comp][lz][ly][lx]
 constexpr int MODE_TSPLIT = 3;
+// internal mode of the general cell kernel on a general mesh: a workgroup
+// takes a batch of CT cells (cell lists LevelDesc::cell_u / cell_p), reads the
+// constraint-resolved source through them and writes every cell's nodal
+// contributions to dst[batch][comp][node][CT] (velocity) and
+// dst_p[batch][node][CT] (pressure); a node-wise pass assembles them
+constexpr int MODE_CELLS = 4;
 // mode value of launch_planes_split: every in-tile node sum to the full work
 // array [tile][comp][lz][ly][lx] (the Chebyshev-fused A-block step)
 constexpr int SPLIT_FULL_SUMS = 3;
@@ -79,6 +85,11 @@ struct LevelDesc {
   // (row-major, d xi_e / d x_d at e * dim + d) and JxW, tile-blocked
   // [((tile * NQ + q) * (dim * dim + 1) + g) * CT + cell_in_tile]
   const double *geo;
+  // general meshes (MODE_CELLS): node ids of the cells of every batch of CT
+  // cells, [(batch * 3^dim + a) * CT + c] velocity, [(batch * 2^dim + a) * CT + c]
+  // pressure (a lexicographic in the cell, x fastest)
+  const int *cell_u;
+  const int *cell_p;
 };
 
 // Mesh constants of a level in the arithmetic type of a kernel (kernel
@@ -108,6 +119,15 @@ int tile_cells(int dim, int k, int out[3]);
 int tile_threads(int dim, int k);
 
 void upload_basis_tables();
+
+// message returned by mfs_last_error() on the calling thread
+void set_last_error(const char *msg);
+
+// general meshes: the cell kernel in MODE_CELLS over nbatch batches of 64 cells
+// (degree 2, FP64; L.cell_u / L.cell_p / L.geo set); ops Stokes, A block, B^T, mass
+template <typename T>
+void launch_cells(Op op, const LevelDesc &L, long long nbatch, const T *su, const T *sp, const T *eta, T *cu, T *cp,
+                  hipStream_t st);
 
 // Host-side argument check of an operator launch: throws std::invalid_argument
 // naming the first null pointer the (op, mode) kernel would read or write.

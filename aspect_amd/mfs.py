@@ -39,7 +39,8 @@ EXPORTED = [
     "mfs_ensemble_create", "mfs_ensemble_destroy", "mfs_ensemble_member", "mfs_ensemble_setup", "mfs_ensemble_solve",
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
-    "mfs_box_dof_index",
+    "mfs_box_dof_index", "mfs_tria_create", "mfs_tria_destroy", "mfs_tria_sizes", "mfs_tria_set_viscosity",
+    "mfs_tria_vmult", "mfs_tria_vmult_device",
 ]
 
 
@@ -571,5 +572,90 @@ class Ensemble:
                 self.members = []
                 self.lib.mfs_ensemble_destroy(self._e)
                 self._e = C.c_void_p()
+        except Exception:
+            pass
+
+
+class TriaDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("n_cells", C.c_longlong), ("n_nodes_u", C.c_longlong),
+                ("n_nodes_p", C.c_longlong), ("cell_nodes_u", C.c_void_p), ("cell_nodes_p", C.c_void_p),
+                ("geometry", C.c_void_p), ("compressible", C.c_int), ("pressure_scaling", C.c_double)]
+
+
+class Constraints(C.Structure):
+    _fields_ = [("n_lines", C.c_longlong), ("dofs", C.c_void_p), ("offsets", C.c_void_p), ("masters", C.c_void_p),
+                ("weights", C.c_void_p)]
+
+
+class TriaOperators:
+    """Stokes level operators on a general mesh with hanging nodes (the level
+    operators of the global-coarsening handler,
+    source/simulator/solver/stokes_matrix_free_global_coarsening.cc:1455-1625).
+
+    ``mesh`` carries the arrays ``aspect_amd.tria.Tria.build`` returns: cell
+    node lists, geometry at the quadrature points and closed constraint lines
+    ``(dofs, offsets, masters, weights)`` of the velocity and pressure blocks.
+    """
+
+    OPS = {"stokes": 0, "ablock": 1, "bt": 2, "mass": 3}
+
+    def __init__(self, mesh, pressure_scaling=1.0, compressible=False):
+        self.lib = load_library()
+        self.dim = mesh.dim
+        self._keep = []
+
+        def arr(a, t):
+            a = np.ascontiguousarray(a, dtype=t)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        d = TriaDesc()
+        d.dim, d.degree = mesh.dim, 2
+        d.n_cells, d.n_nodes_u, d.n_nodes_p = mesh.n_cells, mesh.n_nodes_u, mesh.n_nodes_p
+        d.cell_nodes_u = arr(mesh.cell_u, np.int32)
+        d.cell_nodes_p = arr(mesh.cell_p, np.int32)
+        d.geometry = arr(mesh.geometry, np.float64)
+        d.compressible = int(bool(compressible))
+        d.pressure_scaling = float(pressure_scaling)
+
+        def con(c):
+            dofs, offs, masters, weights = c
+            k = Constraints()
+            k.n_lines = len(dofs)
+            k.dofs, k.offsets = arr(dofs, np.int64), arr(offs, np.int64)
+            k.masters, k.weights = arr(masters, np.int64), arr(weights, np.float64)
+            return k
+
+        cu, cp = con(mesh.con_u), con(mesh.con_p)
+        self._t = C.c_void_p()
+        _check(self.lib.mfs_tria_create(C.byref(d), C.byref(cu), C.byref(cp), C.byref(self._t)))
+        self._keep = []
+        out = (C.c_longlong * 3)()
+        _check(self.lib.mfs_tria_sizes(self._t, out))
+        self.n_cells, self.nu, self.np = out[0], out[1], out[2]
+
+    def set_viscosity(self, eta_qp):
+        e = np.ascontiguousarray(eta_qp, dtype=np.float64)
+        if e.size != self.n_cells * 3 ** self.dim:
+            raise ValueError("one viscosity per cell and quadrature point expected")
+        _check(self.lib.mfs_tria_set_viscosity(self._t, _ptr(e)))
+
+    def vmult(self, op, src):
+        o = self.OPS[op]
+        n_in = {0: self.nu + self.np, 1: self.nu, 2: self.np, 3: self.np}[o]
+        n_out = {0: self.nu + self.np, 1: self.nu, 2: self.nu, 3: self.np}[o]
+        x = _vec(src, n_in, "source vector")
+        y = np.empty(n_out)
+        _check(self.lib.mfs_tria_vmult(self._t, o, _ptr(x), _ptr(y)))
+        return y
+
+    def vmult_device(self, op, d_src, d_dst):
+        _check(self.lib.mfs_tria_vmult_device(self._t, self.OPS[op], C.c_void_p(d_src), C.c_void_p(d_dst)))
+
+    def __del__(self):
+        try:
+            if self._t:
+                self.lib.mfs_tria_destroy(self._t)
+                self._t = C.c_void_p()
         except Exception:
             pass

@@ -338,6 +338,57 @@ int mfs_ensemble_setup(mfs_ensemble *e, const double *const *eta_qp, int averagi
 int mfs_ensemble_solve(mfs_ensemble *e, const double *const *rhs, double *const *sol, const mfs_solve_params *prm,
                        mfs_solve_result *results, int n_threads, double *seconds);
 
+/* General level meshes (degree 2): the level operators of the global-coarsening
+ * handler on a mesh of any topology with hanging nodes
+ * (source/simulator/solver/stokes_matrix_free_global_coarsening.cc:1455-1625:
+ * one MatrixFree object per level with the velocity and pressure
+ * AffineConstraints of :1481-1598; operators of matrix_free_operators.cc:376-547,
+ * :668-701, :808-845, :974-1009).  The caller describes the mesh the way a
+ * DoFHandler and AffineConstraints do: node ids per cell, the geometry at every
+ * quadrature point, and closed constraint lines. */
+typedef struct mfs_tria mfs_tria;
+typedef struct {
+  int dim;                  /* 2 or 3 */
+  int degree;               /* velocity degree: 2 */
+  long long n_cells;
+  long long n_nodes_u;      /* velocity nodes; velocity DoF = component * n_nodes_u + node */
+  long long n_nodes_p;      /* pressure nodes = pressure DoFs */
+  const int *cell_nodes_u;  /* n_cells x 3^dim node ids, lexicographic in the cell (x fastest) */
+  const int *cell_nodes_p;  /* n_cells x 2^dim node ids, lexicographic */
+  const double *geometry;   /* n_cells x 3^dim QGauss(3) points (lexicographic) x (dim*dim + 1):
+                               J^{-1} row-major (entry e*dim + d = d xi_e / d x_d), then JxW */
+  int compressible;         /* subtract (2/3) eta div(u) I (matrix_free_operators.cc:466-469) */
+  double pressure_scaling;
+} mfs_tria_desc;
+/* Closed, homogeneous constraint lines of one block (AffineConstraints after
+ * close(): no master is itself constrained): DoF dofs[i] = sum over
+ * k in [offsets[i], offsets[i+1]) of weights[k] * DoF masters[k]; a line
+ * without masters fixes the DoF to zero (Dirichlet). */
+typedef struct {
+  long long n_lines;
+  const long long *dofs;
+  const long long *offsets; /* n_lines + 1 */
+  const long long *masters;
+  const double *weights;
+} mfs_constraints;
+/* velocity / pressure may be NULL (no constraints). */
+int mfs_tria_create(const mfs_tria_desc *desc, const mfs_constraints *velocity, const mfs_constraints *pressure,
+                    mfs_tria **out);
+int mfs_tria_destroy(mfs_tria *t);
+/* out[0] cells, out[1] velocity DoFs, out[2] pressure DoFs. */
+int mfs_tria_sizes(const mfs_tria *t, long long out[3]);
+/* Viscosity at every quadrature point, n_cells x 3^dim (OperatorCellData with
+ * per-point tables, matrix_free_operators.h:91-208). */
+int mfs_tria_set_viscosity(mfs_tria *t, const double *eta_qp);
+/* dst = op(src) with the MatrixFree constraint semantics: constrained source
+ * entries are taken from their masters, contributions to constrained DoFs go
+ * to their masters, constrained rows are identity rows (zero for B^T).
+ * op: MFS_OP_STOKES ([u|p] -> [u|p]), MFS_OP_ABLOCK (u -> u), MFS_OP_BT
+ * (p -> u), MFS_OP_MASS (s^2/eta pressure mass, p -> p).  Host vectors. */
+int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst);
+/* The same on device vectors (hipMalloc'd by the caller). */
+int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,3 +149,67 @@ def test_box_dof_index_recovers_permutation(dim, deg, coarse, nl):
     dup[1] = dup[0]
     with pytest.raises(SolverError, match="two DoFs"):
         box_dof_index(box, pts[dup], comp[dup])
+
+
+def test_tria_refuses_bad_meshes_without_gpu_work():
+    """mfs_tria_create checks the mesh and the constraint lines on the host
+    before any device call: node ids in range, degree 2, finite geometry,
+    every DoF constrained once, closed lines (no constrained master)."""
+    if not os.path.exists(mfs.LIB_PATH):
+        pytest.skip("libaspect_mfs.so not built")
+    from aspect_amd import tria as T
+    lib = mfs.load_library()
+    t = T.Tria(T.BoxCoarse(2, (2, 1), (2.0, 1.0)), 0)
+    t.refine(np.array([True, False]))
+    mesh = t.build({0: "zero"})
+    keep = []
+
+    def arr(a, ty):
+        a = np.ascontiguousarray(a, dtype=ty)
+        keep.append(a)
+        return a.ctypes.data
+
+    def desc(cell_u=None, degree=2, geometry=None):
+        d = mfs.TriaDesc()
+        d.dim, d.degree = 2, degree
+        d.n_cells, d.n_nodes_u, d.n_nodes_p = mesh.n_cells, mesh.n_nodes_u, mesh.n_nodes_p
+        d.cell_nodes_u = arr(mesh.cell_u if cell_u is None else cell_u, np.int32)
+        d.cell_nodes_p = arr(mesh.cell_p, np.int32)
+        d.geometry = arr(mesh.geometry if geometry is None else geometry, np.float64)
+        d.pressure_scaling = 1.0
+        return d
+
+    def con(lines):
+        dofs, offs, masters, weights = lines
+        k = mfs.Constraints()
+        k.n_lines = len(dofs)
+        k.dofs, k.offsets = arr(dofs, np.int64), arr(offs, np.int64)
+        k.masters, k.weights = arr(masters, np.int64), arr(weights, np.float64)
+        return k
+
+    def create(d, cu):
+        h = ctypes.c_void_p()
+        rc = lib.mfs_tria_create(ctypes.byref(d), ctypes.byref(cu), None, ctypes.byref(h))
+        return rc, lib.mfs_last_error().decode()
+
+    good = con(mesh.con_u)
+    bad_u = mesh.cell_u.copy()
+    bad_u[1, 3] = mesh.n_nodes_u
+    rc, msg = create(desc(cell_u=bad_u), good)
+    assert rc == mfs.MFS_ERROR and "node id out of range" in msg
+    rc, msg = create(desc(degree=3), good)
+    assert rc == mfs.MFS_ERROR and "degree 2" in msg
+    geo = mesh.geometry.copy()
+    geo[0, 0, 0] = np.nan
+    rc, msg = create(desc(geometry=geo), good)
+    assert rc == mfs.MFS_ERROR and "geometry not finite" in msg
+    dofs, offs, masters, weights = mesh.con_u
+    # a hanging line whose master is a Dirichlet DoF: not closed
+    hang = int(np.nonzero(np.diff(offs) > 0)[0][0])
+    open_masters = masters.copy()
+    open_masters[offs[hang]] = dofs[np.nonzero(np.diff(offs) == 0)[0][0]]
+    rc, msg = create(desc(), con((dofs, offs, open_masters, weights)))
+    assert rc == mfs.MFS_ERROR and "close the constraints" in msg
+    twice = np.concatenate([dofs, dofs[:1]])
+    rc, msg = create(desc(), con((twice, np.concatenate([offs, offs[-1:]]), masters, weights)))
+    assert rc == mfs.MFS_ERROR and "constrained twice" in msg

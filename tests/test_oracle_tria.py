@@ -1,0 +1,280 @@
+"""CPU tests that pin the general-mesh oracle (oracle/tria.py) and the mesh
+builder (aspect_amd/tria.py: DoF maps, hanging-node, Dirichlet and
+no-normal-flux lines, geometry at the quadrature points).
+
+Each check is chosen so that a plausible slip fails it:
+  * meshes without hanging nodes against the box oracle (oracle.cpp, pinned by
+    brute-force element matrices and closed forms in test_oracle.py): plain
+    boxes with every boundary type, and a curved (chunk) geometry, for the
+    Stokes, A, B^T and mass operators — a transposed J^{-1}, a wrong DoF map or
+    a wrong constraint semantic shows here;
+  * textbook hanging-node counts (two cells, one refined: 2 / 1 hanging nodes
+    in 2D, 16 / 5 in 3D);
+  * the hanging-node lines reproduce every global Q2 (velocity) / Q1 (pressure)
+    tensor polynomial at the hanging nodes (a wrong weight or a node matched to
+    the wrong face fails);
+  * on locally refined meshes the constrained operators reproduce closed-form
+    integrals over the domain of polynomial fields: u^T A v = int 2 eta
+    eps(u):eps(v) [- 2/3 eta div u div v], p^T B u = -s int p div u,
+    p^T M p = s^2 int p^2 / eta (a missing C^T on the test side, a dropped
+    hanging contribution or a cell-size error fails);
+  * the shell's maps: Jacobians against finite differences of the points,
+    node counts of the cubed-sphere / annulus lattice in closed form (a seam
+    between panels that does not match adds nodes), the shell volume, and
+    u . n = 0 on the no-normal-flux nodes after the constraints are applied.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from aspect_amd import problems as P
+from aspect_amd import tria as T
+from oracle import tria as OT
+
+BCMAP = {0: None, 1: "tangential", 2: "zero"}
+
+
+@pytest.fixture(autouse=True)
+def _clear_geometry():
+    O.set_geometry(None)
+    yield
+    O.set_geometry(None)
+
+
+def box_mesh(dim, g, bc, extent):
+    t = T.Tria(T.BoxCoarse(dim, (1,) * dim, extent), g)
+    return t.build({f: BCMAP[b] for f, b in enumerate(bc[:2 * dim]) if BCMAP[b]})
+
+
+def box_perm(mesh, dim, n, extent):
+    """Box-vector index of every general-mesh DoF (velocity, pressure)."""
+    h = np.array(extent[:dim], dtype=float) / n
+    iu = np.round(mesh.x_u / (h / 2)).astype(np.int64)
+    nv = 2 * n + 1
+    lin_u = iu[:, 0] + nv * iu[:, 1] + (nv * nv * iu[:, 2] if dim == 3 else 0)
+    ip = np.round(mesh.x_p / h).astype(np.int64)
+    lin_p = ip[:, 0] + (n + 1) * ip[:, 1] + ((n + 1) ** 2 * ip[:, 2] if dim == 3 else 0)
+    vn = nv ** dim
+    return np.concatenate([c * vn + lin_u for c in range(dim)]), lin_p, dim * vn
+
+
+@pytest.mark.parametrize("dim,bc,compressible", [(2, (1, 2, 1, 0, 0, 0), False), (3, (1, 1, 2, 1, 0, 1), True),
+                                                 (3, (2, 2, 2, 2, 2, 2), False)])
+def test_box_mesh_matches_box_oracle(dim, bc, compressible):
+    g = 2
+    n = 1 << g
+    extent = (1.3, 0.9, 1.1)[:dim]
+    mesh = box_mesh(dim, g, bc, extent)
+    assert len(mesh.hanging_u) == 0 and len(mesh.hanging_p) == 0
+    p = O.make_problem(dim, 2, (1,) * dim, g + 1, extent, bc, compressible=compressible, pressure_scaling=1.7)
+    rng = np.random.default_rng(dim)
+    eta = np.exp(rng.normal(size=(mesh.n_cells, 3 ** dim)))
+    ops = OT.Operators(mesh, eta, pressure_scaling=1.7, compressible=compressible)
+    pu, pp, nu_box = box_perm(mesh, dim, n, extent)
+    x = rng.normal(size=nu_box + (n + 1) ** dim)
+    xt = np.concatenate([x[pu], x[nu_box + pp]])
+    y = O.stokes_vmult(p, g, eta, x)
+    yt = ops.stokes(xt)
+    ref = np.concatenate([y[pu], y[nu_box + pp]])
+    assert np.max(np.abs(yt - ref)) <= 1e-12 * np.max(np.abs(ref))
+    ya = O.ablock_vmult(p, g, eta, x[:nu_box])
+    assert np.max(np.abs(ops.ablock(xt[:mesh.n_dofs_u]) - ya[pu])) <= 1e-12 * np.max(np.abs(ya))
+    yb = O.bt_vmult(p, g, x[nu_box:])
+    assert np.max(np.abs(ops.bt(xt[mesh.n_dofs_u:]) - yb[pu])) <= 1e-12 * np.max(np.abs(yb))
+    ym = O.mass_vmult(p, g, eta, x[nu_box:])
+    assert np.max(np.abs(ops.mass(xt[mesh.n_dofs_u:]) - ym[pp])) <= 1e-12 * np.max(np.abs(ym))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_curved_mesh_matches_box_oracle(dim):
+    g = 2
+    n = 1 << g
+    bc = (2, 0, 2, 0, 0, 2)
+    mesh = box_mesh(dim, g, bc, (1.0, 1.0, 1.0))
+    tabs = [P.mapped_geometry(dim, 2, (1 << l,) * dim, P.chunk_map(dim)) for l in range(g + 1)]
+    mesh.geometry = tabs[-1]
+    p = O.make_problem(dim, 2, (1,) * dim, g + 1, (1.0, 1.0, 1.0), bc, pressure_scaling=0.8)
+    O.set_geometry(tabs)
+    rng = np.random.default_rng(10 + dim)
+    eta = np.exp(rng.normal(size=(mesh.n_cells, 3 ** dim)))
+    ops = OT.Operators(mesh, eta, pressure_scaling=0.8)
+    pu, pp, nu_box = box_perm(mesh, dim, n, (1.0, 1.0, 1.0))
+    x = rng.normal(size=nu_box + (n + 1) ** dim)
+    xt = np.concatenate([x[pu], x[nu_box + pp]])
+    y = O.stokes_vmult(p, g, eta, x)
+    ref = np.concatenate([y[pu], y[nu_box + pp]])
+    assert np.max(np.abs(ops.stokes(xt) - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("dim,nu,np_", [(2, 2, 1), (3, 16, 5)])
+def test_hanging_counts_two_cells(dim, nu, np_):
+    t = T.Tria(T.BoxCoarse(dim, (2, 1, 1), (2.0, 1.0, 1.0)), 0)
+    t.refine(np.array([True, False]))
+    m = t.build()
+    assert len(m.hanging_u) == nu and len(m.hanging_p) == np_
+    assert m.n_cells == 2 ** dim + 1
+
+
+# ------------------------------------------------------------------ polynomial fields
+class Poly:
+    """Tensor polynomial sum_a c_a x^a0 y^a1 z^a2 with exponents up to deg per variable."""
+
+    def __init__(self, dim, deg, rng):
+        self.dim, self.deg = dim, deg
+        self.c = rng.normal(size=(deg + 1,) * dim)
+
+    def _terms(self, x, dvar=None):
+        out = np.zeros(len(x))
+        for a in np.ndindex(*self.c.shape):
+            t = np.full(len(x), self.c[a])
+            for d in range(self.dim):
+                e = a[d]
+                if d == dvar:
+                    t = t * (e * x[:, d] ** (e - 1) if e > 0 else 0.0)
+                else:
+                    t = t * x[:, d] ** e
+            out += t
+        return out
+
+    def __call__(self, x):
+        return self._terms(x)
+
+    def grad(self, x):
+        return np.stack([self._terms(x, d) for d in range(self.dim)], axis=1)
+
+
+def domain_rule(dim, extent, npts=5):
+    x, w = np.polynomial.legendre.leggauss(npts)
+    x, w = 0.5 * (x + 1), 0.5 * w
+    pts = T._lattice(dim, x) * np.array(extent[:dim])
+    wt = np.prod(T._lattice(dim, w), axis=1) * np.prod(extent[:dim])
+    return pts, wt
+
+
+def refined_box(dim, seed):
+    reps = (3, 2, 2)[:dim]
+    extent = (1.5, 1.0, 1.2)[:dim]
+    t = T.Tria(T.BoxCoarse(dim, reps, extent), 1)
+    rng = np.random.default_rng(seed)
+    t.refine(rng.random(t.n_cells) < 0.35)
+    return t.build(), extent
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hanging_lines_reproduce_polynomials(dim):
+    mesh, _ = refined_box(dim, 3)
+    assert len(mesh.hanging_u) > 0 and len(mesh.hanging_p) > 0
+    rng = np.random.default_rng(4)
+    for c in range(dim):
+        f = Poly(dim, 2, rng)
+        x = np.zeros(mesh.n_dofs_u)
+        x[c * mesh.n_nodes_u:(c + 1) * mesh.n_nodes_u] = f(mesh.x_u)
+        x0 = x.copy()
+        x0[c * mesh.n_nodes_u + mesh.hanging_u] = 0.0
+        xr = OT.resolve(mesh.con_u, x0)
+        assert np.max(np.abs(xr - x)) <= 1e-12 * np.max(np.abs(x))
+    q = Poly(dim, 1, rng)
+    pv = q(mesh.x_p)
+    p0 = pv.copy()
+    p0[mesh.hanging_p] = 0.0
+    assert np.max(np.abs(OT.resolve(mesh.con_p, p0) - pv)) <= 1e-12 * np.max(np.abs(pv))
+
+
+@pytest.mark.parametrize("dim,compressible", [(2, False), (3, False), (3, True)])
+def test_hanging_operators_closed_forms(dim, compressible):
+    mesh, extent = refined_box(dim, 5 + dim)
+    rng = np.random.default_rng(6)
+    ea = rng.uniform(0.2, 0.6, size=dim)
+    eta_f = lambda x: 1.0 + x @ ea                                   # linear viscosity
+    s = 1.3
+    ops = OT.Operators(mesh, eta_f(mesh.x_q.reshape(-1, dim)).reshape(mesh.n_cells, -1), pressure_scaling=s,
+                       compressible=compressible)
+    U = [Poly(dim, 2, rng) for _ in range(dim)]
+    V = [Poly(dim, 2, rng) for _ in range(dim)]
+    Pp = Poly(dim, 1, rng)
+
+    def nodal(F):
+        x = np.concatenate([f(mesh.x_u) for f in F])
+        for c in range(dim):
+            x[c * mesh.n_nodes_u + mesh.hanging_u] = 0.0
+        return x
+
+    xu, xv = nodal(U), nodal(V)
+    pp = Pp(mesh.x_p)
+    pp[mesh.hanging_p] = 0.0
+    X, W = domain_rule(dim, extent)
+    gu = np.stack([f.grad(X) for f in U], axis=1)                    # [pt, comp, d]
+    gv = np.stack([f.grad(X) for f in V], axis=1)
+    eu = 0.5 * (gu + gu.transpose(0, 2, 1))
+    ev = 0.5 * (gv + gv.transpose(0, 2, 1))
+    div_u, div_v = np.trace(gu, axis1=1, axis2=2), np.trace(gv, axis1=1, axis2=2)
+    ex = eta_f(X)
+    a_uv = np.sum(W * ex * (2 * np.sum(eu * ev, axis=(1, 2)) - (2.0 / 3.0 if compressible else 0.0) * div_u * div_v))
+    a_uu = np.sum(W * ex * (2 * np.sum(eu * eu, axis=(1, 2)) - (2.0 / 3.0 if compressible else 0.0) * div_u ** 2))
+    assert abs(xv @ ops.ablock(xu) - a_uv) <= 1e-11 * abs(a_uu)
+    assert abs(xu @ ops.ablock(xu) - a_uu) <= 1e-11 * abs(a_uu)
+    b = -s * np.sum(W * Pp(X) * div_u)
+    assert abs(xu @ ops.bt(pp) - b) <= 1e-11 * np.sqrt(a_uu * np.sum(W * Pp(X) ** 2))
+    st = ops.stokes(np.concatenate([xu, pp]))
+    assert abs(pp @ st[mesh.n_dofs_u:] - b) <= 1e-11 * np.sqrt(a_uu * np.sum(W * Pp(X) ** 2))
+    # mass with a constant viscosity (p^2 / eta stays polynomial)
+    opc = OT.Operators(mesh, np.full((mesh.n_cells, 3 ** dim), 2.5), pressure_scaling=s)
+    m = s * s * np.sum(W * Pp(X) ** 2) / 2.5
+    assert abs(pp @ opc.mass(pp) - m) <= 1e-12 * m
+
+
+# ------------------------------------------------------------------ the shell
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shell_jacobians_against_differences(dim):
+    sh = T.ShellCoarse(dim)
+    rng = np.random.default_rng(7)
+    root = rng.integers(0, sh.n_roots, size=50)
+    xi = rng.uniform(0.05, 0.95, size=(50, dim))
+    J = sh.jacobians(root, xi)
+    h = 1e-6
+    for e in range(dim):
+        d = np.zeros(dim)
+        d[e] = h
+        fd = (sh.points(root, xi + d) - sh.points(root, xi - d)) / (2 * h)
+        assert np.max(np.abs(fd - J[:, :, e])) <= 1e-8
+    r = np.linalg.norm(sh.points(root, xi), axis=1)
+    assert np.all((r >= sh.r0 - 1e-14) & (r <= sh.r1 + 1e-14))
+
+
+@pytest.mark.parametrize("dim,g", [(2, 1), (3, 0), (3, 1)])
+def test_shell_node_counts_and_volume(dim, g):
+    m = T.Tria(T.ShellCoarse(dim), g).build()
+    if dim == 3:
+        M, Mp = 2 * 2 * (1 << g), 2 * (1 << g)
+        assert m.n_nodes_u == (6 * M * M + 2) * (2 * 4 * (1 << g) + 1)
+        assert m.n_nodes_p == (6 * Mp * Mp + 2) * (4 * (1 << g) + 1)
+        vol = 4.0 / 3.0 * np.pi * (1.0 - 0.55 ** 3)
+    else:
+        assert m.n_nodes_u == 12 * (1 << g) * 2 * (2 * (1 << g) + 1)
+        assert m.n_nodes_p == 12 * (1 << g) * ((1 << g) + 1)
+        vol = np.pi * (1.0 - 0.55 ** 2)
+    jxw = m.geometry[:, :, -1].sum()
+    assert abs(jxw - vol) <= 1e-4 * vol
+    assert np.all(m.geometry[:, :, -1] > 0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shell_no_normal_flux_and_hanging_lines(dim):
+    t = T.Tria(T.ShellCoarse(dim), 1 if dim == 2 else 0)
+    c = t.cell_centers()
+    t.refine((c[:, 0] > 0.25) & (np.linalg.norm(c, axis=1) > 0.7))
+    m = t.build({0: "tangential", 1: "tangential"})
+    assert len(m.hanging_u) > 0
+    rng = np.random.default_rng(8)
+    x = OT.resolve(m.con_u, rng.normal(size=m.n_dofs_u))
+    nn = m.n_nodes_u
+    hang = set(m.hanging_u.tolist())
+    for bid in (0, 1):
+        ids = np.array([i for i in m.boundary_nodes[bid] if i not in hang])
+        u = np.stack([x[c * nn + ids] for c in range(dim)], axis=1)
+        n = m.x_u[ids] / np.linalg.norm(m.x_u[ids], axis=1)[:, None]
+        assert np.max(np.abs(np.sum(u * n, axis=1))) <= 1e-12 * np.max(np.abs(u))
+    # hanging lines (pressure: no other lines) form a partition of unity
+    dofs, offs, _, w = m.con_p
+    sums = np.add.reduceat(w, offs[:-1])
+    assert np.max(np.abs(sums - 1.0)) <= 1e-13
