@@ -5,7 +5,7 @@ package is the Python mirror of the StokesMatrixFreeHandler interface on top
 of its C-ABI (``include/aspect_mfs.h``).
 """
 from .mfs import (AVERAGE, BC_OPEN, BC_TANGENTIAL, BC_ZERO, Box, Ensemble, NoConvergence, SolverError,
-                  SolverOutputs, StokesMatrixFreeHandler, TriaOperators, load_library, make_box)
+                  SolverOutputs, StokesMatrixFreeHandler, TriaMultigrid, TriaOperators, load_library, make_box)
 
 __all__ = ["AVERAGE", "BC_OPEN", "BC_TANGENTIAL", "BC_ZERO", "Box", "Ensemble", "NoConvergence", "SolverError",
-           "SolverOutputs", "StokesMatrixFreeHandler", "TriaOperators", "load_library", "make_box"]
+           "SolverOutputs", "StokesMatrixFreeHandler", "TriaMultigrid", "TriaOperators", "load_library", "make_box"]

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,6 +45,7 @@ struct Buf {
   Buf() = default;
   Buf(const Buf &) = delete;
   Buf &operator=(const Buf &) = delete;
+  Buf(Buf &&o) noexcept : p(o.p) { o.p = nullptr; }
   ~Buf() {
     if (p) (void)hipFree(p);
   }
@@ -131,9 +133,6 @@ struct DevCon {
     tdof.upload(h_tdof, st);
     tw.upload(h_tw, st);
     MFS_HIP(hipStreamSynchronize(st));
-    h_row.clear();
-    h_off.clear();
-    h_mas.clear();
     h_w.clear();
     h_tptr.clear();
     h_tdof.clear();
@@ -211,6 +210,7 @@ struct mfs_tria {
   Buf<double> geo, eta, cbu, cbp, xu, xp, io_src, io_dst;
   Buf<long long> iptr_u, ioff_u, iptr_p, ioff_p;
   DevCon con_u, con_p;
+  std::vector<int> h_cell_u, h_cell_p;  // host copies of the cell lists (diagonal colouring)
   bool have_eta = false;
 
   ~mfs_tria() {
@@ -275,6 +275,8 @@ struct mfs_tria {
       for (int q = 0; q < nq; ++q)
         for (int g = 0; g < G; ++g) bg[((b * nq + q) * G + g) * CT + cc] = d->geometry[(c * nq + q) * G + g];
     }
+    h_cell_u.assign(d->cell_nodes_u, d->cell_nodes_u + ncells * nv);
+    h_cell_p.assign(d->cell_nodes_p, d->cell_nodes_p + ncells * npl);
     cell_u.upload(bu, st);
     cell_p.upload(bp, st);
     geo.upload(bg, st);
@@ -324,21 +326,22 @@ struct mfs_tria {
     have_eta = true;
   }
 
-  void resolve(const double *src, double *out, DevCon &c) {
+  void resolve(const double *src, double *out, DevCon &c, hipStream_t s) {
     const unsigned nb = (unsigned)((c.n + 255) / 256);
-    hipLaunchKernelGGL(k_resolve, dim3(nb), dim3(256), 0, st, c.n, src, c.row.p, c.off.p, c.mas.p, c.w.p, out);
+    hipLaunchKernelGGL(k_resolve, dim3(nb), dim3(256), 0, s, c.n, src, c.row.p, c.off.p, c.mas.p, c.w.p, out);
     MFS_HIP(hipGetLastError());
   }
   void assemble(const double *cb, long long nn, long long cs, const Buf<long long> &ip, const Buf<long long> &io,
-                DevCon &c, const double *src, double *dst, int ident) {
+                DevCon &c, const double *src, double *dst, int ident, hipStream_t s) {
     const unsigned nb = (unsigned)((c.n + 255) / 256);
-    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(256), 0, st, c.n, nn, cs, cb, ip.p, io.p, c.row.p, c.tptr.p,
+    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(256), 0, s, c.n, nn, cs, cb, ip.p, io.p, c.row.p, c.tptr.p,
                        c.tdof.p, c.tw.p, src, dst, ident);
     MFS_HIP(hipGetLastError());
   }
 
   // device vectors: op MFS_OP_STOKES [u|p] -> [u|p], ABLOCK u -> u, BT p -> u, MASS p -> p
-  void vmult(int op, const double *src, double *dst) {
+  void vmult(int op, const double *src, double *dst) { vmult(op, src, dst, st); }
+  void vmult(int op, const double *src, double *dst, hipStream_t s) {
     if (op != MFS_OP_STOKES && op != MFS_OP_ABLOCK && op != MFS_OP_BT && op != MFS_OP_MASS)
       throw std::invalid_argument("mfs_tria_vmult: operator must be Stokes, A block, B^T or mass");
     if (!src || !dst) throw std::invalid_argument("mfs_tria_vmult: null vector");
@@ -350,11 +353,63 @@ struct mfs_tria {
     const double *sp = op == MFS_OP_STOKES ? src + ndof_u : (rd_p ? src : nullptr);
     double *du = wr_u ? dst : nullptr;
     double *dp = op == MFS_OP_STOKES ? dst + ndof_u : (wr_p ? dst : nullptr);
-    if (rd_u) resolve(su, xu.p, con_u);
-    if (rd_p) resolve(sp, xp.p, con_p);
-    launch_cells<double>((Op)op, L, nbatch, rd_u ? xu.p : nullptr, rd_p ? xp.p : nullptr, eta.p, cbu.p, cbp.p, st);
-    if (wr_u) assemble(cbu.p, nn_u, (long long)nv * CT, iptr_u, ioff_u, con_u, su, du, op == MFS_OP_BT ? 0 : 1);
-    if (wr_p) assemble(cbp.p, nn_p, 0, iptr_p, ioff_p, con_p, sp, dp, 1);
+    if (rd_u) resolve(su, xu.p, con_u, s);
+    if (rd_p) resolve(sp, xp.p, con_p, s);
+    launch_cells<double>((Op)op, L, nbatch, rd_u ? xu.p : nullptr, rd_p ? xp.p : nullptr, eta.p, cbu.p, cbp.p, s);
+    if (wr_u) assemble(cbu.p, nn_u, (long long)nv * CT, iptr_u, ioff_u, con_u, su, du, op == MFS_OP_BT ? 0 : 1, s);
+    if (wr_p) assemble(cbp.p, nn_p, 0, iptr_p, ioff_p, con_p, sp, dp, 1, s);
+  }
+
+  // Greedy colouring of the DoFs of one block (kind 0 velocity, 1 pressure)
+  // such that no two DoFs of a colour meet in one cell's effective DoF set
+  // (its nodes and the masters of their constraint lines): A applied to the
+  // indicator of a colour then returns the diagonal on that colour.  DoFs of
+  // one node get distinct colours (node colour * components + component);
+  // constrained DoFs get -1.  Returns the number of colours.
+  int colour_dofs(int kind, std::vector<int> &colour) const {
+    const std::vector<int> &cells = kind == 0 ? h_cell_u : h_cell_p;
+    const int per = kind == 0 ? nv : npl, comps = kind == 0 ? dim : 1;
+    const long long nn = kind == 0 ? nn_u : nn_p;
+    const DevCon &c = kind == 0 ? con_u : con_p;
+    // master nodes of the lines of every node
+    std::vector<std::vector<int>> mnodes(nn);
+    for (long long dof = 0; dof < c.n; ++dof) {
+      const int r = c.h_row[dof];
+      if (r < 0) continue;
+      for (long long k = c.h_off[r]; k < c.h_off[r + 1]; ++k) mnodes[dof % nn].push_back((int)(c.h_mas[k] % nn));
+    }
+    std::vector<std::vector<int>> eff(ncells);
+    std::vector<std::vector<int>> inv(nn);
+    for (long long e = 0; e < ncells; ++e) {
+      std::vector<int> &v = eff[e];
+      for (int a = 0; a < per; ++a) {
+        const int n = cells[e * per + a];
+        v.push_back(n);
+        v.insert(v.end(), mnodes[n].begin(), mnodes[n].end());
+      }
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      for (int n : v) inv[n].push_back((int)e);
+    }
+    std::vector<int> ncol(nn, -1), stamp;
+    int ncolours = 0;
+    for (long long n = 0; n < nn; ++n) {
+      for (int e : inv[n])
+        for (int m : eff[e])
+          if (ncol[m] >= 0) {
+            if ((int)stamp.size() <= ncol[m]) stamp.resize(ncol[m] + 1, -1);
+            stamp[ncol[m]] = (int)n;
+          }
+      int k = 0;
+      while (k < (int)stamp.size() && stamp[k] == (int)n) ++k;
+      ncol[n] = k;
+      ncolours = std::max(ncolours, k + 1);
+    }
+    colour.assign(comps * nn, -1);
+    for (int cc = 0; cc < comps; ++cc)
+      for (long long n = 0; n < nn; ++n)
+        if (c.h_row[cc * nn + n] < 0) colour[cc * nn + n] = ncol[n] * comps + cc;
+    return ncolours * comps;
   }
 
   long long n_in(int op) const {
@@ -447,6 +502,536 @@ int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_ds
     t->vmult(op, d_src, d_dst);
     MFS_HIP(hipStreamSynchronize(t->st));
     return MFS_OK;
+  });
+}
+
+}  // extern "C"
+
+// ====================================================================== multigrid on general levels
+namespace {
+
+// y = A x (add: y += A x), CSR
+__global__ void k_spmv(const long long rows, const long long *__restrict__ ptr, const long long *__restrict__ col,
+                       const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+                       const int add) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double a = 0;
+  for (long long k = ptr[i]; k < ptr[i + 1]; ++k) a += val[k] * x[col[k]];
+  y[i] = add ? y[i] + a : a;
+}
+__global__ void k_probe_set(const long long n, const int *__restrict__ colour, const int c, double *__restrict__ x) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = colour[i] == c ? 1.0 : 0.0;
+}
+__global__ void k_probe_get(const long long n, const int *__restrict__ colour, const int c,
+                            const double *__restrict__ y, double *__restrict__ diag) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n && colour[i] == c) diag[i] = y[i];
+}
+__global__ void k_invert(const long long n, double *__restrict__ x) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 1.0 / x[i];
+}
+inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+
+struct DevCsr {
+  long long rows = 0, cols = 0;
+  Buf<long long> ptr, col;
+  Buf<double> val;
+  void set(long long r, long long c, const std::vector<long long> &p, const std::vector<long long> &ci,
+           const std::vector<double> &v, hipStream_t st) {
+    rows = r;
+    cols = c;
+    ptr.upload(p, st);
+    col.upload(ci, st);
+    val.upload(v, st);
+  }
+  void apply(const double *x, double *y, bool add, hipStream_t st) const {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_spmv, dim3(nblk(rows)), dim3(256), 0, st, rows, ptr.p, col.p, val.p, x, y, add ? 1 : 0);
+    MFS_HIP(hipGetLastError());
+  }
+};
+
+struct Cheb {
+  double lmin = 1, lmax = 1, theta = 1, delta = 0;
+  int degree = 4, its = 0;
+};
+
+struct TLevel {
+  mfs_tria *t = nullptr;
+  long long n[2] = {0, 0};
+  Buf<double> dinv[2], x[2], b[2], r[2], d[2];
+  Buf<int> colour[2];
+  int ncolours[2] = {0, 0};
+  Cheb ch[2];
+  DevCsr P[2], R[2];  // prolongation from the next coarser level and its transpose
+};
+
+}  // namespace
+
+struct mfs_tria_gmg {
+  hipStream_t st = nullptr;
+  std::vector<TLevel> lv;
+  Buf<double> dwork, dout, rtmp;
+  bool built = false;
+
+  ~mfs_tria_gmg() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  double dot(const double *a, const double *b, long long n) {
+    v_dot<double>(a, b, n, dwork.p, dout.p, st);
+    double h = 0;
+    MFS_HIP(hipMemcpyAsync(&h, dout.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    MFS_HIP(hipStreamSynchronize(st));
+    return h;
+  }
+  void op(int kind, int l, const double *x, double *y) {
+    lv[l].t->vmult(kind == 0 ? MFS_OP_ABLOCK : MFS_OP_MASS, x, y, st);
+  }
+
+  static void csr_from(const mfs_csr &m, long long rows, long long cols, std::vector<long long> &p,
+                       std::vector<long long> &c, std::vector<double> &v, const char *what) {
+    if (m.rows != rows || m.cols != cols)
+      throw std::invalid_argument(std::string(what) + ": prolongation size does not match the levels");
+    if (!m.row_ptr || (m.row_ptr[rows] > 0 && (!m.col_idx || !m.values)))
+      throw std::invalid_argument(std::string(what) + ": null CSR array");
+    p.assign(m.row_ptr, m.row_ptr + rows + 1);
+    if (p[0] != 0) throw std::invalid_argument(std::string(what) + ": row_ptr[0] must be 0");
+    for (long long i = 0; i < rows; ++i)
+      if (p[i + 1] < p[i]) throw std::invalid_argument(std::string(what) + ": row_ptr not increasing");
+    c.assign(m.col_idx, m.col_idx + p[rows]);
+    v.assign(m.values, m.values + p[rows]);
+    for (long long k = 0; k < p[rows]; ++k)
+      if (c[k] < 0 || c[k] >= cols) throw std::invalid_argument(std::string(what) + ": column out of range");
+  }
+  static void transpose(long long rows, long long cols, const std::vector<long long> &p, const std::vector<long long> &c,
+                        const std::vector<double> &v, std::vector<long long> &tp, std::vector<long long> &tc,
+                        std::vector<double> &tv) {
+    tp.assign(cols + 1, 0);
+    for (long long k = 0; k < p[rows]; ++k) ++tp[c[k] + 1];
+    for (long long j = 0; j < cols; ++j) tp[j + 1] += tp[j];
+    tc.assign(p[rows], 0);
+    tv.assign(p[rows], 0.0);
+    std::vector<long long> fill(tp.begin(), tp.end() - 1);
+    for (long long i = 0; i < rows; ++i)
+      for (long long k = p[i]; k < p[i + 1]; ++k) {
+        const long long pos = fill[c[k]]++;
+        tc[pos] = i;
+        tv[pos] = v[k];
+      }
+  }
+
+  void create(int n_levels, mfs_tria *const *levels, const mfs_csr *pu, const mfs_csr *pp) {
+    if (n_levels < 1 || !levels) throw std::invalid_argument("mfs_tria_gmg_create: no levels");
+    if (n_levels > 1 && (!pu || !pp)) throw std::invalid_argument("mfs_tria_gmg_create: null prolongations");
+    for (int l = 0; l < n_levels; ++l) {
+      if (!levels[l]) throw std::invalid_argument("mfs_tria_gmg_create: null level");
+      if (levels[l]->dim != levels[0]->dim) throw std::invalid_argument("mfs_tria_gmg_create: levels of mixed dim");
+    }
+    std::vector<std::vector<long long>> P[2][3], R[2][3];
+    lv.resize(n_levels);
+    std::vector<long long> p, c, tp, tc;
+    std::vector<double> v, tv;
+    MFS_HIP(hipStreamCreate(&st));
+    for (int l = 0; l < n_levels; ++l) {
+      TLevel &L = lv[l];
+      L.t = levels[l];
+      L.n[0] = L.t->ndof_u;
+      L.n[1] = L.t->ndof_p;
+      for (int k = 0; k < 2; ++k) {
+        L.dinv[k].alloc(L.n[k]);
+        L.x[k].alloc(L.n[k]);
+        L.b[k].alloc(L.n[k]);
+        L.r[k].alloc(L.n[k]);
+        L.d[k].alloc(L.n[k]);
+      }
+      if (l == 0) continue;
+      for (int k = 0; k < 2; ++k) {
+        const mfs_csr &m = k == 0 ? pu[l - 1] : pp[l - 1];
+        csr_from(m, L.n[k], lv[l - 1].n[k], p, c, v, k == 0 ? "velocity" : "pressure");
+        transpose(L.n[k], lv[l - 1].n[k], p, c, v, tp, tc, tv);
+        L.P[k].set(L.n[k], lv[l - 1].n[k], p, c, v, st);
+        L.R[k].set(lv[l - 1].n[k], L.n[k], tp, tc, tv, st);
+      }
+    }
+    long long nmax = 0;
+    for (auto &L : lv) nmax = std::max(nmax, L.n[0] + L.n[1]);
+    dwork.alloc(dot_work_size());
+    dout.alloc(1);
+    rtmp.alloc(nmax);
+    MFS_HIP(hipStreamSynchronize(st));
+  }
+
+  // exact diagonal of C^T A C (identity on constrained DoFs) by probing
+  // colour classes with the level operator
+  void diagonal(int kind, int l) {
+    TLevel &L = lv[l];
+    const long long n = L.n[kind];
+    if (L.ncolours[kind] == 0) {
+      std::vector<int> col;
+      L.ncolours[kind] = L.t->colour_dofs(kind, col);
+      L.colour[kind].upload(col, st);
+    }
+    v_fill<double>(L.dinv[kind].p, 1.0, n, st);
+    for (int cc = 0; cc < L.ncolours[kind]; ++cc) {
+      hipLaunchKernelGGL(k_probe_set, dim3(nblk(n)), dim3(256), 0, st, n, L.colour[kind].p, cc, L.x[kind].p);
+      op(kind, l, L.x[kind].p, L.r[kind].p);
+      hipLaunchKernelGGL(k_probe_get, dim3(nblk(n)), dim3(256), 0, st, n, L.colour[kind].p, cc, L.r[kind].p,
+                         L.dinv[kind].p);
+    }
+    MFS_HIP(hipGetLastError());
+  }
+
+  static double tridiag_eig(const std::vector<double> &d, const std::vector<double> &e, bool want_max) {
+    // bisection with Sturm counts on the symmetric tridiagonal (d, e)
+    double lo = 1e300, hi = -1e300;
+    for (size_t i = 0; i < d.size(); ++i) {
+      const double r = (i ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < d.size() ? std::fabs(e[i]) : 0.0);
+      lo = std::min(lo, d[i] - r);
+      hi = std::max(hi, d[i] + r);
+    }
+    const int target = want_max ? (int)d.size() : 1;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      int cnt = 0;
+      double q = 1.0;
+      for (size_t i = 0; i < d.size(); ++i) {
+        q = d[i] - mid - (i ? e[i - 1] * e[i - 1] / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0) ++cnt;
+      }
+      if (cnt >= target)
+        hi = mid;
+      else
+        lo = mid;
+      if (hi - lo <= 1e-15 * std::max(1.0, std::fabs(hi))) break;
+    }
+    return 0.5 * (lo + hi);
+  }
+
+  // Chebyshev set-up: Jacobi-preconditioned CG from the mean-free pattern
+  // (i mod 11), stopped at |r| < 1e-2 or max_its; Lanczos tridiagonal from the
+  // CG coefficients; lambda_max scaled by 1.2
+  void estimate(int kind, int l, int max_its, double range) {
+    TLevel &L = lv[l];
+    Cheb &ch = L.ch[kind];
+    const long long n = L.n[kind];
+    double *dinv = L.dinv[kind].p, *r = L.r[kind].p, *z = L.d[kind].p, *p = L.x[kind].p, *q = L.b[kind].p;
+    std::vector<double> pat(n);
+    double mean = 0;
+    for (long long i = 0; i < n; ++i) mean += (double)(i % 11);
+    mean /= (double)n;
+    for (long long i = 0; i < n; ++i) pat[i] = (double)(i % 11) - mean;
+    MFS_HIP(hipMemcpyAsync(r, pat.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    std::vector<double> alphas, betas;
+    double res = std::sqrt(dot(r, r, n));
+    if (res >= 1e-2) {
+      v_cheb_first<double>(dinv, r, z, p, 1.0, true, n, st);
+      double rz = dot(r, z, n);
+      for (int it = 1; it <= max_its; ++it) {
+        op(kind, l, p, q);
+        const double alpha = rz / dot(p, q, n);
+        v_axpby<double>(-alpha, q, 1.0, r, n, st);
+        alphas.push_back(alpha);
+        res = std::sqrt(dot(r, r, n));
+        if (res < 1e-2 || it == max_its) break;
+        v_mul<double>(dinv, r, z, n, st);
+        const double rz2 = dot(r, z, n);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        v_axpby<double>(1.0, z, beta, p, n, st);
+        betas.push_back(beta);
+      }
+    }
+    ch.its = (int)alphas.size();
+    if (alphas.empty()) {
+      ch.lmin = ch.lmax = 1.0;
+    } else {
+      const size_t m = alphas.size();
+      std::vector<double> d(m), e(m > 1 ? m - 1 : 0);
+      d[0] = 1.0 / alphas[0];
+      for (size_t k = 1; k < m; ++k) d[k] = 1.0 / alphas[k] + betas[k - 1] / alphas[k - 1];
+      for (size_t k = 0; k + 1 < m; ++k) e[k] = std::sqrt(betas[k]) / alphas[k];
+      ch.lmin = tridiag_eig(d, e, false);
+      ch.lmax = 1.2 * tridiag_eig(d, e, true);
+    }
+    const double a = range > 1.0 ? ch.lmax / range : std::min(0.9 * ch.lmax, ch.lmin);
+    ch.delta = 0.5 * (ch.lmax - a);
+    ch.theta = 0.5 * (ch.lmax + a);
+  }
+
+  void build() {
+    for (int l = 0; l < (int)lv.size(); ++l) {
+      for (int k = 0; k < 2; ++k) {
+        if (!lv[l].t->have_eta) throw std::runtime_error("mfs_tria_gmg_build: viscosity not set on every level");
+        diagonal(k, l);
+        hipLaunchKernelGGL(k_invert, dim3(nblk(lv[l].n[k])), dim3(256), 0, st, lv[l].n[k], lv[l].dinv[k].p);
+        const bool coarse = l == 0;
+        lv[l].ch[k].degree = coarse ? 8 : 4;
+        estimate(k, l, coarse ? 100 : 10, coarse ? 1e-3 : 15.0);
+      }
+    }
+    MFS_HIP(hipStreamSynchronize(st));
+    built = true;
+  }
+
+  // Chebyshev semi-iteration with Jacobi preconditioning, `steps`
+  // applications, the first from zero when x_zero
+  void smooth(int kind, int l, double *x, const double *b, bool x_zero, int steps) {
+    TLevel &L = lv[l];
+    const Cheb &ch = L.ch[kind];
+    const long long n = L.n[kind];
+    double *dinv = L.dinv[kind].p, *r = L.r[kind].p, *d = L.d[kind].p;
+    auto resid = [&] {
+      op(kind, l, x, r);
+      v_axpby<double>(1.0, b, -1.0, r, n, st);  // r = b - A x
+    };
+    for (int s = 0; s < steps; ++s) {
+      if (x_zero && s == 0) {
+        v_cheb_first<double>(dinv, b, d, x, 1.0 / ch.theta, true, n, st);
+      } else {
+        resid();
+        v_cheb_first<double>(dinv, r, d, x, 1.0 / ch.theta, false, n, st);
+      }
+      if (ch.degree < 2 || std::fabs(ch.delta) < 1e-40) continue;
+      const double sigma = ch.theta / ch.delta;
+      double rho_old = ch.delta / ch.theta;
+      for (int k = 1; k < ch.degree; ++k) {
+        const double rho = 1.0 / (2.0 * sigma - rho_old);
+        resid();
+        v_cheb_step<double>(dinv, r, d, x, rho * rho_old, 2.0 * rho / ch.delta, n, st);
+        rho_old = rho;
+      }
+    }
+  }
+
+  void vcycle_level(int kind, int l) {
+    TLevel &L = lv[l];
+    const int steps = kind == 0 ? 1 : 4;
+    double *x = L.x[kind].p, *b = L.b[kind].p, *r = L.r[kind].p;
+    smooth(kind, l, x, b, true, steps);
+    if (l == 0) return;
+    op(kind, l, x, r);
+    v_axpby<double>(1.0, b, -1.0, r, L.n[kind], st);
+    TLevel &C = lv[l - 1];
+    L.R[kind].apply(r, C.b[kind].p, false, st);
+    vcycle_level(kind, l - 1);
+    L.P[kind].apply(C.x[kind].p, x, true, st);
+    smooth(kind, l, x, b, false, steps);
+  }
+  // one V-cycle on device vectors of the finest level
+  void vcycle(int kind, const double *src, double *dst) {
+    if (!built) throw std::runtime_error("multigrid not built (call mfs_tria_gmg_build)");
+    TLevel &L = lv.back();
+    MFS_HIP(hipMemcpyAsync(L.b[kind].p, src, L.n[kind] * sizeof(double), hipMemcpyDeviceToDevice, st));
+    vcycle_level(kind, (int)lv.size() - 1);
+    MFS_HIP(hipMemcpyAsync(dst, L.x[kind].p, L.n[kind] * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+
+  // BlockSchurPreconditioner::vmult, cheap variant (block_stokes_preconditioner.h:220-255):
+  // p = -S^{-1} r_p, u = A^{-1} (r_u - B^T p), one V-cycle each
+  void block_prec(const double *src, double *dst) {
+    TLevel &L = lv.back();
+    const long long nu = L.n[0], np = L.n[1];
+    vcycle(1, src + nu, rtmp.p);
+    v_scale_neg_copy<double>(rtmp.p, dst + nu, np, st);
+    L.t->vmult(MFS_OP_BT, dst + nu, rtmp.p, st);
+    v_axpby<double>(1.0, src, -1.0, rtmp.p, nu, st);  // r_u - B^T p
+    vcycle(0, rtmp.p, dst);
+  }
+
+  // right-preconditioned restarted GMRES (modified Gram-Schmidt), zero start
+  void gmres(const double *b, double *x, double tol, int maxit, int restart, int &its, double &res) {
+    TLevel &L = lv.back();
+    const long long n = L.n[0] + L.n[1];
+    std::vector<Buf<double>> V(restart + 1), Z(restart);
+    for (auto &v : V) v.alloc(n);
+    for (auto &z : Z) z.alloc(n);
+    Buf<double> w;
+    w.alloc(n);
+    v_fill<double>(x, 0.0, n, st);
+    its = 0;
+    // r = b - K x with x = 0
+    MFS_HIP(hipMemcpyAsync(V[0].p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double beta = std::sqrt(dot(V[0].p, V[0].p, n));
+    while (its < maxit && beta > tol) {
+      std::vector<double> H((restart + 1) * restart, 0.0), cs(restart), sn(restart), g(restart + 1, 0.0);
+      auto h = [&](int i, int k) -> double & { return H[i * restart + k]; };
+      v_axpby<double>(0.0, V[0].p, 1.0 / beta, V[0].p, n, st);
+      g[0] = beta;
+      int k = 0;
+      while (k < restart && its < maxit) {
+        block_prec(V[k].p, Z[k].p);
+        L.t->vmult(MFS_OP_STOKES, Z[k].p, w.p, st);
+        for (int i = 0; i <= k; ++i) {
+          h(i, k) = dot(w.p, V[i].p, n);
+          v_axpby<double>(-h(i, k), V[i].p, 1.0, w.p, n, st);
+        }
+        h(k + 1, k) = std::sqrt(dot(w.p, w.p, n));
+        MFS_HIP(hipMemcpyAsync(V[k + 1].p, w.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (h(k + 1, k) > 0) v_axpby<double>(0.0, w.p, 1.0 / h(k + 1, k), V[k + 1].p, n, st);
+        for (int i = 0; i < k; ++i) {
+          const double t = cs[i] * h(i, k) + sn[i] * h(i + 1, k);
+          h(i + 1, k) = -sn[i] * h(i, k) + cs[i] * h(i + 1, k);
+          h(i, k) = t;
+        }
+        const double den = std::hypot(h(k, k), h(k + 1, k));
+        cs[k] = h(k, k) / den;
+        sn[k] = h(k + 1, k) / den;
+        h(k, k) = den;
+        h(k + 1, k) = 0.0;
+        g[k + 1] = -sn[k] * g[k];
+        g[k] = cs[k] * g[k];
+        ++k;
+        ++its;
+        if (std::fabs(g[k]) <= tol) break;
+      }
+      std::vector<double> y(k, 0.0);
+      for (int i = k - 1; i >= 0; --i) {
+        double a = g[i];
+        for (int j = i + 1; j < k; ++j) a -= h(i, j) * y[j];
+        y[i] = a / h(i, i);
+      }
+      for (int i = 0; i < k; ++i) v_axpby<double>(y[i], Z[i].p, 1.0, x, n, st);
+      beta = std::fabs(g[k]);
+      if (its < maxit && beta > tol) {
+        // restart from the true residual
+        L.t->vmult(MFS_OP_STOKES, x, V[0].p, st);
+        v_axpby<double>(1.0, b, -1.0, V[0].p, n, st);
+        beta = std::sqrt(dot(V[0].p, V[0].p, n));
+      }
+    }
+    res = beta;
+    MFS_HIP(hipStreamSynchronize(st));
+  }
+};
+
+extern "C" {
+
+int mfs_tria_gmg_create(int n_levels, mfs_tria *const *levels, const mfs_csr *prolong_u, const mfs_csr *prolong_p,
+                        mfs_tria_gmg **out) {
+  return tguard([&] {
+    if (!out) throw std::invalid_argument("mfs_tria_gmg_create: null output");
+    *out = nullptr;
+    mfs_tria_gmg *g = new mfs_tria_gmg();
+    try {
+      g->create(n_levels, levels, prolong_u, prolong_p);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_gmg_destroy(mfs_tria_gmg *g) {
+  return tguard([&] {
+    if (g) {
+      (void)hipStreamSynchronize(g->st);
+      delete g;
+    }
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_gmg_build(mfs_tria_gmg *g) {
+  return tguard([&] {
+    if (!g) throw std::invalid_argument("null mfs_tria_gmg");
+    g->build();
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_gmg_info(mfs_tria_gmg *g, int level, int kind, double out[4], double *diagonal) {
+  return tguard([&] {
+    if (!g) throw std::invalid_argument("null mfs_tria_gmg");
+    if (!g->built) throw std::runtime_error("multigrid not built (call mfs_tria_gmg_build)");
+    if (level < 0 || level >= (int)g->lv.size() || (kind != 0 && kind != 1))
+      throw std::invalid_argument("mfs_tria_gmg_info: level or kind out of range");
+    const TLevel &L = g->lv[level];
+    if (out) {
+      out[0] = L.ch[kind].lmin;
+      out[1] = L.ch[kind].lmax;
+      out[2] = L.ch[kind].its;
+      out[3] = L.ncolours[kind];
+    }
+    if (diagonal) {
+      // the diagonal itself (the smoother stores its inverse)
+      Buf<double> tmp;
+      tmp.alloc(L.n[kind]);
+      MFS_HIP(hipMemcpyAsync(tmp.p, L.dinv[kind].p, L.n[kind] * sizeof(double), hipMemcpyDeviceToDevice, g->st));
+      hipLaunchKernelGGL(k_invert, dim3(nblk(L.n[kind])), dim3(256), 0, g->st, L.n[kind], tmp.p);
+      MFS_HIP(hipMemcpyAsync(diagonal, tmp.p, L.n[kind] * sizeof(double), hipMemcpyDeviceToHost, g->st));
+      MFS_HIP(hipStreamSynchronize(g->st));
+    }
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *dst) {
+  return tguard([&] {
+    if (!g) throw std::invalid_argument("null mfs_tria_gmg");
+    if (kind != 0 && kind != 1) throw std::invalid_argument("mfs_tria_gmg_vcycle: kind must be 0 (A) or 1 (Schur)");
+    if (!src || !dst) throw std::invalid_argument("mfs_tria_gmg_vcycle: null vector");
+    const long long n = g->lv.back().n[kind];
+    Buf<double> a, b;
+    a.alloc(n);
+    b.alloc(n);
+    MFS_HIP(hipMemcpyAsync(a.p, src, n * sizeof(double), hipMemcpyHostToDevice, g->st));
+    g->vcycle(kind, a.p, b.p);
+    MFS_HIP(hipMemcpyAsync(dst, b.p, n * sizeof(double), hipMemcpyDeviceToHost, g->st));
+    MFS_HIP(hipStreamSynchronize(g->st));
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_solve_params *prm,
+                   mfs_solve_result *res) {
+  return tguard([&] {
+    if (!g) throw std::invalid_argument("null mfs_tria_gmg");
+    if (!rhs || !sol || !prm || !res) throw std::invalid_argument("mfs_tria_solve: null argument");
+    if (!g->built) throw std::runtime_error("multigrid not built (call mfs_tria_gmg_build)");
+    if (prm->restart < 1 || prm->restart > 1000) throw std::invalid_argument("mfs_tria_solve: restart must be in [1, 1000]");
+    if (prm->n_cheap_steps < 1) throw std::invalid_argument("mfs_tria_solve: needs cheap steps");
+    std::memset(res, 0, sizeof(*res));
+    const long long n = g->lv.back().n[0] + g->lv.back().n[1];
+    Buf<double> b, x;
+    b.alloc(n);
+    x.alloc(n);
+    MFS_HIP(hipMemcpyAsync(b.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, g->st));
+    const double r0 = std::sqrt(g->dot(b.p, b.p, n));
+    const double tol = prm->tolerance * r0;
+    hipEvent_t e0, e1;
+    MFS_HIP(hipEventCreate(&e0));
+    MFS_HIP(hipEventCreate(&e1));
+    MFS_HIP(hipEventRecord(e0, g->st));
+    int its = 0;
+    double fin = 0;
+    g->gmres(b.p, x.p, tol, prm->n_cheap_steps, prm->restart, its, fin);
+    MFS_HIP(hipEventRecord(e1, g->st));
+    MFS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    MFS_HIP(hipMemcpyAsync(sol, x.p, n * sizeof(double), hipMemcpyDeviceToHost, g->st));
+    MFS_HIP(hipStreamSynchronize(g->st));
+    res->cheap_iterations = its;
+    res->a_preconditioner_applications = its;
+    res->s_preconditioner_applications = its;
+    res->converged = fin <= tol ? 1 : 0;
+    res->initial_nonlinear_residual = r0;
+    res->r0_norm = r0;
+    res->final_linear_residual = fin;
+    res->cheap_final_residual = fin;
+    res->solver_tolerance = tol;
+    res->solve_seconds = ms * 1e-3;
+    res->cheap_seconds = ms * 1e-3;
+    return res->converged ? MFS_OK : MFS_NO_CONVERGENCE;
   });
 }
 

@@ -40,7 +40,8 @@ EXPORTED = [
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
     "mfs_box_dof_index", "mfs_tria_create", "mfs_tria_destroy", "mfs_tria_sizes", "mfs_tria_set_viscosity",
-    "mfs_tria_vmult", "mfs_tria_vmult_device",
+    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_build",
+    "mfs_tria_gmg_info", "mfs_tria_gmg_vcycle", "mfs_tria_solve",
 ]
 
 
@@ -657,5 +658,80 @@ class TriaOperators:
             if self._t:
                 self.lib.mfs_tria_destroy(self._t)
                 self._t = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Csr(C.Structure):
+    _fields_ = [("rows", C.c_longlong), ("cols", C.c_longlong), ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p),
+                ("values", C.c_void_p)]
+
+
+class TriaMultigrid:
+    """Global-coarsening multigrid over general levels (coarse first) and the
+    cheap Stokes solve on the finest level
+    (source/simulator/solver/stokes_matrix_free_global_coarsening.cc:800-883,
+    :1259-1313).  ``levels``: ``TriaOperators`` with their viscosity set;
+    ``prolong_u`` / ``prolong_p``: scipy CSR prolongations, entry l mapping
+    level l-1 to level l (entry 0 unused), as ``aspect_amd.tria.hierarchy``
+    returns them."""
+
+    def __init__(self, levels, prolong_u, prolong_p):
+        self.lib = load_library()
+        self.levels = list(levels)   # kept alive: the multigrid object refers to them
+        n = len(self.levels)
+        keep = []
+
+        def csr(P):
+            P = P.tocsr()
+            a = [np.ascontiguousarray(P.indptr, np.int64), np.ascontiguousarray(P.indices, np.int64),
+                 np.ascontiguousarray(P.data, np.float64)]
+            keep.extend(a)
+            m = Csr()
+            m.rows, m.cols = P.shape
+            m.row_ptr, m.col_idx, m.values = a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data
+            return m
+
+        cu = (Csr * max(n - 1, 1))(*[csr(prolong_u[l]) for l in range(1, n)])
+        cp = (Csr * max(n - 1, 1))(*[csr(prolong_p[l]) for l in range(1, n)])
+        lv = (C.c_void_p * n)(*[t._t.value for t in self.levels])
+        self._g = C.c_void_p()
+        _check(self.lib.mfs_tria_gmg_create(n, lv, cu, cp, C.byref(self._g)))
+
+    def build(self):
+        _check(self.lib.mfs_tria_gmg_build(self._g))
+
+    def info(self, level, kind):
+        out = (C.c_double * 4)()
+        t = self.levels[level]
+        diag = np.empty(t.nu if kind == 0 else t.np)
+        _check(self.lib.mfs_tria_gmg_info(self._g, level, kind, out, _ptr(diag)))
+        return {"lmin": out[0], "lmax": out[1], "its": int(out[2]), "colours": int(out[3])}, diag
+
+    def vcycle(self, kind, src):
+        t = self.levels[-1]
+        x = _vec(src, t.nu if kind == 0 else t.np, "source vector")
+        y = np.empty_like(x)
+        _check(self.lib.mfs_tria_gmg_vcycle(self._g, kind, _ptr(x), _ptr(y)))
+        return y
+
+    def solve(self, rhs, tolerance=1e-8, n_cheap_steps=1000, restart=100):
+        t = self.levels[-1]
+        b = _vec(rhs, t.nu + t.np, "right-hand side")
+        x = np.empty_like(b)
+        prm = _params(tolerance, n_cheap_steps, 0, restart, 1e-2, 1e-6, "GMRES", 2)
+        res = SolveResult()
+        rc = _check(self.lib.mfs_tria_solve(self._g, _ptr(b), _ptr(x), C.byref(prm), C.byref(res)))
+        return x, SolverOutputs(res.initial_nonlinear_residual, res.final_linear_residual,
+                                res.pressure_normalization_adjustment, res.cheap_iterations,
+                                res.expensive_iterations, res.a_preconditioner_applications,
+                                res.s_preconditioner_applications, bool(res.converged), res.solver_tolerance,
+                                res.r0_norm, res.solve_seconds, res.cheap_final_residual, res.cheap_seconds), rc
+
+    def __del__(self):
+        try:
+            if self._g:
+                self.lib.mfs_tria_gmg_destroy(self._g)
+                self._g = C.c_void_p()
         except Exception:
             pass

@@ -490,3 +490,94 @@ def _close(lines):
     masters = np.concatenate(ms) if ms else np.zeros(0, np.int64)
     weights = np.concatenate(ws) if ws else np.zeros(0)
     return dofs, offs, masters.astype(np.int64), weights.astype(float)
+
+
+# --------------------------------------------------------------------- level hierarchy
+def coarsening_sequence(tria):
+    """Level meshes of the global-coarsening hierarchy, coarse first
+    (MGTransferGlobalCoarseningTools::create_geometric_coarsening_sequence,
+    used at stokes_matrix_free_global_coarsening.cc:1467): coarsening every
+    cell whose siblings are all active turns the adaptive level (when there
+    is one) into the uniform mesh of the global refinement, then each step
+    removes one global refinement."""
+    levels = [Tria(tria.coarse, k) for k in range(tria.g + 1)]
+    if tria.adapted:
+        levels.append(tria)
+    return levels
+
+
+def _cell_key(root, level, idx):
+    return (int(root), int(level)) + tuple(int(i) for i in idx)
+
+
+def transfer(fine, coarse, mf, mc):
+    """Prolongation matrices (scipy CSR) from level mesh mc (of Tria coarse) to
+    mf (of Tria fine), velocity and pressure: the fine interpolant of the
+    coarse finite element function (the spaces are nested), as
+    MGTwoLevelTransfer applies it (read the coarse vector with its
+    constraints, embed cell by cell, write the fine vector with constrained
+    entries zero).  Restriction is the transpose."""
+    from scipy.sparse import csr_matrix
+    dim = fine.dim
+    lookup = {_cell_key(coarse.root[c], coarse.level[c], coarse.idx[c]): c for c in range(coarse.n_cells)}
+    out = []
+    for nodes1d, cells_f, cells_c, nn_f, nn_c, comps, con_f, con_c in (
+            (np.array([0.0, 0.5, 1.0]), mf.cell_u, mc.cell_u, mf.n_nodes_u, mc.n_nodes_u, dim, mf.con_u, mc.con_u),
+            (np.array([0.0, 1.0]), mf.cell_p, mc.cell_p, mf.n_nodes_p, mc.n_nodes_p, 1, mf.con_p, mc.con_p)):
+        loc = _lattice(dim, nodes1d)
+        seen = np.zeros(nn_f, dtype=bool)
+        rows, cols, vals = [], [], []
+        for c in range(fine.n_cells):
+            lev, idx = fine.level[c], fine.idx[c]
+            key = _cell_key(fine.root[c], lev, idx)
+            if key in lookup:
+                cc, shift, scale = lookup[key], np.zeros(dim), 1.0
+            else:
+                cc = lookup[_cell_key(fine.root[c], lev - 1, idx // 2)]
+                shift, scale = (idx % 2) * 0.5, 0.5
+            new = ~seen[cells_f[c]]
+            if not np.any(new):
+                continue
+            xi = shift + scale * loc[new]
+            w = _basis(dim, nodes1d, xi)
+            w[np.abs(w) < 1e-14] = 0.0
+            fn = cells_f[c][new]
+            seen[fn] = True
+            r, a = np.nonzero(w)
+            rows.append(fn[r])
+            cols.append(cells_c[cc][a])
+            vals.append(w[r, a])
+        rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+        R = np.concatenate([rows + k * nn_f for k in range(comps)])
+        Cc = np.concatenate([cols + k * nn_c for k in range(comps)])
+        V = np.tile(vals, comps)
+        P = csr_matrix((V, (R, Cc)), shape=(comps * nn_f, comps * nn_c))
+        # coarse constraints: constrained coarse DoFs through their masters
+        dofs, offs, masters, weights = con_c
+        free = np.ones(comps * nn_c, dtype=bool)
+        free[dofs] = False
+        fi = np.nonzero(free)[0]
+        Cm = csr_matrix((np.concatenate([np.ones(len(fi)), weights]),
+                         (np.concatenate([fi, np.repeat(dofs, np.diff(offs))]), np.concatenate([fi, masters]))),
+                        shape=(comps * nn_c, comps * nn_c))
+        P = (P @ Cm).tocsr()
+        keep = np.ones(comps * nn_f)
+        keep[con_f[0]] = 0.0
+        P = csr_matrix(P.multiply(keep[:, None]))
+        P.eliminate_zeros()
+        P.sort_indices()
+        out.append(P)
+    return out[0], out[1]
+
+
+def hierarchy(tria, bc=None):
+    """(level meshes coarse first, velocity prolongations, pressure
+    prolongations); prolongation l maps level l-1 to level l."""
+    trias = coarsening_sequence(tria)
+    meshes = [t.build(bc) for t in trias]
+    pu, pp = [None], [None]
+    for l in range(1, len(trias)):
+        a, b = transfer(trias[l], trias[l - 1], meshes[l], meshes[l - 1])
+        pu.append(a)
+        pp.append(b)
+    return trias, meshes, pu, pp

@@ -389,6 +389,45 @@ int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst);
 /* The same on device vectors (hipMalloc'd by the caller). */
 int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_dst);
 
+/* Global-coarsening multigrid over general levels (coarse first) and the
+ * cheap Stokes solve on the finest one (stokes_matrix_free_global_coarsening.cc:
+ * smoothers :800-846, coarse solver :877-883, transfers :1688-1728, cheap GMRES
+ * :1259-1313).  Prolongation l (l = 0 .. n_levels-2) maps level l to level
+ * l+1 as a CSR matrix over the DoFs of the two levels: the fine interpolant of
+ * the coarse function, read with the coarse constraints, zero on constrained
+ * fine DoFs (MGTwoLevelTransfer); restriction is its transpose.  The levels
+ * stay owned by the caller and must outlive the multigrid object. */
+typedef struct mfs_tria_gmg mfs_tria_gmg;
+typedef struct {
+  long long rows, cols;
+  const long long *row_ptr; /* rows + 1 */
+  const long long *col_idx;
+  const double *values;
+} mfs_csr;
+int mfs_tria_gmg_create(int n_levels, mfs_tria *const *levels, const mfs_csr *prolong_u, const mfs_csr *prolong_p,
+                        mfs_tria_gmg **out);
+int mfs_tria_gmg_destroy(mfs_tria_gmg *g);
+/* Diagonals of every level operator (exact, constraints included: probing by
+ * DoF colours; 1 on constrained DoFs) and the Chebyshev set-up: levels > 0
+ * degree 4 over [lambda_max / 15, lambda_max] from 10 eigenvalue iterations,
+ * level 0 degree 8, range 1e-3, 100 iterations.  The viscosity must be set on
+ * every level (mfs_tria_set_viscosity). */
+int mfs_tria_gmg_build(mfs_tria_gmg *g);
+/* out = {lambda_min, lambda_max, eigenvalue iterations, colours} of level /
+ * kind (0 A block, 1 Schur complement); diagonal (may be NULL): the level
+ * operator's diagonal (host, velocity or pressure DoFs of the level). */
+int mfs_tria_gmg_info(mfs_tria_gmg *g, int level, int kind, double out[4], double *diagonal);
+/* One V-cycle (kind 0: A block, 1 smoothing step; 1: Schur complement, 4 steps)
+ * on host vectors of the finest level. */
+int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *dst);
+/* Cheap-phase solve on the finest level: right-preconditioned GMRES
+ * (restart prm->restart, at most prm->n_cheap_steps iterations, zero initial
+ * guess, tolerance prm->tolerance * |rhs|) with the cheap block preconditioner
+ * (one A and one Schur V-cycle).  Host vectors [u|p]; the other fields of prm
+ * are not used.  MFS_NO_CONVERGENCE when the tolerance is not reached. */
+int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_solve_params *prm,
+                   mfs_solve_result *res);
+
 #ifdef __cplusplus
 }
 #endif

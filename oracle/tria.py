@@ -161,3 +161,185 @@ def resolve(con, x):
     """x with constrained entries replaced by their masters' combination."""
     Cm, _ = constraint_map(con, len(x))
     return Cm @ x
+
+
+# ---------------------------------------------------------------------- multigrid
+def _tridiag_eigs(alphas, betas):
+    m = len(alphas)
+    d = np.empty(m)
+    d[0] = 1.0 / alphas[0]
+    for k in range(1, m):
+        d[k] = 1.0 / alphas[k] + betas[k - 1] / alphas[k - 1]
+    e = np.array([np.sqrt(betas[k]) / alphas[k] for k in range(m - 1)])
+    T = np.diag(d) + (np.diag(e, 1) + np.diag(e, -1) if m > 1 else 0.0)
+    w = np.linalg.eigvalsh(T)
+    return w[0], w[-1]
+
+
+class Chebyshev:
+    """PreconditionChebyshev with the inverse diagonal (deal.II, not vendored;
+    parameters of stokes_matrix_free_global_coarsening.cc:800-846): eigenvalue
+    estimate by Jacobi-preconditioned CG from the mean-free pattern (i mod 11),
+    stopped at |r| < 1e-2 or max_its, Lanczos tridiagonal from the CG
+    coefficients, lambda_max scaled by 1.2; interval [lambda_max / range,
+    lambda_max] (range > 1) or [min(0.9 lambda_max, lambda_min), lambda_max]."""
+
+    def __init__(self, M, dinv, degree, max_its, rng_):
+        self.M, self.dinv, self.degree = M, dinv, degree
+        n = M.shape[0]
+        pat = (np.arange(n) % 11).astype(float)
+        r = pat - pat.mean()
+        alphas, betas = [], []
+        if np.sqrt(r @ r) >= 1e-2:
+            z = dinv * r
+            p = z.copy()
+            rz = r @ z
+            for it in range(1, max_its + 1):
+                q = M @ p
+                alpha = rz / (p @ q)
+                r = r - alpha * q
+                alphas.append(alpha)
+                if np.sqrt(r @ r) < 1e-2 or it == max_its:
+                    break
+                z = dinv * r
+                rz2 = r @ z
+                beta = rz2 / rz
+                rz = rz2
+                p = z + beta * p
+                betas.append(beta)
+        if alphas:
+            lmin, lmax = _tridiag_eigs(alphas, betas)
+            lmax *= 1.2
+        else:
+            lmin = lmax = 1.0
+        self.lmin, self.lmax, self.its = lmin, lmax, len(alphas)
+        a = lmax / rng_ if rng_ > 1.0 else min(0.9 * lmax, lmin)
+        self.delta = 0.5 * (lmax - a)
+        self.theta = 0.5 * (lmax + a)
+
+    def smooth(self, x, b, x_zero, steps):
+        """Saad Alg. 12.1, `steps` applications (the first from zero when x_zero)."""
+        M, dinv = self.M, self.dinv
+        for s in range(steps):
+            if x_zero and s == 0:
+                d = dinv * b / self.theta
+                x = d.copy()
+            else:
+                d = dinv * (b - M @ x) / self.theta
+                x = x + d
+            if self.degree < 2 or abs(self.delta) < 1e-40:
+                continue
+            sigma = self.theta / self.delta
+            rho_old = self.delta / self.theta
+            for _ in range(1, self.degree):
+                rho = 1.0 / (2.0 * sigma - rho_old)
+                d = rho * rho_old * d + 2.0 * rho / self.delta * dinv * (b - M @ x)
+                x = x + d
+                rho_old = rho
+        return x
+
+
+def constrained_matrix(K, Cm, fixed):
+    """C^T K C with identity rows and columns on the constrained DoFs."""
+    from scipy.sparse import diags
+    n = K.shape[0]
+    keep = np.ones(n)
+    keep[fixed] = 0.0
+    D = diags(keep)
+    return (D @ (Cm.T @ K @ Cm) @ D + diags(1.0 - keep)).tocsr()
+
+
+class GMG:
+    """Global-coarsening V-cycles for the A block (kind 0) and the Schur
+    complement approximation (kind 1) on general level meshes, coarse level
+    first; prolongation l maps level l-1 to level l (restriction: its
+    transpose).  Levels > 0: Chebyshev degree 4 over [lambda_max/15,
+    lambda_max] from 10 iterations; level 0: the smoother as coarse solver,
+    degree 8, range 1e-3, 100 iterations; A smoothed with 1 step, the
+    Schur complement with 4 (stokes_matrix_free_global_coarsening.cc:800-883)."""
+
+    def __init__(self, meshes, prolong_u, prolong_p, etas, pressure_scaling=1.0, compressible=False):
+        self.ops = [Operators(m, e, pressure_scaling, compressible) for m, e in zip(meshes, etas)]
+        self.P = [prolong_u, prolong_p]
+        self.mats = [[], []]
+        self.cheb = [[], []]
+        self.diag = [[], []]
+        for l, o in enumerate(self.ops):
+            for kind, (K, Cm, fx) in enumerate(((o.A, o.Cu, o.fixed_u), (o.M, o.Cp, o.fixed_p))):
+                M = constrained_matrix(K, Cm, fx)
+                dg = M.diagonal()
+                self.mats[kind].append(M)
+                self.diag[kind].append(dg)
+                coarse = l == 0
+                self.cheb[kind].append(Chebyshev(M, 1.0 / dg, 8 if coarse else 4, 100 if coarse else 10,
+                                                 1e-3 if coarse else 15.0))
+
+    def vcycle(self, kind, b, l=None):
+        l = len(self.ops) - 1 if l is None else l
+        steps = 1 if kind == 0 else 4
+        ch = self.cheb[kind][l]
+        if l == 0:
+            return ch.smooth(None, b, True, steps)
+        x = ch.smooth(None, b, True, steps)
+        r = b - self.mats[kind][l] @ x
+        P = self.P[kind][l]
+        x = x + P @ self.vcycle(kind, P.T @ r, l - 1)
+        return ch.smooth(x, b, False, steps)
+
+    def block_preconditioner(self, r):
+        """BlockSchurPreconditioner::vmult, cheap variant
+        (block_stokes_preconditioner.h:220-255): p = -S^{-1} r_p,
+        u = A^{-1} (r_u - B^T p), one V-cycle each."""
+        top = self.ops[-1]
+        p = -self.vcycle(1, r[top.nu:])
+        u = self.vcycle(0, r[:top.nu] - top.bt(p))
+        return np.concatenate([u, p])
+
+
+def gmres(apply, prec, b, tol, maxit, restart):
+    """Right-preconditioned restarted GMRES with modified Gram-Schmidt, zero
+    initial guess; stops when the residual estimate reaches tol.  Returns
+    (x, iterations, residual estimate)."""
+    x = np.zeros_like(b)
+    its = 0
+    r = b - apply(x)
+    beta = np.linalg.norm(r)
+    while its < maxit and beta > tol:
+        V = [r / beta]
+        Z = []
+        H = np.zeros((restart + 1, restart))
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        k = 0
+        while k < restart and its < maxit:
+            z = prec(V[k])
+            Z.append(z)
+            w = apply(z)
+            for i in range(k + 1):
+                H[i, k] = w @ V[i]
+                w = w - H[i, k] * V[i]
+            H[k + 1, k] = np.linalg.norm(w)
+            V.append(w / H[k + 1, k] if H[k + 1, k] > 0 else w)
+            for i in range(k):
+                t = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
+                H[i + 1, k] = -sn[i] * H[i, k] + cs[i] * H[i + 1, k]
+                H[i, k] = t
+            den = np.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = H[k, k] / den, H[k + 1, k] / den
+            H[k, k] = den
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            k += 1
+            its += 1
+            if abs(g[k]) <= tol:
+                break
+        y = np.zeros(k)
+        for i in range(k - 1, -1, -1):
+            y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
+        for i in range(k):
+            x = x + y[i] * Z[i]
+        r = b - apply(x)
+        beta = abs(g[k])
+    return x, its, beta

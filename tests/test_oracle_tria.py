@@ -278,3 +278,121 @@ def test_shell_no_normal_flux_and_hanging_lines(dim):
     dofs, offs, _, w = m.con_p
     sums = np.add.reduceat(w, offs[:-1])
     assert np.max(np.abs(sums - 1.0)) <= 1e-13
+
+
+# ------------------------------------------------------------------ multigrid hierarchy
+def box_hierarchy(dim, bc, seed=1):
+    t = T.Tria(T.BoxCoarse(dim, (3, 2, 2)[:dim], (1.5, 1.0, 1.2)[:dim]), 1)
+    t.refine(np.random.default_rng(seed).random(t.n_cells) < 0.35)
+    return T.hierarchy(t, bc)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_transfers_are_galerkin_on_boxes(dim):
+    """On affine cells with a constant viscosity the spaces are nested and the
+    quadrature exact, so P^T K_fine P = K_coarse on the free coarse DoFs for
+    the constrained A block and mass (a wrong embedding weight, a dropped
+    coarse constraint or a fine constrained row left in P fails)."""
+    _, meshes, pu, pp = box_hierarchy(dim, {0: "zero", 2: "tangential", 3: "tangential"})
+    for l in range(1, len(meshes)):
+        of = OT.Operators(meshes[l], np.full((meshes[l].n_cells, 3 ** dim), 1.7), pressure_scaling=1.2)
+        oc = OT.Operators(meshes[l - 1], np.full((meshes[l - 1].n_cells, 3 ** dim), 1.7), pressure_scaling=1.2)
+        for P, K_f, C_f, fx_f, K_c, C_c, fx_c in ((pu[l], of.A, of.Cu, of.fixed_u, oc.A, oc.Cu, oc.fixed_u),
+                                                   (pp[l], of.M, of.Cp, of.fixed_p, oc.M, oc.Cp, oc.fixed_p)):
+            Af = OT.constrained_matrix(K_f, C_f, fx_f)
+            Ac = OT.constrained_matrix(K_c, C_c, fx_c)
+            G = (P.T @ Af @ P).toarray()
+            free = np.ones(Ac.shape[0], bool)
+            free[fx_c] = False
+            ref = Ac.toarray()[np.ix_(free, free)]
+            assert np.max(np.abs(G[np.ix_(free, free)] - ref)) <= 1e-12 * np.max(np.abs(ref))
+            assert np.all(G[~free] == 0) and np.all(G[:, ~free] == 0)
+
+
+def shell_hierarchy(dim, g, bc):
+    t = T.Tria(T.ShellCoarse(dim), g)
+    c = t.cell_centers()
+    t.refine((c[:, 0] > 0.2) & (np.linalg.norm(c, axis=1) > 0.75))
+    return T.hierarchy(t, bc)
+
+
+def level_viscosities(meshes):
+    return [np.exp(2.0 * np.sin(3.0 * np.linalg.norm(m.x_q, axis=2))) for m in meshes]
+
+
+def test_shell_transfers_reproduce_constants():
+    """Curved cells: constants (velocity per component with open boundaries,
+    pressure) are prolongated exactly at the free fine DoFs."""
+    _, meshes, pu, pp = shell_hierarchy(3, 0, None)
+    mf, mc = meshes[1], meshes[0]
+    for c in range(3):
+        x = np.zeros(mc.n_dofs_u)
+        x[c * mc.n_nodes_u:(c + 1) * mc.n_nodes_u] = 1.0
+        y = pu[1] @ x
+        free = np.ones(mf.n_dofs_u, bool)
+        free[mf.con_u[0]] = False
+        ref = np.zeros(mf.n_dofs_u)
+        ref[c * mf.n_nodes_u:(c + 1) * mf.n_nodes_u] = 1.0
+        assert np.max(np.abs(y[free] - ref[free])) <= 1e-13
+    y = pp[1] @ np.ones(mc.n_dofs_p)
+    free = np.ones(mf.n_dofs_p, bool)
+    free[mf.con_p[0]] = False
+    assert np.max(np.abs(y[free] - 1.0)) <= 1e-13
+
+
+def test_oracle_vcycle_properties():
+    """The V-cycle is symmetric (same pre- and post-smoother, R = P^T) and a
+    contraction in the energy norm; the Chebyshev eigenvalue estimates bound
+    the largest eigenvalue of D^{-1} A from below (Lanczos) before the 1.2
+    safety factor."""
+    from scipy.sparse.linalg import eigsh
+    from scipy.sparse import diags
+    _, meshes, pu, pp = box_hierarchy(3, {0: "zero", 2: "tangential", 4: "zero"})
+    g = OT.GMG(meshes, pu, pp, level_viscosities(meshes), pressure_scaling=1.0)
+    rng = np.random.default_rng(3)
+    for kind in (0, 1):
+        M = g.mats[kind][-1]
+        n = M.shape[0]
+        x, y = rng.normal(size=n), rng.normal(size=n)
+        vx, vy = g.vcycle(kind, x), g.vcycle(kind, y)
+        assert abs(y @ vx - x @ vy) <= 1e-10 * np.sqrt(abs((x @ vx) * (y @ vy)))
+        e = rng.normal(size=n)
+        e2 = e - g.vcycle(kind, M @ e)
+        assert np.sqrt(e2 @ (M @ e2)) <= 0.6 * np.sqrt(e @ (M @ e))
+        for l in range(1, len(meshes)):
+            Ml = g.mats[kind][l]
+            d = 1.0 / np.sqrt(g.diag[kind][l])
+            lam = eigsh(diags(d) @ Ml @ diags(d), k=1, which="LA", return_eigenvectors=False)[0]
+            est = g.cheb[kind][l].lmax / 1.2
+            assert est <= lam * (1 + 1e-10) and est >= 0.85 * lam
+
+
+def stokes_matrix(o):
+    """The constrained block matrix [[A, B^T], [B, 0]] with identity rows on the
+    constrained DoFs (the operator the solve inverts)."""
+    from scipy.sparse import bmat, diags
+    ku = np.ones(o.nu)
+    ku[o.fixed_u] = 0.0
+    kp = np.ones(o.np)
+    kp[o.fixed_p] = 0.0
+    Auu = OT.constrained_matrix(o.A, o.Cu, o.fixed_u)
+    Bpu = diags(kp) @ (o.Cp.T @ o.B @ o.Cu) @ diags(ku)
+    return bmat([[Auu, Bpu.T], [Bpu, diags(1.0 - kp)]]).tocsr()
+
+
+def test_oracle_solve_matches_direct_solver():
+    from scipy.sparse.linalg import spsolve
+    _, meshes, pu, pp = box_hierarchy(3, {0: "zero", 2: "tangential", 4: "zero"})
+    g = OT.GMG(meshes, pu, pp, level_viscosities(meshes), pressure_scaling=1.0)
+    top = g.ops[-1]
+    K = stokes_matrix(top)
+    rng = np.random.default_rng(4)
+    x0 = rng.normal(size=K.shape[0])
+    assert np.max(np.abs(K @ x0 - top.stokes(x0))) <= 1e-12 * np.max(np.abs(K @ x0))
+    b = rng.normal(size=K.shape[0])
+    b[top.fixed_u] = 0.0
+    b[top.nu + top.fixed_p] = 0.0
+    x, its, res = OT.gmres(top.stokes, g.block_preconditioner, b, 1e-10 * np.linalg.norm(b), 300, 100)
+    xd = spsolve(K.tocsc(), b)
+    assert its < 60
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
