@@ -496,6 +496,26 @@ int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst) {
   });
 }
 
+int mfs_tria_time(mfs_tria *t, int op, const double *d_src, double *d_dst, int reps, double *avg_seconds) {
+  return tguard([&] {
+    need(t);
+    if (reps < 1 || !avg_seconds) throw std::invalid_argument("mfs_tria_time: reps >= 1 and an output required");
+    hipEvent_t e0, e1;
+    MFS_HIP(hipEventCreate(&e0));
+    MFS_HIP(hipEventCreate(&e1));
+    MFS_HIP(hipEventRecord(e0, t->st));
+    for (int r = 0; r < reps; ++r) t->vmult(op, d_src, d_dst);
+    MFS_HIP(hipEventRecord(e1, t->st));
+    MFS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_seconds = ms * 1e-3 / reps;
+    return MFS_OK;
+  });
+}
+
 int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_dst) {
   return tguard([&] {
     need(t);
@@ -843,8 +863,67 @@ struct mfs_tria_gmg {
     vcycle(0, rtmp.p, dst);
   }
 
-  // right-preconditioned restarted GMRES (modified Gram-Schmidt), zero start
-  void gmres(const double *b, double *x, double tol, int maxit, int restart, int &its, double &res) {
+  // preconditioned CG on the finest level's A block (kind 0) or mass (kind 1)
+  // with one V-cycle as preconditioner, from zero, to |r| <= tol; returns
+  // the iterations (the inner solves of the expensive preconditioner)
+  int cg(int kind, const double *b, double *x, double tol, int maxit) {
+    TLevel &L = lv.back();
+    const long long n = L.n[kind];
+    const int l = (int)lv.size() - 1;
+    for (Buf<double> *v : {&cg_r[kind], &cg_z[kind], &cg_p[kind], &cg_q[kind]})
+      if (!v->p) v->alloc(n);
+    double *r = cg_r[kind].p, *z = cg_z[kind].p, *p = cg_p[kind].p, *q = cg_q[kind].p;
+    v_fill<double>(x, 0.0, n, st);
+    MFS_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double res = std::sqrt(dot(r, r, n));
+    if (res <= tol) return 0;
+    vcycle(kind, r, z);
+    double rz = dot(r, z, n);
+    MFS_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int it = 1; it <= maxit; ++it) {
+      op(kind, l, p, q);
+      const double alpha = rz / dot(p, q, n);
+      v_axpby<double>(alpha, p, 1.0, x, n, st);
+      v_axpby<double>(-alpha, q, 1.0, r, n, st);
+      res = std::sqrt(dot(r, r, n));
+      if (res <= tol) return it;
+      vcycle(kind, r, z);
+      const double rz2 = dot(r, z, n);
+      const double beta = rz2 / rz;
+      rz = rz2;
+      v_axpby<double>(1.0, z, beta, p, n, st);
+    }
+    return maxit;
+  }
+  Buf<double> cg_r[2], cg_z[2], cg_p[2], cg_q[2], cg_t;
+  double a_tol = 1e-2, s_tol = 1e-6;
+  long long a_apps = 0, s_apps = 0;
+
+  // BlockSchurPreconditioner::vmult, expensive variant: S^{-1} by CG to
+  // s_tol |r_p| (at most 100 iterations; skipped when |r_p| <= 1e-50), A^{-1}
+  // by CG to a_tol |r_u - B^T p| (block_stokes_preconditioner.h:220-255, 308-352)
+  void block_prec_expensive(const double *src, double *dst) {
+    TLevel &L = lv.back();
+    const long long nu = L.n[0], np = L.n[1];
+    if (!cg_t.p) cg_t.alloc(std::max(nu, np));
+    const double nsp = std::sqrt(dot(src + nu, src + nu, np));
+    if (nsp > 1e-50) {
+      s_apps += cg(1, src + nu, cg_t.p, nsp * s_tol, 100);
+      v_scale_neg_copy<double>(cg_t.p, dst + nu, np, st);
+    } else {
+      v_fill<double>(dst + nu, 0.0, np, st);
+    }
+    L.t->vmult(MFS_OP_BT, dst + nu, rtmp.p, st);
+    v_axpby<double>(1.0, src, -1.0, rtmp.p, nu, st);
+    const double nt = std::sqrt(dot(rtmp.p, rtmp.p, nu));
+    a_apps += cg(0, rtmp.p, dst, nt * a_tol, 10000);
+  }
+
+  // right-preconditioned restarted GMRES (modified Gram-Schmidt) from the
+  // initial guess in x, keeping the preconditioned vectors (flexible: the
+  // expensive preconditioner is not a fixed linear map)
+  template <typename Prec>
+  void gmres(const double *b, double *x, double tol, int maxit, int restart, Prec prec, int &its, double &res) {
     TLevel &L = lv.back();
     const long long n = L.n[0] + L.n[1];
     std::vector<Buf<double>> V(restart + 1), Z(restart);
@@ -852,10 +931,9 @@ struct mfs_tria_gmg {
     for (auto &z : Z) z.alloc(n);
     Buf<double> w;
     w.alloc(n);
-    v_fill<double>(x, 0.0, n, st);
     its = 0;
-    // r = b - K x with x = 0
-    MFS_HIP(hipMemcpyAsync(V[0].p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    L.t->vmult(MFS_OP_STOKES, x, V[0].p, st);
+    v_axpby<double>(1.0, b, -1.0, V[0].p, n, st);  // r = b - K x
     double beta = std::sqrt(dot(V[0].p, V[0].p, n));
     while (its < maxit && beta > tol) {
       std::vector<double> H((restart + 1) * restart, 0.0), cs(restart), sn(restart), g(restart + 1, 0.0);
@@ -864,7 +942,7 @@ struct mfs_tria_gmg {
       g[0] = beta;
       int k = 0;
       while (k < restart && its < maxit) {
-        block_prec(V[k].p, Z[k].p);
+        prec(V[k].p, Z[k].p);
         L.t->vmult(MFS_OP_STOKES, Z[k].p, w.p, st);
         for (int i = 0; i <= k; ++i) {
           h(i, k) = dot(w.p, V[i].p, n);
@@ -996,7 +1074,7 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_so
     if (!rhs || !sol || !prm || !res) throw std::invalid_argument("mfs_tria_solve: null argument");
     if (!g->built) throw std::runtime_error("multigrid not built (call mfs_tria_gmg_build)");
     if (prm->restart < 1 || prm->restart > 1000) throw std::invalid_argument("mfs_tria_solve: restart must be in [1, 1000]");
-    if (prm->n_cheap_steps < 1) throw std::invalid_argument("mfs_tria_solve: needs cheap steps");
+    if (prm->n_cheap_steps < 0 || prm->n_expensive_steps < 0) throw std::invalid_argument("mfs_tria_solve: negative step limit");
     std::memset(res, 0, sizeof(*res));
     const long long n = g->lv.back().n[0] + g->lv.back().n[1];
     Buf<double> b, x;
@@ -1009,28 +1087,44 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_so
     MFS_HIP(hipEventCreate(&e0));
     MFS_HIP(hipEventCreate(&e1));
     MFS_HIP(hipEventRecord(e0, g->st));
-    int its = 0;
-    double fin = 0;
-    g->gmres(b.p, x.p, tol, prm->n_cheap_steps, prm->restart, its, fin);
+    int its = 0, its_exp = 0;
+    double fin = r0;
+    v_fill<double>(x.p, 0.0, n, g->st);
+    hipEvent_t em;
+    MFS_HIP(hipEventCreate(&em));
+    if (prm->n_cheap_steps > 0)
+      g->gmres(b.p, x.p, tol, prm->n_cheap_steps, prm->restart,
+               [&](const double *a, double *z) { g->block_prec(a, z); }, its, fin);
+    MFS_HIP(hipEventRecord(em, g->st));
+    const double cheap_res = fin;
+    g->a_apps = g->s_apps = 0;
+    g->a_tol = prm->a_block_tolerance;
+    g->s_tol = prm->s_block_tolerance;
+    if (fin > tol && prm->n_expensive_steps > 0)
+      g->gmres(b.p, x.p, tol, prm->n_expensive_steps, prm->restart,
+               [&](const double *a, double *z) { g->block_prec_expensive(a, z); }, its_exp, fin);
     MFS_HIP(hipEventRecord(e1, g->st));
     MFS_HIP(hipEventSynchronize(e1));
-    float ms = 0;
+    float ms = 0, ms_cheap = 0;
     MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    MFS_HIP(hipEventElapsedTime(&ms_cheap, e0, em));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(em);
     MFS_HIP(hipMemcpyAsync(sol, x.p, n * sizeof(double), hipMemcpyDeviceToHost, g->st));
     MFS_HIP(hipStreamSynchronize(g->st));
     res->cheap_iterations = its;
-    res->a_preconditioner_applications = its;
-    res->s_preconditioner_applications = its;
+    res->expensive_iterations = its_exp;
+    res->a_preconditioner_applications = its + (int)g->a_apps;
+    res->s_preconditioner_applications = its + (int)g->s_apps;
     res->converged = fin <= tol ? 1 : 0;
     res->initial_nonlinear_residual = r0;
     res->r0_norm = r0;
     res->final_linear_residual = fin;
-    res->cheap_final_residual = fin;
+    res->cheap_final_residual = cheap_res;
     res->solver_tolerance = tol;
     res->solve_seconds = ms * 1e-3;
-    res->cheap_seconds = ms * 1e-3;
+    res->cheap_seconds = ms_cheap * 1e-3;
     return res->converged ? MFS_OK : MFS_NO_CONVERGENCE;
   });
 }

@@ -40,7 +40,7 @@ EXPORTED = [
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
     "mfs_box_dof_index", "mfs_tria_create", "mfs_tria_destroy", "mfs_tria_sizes", "mfs_tria_set_viscosity",
-    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_build",
+    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_time", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_build",
     "mfs_tria_gmg_info", "mfs_tria_gmg_vcycle", "mfs_tria_solve",
 ]
 
@@ -653,6 +653,13 @@ class TriaOperators:
     def vmult_device(self, op, d_src, d_dst):
         _check(self.lib.mfs_tria_vmult_device(self._t, self.OPS[op], C.c_void_p(d_src), C.c_void_p(d_dst)))
 
+    def time(self, op, d_src, d_dst, reps):
+        """Average device seconds of reps applies (hipEvents on the object's stream)."""
+        out = C.c_double(0)
+        _check(self.lib.mfs_tria_time(self._t, self.OPS[op], C.c_void_p(d_src), C.c_void_p(d_dst), int(reps),
+                                      C.byref(out)))
+        return out.value
+
     def __del__(self):
         try:
             if self._t:
@@ -715,11 +722,13 @@ class TriaMultigrid:
         _check(self.lib.mfs_tria_gmg_vcycle(self._g, kind, _ptr(x), _ptr(y)))
         return y
 
-    def solve(self, rhs, tolerance=1e-8, n_cheap_steps=1000, restart=100):
+    def solve(self, rhs, tolerance=1e-8, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
+              a_block_tolerance=1e-2, s_block_tolerance=1e-6):
         t = self.levels[-1]
         b = _vec(rhs, t.nu + t.np, "right-hand side")
         x = np.empty_like(b)
-        prm = _params(tolerance, n_cheap_steps, 0, restart, 1e-2, 1e-6, "GMRES", 2)
+        prm = _params(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance,
+                      "GMRES", 2)
         res = SolveResult()
         rc = _check(self.lib.mfs_tria_solve(self._g, _ptr(b), _ptr(x), C.byref(prm), C.byref(res)))
         return x, SolverOutputs(res.initial_nonlinear_residual, res.final_linear_residual,

@@ -316,3 +316,46 @@ def mapped_qp_coordinates(dim, degree, n_cells, mapping):
     for d in reversed(range(dim)):
         M = np.kron(M, S)
     return np.einsum("qa,cai->cqi", M, mapped_nodes(dim, degree, n_cells, mapping))
+
+
+def shell_temperature(x, seed=0, r0=0.55, r1=1.0, amplitude=0.01):
+    """Conductive profile of a spherical shell, (1/r - 1/R1) / (1/R0 - 1/R1),
+    plus a seeded perturbation of spherical-harmonic degree <= 8 (a sum of
+    powers up to 8 of x/|x| . d_i for random directions d_i) under a radial
+    envelope vanishing at both surfaces."""
+    rng = np.random.default_rng(seed)
+    r = np.linalg.norm(x, axis=-1)
+    n = x / r[..., None]
+    dim = x.shape[-1]
+    pert = np.zeros_like(r)
+    for _ in range(6):
+        d = rng.normal(size=dim)
+        d /= np.linalg.norm(d)
+        pert += rng.normal() * (n @ d) ** rng.integers(1, 9)
+    env = np.sin(np.pi * (r - r0) / (r1 - r0))
+    return (1.0 / r - 1.0 / r1) / (1.0 / r0 - 1.0 / r1) + amplitude * env * pert
+
+
+def shell_viscosity(x, seed=0):
+    """eta = exp(-ln(1e4) T), times 30 below r = 0.9."""
+    T = shell_temperature(x, seed)
+    r = np.linalg.norm(x, axis=-1)
+    return np.exp(-np.log(1e4) * T) * np.where(r < 0.9, 30.0, 1.0)
+
+
+def config_shell(dim=3, global_refinement=2, seed=0, fraction=0.2):
+    """Spherical shell R = [0.55, 1] (96 coarse cells in 3D), global refinement,
+    then one adaptive level on the cells with the largest |dT/dr| (the given
+    fraction, by cell count).  Returns the refined Tria."""
+    from .tria import ShellCoarse, Tria
+    t = Tria(ShellCoarse(dim), global_refinement)
+    c = t.cell_centers()
+    r = np.linalg.norm(c, axis=1)
+    h = 1e-4
+    dTdr = (shell_temperature(c * ((r + h) / r)[:, None], seed) -
+            shell_temperature(c * ((r - h) / r)[:, None], seed)) / (2 * h)
+    k = int(round(fraction * t.n_cells))
+    flags = np.zeros(t.n_cells, dtype=bool)
+    flags[np.argsort(-np.abs(dTdr), kind="stable")[:k]] = True
+    t.refine(flags)
+    return t

@@ -111,6 +111,10 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tria", action="store_true",
+                    help="skip the general-mesh (spherical shell with hanging nodes) apply and solve")
+    ap.add_argument("--shell-refinement", type=int, default=3,
+                    help="global refinement of the 96-cell shell for the general-mesh apply")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble of 48 independent 2D problems")
     ap.add_argument("--cheap-steps", type=int, default=1000,
@@ -299,6 +303,76 @@ def main():
         log(f"ensemble of {len(cfgs)} 2D problems: {secs:.2f}s, {its} iterations, "
             f"converged={result['ensemble']['converged']}")
         del ens
+
+    if not args.no_tria:
+        # General level meshes: the spherical shell (96 coarse cells, global
+        # refinement, one adaptive level on the 20% of cells with the largest
+        # |dT/dr|), free slip on both surfaces, per-point geometry and
+        # viscosity, hanging-node and no-normal-flux constraints.  FP64 Stokes
+        # apply; then the global-coarsening multigrid and the cheap GMRES solve
+        # on the next smaller shell (zero velocity inside, free slip outside).
+        from aspect_amd import TriaMultigrid, TriaOperators
+        from aspect_amd import tria as TR
+        t0 = time.perf_counter()
+        tr = P.config_shell(3, args.shell_refinement)
+        mesh = tr.build({0: "tangential", 1: "tangential"})
+        t_mesh = time.perf_counter() - t0
+        op = TriaOperators(mesh, pressure_scaling=1.0)
+        op.set_viscosity(P.shell_viscosity(mesh.x_q))
+        nt = mesh.n_dofs_u + mesh.n_dofs_p
+        gen = torch.Generator(device="cuda").manual_seed(5)
+        xs = torch.randn(nt, dtype=torch.float64, device="cuda", generator=gen)
+        ys = torch.empty_like(xs)
+        torch.cuda.synchronize()
+        op.time("stokes", xs.data_ptr(), ys.data_ptr(), 3)
+        t_app = op.time("stokes", xs.data_ptr(), ys.data_ptr(), 20)
+        nq = 27
+        alg = 8 * 2 * nt + 8 * mesh.n_cells * nq + 8 * mesh.n_cells * nq * 10
+        rec = {"workload": f"shell 96 x 8^{args.shell_refinement} cells + 1 adaptive level (top 20% |dT/dr|), "
+                           "free slip, Q2xQ1, per-point J^-1/JxW and viscosity",
+               "cells": mesh.n_cells, "dofs": nt, "hanging_velocity_nodes": int(len(mesh.hanging_u)),
+               "constraint_lines": int(len(mesh.con_u[0]) + len(mesh.con_p[0])),
+               "stokes_apply_ms": t_app * 1e3, "gdofs_per_s": nt / t_app / 1e9,
+               "algorithmic_bytes": alg, "achieved_GBs": alg / t_app / 1e9,
+               "frac_hbm": alg / t_app / 1e9 / HBM_PEAK_GBS, "mesh_setup_seconds": t_mesh}
+        log(f"general mesh: {mesh.n_cells} cells, {nt} DoFs, Stokes apply {t_app * 1e3:.3f} ms "
+            f"({nt / t_app / 1e9:.2f} GDoF/s, {alg / t_app / 1e9:.0f} GB/s algorithmic)")
+        del op, xs, ys
+        if not args.no_solve:
+            t0 = time.perf_counter()
+            tr2 = P.config_shell(3, max(args.shell_refinement - 1, 0))
+            _, meshes, pu, pp = TR.hierarchy(tr2, {0: "zero", 1: "tangential"})
+            levels = []
+            for m in meshes:
+                lo = TriaOperators(m, pressure_scaling=1.0)
+                lo.set_viscosity(P.shell_viscosity(m.x_q))
+                levels.append(lo)
+            t_h = time.perf_counter() - t0
+            mg = TriaMultigrid(levels, pu, pp)
+            t0 = time.perf_counter()
+            mg.build()
+            t_b = time.perf_counter() - t0
+            top = meshes[-1]
+            rng = np.random.default_rng(6)
+            b = rng.normal(size=top.n_dofs_u + top.n_dofs_p)
+            b[top.con_u[0]] = 0.0
+            b[top.n_dofs_u + top.con_p[0]] = 0.0
+            fp = np.ones(top.n_dofs_p, bool)
+            fp[top.con_p[0]] = False
+            bp = b[top.n_dofs_u:]
+            bp[fp] -= bp[fp].mean()   # enclosed flow: right-hand side orthogonal to the constant pressure
+            xsol, out, rc = mg.solve(b, tolerance=1e-8, n_cheap_steps=1000, restart=100)
+            resid = np.linalg.norm(b - levels[-1].vmult("stokes", xsol)) / np.linalg.norm(b)
+            rec["solve"] = {"levels": [m.n_cells for m in meshes], "dofs": int(b.size), "seconds": out.solve_seconds,
+                            "cheap_iterations": out.cheap_iterations,
+                            "expensive_iterations": out.expensive_iterations,
+                            "cheap_phase_seconds": out.cheap_seconds, "converged": bool(out.converged),
+                            "rel_residual": resid, "gmg_build_seconds": t_b, "host_setup_seconds": t_h,
+                            "boundary": "zero velocity inside, free slip outside"}
+            log(f"general mesh solve: levels {[m.n_cells for m in meshes]}, {out.cheap_iterations}+"
+                f"{out.expensive_iterations} iterations, {out.solve_seconds:.2f} s, residual {resid:.2e}")
+            del mg, levels
+        result["general_mesh"] = rec
 
     if not args.no_cpu:
         # CPU baseline (oracle/cpu_port.cpp, a CPU restatement, not the

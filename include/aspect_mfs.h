@@ -388,6 +388,9 @@ int mfs_tria_set_viscosity(mfs_tria *t, const double *eta_qp);
 int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst);
 /* The same on device vectors (hipMalloc'd by the caller). */
 int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_dst);
+/* Average device time of reps applies on device vectors, bracketed by hipEvents
+ * on the object's stream ("Execute solver timings", local_smoothing.cc:1119-1258). */
+int mfs_tria_time(mfs_tria *t, int op, const double *d_src, double *d_dst, int reps, double *avg_seconds);
 
 /* Global-coarsening multigrid over general levels (coarse first) and the
  * cheap Stokes solve on the finest one (stokes_matrix_free_global_coarsening.cc:
@@ -420,11 +423,15 @@ int mfs_tria_gmg_info(mfs_tria_gmg *g, int level, int kind, double out[4], doubl
 /* One V-cycle (kind 0: A block, 1 smoothing step; 1: Schur complement, 4 steps)
  * on host vectors of the finest level. */
 int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *dst);
-/* Cheap-phase solve on the finest level: right-preconditioned GMRES
- * (restart prm->restart, at most prm->n_cheap_steps iterations, zero initial
- * guess, tolerance prm->tolerance * |rhs|) with the cheap block preconditioner
- * (one A and one Schur V-cycle).  Host vectors [u|p]; the other fields of prm
- * are not used.  MFS_NO_CONVERGENCE when the tolerance is not reached. */
+/* Solve on the finest level from a zero initial guess to prm->tolerance * |rhs|
+ * (local_smoothing.cc:1259-1397 sequence): right-preconditioned GMRES
+ * (restart prm->restart) with the cheap block preconditioner (one A and one
+ * Schur V-cycle) for at most prm->n_cheap_steps iterations, then, if not
+ * converged, flexible GMRES with the expensive one (CG inner solves to
+ * prm->a_block_tolerance / prm->s_block_tolerance, V-cycle preconditioned)
+ * for at most prm->n_expensive_steps.  Host vectors [u|p]; the Newton, IDR(s)
+ * and failure-strategy fields of prm are not used.  MFS_NO_CONVERGENCE when the
+ * tolerance is not reached. */
 int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_solve_params *prm,
                    mfs_solve_result *res);
 

@@ -295,12 +295,61 @@ class GMG:
         u = self.vcycle(0, r[:top.nu] - top.bt(p))
         return np.concatenate([u, p])
 
+    def cg(self, kind, b, tol, maxit):
+        """Preconditioned CG (one V-cycle) on the finest A block / mass from
+        zero to |r| <= tol; returns (x, iterations)."""
+        M = self.mats[kind][-1]
+        x = np.zeros_like(b)
+        r = b.copy()
+        if np.sqrt(r @ r) <= tol:
+            return x, 0
+        z = self.vcycle(kind, r)
+        rz = r @ z
+        p = z.copy()
+        for it in range(1, maxit + 1):
+            q = M @ p
+            alpha = rz / (p @ q)
+            x = x + alpha * p
+            r = r - alpha * q
+            if np.sqrt(r @ r) <= tol:
+                return x, it
+            z = self.vcycle(kind, r)
+            rz2 = r @ z
+            p = z + (rz2 / rz) * p
+            rz = rz2
+        return x, maxit
 
-def gmres(apply, prec, b, tol, maxit, restart):
-    """Right-preconditioned restarted GMRES with modified Gram-Schmidt, zero
-    initial guess; stops when the residual estimate reaches tol.  Returns
-    (x, iterations, residual estimate)."""
-    x = np.zeros_like(b)
+    def block_preconditioner_expensive(self, r, a_tol=1e-2, s_tol=1e-6):
+        """Expensive variant: S^{-1} by CG to s_tol |r_p| (at most 100
+        iterations), A^{-1} by CG to a_tol |r_u - B^T p| (at most 10000)."""
+        top = self.ops[-1]
+        rp = r[top.nu:]
+        nsp = np.linalg.norm(rp)
+        p = -self.cg(1, rp, nsp * s_tol, 100)[0] if nsp > 1e-50 else np.zeros_like(rp)
+        t = r[:top.nu] - top.bt(p)
+        u = self.cg(0, t, np.linalg.norm(t) * a_tol, 10000)[0]
+        return np.concatenate([u, p])
+
+    def solve(self, b, tol, n_cheap, n_expensive, restart=100, a_tol=1e-2, s_tol=1e-6):
+        """Cheap GMRES steps, then flexible GMRES with the expensive
+        preconditioner (stokes_matrix_free_local_smoothing.cc:1259-1397).
+        Returns (x, cheap iterations, expensive iterations, residual)."""
+        top = self.ops[-1]
+        x, its, res = gmres(top.stokes, self.block_preconditioner, b, tol, n_cheap, restart) if n_cheap > 0 \
+            else (np.zeros_like(b), 0, np.linalg.norm(b))
+        its_e = 0
+        if res > tol and n_expensive > 0:
+            x, its_e, res = gmres(top.stokes, lambda v: self.block_preconditioner_expensive(v, a_tol, s_tol), b, tol,
+                                  n_expensive, restart, x0=x)
+        return x, its, its_e, res
+
+
+def gmres(apply, prec, b, tol, maxit, restart, x0=None):
+    """Right-preconditioned restarted GMRES with modified Gram-Schmidt from x0
+    (zero by default), the update built from the preconditioned vectors
+    (flexible); stops when the residual estimate reaches tol.  Returns (x,
+    iterations, residual estimate)."""
+    x = np.zeros_like(b) if x0 is None else x0.copy()
     its = 0
     r = b - apply(x)
     beta = np.linalg.norm(r)

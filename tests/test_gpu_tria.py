@@ -159,3 +159,37 @@ def test_tria_multigrid_and_solve_match_oracle(name):
     assert rc == 0 and out.converged
     assert abs(out.cheap_iterations - its_ref) <= max(2, 0.1 * its_ref)
     assert np.linalg.norm(b - top.stokes(x)) <= 1.5e-8 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", ["box3d", "shell3d"])
+def test_tria_solve_expensive_phase_matches_oracle(name):
+    """Few cheap steps, then flexible GMRES with CG inner solves (the sequence
+    of stokes_matrix_free_local_smoothing.cc:1259-1397)."""
+    meshes, pu, pp, etas, consistent = gmg_case(name)
+    ref = OT.GMG(meshes, pu, pp, etas, pressure_scaling=1.0)
+    levels = []
+    for m, e in zip(meshes, etas):
+        t = TriaOperators(m, pressure_scaling=1.0)
+        t.set_viscosity(e)
+        levels.append(t)
+    mg = TriaMultigrid(levels, pu, pp)
+    mg.build()
+    top = ref.ops[-1]
+    rng = np.random.default_rng(13)
+    b = rng.normal(size=top.nu + top.np)
+    b[top.fixed_u] = 0.0
+    b[top.nu + top.fixed_p] = 0.0
+    if consistent:
+        fp = np.ones(top.np, bool)
+        fp[top.fixed_p] = False
+        bp = b[top.nu:]
+        bp[fp] -= bp[fp].mean()
+    tol = 1e-8 * np.linalg.norm(b)
+    _, its_c, its_e, _ = ref.solve(b, tol, 5, 200)
+    x, out, rc = mg.solve(b, tolerance=1e-8, n_cheap_steps=5, n_expensive_steps=200, restart=100)
+    print(f"\n[tria gmg] {name}: cheap {out.cheap_iterations} + expensive {out.expensive_iterations} "
+          f"(oracle {its_c} + {its_e})", flush=True)
+    assert rc == 0 and out.converged
+    assert out.cheap_iterations == its_c == 5
+    assert abs(out.expensive_iterations - its_e) <= max(1, 0.1 * its_e)
+    assert np.linalg.norm(b - top.stokes(x)) <= 1.5e-8 * np.linalg.norm(b)
