@@ -78,7 +78,7 @@ __global__ void k_resolve(const long long n, const double *__restrict__ src, con
 
 // sum of the cell contributions of one node (component offset coff)
 __device__ __forceinline__ double node_sum(const double *__restrict__ cb, const long long *__restrict__ iptr,
-                                           const long long *__restrict__ ioff, long long node, long long coff) {
+                                           const int *__restrict__ ioff, long long node, long long coff) {
   double a = 0;
   for (long long k = iptr[node]; k < iptr[node + 1]; ++k) a += cb[ioff[k] + coff];
   return a;
@@ -89,7 +89,7 @@ __device__ __forceinline__ double node_sum(const double *__restrict__ cb, const 
 // sums of the constrained DoFs they are a master of; constrained DoFs
 // src[i] (ident) or 0
 __global__ void k_assemble(const long long n, const long long nn, const long long cs, const double *__restrict__ cb,
-                           const long long *__restrict__ iptr, const long long *__restrict__ ioff,
+                           const long long *__restrict__ iptr, const int *__restrict__ ioff,
                            const int *__restrict__ row, const long long *__restrict__ tptr,
                            const long long *__restrict__ tdof, const double *__restrict__ tw,
                            const double *__restrict__ src, double *__restrict__ dst, const int ident) {
@@ -208,7 +208,8 @@ struct mfs_tria {
   LevelDesc L{};
   Buf<int> cell_u, cell_p;
   Buf<double> geo, eta, cbu, cbp, xu, xp, io_src, io_dst;
-  Buf<long long> iptr_u, ioff_u, iptr_p, ioff_p;
+  Buf<long long> iptr_u, iptr_p;
+  Buf<int> ioff_u, ioff_p;  // offsets into the per-cell buffers (checked < 2^31 at creation)
   DevCon con_u, con_p;
   std::vector<int> h_cell_u, h_cell_p;  // host copies of the cell lists (diagonal colouring)
   bool have_eta = false;
@@ -219,7 +220,7 @@ struct mfs_tria {
 
   // node -> (offset of its contribution in the per-cell buffer, component 0)
   static void incidence(const int *cells, long long ncells, int per, long long nn, int comps, std::vector<long long> &ptr,
-                        std::vector<long long> &off) {
+                        std::vector<int> &off) {
     ptr.assign(nn + 1, 0);
     for (long long c = 0; c < ncells * per; ++c) ++ptr[cells[c] + 1];
     for (long long i = 0; i < nn; ++i) ptr[i + 1] += ptr[i];
@@ -227,7 +228,7 @@ struct mfs_tria {
     std::vector<long long> fill(ptr.begin(), ptr.end() - 1);
     for (long long c = 0; c < ncells; ++c) {
       const long long b = c / CT, cc = c % CT;
-      for (int a = 0; a < per; ++a) off[fill[cells[c * per + a]]++] = ((b * comps) * per + a) * CT + cc;
+      for (int a = 0; a < per; ++a) off[fill[cells[c * per + a]]++] = (int)(((b * comps) * per + a) * CT + cc);
     }
   }
 
@@ -280,7 +281,9 @@ struct mfs_tria {
     cell_u.upload(bu, st);
     cell_p.upload(bp, st);
     geo.upload(bg, st);
-    std::vector<long long> ptr, off;
+    if (nbatch * dim * nv * CT >= 0x7fffffffLL) throw std::invalid_argument("mfs_tria_create: too many cells");
+    std::vector<long long> ptr;
+    std::vector<int> off;
     incidence(d->cell_nodes_u, ncells, nv, nn_u, dim, ptr, off);
     iptr_u.upload(ptr, st);
     ioff_u.upload(off, st);
@@ -331,7 +334,7 @@ struct mfs_tria {
     hipLaunchKernelGGL(k_resolve, dim3(nb), dim3(256), 0, s, c.n, src, c.row.p, c.off.p, c.mas.p, c.w.p, out);
     MFS_HIP(hipGetLastError());
   }
-  void assemble(const double *cb, long long nn, long long cs, const Buf<long long> &ip, const Buf<long long> &io,
+  void assemble(const double *cb, long long nn, long long cs, const Buf<long long> &ip, const Buf<int> &io,
                 DevCon &c, const double *src, double *dst, int ident, hipStream_t s) {
     const unsigned nb = (unsigned)((c.n + 255) / 256);
     hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(256), 0, s, c.n, nn, cs, cb, ip.p, io.p, c.row.p, c.tptr.p,

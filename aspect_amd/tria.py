@@ -61,13 +61,18 @@ def _lattice(dim, pts):
 class BoxCoarse:
     """Box [0, extent] with reps cells per direction; optional smooth map
     ``deform(x) -> (y, dy/dx)`` applied to the box points.  Boundary ids
-    0..2*dim-1 are the faces x-, x+, y-, y+, z-, z+."""
+    0..2*dim-1 are the faces x-, x+, y-, y+, z-, z+.  ``periodic``: axes whose
+    two faces are identified (the + face's DoFs follow the - face's, as
+    make_periodicity_constraints gives them; undeformed boxes only)."""
 
-    def __init__(self, dim, reps, extent, deform=None):
+    def __init__(self, dim, reps, extent, deform=None, periodic=()):
         self.dim = dim
         self.reps = np.array(reps[:dim], dtype=np.int64)
         self.extent = np.array(extent[:dim], dtype=float)
         self.deform = deform
+        self.periodic = tuple(int(a) for a in periodic)
+        if self.periodic and deform is not None:
+            raise ValueError("periodic boxes cannot be deformed")
         self.n_roots = int(np.prod(self.reps))
         r = np.arange(self.n_roots)
         idx = np.zeros((self.n_roots, dim), dtype=np.int64)
@@ -78,6 +83,8 @@ class BoxCoarse:
         self.idx = idx
         self.root_boundary = -np.ones((self.n_roots, 2 * dim), dtype=np.int64)
         for d in range(dim):
+            if d in self.periodic:
+                continue
             self.root_boundary[idx[:, d] == 0, 2 * d] = 2 * d
             self.root_boundary[idx[:, d] == self.reps[d] - 1, 2 * d + 1] = 2 * d + 1
 
@@ -301,6 +308,22 @@ class Tria:
         self.adapted = True
 
     # ---------------------------------------------------------------- build
+    def _periodic_shifts(self, P, tol):
+        """Translations that carry points on a periodic face to the opposite
+        face (one per combination of periodic axes the points can cross)."""
+        axes = getattr(self.coarse, "periodic", ())
+        out = []
+        for mask in range(1, 1 << len(axes)):
+            for signs in range(1 << bin(mask).count("1")):
+                sh = np.zeros(self.dim)
+                k = 0
+                for i, ax in enumerate(axes):
+                    if mask >> i & 1:
+                        sh[ax] = self.coarse.extent[ax] * (1 if signs >> k & 1 else -1)
+                        k += 1
+                out.append(sh)
+        return out
+
     def _map(self, cells, xi_local):
         """Physical points of local reference points (per cell, (P, dim)) of cells."""
         m = len(xi_local)
@@ -358,8 +381,12 @@ class Tria:
                 step = nodes_q[1] - nodes_q[0]
                 off = np.any(np.abs(fp / step - np.round(fp / step)) > 1e-12, axis=1)
                 fp = fp[off]
-                P = self._map(coarse_cells, fp)
-                dist, hit = tree.query(P.reshape(-1, dim), distance_upper_bound=tol)
+                P = self._map(coarse_cells, fp).reshape(-1, dim)
+                dist, hit = tree.query(P, distance_upper_bound=tol)
+                for shift in self._periodic_shifts(P, tol):
+                    d2, h2 = tree.query(P + shift, distance_upper_bound=tol)
+                    better = np.isfinite(d2) & ~np.isfinite(dist)
+                    dist[better], hit[better] = d2[better], h2[better]
                 hit = hit.reshape(len(coarse_cells), len(fp))
                 found = np.isfinite(dist.reshape(len(coarse_cells), len(fp)))
                 W = _basis(dim, nodes_q, fp)                # (len(fp), n_local)
@@ -430,8 +457,26 @@ class Tria:
                         raise NotImplementedError("non-axis-aligned normals meeting at a node")
                     vel[k * nn + n] = (np.zeros(0, np.int64), np.zeros(0))
 
+        # periodic faces: nodes on the + face follow the matching - face node
+        # (hanging-node and boundary lines take precedence)
+        pl_p = dict(lines_p)
+        for ax in getattr(self.coarse, "periodic", ()):
+            L = self.coarse.extent[ax]
+            for xs, nnodes, cons, comps in ((x_u, nn, vel, dim), (x_p, npn, pl_p, 1)):
+                plus = np.nonzero(np.abs(xs[:, ax] - L) <= tol)[0]
+                target = xs[plus].copy()
+                target[:, ax] -= L
+                dd, partner = cKDTree(xs).query(target, distance_upper_bound=tol)
+                for n, ok, m in zip(plus, np.isfinite(dd), partner):
+                    if not ok:
+                        continue   # a hanging position of the + face (no node opposite)
+                    for comp in range(comps):
+                        key = comp * nnodes + int(n)
+                        if key in cons:
+                            continue
+                        cons[key] = (np.array([comp * nnodes + int(m)], np.int64), np.array([1.0]))
         con_u = _close(vel)
-        con_p = _close({n: v for n, v in lines_p.items()})
+        con_p = _close(pl_p)
 
         # ---- geometry at the QGauss(3) points
         xg, wg = gauss3()

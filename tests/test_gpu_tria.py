@@ -21,6 +21,13 @@ def refined_box(dim, seed, bc):
     return t.build(bc)
 
 
+def periodic_box(dim, bc):
+    t = T.Tria(T.BoxCoarse(dim, (3, 2, 2)[:dim], (1.5, 1.0, 1.2)[:dim], periodic=(0,)), 1)
+    c = t.cell_centers()
+    t.refine((c[:, 0] < 0.3) | ((c[:, 0] > 1.3) & (c[:, 1] < 0.5)))   # hanging nodes across the seam
+    return t, bc
+
+
 def shell(dim, g):
     t = T.Tria(T.ShellCoarse(dim), g)
     c = t.cell_centers()
@@ -41,6 +48,7 @@ CASES = {
     "box3d_compressible": lambda: (refined_box(3, 2, {1: "tangential", 4: "zero"}), 0.7, True),
     "box2d": lambda: (refined_box(2, 3, {0: "zero", 1: "tangential", 2: "tangential"}), 1.1, False),
     "shell3d": lambda: (shell(3, 0), 1.0, False),
+    "periodic3d": lambda: (periodic_box(3, None)[0].build({2: "zero", 3: "tangential"}), 1.2, False),
     "shell2d": lambda: (shell(2, 2), 2.0, False),
 }
 
@@ -105,6 +113,11 @@ from aspect_amd import TriaMultigrid  # noqa: E402
 
 
 def gmg_case(name):
+    if name == "periodic3d":
+        t, bc = periodic_box(3, {2: "zero", 3: "zero"})
+        _, meshes, pu, pp = T.hierarchy(t, bc)
+        etas = [np.exp(2.0 * np.sin(3.0 * np.linalg.norm(m.x_q, axis=2))) for m in meshes]
+        return meshes, pu, pp, etas, False
     if name == "box3d":
         t = T.Tria(T.BoxCoarse(3, (3, 2, 2), (1.5, 1.0, 1.2)), 1)
         t.refine(np.random.default_rng(1).random(t.n_cells) < 0.35)
@@ -120,7 +133,7 @@ def gmg_case(name):
     return meshes, pu, pp, etas, consistent
 
 
-@pytest.mark.parametrize("name", ["box3d", "shell3d", "shell2d"])
+@pytest.mark.parametrize("name", ["box3d", "shell3d", "shell2d", "periodic3d"])
 def test_tria_multigrid_and_solve_match_oracle(name):
     meshes, pu, pp, etas, consistent = gmg_case(name)
     print(f"\n[tria gmg] {name}: levels {[m.n_cells for m in meshes]}", flush=True)

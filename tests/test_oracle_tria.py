@@ -396,3 +396,126 @@ def test_oracle_solve_matches_direct_solver():
     xd = spsolve(K.tocsc(), b)
     assert its < 60
     assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+# ------------------------------------------------------------------ periodic boxes
+def merged_mesh(mesh):
+    """The same mesh with every pure periodic line (one master, weight 1, same
+    component) replaced by identifying the two nodes in the cell lists."""
+    import copy
+    m = copy.copy(mesh)
+    nn = mesh.n_nodes_u
+    dofs, offs, masters, weights = mesh.con_u
+    node_map = np.arange(nn)
+    keep = np.ones(len(dofs), bool)
+    for i, d in enumerate(dofs):
+        if offs[i + 1] - offs[i] == 1 and weights[offs[i]] == 1.0 and masters[offs[i]] % nn != d % nn \
+                and masters[offs[i]] // nn == d // nn:
+            node_map[d % nn] = masters[offs[i]] % nn
+            keep[i] = False
+    m.cell_u = node_map[mesh.cell_u]
+    ld = dofs[keep]
+    lo = np.concatenate([[0], np.cumsum(np.diff(offs)[keep])])
+    lm = np.concatenate([masters[offs[i]:offs[i + 1]] for i in np.nonzero(keep)[0]]) if keep.any() else masters[:0]
+    lw = np.concatenate([weights[offs[i]:offs[i + 1]] for i in np.nonzero(keep)[0]]) if keep.any() else weights[:0]
+    m.con_u = (ld, lo, lm, lw)
+    dofs, offs, masters, weights = mesh.con_p
+    pmap = np.arange(mesh.n_nodes_p)
+    keep = np.diff(offs) != 1
+    for i in np.nonzero(~keep)[0]:
+        pmap[dofs[i]] = masters[offs[i]]
+    m.cell_p = pmap[mesh.cell_p]
+    m.con_p = (dofs[keep], np.concatenate([[0], np.cumsum(np.diff(offs)[keep])]), masters[:0], weights[:0])
+    return m, node_map, pmap
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_periodic_lines_match_node_identification(dim):
+    """A periodic box through constraint lines equals the box whose + face
+    nodes are identified with the - face nodes in the cell lists."""
+    t = T.Tria(T.BoxCoarse(dim, (3, 2, 2)[:dim], (1.5, 1.0, 1.2)[:dim], periodic=(0,)), 1)
+    mesh = t.build({2: "zero", 3: "tangential"})
+    merged, node_map, pmap = merged_mesh(mesh)
+    rng = np.random.default_rng(21)
+    eta = np.exp(rng.normal(size=(mesh.n_cells, 3 ** dim)))
+    a = OT.Operators(mesh, eta, pressure_scaling=1.3)
+    b = OT.Operators(merged, eta, pressure_scaling=1.3)
+    x = rng.normal(size=a.nu + a.np)
+    slave_u = np.zeros(a.nu, bool)
+    for c in range(dim):
+        slave_u[c * mesh.n_nodes_u + np.nonzero(node_map != np.arange(mesh.n_nodes_u))[0]] = True
+    slave_p = pmap != np.arange(mesh.n_nodes_p)
+    x[:a.nu][slave_u] = 0.0
+    x[a.nu:][slave_p] = 0.0
+    ya, yb = a.stokes(x), b.stokes(x)
+    keep = np.concatenate([~slave_u, ~slave_p])
+    assert slave_u.any() and slave_p.any()
+    assert np.max(np.abs(ya[keep] - yb[keep])) <= 1e-12 * np.max(np.abs(yb))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_periodic_seam_hanging_nodes(dim):
+    """Hanging nodes across the periodic seam: fields of the non-periodic
+    coordinates are reproduced there, and the energy of a quadratic field
+    independent of x matches its closed form."""
+    extent = (1.5, 1.0, 1.2)[:dim]
+    t = T.Tria(T.BoxCoarse(dim, (3, 2, 2)[:dim], extent, periodic=(0,)), 1)
+    c = t.cell_centers()
+    t.refine((c[:, 0] < 0.3) & (c[:, 1] < 0.6))
+    mesh = t.build()
+    assert len(mesh.hanging_u) > 0
+    seam = np.abs(mesh.x_u[mesh.hanging_u, 0]) < 1e-9
+    assert seam.any()                       # hanging nodes found across the seam
+    rng = np.random.default_rng(22)
+    U = []
+    for _ in range(dim):
+        cf = rng.normal(size=(3,) * (dim - 1))
+        U.append(cf)
+
+    def val(cf, x):
+        y = x[:, 1:]
+        out = np.zeros(len(x))
+        for a in np.ndindex(*cf.shape):
+            out += cf[a] * np.prod([y[:, k] ** a[k] for k in range(dim - 1)], axis=0)
+        return out
+
+    def grad(cf, x):
+        y = x[:, 1:]
+        g = np.zeros((len(x), dim))
+        for a in np.ndindex(*cf.shape):
+            for k in range(dim - 1):
+                if a[k] == 0:
+                    continue
+                t_ = cf[a] * a[k] * y[:, k] ** (a[k] - 1)
+                for j in range(dim - 1):
+                    if j != k:
+                        t_ = t_ * y[:, j] ** a[j]
+                g[:, k + 1] += t_
+        return g
+
+    x = np.concatenate([val(cf, mesh.x_u) for cf in U])
+    x0 = x.copy()
+    fixed = mesh.con_u[0]
+    x0[fixed] = 0.0
+    assert np.max(np.abs(OT.resolve(mesh.con_u, x0) - x)) <= 1e-12 * np.max(np.abs(x))
+    ops = OT.Operators(mesh, np.full((mesh.n_cells, 3 ** dim), 1.9))
+    X, W = domain_rule(dim, extent)
+    gu = np.stack([grad(cf, X) for cf in U], axis=1)
+    e = 0.5 * (gu + gu.transpose(0, 2, 1))
+    ref = np.sum(W * 1.9 * 2 * np.sum(e * e, axis=(1, 2)))
+    assert abs(x0 @ ops.ablock(x0) - ref) <= 1e-11 * ref
+
+
+def test_fully_periodic_translations_in_kernel():
+    t = T.Tria(T.BoxCoarse(2, (2, 3), (1.0, 1.5), periodic=(0, 1)), 1)
+    c = t.cell_centers()
+    t.refine(c[:, 0] + c[:, 1] < 0.8)
+    mesh = t.build()
+    ops = OT.Operators(mesh, np.exp(np.random.default_rng(23).normal(size=(mesh.n_cells, 9))))
+    free = np.ones(ops.nu, bool)
+    free[mesh.con_u[0]] = False
+    for comp in range(2):
+        u = np.zeros(ops.nu)
+        u[comp * mesh.n_nodes_u:(comp + 1) * mesh.n_nodes_u] = 1.0
+        u[~free] = 0.0
+        assert np.max(np.abs(ops.ablock(u)[free])) <= 1e-12
