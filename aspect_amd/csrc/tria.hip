@@ -499,6 +499,20 @@ int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst) {
   });
 }
 
+int mfs_tria_get_viscosity(mfs_tria *t, double *eta_qp) {
+  return tguard([&] {
+    need(t);
+    if (!eta_qp) throw std::invalid_argument("mfs_tria_get_viscosity: null output");
+    if (!t->have_eta) throw std::runtime_error("viscosity not set");
+    std::vector<double> be(t->nbatch * t->nq * CT);
+    MFS_HIP(hipMemcpyAsync(be.data(), t->eta.p, be.size() * sizeof(double), hipMemcpyDeviceToHost, t->st));
+    MFS_HIP(hipStreamSynchronize(t->st));
+    for (long long c = 0; c < t->ncells; ++c)
+      for (int q = 0; q < t->nq; ++q) eta_qp[c * t->nq + q] = be[((c / CT) * t->nq + q) * CT + c % CT];
+    return MFS_OK;
+  });
+}
+
 int mfs_tria_time(mfs_tria *t, int op, const double *d_src, double *d_dst, int reps, double *avg_seconds) {
   return tguard([&] {
     need(t);
@@ -558,6 +572,60 @@ __global__ void k_invert(const long long n, double *__restrict__ x) {
 }
 inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
+// Active table of a general level from the viscosity at the quadrature points
+// (natural [cell][q]): mode 0 as given, 1/2/3 the arithmetic / harmonic /
+// geometric mean of the cell's values at every point (material averaging,
+// source/material_model/interface.cc:556-680); and the DGQ0 value of the cell
+// (JxW-weighted mean of the table: the cellwise L2 projection onto constants)
+__global__ void k_tria_active_eta(const long long ncells, const int nq, const int G, const double *__restrict__ eta,
+                                  const int mode, const double *__restrict__ geo, double *__restrict__ table,
+                                  double *__restrict__ cellval) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const long long b = c / 64, cc = c - b * 64;
+  double acc = 0;
+  bool zero = false;
+  for (int q = 0; q < nq; ++q) {
+    const double v = eta[c * nq + q];
+    if (mode == 2) {
+      if (v == 0.0) zero = true;
+      else acc += 1.0 / v;
+    } else if (mode == 3) {
+      acc += log(v);
+    } else {
+      acc += v;
+    }
+  }
+  const double avg = mode == 0 ? 0.0 : mode == 1 ? acc / nq : mode == 2 ? (zero ? 0.0 : nq / acc) : exp(acc / nq);
+  double s = 0, ws = 0;
+  for (int q = 0; q < nq; ++q) {
+    const double v = mode == 0 ? eta[c * nq + q] : avg;
+    table[(b * nq + q) * 64 + cc] = v;
+    const double w = geo[((b * nq + q) * G + G - 1) * 64 + cc];
+    s += w * v;
+    ws += w;
+  }
+  cellval[c] = s / ws;
+}
+// coarse cell value = mean of its fine cells (the FE_DGQ(0) restriction: the
+// children of a refined cell, or the same cell when it was not refined)
+__global__ void k_tria_restrict(const long long ncoarse, const long long *__restrict__ ptr,
+                                const long long *__restrict__ idx, const double *__restrict__ fine,
+                                double *__restrict__ coarse) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= ncoarse) return;
+  double s = 0;
+  for (long long k = ptr[c]; k < ptr[c + 1]; ++k) s += fine[idx[k]];
+  coarse[c] = s / (double)(ptr[c + 1] - ptr[c]);
+}
+__global__ void k_tria_fill_eta(const long long ncells, const int nq, const double *__restrict__ cellval,
+                                double *__restrict__ table) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= ncells * nq) return;
+  const long long c = i / nq, q = i - c * nq, b = c / 64, cc = c - b * 64;
+  table[(b * nq + q) * 64 + cc] = cellval[c];
+}
+
 struct DevCsr {
   long long rows = 0, cols = 0;
   Buf<long long> ptr, col;
@@ -590,6 +658,8 @@ struct TLevel {
   int ncolours[2] = {0, 0};
   Cheb ch[2];
   DevCsr P[2], R[2];  // prolongation from the next coarser level and its transpose
+  Buf<long long> kid_ptr, kid_idx;  // cells of the next finer level inside each cell of this level
+  Buf<double> cellval;              // DGQ0 viscosity of every cell
 };
 
 }  // namespace
@@ -647,7 +717,10 @@ struct mfs_tria_gmg {
       }
   }
 
-  void create(int n_levels, mfs_tria *const *levels, const mfs_csr *pu, const mfs_csr *pp) {
+  bool have_parents = false;
+
+  void create(int n_levels, mfs_tria *const *levels, const mfs_csr *pu, const mfs_csr *pp,
+              const long long *const *parents) {
     if (n_levels < 1 || !levels) throw std::invalid_argument("mfs_tria_gmg_create: no levels");
     if (n_levels > 1 && (!pu || !pp)) throw std::invalid_argument("mfs_tria_gmg_create: null prolongations");
     for (int l = 0; l < n_levels; ++l) {
@@ -680,12 +753,70 @@ struct mfs_tria_gmg {
         L.R[k].set(lv[l - 1].n[k], L.n[k], tp, tc, tv, st);
       }
     }
+    if (parents) {
+      // children lists of every level from the fine-to-coarse cell maps
+      for (int l = 1; l < n_levels; ++l) {
+        const long long nf = lv[l].t->ncells, nc = lv[l - 1].t->ncells;
+        const long long *par = parents[l - 1];
+        if (!par) throw std::invalid_argument("mfs_tria_gmg_create: null parent map");
+        std::vector<long long> ptr(nc + 1, 0), idx(nf);
+        for (long long f = 0; f < nf; ++f) {
+          if (par[f] < 0 || par[f] >= nc) throw std::invalid_argument("mfs_tria_gmg_create: parent out of range");
+          ++ptr[par[f] + 1];
+        }
+        for (long long c = 0; c < nc; ++c) {
+          if (ptr[c + 1] == 0) throw std::invalid_argument("mfs_tria_gmg_create: a coarse cell without fine cells");
+          ptr[c + 1] += ptr[c];
+        }
+        std::vector<long long> fill(ptr.begin(), ptr.end() - 1);
+        for (long long f = 0; f < nf; ++f) idx[fill[par[f]]++] = f;
+        lv[l - 1].kid_ptr.upload(ptr, st);
+        lv[l - 1].kid_idx.upload(idx, st);
+      }
+      for (auto &L : lv) L.cellval.alloc(L.t->ncells);
+      have_parents = true;
+    }
     long long nmax = 0;
     for (auto &L : lv) nmax = std::max(nmax, L.n[0] + L.n[1]);
     dwork.alloc(dot_work_size());
     dout.alloc(1);
     rtmp.alloc(nmax);
     MFS_HIP(hipStreamSynchronize(st));
+  }
+
+  // viscosity of every level from the finest level's values at the quadrature
+  // points (the global-coarsening evaluate_material_model,
+  // stokes_matrix_free_global_coarsening.cc:173-330: active table with the
+  // material averaging, its DGQ0 projection transferred down the levels,
+  // one value per cell on the levels)
+  void set_viscosity(const double *eta_qp, int averaging) {
+    if (!have_parents) throw std::runtime_error("mfs_tria_gmg_set_viscosity: created without parent maps");
+    if (!eta_qp) throw std::invalid_argument("mfs_tria_gmg_set_viscosity: null viscosity");
+    if (averaging < 0 || averaging > 3)
+      throw std::invalid_argument("mfs_tria_gmg_set_viscosity: averaging must be none, arithmetic, harmonic or geometric");
+    TLevel &F = lv.back();
+    mfs_tria *t = F.t;
+    const long long nc = t->ncells;
+    for (long long i = 0; i < nc * t->nq; ++i)
+      if (!(eta_qp[i] > 0) || !std::isfinite(eta_qp[i]))
+        throw std::invalid_argument("mfs_tria_gmg_set_viscosity: viscosity must be positive");
+    Buf<double> nat;
+    nat.alloc(nc * t->nq);
+    MFS_HIP(hipMemcpyAsync(nat.p, eta_qp, nc * t->nq * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_tria_active_eta, dim3(nblk(nc)), dim3(256), 0, st, nc, t->nq, t->dim * t->dim + 1, nat.p,
+                       averaging, t->geo.p, t->eta.p, F.cellval.p);
+    for (int l = (int)lv.size() - 2; l >= 0; --l) {
+      TLevel &C = lv[l];
+      const long long n = C.t->ncells;
+      hipLaunchKernelGGL(k_tria_restrict, dim3(nblk(n)), dim3(256), 0, st, n, C.kid_ptr.p, C.kid_idx.p,
+                         lv[l + 1].cellval.p, C.cellval.p);
+      hipLaunchKernelGGL(k_tria_fill_eta, dim3(nblk(n * C.t->nq)), dim3(256), 0, st, n, C.t->nq, C.cellval.p,
+                         C.t->eta.p);
+    }
+    MFS_HIP(hipGetLastError());
+    MFS_HIP(hipStreamSynchronize(st));
+    for (auto &L : lv) L.t->have_eta = true;
+    built = false;
   }
 
   // exact diagonal of C^T A C (identity on constrained DoFs) by probing
@@ -993,13 +1124,13 @@ struct mfs_tria_gmg {
 extern "C" {
 
 int mfs_tria_gmg_create(int n_levels, mfs_tria *const *levels, const mfs_csr *prolong_u, const mfs_csr *prolong_p,
-                        mfs_tria_gmg **out) {
+                        const long long *const *parents, mfs_tria_gmg **out) {
   return tguard([&] {
     if (!out) throw std::invalid_argument("mfs_tria_gmg_create: null output");
     *out = nullptr;
     mfs_tria_gmg *g = new mfs_tria_gmg();
     try {
-      g->create(n_levels, levels, prolong_u, prolong_p);
+      g->create(n_levels, levels, prolong_u, prolong_p, parents);
     } catch (...) {
       delete g;
       throw;
@@ -1015,6 +1146,14 @@ int mfs_tria_gmg_destroy(mfs_tria_gmg *g) {
       (void)hipStreamSynchronize(g->st);
       delete g;
     }
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_gmg_set_viscosity(mfs_tria_gmg *g, const double *eta_qp, int averaging) {
+  return tguard([&] {
+    if (!g) throw std::invalid_argument("null mfs_tria_gmg");
+    g->set_viscosity(eta_qp, averaging);
     return MFS_OK;
   });
 }

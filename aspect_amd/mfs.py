@@ -40,7 +40,7 @@ EXPORTED = [
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
     "mfs_box_dof_index", "mfs_tria_create", "mfs_tria_destroy", "mfs_tria_sizes", "mfs_tria_set_viscosity",
-    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_time", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_build",
+    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_time", "mfs_tria_get_viscosity", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_set_viscosity", "mfs_tria_gmg_build",
     "mfs_tria_gmg_info", "mfs_tria_gmg_vcycle", "mfs_tria_solve",
 ]
 
@@ -641,6 +641,12 @@ class TriaOperators:
             raise ValueError("one viscosity per cell and quadrature point expected")
         _check(self.lib.mfs_tria_set_viscosity(self._t, _ptr(e)))
 
+    def viscosity(self):
+        """The object's viscosity table [cell][q]."""
+        out = np.empty(self.n_cells * 3 ** self.dim)
+        _check(self.lib.mfs_tria_get_viscosity(self._t, _ptr(out)))
+        return out.reshape(self.n_cells, -1)
+
     def vmult(self, op, src):
         o = self.OPS[op]
         n_in = {0: self.nu + self.np, 1: self.nu, 2: self.np, 3: self.np}[o]
@@ -681,9 +687,11 @@ class TriaMultigrid:
     :1259-1313).  ``levels``: ``TriaOperators`` with their viscosity set;
     ``prolong_u`` / ``prolong_p``: scipy CSR prolongations, entry l mapping
     level l-1 to level l (entry 0 unused), as ``aspect_amd.tria.hierarchy``
-    returns them."""
+    returns them; ``parents`` (optional): entry l maps the cells of level l to
+    the cells of level l-1 (``aspect_amd.tria.parent_map``), needed by
+    ``set_viscosity``."""
 
-    def __init__(self, levels, prolong_u, prolong_p):
+    def __init__(self, levels, prolong_u, prolong_p, parents=None):
         self.lib = load_library()
         self.levels = list(levels)   # kept alive: the multigrid object refers to them
         n = len(self.levels)
@@ -702,8 +710,24 @@ class TriaMultigrid:
         cu = (Csr * max(n - 1, 1))(*[csr(prolong_u[l]) for l in range(1, n)])
         cp = (Csr * max(n - 1, 1))(*[csr(prolong_p[l]) for l in range(1, n)])
         lv = (C.c_void_p * n)(*[t._t.value for t in self.levels])
+        par = None
+        if parents is not None:
+            arrs = [np.ascontiguousarray(parents[l], np.int64) for l in range(1, n)]
+            keep.extend(arrs)
+            par = (C.c_void_p * max(n - 1, 1))(*[a.ctypes.data for a in arrs])
         self._g = C.c_void_p()
-        _check(self.lib.mfs_tria_gmg_create(n, lv, cu, cp, C.byref(self._g)))
+        _check(self.lib.mfs_tria_gmg_create(n, lv, cu, cp, par, C.byref(self._g)))
+
+    def set_viscosity(self, eta_qp, averaging="none"):
+        """Viscosity of every level from the finest level's values at its
+        quadrature points (material averaging on the finest level, DGQ0 cell
+        means transferred down)."""
+        e = np.ascontiguousarray(eta_qp, dtype=np.float64)
+        t = self.levels[-1]
+        if e.size != t.n_cells * 3 ** t.dim:
+            raise ValueError("one viscosity per cell and quadrature point of the finest level expected")
+        mode = AVERAGE[averaging] if isinstance(averaging, str) else int(averaging)
+        _check(self.lib.mfs_tria_gmg_set_viscosity(self._g, _ptr(e), mode))
 
     def build(self):
         _check(self.lib.mfs_tria_gmg_build(self._g))

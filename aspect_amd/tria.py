@@ -41,7 +41,8 @@ from scipy.sparse import coo_matrix
 from scipy.sparse.csgraph import connected_components
 from scipy.spatial import cKDTree
 
-__all__ = ["BoxCoarse", "ShellCoarse", "Tria", "TriaMesh", "gauss3"]
+__all__ = ["BoxCoarse", "ShellCoarse", "Tria", "TriaMesh", "coarsening_sequence", "gauss3", "hierarchy",
+           "parent_map", "transfer"]
 
 
 def gauss3():
@@ -613,6 +614,18 @@ def transfer(fine, coarse, mf, mc):
         P.sort_indices()
         out.append(P)
     return out[0], out[1]
+
+
+def parent_map(fine, coarse):
+    """For every cell of Tria fine, the cell of Tria coarse (the next level of
+    the coarsening sequence) that contains it: its parent, or the same cell."""
+    lookup = {_cell_key(coarse.root[c], coarse.level[c], coarse.idx[c]): c for c in range(coarse.n_cells)}
+    out = np.empty(fine.n_cells, dtype=np.int64)
+    for c in range(fine.n_cells):
+        key = _cell_key(fine.root[c], fine.level[c], fine.idx[c])
+        out[c] = lookup[key] if key in lookup else lookup[_cell_key(fine.root[c], fine.level[c] - 1,
+                                                                   fine.idx[c] // 2)]
+    return out
 
 
 def hierarchy(tria, bc=None):

@@ -341,15 +341,14 @@ def main():
         if not args.no_solve:
             t0 = time.perf_counter()
             tr2 = P.config_shell(3, max(args.shell_refinement - 1, 0))
-            _, meshes, pu, pp = TR.hierarchy(tr2, {0: "zero", 1: "tangential"})
-            levels = []
-            for m in meshes:
-                lo = TriaOperators(m, pressure_scaling=1.0)
-                lo.set_viscosity(P.shell_viscosity(m.x_q))
-                levels.append(lo)
+            trias, meshes, pu, pp = TR.hierarchy(tr2, {0: "zero", 1: "tangential"})
+            parents = [None] + [TR.parent_map(trias[l], trias[l - 1]) for l in range(1, len(trias))]
+            levels = [TriaOperators(m, pressure_scaling=1.0) for m in meshes]
             t_h = time.perf_counter() - t0
-            mg = TriaMultigrid(levels, pu, pp)
+            mg = TriaMultigrid(levels, pu, pp, parents)
             t0 = time.perf_counter()
+            # material evaluation on the finest level, harmonic averaging, DGQ0 level tables
+            mg.set_viscosity(P.shell_viscosity(meshes[-1].x_q), "harmonic")
             mg.build()
             t_b = time.perf_counter() - t0
             top = meshes[-1]
@@ -368,7 +367,8 @@ def main():
                             "expensive_iterations": out.expensive_iterations,
                             "cheap_phase_seconds": out.cheap_seconds, "converged": bool(out.converged),
                             "rel_residual": resid, "gmg_build_seconds": t_b, "host_setup_seconds": t_h,
-                            "boundary": "zero velocity inside, free slip outside"}
+                            "boundary": "zero velocity inside, free slip outside",
+                            "averaging": "harmonic average only viscosity, DGQ0 level tables"}
             log(f"general mesh solve: levels {[m.n_cells for m in meshes]}, {out.cheap_iterations}+"
                 f"{out.expensive_iterations} iterations, {out.solve_seconds:.2f} s, residual {resid:.2e}")
             del mg, levels

@@ -388,6 +388,8 @@ int mfs_tria_set_viscosity(mfs_tria *t, const double *eta_qp);
 int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst);
 /* The same on device vectors (hipMalloc'd by the caller). */
 int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_dst);
+/* The viscosity table of the object, [cell][q] (host). */
+int mfs_tria_get_viscosity(mfs_tria *t, double *eta_qp);
 /* Average device time of reps applies on device vectors, bracketed by hipEvents
  * on the object's stream ("Execute solver timings", local_smoothing.cc:1119-1258). */
 int mfs_tria_time(mfs_tria *t, int op, const double *d_src, double *d_dst, int reps, double *avg_seconds);
@@ -407,9 +409,20 @@ typedef struct {
   const long long *col_idx;
   const double *values;
 } mfs_csr;
+/* parents (may be NULL): n_levels-1 arrays, parents[l][c] = the cell of level l
+ * that contains cell c of level l+1 (its parent, or the same cell when it was
+ * not refined); needed by mfs_tria_gmg_set_viscosity. */
 int mfs_tria_gmg_create(int n_levels, mfs_tria *const *levels, const mfs_csr *prolong_u, const mfs_csr *prolong_p,
-                        mfs_tria_gmg **out);
+                        const long long *const *parents, mfs_tria_gmg **out);
 int mfs_tria_gmg_destroy(mfs_tria_gmg *g);
+/* Viscosity of every level from the finest level's values at its quadrature
+ * points ([cell][q], host), as the global-coarsening handler's
+ * evaluate_material_model does (stokes_matrix_free_global_coarsening.cc:173-330):
+ * the finest level takes them with the material averaging (MFS_AVERAGE_NONE,
+ * ARITHMETIC, HARMONIC or GEOMETRIC), every coarser level one value per cell,
+ * the mean of its cells on the next finer level of their DGQ0 values (JxW-weighted
+ * cell means).  Replaces mfs_tria_set_viscosity on every level. */
+int mfs_tria_gmg_set_viscosity(mfs_tria_gmg *g, const double *eta_qp, int averaging);
 /* Diagonals of every level operator (exact, constraints included: probing by
  * DoF colours; 1 on constrained DoFs) and the Chebyshev set-up: levels > 0
  * degree 4 over [lambda_max / 15, lambda_max] from 10 eigenvalue iterations,

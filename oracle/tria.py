@@ -392,3 +392,31 @@ def gmres(apply, prec, b, tol, maxit, restart, x0=None):
         r = b - apply(x)
         beta = abs(g[k])
     return x, its, beta
+
+
+def level_viscosities(meshes, parents, eta_qp, averaging=0):
+    """Viscosity tables of every level (coarse first) from the finest level's
+    values at its quadrature points: material averaging on the finest level
+    (0 none, 1 arithmetic, 2 harmonic, 3 geometric mean of the cell's point
+    values; material_model/interface.cc:556-680), its DGQ0 projection (JxW-
+    weighted cell mean), and on coarser levels one value per cell, the mean of
+    the contained cells of the next finer level (FE_DGQ(0) restriction in
+    MGTwoLevelTransfer::interpolate, stokes_matrix_free_global_coarsening.cc:
+    209-246)."""
+    e = np.asarray(eta_qp, float).reshape(meshes[-1].n_cells, -1)
+    nq = e.shape[1]
+    if averaging == 1:
+        e = np.repeat(e.mean(axis=1, keepdims=True), nq, axis=1)
+    elif averaging == 2:
+        e = np.repeat(1.0 / np.mean(1.0 / e, axis=1, keepdims=True), nq, axis=1)
+    elif averaging == 3:
+        e = np.repeat(np.exp(np.mean(np.log(e), axis=1, keepdims=True)), nq, axis=1)
+    jxw = meshes[-1].geometry[:, :, -1]
+    val = np.sum(jxw * e, axis=1) / np.sum(jxw, axis=1)
+    out = [e]
+    for l in range(len(meshes) - 2, -1, -1):
+        par = parents[l + 1]
+        cnt = np.bincount(par, minlength=meshes[l].n_cells)
+        val = np.bincount(par, weights=val, minlength=meshes[l].n_cells) / cnt
+        out.append(np.repeat(val[:, None], nq, axis=1))
+    return out[::-1]

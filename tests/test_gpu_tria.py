@@ -206,3 +206,38 @@ def test_tria_solve_expensive_phase_matches_oracle(name):
     assert out.cheap_iterations == its_c == 5
     assert abs(out.expensive_iterations - its_e) <= max(1, 0.1 * its_e)
     assert np.linalg.norm(b - top.stokes(x)) <= 1.5e-8 * np.linalg.norm(b)
+
+
+def test_tria_level_viscosity_and_solve_match_oracle():
+    """Level tables from the finest level's point values (material averaging,
+    DGQ0 cell means, transferred down), every averaging mode, and a solve with
+    them."""
+    t = T.Tria(T.ShellCoarse(3), 0)
+    c = t.cell_centers()
+    t.refine((c[:, 0] > 0.2) & (np.linalg.norm(c, axis=1) > 0.75))
+    trias, meshes, pu, pp = T.hierarchy(t, {0: "zero", 1: "tangential"})
+    parents = [None] + [T.parent_map(trias[l], trias[l - 1]) for l in range(1, len(trias))]
+    eta = np.exp(2.0 * np.sin(3.0 * np.linalg.norm(meshes[-1].x_q, axis=2)) +
+                 0.5 * np.random.default_rng(14).normal(size=meshes[-1].x_q.shape[:2]))
+    levels = [TriaOperators(m, pressure_scaling=1.0) for m in meshes]
+    mg = TriaMultigrid(levels, pu, pp, parents)
+    for mode in (0, 1, 2, 3):
+        mg.set_viscosity(eta, mode)
+        ref = OT.level_viscosities(meshes, parents, eta, mode)
+        for l, lo in enumerate(levels):
+            assert rel(lo.viscosity(), ref[l]) <= 1e-14
+    mg.set_viscosity(eta, "harmonic")
+    mg.build()
+    og = OT.GMG(meshes, pu, pp, OT.level_viscosities(meshes, parents, eta, 2), pressure_scaling=1.0)
+    top = og.ops[-1]
+    b = np.random.default_rng(15).normal(size=top.nu + top.np)
+    b[top.fixed_u] = 0.0
+    b[top.nu + top.fixed_p] = 0.0
+    fp = np.ones(top.np, bool)
+    fp[top.fixed_p] = False
+    bp = b[top.nu:]
+    bp[fp] -= bp[fp].mean()
+    _, its_ref, _ = OT.gmres(top.stokes, og.block_preconditioner, b, 1e-8 * np.linalg.norm(b), 500, 100)
+    x, out, rc = mg.solve(b, tolerance=1e-8, n_cheap_steps=500, restart=100)
+    assert rc == 0 and abs(out.cheap_iterations - its_ref) <= max(2, 0.1 * its_ref)
+    assert np.linalg.norm(b - top.stokes(x)) <= 1.5e-8 * np.linalg.norm(b)

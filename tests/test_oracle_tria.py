@@ -519,3 +519,23 @@ def test_fully_periodic_translations_in_kernel():
         u[comp * mesh.n_nodes_u:(comp + 1) * mesh.n_nodes_u] = 1.0
         u[~free] = 0.0
         assert np.max(np.abs(ops.ablock(u)[free])) <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_level_viscosities_of_a_linear_field(dim):
+    """On affine cells the DGQ0 value of a linear field is its value at the
+    cell centre, and the mean over the children of a cell is the value at the
+    parent's centre: every level's table is the field at its cell centres."""
+    t = T.Tria(T.BoxCoarse(dim, (3, 2, 2)[:dim], (1.5, 1.0, 1.2)[:dim]), 1)
+    t.refine(np.random.default_rng(31).random(t.n_cells) < 0.35)
+    trias, meshes, _, _ = T.hierarchy(t)
+    parents = [None] + [T.parent_map(trias[l], trias[l - 1]) for l in range(1, len(trias))]
+    a = np.array([0.7, -0.4, 0.9])[:dim]
+    f = lambda x: 3.0 + x @ a
+    tabs = OT.level_viscosities(meshes, parents, f(meshes[-1].x_q.reshape(-1, dim)).reshape(meshes[-1].n_cells, -1))
+    for l in range(len(meshes) - 1):
+        ref = f(trias[l].cell_centers())
+        assert np.max(np.abs(tabs[l] - ref[:, None])) <= 1e-13 * np.max(np.abs(ref))
+    # harmonic averaging of a cellwise constant table leaves it unchanged
+    cst = np.repeat(np.exp(np.random.default_rng(32).normal(size=(meshes[-1].n_cells, 1))), 3 ** dim, axis=1)
+    assert np.allclose(OT.level_viscosities(meshes, parents, cst, 2)[-1], cst, rtol=1e-14)
