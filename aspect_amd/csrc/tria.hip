@@ -670,7 +670,21 @@ struct mfs_tria_gmg {
   Buf<double> dwork, dout, rtmp;
   bool built = false;
 
+  // each V-cycle kind captured once into a graph after build() (the level
+  // right-hand side and solution buffers are fixed, so the graph replays
+  // unchanged); dropped when the smoothers change
+  hipGraphExec_t vexec[2] = {nullptr, nullptr};
+  bool graphs = true;
+  void drop_graphs() {
+    for (auto &g : vexec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
+
   ~mfs_tria_gmg() {
+    drop_graphs();
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -816,6 +830,7 @@ struct mfs_tria_gmg {
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipStreamSynchronize(st));
     for (auto &L : lv) L.t->have_eta = true;
+    drop_graphs();
     built = false;
   }
 
@@ -918,6 +933,7 @@ struct mfs_tria_gmg {
   }
 
   void build() {
+    drop_graphs();
     for (int l = 0; l < (int)lv.size(); ++l) {
       for (int k = 0; k < 2; ++k) {
         if (!lv[l].t->have_eta) throw std::runtime_error("mfs_tria_gmg_build: viscosity not set on every level");
@@ -981,7 +997,26 @@ struct mfs_tria_gmg {
     if (!built) throw std::runtime_error("multigrid not built (call mfs_tria_gmg_build)");
     TLevel &L = lv.back();
     MFS_HIP(hipMemcpyAsync(L.b[kind].p, src, L.n[kind] * sizeof(double), hipMemcpyDeviceToDevice, st));
-    vcycle_level(kind, (int)lv.size() - 1);
+    if (graphs) {
+      if (!vexec[kind]) {
+        hipGraph_t gr = nullptr;
+        MFS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+          vcycle_level(kind, (int)lv.size() - 1);
+        } catch (...) {
+          (void)hipStreamEndCapture(st, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        MFS_HIP(hipStreamEndCapture(st, &gr));
+        const hipError_t e = hipGraphInstantiate(&vexec[kind], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        MFS_HIP(e);
+      }
+      MFS_HIP(hipGraphLaunch(vexec[kind], st));
+    } else {
+      vcycle_level(kind, (int)lv.size() - 1);
+    }
     MFS_HIP(hipMemcpyAsync(dst, L.x[kind].p, L.n[kind] * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
 
@@ -1053,38 +1088,66 @@ struct mfs_tria_gmg {
     a_apps += cg(0, rtmp.p, dst, nt * a_tol, 10000);
   }
 
-  // right-preconditioned restarted GMRES (modified Gram-Schmidt) from the
-  // initial guess in x, keeping the preconditioned vectors (flexible: the
-  // expensive preconditioner is not a fixed linear map)
+  // one classical Gram-Schmidt pass against the first m basis vectors (one
+  // fused partial-dot sweep, one fixed-order reduction, one fused update),
+  // coefficients to the host
+  Buf<double> gs_work, gs_h;
+  void cgs_pass(const double *Vbase, long long n, int m, double *w, std::vector<double> &h) {
+    VecSet V{};
+    for (int c = 0; c * 8 < m; ++c) V.p[c] = Vbase + (long long)8 * c * n;
+    v_mdot_all(V, n, m, w, n, gs_work.p, st);
+    v_sum_partials(gs_work.p, m, gs_h.p, st);
+    v_maxpy_all(V, n, m, gs_h.p, w, n, -1.0, st);
+    h.resize(m);
+    MFS_HIP(hipMemcpyAsync(h.data(), gs_h.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    MFS_HIP(hipStreamSynchronize(st));
+  }
+
+  // right-preconditioned restarted GMRES from the initial guess in x,
+  // keeping the preconditioned vectors (flexible: the expensive
+  // preconditioner is not a fixed linear map); classical Gram-Schmidt with a
+  // second pass when the first cancelled more than 1/sqrt(2) of |w|
   template <typename Prec>
   void gmres(const double *b, double *x, double tol, int maxit, int restart, Prec prec, int &its, double &res) {
     TLevel &L = lv.back();
     const long long n = L.n[0] + L.n[1];
-    std::vector<Buf<double>> V(restart + 1), Z(restart);
-    for (auto &v : V) v.alloc(n);
+    if (restart + 1 > MV_MAX) throw std::invalid_argument("mfs_tria_solve: restart must be below 127");
+    Buf<double> Vb, w;
+    Vb.alloc((restart + 1) * n);
+    std::vector<Buf<double>> Z(restart);
     for (auto &z : Z) z.alloc(n);
-    Buf<double> w;
     w.alloc(n);
+    if (!gs_work.p) {
+      gs_work.alloc((long long)MV_MAX * dot_work_size());
+      gs_h.alloc(MV_MAX);
+    }
+    auto Vp = [&](int i) { return Vb.p + (long long)i * n; };
+    std::vector<double> hv, hv2;
     its = 0;
-    L.t->vmult(MFS_OP_STOKES, x, V[0].p, st);
-    v_axpby<double>(1.0, b, -1.0, V[0].p, n, st);  // r = b - K x
-    double beta = std::sqrt(dot(V[0].p, V[0].p, n));
+    L.t->vmult(MFS_OP_STOKES, x, Vp(0), st);
+    v_axpby<double>(1.0, b, -1.0, Vp(0), n, st);  // r = b - K x
+    double beta = std::sqrt(dot(Vp(0), Vp(0), n));
     while (its < maxit && beta > tol) {
       std::vector<double> H((restart + 1) * restart, 0.0), cs(restart), sn(restart), g(restart + 1, 0.0);
       auto h = [&](int i, int k) -> double & { return H[i * restart + k]; };
-      v_axpby<double>(0.0, V[0].p, 1.0 / beta, V[0].p, n, st);
+      v_axpby<double>(0.0, Vp(0), 1.0 / beta, Vp(0), n, st);
       g[0] = beta;
       int k = 0;
       while (k < restart && its < maxit) {
-        prec(V[k].p, Z[k].p);
+        prec(Vp(k), Z[k].p);
         L.t->vmult(MFS_OP_STOKES, Z[k].p, w.p, st);
-        for (int i = 0; i <= k; ++i) {
-          h(i, k) = dot(w.p, V[i].p, n);
-          v_axpby<double>(-h(i, k), V[i].p, 1.0, w.p, n, st);
+        const double w0 = std::sqrt(dot(w.p, w.p, n));
+        cgs_pass(Vb.p, n, k + 1, w.p, hv);
+        double wn = std::sqrt(dot(w.p, w.p, n));
+        if (wn < 0.70710678118654752 * w0) {
+          cgs_pass(Vb.p, n, k + 1, w.p, hv2);
+          for (int i = 0; i <= k; ++i) hv[i] += hv2[i];
+          wn = std::sqrt(dot(w.p, w.p, n));
         }
-        h(k + 1, k) = std::sqrt(dot(w.p, w.p, n));
-        MFS_HIP(hipMemcpyAsync(V[k + 1].p, w.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (h(k + 1, k) > 0) v_axpby<double>(0.0, w.p, 1.0 / h(k + 1, k), V[k + 1].p, n, st);
+        for (int i = 0; i <= k; ++i) h(i, k) = hv[i];
+        h(k + 1, k) = wn;
+        MFS_HIP(hipMemcpyAsync(Vp(k + 1), w.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (h(k + 1, k) > 0) v_axpby<double>(0.0, w.p, 1.0 / h(k + 1, k), Vp(k + 1), n, st);
         for (int i = 0; i < k; ++i) {
           const double t = cs[i] * h(i, k) + sn[i] * h(i + 1, k);
           h(i + 1, k) = -sn[i] * h(i, k) + cs[i] * h(i + 1, k);
@@ -1111,9 +1174,9 @@ struct mfs_tria_gmg {
       beta = std::fabs(g[k]);
       if (its < maxit && beta > tol) {
         // restart from the true residual
-        L.t->vmult(MFS_OP_STOKES, x, V[0].p, st);
-        v_axpby<double>(1.0, b, -1.0, V[0].p, n, st);
-        beta = std::sqrt(dot(V[0].p, V[0].p, n));
+        L.t->vmult(MFS_OP_STOKES, x, Vp(0), st);
+        v_axpby<double>(1.0, b, -1.0, Vp(0), n, st);
+        beta = std::sqrt(dot(Vp(0), Vp(0), n));
       }
     }
     res = beta;
@@ -1154,6 +1217,16 @@ int mfs_tria_gmg_set_viscosity(mfs_tria_gmg *g, const double *eta_qp, int averag
   return tguard([&] {
     if (!g) throw std::invalid_argument("null mfs_tria_gmg");
     g->set_viscosity(eta_qp, averaging);
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_gmg_set_graph_capture(mfs_tria_gmg *g, int enable) {
+  return tguard([&] {
+    if (!g) throw std::invalid_argument("null mfs_tria_gmg");
+    MFS_HIP(hipStreamSynchronize(g->st));
+    g->drop_graphs();
+    g->graphs = enable != 0;
     return MFS_OK;
   });
 }

@@ -429,6 +429,9 @@ int mfs_tria_gmg_set_viscosity(mfs_tria_gmg *g, const double *eta_qp, int averag
  * level 0 degree 8, range 1e-3, 100 iterations.  The viscosity must be set on
  * every level (mfs_tria_set_viscosity). */
 int mfs_tria_gmg_build(mfs_tria_gmg *g);
+/* V-cycles captured once into hipGraphs and replayed (default on); 0 launches
+ * them kernel by kernel.  No counterpart in the reference. */
+int mfs_tria_gmg_set_graph_capture(mfs_tria_gmg *g, int enable);
 /* out = {lambda_min, lambda_max, eigenvalue iterations, colours} of level /
  * kind (0 A block, 1 Schur complement); diagonal (may be NULL): the level
  * operator's diagonal (host, velocity or pressure DoFs of the level). */

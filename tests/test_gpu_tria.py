@@ -241,3 +241,27 @@ def test_tria_level_viscosity_and_solve_match_oracle():
     x, out, rc = mg.solve(b, tolerance=1e-8, n_cheap_steps=500, restart=100)
     assert rc == 0 and abs(out.cheap_iterations - its_ref) <= max(2, 0.1 * its_ref)
     assert np.linalg.norm(b - top.stokes(x)) <= 1.5e-8 * np.linalg.norm(b)
+
+
+def test_tria_vcycle_graph_replays_match_eager():
+    """Captured V-cycles replay bit for bit like the eager launches, on
+    several replays with other work and synchronizations in between."""
+    meshes, pu, pp, etas, _ = gmg_case("shell2d")
+    levels = []
+    for m, e in zip(meshes, etas):
+        t = TriaOperators(m, pressure_scaling=1.0)
+        t.set_viscosity(e)
+        levels.append(t)
+    mg = TriaMultigrid(levels, pu, pp)
+    mg.build()
+    rng = np.random.default_rng(16)
+    xs = [rng.normal(size=levels[-1].nu) for _ in range(3)]
+    ps = [rng.normal(size=levels[-1].np) for _ in range(3)]
+    mg.set_graph_capture(False)
+    eager = [(mg.vcycle(0, x), mg.vcycle(1, p)) for x, p in zip(xs, ps)]
+    mg.set_graph_capture(True)
+    for rep in range(2):
+        for (x, p), (ex, ep) in zip(zip(xs, ps), eager):
+            assert np.array_equal(mg.vcycle(0, x), ex)
+            levels[-1].vmult("ablock", x)     # other work on the library between replays
+            assert np.array_equal(mg.vcycle(1, p), ep)
