@@ -108,6 +108,72 @@ __global__ void k_assemble(const long long n, const long long nn, const long lon
   dst[i] = a;
 }
 
+// Cell pressure mass matrices s^2/eta (psi_i, psi_j) of a degree-2 level (Q1
+// pressure, QGauss(3)), the 2^dim x 2^dim symmetric matrix of every cell as
+// its upper triangle, layout [(batch * NT + t) * 64 + c]; computed once per
+// viscosity so that the many mass applies of the Schur smoother read one
+// small matrix per cell instead of the geometry at every point
+template <int DIM>
+__global__ void k_tria_mass_matrix(const long long ncells, const double *__restrict__ geo,
+                                   const double *__restrict__ eta, const double s2, double *__restrict__ mat) {
+  constexpr int NP = 1 << DIM, NQ = DIM == 3 ? 27 : 9, G = DIM * DIM + 1, NT = NP * (NP + 1) / 2;
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const long long b = c / 64, cc = c - b * 64;
+  const double xg[3] = {0.5 - 0.5 * 0.7745966692414834, 0.5, 0.5 + 0.5 * 0.7745966692414834};
+  double m[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) m[t] = 0;
+  for (int q = 0; q < NQ; ++q) {
+    const int qx = q % 3, qy = (q / 3) % 3, qz = q / 9;
+    const double w = s2 * geo[((b * NQ + q) * G + G - 1) * 64 + cc] / eta[(b * NQ + q) * 64 + cc];
+    double psi[NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+      const double vx = (a & 1) ? xg[qx] : 1.0 - xg[qx];
+      const double vy = (a & 2) ? xg[qy] : 1.0 - xg[qy];
+      const double vz = DIM == 3 ? ((a & 4) ? xg[qz] : 1.0 - xg[qz]) : 1.0;
+      psi[a] = vx * vy * vz;
+    }
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+      for (int j = i; j < NP; ++j) m[t++] += w * psi[i] * psi[j];
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) mat[(b * NT + t) * 64 + cc] = m[t];
+}
+
+// per-cell pressure mass contributions from the stored cell matrices,
+// [(batch * NP + a) * 64 + c] as the cell kernel writes them
+template <int DIM>
+__global__ void k_tria_mass_apply(const long long ncells, const int *__restrict__ cell_p,
+                                  const double *__restrict__ mat, const double *__restrict__ x,
+                                  double *__restrict__ out) {
+  constexpr int NP = 1 << DIM, NT = NP * (NP + 1) / 2;
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const long long b = c / 64, cc = c - b * 64;
+  double p[NP], y[NP];
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    p[a] = x[cell_p[(b * NP + a) * 64 + cc]];
+    y[a] = 0;
+  }
+  int t = 0;
+#pragma unroll
+  for (int i = 0; i < NP; ++i)
+#pragma unroll
+    for (int j = i; j < NP; ++j) {
+      const double v = mat[(b * NT + t++) * 64 + cc];
+      y[i] += v * p[j];
+      if (j != i) y[j] += v * p[i];
+    }
+#pragma unroll
+  for (int a = 0; a < NP; ++a) out[(b * NP + a) * 64 + cc] = y[a];
+}
+
 // Constraint lines of one block on the device, with their transpose.
 struct DevCon {
   long long n = 0;
@@ -213,6 +279,23 @@ struct mfs_tria {
   DevCon con_u, con_p;
   std::vector<int> h_cell_u, h_cell_p;  // host copies of the cell lists (diagonal colouring)
   bool have_eta = false;
+  Buf<double> mass_mat;  // cell pressure mass matrices for the current viscosity
+  bool have_mass = false;
+
+  // the cell mass matrices of the current viscosity table (on stream s)
+  void update_mass(hipStream_t s) {
+    const int np = dim == 3 ? 8 : 4, nt = np * (np + 1) / 2;
+    if (!mass_mat.p) mass_mat.alloc(nbatch * nt * CT);
+    const double s2 = L.s * L.s;
+    if (dim == 3)
+      hipLaunchKernelGGL(k_tria_mass_matrix<3>, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, s, ncells,
+                         geo.p, eta.p, s2, mass_mat.p);
+    else
+      hipLaunchKernelGGL(k_tria_mass_matrix<2>, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, s, ncells,
+                         geo.p, eta.p, s2, mass_mat.p);
+    MFS_HIP(hipGetLastError());
+    have_mass = true;
+  }
 
   ~mfs_tria() {
     if (st) (void)hipStreamDestroy(st);
@@ -325,6 +408,7 @@ struct mfs_tria {
         be[((c / CT) * nq + q) * CT + c % CT] = v;
       }
     MFS_HIP(hipMemcpyAsync(eta.p, be.data(), be.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    update_mass(st);
     MFS_HIP(hipStreamSynchronize(st));
     have_eta = true;
   }
@@ -358,7 +442,16 @@ struct mfs_tria {
     double *dp = op == MFS_OP_STOKES ? dst + ndof_u : (wr_p ? dst : nullptr);
     if (rd_u) resolve(su, xu.p, con_u, s);
     if (rd_p) resolve(sp, xp.p, con_p, s);
-    launch_cells<double>((Op)op, L, nbatch, rd_u ? xu.p : nullptr, rd_p ? xp.p : nullptr, eta.p, cbu.p, cbp.p, s);
+    if (op == MFS_OP_MASS && have_mass) {
+      const unsigned nb = (unsigned)((ncells + 255) / 256);
+      if (dim == 3)
+        hipLaunchKernelGGL(k_tria_mass_apply<3>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, xp.p, cbp.p);
+      else
+        hipLaunchKernelGGL(k_tria_mass_apply<2>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, xp.p, cbp.p);
+      MFS_HIP(hipGetLastError());
+    } else {
+      launch_cells<double>((Op)op, L, nbatch, rd_u ? xu.p : nullptr, rd_p ? xp.p : nullptr, eta.p, cbu.p, cbp.p, s);
+    }
     if (wr_u) assemble(cbu.p, nn_u, (long long)nv * CT, iptr_u, ioff_u, con_u, su, du, op == MFS_OP_BT ? 0 : 1, s);
     if (wr_p) assemble(cbp.p, nn_p, 0, iptr_p, ioff_p, con_p, sp, dp, 1, s);
   }
@@ -827,6 +920,7 @@ struct mfs_tria_gmg {
       hipLaunchKernelGGL(k_tria_fill_eta, dim3(nblk(n * C.t->nq)), dim3(256), 0, st, n, C.t->nq, C.cellval.p,
                          C.t->eta.p);
     }
+    for (auto &L : lv) L.t->update_mass(st);
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipStreamSynchronize(st));
     for (auto &L : lv) L.t->have_eta = true;
