@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaspect_mfs.so")
-SOURCES = ["kernels.hip", "kernels_gather.hip", "kernels_planes.hip", "handler.hip", "tria.hip"]
+SOURCES = ["kernels.hip", "kernels_gather.hip", "kernels_planes.hip", "handler.hip", "tria.hip", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

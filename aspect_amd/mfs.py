@@ -41,7 +41,7 @@ EXPORTED = [
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
     "mfs_box_dof_index", "mfs_tria_create", "mfs_tria_destroy", "mfs_tria_sizes", "mfs_tria_set_viscosity",
     "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_time", "mfs_tria_get_viscosity", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_set_viscosity", "mfs_tria_gmg_build",
-    "mfs_tria_gmg_set_graph_capture",
+    "mfs_tria_gmg_set_graph_capture", "mfs_probe_contraction",
     "mfs_tria_gmg_info", "mfs_tria_gmg_vcycle", "mfs_tria_solve",
 ]
 

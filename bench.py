@@ -103,6 +103,34 @@ def fgmres_profile(cfg, eta, rho, s, its):
     return 0
 
 
+def contraction_probe():
+    """FP32 3-point basis contractions (the sum-factorization kernel's inner
+    operation) as VALU FMAs and on the matrix cores (v_mfma_f32_16x16x4f32
+    with the 3 x 3 table packed block-diagonally into 16 x 16 tiles): useful
+    FMA rates, and the cross-check of the two results."""
+    import ctypes as C
+    from aspect_amd.mfs import _check, load_library
+    lib = load_library()
+    n, reps = 80 * 65536, 256
+    rec = {}
+    outs = []
+    for v, name in ((0, "valu_fma"), (1, "mfma_16x16x4f32")):
+        sec = C.c_double(0)
+        out = np.empty(3 * n)
+        _check(lib.mfs_probe_contraction(v, C.c_longlong(n), reps, C.byref(sec),
+                                         out.ctypes.data_as(C.c_void_p)))
+        contractions = n * reps * (1 if v == 0 else 4)
+        rec[name] = {"seconds": sec.value, "useful_tflops": 2 * 9 * contractions / sec.value / 1e12}
+        outs.append(out)
+    i = np.arange(3 * n, dtype=np.int64)
+    x = (((i * 2654435761) % 1000).astype(np.float32) * np.float32(1e-3) - np.float32(0.5)).astype(float)
+    sx = outs[0] - x
+    rec["max_rel_diff_mfma_vs_fma"] = float(np.max(np.abs(outs[1] - 4 * sx)) / np.max(np.abs(4 * sx)))
+    rec["lines"], rec["reps"] = n, reps
+    print(json.dumps({"contraction_probe": rec}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +139,8 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--contraction-probe", action="store_true",
+                    help="time the 3-point basis contraction as VALU FMAs and on the matrix cores, then exit")
     ap.add_argument("--no-tria", action="store_true",
                     help="skip the general-mesh (spherical shell with hanging nodes) apply and solve")
     ap.add_argument("--shell-refinement", type=int, default=3,
@@ -135,6 +165,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8, "copies": 3}))
         return 0
+    if args.contraction_probe:
+        return contraction_probe()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))

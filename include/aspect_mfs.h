@@ -451,6 +451,16 @@ int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *ds
 int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_solve_params *prm,
                    mfs_solve_result *res);
 
+/* Measurement of the matrix-core alternative for the sum-factorization
+ * contractions (no counterpart in the reference): FP32 3-point basis
+ * contractions y = S x of n_lines lines (a positive multiple of 80), each
+ * repeated reps times, variant 0 as VALU FMAs (one line per thread), variant 1
+ * on the matrix cores (v_mfma_f32_16x16x4f32, block-diagonal S packed into
+ * 16 x 16 tiles: 4 products of 80 lines per repetition and wave).  *seconds:
+ * device time of one launch; out (may be NULL, 3 n_lines floats as doubles):
+ * the result of one repetition (variant 0: S x + x, variant 1: 4 S x). */
+int mfs_probe_contraction(int variant, long long n_lines, int reps, double *seconds, double *out);
+
 #ifdef __cplusplus
 }
 #endif
