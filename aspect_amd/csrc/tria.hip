@@ -428,7 +428,9 @@ struct mfs_tria {
 
   // device vectors: op MFS_OP_STOKES [u|p] -> [u|p], ABLOCK u -> u, BT p -> u, MASS p -> p
   void vmult(int op, const double *src, double *dst) { vmult(op, src, dst, st); }
-  void vmult(int op, const double *src, double *dst, hipStream_t s) {
+  // plain: source values read as stored (no constraint resolution) and
+  // constrained rows left zero (correct_stokes_rhs)
+  void vmult(int op, const double *src, double *dst, hipStream_t s, bool plain = false) {
     if (op != MFS_OP_STOKES && op != MFS_OP_ABLOCK && op != MFS_OP_BT && op != MFS_OP_MASS)
       throw std::invalid_argument("mfs_tria_vmult: operator must be Stokes, A block, B^T or mass");
     if (!src || !dst) throw std::invalid_argument("mfs_tria_vmult: null vector");
@@ -440,8 +442,13 @@ struct mfs_tria {
     const double *sp = op == MFS_OP_STOKES ? src + ndof_u : (rd_p ? src : nullptr);
     double *du = wr_u ? dst : nullptr;
     double *dp = op == MFS_OP_STOKES ? dst + ndof_u : (wr_p ? dst : nullptr);
-    if (rd_u) resolve(su, xu.p, con_u, s);
-    if (rd_p) resolve(sp, xp.p, con_p, s);
+    if (plain) {
+      if (rd_u) MFS_HIP(hipMemcpyAsync(xu.p, su, ndof_u * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if (rd_p) MFS_HIP(hipMemcpyAsync(xp.p, sp, ndof_p * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+      if (rd_u) resolve(su, xu.p, con_u, s);
+      if (rd_p) resolve(sp, xp.p, con_p, s);
+    }
     if (op == MFS_OP_MASS && have_mass) {
       const unsigned nb = (unsigned)((ncells + 255) / 256);
       if (dim == 3)
@@ -452,8 +459,9 @@ struct mfs_tria {
     } else {
       launch_cells<double>((Op)op, L, nbatch, rd_u ? xu.p : nullptr, rd_p ? xp.p : nullptr, eta.p, cbu.p, cbp.p, s);
     }
-    if (wr_u) assemble(cbu.p, nn_u, (long long)nv * CT, iptr_u, ioff_u, con_u, su, du, op == MFS_OP_BT ? 0 : 1, s);
-    if (wr_p) assemble(cbp.p, nn_p, 0, iptr_p, ioff_p, con_p, sp, dp, 1, s);
+    if (wr_u)
+      assemble(cbu.p, nn_u, (long long)nv * CT, iptr_u, ioff_u, con_u, su, du, (op == MFS_OP_BT || plain) ? 0 : 1, s);
+    if (wr_p) assemble(cbp.p, nn_p, 0, iptr_p, ioff_p, con_p, sp, dp, plain ? 0 : 1, s);
   }
 
   // Greedy colouring of the DoFs of one block (kind 0 velocity, 1 pressure)
@@ -587,6 +595,24 @@ int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst) {
     MFS_HIP(hipMemcpyAsync(t->io_src.p, src, ni * sizeof(double), hipMemcpyHostToDevice, t->st));
     t->vmult(op, t->io_src.p, t->io_dst.p);
     MFS_HIP(hipMemcpyAsync(dst, t->io_dst.p, no * sizeof(double), hipMemcpyDeviceToHost, t->st));
+    MFS_HIP(hipStreamSynchronize(t->st));
+    return MFS_OK;
+  });
+}
+
+int mfs_tria_correct_stokes_rhs(mfs_tria *t, const double *u0, double *rhs) {
+  return tguard([&] {
+    need(t);
+    if (!u0 || !rhs) throw std::invalid_argument("mfs_tria_correct_stokes_rhs: null vector");
+    if (!t->have_eta) throw std::runtime_error("viscosity not set (call mfs_tria_set_viscosity)");
+    const long long n = t->ndof_u + t->ndof_p;
+    Buf<double> r;
+    r.alloc(n);
+    MFS_HIP(hipMemcpyAsync(t->io_src.p, u0, n * sizeof(double), hipMemcpyHostToDevice, t->st));
+    MFS_HIP(hipMemcpyAsync(r.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, t->st));
+    t->vmult(MFS_OP_STOKES, t->io_src.p, t->io_dst.p, t->st, true);
+    v_axpby<double>(-1.0, t->io_dst.p, 1.0, r.p, n, t->st);  // rhs -= K_plain u0
+    MFS_HIP(hipMemcpyAsync(rhs, r.p, n * sizeof(double), hipMemcpyDeviceToHost, t->st));
     MFS_HIP(hipStreamSynchronize(t->st));
     return MFS_OK;
   });

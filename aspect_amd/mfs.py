@@ -40,7 +40,7 @@ EXPORTED = [
     "mfs_validate_operator_arguments", "mfs_ablock_vmult_device_noalloc", "mfs_set_free_surface",
     "mfs_mesh_deformation_solve", "mfs_mesh_laplace_vmult", "mfs_mesh_vcycle", "mfs_set_geometry",
     "mfs_box_dof_index", "mfs_tria_create", "mfs_tria_destroy", "mfs_tria_sizes", "mfs_tria_set_viscosity",
-    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_time", "mfs_tria_get_viscosity", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_set_viscosity", "mfs_tria_gmg_build",
+    "mfs_tria_vmult", "mfs_tria_vmult_device", "mfs_tria_time", "mfs_tria_get_viscosity", "mfs_tria_correct_stokes_rhs", "mfs_tria_gmg_create", "mfs_tria_gmg_destroy", "mfs_tria_gmg_set_viscosity", "mfs_tria_gmg_build",
     "mfs_tria_gmg_set_graph_capture", "mfs_probe_contraction",
     "mfs_tria_gmg_info", "mfs_tria_gmg_vcycle", "mfs_tria_solve",
 ]
@@ -641,6 +641,14 @@ class TriaOperators:
         if e.size != self.n_cells * 3 ** self.dim:
             raise ValueError("one viscosity per cell and quadrature point expected")
         _check(self.lib.mfs_tria_set_viscosity(self._t, _ptr(e)))
+
+    def correct_stokes_rhs(self, u0, rhs):
+        """rhs - K u0 with u0 read as stored (prescribed values on constrained
+        DoFs) and nothing added to constrained rows."""
+        u = _vec(u0, self.nu + self.np, "u0")
+        r = _vec(rhs, self.nu + self.np, "rhs").copy()
+        _check(self.lib.mfs_tria_correct_stokes_rhs(self._t, _ptr(u), _ptr(r)))
+        return r
 
     def viscosity(self):
         """The object's viscosity table [cell][q]."""

@@ -388,6 +388,13 @@ int mfs_tria_set_viscosity(mfs_tria *t, const double *eta_qp);
 int mfs_tria_vmult(mfs_tria *t, int op, const double *src, double *dst);
 /* The same on device vectors (hipMalloc'd by the caller). */
 int mfs_tria_vmult_device(mfs_tria *t, int op, const double *d_src, double *d_dst);
+/* correct_stokes_rhs on a general level (local_smoothing.cc:617-754): rhs -= K u0
+ * with u0's entries read as stored (read_dof_values_plain: u0 carries the
+ * prescribed values of the constrained DoFs, e.g. after distribute() with the
+ * inhomogeneous constraints) and the contributions to constrained DoFs
+ * distributed to their masters or dropped (constrained rows receive nothing).
+ * Host vectors [u|p]. */
+int mfs_tria_correct_stokes_rhs(mfs_tria *t, const double *u0, double *rhs);
 /* The viscosity table of the object, [cell][q] (host). */
 int mfs_tria_get_viscosity(mfs_tria *t, double *eta_qp);
 /* Average device time of reps applies on device vectors, bracketed by hipEvents

@@ -144,6 +144,17 @@ class Operators:
         return np.concatenate([yu, yp])
 
 
+def stokes_rhs_correction(ops, u0):
+    """correct_stokes_rhs (stokes_matrix_free_local_smoothing.cc:617-754) on a
+    general level: -C^T K u0 with u0 read as stored, constrained rows zero."""
+    u, p = u0[:ops.nu], u0[ops.nu:]
+    yu = -(ops.Cu.T @ (ops.A @ u + ops.B.T @ p))
+    yp = -(ops.Cp.T @ (ops.B @ u))
+    yu[ops.fixed_u] = 0.0
+    yp[ops.fixed_p] = 0.0
+    return np.concatenate([yu, yp])
+
+
 def constraint_map(con, n):
     """C (n x n): free DoF -> itself, constrained DoF -> its masters' weights;
     and the list of constrained DoFs."""

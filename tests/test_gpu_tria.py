@@ -265,3 +265,19 @@ def test_tria_vcycle_graph_replays_match_eager():
             assert np.array_equal(mg.vcycle(0, x), ex)
             levels[-1].vmult("ablock", x)     # other work on the library between replays
             assert np.array_equal(mg.vcycle(1, p), ep)
+
+
+@pytest.mark.parametrize("case", ["box3d", "shell3d"])
+def test_tria_rhs_correction_matches_oracle(case):
+    """correct_stokes_rhs on general levels: u0 read as stored (values on the
+    constrained DoFs included), constrained rows receive nothing."""
+    mesh, s, comp = CASES[case]()
+    eta = viscosity(mesh, 17)
+    ref = OT.Operators(mesh, eta, pressure_scaling=s, compressible=comp)
+    gpu = TriaOperators(mesh, pressure_scaling=s, compressible=comp)
+    gpu.set_viscosity(eta)
+    rng = np.random.default_rng(18)
+    u0 = rng.normal(size=mesh.n_dofs_u + mesh.n_dofs_p)
+    rhs = rng.normal(size=u0.size)
+    out = gpu.correct_stokes_rhs(u0, rhs)
+    assert rel(out - rhs, OT.stokes_rhs_correction(ref, u0)) <= 1e-12

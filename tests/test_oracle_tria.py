@@ -539,3 +539,61 @@ def test_level_viscosities_of_a_linear_field(dim):
     # harmonic averaging of a cellwise constant table leaves it unchanged
     cst = np.repeat(np.exp(np.random.default_rng(32).normal(size=(meshes[-1].n_cells, 1))), 3 ** dim, axis=1)
     assert np.allclose(OT.level_viscosities(meshes, parents, cst, 2)[-1], cst, rtol=1e-14)
+
+
+def test_rhs_correction_recovers_prescribed_translation():
+    """Prescribed boundary velocity through the right-hand side: with u0 = a
+    constant velocity on the zero-velocity boundary DoFs (prescribed values)
+    and zero elsewhere, solving the homogeneous constrained system with
+    rhs = correction(u0) gives x with x + u0 = the constant field (a
+    translation with zero pressure solves the Stokes equations exactly)."""
+    from scipy.sparse.linalg import spsolve
+    t = T.Tria(T.BoxCoarse(3, (3, 2, 2), (1.5, 1.0, 1.2)), 1)
+    c = t.cell_centers()
+    t.refine((np.abs(c[:, 0] - 0.75) < 0.3) & (np.abs(c[:, 1] - 0.5) < 0.3) & (np.abs(c[:, 2] - 0.6) < 0.3))
+    mesh = t.build({f: "zero" for f in range(6)})
+    assert len(mesh.hanging_u) > 0
+    ops = OT.Operators(mesh, np.exp(np.random.default_rng(33).normal(size=(mesh.n_cells, 27))),
+                       pressure_scaling=1.4)
+    vel = np.array([0.3, -1.1, 0.7])
+    nn = mesh.n_nodes_u
+    bnd = np.unique(np.concatenate([mesh.boundary_nodes[f] for f in range(6)]))
+    u0 = np.zeros(ops.nu + ops.np)
+    for cc in range(3):
+        u0[cc * nn + bnd] = vel[cc]
+    # hanging nodes lie inside (their masters are free), so u0 is consistent there
+    rhs = OT.stokes_rhs_correction(ops, u0)
+    K = stokes_matrix(ops).tolil()
+    # enclosed flow (pressure up to a constant; the right-hand side is
+    # consistent): pin one free pressure DoF to zero
+    fp = np.ones(ops.np, bool)
+    fp[mesh.con_p[0]] = False
+    j = ops.nu + int(np.nonzero(fp)[0][0])
+    K[j, :] = 0.0
+    K[:, j] = 0.0
+    K[j, j] = 1.0
+    rhs[j] = 0.0
+    x = spsolve(K.tocsc(), rhs)
+    u = x[:ops.nu] + u0[:ops.nu]
+    ur = OT.resolve(mesh.con_u, x[:ops.nu]) + u0[:ops.nu]
+    free = np.ones(ops.nu, bool)
+    free[mesh.con_u[0]] = False
+    ref = np.concatenate([np.full(nn, v) for v in vel])
+    assert np.max(np.abs(u[free] - ref[free])) <= 1e-8
+    assert np.max(np.abs(ur[mesh.hanging_u] - ref[mesh.hanging_u])) <= 1e-8
+    p = x[ops.nu:]
+    assert np.max(np.abs(p[fp])) <= 1e-8
+
+
+def test_rhs_correction_of_a_consistent_vector_is_the_operator():
+    """u0 already satisfying the (homogeneous) lines: the plain read equals the
+    constrained read, so the correction is -K u0 on the free rows."""
+    mesh, _ = refined_box(3, 34)
+    ops = OT.Operators(mesh, np.exp(np.random.default_rng(35).normal(size=(mesh.n_cells, 27))))
+    rng = np.random.default_rng(36)
+    x = rng.normal(size=ops.nu + ops.np)
+    u0 = np.concatenate([OT.resolve(mesh.con_u, x[:ops.nu]), OT.resolve(mesh.con_p, x[ops.nu:])])
+    y = -ops.stokes(u0)
+    y[mesh.con_u[0]] = 0.0
+    y[ops.nu + mesh.con_p[0]] = 0.0
+    assert np.max(np.abs(OT.stokes_rhs_correction(ops, u0) - y)) <= 1e-12 * np.max(np.abs(y))
