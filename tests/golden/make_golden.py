@@ -9,7 +9,13 @@ oracle solve takes minutes (run once on the CPU, outputs committed):
                          cells) at their real sizes: per problem the FP64-GMG
                          iteration count and 400 sampled solution entries
 
-    python tests/golden/make_golden.py [config3] [ensemble]
+  hanging2_{2,3}d.npz    two cells, one of them refined once (the smallest mesh
+                         with hanging nodes), zero velocity on the x- face and
+                         no normal flux on the others: cell lists, geometry,
+                         closed constraint lines, viscosity, a source vector
+                         and the oracle's Stokes, A, B^T and mass applies
+
+    python tests/golden/make_golden.py [config3] [ensemble] [hanging]
 """
 import os
 import sys
@@ -48,6 +54,29 @@ def config3_32():
     print(f"config3 32^3: {res['cheap_iterations']} iterations, {time.time() - t:.0f} s", flush=True)
 
 
+def hanging2():
+    from aspect_amd import tria as T
+    from oracle import tria as OT
+    for dim in (2, 3):
+        t = T.Tria(T.BoxCoarse(dim, (2, 1, 1)[:dim], (2.0, 1.0, 1.0)[:dim]), 0)
+        t.refine(np.array([True, False]))
+        bc = {0: "zero"}
+        bc.update({f: "tangential" for f in range(1, 2 * dim)})
+        m = t.build(bc)
+        rng = np.random.default_rng(200 + dim)
+        eta = np.exp(rng.normal(size=(m.n_cells, 3 ** dim)))
+        ops = OT.Operators(m, eta, pressure_scaling=1.25)
+        x = rng.normal(size=ops.nu + ops.np)
+        np.savez_compressed(os.path.join(HERE, f"hanging2_{dim}d.npz"), cell_u=m.cell_u, cell_p=m.cell_p,
+                            n_nodes_u=m.n_nodes_u, n_nodes_p=m.n_nodes_p, geometry=m.geometry,
+                            con_u_dofs=m.con_u[0], con_u_offsets=m.con_u[1], con_u_masters=m.con_u[2],
+                            con_u_weights=m.con_u[3], con_p_dofs=m.con_p[0], con_p_offsets=m.con_p[1],
+                            con_p_masters=m.con_p[2], con_p_weights=m.con_p[3], eta=eta, x=x,
+                            stokes=ops.stokes(x), ablock=ops.ablock(x[:ops.nu]), bt=ops.bt(x[ops.nu:]),
+                            mass=ops.mass(x[ops.nu:]), pressure_scaling=1.25)
+        print(f"hanging2 {dim}D: {m.n_cells} cells, {len(m.hanging_u)} hanging velocity nodes", flush=True)
+
+
 def ensemble48():
     cfgs = P.config_ensemble(48, seed=4)
     rng = np.random.default_rng(48)
@@ -71,7 +100,9 @@ def ensemble48():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["config3", "ensemble"]
+    which = sys.argv[1:] or ["config3", "ensemble", "hanging"]
+    if "hanging" in which:
+        hanging2()
     if "ensemble" in which:
         ensemble48()
     if "config3" in which:

@@ -281,3 +281,25 @@ def test_tria_rhs_correction_matches_oracle(case):
     rhs = rng.normal(size=u0.size)
     out = gpu.correct_stokes_rhs(u0, rhs)
     assert rel(out - rhs, OT.stokes_rhs_correction(ref, u0)) <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tria_hanging2_golden_fixture(dim):
+    """The library on the committed two-cell hanging-node fixture's own arrays
+    (cell lists, geometry, constraint lines) against its stored oracle
+    outputs."""
+    import os
+    from types import SimpleNamespace
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"hanging2_{dim}d.npz"))
+    mesh = SimpleNamespace(dim=dim, cell_u=g["cell_u"], cell_p=g["cell_p"], n_cells=len(g["cell_u"]),
+                           n_nodes_u=int(g["n_nodes_u"]), n_nodes_p=int(g["n_nodes_p"]), geometry=g["geometry"],
+                           con_u=(g["con_u_dofs"], g["con_u_offsets"], g["con_u_masters"], g["con_u_weights"]),
+                           con_p=(g["con_p_dofs"], g["con_p_offsets"], g["con_p_masters"], g["con_p_weights"]))
+    gpu = TriaOperators(mesh, pressure_scaling=float(g["pressure_scaling"]))
+    gpu.set_viscosity(g["eta"])
+    x = g["x"]
+    nu = dim * mesh.n_nodes_u
+    assert rel(gpu.vmult("stokes", x), g["stokes"]) <= 1e-12
+    assert rel(gpu.vmult("ablock", x[:nu]), g["ablock"]) <= 1e-12
+    assert rel(gpu.vmult("bt", x[nu:]), g["bt"]) <= 1e-12
+    assert rel(gpu.vmult("mass", x[nu:]), g["mass"]) <= 1e-12

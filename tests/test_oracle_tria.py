@@ -597,3 +597,22 @@ def test_rhs_correction_of_a_consistent_vector_is_the_operator():
     y[mesh.con_u[0]] = 0.0
     y[ops.nu + mesh.con_p[0]] = 0.0
     assert np.max(np.abs(OT.stokes_rhs_correction(ops, u0) - y)) <= 1e-12 * np.max(np.abs(y))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hanging2_golden_fixture(dim):
+    """The committed two-cell hanging-node fixture (tests/golden/make_golden.py)
+    is reproduced by the mesh builder and the oracle."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"hanging2_{dim}d.npz"))
+    t = T.Tria(T.BoxCoarse(dim, (2, 1, 1)[:dim], (2.0, 1.0, 1.0)[:dim]), 0)
+    t.refine(np.array([True, False]))
+    bc = {0: "zero"}
+    bc.update({f: "tangential" for f in range(1, 2 * dim)})
+    m = t.build(bc)
+    assert np.array_equal(m.cell_u, g["cell_u"]) and np.array_equal(m.cell_p, g["cell_p"])
+    assert np.array_equal(m.con_u[0], g["con_u_dofs"]) and np.array_equal(m.con_u[2], g["con_u_masters"])
+    assert np.allclose(m.con_u[3], g["con_u_weights"], rtol=0, atol=1e-15)
+    assert np.allclose(m.geometry, g["geometry"], rtol=1e-15, atol=1e-15)
+    ops = OT.Operators(m, g["eta"], pressure_scaling=float(g["pressure_scaling"]))
+    assert np.max(np.abs(ops.stokes(g["x"]) - g["stokes"])) <= 1e-13 * np.max(np.abs(g["stokes"]))
