@@ -88,24 +88,47 @@ __device__ __forceinline__ double node_sum(const double *__restrict__ cb, const 
 // cs in the contribution buffer): free DoFs their own sum plus the weighted
 // sums of the constrained DoFs they are a master of; constrained DoFs
 // src[i] (ident) or 0
+// Fused modes of the assemble pass (the smoother's steps on a level):
+// AS_RESID dst = b - K src; AS_CHEB one Jacobi-Chebyshev step on x = src (in
+// place) with d: r = b - K x, d = first ? dinv r / theta : f1 d + f2 dinv r,
+// x += d (the arithmetic of v_cheb_first / v_cheb_step)
+enum { AS_SET = 0, AS_RESID = 1, AS_CHEB = 2 };
+struct ChebStep {
+  const double *b, *dinv;
+  double *d;
+  double f1, f2;  // first step: f2 = 1 / theta
+  int first;
+};
+template <int MODE>
 __global__ void k_assemble(const long long n, const long long nn, const long long cs, const double *__restrict__ cb,
                            const long long *__restrict__ iptr, const int *__restrict__ ioff,
                            const int *__restrict__ row, const long long *__restrict__ tptr,
                            const long long *__restrict__ tdof, const double *__restrict__ tw,
-                           const double *__restrict__ src, double *__restrict__ dst, const int ident) {
+                           const double *__restrict__ src, double *__restrict__ dst, const int ident,
+                           const ChebStep ch) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  double a;
   if (row[i] >= 0) {
-    dst[i] = ident ? src[i] : 0.0;
-    return;
+    a = ident ? src[i] : 0.0;
+  } else {
+    const long long comp = i / nn, node = i - comp * nn;
+    a = node_sum(cb, iptr, ioff, node, comp * cs);
+    for (long long k = tptr[i]; k < tptr[i + 1]; ++k) {
+      const long long c = tdof[k], cc = c / nn;
+      a += tw[k] * node_sum(cb, iptr, ioff, c - cc * nn, cc * cs);
+    }
   }
-  const long long comp = i / nn, node = i - comp * nn;
-  double a = node_sum(cb, iptr, ioff, node, comp * cs);
-  for (long long k = tptr[i]; k < tptr[i + 1]; ++k) {
-    const long long c = tdof[k], cc = c / nn;
-    a += tw[k] * node_sum(cb, iptr, ioff, c - cc * nn, cc * cs);
+  if constexpr (MODE == AS_SET) {
+    dst[i] = a;
+  } else if constexpr (MODE == AS_RESID) {
+    dst[i] = 1.0 * ch.b[i] + -1.0 * a;
+  } else {
+    const double r = 1.0 * ch.b[i] + -1.0 * a;
+    const double dd = ch.first ? ch.dinv[i] * r * ch.f2 : ch.f1 * ch.d[i] + ch.f2 * ch.dinv[i] * r;
+    ch.d[i] = dd;
+    dst[i] = src[i] + dd;
   }
-  dst[i] = a;
 }
 
 // Cell pressure mass matrices s^2/eta (psi_i, psi_j) of a degree-2 level (Q1
@@ -147,9 +170,12 @@ __global__ void k_tria_mass_matrix(const long long ncells, const double *__restr
 
 // per-cell pressure mass contributions from the stored cell matrices,
 // [(batch * NP + a) * 64 + c] as the cell kernel writes them
+// (constrained pressure DoFs read through their lines: row / off / mas / w)
 template <int DIM>
 __global__ void k_tria_mass_apply(const long long ncells, const int *__restrict__ cell_p,
                                   const double *__restrict__ mat, const double *__restrict__ x,
+                                  const int *__restrict__ row, const long long *__restrict__ off,
+                                  const long long *__restrict__ mas, const double *__restrict__ w,
                                   double *__restrict__ out) {
   constexpr int NP = 1 << DIM, NT = NP * (NP + 1) / 2;
   const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -158,7 +184,16 @@ __global__ void k_tria_mass_apply(const long long ncells, const int *__restrict_
   double p[NP], y[NP];
 #pragma unroll
   for (int a = 0; a < NP; ++a) {
-    p[a] = x[cell_p[(b * NP + a) * 64 + cc]];
+    const int node = cell_p[(b * NP + a) * 64 + cc];
+    const int r = row[node];
+    double v;
+    if (r < 0) {
+      v = x[node];
+    } else {
+      v = 0;
+      for (long long k = off[r]; k < off[r + 1]; ++k) v += w[k] * x[mas[k]];
+    }
+    p[a] = v;
     y[a] = 0;
   }
   int t = 0;
@@ -419,11 +454,41 @@ struct mfs_tria {
     MFS_HIP(hipGetLastError());
   }
   void assemble(const double *cb, long long nn, long long cs, const Buf<long long> &ip, const Buf<int> &io,
-                DevCon &c, const double *src, double *dst, int ident, hipStream_t s) {
+                DevCon &c, const double *src, double *dst, int ident, hipStream_t s, int mode = AS_SET,
+                const ChebStep &ch = ChebStep{}) {
     const unsigned nb = (unsigned)((c.n + 255) / 256);
-    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(256), 0, s, c.n, nn, cs, cb, ip.p, io.p, c.row.p, c.tptr.p,
-                       c.tdof.p, c.tw.p, src, dst, ident);
+    if (mode == AS_SET)
+      hipLaunchKernelGGL(k_assemble<AS_SET>, dim3(nb), dim3(256), 0, s, c.n, nn, cs, cb, ip.p, io.p, c.row.p,
+                         c.tptr.p, c.tdof.p, c.tw.p, src, dst, ident, ch);
+    else if (mode == AS_RESID)
+      hipLaunchKernelGGL(k_assemble<AS_RESID>, dim3(nb), dim3(256), 0, s, c.n, nn, cs, cb, ip.p, io.p, c.row.p,
+                         c.tptr.p, c.tdof.p, c.tw.p, src, dst, ident, ch);
+    else
+      hipLaunchKernelGGL(k_assemble<AS_CHEB>, dim3(nb), dim3(256), 0, s, c.n, nn, cs, cb, ip.p, io.p, c.row.p,
+                         c.tptr.p, c.tdof.p, c.tw.p, src, dst, ident, ch);
     MFS_HIP(hipGetLastError());
+  }
+
+  // level operator of the smoother (kind 0: A block, 1: pressure mass) on x,
+  // its result assembled in mode `mode` (AS_SET: dst = K x; AS_RESID: dst =
+  // ch.b - K x; AS_CHEB: a Jacobi-Chebyshev step on x in place, dst = x)
+  void level_op(int kind, const double *x, double *dst, int mode, const ChebStep &ch, hipStream_t s) {
+    if (kind == 0) {
+      resolve(x, xu.p, con_u, s);
+      launch_cells<double>(OP_ABLOCK, L, nbatch, xu.p, nullptr, eta.p, cbu.p, cbp.p, s);
+      assemble(cbu.p, nn_u, (long long)nv * CT, iptr_u, ioff_u, con_u, x, dst, 1, s, mode, ch);
+      return;
+    }
+    if (!have_mass) throw std::runtime_error("pressure mass matrices not set");
+    const unsigned nb = (unsigned)((ncells + 255) / 256);
+    if (dim == 3)
+      hipLaunchKernelGGL(k_tria_mass_apply<3>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, x,
+                         con_p.row.p, con_p.off.p, con_p.mas.p, con_p.w.p, cbp.p);
+    else
+      hipLaunchKernelGGL(k_tria_mass_apply<2>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, x,
+                         con_p.row.p, con_p.off.p, con_p.mas.p, con_p.w.p, cbp.p);
+    MFS_HIP(hipGetLastError());
+    assemble(cbp.p, nn_p, 0, iptr_p, ioff_p, con_p, x, dst, 1, s, mode, ch);
   }
 
   // device vectors: op MFS_OP_STOKES [u|p] -> [u|p], ABLOCK u -> u, BT p -> u, MASS p -> p
@@ -442,19 +507,22 @@ struct mfs_tria {
     const double *sp = op == MFS_OP_STOKES ? src + ndof_u : (rd_p ? src : nullptr);
     double *du = wr_u ? dst : nullptr;
     double *dp = op == MFS_OP_STOKES ? dst + ndof_u : (wr_p ? dst : nullptr);
+    const bool mass_fused = op == MFS_OP_MASS && have_mass && !plain;  // resolve inside the mass kernel
     if (plain) {
       if (rd_u) MFS_HIP(hipMemcpyAsync(xu.p, su, ndof_u * sizeof(double), hipMemcpyDeviceToDevice, s));
       if (rd_p) MFS_HIP(hipMemcpyAsync(xp.p, sp, ndof_p * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
       if (rd_u) resolve(su, xu.p, con_u, s);
-      if (rd_p) resolve(sp, xp.p, con_p, s);
+      if (rd_p && !mass_fused) resolve(sp, xp.p, con_p, s);
     }
-    if (op == MFS_OP_MASS && have_mass) {
+    if (op == MFS_OP_MASS && have_mass && !plain) {
       const unsigned nb = (unsigned)((ncells + 255) / 256);
       if (dim == 3)
-        hipLaunchKernelGGL(k_tria_mass_apply<3>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, xp.p, cbp.p);
+        hipLaunchKernelGGL(k_tria_mass_apply<3>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, sp,
+                           con_p.row.p, con_p.off.p, con_p.mas.p, con_p.w.p, cbp.p);
       else
-        hipLaunchKernelGGL(k_tria_mass_apply<2>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, xp.p, cbp.p);
+        hipLaunchKernelGGL(k_tria_mass_apply<2>, dim3(nb), dim3(256), 0, s, ncells, cell_p.p, mass_mat.p, sp,
+                           con_p.row.p, con_p.off.p, con_p.mas.p, con_p.w.p, cbp.p);
       MFS_HIP(hipGetLastError());
     } else {
       launch_cells<double>((Op)op, L, nbatch, rd_u ? xu.p : nullptr, rd_p ? xp.p : nullptr, eta.p, cbu.p, cbp.p, s);
@@ -1075,24 +1143,23 @@ struct mfs_tria_gmg {
     const Cheb &ch = L.ch[kind];
     const long long n = L.n[kind];
     double *dinv = L.dinv[kind].p, *r = L.r[kind].p, *d = L.d[kind].p;
-    auto resid = [&] {
-      op(kind, l, x, r);
-      v_axpby<double>(1.0, b, -1.0, r, n, st);  // r = b - A x
+    (void)r;
+    // residual and Chebyshev update fused into the assemble pass of the level operator
+    auto step = [&](double f1, double f2, int first) {
+      ChebStep cs{b, dinv, d, f1, f2, first};
+      L.t->level_op(kind, x, x, AS_CHEB, cs, st);
     };
     for (int s = 0; s < steps; ++s) {
-      if (x_zero && s == 0) {
+      if (x_zero && s == 0)
         v_cheb_first<double>(dinv, b, d, x, 1.0 / ch.theta, true, n, st);
-      } else {
-        resid();
-        v_cheb_first<double>(dinv, r, d, x, 1.0 / ch.theta, false, n, st);
-      }
+      else
+        step(0.0, 1.0 / ch.theta, 1);
       if (ch.degree < 2 || std::fabs(ch.delta) < 1e-40) continue;
       const double sigma = ch.theta / ch.delta;
       double rho_old = ch.delta / ch.theta;
       for (int k = 1; k < ch.degree; ++k) {
         const double rho = 1.0 / (2.0 * sigma - rho_old);
-        resid();
-        v_cheb_step<double>(dinv, r, d, x, rho * rho_old, 2.0 * rho / ch.delta, n, st);
+        step(rho * rho_old, 2.0 * rho / ch.delta, 0);
         rho_old = rho;
       }
     }
@@ -1104,8 +1171,8 @@ struct mfs_tria_gmg {
     double *x = L.x[kind].p, *b = L.b[kind].p, *r = L.r[kind].p;
     smooth(kind, l, x, b, true, steps);
     if (l == 0) return;
-    op(kind, l, x, r);
-    v_axpby<double>(1.0, b, -1.0, r, L.n[kind], st);
+    ChebStep cs{b, nullptr, nullptr, 0.0, 0.0, 0};
+    L.t->level_op(kind, x, r, AS_RESID, cs, st);  // r = b - A x
     TLevel &C = lv[l - 1];
     L.R[kind].apply(r, C.b[kind].p, false, st);
     vcycle_level(kind, l - 1);

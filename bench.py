@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--contraction-probe", action="store_true",
                     help="time the 3-point basis contraction as VALU FMAs and on the matrix cores, then exit")
+    ap.add_argument("--no-box-solve", action="store_true",
+                    help="skip the 128^3 time-to-solve runs (the general-mesh solve still runs)")
     ap.add_argument("--no-tria", action="store_true",
                     help="skip the general-mesh (spherical shell with hanging nodes) apply and solve")
     ap.add_argument("--shell-refinement", type=int, default=3,
@@ -260,7 +262,7 @@ def main():
         **t_vc,
     }
 
-    if not args.no_solve:
+    if not args.no_solve and not args.no_box_solve:
         # Time to solve: ASPECT's GMG requires material averaging (with averaging
         # disabled its default solver selection falls back to AMG,
         # source/simulator/helper_functions.cc:2833); the solve uses "harmonic
