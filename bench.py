@@ -9,14 +9,19 @@ A step is one FP64 application of the fine-level Stokes operator (the metric's
 average device time per step, timed with hipEvents on the library's stream.
 After the timed steps the script reports, outside ``value``: the FP32 level
 A-block apply rate, and time-to-solve to 1e-8 relative residual with the FP32
-V-cycle and with the FP64 V-cycle (both inside FP64 GMRES/FGMRES), and the
-time to solve an ensemble of 48 independent 2D problems together.
+V-cycle and with the FP64 V-cycle (both inside FP64 GMRES/FGMRES), the time
+to solve an ensemble of 48 independent 2D problems together, and on general
+meshes (the spherical shell with one adaptive level, hanging nodes, free
+slip) the FP64 Stokes apply and the global-coarsening multigrid solve.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-solve] [--no-cpu]
-                      [--no-ensemble] [--fgmres-profile K]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-solve] [--no-box-solve]
+                      [--no-cpu] [--no-ensemble] [--no-tria] [--shell-refinement G] [--fgmres-profile K]
+                      [--contraction-probe]
 
 ``--fgmres-profile K`` runs only K cheap iterations of the FP32-GMG solve and
 prints their time (the command the per-iteration kernel profile is taken on).
+``--contraction-probe`` times the 3-point basis contraction as VALU FMAs and
+on the matrix cores and exits.
 """
 from __future__ import annotations
 
