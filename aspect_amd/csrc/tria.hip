@@ -753,6 +753,10 @@ __global__ void k_probe_get(const long long n, const int *__restrict__ colour, c
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n && colour[i] == c) diag[i] = y[i];
 }
+__global__ void k_zero_rows(const long long n, const int *__restrict__ row, double *__restrict__ x) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n && row[i] >= 0) x[i] = 0.0;
+}
 __global__ void k_invert(const long long n, double *__restrict__ x) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) x[i] = 1.0 / x[i];
@@ -1469,7 +1473,7 @@ int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *ds
   });
 }
 
-int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_solve_params *prm,
+int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, const double *x0, double *sol, const mfs_solve_params *prm,
                    mfs_solve_result *res) {
   return tguard([&] {
     if (!g) throw std::invalid_argument("null mfs_tria_gmg");
@@ -1483,15 +1487,40 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_so
     b.alloc(n);
     x.alloc(n);
     MFS_HIP(hipMemcpyAsync(b.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, g->st));
-    const double r0 = std::sqrt(g->dot(b.p, b.p, n));
+    TLevel &T = g->lv.back();
+    const long long nu = T.n[0], np = T.n[1];
+    const double s = T.t->L.s;
+    double r0, inr;
+    if (x0 && !prm->solve_newton_system) {
+      // initial guess: constrained velocity entries zeroed, pressure / s
+      // (local_smoothing.cc:944-975); the nonlinear residual |rhs - K x0| and
+      // the tolerance from the residual with the velocity zeroed (:977-1025)
+      MFS_HIP(hipMemcpyAsync(x.p, x0, n * sizeof(double), hipMemcpyHostToDevice, g->st));
+      hipLaunchKernelGGL(k_zero_rows, dim3(nblk(nu)), dim3(256), 0, g->st, nu, T.t->con_u.row.p, x.p);
+      v_scale(x.p + nu, 1.0 / s, np, g->st);
+      Buf<double> w, z;
+      w.alloc(n);
+      z.alloc(n);
+      T.t->vmult(MFS_OP_STOKES, x.p, w.p, g->st);
+      v_axpby<double>(1.0, b.p, -1.0, w.p, n, g->st);
+      inr = std::sqrt(g->dot(w.p, w.p, n));
+      MFS_HIP(hipMemcpyAsync(z.p, x.p, n * sizeof(double), hipMemcpyDeviceToDevice, g->st));
+      v_fill<double>(z.p, 0.0, nu, g->st);
+      T.t->vmult(MFS_OP_STOKES, z.p, w.p, g->st);
+      v_axpby<double>(1.0, b.p, -1.0, w.p, n, g->st);
+      r0 = std::sqrt(g->dot(w.p, w.p, nu) + g->dot(b.p + nu, b.p + nu, np));
+    } else {
+      // zero initial guess (a Newton update's best guess, :950-969); tolerance from |rhs|
+      v_fill<double>(x.p, 0.0, n, g->st);
+      r0 = inr = std::sqrt(g->dot(b.p, b.p, n));
+    }
     const double tol = prm->tolerance * r0;
     hipEvent_t e0, e1;
     MFS_HIP(hipEventCreate(&e0));
     MFS_HIP(hipEventCreate(&e1));
     MFS_HIP(hipEventRecord(e0, g->st));
     int its = 0, its_exp = 0;
-    double fin = r0;
-    v_fill<double>(x.p, 0.0, n, g->st);
+    double fin = inr;
     hipEvent_t em;
     MFS_HIP(hipEventCreate(&em));
     if (prm->n_cheap_steps > 0)
@@ -1513,6 +1542,7 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_so
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(em);
+    v_scale(x.p + nu, s, np, g->st);  // post-step: pressure * s (:1406-1445)
     MFS_HIP(hipMemcpyAsync(sol, x.p, n * sizeof(double), hipMemcpyDeviceToHost, g->st));
     MFS_HIP(hipStreamSynchronize(g->st));
     res->cheap_iterations = its;
@@ -1520,7 +1550,7 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_so
     res->a_preconditioner_applications = its + (int)g->a_apps;
     res->s_preconditioner_applications = its + (int)g->s_apps;
     res->converged = fin <= tol ? 1 : 0;
-    res->initial_nonlinear_residual = r0;
+    res->initial_nonlinear_residual = inr;
     res->r0_norm = r0;
     res->final_linear_residual = fin;
     res->cheap_final_residual = cheap_res;

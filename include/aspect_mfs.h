@@ -446,16 +446,21 @@ int mfs_tria_gmg_info(mfs_tria_gmg *g, int level, int kind, double out[4], doubl
 /* One V-cycle (kind 0: A block, 1 smoothing step; 1: Schur complement, 4 steps)
  * on host vectors of the finest level. */
 int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *dst);
-/* Solve on the finest level from a zero initial guess to prm->tolerance * |rhs|
- * (local_smoothing.cc:1259-1397 sequence): right-preconditioned GMRES
+/* Solve on the finest level (local_smoothing.cc:944-1445 sequence).  Initial
+ * guess x0 (host, may be NULL: zero; ignored for a Newton update): its
+ * constrained velocity entries are zeroed and its pressure divided by the
+ * pressure scaling; tolerance prm->tolerance * r0 with r0 the norm of
+ * (rhs_u - [K (0, p0)]_u, rhs_p) (|rhs| without x0 or for a Newton update);
+ * the solution's pressure is multiplied by the pressure scaling (no pressure
+ * normalization: the caller's).  Right-preconditioned GMRES
  * (restart prm->restart) with the cheap block preconditioner (one A and one
  * Schur V-cycle) for at most prm->n_cheap_steps iterations, then, if not
  * converged, flexible GMRES with the expensive one (CG inner solves to
  * prm->a_block_tolerance / prm->s_block_tolerance, V-cycle preconditioned)
- * for at most prm->n_expensive_steps.  Host vectors [u|p]; the Newton, IDR(s)
- * and failure-strategy fields of prm are not used.  MFS_NO_CONVERGENCE when the
+ * for at most prm->n_expensive_steps.  Host vectors [u|p]; the IDR(s) and
+ * failure-strategy fields of prm are not used.  MFS_NO_CONVERGENCE when the
  * tolerance is not reached. */
-int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, double *sol, const mfs_solve_params *prm,
+int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, const double *x0, double *sol, const mfs_solve_params *prm,
                    mfs_solve_result *res);
 
 /* Measurement of the matrix-core alternative for the sum-factorization

@@ -303,3 +303,44 @@ def test_tria_hanging2_golden_fixture(dim):
     assert rel(gpu.vmult("ablock", x[:nu]), g["ablock"]) <= 1e-12
     assert rel(gpu.vmult("bt", x[nu:]), g["bt"]) <= 1e-12
     assert rel(gpu.vmult("mass", x[nu:]), g["mass"]) <= 1e-12
+
+
+def test_tria_solve_initial_guess_and_tolerance_rule():
+    """Initial guess with constrained velocity entries zeroed and pressure / s,
+    r0 from the residual with the velocity zeroed, the solution's pressure
+    * s (stokes_matrix_free_local_smoothing.cc:944-1041, :1406-1445)."""
+    meshes, pu, pp, etas, _ = gmg_case("box3d")
+    s = 1.7
+    levels = []
+    for m, e in zip(meshes, etas):
+        t = TriaOperators(m, pressure_scaling=s)
+        t.set_viscosity(e)
+        levels.append(t)
+    mg = TriaMultigrid(levels, pu, pp)
+    mg.build()
+    ops = OT.Operators(meshes[-1], etas[-1], pressure_scaling=s)
+    nu = ops.nu
+    rng = np.random.default_rng(19)
+    b = rng.normal(size=nu + ops.np)
+    b[ops.fixed_u] = 0.0
+    b[nu + ops.fixed_p] = 0.0
+    x0 = rng.normal(size=b.size)
+    x, out, rc = mg.solve(b, tolerance=1e-8, n_cheap_steps=300, n_expensive_steps=100, x0=x0)
+    assert rc == 0 and out.converged
+    xs = x0.copy()
+    xs[ops.fixed_u] = 0.0
+    xs[nu:] /= s
+    z = xs.copy()
+    z[:nu] = 0.0
+    w = b - ops.stokes(z)
+    r0 = np.sqrt(w[:nu] @ w[:nu] + b[nu:] @ b[nu:])
+    assert abs(out.r0_norm - r0) <= 1e-12 * r0
+    inr = np.linalg.norm(b - ops.stokes(xs))
+    assert abs(out.initial_nonlinear_residual - inr) <= 1e-12 * inr
+    xr = x.copy()
+    xr[nu:] /= s
+    assert np.linalg.norm(b - ops.stokes(xr)) <= 1.5e-8 * r0
+    # Newton update: zero initial guess, tolerance from |rhs|
+    xn, outn, rcn = mg.solve(b, tolerance=1e-8, n_cheap_steps=300, n_expensive_steps=100, x0=x0,
+                             solve_newton_system=True)
+    assert rcn == 0 and abs(outn.r0_norm - np.linalg.norm(b)) <= 1e-12 * np.linalg.norm(b)
