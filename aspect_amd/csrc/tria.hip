@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -1481,6 +1482,8 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, const double *x0, double 
     if (!g->built) throw std::runtime_error("multigrid not built (call mfs_tria_gmg_build)");
     if (prm->restart < 1 || prm->restart > 1000) throw std::invalid_argument("mfs_tria_solve: restart must be in [1, 1000]");
     if (prm->n_cheap_steps < 0 || prm->n_expensive_steps < 0) throw std::invalid_argument("mfs_tria_solve: negative step limit");
+    if (prm->failure_strategy != MFS_FAILURE_ABORT && prm->failure_strategy != MFS_FAILURE_CONTINUE)
+      throw std::invalid_argument("mfs_tria_solve: unknown linear solver failure strategy");
     std::memset(res, 0, sizeof(*res));
     const long long n = g->lv.back().n[0] + g->lv.back().n[1];
     Buf<double> b, x;
@@ -1557,7 +1560,16 @@ int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, const double *x0, double 
     res->solver_tolerance = tol;
     res->solve_seconds = ms * 1e-3;
     res->cheap_seconds = ms_cheap * 1e-3;
-    return res->converged ? MFS_OK : MFS_NO_CONVERGENCE;
+    if (res->converged) return MFS_OK;
+    // "Linear solver failure strategy" (local_smoothing.cc:1353-1395)
+    if (prm->failure_strategy == MFS_FAILURE_CONTINUE) return MFS_OK;
+    char msg[256];
+    std::snprintf(msg, sizeof msg,
+                  "the iterative Stokes solver on the general mesh did not converge: initial residual %.6e, "
+                  "final residual %.6e, required %.6e",
+                  r0, fin, tol);
+    set_last_error(msg);
+    return MFS_NO_CONVERGENCE;
   });
 }
 

@@ -760,13 +760,14 @@ class TriaMultigrid:
         return y
 
     def solve(self, rhs, tolerance=1e-8, n_cheap_steps=1000, n_expensive_steps=1000, restart=100,
-              a_block_tolerance=1e-2, s_block_tolerance=1e-6, x0=None, solve_newton_system=False):
+              a_block_tolerance=1e-2, s_block_tolerance=1e-6, x0=None, solve_newton_system=False,
+              failure_strategy="abort"):
         t = self.levels[-1]
         b = _vec(rhs, t.nu + t.np, "right-hand side")
         xg = None if x0 is None else _vec(x0, t.nu + t.np, "initial guess")
         x = np.empty_like(b)
         prm = _params(tolerance, n_cheap_steps, n_expensive_steps, restart, a_block_tolerance, s_block_tolerance,
-                      "GMRES", 2, solve_newton_system)
+                      "GMRES", 2, solve_newton_system, failure_strategy)
         res = SolveResult()
         rc = _check(self.lib.mfs_tria_solve(self._g, _ptr(b), _ptr(xg) if xg is not None else None, _ptr(x),
                                             C.byref(prm), C.byref(res)))

@@ -457,9 +457,10 @@ int mfs_tria_gmg_vcycle(mfs_tria_gmg *g, int kind, const double *src, double *ds
  * Schur V-cycle) for at most prm->n_cheap_steps iterations, then, if not
  * converged, flexible GMRES with the expensive one (CG inner solves to
  * prm->a_block_tolerance / prm->s_block_tolerance, V-cycle preconditioned)
- * for at most prm->n_expensive_steps.  Host vectors [u|p]; the IDR(s) and
- * failure-strategy fields of prm are not used.  MFS_NO_CONVERGENCE when the
- * tolerance is not reached. */
+ * for at most prm->n_expensive_steps.  Host vectors [u|p]; the IDR(s)
+ * fields of prm are not used.  Not converged: MFS_NO_CONVERGENCE with a
+ * message (MFS_FAILURE_ABORT) or MFS_OK with res->converged = 0
+ * (MFS_FAILURE_CONTINUE). */
 int mfs_tria_solve(mfs_tria_gmg *g, const double *rhs, const double *x0, double *sol, const mfs_solve_params *prm,
                    mfs_solve_result *res);
 

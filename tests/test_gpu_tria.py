@@ -344,3 +344,27 @@ def test_tria_solve_initial_guess_and_tolerance_rule():
     xn, outn, rcn = mg.solve(b, tolerance=1e-8, n_cheap_steps=300, n_expensive_steps=100, x0=x0,
                              solve_newton_system=True)
     assert rcn == 0 and abs(outn.r0_norm - np.linalg.norm(b)) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_tria_solve_failure_strategies():
+    """Linear solver failure strategy (local_smoothing.cc:1353-1395): abort
+    reports MFS_NO_CONVERGENCE with a message, continue returns the iterate."""
+    from aspect_amd import mfs
+    meshes, pu, pp, etas, _ = gmg_case("shell2d")
+    levels = []
+    for m, e in zip(meshes, etas):
+        t = TriaOperators(m, pressure_scaling=1.0)
+        t.set_viscosity(e)
+        levels.append(t)
+    mg = TriaMultigrid(levels, pu, pp)
+    mg.build()
+    top = meshes[-1]
+    b = np.random.default_rng(20).normal(size=top.n_dofs_u + top.n_dofs_p)
+    b[top.con_u[0]] = 0.0
+    b[top.n_dofs_u + top.con_p[0]] = 0.0
+    _, out, rc = mg.solve(b, tolerance=1e-12, n_cheap_steps=2, n_expensive_steps=0)
+    assert rc == mfs.MFS_NO_CONVERGENCE and not out.converged
+    assert b"did not converge" in mfs.load_library().mfs_last_error()
+    _, out, rc = mg.solve(b, tolerance=1e-12, n_cheap_steps=2, n_expensive_steps=0,
+                          failure_strategy="continue with nonlinear solver")
+    assert rc == mfs.MFS_OK and not out.converged and out.cheap_iterations == 2
