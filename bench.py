@@ -348,8 +348,10 @@ def main():
         # refinement, one adaptive level on the 20% of cells with the largest
         # |dT/dr|), free slip on both surfaces, per-point geometry and
         # viscosity, hanging-node and no-normal-flux constraints.  FP64 Stokes
-        # apply; then the global-coarsening multigrid and the cheap GMRES solve
-        # on the next smaller shell (zero velocity inside, free slip outside).
+        # apply; then the global-coarsening multigrid and the solve on the next
+        # smaller shell (free slip on both spheres: the discrete rigid
+        # rotations are only nearly in the kernel on curved cells, so the
+        # true residual can end a little above the GMRES estimate).
         from aspect_amd import TriaMultigrid, TriaOperators
         from aspect_amd import tria as TR
         t0 = time.perf_counter()
@@ -380,7 +382,7 @@ def main():
         if not args.no_solve:
             t0 = time.perf_counter()
             tr2 = P.config_shell(3, max(args.shell_refinement - 1, 0))
-            trias, meshes, pu, pp = TR.hierarchy(tr2, {0: "zero", 1: "tangential"})
+            trias, meshes, pu, pp = TR.hierarchy(tr2, {0: "tangential", 1: "tangential"})
             parents = [None] + [TR.parent_map(trias[l], trias[l - 1]) for l in range(1, len(trias))]
             levels = [TriaOperators(m, pressure_scaling=1.0) for m in meshes]
             t_h = time.perf_counter() - t0
@@ -406,7 +408,7 @@ def main():
                             "expensive_iterations": out.expensive_iterations,
                             "cheap_phase_seconds": out.cheap_seconds, "converged": bool(out.converged),
                             "rel_residual": resid, "gmg_build_seconds": t_b, "host_setup_seconds": t_h,
-                            "boundary": "zero velocity inside, free slip outside",
+                            "boundary": "free slip on both spheres",
                             "averaging": "harmonic average only viscosity, DGQ0 level tables"}
             log(f"general mesh solve: levels {[m.n_cells for m in meshes]}, {out.cheap_iterations}+"
                 f"{out.expensive_iterations} iterations, {out.solve_seconds:.2f} s, residual {resid:.2e}")

@@ -30,7 +30,7 @@ exactly for fields in the conforming space.
 from __future__ import annotations
 
 import numpy as np
-from scipy.sparse import coo_matrix, csr_matrix, identity
+from scipy.sparse import coo_matrix, csr_matrix
 
 _XG = np.array([0.5 - 0.5 * np.sqrt(0.6), 0.5, 0.5 + 0.5 * np.sqrt(0.6)])
 
