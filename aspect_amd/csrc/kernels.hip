@@ -282,7 +282,8 @@ __global__ void __launch_bounds__((Cfg<DIM, K>::THREADS), (MinWaves<T>::value))
     tz = 2 * b + pz;
   }
   const long long tile_id =
-      MODE == MODE_CELLS ? (long long)blockIdx.x : tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
+      MODE == MODE_CELLS ? (long long)xcd_order(blockIdx.x, gridDim.x)  // batches sharing nodes meet in one L2
+                         : tx + (long long)L.ntiles[0] * (ty + (long long)L.ntiles[1] * tz);
   const int tid = threadIdx.x;
   const int c = tid % CT;
   // the y-slice index is uniform over a wave when a slice spans whole waves

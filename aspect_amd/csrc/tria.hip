@@ -33,6 +33,7 @@
 
 #include "../../include/aspect_mfs.h"
 #include "mfs_internal.h"
+#include "tile_common.h"
 
 using namespace mfs;
 
@@ -107,7 +108,9 @@ __global__ void k_assemble(const long long n, const long long nn, const long lon
                            const long long *__restrict__ tdof, const double *__restrict__ tw,
                            const double *__restrict__ src, double *__restrict__ dst, const int ident,
                            const ChebStep ch) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // XCD-aware order: neighbouring nodes gather from the same cells' contribution
+  // lines, so contiguous node ranges go to one XCD (one L2)
+  const long long i = xcd_order(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double a;
   if (row[i] >= 0) {
@@ -179,7 +182,8 @@ __global__ void k_tria_mass_apply(const long long ncells, const int *__restrict_
                                   const long long *__restrict__ mas, const double *__restrict__ w,
                                   double *__restrict__ out) {
   constexpr int NP = 1 << DIM, NT = NP * (NP + 1) / 2;
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // XCD-aware order: neighbouring cells share pressure nodes
+  const long long c = xcd_order(blockIdx.x, gridDim.x) * (long long)blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const long long b = c / 64, cc = c - b * 64;
   double p[NP], y[NP];
